@@ -1,0 +1,357 @@
+// CPU oracle for the DDD.jl force path: C entry points over the templated
+// restatement in ddd_oracle.hpp.  TEST INFRASTRUCTURE ONLY -- loaded by tests/,
+// the smoke check and the CPU-baseline leg of the benchmark, never by the
+// product path.
+//
+// Array arguments use the reference's Julia memory layout (column-major,
+// 1-based Int64 indices): coord 3 x nodeBuf, bVec 3 x linkBuf, segIdx
+// ld x 3 (columns: link, node1, node2), extSeg one byte per segment,
+// segForce 3 x 2 x ld, connectivity (1 + 2 maxConnect) x nodeBuf.
+#include "ddd_oracle.hpp"
+
+#include <atomic>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+
+using namespace ddd_oracle;
+
+namespace {
+
+template <class T> SegConsts<T> consts_from(const double* K) {
+  return SegConsts<T>{T(K[0]), T(K[1]), T(K[2]), T(K[3]), T(K[4]), T(K[5])};
+}
+
+struct SegView {
+  int64_t numSeg;
+  const int64_t* segIdx;
+  int64_t ld;
+  const double* coord;
+  const double* bVec;
+  const uint8_t* extSeg;
+  // segment i (0-based row of segIdx)
+  V3<double> b(int64_t i) const {
+    const double* p = bVec + 3 * (segIdx[i] - 1);
+    return mk(p[0], p[1], p[2]);
+  }
+  V3<double> n1(int64_t i) const {
+    const double* p = coord + 3 * (segIdx[i + ld] - 1);
+    return mk(p[0], p[1], p[2]);
+  }
+  V3<double> n2(int64_t i) const {
+    const double* p = coord + 3 * (segIdx[i + 2 * ld] - 1);
+    return mk(p[0], p[1], p[2]);
+  }
+  bool ext(int64_t i) const { return extSeg && extSeg[i]; }
+};
+
+template <class T> V3<T> cvt(const V3<double>& v) { return mk(T(v.x), T(v.y), T(v.z)); }
+
+inline void add3(double* dst, const V3<double>& v) {
+  dst[0] += v.x;
+  dst[1] += v.y;
+  dst[2] += v.z;
+}
+template <class T> inline void add3t(double* dst, const V3<T>& v) {
+  dst[0] += double(v.x);
+  dst[1] += double(v.y);
+  dst[2] += double(v.z);
+}
+
+inline void atomic_add(double* p, double v) {
+  // CAS loop: one atomic add per component, as Threads.atomic_add! does.
+  auto* a = reinterpret_cast<std::atomic<double>*>(p);
+  double old = a->load(std::memory_order_relaxed);
+  while (!a->compare_exchange_weak(old, old + v, std::memory_order_relaxed)) {
+  }
+}
+
+// calcSegSegForceSer!: DislocationForces.jl:778-829, in precision T,
+// accumulated in T per segment then added into segForce.
+template <class T>
+void segseg_ser(const double* K, const SegView& s, double* segForce) {
+  SegConsts<T> KK = consts_from<T>(K);
+  const int64_t N = s.numSeg;
+  std::vector<T> acc(6 * N);
+  for (int64_t i = 0; i < 6 * N; ++i) acc[i] = T(segForce[i]);
+  for (int64_t i = 0; i < N; ++i) {
+    if (s.ext(i)) continue;
+    V3<T> b1 = cvt<T>(s.b(i)), n11 = cvt<T>(s.n1(i)), n12 = cvt<T>(s.n2(i));
+    for (int64_t j = i + 1; j < N; ++j) {
+      if (s.ext(j)) continue;
+      Forces4<T> f = seg_seg_force(KK, b1, n11, n12, cvt<T>(s.b(j)), cvt<T>(s.n1(j)),
+                                   cvt<T>(s.n2(j)));
+      T* ai = &acc[6 * i];
+      T* aj = &acc[6 * j];
+      ai[0] += f.f1.x; ai[1] += f.f1.y; ai[2] += f.f1.z;
+      ai[3] += f.f2.x; ai[4] += f.f2.y; ai[5] += f.f2.z;
+      aj[0] += f.f3.x; aj[1] += f.f3.y; aj[2] += f.f3.z;
+      aj[3] += f.f4.x; aj[4] += f.f4.y; aj[5] += f.f4.z;
+    }
+  }
+  for (int64_t i = 0; i < 6 * N; ++i) segForce[i] = double(acc[i]);
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_version() { return 1; }
+
+// One pair, F1..F4 in out[12].  prec: 0 double, 1 long double, 2 float.
+// Returns 1 when the near-parallel branch was taken.
+int orc_segseg_pair(int prec, const double* K, const double* b1, const double* n11,
+                    const double* n12, const double* b2, const double* n21, const double* n22,
+                    double* out) {
+  auto run = [&](auto tag) {
+    using T = decltype(tag);
+    SegConsts<T> KK = consts_from<T>(K);
+    auto v = [](const double* p) { return mk(T(p[0]), T(p[1]), T(p[2])); };
+    Forces4<T> f = seg_seg_force(KK, v(b1), v(n11), v(n12), v(b2), v(n21), v(n22));
+    const V3<T>* fs[4] = {&f.f1, &f.f2, &f.f3, &f.f4};
+    for (int k = 0; k < 4; ++k) {
+      out[3 * k] = double(fs[k]->x);
+      out[3 * k + 1] = double(fs[k]->y);
+      out[3 * k + 2] = double(fs[k]->z);
+    }
+    return seg_seg_is_parallel(v(n11), v(n12), v(n21), v(n22)) ? 1 : 0;
+  };
+  if (prec == 1) return run((long double)0);
+  if (prec == 2) return run(0.0f);
+  return run(0.0);
+}
+
+// Whole-network serial seg-seg (calcSegSegForceSer!, :778-829); adds into
+// segForce (3 x 2 x ld).  prec as above.
+void orc_segseg_ser(int prec, const double* K, int64_t numSeg, const int64_t* segIdx, int64_t ld,
+                    const double* coord, const double* bVec, const uint8_t* extSeg,
+                    double* segForce) {
+  SegView s{numSeg, segIdx, ld, coord, bVec, extSeg};
+  if (prec == 1) segseg_ser<long double>(K, s, segForce);
+  else if (prec == 2) segseg_ser<float>(K, s, segForce);
+  else segseg_ser<double>(K, s, segForce);
+}
+
+// calcSegSegForcePar (:719-777) as the reference designs it: nthreads static
+// chunks of the outer index over the upper triangle, 12 atomic adds per pair.
+void orc_segseg_par_faithful(const double* K, int64_t numSeg, const int64_t* segIdx, int64_t ld,
+                             const double* coord, const double* bVec, const uint8_t* extSeg,
+                             double* segForce, int nthreads) {
+  SegView s{numSeg, segIdx, ld, coord, bVec, extSeg};
+  SegConsts<double> KK = consts_from<double>(K);
+  std::vector<double> par(6 * numSeg, 0.0);
+  double* P = par.data();
+#pragma omp parallel num_threads(nthreads)
+  {
+    int tid = omp_get_thread_num();
+    int nt = omp_get_num_threads();
+    int64_t start = (int64_t(tid) * numSeg) / nt;
+    int64_t stop = (int64_t(tid + 1) * numSeg) / nt;
+    for (int64_t i = start; i < stop; ++i) {
+      if (s.ext(i)) continue;
+      V3<double> b1 = s.b(i), n11 = s.n1(i), n12 = s.n2(i);
+      for (int64_t j = i + 1; j < numSeg; ++j) {
+        if (s.ext(j)) continue;
+        Forces4<double> f = seg_seg_force(KK, b1, n11, n12, s.b(j), s.n1(j), s.n2(j));
+        double* pi = P + 6 * i;
+        double* pj = P + 6 * j;
+        atomic_add(pi + 0, f.f1.x); atomic_add(pi + 1, f.f1.y); atomic_add(pi + 2, f.f1.z);
+        atomic_add(pi + 3, f.f2.x); atomic_add(pi + 4, f.f2.y); atomic_add(pi + 5, f.f2.z);
+        atomic_add(pj + 0, f.f3.x); atomic_add(pj + 1, f.f3.y); atomic_add(pj + 2, f.f3.z);
+        atomic_add(pj + 3, f.f4.x); atomic_add(pj + 4, f.f4.y); atomic_add(pj + 5, f.f4.z);
+      }
+    }
+  }
+  for (int64_t i = 0; i < 6 * numSeg; ++i) segForce[i] += P[i];
+}
+
+// Rows-only seg-seg over a subset (calcSegSegForceIdx!, :831-877).  For each
+// k, row i = idx[k]-1 is computed against every j != i with segment i as the
+// first segment; F1/F2 are added to out column (byPosition ? k : i).
+void orc_segseg_idx(const double* K, int64_t numSeg, const int64_t* segIdx, int64_t ld,
+                    const double* coord, const double* bVec, const uint8_t* extSeg,
+                    const int64_t* idx, int64_t nIdx, double* out, int byPosition) {
+  SegView s{numSeg, segIdx, ld, coord, bVec, extSeg};
+  SegConsts<double> KK = consts_from<double>(K);
+  for (int64_t k = 0; k < nIdx; ++k) {
+    int64_t i = idx[k] - 1;
+    if (s.ext(i)) continue;
+    V3<double> b1 = s.b(i), n11 = s.n1(i), n12 = s.n2(i);
+    double* o = out + 6 * (byPosition ? k : i);
+    for (int64_t j = 0; j < numSeg; ++j) {
+      if (j == i || s.ext(j)) continue;
+      Forces4<double> f = seg_seg_force(KK, b1, n11, n12, s.b(j), s.n1(j), s.n2(j));
+      add3(o, f.f1);
+      add3(o + 3, f.f2);
+    }
+  }
+}
+
+// Self force (calcSelfForce!, :354-388 and _calcSelfForce :414-455).
+// sc = {a, aSq, mu4pi, nuOmNuInv, Ec, omNuInv}.  idx NULL means all
+// segments.  Node 1 receives -f, node 2 +f, at out column
+// (byPosition ? k : i).
+void orc_self_force(const double* sc, int64_t numSeg, const int64_t* segIdx, int64_t ld,
+                    const double* coord, const double* bVec, const uint8_t* extSeg,
+                    const int64_t* idx, int64_t nIdx, double* out, int byPosition) {
+  SegView s{numSeg, segIdx, ld, coord, bVec, extSeg};
+  int64_t n = idx ? nIdx : numSeg;
+  for (int64_t k = 0; k < n; ++k) {
+    int64_t i = idx ? idx[k] - 1 : k;
+    if (s.ext(i)) continue;
+    V3<double> f = self_force(s.n2(i) - s.n1(i), s.b(i), sc[0], sc[1], sc[2], sc[3], sc[4], sc[5]);
+    double* o = out + 6 * (byPosition ? k : i);
+    o[0] -= f.x; o[1] -= f.y; o[2] -= f.z;
+    o[3] += f.x; o[4] += f.y; o[5] += f.z;
+  }
+}
+
+// Stress from the FE correction field at x0 (calc_sigmaHat,
+// DislocationForces.jl:92-192; shapeFunctionDeriv FEMShapeFunction.jl:83-122).
+// C is 6x6 column-major, mcoord 3 x numNode, conn 8 x numElem (1-based),
+// uHat dense (3 numNode).  out = 3x3 column-major.
+void orc_sigma_hat(const double* C, const double* mcoord, const int64_t* conn, int64_t mx,
+                   int64_t my, int64_t mz, double w, double h, double d, const double* uHat,
+                   const double* x0, double* out) {
+  double wInv = 1 / w, hInv = 1 / h, dInv = 1 / d;
+  double x = x0[0], y = x0[1], z = x0[2];
+  auto clampi = [](double v, int64_t lo, int64_t hi) {
+    double c = std::ceil(v);
+    if (c < lo) c = double(lo);
+    if (c > hi) c = double(hi);
+    return int64_t(c);
+  };
+  int64_t i = clampi(x * wInv, 1, mx), j = clampi(y * hInv, 1, my), k = clampi(z * dInv, 1, mz);
+  int64_t e = i + (j - 1) * mx + (k - 1) * mx * my;  // 1-based element
+  const int64_t* ce = conn + 8 * (e - 1);
+  int64_t nn1 = ce[0], nn7 = ce[6];
+  double xc = 0.5 * (mcoord[3 * (nn1 - 1)] + mcoord[3 * (nn7 - 1)]);
+  double yc = 0.5 * (mcoord[3 * (nn1 - 1) + 1] + mcoord[3 * (nn7 - 1) + 1]);
+  double zc = 0.5 * (mcoord[3 * (nn1 - 1) + 2] + mcoord[3 * (nn7 - 1) + 2]);
+  double ds1dx = 2 * wInv, ds2dy = 2 * hInv, ds3dz = 2 * dInv;
+  double s1 = (x - xc) * ds1dx, s2 = (y - yc) * ds2dy, s3 = (z - zc) * ds3dz;
+  double omx = 1 - s1, omy = 1 - s2, omz = 1 - s3, opx = 1 + s1, opy = 1 + s2, opz = 1 + s3;
+  // dNdS[r][n], r = derivative direction, n = local node.
+  double dN[3][8] = {
+      {-omy * omz, omy * omz, opy * omz, -opy * omz, -omy * opz, omy * opz, opy * opz, -opy * opz},
+      {-omx * omz, -opx * omz, opx * omz, omx * omz, -omx * opz, -opx * opz, opx * opz, omx * opz},
+      {-omx * omy, -opx * omy, -opx * opy, -omx * opy, omx * omy, opx * omy, opx * opy, omx * opy}};
+  for (int r = 0; r < 3; ++r)
+    for (int n = 0; n < 8; ++n) dN[r][n] = dN[r][n] / 8;
+  double B[6][24];
+  double U[24];
+  std::memset(B, 0, sizeof(B));
+  for (int n = 1; n <= 8; ++n) {
+    int idx1 = 3 * n, idx2 = 3 * (n - 1);
+    int64_t idx3 = 3 * ce[n - 1];
+    B[0][idx2 + 0] = dN[0][n - 1] * ds1dx;
+    B[1][idx2 + 1] = dN[1][n - 1] * ds2dy;
+    B[2][idx2 + 2] = dN[2][n - 1] * ds3dz;
+    B[3][idx2 + 0] = B[1][idx2 + 1];
+    B[3][idx2 + 1] = B[0][idx2 + 0];
+    B[4][idx2 + 0] = B[2][idx1 - 1];
+    B[4][idx1 - 1] = B[0][idx2 + 0];
+    B[5][idx2 + 1] = B[2][idx1 - 1];
+    B[5][idx1 - 1] = B[1][idx2 + 1];
+    U[idx1 - 3] = uHat[idx3 - 3];
+    U[idx1 - 2] = uHat[idx3 - 2];
+    U[idx1 - 1] = uHat[idx3 - 1];
+  }
+  double eps6[6], sv[6];
+  for (int r = 0; r < 6; ++r) {
+    double acc = 0;
+    for (int c = 0; c < 24; ++c) acc += B[r][c] * U[c];
+    eps6[r] = acc;
+  }
+  for (int r = 0; r < 6; ++r) {
+    double acc = 0;
+    for (int c = 0; c < 6; ++c) acc += C[r + 6 * c] * eps6[c];
+    sv[r] = acc;
+  }
+  // Voigt (xx, yy, zz, xy, xz, yz) -> 3x3 column-major.
+  double S[9] = {sv[0], sv[3], sv[4], sv[3], sv[1], sv[5], sv[4], sv[5], sv[2]};
+  std::memcpy(out, S, sizeof(S));
+}
+
+// PK force (calcPKForce!, :244-300): pk = (sigmaHat(mid) b) x (n2 - n1);
+// half to each node at out column (byPosition ? k : i).
+void orc_pk_force(const double* C, const double* mcoord, const int64_t* conn, int64_t mx,
+                  int64_t my, int64_t mz, double w, double h, double d, const double* uHat,
+                  int64_t numSeg, const int64_t* segIdx, int64_t ld, const double* coord,
+                  const double* bVec, const uint8_t* extSeg, const int64_t* idx, int64_t nIdx,
+                  double* out, int byPosition, int halfToNodes) {
+  SegView s{numSeg, segIdx, ld, coord, bVec, extSeg};
+  int64_t n = idx ? nIdx : numSeg;
+  for (int64_t k = 0; k < n; ++k) {
+    int64_t i = idx ? idx[k] - 1 : k;
+    if (s.ext(i)) continue;
+    V3<double> a = s.n1(i), bb = s.n2(i), b = s.b(i);
+    V3<double> t = bb - a;
+    V3<double> mid = (bb + a) / 2.0;
+    double x0[3] = {mid.x, mid.y, mid.z}, S[9];
+    orc_sigma_hat(C, mcoord, conn, mx, my, mz, w, h, d, uHat, x0, S);
+    V3<double> sb = mk(S[0] * b.x + S[3] * b.y + S[6] * b.z, S[1] * b.x + S[4] * b.y + S[7] * b.z,
+                       S[2] * b.x + S[5] * b.y + S[8] * b.z);
+    V3<double> pk = cross(sb, t);
+    if (halfToNodes) {
+      double* o = out + 6 * (byPosition ? k : i);
+      o[0] += pk.x / 2; o[1] += pk.y / 2; o[2] += pk.z / 2;
+      o[3] += pk.x / 2; o[4] += pk.y / 2; o[5] += pk.z / 2;
+    } else {
+      double* o = out + 3 * (byPosition ? k : i);
+      o[0] += pk.x; o[1] += pk.y; o[2] += pk.z;
+    }
+  }
+}
+
+// Field-point stress (_calc_sigmaTilde!, DislocationFEM.jl:86-303):
+// sc = {a2, mu4pi, mu4pinu, nuMu4pinu, a2mu8pi}; x0 3 x P; sigma 6 x P
+// accumulated.  Does not skip external segments, as the reference does not.
+void orc_sigma_tilde(const double* sc, int64_t numSeg, const int64_t* segIdx, int64_t ld,
+                     const double* coord, const double* bVec, const int64_t* idx, int64_t nIdx,
+                     const double* x0, int64_t P, double* sigma, int nthreads) {
+  SegView s{numSeg, segIdx, ld, coord, bVec, nullptr};
+  int64_t n = idx ? nIdx : numSeg;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int64_t p = 0; p < P; ++p) {
+    V3<double> x = mk(x0[3 * p], x0[3 * p + 1], x0[3 * p + 2]);
+    double* sg = sigma + 6 * p;
+    for (int64_t k = 0; k < n; ++k) {
+      int64_t i = idx ? idx[k] - 1 : k;
+      sigma_tilde_point(s.b(i), s.n1(i), s.n2(i), x, sc[0], sc[1], sc[2], sc[3], sc[4], sg);
+    }
+  }
+}
+
+// Node-force gather (dlnMobility!, DislocationMobility.jl:220-255):
+// nodeForce[:, n] = sum over connections of segForce[:, colLink, link],
+// skipping zero-length links and contributions below sigmaPN.
+void orc_node_gather(const int64_t* connectivity, int64_t ldConn, int64_t numNode,
+                     const int64_t* links, const double* coord, const double* segForce,
+                     double sigmaPN, double* nodeForce) {
+  const double eps = std::numeric_limits<double>::epsilon();
+  for (int64_t n = 0; n < numNode; ++n) {
+    const int64_t* cn = connectivity + ldConn * n;
+    double f[3] = {0, 0, 0};
+    for (int64_t c = 1; c <= cn[0]; ++c) {
+      int64_t link = cn[2 * c - 1];
+      int64_t col = cn[2 * c];
+      int64_t node2 = links[2 * (link - 1) + (3 - col) - 1];
+      double tx = coord[3 * (node2 - 1)] - coord[3 * n];
+      double ty = coord[3 * (node2 - 1) + 1] - coord[3 * n + 1];
+      double tz = coord[3 * (node2 - 1) + 2] - coord[3 * n + 2];
+      double nT = std::sqrt(tx * tx + ty * ty + tz * tz);
+      if (nT < eps) continue;
+      double tN = 1 / nT;
+      const double* sf = segForce + 6 * (link - 1) + 3 * (col - 1);
+      double fx = sf[0], fy = sf[1], fz = sf[2];
+      if (std::sqrt(fx * fx + fy * fy + fz * fz) * tN < sigmaPN) fx = fy = fz = 0;
+      f[0] += fx; f[1] += fy; f[2] += fz;
+    }
+    nodeForce[3 * n] = f[0];
+    nodeForce[3 * n + 1] = f[1];
+    nodeForce[3 * n + 2] = f[2];
+  }
+}
+
+}  // extern "C"

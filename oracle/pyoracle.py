@@ -1,0 +1,243 @@
+"""ctypes front end of the CPU oracle.  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this module, and only as the checker; the product path
+(the HIP library and its Python mirror) never does.
+
+Every function takes the same host structs as the product API
+(``dddhip.types``) and returns arrays in the reference's layout.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "libddd_oracle.so"
+_lib = None
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int64)
+up = C.POINTER(C.c_uint8)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = C.CDLL(str(LIB))
+        i64, d, i32 = C.c_int64, C.c_double, C.c_int
+        mesh = [dp, dp, ip, i64, i64, i64, d, d, d, dp]
+        net = [i64, ip, i64, dp, dp, up]
+        sig = {
+            "orc_segseg_pair": (i32, [i32, dp] + [dp] * 7),
+            "orc_count_segseg_pair": (i32, [dp] * 8),
+            "orc_count_sigma_tilde": (None, [dp] * 6),
+            "orc_count_self_force": (None, [dp] * 4),
+            "orc_segseg_ser": (None, [i32, dp] + net + [dp]),
+            "orc_segseg_par_faithful": (None, [dp] + net + [dp, i32]),
+            "orc_segseg_idx": (None, [dp] + net + [ip, i64, dp, i32]),
+            "orc_self_force": (None, [dp] + net + [ip, i64, dp, i32]),
+            "orc_sigma_hat": (None, mesh + [dp, dp]),
+            "orc_pk_force": (None, mesh + net + [ip, i64, dp, i32, i32]),
+            "orc_sigma_tilde": (None, [dp, i64, ip, i64, dp, dp, ip, i64, dp, i64, dp, i32]),
+            "orc_node_gather": (None, [ip, i64, i64, ip, dp, dp, d, dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(_lib, name)
+            f.restype = res
+            f.argtypes = args
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(ip)
+
+
+def _u(a):
+    return a.ctypes.data_as(up)
+
+
+def _f(a, dt=np.float64):
+    return np.asfortranarray(a, dtype=dt)
+
+
+def seg_consts(dln, mat):
+    """(aSq, mu4pi, mu8pi, aSq mu8pi, mu4pinu, aSq mu4pinu): DislocationForces.jl:670-718."""
+    aSq = dln.coreRadSq
+    return np.array([aSq, mat.μ4π, mat.μ8π, aSq * mat.μ8π, mat.μ4πν, aSq * mat.μ4πν])
+
+
+def self_consts(dln, mat):
+    return np.array([dln.coreRad, dln.coreRadSq, mat.μ4π, mat.νomνInv, dln.coreEnergy, mat.omνInv])
+
+
+def sigma_consts(dln, mat):
+    return np.array([dln.coreRadSq, mat.μ4π, mat.μ4πν, mat.νμ4πν, dln.coreRadSq * mat.μ8π])
+
+
+class _Net:
+    def __init__(self, net):
+        self.numSeg = int(net.numSeg[0])
+        self.segIdx = _f(net.segIdx, np.int64)
+        self.ld = self.segIdx.shape[0]
+        self.coord = _f(net.coord)
+        self.bVec = _f(net.bVec)
+        self.ext = np.ascontiguousarray(net.extSeg, dtype=np.uint8)
+
+
+def _idx(idx):
+    if idx is None:
+        return None, 0
+    a = np.atleast_1d(np.asarray(list(idx) if isinstance(idx, range) else idx, dtype=np.int64))
+    return np.ascontiguousarray(a), len(a)
+
+
+def pair(b1, n11, n12, b2, n21, n22, consts, prec=0):
+    """F1..F4 (4 x 3) and whether the near-parallel branch ran."""
+    out = np.zeros(12)
+    v = [np.ascontiguousarray(x, dtype=np.float64) for x in (b1, n11, n12, b2, n21, n22)]
+    k = np.ascontiguousarray(consts, dtype=np.float64)
+    par = lib().orc_segseg_pair(prec, _d(k), *[_d(x) for x in v], _d(out))
+    return out.reshape(4, 3), bool(par)
+
+
+def count_pair(b1, n11, n12, b2, n21, n22, consts):
+    out = np.zeros(7)
+    v = [np.ascontiguousarray(x, dtype=np.float64) for x in (b1, n11, n12, b2, n21, n22)]
+    k = np.ascontiguousarray(consts, dtype=np.float64)
+    par = lib().orc_count_segseg_pair(_d(k), *[_d(x) for x in v], _d(out))
+    keys = ("flops", "add", "mul", "div", "sqrt", "log", "atan")
+    return dict(zip(keys, out.tolist())), bool(par)
+
+
+def count_sigma_tilde(sc, b, n1, n2, x):
+    out = np.zeros(7)
+    v = [np.ascontiguousarray(a, dtype=np.float64) for a in (sc, b, n1, n2, x)]
+    lib().orc_count_sigma_tilde(*[_d(a) for a in v], _d(out))
+    return dict(zip(("flops", "add", "mul", "div", "sqrt", "log", "atan"), out.tolist()))
+
+
+def count_self_force(sc, tv, b):
+    out = np.zeros(7)
+    v = [np.ascontiguousarray(a, dtype=np.float64) for a in (sc, tv, b)]
+    lib().orc_count_self_force(*[_d(a) for a in v], _d(out))
+    return dict(zip(("flops", "add", "mul", "div", "sqrt", "log", "atan"), out.tolist()))
+
+
+def calcSegSegForce(dln, mat, net, idx=None, prec=0):
+    """Out-of-place seg-seg force: 3 x 2 x numSeg (serial, :778-829) or
+    3 x 2 x length(idx) (rows only, :831-877)."""
+    n = _Net(net)
+    K = seg_consts(dln, mat)
+    if idx is None:
+        out = np.zeros((3, 2, n.numSeg), order="F")
+        lib().orc_segseg_ser(prec, _d(K), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord), _d(n.bVec),
+                             _u(n.ext), _d(out))
+        return out
+    ix, ni = _idx(idx)
+    out = np.zeros((3, 2, ni), order="F")
+    lib().orc_segseg_idx(_d(K), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord), _d(n.bVec),
+                         _u(n.ext), _i(ix), ni, _d(out), 1)
+    return out
+
+
+def calcSegSegForcePar(dln, mat, net, nthreads):
+    """The reference's threaded design (:719-777): static chunks + atomic adds."""
+    n = _Net(net)
+    K = seg_consts(dln, mat)
+    out = np.zeros((3, 2, n.numSeg), order="F")
+    lib().orc_segseg_par_faithful(_d(K), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord), _d(n.bVec),
+                                  _u(n.ext), _d(out), int(nthreads))
+    return out
+
+
+def calcSelfForce(dln, mat, net, idx=None):
+    n = _Net(net)
+    ix, ni = _idx(idx)
+    cnt = ni if ix is not None else n.numSeg
+    out = np.zeros((3, 2, cnt), order="F")
+    lib().orc_self_force(_d(self_consts(dln, mat)), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord),
+                         _d(n.bVec), _u(n.ext), _i(ix) if ix is not None else None, ni, _d(out), 1)
+    return out[:, 0, :].copy(order="F"), out[:, 1, :].copy(order="F")
+
+
+def _mesh_args(mesh, fd):
+    return (_d(_f(mesh.C)), _d(_f(mesh.coord)), _i(_f(mesh.connectivity, np.int64)),
+            int(mesh.mx), int(mesh.my), int(mesh.mz), C.c_double(mesh.w), C.c_double(mesh.h),
+            C.c_double(mesh.d), _d(np.ascontiguousarray(fd.uHat, dtype=np.float64)))
+
+
+def calc_σHat(mesh, fd, x0):
+    out = np.zeros(9)
+    x = np.ascontiguousarray(x0, dtype=np.float64)
+    keep = (_f(mesh.C), _f(mesh.coord), _f(mesh.connectivity, np.int64),
+            np.ascontiguousarray(fd.uHat, dtype=np.float64))
+    lib().orc_sigma_hat(_d(keep[0]), _d(keep[1]), _i(keep[2]), int(mesh.mx), int(mesh.my),
+                        int(mesh.mz), C.c_double(mesh.w), C.c_double(mesh.h), C.c_double(mesh.d), _d(keep[3]),
+                        _d(x), _d(out))
+    return out.reshape(3, 3, order="F")
+
+
+def calcPKForce(mesh, fd, net, idx=None):
+    n = _Net(net)
+    ix, ni = _idx(idx)
+    cnt = ni if ix is not None else n.numSeg
+    out = np.zeros((3, cnt), order="F")
+    keep = (_f(mesh.C), _f(mesh.coord), _f(mesh.connectivity, np.int64),
+            np.ascontiguousarray(fd.uHat, dtype=np.float64))
+    lib().orc_pk_force(_d(keep[0]), _d(keep[1]), _i(keep[2]), int(mesh.mx), int(mesh.my),
+                       int(mesh.mz), C.c_double(mesh.w), C.c_double(mesh.h), C.c_double(mesh.d), _d(keep[3]),
+                       n.numSeg, _i(n.segIdx), n.ld, _d(n.coord), _d(n.bVec), _u(n.ext),
+                       _i(ix) if ix is not None else None, ni, _d(out), 1, 0)
+    return out
+
+
+def calcSegForce(dln, mat, mesh, fd, net, idx=None):
+    """segseg + self + PK/2 (DislocationForces.jl:23-46)."""
+    ss = calcSegSegForce(dln, mat, net, idx)
+    s1, s2 = calcSelfForce(dln, mat, net, idx)
+    pk = calcPKForce(mesh, fd, net, idx)
+    ss[:, 0, :] += s1 + 0.5 * pk
+    ss[:, 1, :] += s2 + 0.5 * pk
+    return ss
+
+
+def calc_σTilde(x0, dln, mat, net, idx=None, nthreads=None):
+    """6 x P stress (xx, yy, zz, xy, xz, yz): DislocationFEM.jl:24-41,86-303."""
+    n = _Net(net)
+    x = _f(np.asarray(x0, dtype=np.float64).reshape(3, -1, order="F"))
+    P = x.shape[1]
+    ix, ni = _idx(idx)
+    out = np.zeros((6, P), order="F")
+    nt = nthreads or os.cpu_count() or 1
+    lib().orc_sigma_tilde(_d(sigma_consts(dln, mat)), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord),
+                          _d(n.bVec), _i(ix) if ix is not None else None, ni, _d(x), P, _d(out),
+                          int(nt))
+    return out
+
+
+def nodeForce(mat, net, segForce=None):
+    """Node-force gather of dlnMobility! (DislocationMobility.jl:220-255)."""
+    sf = _f(net.segForce if segForce is None else segForce)
+    conn = _f(net.connectivity, np.int64)
+    links = _f(net.links, np.int64)
+    coord = _f(net.coord)
+    numNode = int(net.numNode[0])
+    out = np.zeros((3, numNode), order="F")
+    lib().orc_node_gather(_i(conn), conn.shape[0], numNode, _i(links), _d(coord), _d(sf),
+                          C.c_double(mat.σPN), _d(out))
+    return out
