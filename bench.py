@@ -1,0 +1,165 @@
+"""Force-evaluation benchmark on one MI355X.
+
+Workload: 4,096 shear loops x 32 segments = 131,072 segments (seeded
+generator, seed 0xD0; loops of radius U(100, 300) on random BCC {110}<111>
+slip systems, centres uniform in a 2000^3 cube; mu = 1, nu = 0.28, core radius
+90.78, Ec = 1).  One step = one whole-network force evaluation (calcSegForce!
+without an FE mesh: self force + all N(N-1)/2 segment pairs + reduction into
+segForce) with the network already resident in HBM.
+
+Prints one JSON line.  The CPU baseline is the reference's threaded design
+(static outer-row chunks, atomic adds) restated in C++ and timed on a bounded
+sample of outer rows on this host's cores, extrapolated by exact pair count.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT / "ddd.jl_amd", ROOT / "oracle", ROOT / "tests"):
+    sys.path.insert(0, str(p))
+
+METRIC = "Force-eval time @131k segs: plain FP64 vs hybrid at ≤1e-3 RMS rel force err"
+FLOPS_PER_PAIR = 1466  # counted by the operation-counting oracle (generic pair)
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--loops", type=int, default=4096)
+    ap.add_argument("--seed", type=lambda s: int(s, 0), default=0xD0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-row-stride", type=int, default=64)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU layout is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from dddhip import forces as G
+    from dddhip import _lib
+    from dddhip.generate import random_loop_network
+    from dddhip.types import DislocationParameters, MaterialParameters
+
+    dln = DislocationParameters(coreRad=90.78, coreEnergy=1.0)
+    mat = MaterialParameters(μ=1.0, ν=0.28)
+    t0 = time.time()
+    net = random_loop_network(args.loops, seed=args.seed)
+    N = int(net.numSeg[0])
+    pairs = N * (N - 1) // 2
+    log(f"network: {N} segments, {pairs} pairs (generated in {time.time() - t0:.1f} s)")
+
+    torch.cuda.init()
+    ctx = _lib.Context(0)
+    res = G.ResidentNetwork(dln, mat, net, ctx=ctx)
+    # HIP events are recorded on the context's own stream, the one every
+    # kernel of the evaluation is launched on.
+    stream = torch.cuda.ExternalStream(res.stream(), device=torch.device("cuda:0"))
+    log("uploaded")
+
+    for w in range(args.warmup):
+        res.seg_force()
+        stream.synchronize()
+        log(f"warmup {w + 1}/{args.warmup} done")
+
+    # Timed region: K whole evaluations; per-stage HIP events on the same
+    # stream time the pair-tile kernel on its own.
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    stream.synchronize()
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        e = ev[k]
+        e[0].record(stream)
+        res.stage(0)
+        e[1].record(stream)
+        res.stage(1)
+        e[2].record(stream)
+        res.stage(2)
+        e[3].record(stream)
+        e[3].synchronize()
+        log(f"step {k + 1}/{args.steps}: {e[0].elapsed_time(e[3]):.1f} ms")
+    stream.synchronize()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t_start) * 1e3 / args.steps
+    step_ms = [e[0].elapsed_time(e[3]) for e in ev]
+    tile_ms = [e[1].elapsed_time(e[2]) for e in ev]
+    red_ms = [e[2].elapsed_time(e[3]) for e in ev]
+    ms = float(np.mean(step_ms))
+    tms = float(np.mean(tile_ms))
+    out = res.download()
+    assert np.all(np.isfinite(out)), "non-finite forces"
+
+    achieved = FLOPS_PER_PAIR * pairs / (tms * 1e-3) / 1e12
+    result = {
+        "metric": METRIC,
+        "value": round(ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "wall_ms_per_step": round(wall, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.loops} shear loops x 32 = {N} segments (seed {args.seed:#x}), "
+                        "plain FP64 self + seg-seg force, network resident in HBM",
+            "segments": N,
+            "pairs": pairs,
+            "path": "plain-fp64",
+            "pairs_per_s": round(pairs / (ms * 1e-3), 1),
+            "tile_kernel_ms": round(tms, 3),
+            "reduce_kernel_ms": round(float(np.mean(red_ms)), 3),
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(achieved, 3),
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+            "traffic": None,
+            "kernel": "k_segseg_tiles",
+            "flops_per_pair": FLOPS_PER_PAIR,
+        },
+    }
+
+    if not args.no_cpu:
+        import pyoracle as O
+        nthreads = max(1, min(16, os.cpu_count() or 1))
+        rows = np.arange(1, N + 1, args.cpu_row_stride, dtype=np.int64)
+        log(f"cpu baseline: {len(rows)} rows on {nthreads} threads")
+        t = time.perf_counter()
+        _, sp = O.segseg_par_rows(dln, mat, net, rows, nthreads)
+        dt = time.perf_counter() - t
+        full_ms = dt * (pairs / sp) * 1e3
+        result["cpu_baseline"] = {
+            "value": round(full_ms, 1),
+            "unit": "ms",
+            "cores": nthreads,
+            "kind": "port",
+            "sample": f"{len(rows)} of {N} outer rows (every {args.cpu_row_stride}th) each against "
+                      f"all j > i: {sp} pairs in {dt:.2f} s, extrapolated to {pairs} pairs",
+        }
+        result["cpu_baseline"]["speedup_gpu_vs_cpu"] = round(full_ms / ms, 1)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,324 @@
+// Per-segment, per-node and field-point kernels: self force, Peach-Koehler
+// force from the FE correction stress, node-force gather, and the
+// field-point stress sum.
+#include "kernels.hpp"
+
+namespace ddd {
+
+namespace {
+
+// _calcSelfForce (DislocationForces.jl:414-455).  Node 1 receives -f, node 2
+// +f (:381-384).  rows[k] is the segment of output column k (or k itself
+// when rows is null).
+__global__ void k_self_force(const double* __restrict__ seg, int64_t ld,
+                             const int64_t* __restrict__ rows, int n, SelfK K,
+                             double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t i = rows ? rows[k] : k;
+  if (i < 0 || seg[F_EXT * ld + i] != 0.0) return;
+  const v3<double> n1 = V(seg[F_N1X * ld + i], seg[F_N1Y * ld + i], seg[F_N1Z * ld + i]);
+  const v3<double> n2 = V(seg[F_N2X * ld + i], seg[F_N2Y * ld + i], seg[F_N2Z * ld + i]);
+  const v3<double> b = V(seg[F_BX * ld + i], seg[F_BY * ld + i], seg[F_BZ * ld + i]);
+  const v3<double> tv = n2 - n1;
+  const double tvSq = dot(tv, tv);
+  const double L = sqrt(tvSq);
+  const double Linv = 1.0 / L;
+  const v3<double> t = tv * Linv;
+  const double La = sqrt(tvSq + K.aSq);
+  const double bScrew = dot(t, b);
+  const v3<double> bEdge = b - bScrew * t;
+  const double bEdgeSq = dot(bEdge, bEdge);
+  const double LaMa = La - K.a;
+  const double tor = K.mu4pi * bScrew *
+                     (K.nuOmNuInv * (ulog((La + L) / K.a) - 2.0 * LaMa * Linv) -
+                      LaMa * LaMa / (2.0 * La * L));
+  const double torCore = 2.0 * K.Ec * K.nuOmNuInv * bScrew;
+  const double lonCore = (bScrew * bScrew + bEdgeSq * K.omNuInv) * K.Ec;
+  const v3<double> f = (tor + torCore) * bEdge - lonCore * t;
+  double* o = out + 6 * (int64_t)k;
+  o[0] -= f.x; o[1] -= f.y; o[2] -= f.z;
+  o[3] += f.x; o[4] += f.y; o[5] += f.z;
+}
+
+// calc_sigmaHat (DislocationForces.jl:92-192) at x: element by ceil/clamp,
+// trilinear shape-function derivatives (FEMShapeFunction.jl:83-122), strain
+// B U of the element's 24 uHat dofs, then C times strain; Voigt order
+// (xx, yy, zz, xy, xz, yz).
+__device__ void sigma_hat(const MeshDev& M, v3<double> x, double sv[6]) {
+  const double wInv = 1.0 / M.w, hInv = 1.0 / M.h, dInv = 1.0 / M.d;
+  auto cl = [](double v, int64_t hi) {
+    double c = ceil(v);
+    c = c < 1.0 ? 1.0 : c;
+    c = c > (double)hi ? (double)hi : c;
+    return (int64_t)c;
+  };
+  const int64_t ei = cl(x.x * wInv, M.mx), ej = cl(x.y * hInv, M.my), ek = cl(x.z * dInv, M.mz);
+  const int64_t e = ei + (ej - 1) * M.mx + (ek - 1) * M.mx * M.my;
+  const int64_t* ce = M.conn + 8 * (e - 1);
+  const double* c1 = M.coord + 3 * (ce[0] - 1);
+  const double* c7 = M.coord + 3 * (ce[6] - 1);
+  const double ds1dx = 2.0 * wInv, ds2dy = 2.0 * hInv, ds3dz = 2.0 * dInv;
+  const double s1 = (x.x - 0.5 * (c1[0] + c7[0])) * ds1dx;
+  const double s2 = (x.y - 0.5 * (c1[1] + c7[1])) * ds2dy;
+  const double s3 = (x.z - 0.5 * (c1[2] + c7[2])) * ds3dz;
+  const double omx = 1 - s1, omy = 1 - s2, omz = 1 - s3, opx = 1 + s1, opy = 1 + s2, opz = 1 + s3;
+  const double dN[3][8] = {
+      {-omy * omz, omy * omz, opy * omz, -opy * omz, -omy * opz, omy * opz, opy * opz, -opy * opz},
+      {-omx * omz, -opx * omz, opx * omz, omx * omz, -omx * opz, -opx * opz, opx * opz, omx * opz},
+      {-omx * omy, -opx * omy, -opx * opy, -omx * opy, omx * omy, opx * omy, opx * opy, omx * opy}};
+  double eps[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const double* u = M.uHat + 3 * (ce[q] - 1);
+    const double gx = dN[0][q] / 8 * ds1dx, gy = dN[1][q] / 8 * ds2dy, gz = dN[2][q] / 8 * ds3dz;
+    eps[0] += gx * u[0];
+    eps[1] += gy * u[1];
+    eps[2] += gz * u[2];
+    eps[3] += gy * u[0] + gx * u[1];
+    eps[4] += gz * u[0] + gx * u[2];
+    eps[5] += gz * u[1] + gy * u[2];
+  }
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    double a = 0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a += M.C[r + 6 * c] * eps[c];
+    sv[r] = a;
+  }
+}
+
+// _calcPKForce (:292-300): pk = (sigmaHat(mid) b) x (n2 - n1), unnormalised
+// tangent.  mode 0 writes pk to a 3 x n array (calcPKForce); mode 1 adds pk/2
+// to both nodes of a 3 x 2 x n array (calcPKForce!, :266-269).
+__global__ void k_pk_force(const double* __restrict__ seg, int64_t ld,
+                           const int64_t* __restrict__ rows, int n, MeshDev M,
+                           double* __restrict__ out, int mode) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t i = rows ? rows[k] : k;
+  if (i < 0 || seg[F_EXT * ld + i] != 0.0) return;
+  const v3<double> n1 = V(seg[F_N1X * ld + i], seg[F_N1Y * ld + i], seg[F_N1Z * ld + i]);
+  const v3<double> n2 = V(seg[F_N2X * ld + i], seg[F_N2Y * ld + i], seg[F_N2Z * ld + i]);
+  const v3<double> b = V(seg[F_BX * ld + i], seg[F_BY * ld + i], seg[F_BZ * ld + i]);
+  const v3<double> t = n2 - n1;
+  const v3<double> mid = V((n2.x + n1.x) / 2, (n2.y + n1.y) / 2, (n2.z + n1.z) / 2);
+  double s[6];
+  sigma_hat(M, mid, s);
+  const v3<double> sb = V(s[0] * b.x + s[3] * b.y + s[4] * b.z, s[3] * b.x + s[1] * b.y + s[5] * b.z,
+                          s[4] * b.x + s[5] * b.y + s[2] * b.z);
+  const v3<double> pk = cross(sb, t);
+  if (mode == 0) {
+    double* o = out + 3 * (int64_t)k;
+    o[0] += pk.x; o[1] += pk.y; o[2] += pk.z;
+  } else {
+    double* o = out + 6 * (int64_t)k;
+    o[0] += pk.x / 2; o[1] += pk.y / 2; o[2] += pk.z / 2;
+    o[3] += pk.x / 2; o[4] += pk.y / 2; o[5] += pk.z / 2;
+  }
+}
+
+// Field-point stress: 12-field SoA per segment (n1, n2, b, t) with t the
+// normalised tangent of :133.
+enum { S_N1 = 0, S_N2 = 3, S_B = 6, S_T = 9, S_COUNT = 12 };
+
+__global__ void k_prep_sigma_segments(double* __restrict__ s, int64_t ld, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dx = s[(S_N2 + 0) * ld + i] - s[(S_N1 + 0) * ld + i];
+  const double dy = s[(S_N2 + 1) * ld + i] - s[(S_N1 + 1) * ld + i];
+  const double dz = s[(S_N2 + 2) * ld + i] - s[(S_N1 + 2) * ld + i];
+  const double nn = __dsqrt_rn(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)),
+                                         __dmul_rn(dz, dz)));
+  s[(S_T + 0) * ld + i] = __ddiv_rn(dx, nn);
+  s[(S_T + 1) * ld + i] = __ddiv_rn(dy, nn);
+  s[(S_T + 2) * ld + i] = __ddiv_rn(dz, nn);
+}
+
+constexpr int kSigBlock = 256;
+constexpr int kSigTile = 128;
+
+// Stress of segments [c * chunk, (c + 1) * chunk) at one point per thread,
+// summed in segment order (the reference's order for each point).
+__global__ __launch_bounds__(kSigBlock) void k_sigma_tilde(const double* __restrict__ sseg,
+                                                           int64_t sld, int nseg, int chunk,
+                                                           const double* __restrict__ x0,
+                                                           int64_t P, SigmaK K,
+                                                           double* __restrict__ partial) {
+  __shared__ double sh[S_COUNT][kSigTile];
+  const int64_t p = blockIdx.x * (int64_t)kSigBlock + threadIdx.x;
+  const int c = blockIdx.y;
+  const int s0 = c * chunk;
+  const int s1 = min(nseg, s0 + chunk);
+  const bool live = p < P;
+  const v3<double> x = live ? V(x0[3 * p], x0[3 * p + 1], x0[3 * p + 2]) : V(0.0, 0.0, 0.0);
+  double sg[6] = {0, 0, 0, 0, 0, 0};
+  for (int base = s0; base < s1; base += kSigTile) {
+    const int cnt = min(kSigTile, s1 - base);
+    __syncthreads();
+    for (int q = threadIdx.x; q < S_COUNT * kSigTile; q += kSigBlock) {
+      const int f = q / kSigTile, l = q % kSigTile;
+      sh[f][l] = (l < cnt) ? sseg[f * sld + base + l] : 0.0;
+    }
+    __syncthreads();
+    for (int l = 0; l < cnt; ++l) {
+      const v3<double> n1 = V(sh[S_N1][l], sh[S_N1 + 1][l], sh[S_N1 + 2][l]);
+      const v3<double> n2 = V(sh[S_N2][l], sh[S_N2 + 1][l], sh[S_N2 + 2][l]);
+      const v3<double> b = V(sh[S_B][l], sh[S_B + 1][l], sh[S_B + 2][l]);
+      const v3<double> t = V(sh[S_T][l], sh[S_T + 1][l], sh[S_T + 2][l]);
+      // DislocationFEM.jl:142-299
+      const v3<double> R = x - n1;
+      const double Rdt = dot(R, t);
+      const v3<double> nd = R - Rdt * t;
+      const double d2 = dot(nd, nd);
+      const double sa = -Rdt;
+      const double sb = -dot(x - n2, t);
+      const double a2_d2 = K.a2 + d2;
+      const double a2d2inv = 1.0 / a2_d2;
+      double Ra = sqrt(a2_d2 + sa * sa);
+      double Rainv = 1.0 / Ra;
+      double Ra3inv = Rainv * Rainv * Rainv;
+      double sRa3inv = sa * Ra3inv;
+      const double s03a = sa * Rainv * a2d2inv;
+      const double s13a = -Rainv;
+      const double s05a = (2.0 * s03a + sRa3inv) * a2d2inv;
+      const double s15a = -Ra3inv;
+      const double s25a = s03a - sRa3inv;
+      Ra = sqrt(a2_d2 + sb * sb);
+      Rainv = 1.0 / Ra;
+      Ra3inv = Rainv * Rainv * Rainv;
+      sRa3inv = sb * Ra3inv;
+      const double s03b = sb * Rainv * a2d2inv;
+      const double s13b = -Rainv;
+      const double s05b = (2.0 * s03b + sRa3inv) * a2d2inv;
+      const double s15b = -Ra3inv;
+      const double s25b = s03b - sRa3inv;
+      const double s03 = s03b - s03a, s13 = s13b - s13a, s05 = s05b - s05a, s15 = s15b - s15a,
+                   s25 = s25b - s25a;
+      const v3<double> txb = cross(t, b);
+      const v3<double> dxb = cross(nd, b);
+      const double common = K.mu4pinu * dot(dxb, t);
+      const double tt[3] = {t.x, t.y, t.z}, dd[3] = {nd.x, nd.y, nd.z};
+      const double xb[3] = {txb.x, txb.y, txb.z}, db[3] = {dxb.x, dxb.y, dxb.z};
+      const int PP[6] = {0, 1, 2, 0, 0, 1}, QQ[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const int a = PP[q], bq = QQ[q];
+        const bool dg = q < 3;
+        const double dmd = dd[a] * dd[bq];
+        const double tmt = tt[a] * tt[bq];
+        const double tmd = dg ? 2.0 * tt[a] * dd[a] : tt[a] * dd[bq] + tt[bq] * dd[a];
+        const double tmtxb = dg ? 2.0 * tt[a] * xb[a] : tt[a] * xb[bq] + tt[bq] * xb[a];
+        const double dmtxb = dg ? 2.0 * dd[a] * xb[a] : dd[a] * xb[bq] + dd[bq] * xb[a];
+        const double tmdxb = dg ? 2.0 * tt[a] * db[a] : tt[a] * db[bq] + tt[bq] * db[a];
+        const double I03 = (dg ? common : 0.0) + K.mu4pinu * dmtxb - K.mu4pi * tmdxb;
+        const double I13 = -K.nuMu4pinu * tmtxb;
+        const double I05 = common * ((dg ? K.a2 : 0.0) + dmd) - K.a2mu8pi * tmdxb;
+        const double I15 = K.a2mu8pi * tmtxb - common * tmd;
+        const double I25 = common * tmt;
+        sg[q] += I03 * s03 + I13 * s13 + I05 * s05 + I15 * s15 + I25 * s25;
+      }
+    }
+  }
+  if (live) {
+    double* o = partial + ((int64_t)c * P + p) * 6;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) o[q] = sg[q];
+  }
+}
+
+__global__ void k_sigma_reduce(const double* __restrict__ partial, int nChunk, int64_t P,
+                               double* __restrict__ sigma, int accumulate) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= 6 * P) return;
+  double a = 0.0;
+  for (int c = 0; c < nChunk; ++c) a += partial[(int64_t)c * 6 * P + e];
+  sigma[e] = accumulate ? sigma[e] + a : a;
+}
+
+// dlnMobility! node-force gather (DislocationMobility.jl:220-255).
+__global__ void k_node_gather(const int64_t* __restrict__ conn, int64_t ldConn, int64_t numNode,
+                              const int64_t* __restrict__ links, const double* __restrict__ coord,
+                              const double* __restrict__ segForce, double sigmaPN,
+                              double* __restrict__ nodeForce) {
+  const int64_t nd = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (nd >= numNode) return;
+  const int64_t* cn = conn + ldConn * nd;
+  double f0 = 0, f1 = 0, f2 = 0;
+  const int64_t cnt = cn[0];
+  for (int64_t c = 1; c <= cnt; ++c) {
+    const int64_t link = cn[2 * c - 1], col = cn[2 * c];
+    const int64_t node2 = links[2 * (link - 1) + (2 - col)];
+    const double tx = coord[3 * (node2 - 1)] - coord[3 * nd];
+    const double ty = coord[3 * (node2 - 1) + 1] - coord[3 * nd + 1];
+    const double tz = coord[3 * (node2 - 1) + 2] - coord[3 * nd + 2];
+    const double nT = sqrt(tx * tx + ty * ty + tz * tz);
+    if (nT < 2.220446049250313e-16) continue;
+    const double* sf = segForce + 6 * (link - 1) + 3 * (col - 1);
+    double gx = sf[0], gy = sf[1], gz = sf[2];
+    if (sqrt(gx * gx + gy * gy + gz * gz) * (1.0 / nT) < sigmaPN) gx = gy = gz = 0.0;
+    f0 += gx;
+    f1 += gy;
+    f2 += gz;
+  }
+  nodeForce[3 * nd] = f0;
+  nodeForce[3 * nd + 1] = f1;
+  nodeForce[3 * nd + 2] = f2;
+}
+
+}  // namespace
+
+void launch_self_force(const double* seg, int64_t ld, const int64_t* rows, int n, const SelfK& K,
+                       double* out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_self_force, dim3((n + 255) / 256), dim3(256), 0, st, seg, ld, rows, n, K,
+                     out);
+}
+
+void launch_pk_force(const double* seg, int64_t ld, const int64_t* rows, int n, const MeshDev& M,
+                     double* out, int mode, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_pk_force, dim3((n + 255) / 256), dim3(256), 0, st, seg, ld, rows, n, M,
+                     out, mode);
+}
+
+void launch_prep_sigma_segments(double* sseg, int64_t ld, int n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_prep_sigma_segments, dim3((n + 255) / 256), dim3(256), 0, st, sseg, ld, n);
+}
+
+int sigma_chunks_for(int64_t P, int nseg) {
+  // Enough blocks to fill the device several times over, each chunk still
+  // long enough to amortise its LDS staging.
+  const int64_t pblocks = (P + kSigBlock - 1) / kSigBlock;
+  int c = 1;
+  while (pblocks * c < 4096 && nseg / (2 * c) >= 2 * kSigTile) c *= 2;
+  return c;
+}
+
+void launch_sigma_tilde(const double* sseg, int64_t sld, int nseg, const double* x0, int64_t P,
+                        const SigmaK& K, double* partial, int nChunk, hipStream_t st) {
+  if (P <= 0) return;
+  const int chunk = (nseg + nChunk - 1) / nChunk;
+  dim3 grid((unsigned)((P + kSigBlock - 1) / kSigBlock), (unsigned)nChunk);
+  hipLaunchKernelGGL(k_sigma_tilde, grid, dim3(kSigBlock), 0, st, sseg, sld, nseg, chunk, x0, P, K,
+                     partial);
+}
+
+void launch_sigma_reduce(const double* partial, int nChunk, int64_t P, double* sigma,
+                         int accumulate, hipStream_t st) {
+  if (P <= 0) return;
+  const int64_t n = 6 * P;
+  hipLaunchKernelGGL(k_sigma_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, partial,
+                     nChunk, P, sigma, accumulate);
+}
+
+void launch_node_gather(const int64_t* conn, int64_t ldConn, int64_t numNode,
+                        const int64_t* links, const double* coord, const double* segForce,
+                        double sigmaPN, double* nodeForce, hipStream_t st) {
+  if (numNode <= 0) return;
+  hipLaunchKernelGGL(k_node_gather, dim3((unsigned)((numNode + 255) / 256)), dim3(256), 0, st, conn,
+                     ldConn, numNode, links, coord, segForce, sigmaPN, nodeForce);
+}
+
+}  // namespace ddd

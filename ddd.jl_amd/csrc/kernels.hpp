@@ -1,0 +1,69 @@
+// Internal launch interface between the C-ABI layer and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pair_math.hpp"
+
+namespace ddd {
+
+constexpr int kTile = 64;  // segments per tile side (one wavefront)
+
+// Per-segment SoA record of the pair kernels: field f of segment i at
+// seg[f * ld + i].  Fields 0-8 are filled by the host (n1, n2, b), 9-12 by
+// prep_segments (unit tangent, 1/L), 13 is 1.0 for an external segment.
+enum SegField {
+  F_N1X = 0, F_N1Y, F_N1Z, F_N2X, F_N2Y, F_N2Z, F_BX, F_BY, F_BZ,
+  F_TX, F_TY, F_TZ, F_TN, F_EXT, F_COUNT
+};
+
+// Error bits reported back through the context.
+enum : unsigned {
+  ERRBIT_NESTED_PARALLEL = 1u,  // a corrective sub-pair was itself near-parallel
+};
+
+__host__ __device__ inline int64_t tiles_for(int64_t nb) { return nb * (nb + 1) / 2; }
+
+void launch_prep_segments(double* seg, int64_t ld, int64_t n, hipStream_t st);
+void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
+                         double* rowPart, double* colPart, unsigned* flags, hipStream_t st);
+void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, double* out,
+                          hipStream_t st);
+void launch_segseg_rows(const double* seg, int64_t ld, int n, const int64_t* rows, int nRows,
+                        const SegK<double>& K, double* out, unsigned* flags, hipStream_t st);
+
+struct SelfK {
+  double a, aSq, mu4pi, nuOmNuInv, Ec, omNuInv;
+};
+void launch_self_force(const double* seg, int64_t ld, const int64_t* rows, int n,
+                       const SelfK& K, double* out, hipStream_t st);
+
+struct MeshDev {
+  const double* C;       // 36, column-major
+  const double* coord;   // 3 x numNode
+  const int64_t* conn;   // 8 x numElem, 1-based
+  const double* uHat;    // 3 numNode
+  int64_t mx, my, mz;
+  double w, h, d;
+};
+// mode 0: out is 3 x n (pk per segment); mode 1: out is 6 x n, pk/2 to both nodes.
+void launch_pk_force(const double* seg, int64_t ld, const int64_t* rows, int n, const MeshDev& M,
+                     double* out, int mode, hipStream_t st);
+
+struct SigmaK {
+  double a2, mu4pi, mu4pinu, nuMu4pinu, a2mu8pi;
+};
+// Stress at P points (3 x P) from nseg segments given as a 12-field SoA
+// (n1, n2, b, t); partial holds nChunk x P x 6 doubles.
+int sigma_chunks_for(int64_t P, int nseg);
+void launch_sigma_tilde(const double* sseg, int64_t sld, int nseg, const double* x0, int64_t P,
+                        const SigmaK& K, double* partial, int nChunk, hipStream_t st);
+void launch_sigma_reduce(const double* partial, int nChunk, int64_t P, double* sigma,
+                         int accumulate, hipStream_t st);
+void launch_prep_sigma_segments(double* sseg, int64_t ld, int n, hipStream_t st);
+
+void launch_node_gather(const int64_t* conn, int64_t ldConn, int64_t numNode,
+                        const int64_t* links, const double* coord, const double* segForce,
+                        double sigmaPN, double* nodeForce, hipStream_t st);
+
+}  // namespace ddd

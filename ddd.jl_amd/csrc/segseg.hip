@@ -1,0 +1,332 @@
+// FP64 segment-segment force kernels.
+//
+// Work decomposition: the upper triangle of segment pairs (i < j, the pairs
+// calcSegSegForceSer! visits, DislocationForces.jl:794-800) is cut into
+// 64 x 64 tiles, one wavefront per tile.  Lane l owns row i = 64 I + l and
+// holds its node forces in registers; at step k it pairs with column slot
+// s = (l + k) mod 64 of the J block, so within one step every lane touches a
+// different column and the column forces accumulate in LDS without atomics.
+// Each tile writes its row and column partial sums; a second kernel adds the
+// partials of every segment in a fixed order, so results are bitwise
+// reproducible run to run.
+//
+// Near-parallel pairs (1 - c^2 <= sqrt(eps), the reference's special branch
+// :1147-1162) are not evaluated inline, which would stall the whole wavefront
+// on one lane.  They are queued in LDS and evaluated 64 at a time, one per
+// lane, then added to their row and column in queue order.
+#include "kernels.hpp"
+
+namespace ddd {
+
+namespace {
+
+__device__ __forceinline__ Seg<double> seg_from_lds(const double (*s)[kTile], int l) {
+  Seg<double> r;
+  r.n1 = V(s[F_N1X][l], s[F_N1Y][l], s[F_N1Z][l]);
+  r.n2 = V(s[F_N2X][l], s[F_N2Y][l], s[F_N2Z][l]);
+  r.b = V(s[F_BX][l], s[F_BY][l], s[F_BZ][l]);
+  r.t = V(s[F_TX][l], s[F_TY][l], s[F_TZ][l]);
+  r.tN = s[F_TN][l];
+  return r;
+}
+
+__device__ __forceinline__ Seg<double> seg_from_global(const double* __restrict__ seg, int64_t ld,
+                                                       int64_t i) {
+  Seg<double> r;
+  r.n1 = V(seg[F_N1X * ld + i], seg[F_N1Y * ld + i], seg[F_N1Z * ld + i]);
+  r.n2 = V(seg[F_N2X * ld + i], seg[F_N2Y * ld + i], seg[F_N2Z * ld + i]);
+  r.b = V(seg[F_BX * ld + i], seg[F_BY * ld + i], seg[F_BZ * ld + i]);
+  r.t = V(seg[F_TX * ld + i], seg[F_TY * ld + i], seg[F_TZ * ld + i]);
+  r.tN = seg[F_TN * ld + i];
+  return r;
+}
+
+// A corrective sub-pair of the near-parallel branch: generic force on the
+// nodes of `other` from the straight piece p -> q carrying Burgers vector bp.
+// In the reference this is a full calcSegSegForce call whose third and fourth
+// outputs are kept (:1285-1317, :1401-1433).  The tilt construction makes the
+// piece sit at the critical angle, sin^2 = 1.01 sqrt(eps), so it always takes
+// the generic branch; the flag reports it if rounding ever says otherwise.
+__device__ __noinline__ void corrective_pair(const SegK<double>& K, v3<double> bp, v3<double> p,
+                                             v3<double> q, const Seg<double>& other,
+                                             v3<double>& Fo1, v3<double>& Fo2,
+                                             unsigned* flags) {
+  Seg<double> sp;
+  sp.n1 = p;
+  sp.n2 = q;
+  sp.b = bp;
+  unit_tangent(p, q, sp.t, sp.tN);
+  double c;
+  double omcSq = one_minus_c2(sp.t, other.t, c);
+  if (!(omcSq > DDD_SQRT_EPS)) atomicOr(flags, ERRBIT_NESTED_PARALLEL);
+  v3<double> d1, d2, F3, F4;
+  pair_generic<double, false, true>(K, sp, other, c, omcSq, d1, d2, F3, F4);
+  Fo1 = Fo1 + F3;
+  Fo2 = Fo2 + F4;
+}
+
+// calcParSegSegForce (:1167-1442) on device.
+__device__ __noinline__ void special_pair(const SegK<double>& K, const Seg<double>& s1in,
+                                          const Seg<double>& s2, v3<double>& F1, v3<double>& F2,
+                                          v3<double>& F3, v3<double>& F4, unsigned* flags) {
+  const double c = dot(s2.t, s1in.t);
+  const bool flip = c < 0.0;
+  Seg<double> s1 = s1in;
+  if (flip) {  // :1197-1202; the flipped tangent is the exact negation
+    s1.n1 = s1in.n2;
+    s1.n2 = s1in.n1;
+    s1.t = -s1in.t;
+    s1.b = -s1in.b;
+  }
+  v3<double> c1a, c1b, c2a, c2b, d1a, d1b, d2a, d2b;
+  bool corr1, corr2;
+  par_side<double>(K, s1.n1, s1.n2, s1.t, s1.tN, s1.b, s2.n1, s2.n2, s2.b, true, F1, F2, c1a,
+                   c1b, c2a, c2b, corr1);
+  par_side<double>(K, s2.n1, s2.n2, s2.t, s2.tN, s2.b, s1.n1, s1.n2, s1.b, false, F3, F4, d1a,
+                   d1b, d2a, d2b, corr2);
+  if (corr1) {
+    corrective_pair(K, s2.b, c1a, c1b, s1, F1, F2, flags);
+    corrective_pair(K, s2.b, c2a, c2b, s1, F1, F2, flags);
+  }
+  if (corr2) {
+    corrective_pair(K, s1.b, d1a, d1b, s2, F3, F4, flags);
+    corrective_pair(K, s1.b, d2a, d2b, s2, F3, F4, flags);
+  }
+  if (flip) {
+    v3<double> tmp = F1;
+    F1 = F2;
+    F2 = tmp;
+  }
+}
+
+__global__ void k_prep_segments(double* __restrict__ seg, int64_t ld, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= ld) return;
+  if (i >= n) {  // padding: a harmless unit segment, never paired
+    for (int f = 0; f < F_COUNT; ++f) seg[f * ld + i] = 0.0;
+    seg[F_N2X * ld + i] = 1.0;
+    seg[F_TX * ld + i] = 1.0;
+    seg[F_TN * ld + i] = 1.0;
+    seg[F_EXT * ld + i] = 1.0;
+    return;
+  }
+  v3<double> n1 = V(seg[F_N1X * ld + i], seg[F_N1Y * ld + i], seg[F_N1Z * ld + i]);
+  v3<double> n2 = V(seg[F_N2X * ld + i], seg[F_N2Y * ld + i], seg[F_N2Z * ld + i]);
+  v3<double> t;
+  double tN;
+  unit_tangent(n1, n2, t, tN);
+  seg[F_TX * ld + i] = t.x;
+  seg[F_TY * ld + i] = t.y;
+  seg[F_TZ * ld + i] = t.z;
+  seg[F_TN * ld + i] = tN;
+}
+
+constexpr int kQueue = 2 * kTile;
+
+__global__ __launch_bounds__(kTile) void k_segseg_tiles(const double* __restrict__ seg, int64_t ld,
+                                                        int n, int64_t tile0, SegK<double> K,
+                                                        double* __restrict__ rowPart,
+                                                        double* __restrict__ colPart,
+                                                        unsigned* __restrict__ flags) {
+  const int64_t t = tile0 + blockIdx.x;
+  // Tile t of the upper triangle, enumerated column-block major:
+  // t = J (J + 1) / 2 + I with 0 <= I <= J.
+  int J = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((int64_t)J * (J + 1) / 2 > t) --J;
+  while ((int64_t)(J + 1) * (J + 2) / 2 <= t) ++J;
+  const int I = (int)(t - (int64_t)J * (J + 1) / 2);
+  const int lane = threadIdx.x;
+
+  __shared__ double si[F_COUNT][kTile];
+  __shared__ double sj[F_COUNT][kTile];
+  __shared__ double cacc[6][kTile];
+  __shared__ double sres[12][kTile];
+  __shared__ unsigned short queue[kQueue];
+
+  const int64_t i = (int64_t)I * kTile + lane;
+  const int64_t jl = (int64_t)J * kTile + lane;
+#pragma unroll
+  for (int f = 0; f < F_COUNT; ++f) {
+    si[f][lane] = seg[f * ld + i];
+    sj[f][lane] = seg[f * ld + jl];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) cacc[c][lane] = 0.0;
+  __syncthreads();
+
+  const Seg<double> A = seg_from_lds(si, lane);
+  const bool rowLive = (i < n) && si[F_EXT][lane] == 0.0;
+  const bool diag = (I == J);
+  v3<double> a1 = V(0.0, 0.0, 0.0), a2 = V(0.0, 0.0, 0.0);
+  int nq = 0;  // wave-uniform queue length
+  const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+  auto flush = [&](int cnt) {
+    if (lane < cnt) {
+      const unsigned e = queue[lane];
+      const int rl = e & 63, sl = (rl + (e >> 6)) & 63;
+      v3<double> F1, F2, F3, F4;
+      special_pair(K, seg_from_lds(si, rl), seg_from_lds(sj, sl), F1, F2, F3, F4, flags);
+      sres[0][lane] = F1.x; sres[1][lane] = F1.y; sres[2][lane] = F1.z;
+      sres[3][lane] = F2.x; sres[4][lane] = F2.y; sres[5][lane] = F2.z;
+      sres[6][lane] = F3.x; sres[7][lane] = F3.y; sres[8][lane] = F3.z;
+      sres[9][lane] = F4.x; sres[10][lane] = F4.y; sres[11][lane] = F4.z;
+    }
+    __syncthreads();
+    for (int e = 0; e < cnt; ++e) {  // queue order: deterministic
+      const unsigned q = queue[e];
+      const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
+      if (rl == lane) {
+        a1 = a1 + V(sres[0][e], sres[1][e], sres[2][e]);
+        a2 = a2 + V(sres[3][e], sres[4][e], sres[5][e]);
+      }
+      if (sl == lane) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) cacc[c][lane] += sres[6 + c][e];
+      }
+    }
+    __syncthreads();
+  };
+
+  for (int k = 0; k < kTile; ++k) {
+    const int s = (lane + k) & 63;
+    const int64_t j = (int64_t)J * kTile + s;
+    const bool live = rowLive && (j < n) && sj[F_EXT][s] == 0.0 && (!diag || s > lane);
+    const Seg<double> B = seg_from_lds(sj, s);
+    double c;
+    const double omcSq = one_minus_c2(A.t, B.t, c);
+    const bool special = live && !(omcSq > DDD_SQRT_EPS);
+    const uint64_t m = __ballot(special);
+    if (m) {
+      if (special) queue[nq + __popcll(m & ltMask)] = (unsigned short)((k << 6) | lane);
+      nq += __popcll(m);
+    }
+    if (live && !special) {
+      v3<double> F1, F2, F3, F4;
+      pair_generic<double, true, true>(K, A, B, c, omcSq, F1, F2, F3, F4);
+      a1 = a1 + F1;
+      a2 = a2 + F2;
+      cacc[0][s] += F3.x; cacc[1][s] += F3.y; cacc[2][s] += F3.z;
+      cacc[3][s] += F4.x; cacc[4][s] += F4.y; cacc[5][s] += F4.z;
+    }
+    __syncthreads();
+    if (nq >= kTile) {
+      flush(kTile);
+      const int rest = nq - kTile;
+      unsigned short keep = (lane < rest) ? queue[kTile + lane] : 0;
+      __syncthreads();
+      if (lane < rest) queue[lane] = keep;
+      __syncthreads();
+      nq = rest;
+    }
+  }
+  if (nq > 0) flush(nq);
+
+  double* rp = rowPart + t * 6 * kTile;
+  double* cp = colPart + t * 6 * kTile;
+  rp[0 * kTile + lane] = a1.x; rp[1 * kTile + lane] = a1.y; rp[2 * kTile + lane] = a1.z;
+  rp[3 * kTile + lane] = a2.x; rp[4 * kTile + lane] = a2.y; rp[5 * kTile + lane] = a2.z;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) cp[c * kTile + lane] = cacc[c][lane];
+}
+
+// Sum of a segment's partials: row role in tiles (B, J), J >= B, then column
+// role in tiles (I, B), I <= B.
+__global__ void k_segseg_reduce(const double* __restrict__ rowPart,
+                                const double* __restrict__ colPart, int n, int nb,
+                                double* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int B = s / kTile, l = s % kTile;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int J = B; J < nb; ++J) {
+    const double* p = rowPart + (tiles_for(J) + B) * 6 * kTile + l;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[c] += p[c * kTile];
+  }
+  const int64_t t0 = tiles_for(B);
+  for (int I = 0; I <= B; ++I) {
+    const double* p = colPart + (t0 + I) * 6 * kTile + l;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[c] += p[c * kTile];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) out[6 * (int64_t)s + c] += acc[c];
+}
+
+// Rows-only forces (calcSegSegForceIdx!, :831-877): row r = rows[w] against
+// every other live segment, with the row segment as segment 1; one
+// wavefront per row, reduced across lanes in a fixed butterfly order.
+__global__ __launch_bounds__(kTile) void k_segseg_rows(const double* __restrict__ seg, int64_t ld,
+                                                       int n, const int64_t* __restrict__ rows,
+                                                       SegK<double> K, double* __restrict__ out,
+                                                       unsigned* __restrict__ flags) {
+  const int w = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t r = rows[w];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  if (r >= 0 && r < n && seg[F_EXT * ld + r] == 0.0) {
+    const Seg<double> A = seg_from_global(seg, ld, r);
+    for (int64_t j = lane; j < n; j += kTile) {
+      if (j == r || seg[F_EXT * ld + j] != 0.0) continue;
+      const Seg<double> B = seg_from_global(seg, ld, j);
+      double c;
+      const double omcSq = one_minus_c2(A.t, B.t, c);
+      v3<double> F1, F2, F3, F4;
+      if (omcSq > DDD_SQRT_EPS) {
+        pair_generic<double, true, false>(K, A, B, c, omcSq, F1, F2, F3, F4);
+      } else {
+        special_pair(K, A, B, F1, F2, F3, F4, flags);
+      }
+      acc[0] += F1.x; acc[1] += F1.y; acc[2] += F1.z;
+      acc[3] += F2.x; acc[4] += F2.y; acc[5] += F2.z;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    double v = acc[c];
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kTile);
+    acc[c] = v;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) out[6 * (int64_t)w + c] += acc[c];
+  }
+}
+
+}  // namespace
+
+void launch_prep_segments(double* seg, int64_t ld, int64_t n, hipStream_t st) {
+  const int bs = 256;
+  hipLaunchKernelGGL(k_prep_segments, dim3((unsigned)((ld + bs - 1) / bs)), dim3(bs), 0, st, seg,
+                     ld, n);
+}
+
+void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
+                         double* rowPart, double* colPart, unsigned* flags, hipStream_t st) {
+  const int64_t nb = (n + kTile - 1) / kTile;
+  const int64_t T = tiles_for(nb);
+  const int64_t maxGrid = 1ll << 30;
+  for (int64_t t0 = 0; t0 < T; t0 += maxGrid) {
+    const int64_t cnt = (T - t0 < maxGrid) ? (T - t0) : maxGrid;
+    hipLaunchKernelGGL(k_segseg_tiles, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, t0, K,
+                       rowPart, colPart, flags);
+  }
+}
+
+void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, double* out,
+                          hipStream_t st) {
+  const int nb = (n + kTile - 1) / kTile;
+  const int bs = 256;
+  hipLaunchKernelGGL(k_segseg_reduce, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, st,
+                     rowPart, colPart, n, nb, out);
+}
+
+void launch_segseg_rows(const double* seg, int64_t ld, int n, const int64_t* rows, int nRows,
+                        const SegK<double>& K, double* out, unsigned* flags, hipStream_t st) {
+  if (nRows <= 0) return;
+  hipLaunchKernelGGL(k_segseg_rows, dim3((unsigned)nRows), dim3(kTile), 0, st, seg, ld, n, rows, K,
+                     out, flags);
+}
+
+}  // namespace ddd

@@ -1,0 +1,182 @@
+"""ctypes binding of libdddhip.so (include/ddd_hip.h).
+
+This is what a Julia ``ccall`` shim would bind; the structs mirror the C
+declarations field for field.  There is no fallback: if the library or a
+gfx950 device is missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parents[1]
+LIB_PATH = PKG / "libdddhip.so"
+
+DDD_OK, DDD_ERR_ARG, DDD_ERR_OOM, DDD_ERR_HIP, DDD_ERR_NO_DEVICE, DDD_ERR_NUMERIC = (
+    0, -1, -2, -3, -4, -5)
+DDD_ACCUMULATE, DDD_OUT_BY_POSITION, DDD_PK_HALF_TO_NODES = 1, 2, 4
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int64)
+up = C.POINTER(C.c_uint8)
+i64 = C.c_int64
+
+
+class ddd_consts(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "mu4pi", "mu8pi", "mu4pinu", "nuMu4pinu", "omNuInv", "nuOmNuInv", "coreRad",
+        "coreRadSq", "coreEnergy", "sigmaPN")]
+
+
+class ddd_network(C.Structure):
+    _fields_ = [("numNode", i64), ("numSeg", i64), ("segIdx", ip), ("segIdxLd", i64),
+                ("coord", dp), ("coordCols", i64), ("bVec", dp), ("bVecCols", i64),
+                ("extSeg", up), ("links", ip), ("linksCols", i64), ("connectivity", ip),
+                ("connLd", i64), ("connCols", i64)]
+
+
+class ddd_mesh(C.Structure):
+    _fields_ = [("C", dp), ("coord", dp), ("numNode", i64), ("connectivity", ip),
+                ("numElem", i64), ("mx", i64), ("my", i64), ("mz", i64), ("w", C.c_double),
+                ("h", C.c_double), ("d", C.c_double), ("uHat", dp)]
+
+
+class DDDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise DDDError(DDD_ERR_NO_DEVICE,
+                               f"{LIB_PATH} is not built (run __graft_entry__.build())")
+            L = C.CDLL(str(LIB_PATH))
+            P = C.c_void_p
+            sigs = {
+                "ddd_version": (C.c_int, []),
+                "ddd_ctx_create": (C.c_int, [C.POINTER(P), C.c_int]),
+                "ddd_ctx_destroy": (C.c_int, [P]),
+                "ddd_ctx_set_stream": (C.c_int, [P, P]),
+                "ddd_ctx_stream": (P, [P]),
+                "ddd_last_error": (C.c_char_p, [P]),
+                "ddd_segseg_force": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_network),
+                                               ip, i64, dp, i64, C.c_int]),
+                "ddd_self_force": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_network),
+                                             ip, i64, dp, i64, C.c_int]),
+                "ddd_pk_force": (C.c_int, [P, C.POINTER(ddd_network), C.POINTER(ddd_mesh), ip,
+                                           i64, dp, i64, C.c_int]),
+                "ddd_seg_force": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_network),
+                                            C.POINTER(ddd_mesh), ip, i64, dp, i64, C.c_int]),
+                "ddd_sigma_tilde": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_network),
+                                              ip, i64, dp, i64, dp, C.c_int]),
+                "ddd_node_force": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_network),
+                                             dp, i64, dp]),
+                "ddd_net_upload": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_network),
+                                             C.POINTER(ddd_mesh)]),
+                "ddd_net_seg_force": (C.c_int, [P, C.c_int]),
+                "ddd_net_stage": (C.c_int, [P, C.c_int, C.c_int]),
+                "ddd_net_download": (C.c_int, [P, dp, i64]),
+                "ddd_net_segforce_device": (P, [P]),
+            }
+            for name, (res, args) in sigs.items():
+                f = getattr(L, name)
+                f.restype = res
+                f.argtypes = args
+            _lib = L
+    return _lib
+
+
+class Context:
+    """One device context (ddd_ctx_create / ddd_ctx_destroy)."""
+
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        rc = lib().ddd_ctx_create(C.byref(self.h), int(device))
+        if rc != DDD_OK:
+            raise DDDError(rc, f"ddd_ctx_create(device={device}) failed: no usable gfx950 device")
+
+    def check(self, rc):
+        if rc != DDD_OK:
+            raise DDDError(rc, lib().ddd_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            lib().ddd_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default = {}
+
+
+def default_context(device: int = 0) -> Context:
+    with _lock:
+        ctx = _default.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        with _lock:
+            _default[device] = ctx
+    return ctx
+
+
+def ptr(a, kind):
+    if a is None:
+        return None
+    return a.ctypes.data_as({"d": dp, "i": ip, "u": up}[kind])
+
+
+class NetView:
+    """Holds Fortran-ordered views of a DislocationNetwork and the ddd_network
+    struct that points into them (kept alive together)."""
+
+    def __init__(self, net):
+        self.segIdx = np.asfortranarray(net.segIdx, dtype=np.int64)
+        self.coord = np.asfortranarray(net.coord, dtype=np.float64)
+        self.bVec = np.asfortranarray(net.bVec, dtype=np.float64)
+        self.ext = np.ascontiguousarray(net.extSeg, dtype=np.uint8)
+        self.links = np.asfortranarray(net.links, dtype=np.int64)
+        self.conn = np.asfortranarray(net.connectivity, dtype=np.int64)
+        self.s = ddd_network(
+            numNode=int(net.numNode[0]), numSeg=int(net.numSeg[0]), segIdx=ptr(self.segIdx, "i"),
+            segIdxLd=self.segIdx.shape[0], coord=ptr(self.coord, "d"),
+            coordCols=self.coord.shape[1], bVec=ptr(self.bVec, "d"), bVecCols=self.bVec.shape[1],
+            extSeg=ptr(self.ext, "u"), links=ptr(self.links, "i"), linksCols=self.links.shape[1],
+            connectivity=ptr(self.conn, "i"), connLd=self.conn.shape[0],
+            connCols=self.conn.shape[1])
+
+
+class MeshView:
+    def __init__(self, mesh, fd):
+        self.C = np.asfortranarray(mesh.C, dtype=np.float64)
+        self.coord = np.asfortranarray(mesh.coord, dtype=np.float64)
+        self.conn = np.asfortranarray(mesh.connectivity, dtype=np.int64)
+        self.uHat = np.ascontiguousarray(fd.uHat, dtype=np.float64)
+        self.s = ddd_mesh(C=ptr(self.C, "d"), coord=ptr(self.coord, "d"),
+                          numNode=self.coord.shape[1], connectivity=ptr(self.conn, "i"),
+                          numElem=self.conn.shape[1], mx=int(mesh.mx), my=int(mesh.my),
+                          mz=int(mesh.mz), w=float(mesh.w), h=float(mesh.h), d=float(mesh.d),
+                          uHat=ptr(self.uHat, "d"))
+
+
+def consts(dlnParams, matParams) -> ddd_consts:
+    return ddd_consts(mu4pi=matParams.μ4π, mu8pi=matParams.μ8π, mu4pinu=matParams.μ4πν,
+                      nuMu4pinu=matParams.νμ4πν, omNuInv=matParams.omνInv,
+                      nuOmNuInv=matParams.νomνInv, coreRad=dlnParams.coreRad,
+                      coreRadSq=dlnParams.coreRadSq, coreEnergy=dlnParams.coreEnergy,
+                      sigmaPN=matParams.σPN)

@@ -1,0 +1,213 @@
+"""The reference's force-path API, routed to the HIP library.
+
+Names, argument order, output shapes and in-place semantics follow
+src/Processing/DislocationForces.jl, src/Processing/DislocationFEM.jl and the
+force half of src/Processing/DislocationMobility.jl.  Julia's ``f!`` is
+spelled ``f_`` here.  ``idx`` is None, an int, a range or a sequence of
+1-based segment indices, as in the reference.
+
+Every function calls libdddhip.so; there is no CPU path behind them.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _idx(idx):
+    if idx is None:
+        return None
+    if isinstance(idx, (int, np.integer)):
+        return np.array([int(idx)], dtype=np.int64)
+    return np.ascontiguousarray(np.asarray(list(idx) if isinstance(idx, range) else idx,
+                                           dtype=np.int64).ravel())
+
+
+def _ctx(ctx):
+    return ctx if ctx is not None else L.default_context()
+
+
+def _call(ctx, fn, *args):
+    ctx.check(getattr(L.lib(), fn)(ctx.h, *args))
+
+
+def _nidx(ix):
+    return (L.ptr(ix, "i"), len(ix)) if ix is not None else (None, 0)
+
+
+def calcSegSegForce(dlnParams, matParams, network, idx=None, ctx=None):
+    """calcSegSegForce (DislocationForces.jl:470-548): 3 x 2 x numSeg, or
+    3 x 2 x length(idx) with forces on the idx rows only."""
+    ctx = _ctx(ctx)
+    nv = L.NetView(network)
+    k = L.consts(dlnParams, matParams)
+    ix = _idx(idx)
+    m = int(network.numSeg[0]) if ix is None else len(ix)
+    out = np.zeros((3, 2, m), order="F")
+    _call(ctx, "ddd_segseg_force", k, nv.s, *_nidx(ix), L.ptr(out, "d"), m,
+          L.DDD_OUT_BY_POSITION)
+    return out
+
+
+def calcSegSegForce_(dlnParams, matParams, network, idx=None, ctx=None):
+    """calcSegSegForce! (DislocationForces.jl:563-669): adds into
+    network.segForce."""
+    ctx = _ctx(ctx)
+    nv = L.NetView(network)
+    k = L.consts(dlnParams, matParams)
+    ix = _idx(idx)
+    sf = network.segForce
+    assert sf.flags.f_contiguous and sf.dtype == np.float64
+    _call(ctx, "ddd_segseg_force", k, nv.s, *_nidx(ix), L.ptr(sf, "d"), sf.shape[2],
+          L.DDD_ACCUMULATE)
+
+
+def calcSelfForce(dlnParams, matParams, network, idx=None, ctx=None):
+    """calcSelfForce (DislocationForces.jl:320-352): (node1, node2), each
+    3 x numSeg (or 3 x length(idx))."""
+    ctx = _ctx(ctx)
+    nv = L.NetView(network)
+    k = L.consts(dlnParams, matParams)
+    ix = _idx(idx)
+    m = int(network.numSeg[0]) if ix is None else len(ix)
+    out = np.zeros((3, 2, m), order="F")
+    _call(ctx, "ddd_self_force", k, nv.s, *_nidx(ix), L.ptr(out, "d"), m, L.DDD_OUT_BY_POSITION)
+    return out[:, 0, :].copy(order="F"), out[:, 1, :].copy(order="F")
+
+
+def calcSelfForce_(dlnParams, matParams, network, idx=None, ctx=None):
+    """calcSelfForce! (DislocationForces.jl:354-388)."""
+    ctx = _ctx(ctx)
+    nv = L.NetView(network)
+    k = L.consts(dlnParams, matParams)
+    ix = _idx(idx)
+    sf = network.segForce
+    _call(ctx, "ddd_self_force", k, nv.s, *_nidx(ix), L.ptr(sf, "d"), sf.shape[2],
+          L.DDD_ACCUMULATE)
+
+
+def calcPKForce(mesh, forceDisplacement, network, idx=None, ctx=None):
+    """calcPKForce (DislocationForces.jl:216-243): 3 x numSeg (or length(idx))."""
+    ctx = _ctx(ctx)
+    nv = L.NetView(network)
+    mv = L.MeshView(mesh, forceDisplacement)
+    ix = _idx(idx)
+    m = int(network.numSeg[0]) if ix is None else len(ix)
+    out = np.zeros((3, m), order="F")
+    _call(ctx, "ddd_pk_force", nv.s, mv.s, *_nidx(ix), L.ptr(out, "d"), m, L.DDD_OUT_BY_POSITION)
+    return out
+
+
+def calcPKForce_(mesh, forceDisplacement, network, idx=None, ctx=None):
+    """calcPKForce! (DislocationForces.jl:244-273): pk/2 added to both nodes."""
+    ctx = _ctx(ctx)
+    nv = L.NetView(network)
+    mv = L.MeshView(mesh, forceDisplacement)
+    ix = _idx(idx)
+    sf = network.segForce
+    _call(ctx, "ddd_pk_force", nv.s, mv.s, *_nidx(ix), L.ptr(sf, "d"), sf.shape[2],
+          L.DDD_ACCUMULATE | L.DDD_PK_HALF_TO_NODES)
+
+
+def calcSegForce(dlnParams, matParams, mesh, forceDisplacement, network, idx=None, ctx=None):
+    """calcSegForce (DislocationForces.jl:23-46): seg-seg + self + PK/2."""
+    ctx = _ctx(ctx)
+    nv = L.NetView(network)
+    mv = L.MeshView(mesh, forceDisplacement) if mesh is not None else None
+    k = L.consts(dlnParams, matParams)
+    ix = _idx(idx)
+    m = int(network.numSeg[0]) if ix is None else len(ix)
+    out = np.zeros((3, 2, m), order="F")
+    _call(ctx, "ddd_seg_force", k, nv.s, mv.s if mv else None, *_nidx(ix), L.ptr(out, "d"), m,
+          L.DDD_OUT_BY_POSITION)
+    return out
+
+
+def calcSegForce_(dlnParams, matParams, mesh, forceDisplacement, network, idx=None, ctx=None):
+    """calcSegForce! (DislocationForces.jl:48-71): zero segForce over the
+    range, then add PK/2, self and seg-seg."""
+    ctx = _ctx(ctx)
+    nv = L.NetView(network)
+    mv = L.MeshView(mesh, forceDisplacement) if mesh is not None else None
+    k = L.consts(dlnParams, matParams)
+    ix = _idx(idx)
+    sf = network.segForce
+    if ix is None:
+        sf[:, :, :int(network.numSeg[0])] = 0
+    else:
+        sf[:, :, ix - 1] = 0
+    _call(ctx, "ddd_seg_force", k, nv.s, mv.s if mv else None, *_nidx(ix), L.ptr(sf, "d"),
+          sf.shape[2], L.DDD_ACCUMULATE)
+
+
+def _points(x0):
+    x = np.asarray(x0, dtype=np.float64)
+    if x.ndim == 1:
+        x = x.reshape(3, 1)
+    return np.asfortranarray(x.reshape(3, -1, order="F"))
+
+
+def calc_σTilde(x0, dlnParams, matParams, network, idx=None, ctx=None):
+    """calc_σTilde (DislocationFEM.jl:24-41): 6 x P stress (xx, yy, zz, xy,
+    xz, yz), or a 6-vector for a single point."""
+    x = _points(x0)
+    σ = np.zeros((6, x.shape[1]), order="F")
+    calc_σTilde_(σ, x, dlnParams, matParams, network, idx, ctx=ctx)
+    return σ[:, 0].copy() if x.shape[1] == 1 else σ
+
+
+def calc_σTilde_(σ, x0, dlnParams, matParams, network, idx=None, ctx=None):
+    """calc_σTilde! (DislocationFEM.jl:67-84): overwrites σ (6 x P)."""
+    ctx = _ctx(ctx)
+    x = _points(x0)
+    assert σ.shape[0] == 6 and σ.size == 6 * x.shape[1], "calc_σTilde!: σ must be 6 x P"
+    assert σ.dtype == np.float64 and (σ.flags.f_contiguous or σ.ndim == 1)
+    nv = L.NetView(network)
+    k = L.consts(dlnParams, matParams)
+    ix = _idx(idx)
+    _call(ctx, "ddd_sigma_tilde", k, nv.s, *_nidx(ix), L.ptr(x, "d"), x.shape[1],
+          L.ptr(σ, "d"), 0)
+    return σ
+
+
+def calcNodeForce_(dlnParams, matParams, network, ctx=None):
+    """The node-force gather of dlnMobility! (DislocationMobility.jl:220-255,
+    336): network.nodeForce[:, 1:numNode] from network.segForce."""
+    ctx = _ctx(ctx)
+    nv = L.NetView(network)
+    k = L.consts(dlnParams, matParams)
+    sf = np.asfortranarray(network.segForce)
+    n = int(network.numNode[0])
+    nf = np.zeros((3, n), order="F")
+    _call(ctx, "ddd_node_force", k, nv.s, L.ptr(sf, "d"), sf.shape[2], L.ptr(nf, "d"))
+    network.nodeForce[:, :n] = nf
+    return nf
+
+
+class ResidentNetwork:
+    """Device-resident evaluation (ddd_net_*): upload once, evaluate many
+    times.  Used by time-stepping callers and the benchmark."""
+
+    def __init__(self, dlnParams, matParams, network, mesh=None, forceDisplacement=None,
+                 ctx=None):
+        self.ctx = _ctx(ctx)
+        self.nv = L.NetView(network)
+        self.mv = L.MeshView(mesh, forceDisplacement) if mesh is not None else None
+        self.k = L.consts(dlnParams, matParams)
+        self.numSeg = int(network.numSeg[0])
+        _call(self.ctx, "ddd_net_upload", self.k, self.nv.s, self.mv.s if self.mv else None)
+
+    def seg_force(self, with_pk=False):
+        _call(self.ctx, "ddd_net_seg_force", 1 if with_pk else 0)
+
+    def stage(self, stage, with_pk=False):
+        _call(self.ctx, "ddd_net_stage", int(stage), 1 if with_pk else 0)
+
+    def download(self):
+        out = np.zeros((3, 2, self.numSeg), order="F")
+        _call(self.ctx, "ddd_net_download", L.ptr(out, "d"), self.numSeg)
+        return out
+
+    def stream(self):
+        return L.lib().ddd_ctx_stream(self.ctx.h)

@@ -1,0 +1,177 @@
+/*
+ * ddd_hip.h -- C ABI of the MI355X (gfx950) force path for DDD.jl networks.
+ *
+ * The entry points replace, one for one, the force-path functions of DDD.jl
+ * (src/Processing/DislocationForces.jl, src/Processing/DislocationFEM.jl,
+ * src/Processing/DislocationMobility.jl).  They take the reference's Julia
+ * arrays as they lie in memory -- column-major, Int64 indices 1-based -- so a
+ * Julia ccall passes its arrays straight through.
+ *
+ * Conventions
+ *   - All pointers are host pointers unless a name says otherwise.  Every
+ *     call is synchronous: results are in the caller's arrays on return and
+ *     no host pointer is kept.
+ *   - Return value: DDD_OK (0) or a negative DDD_ERR_* code; the message is
+ *     in ddd_last_error(ctx).  Nothing aborts or exits.
+ *   - `idx`/`nIdx`: NULL/0 means "all numSeg segments"; otherwise nIdx
+ *     1-based segment indices (the reference's `idx` argument as an Int64
+ *     vector; the Julia shim turns an Int or a range into one).
+ *   - Output columns: without DDD_OUT_BY_POSITION, results for segment i go
+ *     to column i (the in-place `!` forms); with it, results for idx[k] go to
+ *     column k (the out-of-place forms, which return length(idx) columns).
+ *   - DDD_ACCUMULATE adds into the output (the reference's in-place forms
+ *     accumulate); without it the addressed output columns are overwritten.
+ *   - A context is used by one caller thread at a time (calls on one context
+ *     are serialised by an internal lock); contexts are independent.
+ */
+#ifndef DDD_HIP_H
+#define DDD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DDD_ABI_VERSION 1
+
+enum {
+  DDD_OK = 0,
+  DDD_ERR_ARG = -1,       /* bad argument, shape or index            */
+  DDD_ERR_OOM = -2,       /* device allocation failed                */
+  DDD_ERR_HIP = -3,       /* HIP runtime error                       */
+  DDD_ERR_NO_DEVICE = -4, /* no usable gfx950 device                 */
+  DDD_ERR_NUMERIC = -5    /* a numerical guard fired (see message)   */
+};
+
+enum {
+  DDD_ACCUMULATE = 1,         /* add into the output instead of overwriting */
+  DDD_OUT_BY_POSITION = 2,    /* output column k for idx[k]                 */
+  DDD_PK_HALF_TO_NODES = 4    /* ddd_pk_force: 3x2xn, pk/2 to each node     */
+};
+
+typedef struct ddd_ctx ddd_ctx;
+
+/* MaterialParameters (src/Type/MaterialType.jl:37-53, derived constants
+ * MaterialConstructor.jl:20-29) and DislocationParameters
+ * (src/Type/DislocationType.jl:205-255) fields the force path reads. */
+typedef struct ddd_consts {
+  double mu4pi;     /* mu / (4 pi)                 */
+  double mu8pi;     /* mu / (8 pi)                 */
+  double mu4pinu;   /* mu / (4 pi (1 - nu))        */
+  double nuMu4pinu; /* nu mu / (4 pi (1 - nu))     */
+  double omNuInv;   /* 1 / (1 - nu)                */
+  double nuOmNuInv; /* nu / (1 - nu)               */
+  double coreRad;   /* a                           */
+  double coreRadSq; /* a^2                         */
+  double coreEnergy;/* Ec                          */
+  double sigmaPN;   /* Peierls-Nabarro threshold   */
+} ddd_consts;
+
+/* DislocationNetwork (src/Type/DislocationType.jl:324-340) as Julia lays it
+ * out.  Sizes are the allocated column counts, used to bounds-check the
+ * 1-based indices in segIdx, links and connectivity. */
+typedef struct ddd_network {
+  int64_t numNode;              /* network.numNode[1]                       */
+  int64_t numSeg;               /* network.numSeg[1]                        */
+  const int64_t* segIdx;        /* segIdxLd x 3: (link, node1, node2)       */
+  int64_t segIdxLd;
+  const double* coord;          /* 3 x coordCols                            */
+  int64_t coordCols;
+  const double* bVec;           /* 3 x bVecCols                             */
+  int64_t bVecCols;
+  const uint8_t* extSeg;        /* numSeg bytes (Vector{Bool}), may be NULL */
+  const int64_t* links;         /* 2 x linksCols (gather only)              */
+  int64_t linksCols;
+  const int64_t* connectivity;  /* connLd x connCols (gather only)          */
+  int64_t connLd;
+  int64_t connCols;
+} ddd_network;
+
+/* RegularCuboidMesh with linear elements (src/Type/FEMType.jl:212-272) as
+ * calc_sigmaHat reads it, plus the dense uHat of ForceDisplacement. */
+typedef struct ddd_mesh {
+  const double* C;             /* 6 x 6 stiffness, column-major            */
+  const double* coord;         /* 3 x numNode                              */
+  int64_t numNode;
+  const int64_t* connectivity; /* 8 x numElem, 1-based                     */
+  int64_t numElem;
+  int64_t mx, my, mz;
+  double w, h, d;              /* element width, height, depth             */
+  const double* uHat;          /* 3 numNode, dense (the shim densifies)    */
+} ddd_mesh;
+
+int ddd_version(void);
+
+/* Create a context on HIP device `device`; DDD_ERR_NO_DEVICE when none. */
+int ddd_ctx_create(ddd_ctx** ctx, int device);
+int ddd_ctx_destroy(ddd_ctx* ctx);
+/* Run the context's work on an existing HIP stream (a hipStream_t);
+ * NULL returns to the context's own stream. */
+int ddd_ctx_set_stream(ddd_ctx* ctx, void* stream);
+void* ddd_ctx_stream(ddd_ctx* ctx);
+const char* ddd_last_error(ddd_ctx* ctx);
+
+/* calcSegSegForce / calcSegSegForce! (DislocationForces.jl:470-669; pair
+ * math :879-1561).  out: 3 x 2 x outCols.  Without idx: all unordered pairs
+ * i < j of non-external segments (calcSegSegForceSer!, :778-829); with idx:
+ * full rows, forces on the row segment only (calcSegSegForceIdx!,
+ * :831-877). */
+int ddd_segseg_force(ddd_ctx* ctx, const ddd_consts* k, const ddd_network* net,
+                     const int64_t* idx, int64_t nIdx, double* out, int64_t outCols, int flags);
+
+/* calcSelfForce / calcSelfForce! (DislocationForces.jl:320-455).  out:
+ * 3 x 2 x outCols; node 1 gets -f, node 2 +f. */
+int ddd_self_force(ddd_ctx* ctx, const ddd_consts* k, const ddd_network* net,
+                   const int64_t* idx, int64_t nIdx, double* out, int64_t outCols, int flags);
+
+/* calcPKForce / calcPKForce! (DislocationForces.jl:216-300, calc_sigmaHat
+ * :92-192).  Without DDD_PK_HALF_TO_NODES out is 3 x outCols (the
+ * out-of-place form); with it, 3 x 2 x outCols with pk/2 to each node. */
+int ddd_pk_force(ddd_ctx* ctx, const ddd_network* net, const ddd_mesh* mesh,
+                 const int64_t* idx, int64_t nIdx, double* out, int64_t outCols, int flags);
+
+/* calcSegForce / calcSegForce! (DislocationForces.jl:23-71): PK/2 + self +
+ * seg-seg.  mesh may be NULL (no FE correction: PK = 0).  out: 3 x 2 x
+ * outCols.  The in-place form zeroes the addressed columns first: call it
+ * without DDD_ACCUMULATE. */
+int ddd_seg_force(ddd_ctx* ctx, const ddd_consts* k, const ddd_network* net,
+                  const ddd_mesh* mesh, const int64_t* idx, int64_t nIdx, double* out,
+                  int64_t outCols, int flags);
+
+/* calc_sigmaTilde / calc_sigmaTilde! (DislocationFEM.jl:24-303).  x0: 3 x P
+ * points; sigma: 6 x P in the order (xx, yy, zz, xy, xz, yz).  External
+ * segments are included, as in the reference. */
+int ddd_sigma_tilde(ddd_ctx* ctx, const ddd_consts* k, const ddd_network* net,
+                    const int64_t* idx, int64_t nIdx, const double* x0, int64_t P,
+                    double* sigma, int flags);
+
+/* Node-force gather of dlnMobility! (DislocationMobility.jl:220-255):
+ * nodeForce[:, n] = sum over n's connections of segForce[:, col, link],
+ * skipping zero-length links and forces below sigmaPN.  segForce: 3 x 2 x
+ * segForceCols; nodeForce: 3 x numNode (overwritten). */
+int ddd_node_force(ddd_ctx* ctx, const ddd_consts* k, const ddd_network* net,
+                   const double* segForce, int64_t segForceCols, double* nodeForce);
+
+/* ---- Device-resident path (no counterpart in the reference) ----------
+ * For callers that evaluate forces many times on one network (time steps,
+ * benchmarks): upload once, compute on the device, fetch when needed.
+ * ddd_net_upload packs the network into the context's device buffers;
+ * ddd_net_seg_force computes the calcSegForce! total (mesh may be NULL)
+ * into the context's device segForce; ddd_net_download copies it into a
+ * 3 x 2 x outCols host array (columns 1..numSeg); ddd_net_segforce_device
+ * returns the device address of that buffer (6 doubles per segment). */
+int ddd_net_upload(ddd_ctx* ctx, const ddd_consts* k, const ddd_network* net,
+                   const ddd_mesh* mesh);
+int ddd_net_seg_force(ddd_ctx* ctx, int withPK);
+/* One stage of ddd_net_seg_force, for per-kernel timing on the context's
+ * stream: 0 = zero segForce + PK/2 + self force, 1 = seg-seg pair tiles,
+ * 2 = reduction of the tile partials into segForce. */
+int ddd_net_stage(ddd_ctx* ctx, int stage, int withPK);
+int ddd_net_download(ddd_ctx* ctx, double* out, int64_t outCols);
+const double* ddd_net_segforce_device(ddd_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DDD_HIP_H */

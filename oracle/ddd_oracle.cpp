@@ -94,9 +94,18 @@ void segseg_ser(const double* K, const SegView& s, double* segForce) {
 
 }  // namespace
 
+namespace ddd_oracle {
+int g_glibc_libm = 0;
+}
+
 extern "C" {
 
 int orc_version() { return 1; }
+
+// 0: fdlibm-method log/atan (default, shared with the kernels); 1: glibc.
+void orc_set_libm(int glibc) { g_glibc_libm = glibc; }
+double orc_libm_log(double x) { return ddd_libm::log(x); }
+double orc_libm_atan(double x) { return ddd_libm::atan(x); }
 
 // One pair, F1..F4 in out[12].  prec: 0 double, 1 long double, 2 float.
 // Returns 1 when the near-parallel branch was taken.
@@ -163,6 +172,43 @@ void orc_segseg_par_faithful(const double* K, int64_t numSeg, const int64_t* seg
     }
   }
   for (int64_t i = 0; i < 6 * numSeg; ++i) segForce[i] += P[i];
+}
+
+// calcSegSegForcePar's design restricted to a sample of outer rows: the rows
+// are split into nthreads static chunks, each row i is paired with every
+// j > i, and the 12 force components of each pair are added atomically.
+// Used to time the reference's threaded path on a bounded share of a large
+// network; returns the number of pairs evaluated.
+int64_t orc_segseg_par_rows(const double* K, int64_t numSeg, const int64_t* segIdx, int64_t ld,
+                            const double* coord, const double* bVec, const uint8_t* extSeg,
+                            const int64_t* rows, int64_t nRows, double* segForce, int nthreads) {
+  SegView s{numSeg, segIdx, ld, coord, bVec, extSeg};
+  SegConsts<double> KK = consts_from<double>(K);
+  int64_t pairs = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : pairs)
+  {
+    int tid = omp_get_thread_num();
+    int nt = omp_get_num_threads();
+    int64_t start = (int64_t(tid) * nRows) / nt;
+    int64_t stop = (int64_t(tid + 1) * nRows) / nt;
+    for (int64_t r = start; r < stop; ++r) {
+      int64_t i = rows[r] - 1;
+      if (s.ext(i)) continue;
+      V3<double> b1 = s.b(i), n11 = s.n1(i), n12 = s.n2(i);
+      for (int64_t j = i + 1; j < numSeg; ++j) {
+        if (s.ext(j)) continue;
+        Forces4<double> f = seg_seg_force(KK, b1, n11, n12, s.b(j), s.n1(j), s.n2(j));
+        double* pi = segForce + 6 * i;
+        double* pj = segForce + 6 * j;
+        atomic_add(pi + 0, f.f1.x); atomic_add(pi + 1, f.f1.y); atomic_add(pi + 2, f.f1.z);
+        atomic_add(pi + 3, f.f2.x); atomic_add(pi + 4, f.f2.y); atomic_add(pi + 5, f.f2.z);
+        atomic_add(pj + 0, f.f3.x); atomic_add(pj + 1, f.f3.y); atomic_add(pj + 2, f.f3.z);
+        atomic_add(pj + 3, f.f4.x); atomic_add(pj + 4, f.f4.y); atomic_add(pj + 5, f.f4.z);
+        ++pairs;
+      }
+    }
+  }
+  return pairs;
 }
 
 // Rows-only seg-seg over a subset (calcSegSegForceIdx!, :831-877).  For each

@@ -16,7 +16,27 @@
 #include <cfloat>
 #include <limits>
 
+// log and atan of the double-precision oracle.  The reference calls Julia's
+// Base.Math log/atan, which no build here can reproduce bit for bit; any
+// correctly-specified <1 ulp implementation stands in for them.  The oracle
+// uses the fdlibm-method functions that the GPU kernels use, so that both
+// sides round identically and agree per pair; those functions are pinned
+// against glibc and mpmath in tests/test_oracle.py.  orc_set_libm(1)
+// switches the oracle to glibc's log/atan to measure how much the choice of
+// implementation moves a result.
+#include "../ddd.jl_amd/csrc/ddd_libm.hpp"
+
 namespace ddd_oracle {
+
+extern int g_glibc_libm;
+inline double ulog(double x) { return g_glibc_libm ? std::log(x) : ddd_libm::log(x); }
+inline double uatan(double x) { return g_glibc_libm ? std::atan(x) : ddd_libm::atan(x); }
+inline long double ulog(long double x) { return std::log(x); }
+inline long double uatan(long double x) { return std::atan(x); }
+inline float ulog(float x) { return std::log(x); }
+inline float uatan(float x) { return std::atan(x); }
+template <class T> inline T ulog(T x) { return log(x); }
+template <class T> inline T uatan(T x) { return atan(x); }
 
 template <class T> struct V3 {
   T x, y, z;
@@ -54,7 +74,7 @@ template <class T> inline T sqrt_eps() {
 // Antiderivative values at one corner (x, y): DislocationForces.jl:1480-1561.
 template <class T>
 inline void seg_seg_integ(T aSq, T d, T c, T cSq, T omcSq, T omcSqI, T x, T y, T* integ) {
-  using std::sqrt; using std::log; using std::atan;
+  using std::sqrt;
   T aSq_dSq = aSq + d * d * omcSq;
   T xSq = x * x;
   T ySq = y * y;
@@ -64,10 +84,10 @@ inline void seg_seg_integ(T aSq, T d, T c, T cSq, T omcSq, T omcSqI, T x, T y, T
   T Ra_Rd_t1 = Ra + y + x * c;
   T Ra_Rd_t2 = Ra + x + y * c;
 
-  T log_Ra_Rd_t1 = log(Ra_Rd_t1);
+  T log_Ra_Rd_t1 = ulog(Ra_Rd_t1);
   T xlog_Ra_Rd_t1 = x * log_Ra_Rd_t1;
 
-  T log_Ra_Rd_t2 = log(Ra_Rd_t2);
+  T log_Ra_Rd_t2 = ulog(Ra_Rd_t2);
   T ylog_Ra_Rd_t2 = y * log_Ra_Rd_t2;
 
   T RaSq_R_t1_I = RaInv / Ra_Rd_t1;
@@ -80,7 +100,7 @@ inline void seg_seg_integ(T aSq, T d, T c, T cSq, T omcSq, T omcSqI, T x, T y, T
 
   T den = T(1) / sqrt(omcSq * aSq_dSq);
 
-  T i1 = T(-2) * den * atan((T(1) + c) * (Ra + x + y) * den);
+  T i1 = T(-2) * den * uatan((T(1) + c) * (Ra + x + y) * den);
 
   T c_1 = aSq_dSq * i1;
   T c_5_6 = (c * Ra - c_1) * omcSqI;
@@ -126,12 +146,12 @@ inline void seg_seg_integ(T aSq, T d, T c, T cSq, T omcSq, T omcSqI, T x, T y, T
 // Near-parallel antiderivatives: DislocationForces.jl:1444-1478.
 template <class T>
 inline void par_seg_seg_integ(T aSq_dSq, T aSq_dSqI, T x, T y, T* integ) {
-  using std::sqrt; using std::log;
+  using std::sqrt;
   T xpy = x + y;
   T xmy = x - y;
   T Ra = sqrt(aSq_dSq + xpy * xpy);
   T RaInv = T(1) / Ra;
-  T Log_Ra_ypz = log(Ra + xpy);
+  T Log_Ra_ypz = ulog(Ra + xpy);
   T tmp = xmy * Ra * aSq_dSqI;
   integ[0] = Ra * aSq_dSqI;
   integ[1] = T(-0.5) * (Log_Ra_ypz - tmp);
@@ -559,7 +579,7 @@ bool seg_seg_is_parallel(V3<T> n11, V3<T> n12, V3<T> n21, V3<T> n22) {
 // force on node 2; node 1 receives its negative.
 template <class T>
 V3<T> self_force(V3<T> tVec, V3<T> b, T a, T aSq, T mu4pi, T nuOmNuInv, T Ec, T omNuInv) {
-  using std::sqrt; using std::log;
+  using std::sqrt;
   T tVecSq = dot(tVec, tVec);
   T L = sqrt(tVecSq);
   T Linv = T(1) / L;
@@ -570,7 +590,7 @@ V3<T> self_force(V3<T> tVec, V3<T> b, T a, T aSq, T mu4pi, T nuOmNuInv, T Ec, T 
   T bEdgeSq = dot(bEdge, bEdge);
   T LaMa = La - a;
   T tor = mu4pi * bScrew *
-          (nuOmNuInv * (log((La + L) / a) - T(2) * LaMa * Linv) - LaMa * LaMa / (T(2) * La * L));
+          (nuOmNuInv * (ulog((La + L) / a) - T(2) * LaMa * Linv) - LaMa * LaMa / (T(2) * La * L));
   T torCore = T(2) * Ec * nuOmNuInv * bScrew;
   T torTot = tor + torCore;
   T lonCore = (bScrew * bScrew + bEdgeSq * omNuInv) * Ec;

@@ -51,6 +51,10 @@ def lib():
             "orc_pk_force": (None, mesh + net + [ip, i64, dp, i32, i32]),
             "orc_sigma_tilde": (None, [dp, i64, ip, i64, dp, dp, ip, i64, dp, i64, dp, i32]),
             "orc_node_gather": (None, [ip, i64, i64, ip, dp, dp, d, dp]),
+            "orc_set_libm": (None, [i32]),
+            "orc_segseg_par_rows": (i64, [dp] + net + [ip, i64, dp, i32]),
+            "orc_libm_log": (d, [d]),
+            "orc_libm_atan": (d, [d]),
         }
         for name, (res, args) in sig.items():
             f = getattr(_lib, name)
@@ -73,6 +77,19 @@ def _u(a):
 
 def _f(a, dt=np.float64):
     return np.asfortranarray(a, dtype=dt)
+
+
+def set_libm(glibc: bool):
+    """Use glibc's log/atan (True) or the fdlibm-method ones the kernels use."""
+    lib().orc_set_libm(1 if glibc else 0)
+
+
+def libm_log(x):
+    return lib().orc_libm_log(float(x))
+
+
+def libm_atan(x):
+    return lib().orc_libm_atan(float(x))
 
 
 def seg_consts(dln, mat):
@@ -163,6 +180,19 @@ def calcSegSegForcePar(dln, mat, net, nthreads):
     lib().orc_segseg_par_faithful(_d(K), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord), _d(n.bVec),
                                   _u(n.ext), _d(out), int(nthreads))
     return out
+
+
+def segseg_par_rows(dln, mat, net, rows, nthreads):
+    """The reference's threaded design on a sample of outer rows (each row i
+    against all j > i); returns (segForce 3 x 2 x numSeg, pairs evaluated)."""
+    n = _Net(net)
+    K = seg_consts(dln, mat)
+    r = np.ascontiguousarray(rows, dtype=np.int64)
+    out = np.zeros((3, 2, n.numSeg), order="F")
+    pairs = lib().orc_segseg_par_rows(_d(K), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord),
+                                      _d(n.bVec), _u(n.ext), _i(r), len(r), _d(out),
+                                      int(nthreads))
+    return out, int(pairs)
 
 
 def calcSelfForce(dln, mat, net, idx=None):

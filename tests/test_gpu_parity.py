@@ -1,0 +1,251 @@
+"""GPU parity: the HIP path (through the C-ABI) against the reference's golden
+vectors and against the CPU oracle on identical seeded networks.
+
+Bar: reference vectors at the reference's own tolerance (Julia isapprox,
+sqrt(eps) norm-wise); oracle comparisons at <= 1e-10 per-node relative error
+(floor 1e-6 RMS, see metrics.per_node_rel).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+import refnets as R
+from dddhip import forces as G
+from dddhip.generate import random_loop_network, face_grid_points, random_uhat
+from dddhip.types import RegularCuboidMesh
+from metrics import per_node_rel, isapprox, normwise
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def prm():
+    return R.params()
+
+
+# ---- reference golden vectors ------------------------------------------------
+
+def test_golden_self_force(prm):
+    dln, mat = prm
+    net = R.pentagons(10.0)
+    s1, s2 = G.calcSelfForce(dln, mat, net)
+    assert isapprox(s1, R.g("self_f1").T) and isapprox(s2, R.g("self_f2").T)
+    o1, o2 = O.calcSelfForce(dln, mat, net)
+    assert per_node_rel(s1, o1)[0] <= TOL and per_node_rel(s2, o2)[0] <= TOL
+
+
+@pytest.mark.parametrize("name,builder", [
+    ("coincident", lambda: R.pentagons(0.0)),
+    ("offset20", lambda: R.pentagons(20.0)),
+    ("hexagons", R.hexagons),
+])
+def test_golden_segseg(prm, name, builder):
+    dln, mat = prm
+    net = builder()
+    _log(f"segseg golden {name}")
+    f = G.calcSegSegForce(dln, mat, net)
+    assert isapprox(f[:, 0, :], R.g(f"segseg_{name}_f1").T)
+    assert isapprox(f[:, 1, :], R.g(f"segseg_{name}_f2").T)
+    o = O.calcSegSegForce(dln, mat, net)
+    assert per_node_rel(f, o)[0] <= TOL
+
+
+def test_golden_pk(prm):
+    m = R.mesh()
+    fd = R.sparse_uhat(m, "pk_uHat_index", "pk_uHat_value")
+    net = R.pk_network()
+    pk = G.calcPKForce(m, fd, net)
+    assert isapprox(pk, R.g("pk_fPKTest").T)
+    # in-place form: half to each node (SegmentForcesTest.jl:169-172)
+    net.segForce[:] = 0
+    G.calcPKForce_(m, fd, net)
+    n = int(net.numSeg[0])
+    assert isapprox(pk, net.segForce[:, 0, :n] * 2)
+    assert np.array_equal(net.segForce[:, 0, :n], net.segForce[:, 1, :n])
+    for idx in (5, [5, 10, 9, 23], range(15, 33)):
+        p = G.calcPKForce(m, fd, net, idx)
+        ii = np.atleast_1d(np.asarray(list(idx) if isinstance(idx, range) else idx)) - 1
+        assert isapprox(p, R.g("pk_fPKTest")[ii].T)
+    # outside the mesh the FE stress still interpolates from the clamped
+    # element; the reference's check (:552-555) is that PK is zero when uHat
+    # is zero there, which the zero-uHat mobility test covers.
+
+
+@pytest.mark.parametrize("b,key", [([1, 0, 0], "sigmatilde_edge"), ([0, 0, 1], "sigmatilde_screw")])
+def test_golden_sigma_tilde(b, key):
+    pts, dln, mat, net = R.sigma_segment(b)
+    s = G.calc_σTilde(pts, dln, mat, net)
+    t = R.g(key).copy()
+    t[:, [4, 5]] = t[:, [5, 4]]  # the test matrices store (..., yz, xz)
+    assert isapprox(s.T, t)
+    σ = np.zeros((6, pts.shape[1]), order="F")
+    G.calc_σTilde_(σ, pts, dln, mat, net)
+    assert np.array_equal(σ, s)
+    o = O.calc_σTilde(pts, dln, mat, net)
+    assert normwise(s, o) <= TOL
+
+
+def test_golden_node_force(prm):
+    """Whole path + gather against MobilityTest.jl:78-90 (PK = 0)."""
+    dln, mat = prm
+    net = R.mobility_network()
+    m = R.mesh()
+    fd = R.ForceDisplacement(np.zeros(3 * m.numNode))
+    G.calcSegForce_(dln, mat, m, fd, net)
+    nf = G.calcNodeForce_(dln, mat, net)
+    assert isapprox(nf, R.g("mobility_testForce").T)
+    onet = R.mobility_network()
+    sf = O.calcSegForce(dln, mat, m, fd, onet)
+    onet.segForce[:, :, :sf.shape[2]] = sf
+    assert per_node_rel(nf, O.nodeForce(mat, onet))[0] <= TOL
+
+
+# ---- seeded random networks against the oracle -------------------------------
+
+@pytest.fixture(scope="module")
+def netA():
+    # 20 loops x 32 segments, seed 0xA0
+    return random_loop_network(20, seed=0xA0)
+
+
+def test_config_a_segseg(prm, netA):
+    dln, mat = prm
+    _log("config A seg-seg")
+    g = G.calcSegSegForce(dln, mat, netA)
+    o = O.calcSegSegForce(dln, mat, netA)
+    mx, rms = per_node_rel(g, o)
+    _log(f"config A per-node rel err max {mx:.3e} rms {rms:.3e}")
+    assert mx <= TOL
+
+
+def test_config_a_total_with_pk(prm, netA):
+    dln, mat = prm
+    mesh = RegularCuboidMesh(dx=2000.0, dy=2000.0, dz=2000.0, mx=5, my=5, mz=5, μ=mat.μ, ν=mat.ν)
+    fd = random_uhat(mesh, seed=0xA1)
+    g = G.calcSegForce(dln, mat, mesh, fd, netA)
+    o = O.calcSegForce(dln, mat, mesh, fd, netA)
+    assert per_node_rel(g, o)[0] <= TOL
+    # in-place on a buffer longer than numSeg (default memBuffer), then gather
+    net = random_loop_network(20, seed=0xA0, memBuffer=None)
+    assert net.segForce.shape[2] > int(net.numSeg[0])
+    net.segForce[:] = 7.0  # calcSegForce! must zero the live range first
+    G.calcSegForce_(dln, mat, mesh, fd, net)
+    n = int(net.numSeg[0])
+    assert per_node_rel(net.segForce[:, :, :n], o)[0] <= TOL
+    assert np.all(net.segForce[:, :, n:] == 7.0)
+    nf = G.calcNodeForce_(dln, mat, net)
+    onet = random_loop_network(20, seed=0xA0, memBuffer=None)
+    onet.segForce[:, :, :n] = o
+    assert per_node_rel(nf, O.nodeForce(mat, onet))[0] <= TOL
+
+
+def test_idx_rows(prm, netA):
+    dln, mat = prm
+    idx = [5, 17, 17, 600, 1, 640, 333]  # duplicates allowed, as rand(1:n, m) gives
+    g = G.calcSegSegForce(dln, mat, netA, idx)
+    o = O.calcSegSegForce(dln, mat, netA, idx)
+    assert per_node_rel(g, o)[0] <= TOL
+    # Rows put the row segment first in every pair, the full triangle puts
+    # the lower index first; the two orders agree to the conditioning of the
+    # pair formula, which the reference checks with isapprox
+    # (SegmentForcesTest.jl:369-380).
+    full = O.calcSegSegForce(dln, mat, netA)
+    assert isapprox(g, full[:, :, np.asarray(idx) - 1])
+    # in-place rows add into columns idx
+    net = random_loop_network(20, seed=0xA0)
+    G.calcSegSegForce_(dln, mat, net, 42)
+    assert per_node_rel(net.segForce[:, :, 41], O.calcSegSegForce(dln, mat, net, 42))[0] <= TOL
+    assert isapprox(net.segForce[:, :, 41], full[:, :, 41])
+    assert np.count_nonzero(net.segForce[:, :, :41]) == 0
+    s1, s2 = G.calcSelfForce(dln, mat, netA, range(3, 9))
+    o1, o2 = O.calcSelfForce(dln, mat, netA, range(3, 9))
+    assert per_node_rel(s1, o1)[0] <= TOL and per_node_rel(s2, o2)[0] <= TOL
+    t = G.calcSegForce(dln, mat, None, None, netA, idx)
+    ot = O.calcSegSegForce(dln, mat, netA, idx)
+    a, b = O.calcSelfForce(dln, mat, netA, idx)
+    ot[:, 0, :] += a
+    ot[:, 1, :] += b
+    assert per_node_rel(t, ot)[0] <= TOL
+
+
+def test_external_segments(prm):
+    dln, mat = prm
+    net = random_loop_network(12, seed=0xA2, ext_fraction=0.15)
+    assert net.extSeg[:int(net.numSeg[0])].any()
+    g = G.calcSegSegForce(dln, mat, net)
+    o = O.calcSegSegForce(dln, mat, net)
+    assert per_node_rel(g, o)[0] <= TOL
+    ext = net.extSeg[:int(net.numSeg[0])]
+    assert np.all(g[:, :, ext] == 0)
+
+
+def test_deterministic(prm):
+    dln, mat = prm
+    net = random_loop_network(64, seed=0xA3)
+    a = G.calcSegSegForce(dln, mat, net)
+    b = G.calcSegSegForce(dln, mat, net)
+    assert np.array_equal(a, b)
+
+
+def test_edge_cases(prm):
+    dln, mat = prm
+    net = random_loop_network(1, numSides=3, seed=0xA4)  # 3 segments, one tile
+    assert per_node_rel(G.calcSegSegForce(dln, mat, net), O.calcSegSegForce(dln, mat, net))[0] <= TOL
+    # 65 segments: one full tile plus a ragged one
+    net = random_loop_network(13, numSides=5, seed=0xA5)
+    assert per_node_rel(G.calcSegSegForce(dln, mat, net), O.calcSegSegForce(dln, mat, net))[0] <= TOL
+
+
+def test_sigma_tilde_random(prm):
+    dln, mat = prm
+    net = random_loop_network(20, seed=0xA6)
+    pts = face_grid_points(16)
+    g = G.calc_σTilde(pts, dln, mat, net)
+    o = O.calc_σTilde(pts, dln, mat, net)
+    mx, _ = per_node_rel(g[:3], o[:3])
+    assert normwise(g, o) <= TOL
+    assert np.max(np.abs(g - o)) <= TOL * np.max(np.abs(o))
+    # idx subset and single point
+    sub = G.calc_σTilde(pts, dln, mat, net, [3, 1, 200])
+    assert normwise(sub, O.calc_σTilde(pts, dln, mat, net, [3, 1, 200])) <= TOL
+    one = G.calc_σTilde(pts[:, 7], dln, mat, net)
+    assert one.shape == (6,) and normwise(one, o[:, 7]) <= TOL
+
+
+def test_medium_network(prm):
+    """4,096 segments against the oracle's threaded restatement."""
+    dln, mat = prm
+    net = random_loop_network(128, seed=0xB1)
+    _log("medium 4096 seg-seg")
+    g = G.calcSegSegForce(dln, mat, net)
+    o = O.calcSegSegForcePar(dln, mat, net, 16)
+    mx, rms = per_node_rel(g, o)
+    _log(f"medium per-node rel err max {mx:.3e} rms {rms:.3e}")
+    assert mx <= TOL
+
+
+def test_config_b(prm):
+    """configs[1] size (16,384 segments, 1.34e8 pairs): the full GPU triangle
+    against the oracle's threaded triangle, GPU rows against oracle rows, and
+    the vanishing net force of closed loops."""
+    dln, mat = prm
+    net = random_loop_network(512, seed=0xB0)
+    _log("config B seg-seg")
+    g = G.calcSegSegForce(dln, mat, net)
+    o = O.calcSegSegForcePar(dln, mat, net, 16)
+    mx, rms = per_node_rel(g, o)
+    _log(f"config B per-node rel err max {mx:.3e} rms {rms:.3e}")
+    assert mx <= TOL
+    rows = np.linspace(1, 16384, 48).astype(np.int64)
+    orow = O.calcSegSegForce(dln, mat, net, rows)
+    gr = G.calcSegSegForce(dln, mat, net, rows)
+    assert per_node_rel(gr, orow)[0] <= TOL
+    # Net force of pair interactions of closed loops vanishes.
+    tot = g.reshape(3, -1, order="F").sum(axis=1)
+    assert np.all(np.abs(tot) <= 1e-12 * np.abs(g).sum())
