@@ -70,6 +70,14 @@ def main():
     stream = torch.cuda.ExternalStream(res.stream(), device=torch.device("cuda:0"))
     log("uploaded")
 
+    # Calibration kernel for the HBM counters: one plain 1 GiB device copy
+    # (1 GiB read + 1 GiB written), profiled in the same process.
+    cal_src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    cal_dst = torch.empty_like(cal_src)
+    cal_dst.copy_(cal_src)
+    torch.cuda.synchronize()
+    del cal_src, cal_dst
+
     for w in range(args.warmup):
         res.seg_force()
         stream.synchronize()
