@@ -1,0 +1,178 @@
+# DDDHip.jl -- routes DDD.jl's force path to libdddhip.so when
+# dlnParams.parGPU is true.  Include after `using DDD`; the methods below take
+# the same arguments as the reference's (src/Processing/DislocationForces.jl,
+# src/Processing/DislocationFEM.jl) and pass the network's arrays to the C ABI
+# declared in include/ddd_hip.h without copying them.
+#
+# This file cannot be executed in the build container (no Julia); the same
+# ABI is exercised from Python ctypes with identical memory layouts.
+module DDDHip
+
+using DDD
+using SparseArrays
+
+const libddd = joinpath(@__DIR__, "..", "libdddhip.so")
+
+const DDD_ACCUMULATE = Cint(1)
+const DDD_OUT_BY_POSITION = Cint(2)
+const DDD_PK_HALF_TO_NODES = Cint(4)
+
+struct Consts
+    mu4pi::Float64
+    mu8pi::Float64
+    mu4pinu::Float64
+    nuMu4pinu::Float64
+    omNuInv::Float64
+    nuOmNuInv::Float64
+    coreRad::Float64
+    coreRadSq::Float64
+    coreEnergy::Float64
+    sigmaPN::Float64
+end
+
+struct Network
+    numNode::Int64
+    numSeg::Int64
+    segIdx::Ptr{Int64}
+    segIdxLd::Int64
+    coord::Ptr{Float64}
+    coordCols::Int64
+    bVec::Ptr{Float64}
+    bVecCols::Int64
+    extSeg::Ptr{UInt8}
+    links::Ptr{Int64}
+    linksCols::Int64
+    connectivity::Ptr{Int64}
+    connLd::Int64
+    connCols::Int64
+end
+
+struct Mesh
+    C::Ptr{Float64}
+    coord::Ptr{Float64}
+    numNode::Int64
+    connectivity::Ptr{Int64}
+    numElem::Int64
+    mx::Int64
+    my::Int64
+    mz::Int64
+    w::Float64
+    h::Float64
+    d::Float64
+    uHat::Ptr{Float64}
+end
+
+consts(d::DislocationParameters, m::MaterialParameters) = Consts(
+    m.μ4π, m.μ8π, m.μ4πν, m.νμ4πν, m.omνInv, m.νomνInv,
+    d.coreRad, d.coreRadSq, d.coreEnergy, m.σPN)
+
+# Vector{Bool} is one byte per element, as the ABI expects.
+netstruct(n::DislocationNetwork) = Network(
+    n.numNode[1], n.numSeg[1],
+    pointer(n.segIdx), size(n.segIdx, 1),
+    pointer(n.coord), size(n.coord, 2),
+    pointer(n.bVec), size(n.bVec, 2),
+    Ptr{UInt8}(pointer(n.extSeg)),
+    pointer(n.links), size(n.links, 2),
+    pointer(n.connectivity), size(n.connectivity, 1), size(n.connectivity, 2))
+
+meshstruct(mesh, uHat::Vector{Float64}) = Mesh(
+    pointer(Matrix(mesh.C)), pointer(mesh.coord), size(mesh.coord, 2),
+    pointer(mesh.connectivity), size(mesh.connectivity, 2),
+    mesh.mx, mesh.my, mesh.mz, mesh.w, mesh.h, mesh.d, pointer(uHat))
+
+mutable struct Ctx
+    h::Ptr{Cvoid}
+end
+const CTX = Ref{Ctx}()
+function ctx()
+    if !isassigned(CTX)
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = ccall((:ddd_ctx_create, libddd), Cint, (Ref{Ptr{Cvoid}}, Cint), h, 0)
+        rc == 0 || error("ddd_ctx_create failed ($rc): no gfx950 device")
+        CTX[] = Ctx(h[])
+    end
+    return CTX[].h
+end
+
+check(rc) = rc == 0 || error(unsafe_string(ccall((:ddd_last_error, libddd), Cstring, (Ptr{Cvoid},), ctx())))
+
+idxvec(idx) = isnothing(idx) ? Int64[] : collect(Int64, idx isa Integer ? (idx,) : idx)
+
+function calcSegSegForce!(dlnParams, matParams, network::DislocationNetwork, idx = nothing)
+    dlnParams.parGPU || return DDD.calcSegSegForce!(dlnParams, matParams, network, idx)
+    k = Ref(consts(dlnParams, matParams))
+    ix = idxvec(idx)
+    GC.@preserve network ix begin
+        nw = Ref(netstruct(network))
+        check(ccall((:ddd_segseg_force, libddd), Cint,
+            (Ptr{Cvoid}, Ref{Consts}, Ref{Network}, Ptr{Int64}, Int64, Ptr{Float64}, Int64, Cint),
+            ctx(), k, nw, isnothing(idx) ? C_NULL : pointer(ix), length(ix),
+            pointer(network.segForce), size(network.segForce, 3), DDD_ACCUMULATE))
+    end
+    return nothing
+end
+
+function calcSegSegForce(dlnParams, matParams, network::DislocationNetwork, idx = nothing)
+    dlnParams.parGPU || return DDD.calcSegSegForce(dlnParams, matParams, network, idx)
+    k = Ref(consts(dlnParams, matParams))
+    ix = idxvec(idx)
+    m = isnothing(idx) ? network.numSeg[1] : length(ix)
+    out = zeros(3, 2, m)
+    GC.@preserve network ix out begin
+        nw = Ref(netstruct(network))
+        check(ccall((:ddd_segseg_force, libddd), Cint,
+            (Ptr{Cvoid}, Ref{Consts}, Ref{Network}, Ptr{Int64}, Int64, Ptr{Float64}, Int64, Cint),
+            ctx(), k, nw, isnothing(idx) ? C_NULL : pointer(ix), length(ix),
+            pointer(out), m, DDD_OUT_BY_POSITION))
+    end
+    return out
+end
+
+function calcSegForce!(dlnParams, matParams, mesh, forceDisplacement, network::DislocationNetwork,
+                       idx = nothing)
+    dlnParams.parGPU ||
+        return DDD.calcSegForce!(dlnParams, matParams, mesh, forceDisplacement, network, idx)
+    k = Ref(consts(dlnParams, matParams))
+    ix = idxvec(idx)
+    range = isnothing(idx) ? (1:network.numSeg[1]) : ix
+    network.segForce[:, :, range] .= 0
+    uHat = Vector{Float64}(forceDisplacement.uHat)  # the ABI takes uHat dense
+    C = Matrix(mesh.C)
+    GC.@preserve network ix uHat C mesh begin
+        nw = Ref(netstruct(network))
+        ms = Ref(Mesh(pointer(C), pointer(mesh.coord), size(mesh.coord, 2),
+                      pointer(mesh.connectivity), size(mesh.connectivity, 2),
+                      mesh.mx, mesh.my, mesh.mz, mesh.w, mesh.h, mesh.d, pointer(uHat)))
+        check(ccall((:ddd_seg_force, libddd), Cint,
+            (Ptr{Cvoid}, Ref{Consts}, Ref{Network}, Ref{Mesh}, Ptr{Int64}, Int64, Ptr{Float64},
+             Int64, Cint),
+            ctx(), k, nw, ms, isnothing(idx) ? C_NULL : pointer(ix), length(ix),
+            pointer(network.segForce), size(network.segForce, 3), DDD_ACCUMULATE))
+    end
+    return nothing
+end
+
+function calc_σTilde!(σ, x0, dlnParams, matParams, network::DislocationNetwork, idx = nothing)
+    dlnParams.parGPU || return DDD.calc_σTilde!(σ, x0, dlnParams, matParams, network, idx)
+    pts = x0 isa Base.Iterators.ProductIterator ?
+          reduce(hcat, [collect(p) for p in x0]) : Matrix{Float64}(x0)  # x fastest
+    @assert size(σ, 1) == 6 && size(σ, 2) == size(pts, 2)
+    k = Ref(consts(dlnParams, matParams))
+    ix = idxvec(idx)
+    GC.@preserve network ix pts σ begin
+        nw = Ref(netstruct(network))
+        check(ccall((:ddd_sigma_tilde, libddd), Cint,
+            (Ptr{Cvoid}, Ref{Consts}, Ref{Network}, Ptr{Int64}, Int64, Ptr{Float64}, Int64,
+             Ptr{Float64}, Cint),
+            ctx(), k, nw, isnothing(idx) ? C_NULL : pointer(ix), length(ix),
+            pointer(pts), size(pts, 2), pointer(σ), Cint(0)))
+    end
+    return σ
+end
+
+calc_σTilde(x0, dlnParams, matParams, network, idx = nothing) =
+    calc_σTilde!(zeros(6, x0 isa Base.Iterators.ProductIterator ? prod(size(x0)) :
+                         div(length(x0), 3)), x0, dlnParams, matParams, network, idx)
+
+end # module
