@@ -40,6 +40,8 @@ def main():
     ap.add_argument("--seed", type=lambda s: int(s, 0), default=0xD0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-row-stride", type=int, default=64)
+    ap.add_argument("--paths", default="fp64,mixed",
+                    help="comma list of paths to time after plain FP64: mixed")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -111,6 +113,33 @@ def main():
     out = res.download()
     assert np.all(np.isfinite(out)), "non-finite forces"
 
+    # Other paths on the same resident network, timed the same way; their
+    # error is measured against the plain FP64 result (which agrees with the
+    # CPU oracle to <= 1e-10 per node).
+    from metrics import per_node_rel, normwise
+    extra = {}
+    for path in [p for p in args.paths.split(",") if p and p != "fp64"]:
+        res.set_mode(path)
+        res.seg_force()
+        stream.synchronize()
+        pe = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+        for k in range(args.steps):
+            pe[k][0].record(stream)
+            res.seg_force()
+            pe[k][1].record(stream)
+            pe[k][1].synchronize()
+            log(f"{path} step {k + 1}/{args.steps}: {pe[k][0].elapsed_time(pe[k][1]):.1f} ms")
+        pms = float(np.mean([e[0].elapsed_time(e[1]) for e in pe]))
+        got = res.download()
+        tot, fp64pairs = res.stats()
+        mx, rms = per_node_rel(got, out)
+        extra[path] = {"ms": round(pms, 3), "speedup_vs_fp64": round(ms / pms, 3),
+                       "per_node_rel_err_max": mx, "per_node_rel_err_rms": rms,
+                       "normwise_rel_err": normwise(got, out),
+                       "fp64_pair_share": round(fp64pairs / max(tot, 1), 5)}
+        log(f"{path}: {pms:.1f} ms, per-node err max {mx:.2e} rms {rms:.2e}")
+        res.set_mode("fp64")
+
     achieved = FLOPS_PER_PAIR * pairs / (tms * 1e-3) / 1e12
     result = {
         "metric": METRIC,
@@ -136,6 +165,7 @@ def main():
             "tile_kernel_ms": round(tms, 3),
             "reduce_kernel_ms": round(float(np.mean(red_ms)), 3),
         },
+        "paths": extra,
         "roofline": {
             "bound": "valu",
             "achieved": round(achieved, 3),

@@ -69,7 +69,9 @@ struct ddd_ctx {
   // gather
   DevBuf conn, links, coord, nodeF;
   std::vector<double> h;  // host staging
+  DevBuf counter;  // FP64-pair counter of the mixed path
   // device-resident network
+  ddd_opts opts{DDD_MODE_FP64, 0.0, 0.0};
   int64_t resN = -1;
   int64_t resLd = 0;
   ddd_consts resK{};
@@ -183,17 +185,50 @@ void check_flags(ddd_ctx* c) {
                                "near-parallel (results written, unreliable)"};
 }
 
-// Whole-network seg-seg into dev (6 x n, accumulated).
-void segseg_all(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsigned* flags) {
+MixedRule mixed_rule(const ddd_consts& k, const ddd_opts& o) {
+  MixedRule r;
+  r.farDist = o.farDist > 0 ? o.farDist : 5.0 * k.coreRad;
+  r.sinMin = o.sinMin > 0 ? o.sinMin : 1e-3;
+  r.K = SigK<float>{float(k.coreRadSq), float(k.mu4pi), float(k.mu4pinu), float(k.nuMu4pinu),
+                    float(k.coreRadSq * k.mu8pi)};
+  return r;
+}
+
+void check_opts(const ddd_opts* o) {
+  if (!o) return;
+  if (o->mode != DDD_MODE_FP64 && o->mode != DDD_MODE_MIXED)
+    bad("mode must be DDD_MODE_FP64 or DDD_MODE_MIXED here");
+  if (o->farDist < 0 || o->sinMin < 0) bad("farDist and sinMin must be >= 0");
+}
+
+// Pair tiles of the whole network (mode from opts) without the reduction.
+void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
+                  const ddd_opts* opts) {
   if (n < 2) return;
   const int64_t nb = (n + kTile - 1) / kTile;
   const int64_t T = tiles_for(nb);
   const size_t part = sizeof(double) * 6 * kTile * (size_t)T;
   c->rowPart.ensure(part);
   c->colPart.ensure(part);
-  launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k),
-                      c->rowPart.as<double>(), c->colPart.as<double>(), flags, c->st);
+  c->counter.ensure(sizeof(unsigned long long));
+  if (opts && opts->mode == DDD_MODE_MIXED) {
+    HIPCHK(hipMemsetAsync(c->counter.p, 0, sizeof(unsigned long long), c->st));
+    const MixedRule r = mixed_rule(k, *opts);
+    launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k),
+                        c->rowPart.as<double>(), c->colPart.as<double>(), flags, c->st, &r,
+                        c->counter.as<unsigned long long>());
+  } else {
+    launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k),
+                        c->rowPart.as<double>(), c->colPart.as<double>(), flags, c->st);
+  }
   HIPCHK(hipGetLastError());
+}
+
+// Whole-network seg-seg into dev (6 x n, accumulated).
+void segseg_all(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsigned* flags,
+                const ddd_opts* opts = nullptr) {
+  if (n < 2) return;
+  segseg_tiles(c, k, n, flags, opts);
   launch_segseg_reduce(c->rowPart.as<double>(), c->colPart.as<double>(), (int)n, dev, c->st);
   HIPCHK(hipGetLastError());
 }
@@ -275,7 +310,7 @@ int ddd_ctx_destroy(ddd_ctx* c) {
     std::lock_guard<std::mutex> lk(c->mu);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->st);
-    DevBuf* all[] = {&c->seg, &c->rowPart, &c->colPart, &c->force, &c->rows, &c->out,
+    DevBuf* all[] = {&c->counter, &c->seg, &c->rowPart, &c->colPart, &c->force, &c->rows, &c->out,
                      &c->flags, &c->meshC, &c->meshCoord, &c->meshConn, &c->uHat, &c->sseg,
                      &c->pts, &c->sigPart, &c->sigma, &c->conn, &c->links, &c->coord, &c->nodeF};
     for (DevBuf* b : all) b->release();
@@ -552,10 +587,7 @@ void net_stage(ddd_ctx* c, int stage, int withPK) {
     if (withPK) launch_pk_force(c->seg.as<double>(), c->resLd, nullptr, (int)n, c->resM, dev, 1, c->st);
     launch_self_force(c->seg.as<double>(), c->resLd, nullptr, (int)n, self_consts(c->resK), dev, c->st);
   } else if (stage == 1) {
-    if (n >= 2)
-      launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(c->resK),
-                          c->rowPart.as<double>(), c->colPart.as<double>(), c->flags.as<unsigned>(),
-                          c->st);
+    segseg_tiles(c, c->resK, n, c->flags.as<unsigned>(), &c->opts);
   } else if (stage == 2) {
     if (n >= 2)
       launch_segseg_reduce(c->rowPart.as<double>(), c->colPart.as<double>(), (int)n, dev, c->st);
@@ -582,6 +614,53 @@ int ddd_net_download(ddd_ctx* c, double* out, int64_t outCols) {
     check_out(out, outCols, c->resN);
     check_flags(c);
     store6(c, c->force.as<double>(), c->resN, nullptr, out, outCols, 0);
+  });
+}
+
+void fill_stats(ddd_ctx* c, int64_t n, const ddd_opts* o, ddd_stats* st) {
+  if (!st) return;
+  st->pairs = n > 1 ? (uint64_t)n * (uint64_t)(n - 1) / 2 : 0;
+  st->pairsFP64 = st->pairs;
+  if (o && o->mode == DDD_MODE_MIXED && n > 1) {
+    unsigned long long e = 0;
+    HIPCHK(hipMemcpyAsync(&e, c->counter.p, sizeof(e), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    st->pairsFP64 = e;
+  }
+}
+
+int ddd_segseg_force_opts(ddd_ctx* c, const ddd_consts* k, const ddd_network* net,
+                          const ddd_opts* opts, double* out, int64_t outCols, int flags,
+                          ddd_stats* stats) {
+  return guard(c, "ddd_segseg_force_opts", [&] {
+    if (!k) bad("consts is NULL");
+    check_net(net);
+    check_opts(opts);
+    const int64_t n = net->numSeg;
+    check_out(out, outCols, n);
+    upload_segments(c, net);
+    unsigned* fl = reset_flags(c);
+    double* dev = zeroed(c, c->force, 6 * n);
+    segseg_all(c, *k, n, dev, fl, opts);
+    check_flags(c);
+    fill_stats(c, n, opts, stats);
+    store6(c, dev, n, nullptr, out, outCols, flags);
+    c->resN = -1;
+  });
+}
+
+int ddd_net_set_opts(ddd_ctx* c, const ddd_opts* o) {
+  return guard(c, "ddd_net_set_opts", [&] {
+    check_opts(o);
+    c->opts = o ? *o : ddd_opts{DDD_MODE_FP64, 0.0, 0.0};
+  });
+}
+
+int ddd_net_stats(ddd_ctx* c, ddd_stats* st) {
+  return guard(c, "ddd_net_stats", [&] {
+    if (c->resN < 0) bad("no network uploaded (ddd_net_upload)");
+    if (!st) bad("stats is NULL");
+    fill_stats(c, c->resN, &c->opts, st);
   });
 }
 

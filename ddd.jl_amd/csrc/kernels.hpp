@@ -25,8 +25,17 @@ enum : unsigned {
 __host__ __device__ inline int64_t tiles_for(int64_t nb) { return nb * (nb + 1) / 2; }
 
 void launch_prep_segments(double* seg, int64_t ld, int64_t n, hipStream_t st);
+// Mixed-precision rule: pairs farther than farDist (midpoint distance) whose
+// tangents satisfy |t1 x t2| > sinMin go to the FP32 far-pair quadrature.
+struct MixedRule {
+  double farDist, sinMin;
+  SigK<float> K;
+};
+// mixed == nullptr: plain FP64.  nExact (mixed only) accumulates the number
+// of pairs evaluated on the FP64 path.
 void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
-                         double* rowPart, double* colPart, unsigned* flags, hipStream_t st);
+                         double* rowPart, double* colPart, unsigned* flags, hipStream_t st,
+                         const MixedRule* mixed = nullptr, unsigned long long* nExact = nullptr);
 void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, double* out,
                           hipStream_t st);
 void launch_segseg_rows(const double* seg, int64_t ld, int n, const int64_t* rows, int nRows,

@@ -123,11 +123,22 @@ __global__ void k_prep_segments(double* __restrict__ seg, int64_t ld, int64_t n)
 
 constexpr int kQueue = 2 * kTile;
 
+// Pair-tile kernel.  MODE 0 (plain FP64): every pair uses the FP64 analytic
+// force; generic pairs run inline and near-parallel pairs are queued.
+// MODE 1 (mixed precision): well-separated pairs (midpoint distance > farDist
+// and |t1 x t2| > sinMin) run inline in FP32 by quadrature (far_side); all
+// other pairs -- close, near-parallel or both -- are queued and evaluated in
+// FP64 (generic or near-parallel branch) 64 at a time, so the wavefront never
+// waits on a minority path.  The number of FP64 pairs is added to *nExact.
+template <int MODE>
 __global__ __launch_bounds__(kTile) void k_segseg_tiles(const double* __restrict__ seg, int64_t ld,
                                                         int n, int64_t tile0, SegK<double> K,
+                                                        SigK<float> KF, double farDist2,
+                                                        double sinMin2,
                                                         double* __restrict__ rowPart,
                                                         double* __restrict__ colPart,
-                                                        unsigned* __restrict__ flags) {
+                                                        unsigned* __restrict__ flags,
+                                                        unsigned long long* __restrict__ nExact) {
   const int64_t t = tile0 + blockIdx.x;
   // Tile t of the upper triangle, enumerated column-block major:
   // t = J (J + 1) / 2 + I with 0 <= I <= J.
@@ -159,14 +170,23 @@ __global__ __launch_bounds__(kTile) void k_segseg_tiles(const double* __restrict
   const bool diag = (I == J);
   v3<double> a1 = V(0.0, 0.0, 0.0), a2 = V(0.0, 0.0, 0.0);
   int nq = 0;  // wave-uniform queue length
+  unsigned exact = 0;
   const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const v3<double> midA = V(0.5 * (A.n1.x + A.n2.x), 0.5 * (A.n1.y + A.n2.y), 0.5 * (A.n1.z + A.n2.z));
 
   auto flush = [&](int cnt) {
     if (lane < cnt) {
       const unsigned e = queue[lane];
       const int rl = e & 63, sl = (rl + (e >> 6)) & 63;
+      const Seg<double> P = seg_from_lds(si, rl), Q = seg_from_lds(sj, sl);
       v3<double> F1, F2, F3, F4;
-      special_pair(K, seg_from_lds(si, rl), seg_from_lds(sj, sl), F1, F2, F3, F4, flags);
+      double c;
+      const double omc = one_minus_c2(P.t, Q.t, c);
+      if (MODE == 1 && omc > DDD_SQRT_EPS) {
+        pair_generic<double, true, true>(K, P, Q, c, omc, F1, F2, F3, F4);
+      } else {
+        special_pair(K, P, Q, F1, F2, F3, F4, flags);
+      }
       sres[0][lane] = F1.x; sres[1][lane] = F1.y; sres[2][lane] = F1.z;
       sres[3][lane] = F2.x; sres[4][lane] = F2.y; sres[5][lane] = F2.z;
       sres[6][lane] = F3.x; sres[7][lane] = F3.y; sres[8][lane] = F3.z;
@@ -195,13 +215,32 @@ __global__ __launch_bounds__(kTile) void k_segseg_tiles(const double* __restrict
     const Seg<double> B = seg_from_lds(sj, s);
     double c;
     const double omcSq = one_minus_c2(A.t, B.t, c);
-    const bool special = live && !(omcSq > DDD_SQRT_EPS);
-    const uint64_t m = __ballot(special);
-    if (m) {
-      if (special) queue[nq + __popcll(m & ltMask)] = (unsigned short)((k << 6) | lane);
-      nq += __popcll(m);
+    bool queued, inlineFP64 = false;
+    if (MODE == 0) {
+      queued = live && !(omcSq > DDD_SQRT_EPS);
+      inlineFP64 = live && !queued;
+    } else {
+      const double dx = 0.5 * (B.n1.x + B.n2.x) - midA.x, dy = 0.5 * (B.n1.y + B.n2.y) - midA.y,
+                   dz = 0.5 * (B.n1.z + B.n2.z) - midA.z;
+      const bool far = (dx * dx + dy * dy + dz * dz > farDist2) && (omcSq > sinMin2);
+      queued = live && !far;
+      if (live && far) {
+        v3<double> F1, F2, F3, F4;
+        far_side<float>(KF, A, B, F1, F2);
+        far_side<float>(KF, B, A, F3, F4);
+        a1 = a1 + F1;
+        a2 = a2 + F2;
+        cacc[0][s] += F3.x; cacc[1][s] += F3.y; cacc[2][s] += F3.z;
+        cacc[3][s] += F4.x; cacc[4][s] += F4.y; cacc[5][s] += F4.z;
+      }
     }
-    if (live && !special) {
+    const uint64_t m = __ballot(queued);
+    if (m) {
+      if (queued) queue[nq + __popcll(m & ltMask)] = (unsigned short)((k << 6) | lane);
+      nq += __popcll(m);
+      exact += __popcll(m);
+    }
+    if (MODE == 0 && inlineFP64) {
       v3<double> F1, F2, F3, F4;
       pair_generic<double, true, true>(K, A, B, c, omcSq, F1, F2, F3, F4);
       a1 = a1 + F1;
@@ -221,6 +260,7 @@ __global__ __launch_bounds__(kTile) void k_segseg_tiles(const double* __restrict
     }
   }
   if (nq > 0) flush(nq);
+  if (MODE == 1 && lane == 0 && exact) atomicAdd(nExact, (unsigned long long)exact);
 
   double* rp = rowPart + t * 6 * kTile;
   double* cp = colPart + t * 6 * kTile;
@@ -303,14 +343,22 @@ void launch_prep_segments(double* seg, int64_t ld, int64_t n, hipStream_t st) {
 }
 
 void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
-                         double* rowPart, double* colPart, unsigned* flags, hipStream_t st) {
+                         double* rowPart, double* colPart, unsigned* flags, hipStream_t st,
+                         const MixedRule* mixed, unsigned long long* nExact) {
   const int64_t nb = (n + kTile - 1) / kTile;
   const int64_t T = tiles_for(nb);
   const int64_t maxGrid = 1ll << 30;
+  const SigK<float> KF = mixed ? mixed->K : SigK<float>{};
+  const double fd2 = mixed ? mixed->farDist * mixed->farDist : 0.0;
+  const double sm2 = mixed ? mixed->sinMin * mixed->sinMin : 0.0;
   for (int64_t t0 = 0; t0 < T; t0 += maxGrid) {
     const int64_t cnt = (T - t0 < maxGrid) ? (T - t0) : maxGrid;
-    hipLaunchKernelGGL(k_segseg_tiles, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, t0, K,
-                       rowPart, colPart, flags);
+    if (mixed)
+      hipLaunchKernelGGL(k_segseg_tiles<1>, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, t0,
+                         K, KF, fd2, sm2, rowPart, colPart, flags, nExact);
+    else
+      hipLaunchKernelGGL(k_segseg_tiles<0>, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, t0,
+                         K, KF, fd2, sm2, rowPart, colPart, flags, nExact);
   }
 }
 

@@ -44,6 +44,17 @@ class ddd_mesh(C.Structure):
                 ("h", C.c_double), ("d", C.c_double), ("uHat", dp)]
 
 
+DDD_MODE_FP64, DDD_MODE_MIXED, DDD_MODE_HYBRID = 0, 1, 2
+
+
+class ddd_opts(C.Structure):
+    _fields_ = [("mode", C.c_int), ("farDist", C.c_double), ("sinMin", C.c_double)]
+
+
+class ddd_stats(C.Structure):
+    _fields_ = [("pairs", C.c_uint64), ("pairsFP64", C.c_uint64)]
+
+
 class DDDError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[{code}] {msg}")
@@ -86,6 +97,11 @@ def lib():
                                              C.POINTER(ddd_mesh)]),
                 "ddd_net_seg_force": (C.c_int, [P, C.c_int]),
                 "ddd_net_stage": (C.c_int, [P, C.c_int, C.c_int]),
+                "ddd_net_set_opts": (C.c_int, [P, C.POINTER(ddd_opts)]),
+                "ddd_net_stats": (C.c_int, [P, C.POINTER(ddd_stats)]),
+                "ddd_segseg_force_opts": (C.c_int, [P, C.POINTER(ddd_consts),
+                                                    C.POINTER(ddd_network), C.POINTER(ddd_opts),
+                                                    dp, i64, C.c_int, C.POINTER(ddd_stats)]),
                 "ddd_net_download": (C.c_int, [P, dp, i64]),
                 "ddd_net_segforce_device": (P, [P]),
             }

@@ -50,6 +50,22 @@ def calcSegSegForce(dlnParams, matParams, network, idx=None, ctx=None):
     return out
 
 
+def calcSegSegForceMixed(dlnParams, matParams, network, farDist=0.0, sinMin=0.0, ctx=None):
+    """All-pairs seg-seg force with the mixed-precision rule (no reference
+    counterpart): pairs farther than farDist (default 5 coreRad) with
+    |t1 x t2| > sinMin (default 1e-3) in FP32 by quadrature, others in FP64.
+    Returns (3 x 2 x numSeg forces, pairs on the FP64 path)."""
+    ctx = _ctx(ctx)
+    nv = L.NetView(network)
+    k = L.consts(dlnParams, matParams)
+    n = int(network.numSeg[0])
+    out = np.zeros((3, 2, n), order="F")
+    o = L.ddd_opts(mode=L.DDD_MODE_MIXED, farDist=farDist, sinMin=sinMin)
+    st = L.ddd_stats()
+    _call(ctx, "ddd_segseg_force_opts", k, nv.s, o, L.ptr(out, "d"), n, 0, st)
+    return out, int(st.pairsFP64)
+
+
 def calcSegSegForce_(dlnParams, matParams, network, idx=None, ctx=None):
     """calcSegSegForce! (DislocationForces.jl:563-669): adds into
     network.segForce."""
@@ -200,6 +216,15 @@ class ResidentNetwork:
 
     def seg_force(self, with_pk=False):
         _call(self.ctx, "ddd_net_seg_force", 1 if with_pk else 0)
+
+    def set_mode(self, mode="fp64", farDist=0.0, sinMin=0.0):
+        m = {"fp64": L.DDD_MODE_FP64, "mixed": L.DDD_MODE_MIXED}[mode]
+        _call(self.ctx, "ddd_net_set_opts", L.ddd_opts(mode=m, farDist=farDist, sinMin=sinMin))
+
+    def stats(self):
+        st = L.ddd_stats()
+        _call(self.ctx, "ddd_net_stats", st)
+        return int(st.pairs), int(st.pairsFP64)
 
     def stage(self, stage, with_pk=False):
         _call(self.ctx, "ddd_net_stage", int(stage), 1 if with_pk else 0)

@@ -101,6 +101,23 @@ typedef struct ddd_mesh {
   const double* uHat;          /* 3 numNode, dense (the shim densifies)    */
 } ddd_mesh;
 
+/* Evaluation options (no reference counterpart).  mode DDD_MODE_FP64 is the
+ * reference's arithmetic; DDD_MODE_MIXED evaluates well-separated pairs
+ * (midpoint distance > farDist and |t1 x t2| > sinMin) in FP32 by
+ * Gauss-Legendre quadrature of the non-singular stress, all others in FP64;
+ * DDD_MODE_HYBRID adds a learned far field (see ddd_farfield_*). */
+enum { DDD_MODE_FP64 = 0, DDD_MODE_MIXED = 1, DDD_MODE_HYBRID = 2 };
+typedef struct ddd_opts {
+  int mode;
+  double farDist;  /* mixed: FP32 beyond this midpoint distance (0: 5 coreRad) */
+  double sinMin;   /* mixed: ... and |t1 x t2| above this (0: 1e-3)           */
+} ddd_opts;
+
+typedef struct ddd_stats {
+  uint64_t pairs;      /* unordered pairs evaluated                     */
+  uint64_t pairsFP64;  /* of which on the FP64 path                     */
+} ddd_stats;
+
 int ddd_version(void);
 
 /* Create a context on HIP device `device`; DDD_ERR_NO_DEVICE when none. */
@@ -119,6 +136,12 @@ const char* ddd_last_error(ddd_ctx* ctx);
  * :831-877). */
 int ddd_segseg_force(ddd_ctx* ctx, const ddd_consts* k, const ddd_network* net,
                      const int64_t* idx, int64_t nIdx, double* out, int64_t outCols, int flags);
+
+/* ddd_segseg_force over all pairs with evaluation options (opts NULL: plain
+ * FP64, identical to ddd_segseg_force without idx); stats may be NULL. */
+int ddd_segseg_force_opts(ddd_ctx* ctx, const ddd_consts* k, const ddd_network* net,
+                          const ddd_opts* opts, double* out, int64_t outCols, int flags,
+                          ddd_stats* stats);
 
 /* calcSelfForce / calcSelfForce! (DislocationForces.jl:320-455).  out:
  * 3 x 2 x outCols; node 1 gets -f, node 2 +f. */
@@ -169,6 +192,10 @@ int ddd_net_seg_force(ddd_ctx* ctx, int withPK);
  * 2 = reduction of the tile partials into segForce. */
 int ddd_net_stage(ddd_ctx* ctx, int stage, int withPK);
 int ddd_net_download(ddd_ctx* ctx, double* out, int64_t outCols);
+/* Select the evaluation mode of the resident path (NULL: plain FP64) and
+ * read the pair statistics of the last evaluation (synchronises). */
+int ddd_net_set_opts(ddd_ctx* ctx, const ddd_opts* opts);
+int ddd_net_stats(ddd_ctx* ctx, ddd_stats* stats);
 const double* ddd_net_segforce_device(ddd_ctx* ctx);
 
 #ifdef __cplusplus

@@ -211,6 +211,114 @@ int64_t orc_segseg_par_rows(const double* K, int64_t numSeg, const int64_t* segI
   return pairs;
 }
 
+// Rows-only forces split by distance: for each row i = rows[k]-1 against
+// every j != i (row segment first), the F1/F2 contributions are summed into
+// bin b where edges[b] <= |mid_i - mid_j| < edges[b+1] (edges ascending,
+// nb+1 values).  out: 6 x nb x nRows.  Analysis of how a node's force is
+// distributed over source distance.
+void orc_segseg_rows_binned(const double* K, int64_t numSeg, const int64_t* segIdx, int64_t ld,
+                            const double* coord, const double* bVec, const uint8_t* extSeg,
+                            const int64_t* rows, int64_t nRows, const double* edges, int nb,
+                            double* out, int nthreads) {
+  SegView s{numSeg, segIdx, ld, coord, bVec, extSeg};
+  SegConsts<double> KK = consts_from<double>(K);
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+  for (int64_t k = 0; k < nRows; ++k) {
+    int64_t i = rows[k] - 1;
+    if (s.ext(i)) continue;
+    V3<double> b1 = s.b(i), n11 = s.n1(i), n12 = s.n2(i);
+    V3<double> m1 = (n11 + n12) * 0.5;
+    double* o = out + 6 * nb * k;
+    for (int64_t j = 0; j < numSeg; ++j) {
+      if (j == i || s.ext(j)) continue;
+      V3<double> n21 = s.n1(j), n22 = s.n2(j);
+      double dist = norm((n21 + n22) * 0.5 - m1);
+      int b = 0;
+      while (b < nb && !(dist < edges[b + 1])) ++b;
+      if (b >= nb || dist < edges[0]) continue;
+      Forces4<double> f = seg_seg_force(KK, b1, n11, n12, s.b(j), n21, n22);
+      double* ob = o + 6 * b;
+      ob[0] += f.f1.x; ob[1] += f.f1.y; ob[2] += f.f1.z;
+      ob[3] += f.f2.x; ob[4] += f.f2.y; ob[5] += f.f2.z;
+    }
+  }
+}
+
+// Rows-only forces split by source group: group[j] in [0, ng) names the
+// group of source segment j (negative: skipped).  out: 6 x ng x nRows.
+void orc_segseg_rows_grouped(const double* K, int64_t numSeg, const int64_t* segIdx, int64_t ld,
+                             const double* coord, const double* bVec, const uint8_t* extSeg,
+                             const int64_t* rows, int64_t nRows, const int64_t* group, int64_t ng,
+                             double* out, int nthreads) {
+  SegView s{numSeg, segIdx, ld, coord, bVec, extSeg};
+  SegConsts<double> KK = consts_from<double>(K);
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+  for (int64_t k = 0; k < nRows; ++k) {
+    int64_t i = rows[k] - 1;
+    if (s.ext(i)) continue;
+    V3<double> b1 = s.b(i), n11 = s.n1(i), n12 = s.n2(i);
+    double* o = out + 6 * ng * k;
+    for (int64_t j = 0; j < numSeg; ++j) {
+      if (j == i || s.ext(j) || group[j] < 0) continue;
+      Forces4<double> f = seg_seg_force(KK, b1, n11, n12, s.b(j), s.n1(j), s.n2(j));
+      double* ob = o + 6 * group[j];
+      ob[0] += f.f1.x; ob[1] += f.f1.y; ob[2] += f.f1.z;
+      ob[3] += f.f2.x; ob[4] += f.f2.y; ob[5] += f.f2.z;
+    }
+  }
+}
+
+// Mixed-precision whole-network seg-seg, the rule of the mixed path: pairs
+// with midpoint distance > farDist and |t1 x t2| > sinMin use the far-pair
+// quadrature in precision prec (2 float, 0 double) with nq points; all other
+// pairs the FP64 analytic force.  Accumulation in double.  Returns the number
+// of pairs on the FP64 path.
+int64_t orc_segseg_mixed(int prec, int nq, double farDist, double sinMin, const double* K,
+                         const double* sc, int64_t numSeg, const int64_t* segIdx, int64_t ld,
+                         const double* coord, const double* bVec, const uint8_t* extSeg,
+                         double* segForce, int nthreads) {
+  SegView s{numSeg, segIdx, ld, coord, bVec, extSeg};
+  SegConsts<double> KK = consts_from<double>(K);
+  int64_t exact = 0;
+  std::vector<double> acc(6 * numSeg, 0.0);
+  double* A = acc.data();
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 16) reduction(+ : exact)
+  for (int64_t i = 0; i < numSeg; ++i) {
+    if (s.ext(i)) continue;
+    V3<double> b1 = s.b(i), n11 = s.n1(i), n12 = s.n2(i);
+    V3<double> t1 = (n12 - n11) / norm(n12 - n11), m1 = (n11 + n12) * 0.5;
+    for (int64_t j = i + 1; j < numSeg; ++j) {
+      if (s.ext(j)) continue;
+      V3<double> b2 = s.b(j), n21 = s.n1(j), n22 = s.n2(j);
+      V3<double> t2 = (n22 - n21) / norm(n22 - n21);
+      double dist = norm((n21 + n22) * 0.5 - m1);
+      bool far = dist > farDist && norm(cross(t1, t2)) > sinMin;
+      V3<double> F1, F2, F3, F4;
+      if (far) {
+        if (prec == 2) {
+          far_side_quad<float>(b1, n11, n12, b2, n21, n22, nq, float(sc[0]), float(sc[1]), float(sc[2]), float(sc[3]), float(sc[4]), F1, F2);
+          far_side_quad<float>(b2, n21, n22, b1, n11, n12, nq, float(sc[0]), float(sc[1]), float(sc[2]), float(sc[3]), float(sc[4]), F3, F4);
+        } else {
+          far_side_quad<double>(b1, n11, n12, b2, n21, n22, nq, sc[0], sc[1], sc[2], sc[3], sc[4], F1, F2);
+          far_side_quad<double>(b2, n21, n22, b1, n11, n12, nq, sc[0], sc[1], sc[2], sc[3], sc[4], F3, F4);
+        }
+      } else {
+        Forces4<double> f = seg_seg_force(KK, b1, n11, n12, b2, n21, n22);
+        F1 = f.f1; F2 = f.f2; F3 = f.f3; F4 = f.f4;
+        ++exact;
+      }
+      double* ai = A + 6 * i;
+      double* aj = A + 6 * j;
+      atomic_add(ai + 0, F1.x); atomic_add(ai + 1, F1.y); atomic_add(ai + 2, F1.z);
+      atomic_add(ai + 3, F2.x); atomic_add(ai + 4, F2.y); atomic_add(ai + 5, F2.z);
+      atomic_add(aj + 0, F3.x); atomic_add(aj + 1, F3.y); atomic_add(aj + 2, F3.z);
+      atomic_add(aj + 3, F4.x); atomic_add(aj + 4, F4.y); atomic_add(aj + 5, F4.z);
+    }
+  }
+  for (int64_t i = 0; i < 6 * numSeg; ++i) segForce[i] += A[i];
+  return exact;
+}
+
 // Rows-only seg-seg over a subset (calcSegSegForceIdx!, :831-877).  For each
 // k, row i = idx[k]-1 is computed against every j != i with segment i as the
 // first segment; F1/F2 are added to out column (byPosition ? k : i).

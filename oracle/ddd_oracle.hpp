@@ -668,4 +668,61 @@ void sigma_tilde_point(const V3<T>& b, const V3<T>& n1, const V3<T>& n2, const V
   }
 }
 
+// Far-pair force by quadrature (the mixed-precision path's formulation, no
+// reference counterpart): the nodal forces on segment A from segment B are
+// F_A1 = L_A * sum_q w_q (1 - u_q) (sigma_B(x_A(u_q)) b_A) x t_A and
+// F_A2 = L_A * sum_q w_q u_q (...), with nq-point Gauss-Legendre nodes u_q on
+// [0, 1] and sigma_B the non-singular stress of segment B (sigma_tilde_point).
+// Positions enter as differences taken in double, so T = float only sees
+// lengths of the pair's own scale.
+template <class T> inline void gl_nodes(int nq, T* u, T* w) {
+  if (nq == 1) { u[0] = T(0.5); w[0] = T(1); return; }
+  if (nq == 2) {
+    const double d = 0.5 / std::sqrt(3.0);
+    u[0] = T(0.5 - d); u[1] = T(0.5 + d); w[0] = w[1] = T(0.5); return;
+  }
+  if (nq == 3) {
+    const double d = 0.5 * std::sqrt(0.6);
+    u[0] = T(0.5 - d); u[1] = T(0.5); u[2] = T(0.5 + d);
+    w[0] = w[2] = T(5.0 / 18.0); w[1] = T(8.0 / 18.0); return;
+  }
+  // nq == 4
+  const double a = std::sqrt(3.0 / 7.0 - 2.0 / 7.0 * std::sqrt(6.0 / 5.0));
+  const double b = std::sqrt(3.0 / 7.0 + 2.0 / 7.0 * std::sqrt(6.0 / 5.0));
+  const double wa = (18.0 + std::sqrt(30.0)) / 36.0, wb = (18.0 - std::sqrt(30.0)) / 36.0;
+  u[0] = T(0.5 - 0.5 * b); u[1] = T(0.5 - 0.5 * a); u[2] = T(0.5 + 0.5 * a); u[3] = T(0.5 + 0.5 * b);
+  w[0] = w[3] = T(0.5 * wb); w[1] = w[2] = T(0.5 * wa);
+}
+
+template <class T>
+inline void far_side_quad(const V3<double>& bA, const V3<double>& a1, const V3<double>& a2,
+                          const V3<double>& bB, const V3<double>& p1, const V3<double>& p2,
+                          int nq, T a2c, T mu4pi, T mu4pinu, T nuMu4pinu, T a2mu8pi,
+                          V3<double>& FA1, V3<double>& FA2) {
+  T u[4], w[4];
+  gl_nodes<T>(nq, u, w);
+  const V3<double> dA = a2 - a1, d0 = a1 - p1, dB = p2 - p1;
+  const V3<T> tA = mk(T(dA.x), T(dA.y), T(dA.z));
+  const T LA = norm(tA);
+  const V3<T> tAu = tA / LA;
+  const V3<T> r0 = mk(T(d0.x), T(d0.y), T(d0.z));
+  const V3<T> q2 = mk(T(dB.x), T(dB.y), T(dB.z));
+  const V3<T> b = mk(T(bB.x), T(bB.y), T(bB.z));
+  const V3<T> ba = mk(T(bA.x), T(bA.y), T(bA.z));
+  const V3<T> zero = mk(T(0), T(0), T(0));
+  V3<T> f1 = zero, f2 = zero;
+  for (int q = 0; q < nq; ++q) {
+    V3<T> x = r0 + u[q] * tA;  // point relative to p1
+    T sg[6] = {0, 0, 0, 0, 0, 0};
+    sigma_tilde_point(b, zero, q2, x, a2c, mu4pi, mu4pinu, nuMu4pinu, a2mu8pi, sg);
+    V3<T> sb = mk(sg[0] * ba.x + sg[3] * ba.y + sg[4] * ba.z, sg[3] * ba.x + sg[1] * ba.y + sg[5] * ba.z,
+                  sg[4] * ba.x + sg[5] * ba.y + sg[2] * ba.z);
+    V3<T> f = cross(sb, tAu) * (LA * w[q]);
+    f1 = f1 + f * (T(1) - u[q]);
+    f2 = f2 + f * u[q];
+  }
+  FA1 = mk(double(f1.x), double(f1.y), double(f1.z));
+  FA2 = mk(double(f2.x), double(f2.y), double(f2.z));
+}
+
 }  // namespace ddd_oracle

@@ -52,6 +52,9 @@ def lib():
             "orc_sigma_tilde": (None, [dp, i64, ip, i64, dp, dp, ip, i64, dp, i64, dp, i32]),
             "orc_node_gather": (None, [ip, i64, i64, ip, dp, dp, d, dp]),
             "orc_set_libm": (None, [i32]),
+            "orc_segseg_mixed": (i64, [i32, i32, d, d, dp, dp] + net + [dp, i32]),
+            "orc_segseg_rows_grouped": (None, [dp] + net + [ip, i64, ip, i64, dp, i32]),
+            "orc_segseg_rows_binned": (None, [dp] + net + [ip, i64, dp, i32, dp, i32]),
             "orc_segseg_par_rows": (i64, [dp] + net + [ip, i64, dp, i32]),
             "orc_libm_log": (d, [d]),
             "orc_libm_atan": (d, [d]),
@@ -180,6 +183,51 @@ def calcSegSegForcePar(dln, mat, net, nthreads):
     lib().orc_segseg_par_faithful(_d(K), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord), _d(n.bVec),
                                   _u(n.ext), _d(out), int(nthreads))
     return out
+
+
+def segseg_rows_binned(dln, mat, net, rows, edges, nthreads=None):
+    """Row forces split by source midpoint distance: 6 x nbins x len(rows)
+    (F1 then F2 of each row segment, from sources in each distance bin)."""
+    n = _Net(net)
+    K = seg_consts(dln, mat)
+    r = np.ascontiguousarray(rows, dtype=np.int64)
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    nb = len(e) - 1
+    out = np.zeros((6, nb, len(r)), order="F")
+    lib().orc_segseg_rows_binned(_d(K), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord), _d(n.bVec),
+                                 _u(n.ext), _i(r), len(r), _d(e), nb, _d(out),
+                                 int(nthreads or os.cpu_count() or 1))
+    return out
+
+
+def segseg_rows_grouped(dln, mat, net, rows, group, ngroups, nthreads=None):
+    """Row forces split by source group (group[j] for source j, -1 to skip):
+    6 x ngroups x len(rows)."""
+    n = _Net(net)
+    K = seg_consts(dln, mat)
+    r = np.ascontiguousarray(rows, dtype=np.int64)
+    gr = np.ascontiguousarray(group, dtype=np.int64)
+    out = np.zeros((6, ngroups, len(r)), order="F")
+    lib().orc_segseg_rows_grouped(_d(K), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord), _d(n.bVec),
+                                  _u(n.ext), _i(r), len(r), _i(gr), ngroups, _d(out),
+                                  int(nthreads or os.cpu_count() or 1))
+    return out
+
+
+def calcSegSegForceMixed(dln, mat, net, prec=2, nq=3, farDist=None, sinMin=1e-3,
+                         nthreads=None):
+    """Mixed rule: far (midpoint distance > farDist, default 5 a, and
+    |t1 x t2| > sinMin) pairs by quadrature in float (prec 2) or double (0);
+    others FP64 analytic.  Returns (3 x 2 x numSeg, pairs on FP64)."""
+    n = _Net(net)
+    K = seg_consts(dln, mat)
+    sc = sigma_consts(dln, mat)
+    fd = 5 * dln.coreRad if farDist is None else farDist
+    out = np.zeros((3, 2, n.numSeg), order="F")
+    ex = lib().orc_segseg_mixed(int(prec), int(nq), float(fd), float(sinMin), _d(K), _d(sc),
+                                n.numSeg, _i(n.segIdx), n.ld, _d(n.coord), _d(n.bVec), _u(n.ext),
+                                _d(out), int(nthreads or os.cpu_count() or 1))
+    return out, int(ex)
 
 
 def segseg_par_rows(dln, mat, net, rows, nthreads):
