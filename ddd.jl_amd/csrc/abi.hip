@@ -12,6 +12,7 @@
 
 #include "../../include/ddd_hip.h"
 #include "kernels.hpp"
+#include "farfield.hpp"
 
 using namespace ddd;
 
@@ -54,6 +55,20 @@ struct DevBuf {
 
 }  // namespace
 
+struct ddd_farfield {
+  GridDev g{};
+  int rn = 1;
+  int NP = 0;
+  int64_t vol = 0;
+  bool haveOperator = false;
+  DevBuf What, tw, cellOf, counts, cursor, start, tmp, order, M, Mc, Lc, L, counters;
+  void release() {
+    DevBuf* all[] = {&What, &tw, &cellOf, &counts, &cursor, &start, &tmp, &order, &M, &Mc, &Lc, &L,
+                     &counters};
+    for (DevBuf* b : all) b->release();
+  }
+};
+
 struct ddd_ctx {
   int device = 0;
   hipStream_t own = nullptr;
@@ -71,7 +86,7 @@ struct ddd_ctx {
   std::vector<double> h;  // host staging
   DevBuf counter;  // FP64-pair counter of the mixed path
   // device-resident network
-  ddd_opts opts{DDD_MODE_FP64, 0.0, 0.0};
+  ddd_opts opts{DDD_MODE_FP64, 0.0, 0.0, nullptr};
   int64_t resN = -1;
   int64_t resLd = 0;
   ddd_consts resK{};
@@ -196,9 +211,54 @@ MixedRule mixed_rule(const ddd_consts& k, const ddd_opts& o) {
 
 void check_opts(const ddd_opts* o) {
   if (!o) return;
-  if (o->mode != DDD_MODE_FP64 && o->mode != DDD_MODE_MIXED)
-    bad("mode must be DDD_MODE_FP64 or DDD_MODE_MIXED here");
+  if (o->mode != DDD_MODE_FP64 && o->mode != DDD_MODE_MIXED && o->mode != DDD_MODE_HYBRID)
+    bad("unknown mode");
   if (o->farDist < 0 || o->sinMin < 0) bad("farDist and sinMin must be >= 0");
+  if (o->mode == DDD_MODE_HYBRID) {
+    if (!o->farfield) bad("hybrid mode needs a farfield (ddd_farfield_create)");
+    if (!o->farfield->haveOperator) bad("the farfield has no operator (ddd_farfield_set_operator)");
+  }
+}
+
+bool is_hybrid(const ddd_opts* o) { return o && o->mode == DDD_MODE_HYBRID; }
+
+// Hybrid pair stage: cells, moments, far-field expansions, near field (adds
+// into dev).  The far-field forces are added by hybrid_apply.
+void hybrid_pairs(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsigned* flags,
+                  const ddd_opts& o) {
+  ddd_farfield* ff = o.farfield;
+  const int nc = ff->g.G * ff->g.G * ff->g.G;
+  ff->cellOf.ensure(sizeof(int) * std::max<int64_t>(n, 1));
+  ff->tmp.ensure(sizeof(int) * std::max<int64_t>(n, 1));
+  ff->order.ensure(sizeof(int) * std::max<int64_t>(n, 1));
+  ff->counts.ensure(sizeof(unsigned) * nc);
+  ff->cursor.ensure(sizeof(unsigned) * nc);
+  ff->start.ensure(sizeof(int) * (nc + 1));
+  ff->M.ensure(sizeof(float) * kMoments * nc);
+  ff->L.ensure(sizeof(float) * kLocal * nc);
+  ff->Mc.ensure(sizeof(float2) * kMoments * ff->vol);
+  ff->Lc.ensure(sizeof(float2) * kLocal * ff->vol);
+  ff->counters.ensure(2 * sizeof(unsigned long long));
+  HIPCHK(hipMemsetAsync(ff->counters.p, 0, 2 * sizeof(unsigned long long), c->st));
+  const double* seg = c->seg.as<double>();
+  launch_cells(seg, c->resLd, (int)n, ff->g, ff->cellOf.as<int>(), ff->counts.as<unsigned>(),
+               ff->cursor.as<unsigned>(), ff->start.as<int>(), ff->tmp.as<int>(),
+               ff->order.as<int>(), c->st);
+  launch_moments(seg, c->resLd, ff->g, ff->start.as<int>(), ff->order.as<int>(), ff->M.as<float>(),
+                 c->st);
+  launch_far_field(ff->M.as<float>(), ff->g, ff->What.as<float2>(), ff->tw.as<float2>(),
+                   ff->Mc.as<float2>(), ff->Lc.as<float2>(), ff->L.as<float>(), c->st);
+  const MixedRule mr = mixed_rule(k, o);
+  launch_near_rows(seg, c->resLd, ff->g, ff->rn, ff->start.as<int>(), ff->order.as<int>(),
+                   seg_consts(k), mr, dev, flags, ff->counters.as<unsigned long long>(), c->st);
+  HIPCHK(hipGetLastError());
+}
+
+void hybrid_apply(ddd_ctx* c, int64_t n, double* dev, const ddd_opts& o) {
+  ddd_farfield* ff = o.farfield;
+  launch_far_apply(c->seg.as<double>(), c->resLd, (int)n, ff->g, ff->cellOf.as<int>(),
+                   ff->L.as<float>(), dev, c->st);
+  HIPCHK(hipGetLastError());
 }
 
 // Pair tiles of the whole network (mode from opts) without the reduction.
@@ -228,6 +288,11 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
 void segseg_all(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsigned* flags,
                 const ddd_opts* opts = nullptr) {
   if (n < 2) return;
+  if (is_hybrid(opts)) {
+    hybrid_pairs(c, k, n, dev, flags, *opts);
+    hybrid_apply(c, n, dev, *opts);
+    return;
+  }
   segseg_tiles(c, k, n, flags, opts);
   launch_segseg_reduce(c->rowPart.as<double>(), c->colPart.as<double>(), (int)n, dev, c->st);
   HIPCHK(hipGetLastError());
@@ -587,9 +652,11 @@ void net_stage(ddd_ctx* c, int stage, int withPK) {
     if (withPK) launch_pk_force(c->seg.as<double>(), c->resLd, nullptr, (int)n, c->resM, dev, 1, c->st);
     launch_self_force(c->seg.as<double>(), c->resLd, nullptr, (int)n, self_consts(c->resK), dev, c->st);
   } else if (stage == 1) {
-    segseg_tiles(c, c->resK, n, c->flags.as<unsigned>(), &c->opts);
+    if (is_hybrid(&c->opts)) hybrid_pairs(c, c->resK, n, dev, c->flags.as<unsigned>(), c->opts);
+    else segseg_tiles(c, c->resK, n, c->flags.as<unsigned>(), &c->opts);
   } else if (stage == 2) {
-    if (n >= 2)
+    if (is_hybrid(&c->opts)) hybrid_apply(c, n, dev, c->opts);
+    else if (n >= 2)
       launch_segseg_reduce(c->rowPart.as<double>(), c->colPart.as<double>(), (int)n, dev, c->st);
   } else {
     bad("stage must be 0, 1 or 2");
@@ -621,6 +688,14 @@ void fill_stats(ddd_ctx* c, int64_t n, const ddd_opts* o, ddd_stats* st) {
   if (!st) return;
   st->pairs = n > 1 ? (uint64_t)n * (uint64_t)(n - 1) / 2 : 0;
   st->pairsFP64 = st->pairs;
+  if (is_hybrid(o) && n > 1) {
+    unsigned long long e[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(e, o->farfield->counters.p, sizeof(e), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    st->pairs = e[0];
+    st->pairsFP64 = e[1];
+    return;
+  }
   if (o && o->mode == DDD_MODE_MIXED && n > 1) {
     unsigned long long e = 0;
     HIPCHK(hipMemcpyAsync(&e, c->counter.p, sizeof(e), hipMemcpyDeviceToHost, c->st));
@@ -652,7 +727,7 @@ int ddd_segseg_force_opts(ddd_ctx* c, const ddd_consts* k, const ddd_network* ne
 int ddd_net_set_opts(ddd_ctx* c, const ddd_opts* o) {
   return guard(c, "ddd_net_set_opts", [&] {
     check_opts(o);
-    c->opts = o ? *o : ddd_opts{DDD_MODE_FP64, 0.0, 0.0};
+    c->opts = o ? *o : ddd_opts{DDD_MODE_FP64, 0.0, 0.0, nullptr};
   });
 }
 
@@ -661,6 +736,91 @@ int ddd_net_stats(ddd_ctx* c, ddd_stats* st) {
     if (c->resN < 0) bad("no network uploaded (ddd_net_upload)");
     if (!st) bad("stats is NULL");
     fill_stats(c, c->resN, &c->opts, st);
+  });
+}
+
+int ddd_farfield_create(ddd_ctx* c, int G, const double lo[3], double h, int nearRadius,
+                        ddd_farfield** out) {
+  return guard(c, "ddd_farfield_create", [&] {
+    if (!out || !lo) bad("NULL argument");
+    if (G < 2 || G > 32) bad("G must be in [2, 32]");
+    if (!(h > 0)) bad("h must be positive");
+    if (nearRadius < 0 || nearRadius >= G) bad("nearRadius must be in [0, G)");
+    ddd_farfield* ff = new ddd_farfield();
+    ff->g.G = G;
+    ff->g.h = h;
+    for (int a = 0; a < 3; ++a) ff->g.lo[a] = lo[a];
+    ff->rn = nearRadius;
+    ff->NP = fft_size_for(G);
+    ff->vol = (int64_t)ff->NP * ff->NP * ff->NP;
+    std::vector<float2> tw(ff->NP);
+    for (int k = 0; k < ff->NP; ++k) {
+      const double a = -2.0 * 3.14159265358979323846 * k / ff->NP;
+      tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    try {
+      ff->tw.ensure(sizeof(float2) * ff->NP);
+      HIPCHK(hipMemcpyAsync(ff->tw.p, tw.data(), sizeof(float2) * ff->NP, hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+    } catch (...) {
+      ff->release();
+      delete ff;
+      throw;
+    }
+    *out = ff;
+  });
+}
+
+int ddd_farfield_destroy(ddd_ctx* c, ddd_farfield* ff) {
+  if (!ff) return DDD_OK;
+  return guard(c, "ddd_farfield_destroy", [&] {
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (c->opts.farfield == ff) c->opts = ddd_opts{DDD_MODE_FP64, 0.0, 0.0, nullptr};
+    ff->release();
+    delete ff;
+  });
+}
+
+int ddd_farfield_set_operator(ddd_ctx* c, ddd_farfield* ff, const float* W) {
+  return guard(c, "ddd_farfield_set_operator", [&] {
+    if (!ff || !W) bad("NULL argument");
+    const int S = 2 * ff->g.G - 1;
+    const int64_t nW = (int64_t)S * S * S * kChan;
+    DevBuf dW, scratch;
+    try {
+      dW.ensure(sizeof(float) * nW);
+      scratch.ensure(sizeof(float2) * kChan * ff->vol);
+      ff->What.ensure(sizeof(float2) * kChan * ff->vol);
+      HIPCHK(hipMemcpyAsync(dW.p, W, sizeof(float) * nW, hipMemcpyHostToDevice, c->st));
+      launch_operator_spectrum(dW.as<float>(), ff->g, ff->rn, ff->tw.as<float2>(),
+                               scratch.as<float2>(), ff->What.as<float2>(), c->st);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(c->st));
+    } catch (...) {
+      dW.release();
+      scratch.release();
+      throw;
+    }
+    dW.release();
+    scratch.release();
+    ff->haveOperator = true;
+  });
+}
+
+int ddd_farfield_debug(ddd_ctx* c, ddd_farfield* ff, int what, void* out, int64_t bytes) {
+  return guard(c, "ddd_farfield_debug", [&] {
+    if (!ff || !out) bad("NULL argument");
+    const int64_t nc = (int64_t)ff->g.G * ff->g.G * ff->g.G;
+    const DevBuf* b = nullptr;
+    int64_t need = 0;
+    if (what == 0) { b = &ff->M; need = (int64_t)sizeof(float) * kMoments * nc; }
+    else if (what == 1) { b = &ff->L; need = (int64_t)sizeof(float) * kLocal * nc; }
+    else if (what == 2) { b = &ff->cellOf; need = bytes; }
+    else bad("what must be 0, 1 or 2");
+    if (!b->p) bad("no hybrid evaluation yet");
+    if (bytes < need || (size_t)need > b->bytes) bad("size mismatch");
+    HIPCHK(hipMemcpyAsync(out, b->p, need, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
   });
 }
 

@@ -1,0 +1,573 @@
+// Hybrid force path (no reference counterpart): exact near field on a cell
+// grid plus a far field from per-cell dislocation moments and a learned
+// linear operator applied as a 3-D convolution through FFTs.
+//
+// Cells.  Segments are binned by midpoint into a G^3 grid of edge h starting
+// at lo (clamped at the borders) and ordered by (cell, segment index).
+// Near pairs are those whose cells are within Chebyshev distance rn; they
+// are evaluated exactly (FP64 analytic, or the FP32 far-pair quadrature for
+// well-separated non-parallel pairs).  All other pairs form the far field.
+//
+// Moments.  For cell c with centre x_c, with r = (mid - x_c)/h and
+// l = t L / h (unit tangent times length in cell units), the 90 features are
+//   order 0:  b_i l_j                                   (9)
+//   order 1:  b_i l_j r_k                               (27)
+//   order 2:  b_i l_j (r_k r_l + l_k l_l / 12), k <= l  (54)
+// summed over the cell's segments (the second-order term includes the
+// segment's own extent).
+//
+// Far field.  The stress in target cell t is represented as a quadratic
+// polynomial in rho = (x - x_t)/h with 6 x 10 coefficients
+//   L_t = sum over far source cells s of W(s - t) M_s,   W(o) in R^{60 x 90},
+// a correlation over the grid evaluated by FFT on a (2G)^3 periodic grid,
+// exact for G^3 data with zero padding.  W comes from the far-field model.
+// The nodal force on a segment is the two-point Gauss-Legendre quadrature of
+// (sigma b) x t against its shape functions, as in far_side.
+#include "kernels.hpp"
+#include "farfield.hpp"
+
+namespace ddd {
+
+namespace {
+
+__device__ __forceinline__ int cell_of(const GridDev& g, double x, double y, double z) {
+  int cx = (int)floor((x - g.lo[0]) / g.h), cy = (int)floor((y - g.lo[1]) / g.h),
+      cz = (int)floor((z - g.lo[2]) / g.h);
+  cx = cx < 0 ? 0 : (cx >= g.G ? g.G - 1 : cx);
+  cy = cy < 0 ? 0 : (cy >= g.G ? g.G - 1 : cy);
+  cz = cz < 0 ? 0 : (cz >= g.G ? g.G - 1 : cz);
+  return cx + g.G * (cy + g.G * cz);
+}
+
+__global__ void k_cell_assign(const double* __restrict__ seg, int64_t ld, int n, GridDev g,
+                              int* __restrict__ cellOf, unsigned* __restrict__ counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (seg[F_EXT * ld + i] != 0.0) {
+    cellOf[i] = -1;
+    return;
+  }
+  const double mx = 0.5 * (seg[F_N1X * ld + i] + seg[F_N2X * ld + i]);
+  const double my = 0.5 * (seg[F_N1Y * ld + i] + seg[F_N2Y * ld + i]);
+  const double mz = 0.5 * (seg[F_N1Z * ld + i] + seg[F_N2Z * ld + i]);
+  const int c = cell_of(g, mx, my, mz);
+  cellOf[i] = c;
+  atomicAdd(&counts[c], 1u);
+}
+
+// Exclusive scan of nc counts into start[0..nc] by one workgroup.
+__global__ void k_cell_scan(const unsigned* __restrict__ counts, int nc, int* __restrict__ start) {
+  __shared__ int part[1024];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int per = (nc + nt - 1) / nt;
+  const int b0 = tid * per, b1 = min(nc, b0 + per);
+  int s = 0;
+  for (int c = b0; c < b1; ++c) s += (int)counts[c];
+  part[tid] = s;
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int k = 0; k < nt; ++k) {
+      const int v = part[k];
+      part[k] = acc;
+      acc += v;
+    }
+    start[nc] = acc;
+  }
+  __syncthreads();
+  int acc = part[tid];
+  for (int c = b0; c < b1; ++c) {
+    start[c] = acc;
+    acc += (int)counts[c];
+  }
+}
+
+__global__ void k_cell_scatter(const int* __restrict__ cellOf, int n, const int* __restrict__ start,
+                               unsigned* __restrict__ cursor, int* __restrict__ tmp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || cellOf[i] < 0) return;
+  const int c = cellOf[i];
+  tmp[start[c] + (int)atomicAdd(&cursor[c], 1u)] = i;
+}
+
+// Order each cell's segments by index (the scatter's order is arbitrary):
+// the rank of an entry is the number of smaller indices in its cell.
+__global__ void k_cell_rank(const int* __restrict__ start, const int* __restrict__ tmp,
+                            int* __restrict__ order) {
+  const int c = blockIdx.x;
+  const int s0 = start[c], s1 = start[c + 1];
+  for (int e = s0 + threadIdx.x; e < s1; e += blockDim.x) {
+    const int v = tmp[e];
+    int r = 0;
+    for (int f = s0; f < s1; ++f) r += tmp[f] < v;
+    order[s0 + r] = v;
+  }
+}
+
+// Feature f of the cell moments (see the file comment) for one segment.
+__device__ __forceinline__ double moment_feature(int f, const double b[3], const double l[3],
+                                                 const double r[3]) {
+  if (f < 9) return b[f / 3] * l[f % 3];
+  if (f < 36) {
+    const int g = f - 9;
+    return b[g / 9] * l[(g / 3) % 3] * r[g % 3];
+  }
+  const int g = f - 36;
+  const int KK[6] = {0, 1, 2, 0, 0, 1}, LL[6] = {0, 1, 2, 1, 2, 2};
+  const int k = KK[g % 6], q = LL[g % 6];
+  return b[g / 18] * l[(g / 6) % 3] * (r[k] * r[q] + l[k] * l[q] / 12.0);
+}
+
+// One workgroup per cell, one thread per feature; segments in cell order.
+__global__ void k_cell_moments(const double* __restrict__ seg, int64_t ld, GridDev g,
+                               const int* __restrict__ start, const int* __restrict__ order,
+                               float* __restrict__ M) {
+  const int c = blockIdx.x;
+  const int f = threadIdx.x;
+  if (f >= kMoments) return;
+  const int cx = c % g.G, cy = (c / g.G) % g.G, cz = c / (g.G * g.G);
+  const double xc = g.lo[0] + (cx + 0.5) * g.h, yc = g.lo[1] + (cy + 0.5) * g.h,
+               zc = g.lo[2] + (cz + 0.5) * g.h;
+  const double ih = 1.0 / g.h;
+  double acc = 0.0;
+  for (int e = start[c]; e < start[c + 1]; ++e) {
+    const int i = order[e];
+    const double n1[3] = {seg[F_N1X * ld + i], seg[F_N1Y * ld + i], seg[F_N1Z * ld + i]};
+    const double n2[3] = {seg[F_N2X * ld + i], seg[F_N2Y * ld + i], seg[F_N2Z * ld + i]};
+    const double b[3] = {seg[F_BX * ld + i], seg[F_BY * ld + i], seg[F_BZ * ld + i]};
+    const double l[3] = {(n2[0] - n1[0]) * ih, (n2[1] - n1[1]) * ih, (n2[2] - n1[2]) * ih};
+    const double r[3] = {(0.5 * (n1[0] + n2[0]) - xc) * ih, (0.5 * (n1[1] + n2[1]) - yc) * ih,
+                         (0.5 * (n1[2] + n2[2]) - zc) * ih};
+    acc += moment_feature(f, b, l, r);
+  }
+  M[(int64_t)f * g.G * g.G * g.G + c] = (float)acc;
+}
+
+// ---- 3-D FFT on an NP^3 periodic grid, one line per thread ----------------
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <int NP>
+__global__ void k_fft_lines(float2* __restrict__ data, int64_t nChan, int axis, int inverse,
+                            const float2* __restrict__ tw) {
+  const int64_t line = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)NP * NP;
+  if (line >= nChan * per) return;
+  const int64_t ch = line / per;
+  const int rem = (int)(line % per);
+  const int a = rem % NP, b = rem / NP;
+  int64_t base, stride;
+  // element (x, y, z) of channel ch at ch NP^3 + x + NP (y + NP z)
+  if (axis == 0) { base = (int64_t)NP * (a + (int64_t)NP * b); stride = 1; }
+  else if (axis == 1) { base = a + (int64_t)NP * NP * b; stride = NP; }
+  else { base = a + (int64_t)NP * b; stride = (int64_t)NP * NP; }
+  float2* p = data + ch * per * NP + base;
+  constexpr int LOG = NP == 32 ? 5 : (NP == 64 ? 6 : (NP == 16 ? 4 : 3));
+  float2 v[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    const int r = __brev(k) >> (32 - LOG);
+    v[r] = p[k * stride];
+  }
+#pragma unroll
+  for (int len = 2; len <= NP; len <<= 1) {
+    const int half = len >> 1, step = NP / len;
+#pragma unroll
+    for (int i = 0; i < NP; i += len) {
+#pragma unroll
+      for (int j = 0; j < half; ++j) {
+        float2 w = tw[j * step];
+        if (inverse) w.y = -w.y;
+        const float2 u = v[i + j], t = cmul(w, v[i + j + half]);
+        v[i + j] = make_float2(u.x + t.x, u.y + t.y);
+        v[i + j + half] = make_float2(u.x - t.x, u.y - t.y);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NP; ++k) p[k * stride] = v[k];
+}
+
+// Moments into the zero-padded complex grid.
+__global__ void k_pad_moments(const float* __restrict__ M, int G, int NP, float2* __restrict__ out) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t vol = (int64_t)NP * NP * NP;
+  if (e >= (int64_t)kMoments * vol) return;
+  const int f = (int)(e / vol);
+  const int r = (int)(e % vol);
+  const int x = r % NP, y = (r / NP) % NP, z = r / (NP * NP);
+  float v = 0.f;
+  if (x < G && y < G && z < G) v = M[(int64_t)f * G * G * G + x + G * (y + G * z)];
+  out[e] = make_float2(v, 0.f);
+}
+
+// Lhat(freq)[q] = sum_k What[freq][q][k] Mhat[k](freq).
+__global__ void k_spectral_apply(const float2* __restrict__ What, const float2* __restrict__ Mhat,
+                                 int64_t vol, float2* __restrict__ Lhat) {
+  const int64_t fq = blockIdx.x;
+  const int q = threadIdx.x;
+  __shared__ float2 m[kMoments];
+  for (int k = threadIdx.x; k < kMoments; k += blockDim.x) m[k] = Mhat[(int64_t)k * vol + fq];
+  __syncthreads();
+  if (q >= kLocal) return;
+  const float2* w = What + (fq * kLocal + q) * kMoments;
+  float ax = 0.f, ay = 0.f;
+  for (int k = 0; k < kMoments; ++k) {
+    const float2 a = w[k], b = m[k];
+    ax += a.x * b.x - a.y * b.y;
+    ay += a.x * b.y + a.y * b.x;
+  }
+  Lhat[(int64_t)q * vol + fq] = make_float2(ax, ay);
+}
+
+// Real part of the inverse transform at the G^3 cells, scaled by 1/NP^3.
+__global__ void k_unpad_local(const float2* __restrict__ Lc, int G, int NP, float* __restrict__ L) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t nc = (int64_t)G * G * G;
+  if (e >= (int64_t)kLocal * nc) return;
+  const int q = (int)(e / nc);
+  const int c = (int)(e % nc);
+  const int x = c % G, y = (c / G) % G, z = c / (G * G);
+  const float scale = 1.0f / ((float)NP * NP * NP);
+  L[e] = Lc[(int64_t)q * NP * NP * NP + x + NP * (y + NP * z)].x * scale;
+}
+
+// W(o) (offsets o in [-(G-1), G-1]^3, channel-major [o][60][90]) into the
+// periodic grid at index -o mod NP, channel-major, zero inside the near
+// stencil.
+__global__ void k_place_operator(const float* __restrict__ W, int G, int NP, int rn,
+                                 float2* __restrict__ out) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t vol = (int64_t)NP * NP * NP;
+  if (e >= (int64_t)kChan * vol) return;
+  const int ch = (int)(e / vol);
+  const int r = (int)(e % vol);
+  const int x = r % NP, y = (r / NP) % NP, z = r / (NP * NP);
+  // index x holds offset o = -x (mod NP), folded into (-NP/2, NP/2]
+  auto off = [&](int i) { int o = -i; if (o <= -NP / 2) o += NP; return o; };
+  const int ox = off(x), oy = off(y), oz = off(z);
+  float v = 0.f;
+  const int S = 2 * G - 1;
+  if (abs(ox) <= G - 1 && abs(oy) <= G - 1 && abs(oz) <= G - 1 &&
+      max(abs(ox), max(abs(oy), abs(oz))) > rn) {
+    const int64_t oi = (ox + G - 1) + (int64_t)S * ((oy + G - 1) + (int64_t)S * (oz + G - 1));
+    v = W[oi * kChan + ch];
+  }
+  out[e] = make_float2(v, 0.f);
+}
+
+// [ch][freq] -> [freq][ch]
+__global__ void k_transpose_spectrum(const float2* __restrict__ in, int64_t vol,
+                                     float2* __restrict__ out) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= (int64_t)kChan * vol) return;
+  const int64_t fq = e / kChan;
+  const int ch = (int)(e % kChan);
+  out[e] = in[(int64_t)ch * vol + fq];
+}
+
+__device__ __forceinline__ void local_basis(double x, double y, double z, double P[kPoly]) {
+  P[0] = 1.0; P[1] = x; P[2] = y; P[3] = z;
+  P[4] = x * x; P[5] = y * y; P[6] = z * z; P[7] = x * y; P[8] = x * z; P[9] = y * z;
+}
+
+// Far-field force on every live segment from its cell's local expansion.
+__global__ void k_far_apply(const double* __restrict__ seg, int64_t ld, int n, GridDev g,
+                            const int* __restrict__ cellOf, const float* __restrict__ L,
+                            double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cellOf[i];
+  if (c < 0) return;
+  const int64_t nc = (int64_t)g.G * g.G * g.G;
+  const int cx = c % g.G, cy = (c / g.G) % g.G, cz = c / (g.G * g.G);
+  const double xc = g.lo[0] + (cx + 0.5) * g.h, yc = g.lo[1] + (cy + 0.5) * g.h,
+               zc = g.lo[2] + (cz + 0.5) * g.h;
+  const Seg<double> A = seg_from(seg, ld, i);
+  const double gq[2] = {0.21132486540518711775, 0.78867513459481288225};
+  v3<double> f[2];
+  for (int q = 0; q < 2; ++q) {
+    const double x = A.n1.x + gq[q] * (A.n2.x - A.n1.x);
+    const double y = A.n1.y + gq[q] * (A.n2.y - A.n1.y);
+    const double z = A.n1.z + gq[q] * (A.n2.z - A.n1.z);
+    double P[kPoly];
+    local_basis((x - xc) / g.h, (y - yc) / g.h, (z - zc) / g.h, P);
+    double s[6];
+    for (int comp = 0; comp < 6; ++comp) {
+      double a = 0.0;
+      for (int l = 0; l < kPoly; ++l) a += (double)L[(int64_t)(comp * kPoly + l) * nc + c] * P[l];
+      s[comp] = a;
+    }
+    const v3<double> sb = V(s[0] * A.b.x + s[3] * A.b.y + s[4] * A.b.z,
+                            s[3] * A.b.x + s[1] * A.b.y + s[5] * A.b.z,
+                            s[4] * A.b.x + s[5] * A.b.y + s[2] * A.b.z);
+    f[q] = cross(sb, A.t) * (0.5 / A.tN);
+  }
+  const v3<double> F1 = f[0] * gq[1] + f[1] * gq[0];
+  const v3<double> F2 = f[0] * gq[0] + f[1] * gq[1];
+  double* o = out + 6 * (int64_t)i;
+  o[0] += F1.x; o[1] += F1.y; o[2] += F1.z;
+  o[3] += F2.x; o[4] += F2.y; o[5] += F2.z;
+}
+
+// Near field, rows: one workgroup per target cell.  Lanes hold up to 64
+// target segments; when a cell has fewer, the lanes split into r groups that
+// take every r-th source, and the groups' sums are added in group order.
+// Near-parallel pairs are queued and evaluated 64 at a time.
+constexpr int kNearQ = 128;
+
+__global__ __launch_bounds__(64) void k_near_rows(const double* __restrict__ seg, int64_t ld, GridDev g,
+                                                  int rn, const int* __restrict__ start,
+                                                  const int* __restrict__ order, SegK<double> K,
+                                                  SigK<float> KF, double farDist2, double sinMin2,
+                                                  double* __restrict__ out,
+                                                  unsigned* __restrict__ flags,
+                                                  unsigned long long* __restrict__ counters) {
+  const int T = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int t0 = start[T], nT = start[T + 1] - t0;
+  if (nT == 0) return;
+  const int G = g.G;
+  const int tx = T % G, ty = (T / G) % G, tz = T / (G * G);
+  __shared__ double sj[F_COUNT - 1][64];
+  __shared__ int sjIdx[64];
+  __shared__ double red[6][64];
+  __shared__ int qSrc[kNearQ];
+  __shared__ unsigned char qLane[kNearQ];
+  __shared__ double qRes[6][64];
+  unsigned long long nPairs = 0, nExact = 0;
+
+  for (int base = 0; base < nT; base += 64) {
+    const int cnt = min(64, nT - base);
+    int P2 = 1;
+    while (P2 < cnt) P2 <<= 1;
+    const int r = 64 / P2, grp = lane / P2, ti = lane % P2;
+    const bool valid = ti < cnt;
+    const int i = valid ? order[t0 + base + ti] : -1;
+    const Seg<double> A = valid ? seg_from(seg, ld, i) : seg_from(seg, ld, order[t0]);
+    const v3<double> midA = V(0.5 * (A.n1.x + A.n2.x), 0.5 * (A.n1.y + A.n2.y), 0.5 * (A.n1.z + A.n2.z));
+    v3<double> a1 = V(0.0, 0.0, 0.0), a2 = V(0.0, 0.0, 0.0);
+    int nq = 0;
+    int q = 0;  // running source counter (uniform)
+    const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    auto flush = [&](int cntq) {
+      if (lane < cntq) {
+        const int ln = qLane[lane];
+        const int srcI = qSrc[lane];
+        const Seg<double> TA = seg_from(seg, ld, order[t0 + base + (ln % P2)]);
+        const Seg<double> SB = seg_from(seg, ld, srcI);
+        v3<double> F1, F2, F3, F4;
+        special_pair_dev(K, TA, SB, F1, F2, F3, F4, flags);
+        qRes[0][lane] = F1.x; qRes[1][lane] = F1.y; qRes[2][lane] = F1.z;
+        qRes[3][lane] = F2.x; qRes[4][lane] = F2.y; qRes[5][lane] = F2.z;
+      }
+      __syncthreads();
+      for (int e = 0; e < cntq; ++e) {
+        if (qLane[e] == lane) {
+          a1 = a1 + V(qRes[0][e], qRes[1][e], qRes[2][e]);
+          a2 = a2 + V(qRes[3][e], qRes[4][e], qRes[5][e]);
+        }
+      }
+      __syncthreads();
+    };
+
+    for (int dz = -rn; dz <= rn; ++dz) {
+      const int sz = tz + dz;
+      if (sz < 0 || sz >= G) continue;
+      for (int dy = -rn; dy <= rn; ++dy) {
+        const int sy = ty + dy;
+        if (sy < 0 || sy >= G) continue;
+        for (int dx = -rn; dx <= rn; ++dx) {
+          const int sx = tx + dx;
+          if (sx < 0 || sx >= G) continue;
+          const int S = sx + G * (sy + G * sz);
+          const int s0 = start[S], ns = start[S + 1] - s0;
+          for (int sb = 0; sb < ns; sb += 64) {
+            const int m = min(64, ns - sb);
+            __syncthreads();
+            if (lane < m) {
+              const int j = order[s0 + sb + lane];
+              sjIdx[lane] = j;
+#pragma unroll
+              for (int f = 0; f < F_COUNT - 1; ++f) sj[f][lane] = seg[f * ld + j];
+            }
+            __syncthreads();
+            for (int e = 0; e < m; ++e) {
+              const bool mine = valid && ((q + e) % r == grp) && sjIdx[e] != i;
+              Seg<double> B;
+              B.n1 = V(sj[F_N1X][e], sj[F_N1Y][e], sj[F_N1Z][e]);
+              B.n2 = V(sj[F_N2X][e], sj[F_N2Y][e], sj[F_N2Z][e]);
+              B.b = V(sj[F_BX][e], sj[F_BY][e], sj[F_BZ][e]);
+              B.t = V(sj[F_TX][e], sj[F_TY][e], sj[F_TZ][e]);
+              B.tN = sj[F_TN][e];
+              double c;
+              const double omc = one_minus_c2(A.t, B.t, c);
+              const double dx2 = 0.5 * (B.n1.x + B.n2.x) - midA.x, dy2 = 0.5 * (B.n1.y + B.n2.y) - midA.y,
+                           dz2 = 0.5 * (B.n1.z + B.n2.z) - midA.z;
+              const bool far = (dx2 * dx2 + dy2 * dy2 + dz2 * dz2 > farDist2) && (omc > sinMin2);
+              const bool special = mine && !far && !(omc > DDD_SQRT_EPS);
+              nPairs += mine;
+              nExact += mine && !far;
+              if (mine && far) {
+                v3<double> F1, F2;
+                far_side<float>(KF, A, B, F1, F2);
+                a1 = a1 + F1;
+                a2 = a2 + F2;
+              } else if (mine && !special) {
+                v3<double> F1, F2, F3, F4;
+                pair_generic<double, true, false>(K, A, B, c, omc, F1, F2, F3, F4);
+                a1 = a1 + F1;
+                a2 = a2 + F2;
+              }
+              const uint64_t mk = __ballot(special);
+              if (mk) {
+                if (special) {
+                  const int pos = nq + __popcll(mk & ltMask);
+                  qSrc[pos] = sjIdx[e];
+                  qLane[pos] = (unsigned char)lane;
+                }
+                nq += __popcll(mk);
+              }
+              if (nq >= 64) {
+                __syncthreads();
+                flush(64);
+                const int rest = nq - 64;
+                int ks = 0;
+                unsigned char kl = 0;
+                if (lane < rest) { ks = qSrc[64 + lane]; kl = qLane[64 + lane]; }
+                __syncthreads();
+                if (lane < rest) { qSrc[lane] = ks; qLane[lane] = kl; }
+                __syncthreads();
+                nq = rest;
+              }
+            }
+            q += m;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (nq > 0) flush(nq);
+    // Group sums, in group order, then one write per target.
+    for (int gi = 0; gi < r; ++gi) {
+      if (grp == gi) {
+        if (gi == 0) {
+          red[0][ti] = a1.x; red[1][ti] = a1.y; red[2][ti] = a1.z;
+          red[3][ti] = a2.x; red[4][ti] = a2.y; red[5][ti] = a2.z;
+        } else {
+          red[0][ti] += a1.x; red[1][ti] += a1.y; red[2][ti] += a1.z;
+          red[3][ti] += a2.x; red[4][ti] += a2.y; red[5][ti] += a2.z;
+        }
+      }
+      __syncthreads();
+    }
+    if (grp == 0 && valid) {
+      double* o = out + 6 * (int64_t)i;
+#pragma unroll
+      for (int cc = 0; cc < 6; ++cc) o[cc] += red[cc][ti];
+    }
+    __syncthreads();
+  }
+  // pair statistics (atomics on integer counters only)
+  for (int off = 32; off >= 1; off >>= 1) {
+    nPairs += __shfl_xor(nPairs, off, 64);
+    nExact += __shfl_xor(nExact, off, 64);
+  }
+  if (lane == 0) {
+    atomicAdd(&counters[0], nPairs);
+    atomicAdd(&counters[1], nExact);
+  }
+}
+
+}  // namespace
+
+int fft_size_for(int G) {
+  int np = 8;
+  while (np < 2 * G) np <<= 1;
+  return np;
+}
+
+static void fft3(float2* data, int64_t nChan, int NP, int inverse, const float2* tw, hipStream_t st) {
+  const int64_t lines = nChan * (int64_t)NP * NP;
+  const int bs = 128;
+  const unsigned grid = (unsigned)((lines + bs - 1) / bs);
+  for (int axis = 0; axis < 3; ++axis) {
+    if (NP == 32)
+      hipLaunchKernelGGL(k_fft_lines<32>, dim3(grid), dim3(bs), 0, st, data, nChan, axis, inverse, tw);
+    else if (NP == 64)
+      hipLaunchKernelGGL(k_fft_lines<64>, dim3(grid), dim3(bs), 0, st, data, nChan, axis, inverse, tw);
+    else if (NP == 16)
+      hipLaunchKernelGGL(k_fft_lines<16>, dim3(grid), dim3(bs), 0, st, data, nChan, axis, inverse, tw);
+    else
+      hipLaunchKernelGGL(k_fft_lines<8>, dim3(grid), dim3(bs), 0, st, data, nChan, axis, inverse, tw);
+  }
+}
+
+void launch_cells(const double* seg, int64_t ld, int n, const GridDev& g, int* cellOf,
+                  unsigned* counts, unsigned* cursor, int* start, int* tmp, int* order,
+                  hipStream_t st) {
+  const int nc = g.G * g.G * g.G;
+  (void)hipMemsetAsync(counts, 0, sizeof(unsigned) * nc, st);
+  (void)hipMemsetAsync(cursor, 0, sizeof(unsigned) * nc, st);
+  if (n > 0)
+    hipLaunchKernelGGL(k_cell_assign, dim3((n + 255) / 256), dim3(256), 0, st, seg, ld, n, g, cellOf,
+                       counts);
+  hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, st, counts, nc, start);
+  if (n > 0)
+    hipLaunchKernelGGL(k_cell_scatter, dim3((n + 255) / 256), dim3(256), 0, st, cellOf, n, start,
+                       cursor, tmp);
+  hipLaunchKernelGGL(k_cell_rank, dim3(nc), dim3(64), 0, st, start, tmp, order);
+}
+
+void launch_moments(const double* seg, int64_t ld, const GridDev& g, const int* start,
+                    const int* order, float* M, hipStream_t st) {
+  const int nc = g.G * g.G * g.G;
+  hipLaunchKernelGGL(k_cell_moments, dim3(nc), dim3(128), 0, st, seg, ld, g, start, order, M);
+}
+
+void launch_far_field(const float* M, const GridDev& g, const float2* What, const float2* tw,
+                      float2* Mc, float2* Lc, float* L, hipStream_t st) {
+  const int NP = fft_size_for(g.G);
+  const int64_t vol = (int64_t)NP * NP * NP;
+  const int bs = 256;
+  hipLaunchKernelGGL(k_pad_moments, dim3((unsigned)((kMoments * vol + bs - 1) / bs)), dim3(bs), 0, st,
+                     M, g.G, NP, Mc);
+  fft3(Mc, kMoments, NP, 0, tw, st);
+  hipLaunchKernelGGL(k_spectral_apply, dim3((unsigned)vol), dim3(64), 0, st, What, Mc, vol, Lc);
+  fft3(Lc, kLocal, NP, 1, tw, st);
+  const int64_t nc = (int64_t)g.G * g.G * g.G;
+  hipLaunchKernelGGL(k_unpad_local, dim3((unsigned)((kLocal * nc + bs - 1) / bs)), dim3(bs), 0, st,
+                     Lc, g.G, NP, L);
+}
+
+void launch_operator_spectrum(const float* W, const GridDev& g, int rn, const float2* tw,
+                              float2* scratch, float2* What, hipStream_t st) {
+  const int NP = fft_size_for(g.G);
+  const int64_t vol = (int64_t)NP * NP * NP;
+  const int bs = 256;
+  hipLaunchKernelGGL(k_place_operator, dim3((unsigned)((kChan * vol + bs - 1) / bs)), dim3(bs), 0, st,
+                     W, g.G, NP, rn, scratch);
+  fft3(scratch, kChan, NP, 0, tw, st);
+  hipLaunchKernelGGL(k_transpose_spectrum, dim3((unsigned)((kChan * vol + bs - 1) / bs)), dim3(bs), 0,
+                     st, scratch, vol, What);
+}
+
+void launch_far_apply(const double* seg, int64_t ld, int n, const GridDev& g, const int* cellOf,
+                      const float* L, double* out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_far_apply, dim3((n + 255) / 256), dim3(256), 0, st, seg, ld, n, g, cellOf, L,
+                     out);
+}
+
+void launch_near_rows(const double* seg, int64_t ld, const GridDev& g, int rn, const int* start,
+                      const int* order, const SegK<double>& K, const MixedRule& mr, double* out,
+                      unsigned* flags, unsigned long long* counters, hipStream_t st) {
+  const int nc = g.G * g.G * g.G;
+  hipLaunchKernelGGL(k_near_rows, dim3(nc), dim3(64), 0, st, seg, ld, g, rn, start, order, K, mr.K,
+                     mr.farDist * mr.farDist, mr.sinMin * mr.sinMin, out, flags, counters);
+}
+
+}  // namespace ddd

@@ -48,7 +48,8 @@ DDD_MODE_FP64, DDD_MODE_MIXED, DDD_MODE_HYBRID = 0, 1, 2
 
 
 class ddd_opts(C.Structure):
-    _fields_ = [("mode", C.c_int), ("farDist", C.c_double), ("sinMin", C.c_double)]
+    _fields_ = [("mode", C.c_int), ("farDist", C.c_double), ("sinMin", C.c_double),
+                ("farfield", C.c_void_p)]
 
 
 class ddd_stats(C.Structure):
@@ -98,6 +99,11 @@ def lib():
                 "ddd_net_seg_force": (C.c_int, [P, C.c_int]),
                 "ddd_net_stage": (C.c_int, [P, C.c_int, C.c_int]),
                 "ddd_net_set_opts": (C.c_int, [P, C.POINTER(ddd_opts)]),
+                "ddd_farfield_create": (C.c_int, [P, C.c_int, dp, C.c_double, C.c_int,
+                                                  C.POINTER(P)]),
+                "ddd_farfield_destroy": (C.c_int, [P, P]),
+                "ddd_farfield_set_operator": (C.c_int, [P, P, C.POINTER(C.c_float)]),
+                "ddd_farfield_debug": (C.c_int, [P, P, C.c_int, P, i64]),
                 "ddd_net_stats": (C.c_int, [P, C.POINTER(ddd_stats)]),
                 "ddd_segseg_force_opts": (C.c_int, [P, C.POINTER(ddd_consts),
                                                     C.POINTER(ddd_network), C.POINTER(ddd_opts),

@@ -10,9 +10,14 @@ Every function calls libdddhip.so; there is no CPU path behind them.
 """
 from __future__ import annotations
 
+import ctypes as _c
+
 import numpy as np
 
 from . import _lib as L
+
+C_void = _c.c_void_p
+_byref = _c.byref
 
 
 def _idx(idx):
@@ -201,6 +206,67 @@ def calcNodeForce_(dlnParams, matParams, network, ctx=None):
     return nf
 
 
+class FarField:
+    """Hybrid far field (ddd_farfield_*): a G^3 cell grid of edge h from
+    corner lo, near radius rn (in cells), and the operator W
+    ((2G-1)^3 x 60 x 90) mapping source-cell moments to target-cell stress
+    expansions."""
+
+    def __init__(self, G, lo, h, nearRadius, ctx=None):
+        self.ctx = _ctx(ctx)
+        self.G, self.h, self.rn = int(G), float(h), int(nearRadius)
+        self.lo = np.ascontiguousarray(lo, dtype=np.float64)
+        self.h_ptr = C_void()
+        _call(self.ctx, "ddd_farfield_create", self.G, L.ptr(self.lo, "d"), self.h, self.rn,
+              _byref(self.h_ptr))
+
+    def set_operator(self, W):
+        S = 2 * self.G - 1
+        W = np.ascontiguousarray(W, dtype=np.float32).reshape(S ** 3, 60, 90)
+        _call(self.ctx, "ddd_farfield_set_operator", self.h_ptr,
+              W.ctypes.data_as(_c.POINTER(_c.c_float)))
+
+    def debug(self, what, numSeg=0):
+        nc = self.G ** 3
+        if what == 0:
+            out = np.zeros((90, nc), dtype=np.float32)
+        elif what == 1:
+            out = np.zeros((60, nc), dtype=np.float32)
+        else:
+            out = np.zeros(numSeg, dtype=np.int32)
+        _call(self.ctx, "ddd_farfield_debug", self.h_ptr, int(what),
+              out.ctypes.data_as(_c.c_void_p), out.nbytes)
+        return out
+
+    def close(self):
+        if self.h_ptr:
+            L.lib().ddd_farfield_destroy(self.ctx.h, self.h_ptr)
+            self.h_ptr = C_void()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def calcSegSegForceHybrid(dlnParams, matParams, network, farfield, farDist=0.0, sinMin=0.0,
+                          ctx=None):
+    """All-pairs seg-seg force, hybrid: near pairs exact (mixed rule), far
+    pairs through the far-field operator.  Returns (3 x 2 x numSeg forces,
+    ordered near pairs, of which FP64)."""
+    ctx = _ctx(ctx)
+    nv = L.NetView(network)
+    k = L.consts(dlnParams, matParams)
+    n = int(network.numSeg[0])
+    out = np.zeros((3, 2, n), order="F")
+    o = L.ddd_opts(mode=L.DDD_MODE_HYBRID, farDist=farDist, sinMin=sinMin,
+                   farfield=farfield.h_ptr)
+    st = L.ddd_stats()
+    _call(ctx, "ddd_segseg_force_opts", k, nv.s, o, L.ptr(out, "d"), n, 0, st)
+    return out, int(st.pairs), int(st.pairsFP64)
+
+
 class ResidentNetwork:
     """Device-resident evaluation (ddd_net_*): upload once, evaluate many
     times.  Used by time-stepping callers and the benchmark."""
@@ -217,9 +283,11 @@ class ResidentNetwork:
     def seg_force(self, with_pk=False):
         _call(self.ctx, "ddd_net_seg_force", 1 if with_pk else 0)
 
-    def set_mode(self, mode="fp64", farDist=0.0, sinMin=0.0):
-        m = {"fp64": L.DDD_MODE_FP64, "mixed": L.DDD_MODE_MIXED}[mode]
-        _call(self.ctx, "ddd_net_set_opts", L.ddd_opts(mode=m, farDist=farDist, sinMin=sinMin))
+    def set_mode(self, mode="fp64", farDist=0.0, sinMin=0.0, farfield=None):
+        m = {"fp64": L.DDD_MODE_FP64, "mixed": L.DDD_MODE_MIXED, "hybrid": L.DDD_MODE_HYBRID}[mode]
+        _call(self.ctx, "ddd_net_set_opts",
+              L.ddd_opts(mode=m, farDist=farDist, sinMin=sinMin,
+                         farfield=farfield.h_ptr if farfield is not None else None))
 
     def stats(self):
         st = L.ddd_stats()

@@ -107,16 +107,35 @@ typedef struct ddd_mesh {
  * Gauss-Legendre quadrature of the non-singular stress, all others in FP64;
  * DDD_MODE_HYBRID adds a learned far field (see ddd_farfield_*). */
 enum { DDD_MODE_FP64 = 0, DDD_MODE_MIXED = 1, DDD_MODE_HYBRID = 2 };
+typedef struct ddd_farfield ddd_farfield;
 typedef struct ddd_opts {
   int mode;
-  double farDist;  /* mixed: FP32 beyond this midpoint distance (0: 5 coreRad) */
-  double sinMin;   /* mixed: ... and |t1 x t2| above this (0: 1e-3)           */
+  double farDist;  /* mixed/hybrid: FP32 beyond this midpoint distance (0: 5 coreRad) */
+  double sinMin;   /* mixed/hybrid: ... and |t1 x t2| above this (0: 1e-3)           */
+  ddd_farfield* farfield; /* hybrid: cell grid and far-field operator              */
 } ddd_opts;
 
 typedef struct ddd_stats {
-  uint64_t pairs;      /* unordered pairs evaluated                     */
-  uint64_t pairsFP64;  /* of which on the FP64 path                     */
+  uint64_t pairs;      /* pairs evaluated exactly (hybrid: ordered near pairs) */
+  uint64_t pairsFP64;  /* of which on the FP64 path                            */
 } ddd_stats;
+
+/* Hybrid far field (no reference counterpart).  A G^3 grid of cubic cells
+ * of edge h from corner lo; segments belong to the cell of their midpoint
+ * (clamped).  Pairs whose cells are within Chebyshev distance nearRadius are
+ * evaluated exactly (mixed rule); all other pairs through 90 cell moments and
+ * the operator W: for every cell offset o in [-(G-1), G-1]^3 (x fastest), a
+ * 60 x 90 row-major matrix mapping source moments to the 6 x 10 quadratic
+ * stress expansion of the target cell.  ddd_farfield_set_operator takes W as
+ * (2G-1)^3 x 60 x 90 floats and precomputes its spectrum on the device. */
+int ddd_farfield_create(ddd_ctx* ctx, int G, const double lo[3], double h, int nearRadius,
+                        ddd_farfield** out);
+int ddd_farfield_destroy(ddd_ctx* ctx, ddd_farfield* ff);
+int ddd_farfield_set_operator(ddd_ctx* ctx, ddd_farfield* ff, const float* W);
+/* Intermediate results of the last hybrid evaluation, for tests:
+ * what 0: cell moments, 90 x G^3 floats; 1: local expansions, 60 x G^3
+ * floats; 2: cell of each segment, numSeg int32 (-1 external). */
+int ddd_farfield_debug(ddd_ctx* ctx, ddd_farfield* ff, int what, void* out, int64_t bytes);
 
 int ddd_version(void);
 

@@ -319,6 +319,33 @@ int64_t orc_segseg_mixed(int prec, int nq, double farDist, double sinMin, const 
   return exact;
 }
 
+// Near-field rows of the hybrid split: for every live segment i, the F1/F2
+// of every pair (i, j), j != i, whose cells (cellOf, x + G (y + G z)) are
+// within Chebyshev distance rn, segment i first, in FP64; added to out.
+void orc_segseg_near(const double* K, int64_t numSeg, const int64_t* segIdx, int64_t ld,
+                     const double* coord, const double* bVec, const uint8_t* extSeg,
+                     const int64_t* cellOf, int G, int rn, double* out, int nthreads) {
+  SegView s{numSeg, segIdx, ld, coord, bVec, extSeg};
+  SegConsts<double> KK = consts_from<double>(K);
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 16)
+  for (int64_t i = 0; i < numSeg; ++i) {
+    if (s.ext(i) || cellOf[i] < 0) continue;
+    const int64_t ci = cellOf[i];
+    const int ix = int(ci % G), iy = int((ci / G) % G), iz = int(ci / (int64_t(G) * G));
+    V3<double> b1 = s.b(i), n11 = s.n1(i), n12 = s.n2(i);
+    double* o = out + 6 * i;
+    for (int64_t j = 0; j < numSeg; ++j) {
+      if (j == i || s.ext(j) || cellOf[j] < 0) continue;
+      const int64_t cj = cellOf[j];
+      const int jx = int(cj % G), jy = int((cj / G) % G), jz = int(cj / (int64_t(G) * G));
+      if (std::abs(jx - ix) > rn || std::abs(jy - iy) > rn || std::abs(jz - iz) > rn) continue;
+      Forces4<double> f = seg_seg_force(KK, b1, n11, n12, s.b(j), s.n1(j), s.n2(j));
+      add3(o, f.f1);
+      add3(o + 3, f.f2);
+    }
+  }
+}
+
 // Rows-only seg-seg over a subset (calcSegSegForceIdx!, :831-877).  For each
 // k, row i = idx[k]-1 is computed against every j != i with segment i as the
 // first segment; F1/F2 are added to out column (byPosition ? k : i).

@@ -52,6 +52,7 @@ def lib():
             "orc_sigma_tilde": (None, [dp, i64, ip, i64, dp, dp, ip, i64, dp, i64, dp, i32]),
             "orc_node_gather": (None, [ip, i64, i64, ip, dp, dp, d, dp]),
             "orc_set_libm": (None, [i32]),
+            "orc_segseg_near": (None, [dp] + net + [ip, i32, i32, dp, i32]),
             "orc_segseg_mixed": (i64, [i32, i32, d, d, dp, dp] + net + [dp, i32]),
             "orc_segseg_rows_grouped": (None, [dp] + net + [ip, i64, ip, i64, dp, i32]),
             "orc_segseg_rows_binned": (None, [dp] + net + [ip, i64, dp, i32, dp, i32]),
@@ -228,6 +229,18 @@ def calcSegSegForceMixed(dln, mat, net, prec=2, nq=3, farDist=None, sinMin=1e-3,
                                 n.numSeg, _i(n.segIdx), n.ld, _d(n.coord), _d(n.bVec), _u(n.ext),
                                 _d(out), int(nthreads or os.cpu_count() or 1))
     return out, int(ex)
+
+
+def segseg_near(dln, mat, net, cellOf, G, rn, nthreads=None):
+    """Near-field rows of the hybrid split (pairs whose cells are within
+    Chebyshev distance rn, row segment first): 3 x 2 x numSeg."""
+    n = _Net(net)
+    K = seg_consts(dln, mat)
+    co = np.ascontiguousarray(cellOf, dtype=np.int64)
+    out = np.zeros((3, 2, n.numSeg), order="F")
+    lib().orc_segseg_near(_d(K), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord), _d(n.bVec), _u(n.ext),
+                          _i(co), int(G), int(rn), _d(out), int(nthreads or os.cpu_count() or 1))
+    return out
 
 
 def segseg_par_rows(dln, mat, net, rows, nthreads):

@@ -1,0 +1,79 @@
+"""GPU: the hybrid path's pieces against restatements.
+
+* near field (all FP64) == oracle near-field rows to <= 1e-10 per node;
+  with the mixed rule, <= 2e-5;
+* cell binning identical, moments to FP32 precision, the FFT correlation
+  against a direct sum, and the far-field quadrature, with a random operator.
+"""
+import numpy as np
+import pytest
+
+import hybrid_ref as H
+import pyoracle as O
+import refnets as R
+from dddhip import forces as G
+from dddhip.generate import random_loop_network
+from metrics import per_node_rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def prm():
+    return R.params()
+
+
+def _self(dln, mat, net):
+    s1, s2 = O.calcSelfForce(dln, mat, net)
+    out = np.zeros((3, 2, s1.shape[1]), order="F")
+    out[:, 0] = s1
+    out[:, 1] = s2
+    return out
+
+
+@pytest.mark.parametrize("Gc,rn", [(8, 1), (4, 0)])
+def test_near_field_exact(prm, Gc, rn):
+    dln, mat = prm
+    net = random_loop_network(96, seed=0xD5)
+    h = 2000.0 / Gc
+    ff = G.FarField(Gc, (0.0, 0.0, 0.0), h, rn)
+    S = 2 * Gc - 1
+    ff.set_operator(np.zeros((S ** 3, 60, 90), np.float32))
+    g, pairs, exact = G.calcSegSegForceHybrid(dln, mat, net, ff, farDist=1e30)
+    n = int(net.numSeg[0])
+    cell = H.bin_cells(net, Gc, (0, 0, 0), h)
+    assert np.array_equal(ff.debug(2, n), cell.astype(np.int32))
+    o = O.segseg_near(dln, mat, net, cell, Gc, rn)
+    mx, rms = per_node_rel(g, o)
+    print(f"near G={Gc} rn={rn}: pairs {pairs} exact {exact}; max {mx:.2e} rms {rms:.2e}", flush=True)
+    assert pairs == exact and mx <= 1e-10
+    # with the mixed rule the far part of the stencil runs in FP32
+    gm, pm, em = G.calcSegSegForceHybrid(dln, mat, net, ff)
+    assert pm == pairs and em <= pm
+    tot = o + _self(dln, mat, net)
+    assert per_node_rel(gm + _self(dln, mat, net), tot)[0] <= 2e-5
+
+
+def test_far_field_pipeline(prm):
+    dln, mat = prm
+    Gc, rn = 4, 1
+    h = 500.0
+    net = random_loop_network(24, seed=0xD6)
+    ff = G.FarField(Gc, (0.0, 0.0, 0.0), h, rn)
+    S = 2 * Gc - 1
+    rng = np.random.default_rng(4)
+    W = (rng.standard_normal((S ** 3, 60, 90)) * 1e-3).astype(np.float32)
+    ff.set_operator(W)
+    g, _, _ = G.calcSegSegForceHybrid(dln, mat, net, ff, farDist=1e30)
+    M = ff.debug(0)
+    Mr = H.moments(net, Gc, (0, 0, 0), h)
+    assert np.abs(M - Mr).max() <= 1e-5 * np.abs(Mr).max()
+    L = ff.debug(1)
+    Lr = H.local_expansions(Mr, W.astype(np.float64), Gc, rn)
+    assert np.abs(L - Lr).max() <= 1e-4 * np.abs(Lr).max()
+    # far forces = hybrid - near
+    cell = H.bin_cells(net, Gc, (0, 0, 0), h)
+    near = O.segseg_near(dln, mat, net, cell, Gc, rn)
+    far = H.far_forces(net, Lr, Gc, (0, 0, 0), h)
+    diff = g - near
+    assert np.abs(diff - far).max() <= 1e-4 * np.abs(far).max()
