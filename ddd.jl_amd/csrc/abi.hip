@@ -13,6 +13,11 @@
 #include "../../include/ddd_hip.h"
 #include "kernels.hpp"
 #include "farfield.hpp"
+#include "mlp.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <random>
 
 using namespace ddd;
 
@@ -53,6 +58,14 @@ struct DevBuf {
   }
 };
 
+// A device buffer freed when it leaves scope (temporaries of one call).
+struct ScopedBuf : DevBuf {
+  ScopedBuf() = default;
+  ScopedBuf(const ScopedBuf&) = delete;
+  ScopedBuf& operator=(const ScopedBuf&) = delete;
+  ~ScopedBuf() { release(); }
+};
+
 }  // namespace
 
 struct ddd_farfield {
@@ -66,6 +79,31 @@ struct ddd_farfield {
     DevBuf* all[] = {&What, &tw, &cellOf, &counts, &cursor, &start, &tmp, &order, &M, &Mc, &Lc, &L,
                      &counters};
     for (DevBuf* b : all) b->release();
+  }
+};
+
+struct ddd_mlp {
+  DevBuf P, G, Mm, Vv, Wb;
+  int64_t step = 0;
+  int cap = 0;
+  DevBuf X0, Z1, Z2, Z3, H1, H2, H3, Y, dY, dYb, dH, dZb, lossPer, off3;
+  void ensure_batch(int ns) {
+    if (ns <= cap) return;
+    X0.ensure(sizeof(unsigned short) * ns * kMlpIn);
+    for (DevBuf* b : {&Z1, &Z2, &Z3, &dH}) b->ensure(sizeof(float) * ns * kHidden);
+    for (DevBuf* b : {&H1, &H2, &H3, &dZb}) b->ensure(sizeof(unsigned short) * ns * kHidden);
+    Y.ensure(sizeof(float) * (int64_t)ns * kMlpOutPad);
+    dY.ensure(sizeof(float) * (int64_t)ns * kMlpOutPad);
+    dYb.ensure(sizeof(unsigned short) * (int64_t)ns * kMlpOutPad);
+    lossPer.ensure(sizeof(float) * ns);
+    off3.ensure(sizeof(int) * 3 * ns);
+    cap = ns;
+  }
+  void release() {
+    DevBuf* all[] = {&P, &G, &Mm, &Vv, &Wb, &X0, &Z1, &Z2, &Z3, &H1, &H2, &H3, &Y, &dY, &dYb,
+                     &dH, &dZb, &lossPer, &off3};
+    for (DevBuf* b : all) b->release();
+    cap = 0;
   }
 };
 
@@ -821,6 +859,404 @@ int ddd_farfield_debug(ddd_ctx* c, ddd_farfield* ff, int what, void* out, int64_
     if (bytes < need || (size_t)need > b->bytes) bad("size mismatch");
     HIPCHK(hipMemcpyAsync(out, b->p, need, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+namespace {
+
+const unsigned short* wb(ddd_mlp* m, int64_t off) { return m->Wb.as<unsigned short>() + off; }
+const float* pp(ddd_mlp* m, int64_t off) { return m->P.as<float>() + off; }
+
+// Y = MLP(features of off3) for ns samples (off3 already on the device).
+void mlp_forward(ddd_mlp* m, int ns, hipStream_t st) {
+  mlp_features(m->off3.as<int>(), ns, m->X0.as<unsigned short>(), st);
+  mlp_gemm(m->X0.as<unsigned short>(), kMlpIn, 1, wb(m, kOffW1), kHidden, 1, m->Z1.as<float>(),
+           kHidden, ns, kHidden, kMlpIn, 0, st);
+  mlp_bias_act(m->Z1.as<float>(), pp(m, kOffB1), ns, kHidden, 1, m->H1.as<unsigned short>(), st);
+  mlp_gemm(m->H1.as<unsigned short>(), kHidden, 1, wb(m, kOffW2), kHidden, 1, m->Z2.as<float>(),
+           kHidden, ns, kHidden, kHidden, 0, st);
+  mlp_bias_act(m->Z2.as<float>(), pp(m, kOffB2), ns, kHidden, 1, m->H2.as<unsigned short>(), st);
+  mlp_gemm(m->H2.as<unsigned short>(), kHidden, 1, wb(m, kOffW3), kHidden, 1, m->Z3.as<float>(),
+           kHidden, ns, kHidden, kHidden, 0, st);
+  mlp_bias_act(m->Z3.as<float>(), pp(m, kOffB3), ns, kHidden, 1, m->H3.as<unsigned short>(), st);
+  mlp_gemm(m->H3.as<unsigned short>(), kHidden, 1, wb(m, kOffW4), kMlpOutPad, 1, m->Y.as<float>(),
+           kMlpOutPad, ns, kMlpOutPad, kHidden, 0, st);
+  mlp_bias_act(m->Y.as<float>(), pp(m, kOffB4), ns, kMlpOutPad, 0, nullptr, st);
+}
+
+// Gradient of the loss into m->G from dY (fp32, ns x kMlpOutPad).
+void mlp_backward(ddd_mlp* m, int ns, hipStream_t st) {
+  float* G = m->G.as<float>();
+  unsigned short* dYb = m->dYb.as<unsigned short>();
+  unsigned short* dZ = m->dZb.as<unsigned short>();
+  float* dH = m->dH.as<float>();
+  mlp_to_bf16(m->dY.as<float>(), (int64_t)ns * kMlpOutPad, dYb, st);
+  mlp_gemm(m->H3.as<unsigned short>(), 1, kHidden, dYb, kMlpOutPad, 1, G + kOffW4, kMlpOutPad,
+           kHidden, kMlpOutPad, ns, 0, st);
+  mlp_colsum(dYb, ns, kMlpOutPad, G + kOffB4, st);
+  mlp_gemm(dYb, kMlpOutPad, 1, wb(m, kOffW4), 1, kMlpOutPad, dH, kHidden, ns, kHidden, kMlpOutPad,
+           0, st);
+  mlp_act_grad(dH, m->Z3.as<float>(), (int64_t)ns * kHidden, dZ, st);
+  mlp_gemm(m->H2.as<unsigned short>(), 1, kHidden, dZ, kHidden, 1, G + kOffW3, kHidden, kHidden,
+           kHidden, ns, 0, st);
+  mlp_colsum(dZ, ns, kHidden, G + kOffB3, st);
+  mlp_gemm(dZ, kHidden, 1, wb(m, kOffW3), 1, kHidden, dH, kHidden, ns, kHidden, kHidden, 0, st);
+  mlp_act_grad(dH, m->Z2.as<float>(), (int64_t)ns * kHidden, dZ, st);
+  mlp_gemm(m->H1.as<unsigned short>(), 1, kHidden, dZ, kHidden, 1, G + kOffW2, kHidden, kHidden,
+           kHidden, ns, 0, st);
+  mlp_colsum(dZ, ns, kHidden, G + kOffB2, st);
+  mlp_gemm(dZ, kHidden, 1, wb(m, kOffW2), 1, kHidden, dH, kHidden, ns, kHidden, kHidden, 0, st);
+  mlp_act_grad(dH, m->Z1.as<float>(), (int64_t)ns * kHidden, dZ, st);
+  mlp_gemm(m->X0.as<unsigned short>(), 1, kMlpIn, dZ, kHidden, 1, G + kOffW1, kHidden, kMlpIn,
+           kHidden, ns, 0, st);
+  mlp_colsum(dZ, ns, kHidden, G + kOffB1, st);
+}
+
+void mlp_refresh_bf16(ddd_mlp* m, hipStream_t st) {
+  mlp_to_bf16(m->P.as<float>(), kMlpParams, m->Wb.as<unsigned short>(), st);
+}
+
+float env_scale(const ddd_consts& k, const ddd_farfield* ff) { return (float)(k.mu4pi / ff->g.h); }
+
+void operator_from_mlp(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_consts& k) {
+  const int G = ff->g.G, S = 2 * G - 1;
+  const int no = S * S * S;
+  std::vector<int> off((size_t)3 * no);
+  for (int z = 0; z < S; ++z)
+    for (int y = 0; y < S; ++y)
+      for (int x = 0; x < S; ++x) {
+        const int o = x + S * (y + S * z);
+        off[3 * o] = x - (G - 1);
+        off[3 * o + 1] = y - (G - 1);
+        off[3 * o + 2] = z - (G - 1);
+      }
+  ScopedBuf dW, scratch, dOff;
+  const int chunk = 4096;
+  dW.ensure(sizeof(float) * (int64_t)no * kChan);
+  dOff.ensure(sizeof(int) * 3 * no);
+  HIPCHK(hipMemcpyAsync(dOff.p, off.data(), sizeof(int) * 3 * no, hipMemcpyHostToDevice, c->st));
+  m->ensure_batch(chunk);
+  for (int o0 = 0; o0 < no; o0 += chunk) {
+    const int cnt = std::min(chunk, no - o0);
+    HIPCHK(hipMemcpyAsync(m->off3.p, dOff.as<int>() + 3 * o0, sizeof(int) * 3 * cnt,
+                          hipMemcpyDeviceToDevice, c->st));
+    mlp_forward(m, cnt, c->st);
+    mlp_operator_from_y(m->Y.as<float>(), m->off3.as<int>(), cnt, env_scale(k, ff),
+                        dW.as<float>() + (int64_t)o0 * kChan, c->st);
+  }
+  scratch.ensure(sizeof(float2) * kChan * ff->vol);
+  ff->What.ensure(sizeof(float2) * kChan * ff->vol);
+  launch_operator_spectrum(dW.as<float>(), ff->g, ff->rn, ff->tw.as<float2>(), scratch.as<float2>(),
+                           ff->What.as<float2>(), c->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->st));
+  ff->haveOperator = true;
+}
+
+}  // namespace
+
+int64_t ddd_mlp_num_params(void) { return kMlpParams; }
+
+int ddd_mlp_create(ddd_ctx* c, uint64_t seed, ddd_mlp** out) {
+  return guard(c, "ddd_mlp_create", [&] {
+    if (!out) bad("NULL argument");
+    std::vector<float> p((size_t)kMlpParams, 0.f);
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    auto fill = [&](int64_t off, int fin, int fout, double gain) {
+      const double sd = gain / std::sqrt((double)fin);
+      for (int64_t e = 0; e < (int64_t)fin * fout; ++e) p[(size_t)(off + e)] = (float)(sd * nd(rng));
+    };
+    fill(kOffW1, kMlpIn, kHidden, 1.7);
+    fill(kOffW2, kHidden, kHidden, 1.7);
+    fill(kOffW3, kHidden, kHidden, 1.7);
+    fill(kOffW4, kHidden, kMlpOutPad, 0.05);
+    ddd_mlp* m = new ddd_mlp();
+    try {
+      for (DevBuf* b : {&m->P, &m->G, &m->Mm, &m->Vv}) b->ensure(sizeof(float) * kMlpParams);
+      m->Wb.ensure(sizeof(unsigned short) * kMlpParams);
+      HIPCHK(hipMemcpyAsync(m->P.p, p.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipMemsetAsync(m->Mm.p, 0, sizeof(float) * kMlpParams, c->st));
+      HIPCHK(hipMemsetAsync(m->Vv.p, 0, sizeof(float) * kMlpParams, c->st));
+      mlp_refresh_bf16(m, c->st);
+      HIPCHK(hipStreamSynchronize(c->st));
+    } catch (...) {
+      m->release();
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+int ddd_mlp_destroy(ddd_ctx* c, ddd_mlp* m) {
+  if (!m) return DDD_OK;
+  return guard(c, "ddd_mlp_destroy", [&] {
+    HIPCHK(hipStreamSynchronize(c->st));
+    m->release();
+    delete m;
+  });
+}
+
+int ddd_mlp_params(ddd_ctx* c, ddd_mlp* m, float* get, const float* set) {
+  return guard(c, "ddd_mlp_params", [&] {
+    if (!m) bad("NULL mlp");
+    if (get)
+      HIPCHK(hipMemcpyAsync(get, m->P.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, c->st));
+    if (set) {
+      HIPCHK(hipMemcpyAsync(m->P.p, set, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, c->st));
+      mlp_refresh_bf16(m, c->st);
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+int ddd_mlp_loss_grad(ddd_ctx* c, ddd_mlp* m, int ns, const int* off3, const float* M,
+                      const float* P, const float* sigma, const float* den, float envScale,
+                      double* loss, float* grad) {
+  return guard(c, "ddd_mlp_loss_grad", [&] {
+    if (!m || !off3 || !M || !P || !sigma || !den || !loss || !grad) bad("NULL argument");
+    if (ns < 1 || ns > (1 << 20)) bad("ns out of range");
+    for (int s = 0; s < ns; ++s)
+      if (off3[3 * s] == 0 && off3[3 * s + 1] == 0 && off3[3 * s + 2] == 0) bad("zero offset");
+    m->ensure_batch(ns);
+    ScopedBuf dM, dP, dS, dD;
+    dM.ensure(sizeof(float) * kMoments * ns);
+    dP.ensure(sizeof(float) * kPoly * ns);
+    dS.ensure(sizeof(float) * 6 * ns);
+    dD.ensure(sizeof(float) * ns);
+    HIPCHK(hipMemcpyAsync(m->off3.p, off3, sizeof(int) * 3 * ns, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dM.p, M, sizeof(float) * kMoments * ns, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dP.p, P, sizeof(float) * kPoly * ns, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dS.p, sigma, sizeof(float) * 6 * ns, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dD.p, den, sizeof(float) * ns, hipMemcpyHostToDevice, c->st));
+    mlp_forward(m, ns, c->st);
+    mlp_contract(m->Y.as<float>(), m->off3.as<int>(), dM.as<float>(), dP.as<float>(), dS.as<float>(),
+                 dD.as<float>(), ns, envScale, m->dY.as<float>(), m->lossPer.as<float>(), c->st);
+    mlp_backward(m, ns, c->st);
+    std::vector<float> lp((size_t)ns);
+    HIPCHK(hipMemcpyAsync(lp.data(), m->lossPer.p, sizeof(float) * ns, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(grad, m->G.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double L = 0;
+    for (float v : lp) L += v;
+    *loss = L;
+  });
+}
+
+int ddd_farfield_set_operator_mlp(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_consts* k) {
+  return guard(c, "ddd_farfield_set_operator_mlp", [&] {
+    if (!ff || !m || !k) bad("NULL argument");
+    operator_from_mlp(c, ff, m, *k);
+  });
+}
+
+int ddd_farfield_train(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_consts* k,
+                       const ddd_network* net, const ddd_train_opts* o, ddd_train_report* rep) {
+  return guard(c, "ddd_farfield_train", [&] {
+    if (!ff || !m || !k || !o) bad("NULL argument");
+    check_net(net);
+    if (o->samples < 1 || o->steps < 1 || o->batch < 1 || o->batch > (1 << 16) || o->valSamples < 0)
+      bad("bad training options");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int G = ff->g.G, rn = ff->rn;
+    const int nc = G * G * G;
+    const int64_t n = net->numSeg;
+    // Training network: segments, cells, moments (its own buffers).
+    upload_segments(c, net);
+    c->resN = -1;  // the segment buffer now holds the training network
+    ff->cellOf.ensure(sizeof(int) * std::max<int64_t>(n, 1));
+    ff->tmp.ensure(sizeof(int) * std::max<int64_t>(n, 1));
+    ff->order.ensure(sizeof(int) * std::max<int64_t>(n, 1));
+    ff->counts.ensure(sizeof(unsigned) * nc);
+    ff->cursor.ensure(sizeof(unsigned) * nc);
+    ff->start.ensure(sizeof(int) * (nc + 1));
+    ff->M.ensure(sizeof(float) * kMoments * nc);
+    launch_cells(c->seg.as<double>(), c->resLd, (int)n, ff->g, ff->cellOf.as<int>(),
+                 ff->counts.as<unsigned>(), ff->cursor.as<unsigned>(), ff->start.as<int>(),
+                 ff->tmp.as<int>(), ff->order.as<int>(), c->st);
+    launch_moments(c->seg.as<double>(), c->resLd, ff->g, ff->start.as<int>(), ff->order.as<int>(),
+                   ff->M.as<float>(), c->st);
+    std::vector<int> start(nc + 1), order((size_t)std::max<int64_t>(n, 1));
+    std::vector<float> Mh((size_t)kMoments * nc);
+    HIPCHK(hipMemcpyAsync(start.data(), ff->start.p, sizeof(int) * (nc + 1), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(order.data(), ff->order.p, sizeof(int) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(Mh.data(), ff->M.p, sizeof(float) * kMoments * nc, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    std::vector<int> live;
+    for (int cc = 0; cc < nc; ++cc)
+      if (start[cc + 1] > start[cc]) live.push_back(cc);
+    if (live.size() < 2) bad("training network occupies fewer than two cells");
+    // Samples: target point = a Gauss point of a random segment of a random
+    // occupied cell; source = a random occupied far cell, half of them from
+    // the first three shells outside the near stencil.
+    const int64_t NS = o->samples + o->valSamples;
+    if (NS > INT32_MAX / kMoments) bad("too many samples");
+    std::vector<int> off((size_t)3 * NS), src((size_t)NS);
+    std::vector<float> Ps((size_t)kPoly * NS), Ms((size_t)kMoments * NS);
+    std::vector<double> pts((size_t)3 * NS);
+    std::mt19937_64 rng(o->seed);
+    std::uniform_int_distribution<size_t> pickLive(0, live.size() - 1);
+    int64_t tries = 0;
+    const double gq[2] = {0.21132486540518711775, 0.78867513459481288225};
+    const int64_t L = net->segIdxLd;
+    for (int64_t s = 0; s < NS; ++s) {
+      int t, sc, ox, oy, oz;
+      for (;;) {
+        if (++tries > 1000 * (NS + 1000)) bad("training network has too few far cell pairs");
+        t = live[pickLive(rng)];
+        const int tx = t % G, ty = (t / G) % G, tz = t / (G * G);
+        if (rng() & 1) {
+          sc = live[pickLive(rng)];
+        } else {
+          const int R = rn + 1 + (int)(rng() % 3);
+          const int sx = tx + (int)(rng() % (2 * R + 1)) - R, sy = ty + (int)(rng() % (2 * R + 1)) - R,
+                    sz = tz + (int)(rng() % (2 * R + 1)) - R;
+          if (sx < 0 || sy < 0 || sz < 0 || sx >= G || sy >= G || sz >= G) continue;
+          sc = sx + G * (sy + G * sz);
+          if (start[sc + 1] == start[sc]) continue;
+        }
+        ox = sc % G - tx;
+        oy = (sc / G) % G - ty;
+        oz = sc / (G * G) - tz;
+        if (std::max(std::abs(ox), std::max(std::abs(oy), std::abs(oz))) > rn) break;
+      }
+      const int e = start[t] + (int)(rng() % (uint64_t)(start[t + 1] - start[t]));
+      const int i = order[e];
+      const double* a = net->coord + 3 * (net->segIdx[i + L] - 1);
+      const double* b = net->coord + 3 * (net->segIdx[i + 2 * L] - 1);
+      const double u = gq[rng() & 1];
+      const double x[3] = {a[0] + u * (b[0] - a[0]), a[1] + u * (b[1] - a[1]), a[2] + u * (b[2] - a[2])};
+      const int tx = t % G, ty = (t / G) % G, tz = t / (G * G);
+      const double rho[3] = {(x[0] - (ff->g.lo[0] + (tx + 0.5) * ff->g.h)) / ff->g.h,
+                             (x[1] - (ff->g.lo[1] + (ty + 0.5) * ff->g.h)) / ff->g.h,
+                             (x[2] - (ff->g.lo[2] + (tz + 0.5) * ff->g.h)) / ff->g.h};
+      float* P = &Ps[(size_t)kPoly * s];
+      P[0] = 1.f; P[1] = (float)rho[0]; P[2] = (float)rho[1]; P[3] = (float)rho[2];
+      P[4] = (float)(rho[0] * rho[0]); P[5] = (float)(rho[1] * rho[1]); P[6] = (float)(rho[2] * rho[2]);
+      P[7] = (float)(rho[0] * rho[1]); P[8] = (float)(rho[0] * rho[2]); P[9] = (float)(rho[1] * rho[2]);
+      for (int f = 0; f < kMoments; ++f) Ms[(size_t)kMoments * s + f] = Mh[(size_t)f * nc + sc];
+      off[3 * s] = ox; off[3 * s + 1] = oy; off[3 * s + 2] = oz;
+      src[s] = sc;
+      for (int q = 0; q < 3; ++q) pts[3 * s + q] = x[q];
+    }
+    // Exact stresses.
+    ScopedBuf dSrc, dPts, dSig, dOff, dM, dP, dDen;
+    dSrc.ensure(sizeof(int) * NS);
+    dPts.ensure(sizeof(double) * 3 * NS);
+    dSig.ensure(sizeof(float) * 6 * NS);
+    HIPCHK(hipMemcpyAsync(dSrc.p, src.data(), sizeof(int) * NS, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dPts.p, pts.data(), sizeof(double) * 3 * NS, hipMemcpyHostToDevice, c->st));
+    const SigK<double> SK{k->coreRadSq, k->mu4pi, k->mu4pinu, k->nuMu4pinu, k->coreRadSq * k->mu8pi};
+    sample_stress(c->seg.as<double>(), c->resLd, ff->start.as<int>(), ff->order.as<int>(),
+                  dSrc.as<int>(), dPts.as<double>(), (int)NS, SK, dSig.as<float>(), c->st);
+    HIPCHK(hipGetLastError());
+    std::vector<float> sig((size_t)6 * NS), den((size_t)NS);
+    HIPCHK(hipMemcpyAsync(sig.data(), dSig.p, sizeof(float) * 6 * NS, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    const float es = env_scale(*k, ff);
+    for (int64_t s = 0; s < NS; ++s) {
+      double a = 0;
+      for (int q = 0; q < 6; ++q) a += (double)sig[6 * s + q] * sig[6 * s + q];
+      const double R2 = (double)off[3 * s] * off[3 * s] + (double)off[3 * s + 1] * off[3 * s + 1] +
+                        (double)off[3 * s + 2] * off[3 * s + 2];
+      double m0 = 0;
+      for (int f = 0; f < 9; ++f) m0 += (double)Ms[(size_t)kMoments * s + f] * Ms[(size_t)kMoments * s + f];
+      const double fl = 1e-2 * es / R2 * std::sqrt(m0);
+      den[s] = (float)(a + fl * fl + 1e-30);
+    }
+    dOff.ensure(sizeof(int) * 3 * NS);
+    dM.ensure(sizeof(float) * kMoments * NS);
+    dP.ensure(sizeof(float) * kPoly * NS);
+    dDen.ensure(sizeof(float) * NS);
+    HIPCHK(hipMemcpyAsync(dOff.p, off.data(), sizeof(int) * 3 * NS, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dM.p, Ms.data(), sizeof(float) * kMoments * NS, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dP.p, Ps.data(), sizeof(float) * kPoly * NS, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dDen.p, den.data(), sizeof(float) * NS, hipMemcpyHostToDevice, c->st));
+    // Training: consecutive batches of a shuffled order of the training set,
+    // gathered on the device; a new order each epoch (the tail is dropped).
+    const int B = (int)std::min<int64_t>(o->batch, o->samples);
+    m->ensure_batch(std::max<int>(B, 1024));
+    ScopedBuf bM, bP, bS, bD, dPerm;
+    bM.ensure(sizeof(float) * kMoments * B);
+    bP.ensure(sizeof(float) * kPoly * B);
+    bS.ensure(sizeof(float) * 6 * B);
+    bD.ensure(sizeof(float) * B);
+    dPerm.ensure(sizeof(int) * o->samples);
+    std::vector<int> perm((size_t)o->samples);
+    for (int64_t s = 0; s < o->samples; ++s) perm[(size_t)s] = (int)s;
+    std::vector<float> lp((size_t)std::max<int64_t>(B, 1));
+    std::vector<double> losses;
+    int64_t cursor = o->samples;
+    const double b1 = 0.9, b2 = 0.999;
+    for (int stp = 0; stp < o->steps; ++stp) {
+      if (cursor + B > o->samples) {
+        HIPCHK(hipStreamSynchronize(c->st));  // the previous order is no longer read
+        std::shuffle(perm.begin(), perm.end(), rng);
+        HIPCHK(hipMemcpyAsync(dPerm.p, perm.data(), sizeof(int) * o->samples, hipMemcpyHostToDevice,
+                              c->st));
+        cursor = 0;
+      }
+      gather_batch(dPerm.as<int>() + cursor, B, dOff.as<int>(), dM.as<float>(), dP.as<float>(),
+                   dSig.as<float>(), dDen.as<float>(), m->off3.as<int>(), bM.as<float>(),
+                   bP.as<float>(), bS.as<float>(), bD.as<float>(), c->st);
+      cursor += B;
+      mlp_forward(m, B, c->st);
+      mlp_contract(m->Y.as<float>(), m->off3.as<int>(), bM.as<float>(), bP.as<float>(), bS.as<float>(),
+                   bD.as<float>(), B, es, m->dY.as<float>(), m->lossPer.as<float>(), c->st);
+      mlp_backward(m, B, c->st);
+      ++m->step;
+      const double frac = (double)stp / std::max(1, o->steps - 1);
+      const double lr = o->lr * std::pow(o->lrFinal / o->lr, frac);
+      mlp_adam(m->P.as<float>(), m->G.as<float>(), m->Mm.as<float>(), m->Vv.as<float>(), kMlpParams,
+               (float)lr, (float)b1, (float)b2, (float)(1 - std::pow(b1, (double)m->step)),
+               (float)(1 - std::pow(b2, (double)m->step)), 1e-8f, c->st);
+      mlp_refresh_bf16(m, c->st);
+      if (stp < 50 || stp >= o->steps - 50) {
+        HIPCHK(hipMemcpyAsync(lp.data(), m->lossPer.p, sizeof(float) * B, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        double Ls = 0;
+        for (int q = 0; q < B; ++q) Ls += lp[(size_t)q];
+        losses.push_back(Ls);
+      }
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    // Held-out error: predictions of the trained operator.
+    double num = 0, dnm = 0, numN = 0, dnmN = 0;
+    for (int64_t v0 = o->samples; v0 < NS; v0 += B) {
+      const int cnt = (int)std::min<int64_t>(B, NS - v0);
+      HIPCHK(hipMemcpyAsync(m->off3.p, dOff.as<int>() + 3 * v0, sizeof(int) * 3 * cnt,
+                            hipMemcpyDeviceToDevice, c->st));
+      mlp_forward(m, cnt, c->st);
+      mlp_contract(m->Y.as<float>(), m->off3.as<int>(), dM.as<float>() + kMoments * v0,
+                   dP.as<float>() + kPoly * v0, dSig.as<float>() + 6 * v0, dDen.as<float>() + v0,
+                   cnt, es, m->dY.as<float>(), m->lossPer.as<float>(), c->st);
+      HIPCHK(hipMemcpyAsync(lp.data(), m->lossPer.p, sizeof(float) * cnt, hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+      for (int q = 0; q < cnt; ++q) {
+        const int64_t s = v0 + q;
+        double a = 0;
+        for (int w = 0; w < 6; ++w) a += (double)sig[6 * s + w] * sig[6 * s + w];
+        const double e2 = (double)lp[(size_t)q] * den[s] * cnt;  // |sigma_hat - sigma|^2
+        num += e2;
+        dnm += a;
+        const int ch = std::max(std::abs(off[3 * s]), std::max(std::abs(off[3 * s + 1]), std::abs(off[3 * s + 2])));
+        if (ch <= rn + 2) { numN += e2; dnmN += a; }
+      }
+    }
+    operator_from_mlp(c, ff, m, *k);
+    if (rep) {
+      const size_t h = std::min<size_t>(50, losses.size());
+      double a = 0, b = 0;
+      for (size_t q = 0; q < h; ++q) a += losses[q];
+      for (size_t q = losses.size() - h; q < losses.size(); ++q) b += losses[q];
+      rep->lossFirst = h ? a / h : 0;
+      rep->lossLast = h ? b / h : 0;
+      rep->relErrVal = dnm > 0 ? std::sqrt(num / dnm) : 0;
+      rep->relErrValNear = dnmN > 0 ? std::sqrt(numN / dnmN) : 0;
+      rep->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
   });
 }
 

@@ -56,6 +56,17 @@ class ddd_stats(C.Structure):
     _fields_ = [("pairs", C.c_uint64), ("pairsFP64", C.c_uint64)]
 
 
+class ddd_train_opts(C.Structure):
+    _fields_ = [("samples", i64), ("valSamples", i64), ("steps", C.c_int), ("batch", C.c_int),
+                ("lr", C.c_float), ("lrFinal", C.c_float), ("seed", C.c_uint64)]
+
+
+class ddd_train_report(C.Structure):
+    _fields_ = [("lossFirst", C.c_double), ("lossLast", C.c_double),
+                ("relErrVal", C.c_double), ("relErrValNear", C.c_double),
+                ("seconds", C.c_double)]
+
+
 class DDDError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[{code}] {msg}")
@@ -109,6 +120,19 @@ def lib():
                                                     C.POINTER(ddd_network), C.POINTER(ddd_opts),
                                                     dp, i64, C.c_int, C.POINTER(ddd_stats)]),
                 "ddd_net_download": (C.c_int, [P, dp, i64]),
+                "ddd_mlp_create": (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
+                "ddd_mlp_destroy": (C.c_int, [P, P]),
+                "ddd_mlp_num_params": (i64, []),
+                "ddd_mlp_params": (C.c_int, [P, P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+                "ddd_mlp_loss_grad": (C.c_int, [P, P, C.c_int, C.POINTER(C.c_int),
+                                                C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                                C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                                C.c_float, dp, C.POINTER(C.c_float)]),
+                "ddd_farfield_train": (C.c_int, [P, P, P, C.POINTER(ddd_consts),
+                                                 C.POINTER(ddd_network),
+                                                 C.POINTER(ddd_train_opts),
+                                                 C.POINTER(ddd_train_report)]),
+                "ddd_farfield_set_operator_mlp": (C.c_int, [P, P, P, C.POINTER(ddd_consts)]),
                 "ddd_net_segforce_device": (P, [P]),
             }
             for name, (res, args) in sigs.items():

@@ -250,6 +250,82 @@ class FarField:
             pass
 
 
+class FarFieldMLP:
+    """The far-field network (ddd_mlp_*): offset features -> 256 -> 256 ->
+    256 -> 5408, SiLU, BF16 MFMA products with FP32 accumulation; FP32 master
+    parameters and Adam state on the device."""
+
+    def __init__(self, seed=0, ctx=None):
+        self.ctx = _ctx(ctx)
+        self.h_ptr = C_void()
+        _call(self.ctx, "ddd_mlp_create", int(seed), _byref(self.h_ptr))
+
+    @staticmethod
+    def num_params():
+        return int(L.lib().ddd_mlp_num_params())
+
+    def get_params(self):
+        out = np.zeros(self.num_params(), dtype=np.float32)
+        _call(self.ctx, "ddd_mlp_params", self.h_ptr, out.ctypes.data_as(_c.POINTER(_c.c_float)),
+              None)
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.num_params():
+            raise ValueError("wrong parameter count")
+        _call(self.ctx, "ddd_mlp_params", self.h_ptr, None,
+              p.ctypes.data_as(_c.POINTER(_c.c_float)))
+
+    def loss_grad(self, off3, M, P, sigma, den, envScale):
+        """Loss and parameter gradient on one batch (see ddd_mlp_loss_grad)."""
+        f = _c.POINTER(_c.c_float)
+        off3 = np.ascontiguousarray(off3, dtype=np.int32)
+        ns = off3.shape[0]
+        M = np.ascontiguousarray(M, dtype=np.float32).reshape(ns, 90)
+        P = np.ascontiguousarray(P, dtype=np.float32).reshape(ns, 10)
+        sigma = np.ascontiguousarray(sigma, dtype=np.float32).reshape(ns, 6)
+        den = np.ascontiguousarray(den, dtype=np.float32).reshape(ns)
+        loss = _c.c_double()
+        grad = np.zeros(self.num_params(), dtype=np.float32)
+        _call(self.ctx, "ddd_mlp_loss_grad", self.h_ptr, int(ns),
+              off3.ctypes.data_as(_c.POINTER(_c.c_int)), M.ctypes.data_as(f),
+              P.ctypes.data_as(f), sigma.ctypes.data_as(f), den.ctypes.data_as(f),
+              float(envScale), _byref(loss), grad.ctypes.data_as(f))
+        return loss.value, grad
+
+    def close(self):
+        if self.h_ptr:
+            L.lib().ddd_mlp_destroy(self.ctx.h, self.h_ptr)
+            self.h_ptr = C_void()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def trainFarField(dlnParams, matParams, farfield, mlp, trainNetwork, samples=1 << 20,
+                  valSamples=1 << 15, steps=3000, batch=2048, lr=1e-3, lrFinal=3e-5, seed=1):
+    """Train mlp on exact stresses of trainNetwork binned on farfield's grid
+    and install the operator it predicts.  Returns the report as a dict."""
+    ctx = farfield.ctx
+    nv = L.NetView(trainNetwork)
+    k = L.consts(dlnParams, matParams)
+    o = L.ddd_train_opts(samples=int(samples), valSamples=int(valSamples), steps=int(steps),
+                         batch=int(batch), lr=float(lr), lrFinal=float(lrFinal), seed=int(seed))
+    r = L.ddd_train_report()
+    _call(ctx, "ddd_farfield_train", farfield.h_ptr, mlp.h_ptr, k, nv.s, o, r)
+    return {f: getattr(r, f) for f, _ in L.ddd_train_report._fields_}
+
+
+def setFarFieldFromMLP(dlnParams, matParams, farfield, mlp):
+    """Install the operator mlp predicts at every offset of farfield's grid."""
+    k = L.consts(dlnParams, matParams)
+    _call(farfield.ctx, "ddd_farfield_set_operator_mlp", farfield.h_ptr, mlp.h_ptr, k)
+
+
 def calcSegSegForceHybrid(dlnParams, matParams, network, farfield, farDist=0.0, sinMin=0.0,
                           ctx=None):
     """All-pairs seg-seg force, hybrid: near pairs exact (mixed rule), far

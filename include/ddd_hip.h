@@ -132,6 +132,45 @@ int ddd_farfield_create(ddd_ctx* ctx, int G, const double lo[3], double h, int n
                         ddd_farfield** out);
 int ddd_farfield_destroy(ddd_ctx* ctx, ddd_farfield* ff);
 int ddd_farfield_set_operator(ddd_ctx* ctx, ddd_farfield* ff, const float* W);
+/* The far-field network (no reference counterpart): an MLP from a cell
+ * offset to the operator W(o), trained on exact stresses.  Parameters are
+ * ddd_mlp_num_params() floats (W1, b1, ..., W4, b4; see csrc/mlp.hpp). */
+typedef struct ddd_mlp ddd_mlp;
+int ddd_mlp_create(ddd_ctx* ctx, uint64_t seed, ddd_mlp** out);
+int ddd_mlp_destroy(ddd_ctx* ctx, ddd_mlp* mlp);
+int64_t ddd_mlp_num_params(void);
+/* Copy the master parameters out (get != NULL) and/or in (set != NULL). */
+int ddd_mlp_params(ddd_ctx* ctx, ddd_mlp* mlp, float* get, const float* set);
+/* Loss and gradient on one host batch of ns samples: off3 (3 ns ints, the
+ * cell offset source - target), M (90 ns), P (10 ns), sigma (6 ns), den (ns);
+ * loss = sum_s |sigma_hat_s - sigma_s|^2 / (den_s ns).  For gradient checks. */
+int ddd_mlp_loss_grad(ddd_ctx* ctx, ddd_mlp* mlp, int ns, const int* off3, const float* M,
+                      const float* P, const float* sigma, const float* den, float envScale,
+                      double* loss, float* grad);
+
+typedef struct ddd_train_opts {
+  int64_t samples;     /* training samples (source cell, target point)   */
+  int64_t valSamples;  /* held-out samples                                */
+  int steps;           /* Adam steps                                      */
+  int batch;           /* samples per step                                */
+  float lr, lrFinal;   /* learning rate, decayed geometrically            */
+  uint64_t seed;
+} ddd_train_opts;
+typedef struct ddd_train_report {
+  double lossFirst, lossLast;  /* mean training loss of the first / last 50 steps */
+  double relErrVal;            /* sqrt(sum |sigma_hat - sigma|^2 / sum |sigma|^2) */
+  double relErrValNear;        /* the same over samples with Chebyshev offset <= nearRadius + 2 */
+  double seconds;
+} ddd_train_report;
+/* Train mlp on exact stresses of trainNet binned on ff's grid, then install
+ * the operator it predicts into ff (as ddd_farfield_set_operator_mlp). */
+int ddd_farfield_train(ddd_ctx* ctx, ddd_farfield* ff, ddd_mlp* mlp, const ddd_consts* k,
+                       const ddd_network* trainNet, const ddd_train_opts* opts,
+                       ddd_train_report* report);
+/* Evaluate mlp at every offset of ff's grid and install W(o) (BF16 MFMA). */
+int ddd_farfield_set_operator_mlp(ddd_ctx* ctx, ddd_farfield* ff, ddd_mlp* mlp,
+                                  const ddd_consts* k);
+
 /* Intermediate results of the last hybrid evaluation, for tests:
  * what 0: cell moments, 90 x G^3 floats; 1: local expansions, 60 x G^3
  * floats; 2: cell of each segment, numSeg int32 (-1 external). */

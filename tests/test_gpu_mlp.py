@@ -1,0 +1,88 @@
+"""GPU: the far-field network.
+
+* loss and gradient of ddd_mlp_loss_grad (BF16 MFMA products, FP32
+  accumulation) against the numpy restatement that rounds to bf16 at the same
+  points: loss within 5e-3, each parameter block's gradient within 3e-2
+  normwise (FP32 vs FP64 accumulation and bf16 ties that fall differently);
+* training on the exact stresses of one network lowers the loss, and the
+  operator it predicts improves the hybrid forces of another network over the
+  no-far-field ablation and over the untrained network.
+"""
+import numpy as np
+import pytest
+
+import mlp_ref as R
+import pyoracle as O
+import refnets as RN
+from dddhip import forces as G
+from dddhip.generate import random_loop_network
+from metrics import normwise
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def prm():
+    return RN.params()
+
+
+def test_params_roundtrip():
+    m = G.FarFieldMLP(seed=7)
+    p = m.get_params()
+    assert p.size == R.layout()[1] and np.isfinite(p).all()
+    q = R.init_params(9).astype(np.float32)
+    m.set_params(q)
+    assert np.array_equal(m.get_params(), q)
+    m.close()
+
+
+@pytest.mark.parametrize("ns", [200, 64])
+def test_loss_grad_vs_reference(ns):
+    m = G.FarFieldMLP(seed=1)
+    p = R.init_params(3).astype(np.float32)
+    m.set_params(p)
+    off, M, P, sig, den = R.random_batch(ns, seed=ns)
+    f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+    M, P, sig, den = f32(M), f32(P), f32(sig), f32(den)
+    es = 0.5
+    loss, g = m.loss_grad(off, M, P, sig, den, es)
+    lr, gr = R.loss_grad(p.astype(np.float64), off, M, P, sig, den, es, bf=True)
+    print(f"ns={ns}: loss {loss:.6e} ref {lr:.6e}", flush=True)
+    assert abs(loss - lr) <= 5e-3 * abs(lr)
+    for name, o, shp in R.layout()[0]:
+        n = int(np.prod(shp))
+        e = normwise(g[o:o + n], gr[o:o + n])
+        print(f"  {name}: normwise {e:.2e}", flush=True)
+        assert e <= 3e-2, name
+    m.close()
+
+
+def test_training_and_ablation(prm):
+    dln, mat = prm
+    Gc, rn, h = 8, 1, 250.0
+    train = random_loop_network(512, seed=0xE1)
+    ev = random_loop_network(192, seed=0xE2)
+    ff = G.FarField(Gc, (0.0, 0.0, 0.0), h, rn)
+    S = 2 * Gc - 1
+    ref = O.calcSegSegForcePar(dln, mat, ev, 16)
+
+    def err():
+        g, _, _ = G.calcSegSegForceHybrid(dln, mat, ev, ff, farDist=1e30)
+        return normwise(g, ref)
+
+    ff.set_operator(np.zeros((S ** 3, 60, 90), np.float32))
+    e_none = err()
+    m = G.FarFieldMLP(seed=3)
+    G.setFarFieldFromMLP(dln, mat, ff, m)
+    e_untrained = err()
+    rep = G.trainFarField(dln, mat, ff, m, train, samples=1 << 17, valSamples=1 << 13, steps=600,
+                          batch=1024, lr=1e-3, lrFinal=1e-4, seed=5)
+    e_trained = err()
+    print(f"report {rep}", flush=True)
+    print(f"seg-seg normwise error: no far field {e_none:.3e}, untrained {e_untrained:.3e}, "
+          f"trained {e_trained:.3e}", flush=True)
+    assert rep["lossLast"] < 0.3 * rep["lossFirst"]
+    assert rep["relErrVal"] < 0.5
+    assert e_trained < 0.7 * e_none and e_trained < 0.7 * e_untrained
+    m.close()
+    ff.close()
