@@ -31,6 +31,84 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def node_forces(net, sf):
+    """Gather 3 x 2 x numSeg segment-end forces to the nodes (3 x numNode)."""
+    import numpy as np
+    n = sf.shape[2]
+    si = net.segIdx[:n]
+    F = np.zeros((net.coord.shape[1], 3))
+    np.add.at(F, si[:, 1] - 1, sf[:, 0, :].T)
+    np.add.at(F, si[:, 2] - 1, sf[:, 1, :].T)
+    used = np.unique(si[:, 1:3]) - 1
+    return F[used].T
+
+
+def hybrid_paths(args, dln, mat, net, res, stream, ref, fp64_ms, log):
+    """Hybrid force evaluation (exact near stencil with the mixed rule, far
+    field from the trained network) per G:rn configuration: time, error of
+    segment-end and node forces against the plain FP64 result, training
+    report.  The network is trained once per grid on a separate network."""
+    import numpy as np
+    import torch
+    from dddhip import forces as G
+    from dddhip.generate import random_loop_network
+    from metrics import per_node_rel, normwise
+
+    cfgs = [tuple(int(v) for v in c.split(":")) for c in args.hybrid.split(",") if c]
+    # every node lies within rmax = 300 of a centre in [0, 2000]^3
+    lo, side = -300.0, 2600.0
+    train = random_loop_network(args.train_loops, seed=args.train_seed)
+    Fref = node_forces(net, ref)
+    out = {"grid_lo": lo, "grid_side": side, "train_network": f"{args.train_loops} loops, seed "
+           f"{args.train_seed:#x}", "configs": []}
+    for Gc in sorted({g for g, _ in cfgs}):
+        rns = sorted(r for g, r in cfgs if g == Gc)
+        h = side / Gc
+        ffs = {r: G.FarField(Gc, (lo, lo, lo), h, r, ctx=res.ctx) for r in rns}
+        mlp = G.FarFieldMLP(seed=11, ctx=res.ctx)
+        rep = G.trainFarField(dln, mat, ffs[rns[0]], mlp, train, samples=args.train_samples,
+                              valSamples=1 << 15, steps=args.train_steps, batch=args.train_batch,
+                              lr=1e-3, lrFinal=2e-5, seed=7)
+        log(f"G={Gc}: trained in {rep['seconds']:.1f} s, loss {rep['lossFirst']:.3e} -> "
+            f"{rep['lossLast']:.3e}, held-out stress error {rep['relErrVal']:.3e}")
+        for r in rns[1:]:
+            G.setFarFieldFromMLP(dln, mat, ffs[r], mlp)
+        for r in rns:
+            res.set_mode("hybrid", farfield=ffs[r])
+            res.seg_force()
+            stream.synchronize()
+            ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                  for _ in range(args.steps)]
+            for k in range(args.steps):
+                ev[k][0].record(stream)
+                res.seg_force()
+                ev[k][1].record(stream)
+                ev[k][1].synchronize()
+            hms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+            got = res.download()
+            near, near64 = res.stats()
+            smx, srms = per_node_rel(got, ref)
+            nmx, nrms = per_node_rel(node_forces(net, got), Fref)
+            row = {"G": Gc, "rn": r, "h": h, "ms": round(hms, 3),
+                   "speedup_vs_fp64": round(fp64_ms / hms, 2),
+                   "node_rel_err_rms": nrms, "node_rel_err_max": nmx,
+                   "segend_rel_err_rms": srms, "segend_rel_err_max": smx,
+                   "normwise_rel_err": normwise(got, ref),
+                   "near_pairs": near, "near_pairs_fp64": near64,
+                   "train": {k: ([round(x, 6) for x in v] if isinstance(v, list) else round(v, 6))
+                             for k, v in rep.items()}}
+            log(f"hybrid G={Gc} rn={r}: {hms:.2f} ms ({fp64_ms / hms:.1f}x), node err rms "
+                f"{nrms:.2e} max {nmx:.2e}; {near} near pairs, {near64} FP64")
+            out["configs"].append(row)
+            res.set_mode("fp64")
+        for f in ffs.values():
+            f.close()
+        mlp.close()
+    ok = [c for c in out["configs"] if c["node_rel_err_rms"] <= 1e-3]
+    out["best_within_1e-3"] = min(ok, key=lambda c: c["ms"]) if ok else None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -42,6 +120,15 @@ def main():
     ap.add_argument("--cpu-row-stride", type=int, default=64)
     ap.add_argument("--paths", default="fp64,mixed",
                     help="comma list of paths to time after plain FP64: mixed")
+    ap.add_argument("--hybrid", default="16:1,16:2,16:3",
+                    help="comma list of G:rn hybrid configurations (G^3 cells over the "
+                         "network's bounding cube, near stencil of Chebyshev radius rn); "
+                         "empty to skip")
+    ap.add_argument("--train-loops", type=int, default=4096)
+    ap.add_argument("--train-seed", type=lambda s: int(s, 0), default=0xD1)
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-batch", type=int, default=2048)
+    ap.add_argument("--train-samples", type=int, default=1 << 20)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -140,6 +227,8 @@ def main():
         log(f"{path}: {pms:.1f} ms, per-node err max {mx:.2e} rms {rms:.2e}")
         res.set_mode("fp64")
 
+    hybrid = hybrid_paths(args, dln, mat, net, res, stream, out, ms, log) if args.hybrid else None
+
     achieved = FLOPS_PER_PAIR * pairs / (tms * 1e-3) / 1e12
     result = {
         "metric": METRIC,
@@ -166,6 +255,7 @@ def main():
             "reduce_kernel_ms": round(float(np.mean(red_ms)), 3),
         },
         "paths": extra,
+        "hybrid": hybrid,
         "roofline": {
             "bound": "valu",
             "achieved": round(achieved, 3),

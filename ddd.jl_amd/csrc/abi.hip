@@ -75,18 +75,25 @@ struct ddd_farfield {
   int64_t vol = 0;
   bool haveOperator = false;
   DevBuf What, tw, cellOf, counts, cursor, start, tmp, order, M, Mc, Lc, L, counters;
+  // near-field tile list
+  DevBuf mrank, rankCell, mcount, mstart, segS, key, perm, box, rowCount, rowStart, colCount,
+      colStart, list, tileIdx, colList, rowPart, colPart;
+  int64_t nearTiles = 0;
   void release() {
     DevBuf* all[] = {&What, &tw, &cellOf, &counts, &cursor, &start, &tmp, &order, &M, &Mc, &Lc, &L,
-                     &counters};
+                     &counters, &mrank, &rankCell, &mcount, &mstart, &segS, &key, &perm, &box,
+                     &rowCount, &rowStart, &colCount, &colStart, &list, &tileIdx, &colList,
+                     &rowPart, &colPart};
     for (DevBuf* b : all) b->release();
   }
 };
 
 struct ddd_mlp {
   DevBuf P, G, Mm, Vv, Wb;
+  float aH = 0.f;  // core radius in cell units of the grid it was trained for
   int64_t step = 0;
   int cap = 0;
-  DevBuf X0, Z1, Z2, Z3, H1, H2, H3, Y, dY, dYb, dH, dZb, lossPer, off3;
+  DevBuf X0, Z1, Z2, Z3, H1, H2, H3, Y, dY, dYb, dH, dZb, lossPer, off3, colPart;
   void ensure_batch(int ns) {
     if (ns <= cap) return;
     X0.ensure(sizeof(unsigned short) * ns * kMlpIn);
@@ -97,11 +104,12 @@ struct ddd_mlp {
     dYb.ensure(sizeof(unsigned short) * (int64_t)ns * kMlpOutPad);
     lossPer.ensure(sizeof(float) * ns);
     off3.ensure(sizeof(int) * 3 * ns);
+    colPart.ensure(sizeof(float) * 64 * kMlpOutPad);
     cap = ns;
   }
   void release() {
     DevBuf* all[] = {&P, &G, &Mm, &Vv, &Wb, &X0, &Z1, &Z2, &Z3, &H1, &H2, &H3, &Y, &dY, &dYb,
-                     &dH, &dZb, &lossPer, &off3};
+                     &dH, &dZb, &lossPer, &off3, &colPart};
     for (DevBuf* b : all) b->release();
     cap = 0;
   }
@@ -172,7 +180,9 @@ void check_net(const ddd_network* net) {
 
 // Pack segments 0..n-1 into the pair kernels' SoA (fields 0-8 and 13) and
 // derive the tangents on the device.
-void upload_segments(ddd_ctx* c, const ddd_network* net) {
+// Pack the network's segments (SoA, leading dimension pad64(numSeg)) into
+// dst and derive the per-segment invariants; returns the leading dimension.
+int64_t pack_segments(ddd_ctx* c, const ddd_network* net, DevBuf& dst) {
   const int64_t n = net->numSeg, ld = pad64(n);
   c->h.assign((size_t)F_COUNT * ld, 0.0);
   double* h = c->h.data();
@@ -190,12 +200,15 @@ void upload_segments(ddd_ctx* c, const ddd_network* net) {
     h[F_BX * ld + i] = pv[0]; h[F_BY * ld + i] = pv[1]; h[F_BZ * ld + i] = pv[2];
     h[F_EXT * ld + i] = (net->extSeg && net->extSeg[i]) ? 1.0 : 0.0;
   }
-  c->seg.ensure(sizeof(double) * F_COUNT * ld);
-  HIPCHK(hipMemcpyAsync(c->seg.p, h, sizeof(double) * F_COUNT * ld, hipMemcpyHostToDevice, c->st));
-  launch_prep_segments(c->seg.as<double>(), ld, n, c->st);
+  dst.ensure(sizeof(double) * F_COUNT * ld);
+  HIPCHK(hipMemcpyAsync(dst.p, h, sizeof(double) * F_COUNT * ld, hipMemcpyHostToDevice, c->st));
+  launch_prep_segments(dst.as<double>(), ld, n, c->st);
   HIPCHK(hipGetLastError());
-  c->resLd = ld;
+  HIPCHK(hipStreamSynchronize(c->st));  // the host staging vector is reused
+  return ld;
 }
+
+void upload_segments(ddd_ctx* c, const ddd_network* net) { c->resLd = pack_segments(c, net, c->seg); }
 
 // 0-based row list from 1-based idx, validated.
 std::vector<int64_t> rows_from(const int64_t* idx, int64_t nIdx, int64_t numSeg) {
@@ -260,15 +273,18 @@ void check_opts(const ddd_opts* o) {
 
 bool is_hybrid(const ddd_opts* o) { return o && o->mode == DDD_MODE_HYBRID; }
 
-// Hybrid pair stage: cells, moments, far-field expansions, near field (adds
-// into dev).  The far-field forces are added by hybrid_apply.
+// Hybrid pair stage: cells, the near-field tile list, moments, far-field
+// expansions, near-field tiles and their reduction (adds into dev).  The
+// far-field forces are added by hybrid_apply.  One host synchronisation
+// reads the number of near tiles, which sizes the partial-sum buffers.
 void hybrid_pairs(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsigned* flags,
                   const ddd_opts& o) {
   ddd_farfield* ff = o.farfield;
   const int nc = ff->g.G * ff->g.G * ff->g.G;
-  ff->cellOf.ensure(sizeof(int) * std::max<int64_t>(n, 1));
-  ff->tmp.ensure(sizeof(int) * std::max<int64_t>(n, 1));
-  ff->order.ensure(sizeof(int) * std::max<int64_t>(n, 1));
+  const int64_t n1 = std::max<int64_t>(n, 1);
+  ff->cellOf.ensure(sizeof(int) * n1);
+  ff->tmp.ensure(sizeof(int) * n1);
+  ff->order.ensure(sizeof(int) * n1);
   ff->counts.ensure(sizeof(unsigned) * nc);
   ff->cursor.ensure(sizeof(unsigned) * nc);
   ff->start.ensure(sizeof(int) * (nc + 1));
@@ -277,18 +293,66 @@ void hybrid_pairs(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsig
   ff->Mc.ensure(sizeof(float2) * kMoments * ff->vol);
   ff->Lc.ensure(sizeof(float2) * kLocal * ff->vol);
   ff->counters.ensure(2 * sizeof(unsigned long long));
+  const int64_t ldS = pad64(n1);
+  const int64_t nbS = ldS / kTile;
+  ff->mcount.ensure(sizeof(unsigned) * nc);
+  ff->mstart.ensure(sizeof(int) * (nc + 1));
+  ff->segS.ensure(sizeof(double) * F_COUNT * ldS);
+  ff->key.ensure(sizeof(int) * ldS);
+  ff->perm.ensure(sizeof(int) * ldS);
+  ff->box.ensure(sizeof(int) * 6 * nbS);
+  ff->rowCount.ensure(sizeof(unsigned) * nbS);
+  ff->rowStart.ensure(sizeof(int) * (nbS + 1));
+  ff->colCount.ensure(sizeof(unsigned) * nbS);
+  ff->colStart.ensure(sizeof(int) * (nbS + 1));
+  ff->tileIdx.ensure(sizeof(int) * nbS * nbS);
   HIPCHK(hipMemsetAsync(ff->counters.p, 0, 2 * sizeof(unsigned long long), c->st));
   const double* seg = c->seg.as<double>();
   launch_cells(seg, c->resLd, (int)n, ff->g, ff->cellOf.as<int>(), ff->counts.as<unsigned>(),
                ff->cursor.as<unsigned>(), ff->start.as<int>(), ff->tmp.as<int>(),
                ff->order.as<int>(), c->st);
+  NearListDev nl{};
+  nl.cellOf = ff->cellOf.as<int>();
+  nl.counts = ff->counts.as<unsigned>();
+  nl.start = ff->start.as<int>();
+  nl.order = ff->order.as<int>();
+  nl.mrank = ff->mrank.as<int>();
+  nl.rankCell = ff->rankCell.as<int>();
+  nl.mcount = ff->mcount.as<unsigned>();
+  nl.mstart = ff->mstart.as<int>();
+  nl.ldS = ldS;
+  nl.segS = ff->segS.as<double>();
+  nl.key = ff->key.as<int>();
+  nl.perm = ff->perm.as<int>();
+  nl.box = ff->box.as<int>();
+  nl.rowCount = ff->rowCount.as<unsigned>();
+  nl.rowStart = ff->rowStart.as<int>();
+  nl.colCount = ff->colCount.as<unsigned>();
+  nl.colStart = ff->colStart.as<int>();
+  nl.tileIdx = ff->tileIdx.as<int>();
+  // Sorted copy and per-row tile counts; the total sizes the lists.
+  launch_near_list_count(seg, c->resLd, (int)n, ff->g, ff->rn, nl, c->st);
+  int tiles = 0;
+  HIPCHK(hipMemcpyAsync(&tiles, ff->rowStart.as<int>() + nbS, sizeof(int), hipMemcpyDeviceToHost,
+                        c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  ff->nearTiles = tiles;
+  ff->list.ensure(sizeof(int2) * std::max(tiles, 1));
+  ff->colList.ensure(sizeof(int) * std::max(tiles, 1));
+  ff->rowPart.ensure(sizeof(double) * 6 * kTile * std::max(tiles, 1));
+  ff->colPart.ensure(sizeof(double) * 6 * kTile * std::max(tiles, 1));
+  nl.list = ff->list.as<int2>();
+  nl.colList = ff->colList.as<int>();
+  launch_near_list_fill(nl, c->st);
   launch_moments(seg, c->resLd, ff->g, ff->start.as<int>(), ff->order.as<int>(), ff->M.as<float>(),
                  c->st);
   launch_far_field(ff->M.as<float>(), ff->g, ff->What.as<float2>(), ff->tw.as<float2>(),
                    ff->Mc.as<float2>(), ff->Lc.as<float2>(), ff->L.as<float>(), c->st);
   const MixedRule mr = mixed_rule(k, o);
-  launch_near_rows(seg, c->resLd, ff->g, ff->rn, ff->start.as<int>(), ff->order.as<int>(),
-                   seg_consts(k), mr, dev, flags, ff->counters.as<unsigned long long>(), c->st);
+  launch_near_tiles(ff->segS.as<double>(), ldS, ff->key.as<int>(), ff->rn, ff->list.as<int2>(), tiles,
+                    seg_consts(k), mr, ff->rowPart.as<double>(), ff->colPart.as<double>(), flags,
+                    ff->counters.as<unsigned long long>(), c->st);
+  launch_near_reduce(nl, nc, ff->rowPart.as<double>(), ff->colPart.as<double>(), dev, c->st);
   HIPCHK(hipGetLastError());
 }
 
@@ -796,9 +860,17 @@ int ddd_farfield_create(ddd_ctx* c, int G, const double lo[3], double h, int nea
       const double a = -2.0 * 3.14159265358979323846 * k / ff->NP;
       tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
     }
+    std::vector<int> mrank, rankCell;
+    morton_ranks(G, mrank, rankCell);
+    const int nc = G * G * G;
     try {
       ff->tw.ensure(sizeof(float2) * ff->NP);
       HIPCHK(hipMemcpyAsync(ff->tw.p, tw.data(), sizeof(float2) * ff->NP, hipMemcpyHostToDevice, c->st));
+      ff->mrank.ensure(sizeof(int) * nc);
+      ff->rankCell.ensure(sizeof(int) * nc);
+      HIPCHK(hipMemcpyAsync(ff->mrank.p, mrank.data(), sizeof(int) * nc, hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipMemcpyAsync(ff->rankCell.p, rankCell.data(), sizeof(int) * nc, hipMemcpyHostToDevice,
+                            c->st));
       HIPCHK(hipStreamSynchronize(c->st));
     } catch (...) {
       ff->release();
@@ -869,7 +941,7 @@ const float* pp(ddd_mlp* m, int64_t off) { return m->P.as<float>() + off; }
 
 // Y = MLP(features of off3) for ns samples (off3 already on the device).
 void mlp_forward(ddd_mlp* m, int ns, hipStream_t st) {
-  mlp_features(m->off3.as<int>(), ns, m->X0.as<unsigned short>(), st);
+  mlp_features(m->off3.as<int>(), ns, m->aH, m->X0.as<unsigned short>(), st);
   mlp_gemm(m->X0.as<unsigned short>(), kMlpIn, 1, wb(m, kOffW1), kHidden, 1, m->Z1.as<float>(),
            kHidden, ns, kHidden, kMlpIn, 0, st);
   mlp_bias_act(m->Z1.as<float>(), pp(m, kOffB1), ns, kHidden, 1, m->H1.as<unsigned short>(), st);
@@ -893,23 +965,23 @@ void mlp_backward(ddd_mlp* m, int ns, hipStream_t st) {
   mlp_to_bf16(m->dY.as<float>(), (int64_t)ns * kMlpOutPad, dYb, st);
   mlp_gemm(m->H3.as<unsigned short>(), 1, kHidden, dYb, kMlpOutPad, 1, G + kOffW4, kMlpOutPad,
            kHidden, kMlpOutPad, ns, 0, st);
-  mlp_colsum(dYb, ns, kMlpOutPad, G + kOffB4, st);
+  mlp_colsum(dYb, ns, kMlpOutPad, m->colPart.as<float>(), G + kOffB4, st);
   mlp_gemm(dYb, kMlpOutPad, 1, wb(m, kOffW4), 1, kMlpOutPad, dH, kHidden, ns, kHidden, kMlpOutPad,
            0, st);
   mlp_act_grad(dH, m->Z3.as<float>(), (int64_t)ns * kHidden, dZ, st);
   mlp_gemm(m->H2.as<unsigned short>(), 1, kHidden, dZ, kHidden, 1, G + kOffW3, kHidden, kHidden,
            kHidden, ns, 0, st);
-  mlp_colsum(dZ, ns, kHidden, G + kOffB3, st);
+  mlp_colsum(dZ, ns, kHidden, m->colPart.as<float>(), G + kOffB3, st);
   mlp_gemm(dZ, kHidden, 1, wb(m, kOffW3), 1, kHidden, dH, kHidden, ns, kHidden, kHidden, 0, st);
   mlp_act_grad(dH, m->Z2.as<float>(), (int64_t)ns * kHidden, dZ, st);
   mlp_gemm(m->H1.as<unsigned short>(), 1, kHidden, dZ, kHidden, 1, G + kOffW2, kHidden, kHidden,
            kHidden, ns, 0, st);
-  mlp_colsum(dZ, ns, kHidden, G + kOffB2, st);
+  mlp_colsum(dZ, ns, kHidden, m->colPart.as<float>(), G + kOffB2, st);
   mlp_gemm(dZ, kHidden, 1, wb(m, kOffW2), 1, kHidden, dH, kHidden, ns, kHidden, kHidden, 0, st);
   mlp_act_grad(dH, m->Z1.as<float>(), (int64_t)ns * kHidden, dZ, st);
   mlp_gemm(m->X0.as<unsigned short>(), 1, kMlpIn, dZ, kHidden, 1, G + kOffW1, kHidden, kMlpIn,
            kHidden, ns, 0, st);
-  mlp_colsum(dZ, ns, kHidden, G + kOffB1, st);
+  mlp_colsum(dZ, ns, kHidden, m->colPart.as<float>(), G + kOffB1, st);
 }
 
 void mlp_refresh_bf16(ddd_mlp* m, hipStream_t st) {
@@ -920,6 +992,7 @@ float env_scale(const ddd_consts& k, const ddd_farfield* ff) { return (float)(k.
 
 void operator_from_mlp(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_consts& k) {
   const int G = ff->g.G, S = 2 * G - 1;
+  m->aH = (float)(k.coreRad / ff->g.h);
   const int no = S * S * S;
   std::vector<int> off((size_t)3 * no);
   for (int z = 0; z < S; ++z)
@@ -1012,27 +1085,25 @@ int ddd_mlp_params(ddd_ctx* c, ddd_mlp* m, float* get, const float* set) {
 }
 
 int ddd_mlp_loss_grad(ddd_ctx* c, ddd_mlp* m, int ns, const int* off3, const float* M,
-                      const float* P, const float* sigma, const float* den, float envScale,
-                      double* loss, float* grad) {
+                      const float* Lt, const float* den, float envScale, double* loss,
+                      float* grad) {
   return guard(c, "ddd_mlp_loss_grad", [&] {
-    if (!m || !off3 || !M || !P || !sigma || !den || !loss || !grad) bad("NULL argument");
+    if (!m || !off3 || !M || !Lt || !den || !loss || !grad) bad("NULL argument");
     if (ns < 1 || ns > (1 << 20)) bad("ns out of range");
     for (int s = 0; s < ns; ++s)
       if (off3[3 * s] == 0 && off3[3 * s + 1] == 0 && off3[3 * s + 2] == 0) bad("zero offset");
     m->ensure_batch(ns);
-    ScopedBuf dM, dP, dS, dD;
+    ScopedBuf dM, dL, dD;
     dM.ensure(sizeof(float) * kMoments * ns);
-    dP.ensure(sizeof(float) * kPoly * ns);
-    dS.ensure(sizeof(float) * 6 * ns);
+    dL.ensure(sizeof(float) * kLocal * ns);
     dD.ensure(sizeof(float) * ns);
     HIPCHK(hipMemcpyAsync(m->off3.p, off3, sizeof(int) * 3 * ns, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(dM.p, M, sizeof(float) * kMoments * ns, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(dP.p, P, sizeof(float) * kPoly * ns, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(dS.p, sigma, sizeof(float) * 6 * ns, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dL.p, Lt, sizeof(float) * kLocal * ns, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(dD.p, den, sizeof(float) * ns, hipMemcpyHostToDevice, c->st));
     mlp_forward(m, ns, c->st);
-    mlp_contract(m->Y.as<float>(), m->off3.as<int>(), dM.as<float>(), dP.as<float>(), dS.as<float>(),
-                 dD.as<float>(), ns, envScale, m->dY.as<float>(), m->lossPer.as<float>(), c->st);
+    mlp_contract(m->Y.as<float>(), m->off3.as<int>(), dM.as<float>(), dL.as<float>(), dD.as<float>(),
+                 ns, envScale, m->dY.as<float>(), m->lossPer.as<float>(), c->st);
     mlp_backward(m, ns, c->st);
     std::vector<float> lp((size_t)ns);
     HIPCHK(hipMemcpyAsync(lp.data(), m->lossPer.p, sizeof(float) * ns, hipMemcpyDeviceToHost, c->st));
@@ -1051,6 +1122,75 @@ int ddd_farfield_set_operator_mlp(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, cons
   });
 }
 
+namespace {
+
+// Pack net into seg, bin it on ff's grid and form its cell moments (in ff's
+// buffers); returns the leading dimension.
+int64_t bin_network(ddd_ctx* c, ddd_farfield* ff, const ddd_network* net, DevBuf& seg) {
+  const int nc = ff->g.G * ff->g.G * ff->g.G;
+  const int64_t n = net->numSeg;
+  const int64_t ld = pack_segments(c, net, seg);
+  ff->cellOf.ensure(sizeof(int) * std::max<int64_t>(n, 1));
+  ff->tmp.ensure(sizeof(int) * std::max<int64_t>(n, 1));
+  ff->order.ensure(sizeof(int) * std::max<int64_t>(n, 1));
+  ff->counts.ensure(sizeof(unsigned) * nc);
+  ff->cursor.ensure(sizeof(unsigned) * nc);
+  ff->start.ensure(sizeof(int) * (nc + 1));
+  ff->M.ensure(sizeof(float) * kMoments * nc);
+  launch_cells(seg.as<double>(), ld, (int)n, ff->g, ff->cellOf.as<int>(), ff->counts.as<unsigned>(),
+               ff->cursor.as<unsigned>(), ff->start.as<int>(), ff->tmp.as<int>(), ff->order.as<int>(),
+               c->st);
+  launch_moments(seg.as<double>(), ld, ff->g, ff->start.as<int>(), ff->order.as<int>(),
+                 ff->M.as<float>(), c->st);
+  HIPCHK(hipGetLastError());
+  return ld;
+}
+
+// The 27-point tensor Gauss rule on the unit cell and the L2 projection onto
+// the quadratic basis it defines (exact for the basis products).
+Projection cell_projection() {
+  Projection pj{};
+  const double r = std::sqrt(0.6) / 2;
+  const double xi[3] = {-r, 0.0, r}, w[3] = {5.0 / 18, 8.0 / 18, 5.0 / 18};
+  for (int a = 0; a < 3; ++a) pj.xi[a] = xi[a];
+  double B[27][kPoly], W[27];
+  for (int p = 0; p < 27; ++p) {
+    const double x = xi[p % 3], y = xi[(p / 3) % 3], z = xi[p / 9];
+    const double v[kPoly] = {1, x, y, z, x * x, y * y, z * z, x * y, x * z, y * z};
+    for (int l = 0; l < kPoly; ++l) B[p][l] = v[l];
+    W[p] = w[p % 3] * w[(p / 3) % 3] * w[p / 9];
+  }
+  // Gram matrix and its inverse (Gauss-Jordan, symmetric positive definite).
+  double Gm[kPoly][2 * kPoly] = {};
+  for (int i = 0; i < kPoly; ++i) {
+    for (int j = 0; j < kPoly; ++j)
+      for (int p = 0; p < 27; ++p) Gm[i][j] += W[p] * B[p][i] * B[p][j];
+    Gm[i][kPoly + i] = 1.0;
+  }
+  for (int col = 0; col < kPoly; ++col) {
+    int piv = col;
+    for (int i = col + 1; i < kPoly; ++i)
+      if (std::fabs(Gm[i][col]) > std::fabs(Gm[piv][col])) piv = i;
+    for (int j = 0; j < 2 * kPoly; ++j) std::swap(Gm[col][j], Gm[piv][j]);
+    const double d = Gm[col][col];
+    for (int j = 0; j < 2 * kPoly; ++j) Gm[col][j] /= d;
+    for (int i = 0; i < kPoly; ++i) {
+      if (i == col) continue;
+      const double f = Gm[i][col];
+      for (int j = 0; j < 2 * kPoly; ++j) Gm[i][j] -= f * Gm[col][j];
+    }
+  }
+  for (int l = 0; l < kPoly; ++l)
+    for (int p = 0; p < 27; ++p) {
+      double a = 0;
+      for (int j = 0; j < kPoly; ++j) a += Gm[l][kPoly + j] * B[p][j] * W[p];
+      pj.P[l][p] = a;
+    }
+  return pj;
+}
+
+}  // namespace
+
 int ddd_farfield_train(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_consts* k,
                        const ddd_network* net, const ddd_train_opts* o, ddd_train_report* rep) {
   return guard(c, "ddd_farfield_train", [&] {
@@ -1058,129 +1198,88 @@ int ddd_farfield_train(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_const
     check_net(net);
     if (o->samples < 1 || o->steps < 1 || o->batch < 1 || o->batch > (1 << 16) || o->valSamples < 0)
       bad("bad training options");
+    const int64_t NS = o->samples + o->valSamples;
+    if (NS > INT32_MAX / kMoments) bad("too many samples");
     const auto t0 = std::chrono::steady_clock::now();
     const int G = ff->g.G, rn = ff->rn;
+    if (rn + 1 > G - 1) bad("no far offsets on this grid");
+    m->aH = (float)(k->coreRad / ff->g.h);
     const int nc = G * G * G;
     const int64_t n = net->numSeg;
-    // Training network: segments, cells, moments (its own buffers).
-    upload_segments(c, net);
-    c->resN = -1;  // the segment buffer now holds the training network
-    ff->cellOf.ensure(sizeof(int) * std::max<int64_t>(n, 1));
-    ff->tmp.ensure(sizeof(int) * std::max<int64_t>(n, 1));
-    ff->order.ensure(sizeof(int) * std::max<int64_t>(n, 1));
-    ff->counts.ensure(sizeof(unsigned) * nc);
-    ff->cursor.ensure(sizeof(unsigned) * nc);
-    ff->start.ensure(sizeof(int) * (nc + 1));
-    ff->M.ensure(sizeof(float) * kMoments * nc);
-    launch_cells(c->seg.as<double>(), c->resLd, (int)n, ff->g, ff->cellOf.as<int>(),
-                 ff->counts.as<unsigned>(), ff->cursor.as<unsigned>(), ff->start.as<int>(),
-                 ff->tmp.as<int>(), ff->order.as<int>(), c->st);
-    launch_moments(c->seg.as<double>(), c->resLd, ff->g, ff->start.as<int>(), ff->order.as<int>(),
-                   ff->M.as<float>(), c->st);
-    std::vector<int> start(nc + 1), order((size_t)std::max<int64_t>(n, 1));
+    // Training network: its own segment buffer (an uploaded network stays);
+    // cells and moments in the farfield's buffers, which the next hybrid
+    // evaluation recomputes.
+    ScopedBuf tseg;
+    const int64_t tld = bin_network(c, ff, net, tseg);
+    std::vector<int> start(nc + 1);
     std::vector<float> Mh((size_t)kMoments * nc);
     HIPCHK(hipMemcpyAsync(start.data(), ff->start.p, sizeof(int) * (nc + 1), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipMemcpyAsync(order.data(), ff->order.p, sizeof(int) * n, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipMemcpyAsync(Mh.data(), ff->M.p, sizeof(float) * kMoments * nc, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     std::vector<int> live;
     for (int cc = 0; cc < nc; ++cc)
       if (start[cc + 1] > start[cc]) live.push_back(cc);
-    if (live.size() < 2) bad("training network occupies fewer than two cells");
-    // Samples: target point = a Gauss point of a random segment of a random
-    // occupied cell; source = a random occupied far cell, half of them from
-    // the first three shells outside the near stencil.
-    const int64_t NS = o->samples + o->valSamples;
-    if (NS > INT32_MAX / kMoments) bad("too many samples");
+    if (live.empty()) bad("training network has no binned segments");
+    // Samples: an occupied source cell and an offset outside the near
+    // stencil, a share nearFrac of them within nearShells shells of it.
+    const double nearFrac = o->nearFrac > 0 ? o->nearFrac : 0.5;
+    const int nearShells = o->nearShells > 0 ? o->nearShells : 3;
+    const double denFloor = o->denFloor > 0 ? o->denFloor : 1e-2;
     std::vector<int> off((size_t)3 * NS), src((size_t)NS);
-    std::vector<float> Ps((size_t)kPoly * NS), Ms((size_t)kMoments * NS);
-    std::vector<double> pts((size_t)3 * NS);
     std::mt19937_64 rng(o->seed);
     std::uniform_int_distribution<size_t> pickLive(0, live.size() - 1);
-    int64_t tries = 0;
-    const double gq[2] = {0.21132486540518711775, 0.78867513459481288225};
-    const int64_t L = net->segIdxLd;
+    std::uniform_real_distribution<double> u01(0.0, 1.0);
+    const int Rnear = std::min(G - 1, rn + nearShells);
     for (int64_t s = 0; s < NS; ++s) {
-      int t, sc, ox, oy, oz;
-      for (;;) {
-        if (++tries > 1000 * (NS + 1000)) bad("training network has too few far cell pairs");
-        t = live[pickLive(rng)];
-        const int tx = t % G, ty = (t / G) % G, tz = t / (G * G);
-        if (rng() & 1) {
-          sc = live[pickLive(rng)];
-        } else {
-          const int R = rn + 1 + (int)(rng() % 3);
-          const int sx = tx + (int)(rng() % (2 * R + 1)) - R, sy = ty + (int)(rng() % (2 * R + 1)) - R,
-                    sz = tz + (int)(rng() % (2 * R + 1)) - R;
-          if (sx < 0 || sy < 0 || sz < 0 || sx >= G || sy >= G || sz >= G) continue;
-          sc = sx + G * (sy + G * sz);
-          if (start[sc + 1] == start[sc]) continue;
-        }
-        ox = sc % G - tx;
-        oy = (sc / G) % G - ty;
-        oz = sc / (G * G) - tz;
-        if (std::max(std::abs(ox), std::max(std::abs(oy), std::abs(oz))) > rn) break;
-      }
-      const int e = start[t] + (int)(rng() % (uint64_t)(start[t + 1] - start[t]));
-      const int i = order[e];
-      const double* a = net->coord + 3 * (net->segIdx[i + L] - 1);
-      const double* b = net->coord + 3 * (net->segIdx[i + 2 * L] - 1);
-      const double u = gq[rng() & 1];
-      const double x[3] = {a[0] + u * (b[0] - a[0]), a[1] + u * (b[1] - a[1]), a[2] + u * (b[2] - a[2])};
-      const int tx = t % G, ty = (t / G) % G, tz = t / (G * G);
-      const double rho[3] = {(x[0] - (ff->g.lo[0] + (tx + 0.5) * ff->g.h)) / ff->g.h,
-                             (x[1] - (ff->g.lo[1] + (ty + 0.5) * ff->g.h)) / ff->g.h,
-                             (x[2] - (ff->g.lo[2] + (tz + 0.5) * ff->g.h)) / ff->g.h};
-      float* P = &Ps[(size_t)kPoly * s];
-      P[0] = 1.f; P[1] = (float)rho[0]; P[2] = (float)rho[1]; P[3] = (float)rho[2];
-      P[4] = (float)(rho[0] * rho[0]); P[5] = (float)(rho[1] * rho[1]); P[6] = (float)(rho[2] * rho[2]);
-      P[7] = (float)(rho[0] * rho[1]); P[8] = (float)(rho[0] * rho[2]); P[9] = (float)(rho[1] * rho[2]);
-      for (int f = 0; f < kMoments; ++f) Ms[(size_t)kMoments * s + f] = Mh[(size_t)f * nc + sc];
-      off[3 * s] = ox; off[3 * s + 1] = oy; off[3 * s + 2] = oz;
-      src[s] = sc;
-      for (int q = 0; q < 3; ++q) pts[3 * s + q] = x[q];
+      const int R = (u01(rng) < nearFrac) ? Rnear : G - 1;
+      std::uniform_int_distribution<int> pick(-R, R);
+      int ox, oy, oz;
+      do {
+        ox = pick(rng);
+        oy = pick(rng);
+        oz = pick(rng);
+      } while (std::max(std::abs(ox), std::max(std::abs(oy), std::abs(oz))) <= rn);
+      off[3 * s] = ox;
+      off[3 * s + 1] = oy;
+      off[3 * s + 2] = oz;
+      src[s] = live[pickLive(rng)];
     }
-    // Exact stresses.
-    ScopedBuf dSrc, dPts, dSig, dOff, dM, dP, dDen;
+    // Exact local expansions (FP64 stresses, projected), on the device.
+    ScopedBuf dSrc, dOff, dLt, dDen;
     dSrc.ensure(sizeof(int) * NS);
-    dPts.ensure(sizeof(double) * 3 * NS);
-    dSig.ensure(sizeof(float) * 6 * NS);
+    dOff.ensure(sizeof(int) * 3 * NS);
+    dLt.ensure(sizeof(float) * kLocal * NS);
+    dDen.ensure(sizeof(float) * NS);
     HIPCHK(hipMemcpyAsync(dSrc.p, src.data(), sizeof(int) * NS, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(dPts.p, pts.data(), sizeof(double) * 3 * NS, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dOff.p, off.data(), sizeof(int) * 3 * NS, hipMemcpyHostToDevice, c->st));
     const SigK<double> SK{k->coreRadSq, k->mu4pi, k->mu4pinu, k->nuMu4pinu, k->coreRadSq * k->mu8pi};
-    sample_stress(c->seg.as<double>(), c->resLd, ff->start.as<int>(), ff->order.as<int>(),
-                  dSrc.as<int>(), dPts.as<double>(), (int)NS, SK, dSig.as<float>(), c->st);
+    local_targets(tseg.as<double>(), tld, ff->g, ff->start.as<int>(), ff->order.as<int>(),
+                  dSrc.as<int>(), dOff.as<int>(), (int)NS, SK, cell_projection(), dLt.as<float>(), c->st);
     HIPCHK(hipGetLastError());
-    std::vector<float> sig((size_t)6 * NS), den((size_t)NS);
-    HIPCHK(hipMemcpyAsync(sig.data(), dSig.p, sizeof(float) * 6 * NS, hipMemcpyDeviceToHost, c->st));
+    std::vector<float> Lt((size_t)kLocal * NS), den((size_t)NS);
+    HIPCHK(hipMemcpyAsync(Lt.data(), dLt.p, sizeof(float) * kLocal * NS, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    // den = |L|^2 + (denFloor * typical |L| at that distance)^2, the typical
+    // size being envScale |o|^-2 |M0| (order-0 moments of the source).
     const float es = env_scale(*k, ff);
     for (int64_t s = 0; s < NS; ++s) {
       double a = 0;
-      for (int q = 0; q < 6; ++q) a += (double)sig[6 * s + q] * sig[6 * s + q];
+      for (int q = 0; q < kLocal; ++q) a += (double)Lt[kLocal * s + q] * Lt[kLocal * s + q];
       const double R2 = (double)off[3 * s] * off[3 * s] + (double)off[3 * s + 1] * off[3 * s + 1] +
                         (double)off[3 * s + 2] * off[3 * s + 2];
       double m0 = 0;
-      for (int f = 0; f < 9; ++f) m0 += (double)Ms[(size_t)kMoments * s + f] * Ms[(size_t)kMoments * s + f];
-      const double fl = 1e-2 * es / R2 * std::sqrt(m0);
+      for (int f = 0; f < 9; ++f) m0 += (double)Mh[(size_t)f * nc + src[s]] * Mh[(size_t)f * nc + src[s]];
+      const double fl = denFloor * es / R2 * std::sqrt(m0);
       den[s] = (float)(a + fl * fl + 1e-30);
     }
-    dOff.ensure(sizeof(int) * 3 * NS);
-    dM.ensure(sizeof(float) * kMoments * NS);
-    dP.ensure(sizeof(float) * kPoly * NS);
-    dDen.ensure(sizeof(float) * NS);
-    HIPCHK(hipMemcpyAsync(dOff.p, off.data(), sizeof(int) * 3 * NS, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(dM.p, Ms.data(), sizeof(float) * kMoments * NS, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(dP.p, Ps.data(), sizeof(float) * kPoly * NS, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(dDen.p, den.data(), sizeof(float) * NS, hipMemcpyHostToDevice, c->st));
     // Training: consecutive batches of a shuffled order of the training set,
     // gathered on the device; a new order each epoch (the tail is dropped).
     const int B = (int)std::min<int64_t>(o->batch, o->samples);
     m->ensure_batch(std::max<int>(B, 1024));
-    ScopedBuf bM, bP, bS, bD, dPerm;
+    ScopedBuf bM, bL, bD, dPerm, vIdx;
     bM.ensure(sizeof(float) * kMoments * B);
-    bP.ensure(sizeof(float) * kPoly * B);
-    bS.ensure(sizeof(float) * 6 * B);
+    bL.ensure(sizeof(float) * kLocal * B);
     bD.ensure(sizeof(float) * B);
     dPerm.ensure(sizeof(int) * o->samples);
     std::vector<int> perm((size_t)o->samples);
@@ -1197,13 +1296,13 @@ int ddd_farfield_train(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_const
                               c->st));
         cursor = 0;
       }
-      gather_batch(dPerm.as<int>() + cursor, B, dOff.as<int>(), dM.as<float>(), dP.as<float>(),
-                   dSig.as<float>(), dDen.as<float>(), m->off3.as<int>(), bM.as<float>(),
-                   bP.as<float>(), bS.as<float>(), bD.as<float>(), c->st);
+      gather_batch(dPerm.as<int>() + cursor, B, dOff.as<int>(), dSrc.as<int>(), ff->M.as<float>(), nc,
+                   dLt.as<float>(), dDen.as<float>(), m->off3.as<int>(), bM.as<float>(),
+                   bL.as<float>(), bD.as<float>(), c->st);
       cursor += B;
       mlp_forward(m, B, c->st);
-      mlp_contract(m->Y.as<float>(), m->off3.as<int>(), bM.as<float>(), bP.as<float>(), bS.as<float>(),
-                   bD.as<float>(), B, es, m->dY.as<float>(), m->lossPer.as<float>(), c->st);
+      mlp_contract(m->Y.as<float>(), m->off3.as<int>(), bM.as<float>(), bL.as<float>(), bD.as<float>(),
+                   B, es, m->dY.as<float>(), m->lossPer.as<float>(), c->st);
       mlp_backward(m, B, c->st);
       ++m->step;
       const double frac = (double)stp / std::max(1, o->steps - 1);
@@ -1222,27 +1321,34 @@ int ddd_farfield_train(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_const
       HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(c->st));
-    // Held-out error: predictions of the trained operator.
-    double num = 0, dnm = 0, numN = 0, dnmN = 0;
+    // Held-out error of the predicted local expansions, overall and per shell.
+    double num = 0, dnm = 0, numN = 0, dnmN = 0, numS[8] = {0}, dnmS[8] = {0};
+    std::vector<int> vi((size_t)B);
+    vIdx.ensure(sizeof(int) * B);
     for (int64_t v0 = o->samples; v0 < NS; v0 += B) {
       const int cnt = (int)std::min<int64_t>(B, NS - v0);
-      HIPCHK(hipMemcpyAsync(m->off3.p, dOff.as<int>() + 3 * v0, sizeof(int) * 3 * cnt,
-                            hipMemcpyDeviceToDevice, c->st));
+      for (int q = 0; q < cnt; ++q) vi[(size_t)q] = (int)(v0 + q);
+      HIPCHK(hipMemcpyAsync(vIdx.p, vi.data(), sizeof(int) * cnt, hipMemcpyHostToDevice, c->st));
+      gather_batch(vIdx.as<int>(), cnt, dOff.as<int>(), dSrc.as<int>(), ff->M.as<float>(), nc,
+                   dLt.as<float>(), dDen.as<float>(), m->off3.as<int>(), bM.as<float>(),
+                   bL.as<float>(), bD.as<float>(), c->st);
       mlp_forward(m, cnt, c->st);
-      mlp_contract(m->Y.as<float>(), m->off3.as<int>(), dM.as<float>() + kMoments * v0,
-                   dP.as<float>() + kPoly * v0, dSig.as<float>() + 6 * v0, dDen.as<float>() + v0,
+      mlp_contract(m->Y.as<float>(), m->off3.as<int>(), bM.as<float>(), bL.as<float>(), bD.as<float>(),
                    cnt, es, m->dY.as<float>(), m->lossPer.as<float>(), c->st);
       HIPCHK(hipMemcpyAsync(lp.data(), m->lossPer.p, sizeof(float) * cnt, hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
       for (int q = 0; q < cnt; ++q) {
         const int64_t s = v0 + q;
         double a = 0;
-        for (int w = 0; w < 6; ++w) a += (double)sig[6 * s + w] * sig[6 * s + w];
-        const double e2 = (double)lp[(size_t)q] * den[s] * cnt;  // |sigma_hat - sigma|^2
+        for (int w = 0; w < kLocal; ++w) a += (double)Lt[kLocal * s + w] * Lt[kLocal * s + w];
+        const double e2 = (double)lp[(size_t)q] * den[s] * cnt;  // |L_hat - L|^2
         num += e2;
         dnm += a;
         const int ch = std::max(std::abs(off[3 * s]), std::max(std::abs(off[3 * s + 1]), std::abs(off[3 * s + 2])));
         if (ch <= rn + 2) { numN += e2; dnmN += a; }
+        const int sh = std::min(7, ch - rn - 1);
+        numS[sh] += e2;
+        dnmS[sh] += a;
       }
     }
     operator_from_mlp(c, ff, m, *k);
@@ -1256,7 +1362,34 @@ int ddd_farfield_train(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_const
       rep->relErrVal = dnm > 0 ? std::sqrt(num / dnm) : 0;
       rep->relErrValNear = dnmN > 0 ? std::sqrt(numN / dnmN) : 0;
       rep->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      for (int q = 0; q < 8; ++q) rep->relErrShell[q] = dnmS[q] > 0 ? std::sqrt(numS[q] / dnmS[q]) : 0;
     }
+  });
+}
+
+int ddd_farfield_local_targets(ddd_ctx* c, ddd_farfield* ff, const ddd_consts* k,
+                               const ddd_network* net, int ns, const int* srcCell,
+                               const int* off3, float* out) {
+  return guard(c, "ddd_farfield_local_targets", [&] {
+    if (!ff || !k || !srcCell || !off3 || !out) bad("NULL argument");
+    check_net(net);
+    if (ns < 1 || ns > (1 << 24)) bad("ns out of range");
+    const int nc = ff->g.G * ff->g.G * ff->g.G;
+    for (int s = 0; s < ns; ++s)
+      if (srcCell[s] < 0 || srcCell[s] >= nc) bad("source cell outside the grid");
+    ScopedBuf tseg, dSrc, dOff, dLt;
+    const int64_t tld = bin_network(c, ff, net, tseg);
+    dSrc.ensure(sizeof(int) * ns);
+    dOff.ensure(sizeof(int) * 3 * ns);
+    dLt.ensure(sizeof(float) * kLocal * ns);
+    HIPCHK(hipMemcpyAsync(dSrc.p, srcCell, sizeof(int) * ns, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(dOff.p, off3, sizeof(int) * 3 * ns, hipMemcpyHostToDevice, c->st));
+    const SigK<double> SK{k->coreRadSq, k->mu4pi, k->mu4pinu, k->nuMu4pinu, k->coreRadSq * k->mu8pi};
+    local_targets(tseg.as<double>(), tld, ff->g, ff->start.as<int>(), ff->order.as<int>(),
+                  dSrc.as<int>(), dOff.as<int>(), ns, SK, cell_projection(), dLt.as<float>(), c->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dLt.p, sizeof(float) * kLocal * ns, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
   });
 }
 

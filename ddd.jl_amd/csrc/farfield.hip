@@ -26,6 +26,9 @@
 #include "kernels.hpp"
 #include "farfield.hpp"
 
+#include <algorithm>
+#include <vector>
+
 namespace ddd {
 
 namespace {
@@ -312,174 +315,176 @@ __global__ void k_far_apply(const double* __restrict__ seg, int64_t ld, int n, G
   o[3] += F2.x; o[4] += F2.y; o[5] += F2.z;
 }
 
-// Near field, rows: one workgroup per target cell.  Lanes hold up to 64
-// target segments; when a cell has fewer, the lanes split into r groups that
-// take every r-th source, and the groups' sums are added in group order.
-// Near-parallel pairs are queued and evaluated 64 at a time.
-constexpr int kNearQ = 128;
+// ---- Near field: cell-sorted blocks and their tile list ------------------
+//
+// Binned segments are copied in Morton order of their cells (ties by
+// segment index) into a sorted SoA copy, cut into blocks of 64.  A block pair
+// (I <= J) is a near tile when the cell boxes of the two blocks are within
+// Chebyshev distance rn; the tile kernel then takes only the pairs whose own
+// cells are.  Lists are built per row block with ballots, so their order is
+// fixed and the reduction below is deterministic.
 
-__global__ __launch_bounds__(64) void k_near_rows(const double* __restrict__ seg, int64_t ld, GridDev g,
-                                                  int rn, const int* __restrict__ start,
-                                                  const int* __restrict__ order, SegK<double> K,
-                                                  SigK<float> KF, double farDist2, double sinMin2,
-                                                  double* __restrict__ out,
-                                                  unsigned* __restrict__ flags,
-                                                  unsigned long long* __restrict__ counters) {
-  const int T = blockIdx.x;
-  const int lane = threadIdx.x;
-  const int t0 = start[T], nT = start[T + 1] - t0;
-  if (nT == 0) return;
-  const int G = g.G;
-  const int tx = T % G, ty = (T / G) % G, tz = T / (G * G);
-  __shared__ double sj[F_COUNT - 1][64];
-  __shared__ int sjIdx[64];
-  __shared__ double red[6][64];
-  __shared__ int qSrc[kNearQ];
-  __shared__ unsigned char qLane[kNearQ];
-  __shared__ double qRes[6][64];
-  unsigned long long nPairs = 0, nExact = 0;
+__device__ __forceinline__ int key_x(int k) { return k & 1023; }
+__device__ __forceinline__ int key_y(int k) { return (k >> 10) & 1023; }
+__device__ __forceinline__ int key_z(int k) { return k >> 20; }
 
-  for (int base = 0; base < nT; base += 64) {
-    const int cnt = min(64, nT - base);
-    int P2 = 1;
-    while (P2 < cnt) P2 <<= 1;
-    const int r = 64 / P2, grp = lane / P2, ti = lane % P2;
-    const bool valid = ti < cnt;
-    const int i = valid ? order[t0 + base + ti] : -1;
-    const Seg<double> A = valid ? seg_from(seg, ld, i) : seg_from(seg, ld, order[t0]);
-    const v3<double> midA = V(0.5 * (A.n1.x + A.n2.x), 0.5 * (A.n1.y + A.n2.y), 0.5 * (A.n1.z + A.n2.z));
-    v3<double> a1 = V(0.0, 0.0, 0.0), a2 = V(0.0, 0.0, 0.0);
-    int nq = 0;
-    int q = 0;  // running source counter (uniform)
-    const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+__global__ void k_morton_counts(const unsigned* __restrict__ counts, const int* __restrict__ rankCell,
+                                int nc, unsigned* __restrict__ mcount) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nc) mcount[r] = counts[rankCell[r]];
+}
 
-    auto flush = [&](int cntq) {
-      if (lane < cntq) {
-        const int ln = qLane[lane];
-        const int srcI = qSrc[lane];
-        const Seg<double> TA = seg_from(seg, ld, order[t0 + base + (ln % P2)]);
-        const Seg<double> SB = seg_from(seg, ld, srcI);
-        v3<double> F1, F2, F3, F4;
-        special_pair_dev(K, TA, SB, F1, F2, F3, F4, flags);
-        qRes[0][lane] = F1.x; qRes[1][lane] = F1.y; qRes[2][lane] = F1.z;
-        qRes[3][lane] = F2.x; qRes[4][lane] = F2.y; qRes[5][lane] = F2.z;
-      }
-      __syncthreads();
-      for (int e = 0; e < cntq; ++e) {
-        if (qLane[e] == lane) {
-          a1 = a1 + V(qRes[0][e], qRes[1][e], qRes[2][e]);
-          a2 = a2 + V(qRes[3][e], qRes[4][e], qRes[5][e]);
-        }
-      }
-      __syncthreads();
-    };
-
-    for (int dz = -rn; dz <= rn; ++dz) {
-      const int sz = tz + dz;
-      if (sz < 0 || sz >= G) continue;
-      for (int dy = -rn; dy <= rn; ++dy) {
-        const int sy = ty + dy;
-        if (sy < 0 || sy >= G) continue;
-        for (int dx = -rn; dx <= rn; ++dx) {
-          const int sx = tx + dx;
-          if (sx < 0 || sx >= G) continue;
-          const int S = sx + G * (sy + G * sz);
-          const int s0 = start[S], ns = start[S + 1] - s0;
-          for (int sb = 0; sb < ns; sb += 64) {
-            const int m = min(64, ns - sb);
-            __syncthreads();
-            if (lane < m) {
-              const int j = order[s0 + sb + lane];
-              sjIdx[lane] = j;
+__global__ void k_sort_gather(const double* __restrict__ seg, int64_t ld, int n,
+                              const int* __restrict__ cellOf, const int* __restrict__ start,
+                              const int* __restrict__ order, const int* __restrict__ mrank,
+                              const int* __restrict__ mstart, int G, int64_t ldS,
+                              double* __restrict__ segS, int* __restrict__ key,
+                              int* __restrict__ perm) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n || e >= start[G * G * G]) return;
+  const int i = order[e];
+  const int c = cellOf[i];
+  const int p = mstart[mrank[c]] + (e - start[c]);
+  perm[p] = i;
 #pragma unroll
-              for (int f = 0; f < F_COUNT - 1; ++f) sj[f][lane] = seg[f * ld + j];
-            }
-            __syncthreads();
-            for (int e = 0; e < m; ++e) {
-              const bool mine = valid && ((q + e) % r == grp) && sjIdx[e] != i;
-              Seg<double> B;
-              B.n1 = V(sj[F_N1X][e], sj[F_N1Y][e], sj[F_N1Z][e]);
-              B.n2 = V(sj[F_N2X][e], sj[F_N2Y][e], sj[F_N2Z][e]);
-              B.b = V(sj[F_BX][e], sj[F_BY][e], sj[F_BZ][e]);
-              B.t = V(sj[F_TX][e], sj[F_TY][e], sj[F_TZ][e]);
-              B.tN = sj[F_TN][e];
-              double c;
-              const double omc = one_minus_c2(A.t, B.t, c);
-              const double dx2 = 0.5 * (B.n1.x + B.n2.x) - midA.x, dy2 = 0.5 * (B.n1.y + B.n2.y) - midA.y,
-                           dz2 = 0.5 * (B.n1.z + B.n2.z) - midA.z;
-              const bool far = (dx2 * dx2 + dy2 * dy2 + dz2 * dz2 > farDist2) && (omc > sinMin2);
-              const bool special = mine && !far && !(omc > DDD_SQRT_EPS);
-              nPairs += mine;
-              nExact += mine && !far;
-              if (mine && far) {
-                v3<double> F1, F2;
-                far_side<float>(KF, A, B, F1, F2);
-                a1 = a1 + F1;
-                a2 = a2 + F2;
-              } else if (mine && !special) {
-                v3<double> F1, F2, F3, F4;
-                pair_generic<double, true, false>(K, A, B, c, omc, F1, F2, F3, F4);
-                a1 = a1 + F1;
-                a2 = a2 + F2;
-              }
-              const uint64_t mk = __ballot(special);
-              if (mk) {
-                if (special) {
-                  const int pos = nq + __popcll(mk & ltMask);
-                  qSrc[pos] = sjIdx[e];
-                  qLane[pos] = (unsigned char)lane;
-                }
-                nq += __popcll(mk);
-              }
-              if (nq >= 64) {
-                __syncthreads();
-                flush(64);
-                const int rest = nq - 64;
-                int ks = 0;
-                unsigned char kl = 0;
-                if (lane < rest) { ks = qSrc[64 + lane]; kl = qLane[64 + lane]; }
-                __syncthreads();
-                if (lane < rest) { qSrc[lane] = ks; qLane[lane] = kl; }
-                __syncthreads();
-                nq = rest;
-              }
-            }
-            q += m;
-          }
-        }
-      }
+  for (int f = 0; f < F_COUNT; ++f) segS[f * ldS + p] = seg[f * ld + i];
+  key[p] = (c % G) | (((c / G) % G) << 10) | ((c / (G * G)) << 20);
+}
+
+// Positions past the binned segments: inert unit segments marked external.
+__global__ void k_sort_pad(const int* __restrict__ start, int nc, int64_t ldS,
+                           double* __restrict__ segS, int* __restrict__ key, int* __restrict__ perm) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= ldS || p < start[nc]) return;
+  for (int f = 0; f < F_COUNT; ++f) segS[f * ldS + p] = 0.0;
+  segS[F_N2X * ldS + p] = 1.0;
+  segS[F_TX * ldS + p] = 1.0;
+  segS[F_TN * ldS + p] = 1.0;
+  segS[F_EXT * ldS + p] = 1.0;
+  key[p] = 0;
+  perm[p] = -1;
+}
+
+// Cell box of each block (lo > hi for an empty block).
+__global__ void k_block_boxes(const int* __restrict__ key, const int* __restrict__ start, int nc,
+                              int nbS, int* __restrict__ box) {
+  const int B = blockIdx.x * blockDim.x + threadIdx.x;
+  if (B >= nbS) return;
+  const int nbin = start[nc];
+  int lo[3] = {1 << 20, 1 << 20, 1 << 20}, hi[3] = {-1, -1, -1};
+  const int p1 = min(nbin, 64 * (B + 1));
+  for (int p = 64 * B; p < p1; ++p) {
+    const int k = key[p];
+    const int x[3] = {key_x(k), key_y(k), key_z(k)};
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = min(lo[a], x[a]);
+      hi[a] = max(hi[a], x[a]);
     }
-    __syncthreads();
-    if (nq > 0) flush(nq);
-    // Group sums, in group order, then one write per target.
-    for (int gi = 0; gi < r; ++gi) {
-      if (grp == gi) {
-        if (gi == 0) {
-          red[0][ti] = a1.x; red[1][ti] = a1.y; red[2][ti] = a1.z;
-          red[3][ti] = a2.x; red[4][ti] = a2.y; red[5][ti] = a2.z;
-        } else {
-          red[0][ti] += a1.x; red[1][ti] += a1.y; red[2][ti] += a1.z;
-          red[3][ti] += a2.x; red[4][ti] += a2.y; red[5][ti] += a2.z;
-        }
-      }
-      __syncthreads();
-    }
-    if (grp == 0 && valid) {
-      double* o = out + 6 * (int64_t)i;
+  }
+  for (int a = 0; a < 3; ++a) {
+    box[6 * B + a] = lo[a];
+    box[6 * B + 3 + a] = hi[a];
+  }
+}
+
+__device__ __forceinline__ bool blocks_near(const int* __restrict__ box, int I, int J, int rn) {
+  bool ok = true;
 #pragma unroll
-      for (int cc = 0; cc < 6; ++cc) o[cc] += red[cc][ti];
+  for (int a = 0; a < 3; ++a) {
+    const int loI = box[6 * I + a], hiI = box[6 * I + 3 + a];
+    const int loJ = box[6 * J + a], hiJ = box[6 * J + 3 + a];
+    ok = ok && loI <= hiI && loJ <= hiJ && (loJ - hiI) <= rn && (loI - hiJ) <= rn;
+  }
+  return ok;
+}
+
+// One wavefront per row block I: number of near J >= I.
+__global__ __launch_bounds__(64) void k_near_count_rows(const int* __restrict__ box, int nbS, int rn,
+                                                       unsigned* __restrict__ rowCount) {
+  const int I = blockIdx.x, lane = threadIdx.x;
+  unsigned cnt = 0;
+  for (int J0 = I; J0 < nbS; J0 += 64) {
+    const int J = J0 + lane;
+    cnt += (unsigned)__popcll(__ballot(J < nbS && blocks_near(box, I, J, rn)));
+  }
+  if (lane == 0) rowCount[I] = cnt;
+}
+
+// Tile list in row-block order; tileIdx[I][J] = tile number or -1 (J >= I).
+__global__ __launch_bounds__(64) void k_near_fill_rows(const int* __restrict__ box, int nbS, int rn,
+                                                      const int* __restrict__ rowStart,
+                                                      int2* __restrict__ list,
+                                                      int* __restrict__ tileIdx) {
+  const int I = blockIdx.x, lane = threadIdx.x;
+  const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int pos = rowStart[I];
+  for (int J0 = I; J0 < nbS; J0 += 64) {
+    const int J = J0 + lane;
+    const bool nr = J < nbS && blocks_near(box, I, J, rn);
+    const uint64_t m = __ballot(nr);
+    if (J < nbS) {
+      int t = -1;
+      if (nr) {
+        t = pos + __popcll(m & ltMask);
+        list[t] = make_int2(I, J);
+      }
+      tileIdx[(int64_t)I * nbS + J] = t;
     }
-    __syncthreads();
+    pos += __popcll(m);
   }
-  // pair statistics (atomics on integer counters only)
-  for (int off = 32; off >= 1; off >>= 1) {
-    nPairs += __shfl_xor(nPairs, off, 64);
-    nExact += __shfl_xor(nExact, off, 64);
+}
+
+// One wavefront per column block J: number of near I <= J ...
+__global__ __launch_bounds__(64) void k_near_count_cols(const int* __restrict__ tileIdx, int nbS,
+                                                       unsigned* __restrict__ colCount) {
+  const int J = blockIdx.x, lane = threadIdx.x;
+  unsigned cnt = 0;
+  for (int I0 = 0; I0 <= J; I0 += 64) {
+    const int I = I0 + lane;
+    cnt += (unsigned)__popcll(__ballot(I <= J && tileIdx[(int64_t)I * nbS + J] >= 0));
   }
-  if (lane == 0) {
-    atomicAdd(&counters[0], nPairs);
-    atomicAdd(&counters[1], nExact);
+  if (lane == 0) colCount[J] = cnt;
+}
+
+// ... and their tile numbers in increasing I.
+__global__ __launch_bounds__(64) void k_near_fill_cols(const int* __restrict__ tileIdx, int nbS,
+                                                      const int* __restrict__ colStart,
+                                                      int* __restrict__ colList) {
+  const int J = blockIdx.x, lane = threadIdx.x;
+  const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int pos = colStart[J];
+  for (int I0 = 0; I0 <= J; I0 += 64) {
+    const int I = I0 + lane;
+    const int t = I <= J ? tileIdx[(int64_t)I * nbS + J] : -1;
+    const uint64_t m = __ballot(t >= 0);
+    if (t >= 0) colList[pos + __popcll(m & ltMask)] = t;
+    pos += __popcll(m);
   }
+}
+
+// Sum of a sorted segment's tile partials (row role, then column role, in
+// list order) into out at its original index.
+__global__ void k_near_reduce(const double* __restrict__ rowPart, const double* __restrict__ colPart,
+                              const int* __restrict__ rowStart, const int* __restrict__ colStart,
+                              const int* __restrict__ colList, const int* __restrict__ perm,
+                              const int* __restrict__ start, int nc, double* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= start[nc]) return;
+  const int B = p / kTile, l = p % kTile;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int t = rowStart[B]; t < rowStart[B + 1]; ++t) {
+    const double* r = rowPart + (int64_t)t * 6 * kTile + l;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[c] += r[c * kTile];
+  }
+  for (int q = colStart[B]; q < colStart[B + 1]; ++q) {
+    const double* r = colPart + (int64_t)colList[q] * 6 * kTile + l;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[c] += r[c * kTile];
+  }
+  double* o = out + 6 * (int64_t)perm[p];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) o[c] += acc[c];
 }
 
 }  // namespace
@@ -562,12 +567,60 @@ void launch_far_apply(const double* seg, int64_t ld, int n, const GridDev& g, co
                      out);
 }
 
-void launch_near_rows(const double* seg, int64_t ld, const GridDev& g, int rn, const int* start,
-                      const int* order, const SegK<double>& K, const MixedRule& mr, double* out,
-                      unsigned* flags, unsigned long long* counters, hipStream_t st) {
+void morton_ranks(int G, std::vector<int>& mrank, std::vector<int>& rankCell) {
+  const int nc = G * G * G;
+  auto spread = [](uint64_t x) {
+    uint64_t r = 0;
+    for (int b = 0; b < 10; ++b) r |= ((x >> b) & 1ull) << (3 * b);
+    return r;
+  };
+  std::vector<std::pair<uint64_t, int>> code((size_t)nc);
+  for (int c = 0; c < nc; ++c)
+    code[(size_t)c] = {spread(c % G) | (spread((c / G) % G) << 1) | (spread(c / (G * G)) << 2), c};
+  std::sort(code.begin(), code.end());
+  mrank.assign((size_t)nc, 0);
+  rankCell.assign((size_t)nc, 0);
+  for (int r = 0; r < nc; ++r) {
+    rankCell[(size_t)r] = code[(size_t)r].second;
+    mrank[(size_t)code[(size_t)r].second] = r;
+  }
+}
+
+void launch_near_list_count(const double* seg, int64_t ld, int n, const GridDev& g, int rn,
+                            NearListDev& L, hipStream_t st) {
   const int nc = g.G * g.G * g.G;
-  hipLaunchKernelGGL(k_near_rows, dim3(nc), dim3(64), 0, st, seg, ld, g, rn, start, order, K, mr.K,
-                     mr.farDist * mr.farDist, mr.sinMin * mr.sinMin, out, flags, counters);
+  const int bs = 256;
+  L.rn = rn;
+  hipLaunchKernelGGL(k_morton_counts, dim3((nc + bs - 1) / bs), dim3(bs), 0, st, L.counts, L.rankCell,
+                     nc, L.mcount);
+  hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, st, L.mcount, nc, L.mstart);
+  if (n > 0)
+    hipLaunchKernelGGL(k_sort_gather, dim3((n + bs - 1) / bs), dim3(bs), 0, st, seg, ld, n, L.cellOf,
+                       L.start, L.order, L.mrank, L.mstart, g.G, L.ldS, L.segS, L.key, L.perm);
+  hipLaunchKernelGGL(k_sort_pad, dim3((unsigned)((L.ldS + bs - 1) / bs)), dim3(bs), 0, st, L.start, nc,
+                     L.ldS, L.segS, L.key, L.perm);
+  const int nbS = (int)(L.ldS / kTile);
+  hipLaunchKernelGGL(k_block_boxes, dim3((nbS + bs - 1) / bs), dim3(bs), 0, st, L.key, L.start, nc, nbS,
+                     L.box);
+  hipLaunchKernelGGL(k_near_count_rows, dim3(nbS), dim3(64), 0, st, L.box, nbS, rn, L.rowCount);
+  hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, st, L.rowCount, nbS, L.rowStart);
+}
+
+void launch_near_list_fill(const NearListDev& L, hipStream_t st) {
+  const int nbS = (int)(L.ldS / kTile);
+  hipLaunchKernelGGL(k_near_fill_rows, dim3(nbS), dim3(64), 0, st, L.box, nbS, L.rn, L.rowStart, L.list,
+                     L.tileIdx);
+  hipLaunchKernelGGL(k_near_count_cols, dim3(nbS), dim3(64), 0, st, L.tileIdx, nbS, L.colCount);
+  hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, st, L.colCount, nbS, L.colStart);
+  hipLaunchKernelGGL(k_near_fill_cols, dim3(nbS), dim3(64), 0, st, L.tileIdx, nbS, L.colStart,
+                     L.colList);
+}
+
+void launch_near_reduce(const NearListDev& L, int nc, const double* rowPart, const double* colPart,
+                        double* out, hipStream_t st) {
+  const int bs = 256;
+  hipLaunchKernelGGL(k_near_reduce, dim3((unsigned)((L.ldS + bs - 1) / bs)), dim3(bs), 0, st, rowPart,
+                     colPart, L.rowStart, L.colStart, L.colList, L.perm, L.start, nc, out);
 }
 
 }  // namespace ddd

@@ -3,6 +3,8 @@
 #include "kernels.hpp"
 #include "pair_special.hpp"
 
+#include <vector>
+
 namespace ddd {
 
 constexpr int kMoments = 90;  // cell features: orders 0, 1, 2 of b (x) t L
@@ -28,8 +30,44 @@ void launch_operator_spectrum(const float* W, const GridDev& g, int rn, const fl
                               float2* scratch, float2* What, hipStream_t st);
 void launch_far_apply(const double* seg, int64_t ld, int n, const GridDev& g, const int* cellOf,
                       const float* L, double* out, hipStream_t st);
-void launch_near_rows(const double* seg, int64_t ld, const GridDev& g, int rn, const int* start,
-                      const int* order, const SegK<double>& K, const MixedRule& mr, double* out,
-                      unsigned* flags, unsigned long long* counters, hipStream_t st);
+// Device arrays of the near-field tile list (see farfield.hip).
+struct NearListDev {
+  // inputs: cell binning
+  const int* cellOf;
+  const unsigned* counts;
+  const int* start;
+  const int* order;
+  const int* mrank;     // Morton rank of each cell
+  const int* rankCell;  // cell of each Morton rank
+  // outputs
+  unsigned* mcount;
+  int* mstart;          // nc + 1
+  int64_t ldS;          // pad64(numSeg): sorted copy leading dimension
+  double* segS;         // F_COUNT x ldS
+  int* key;             // packed cell coordinates per sorted segment
+  int* perm;            // original index per sorted segment (-1: padding)
+  int* box;             // 6 per block
+  unsigned* rowCount;
+  int* rowStart;        // nbS + 1; rowStart[nbS] = number of tiles
+  unsigned* colCount;
+  int* colStart;        // nbS + 1
+  int2* list;           // (I, J) per tile
+  int* tileIdx;         // nbS x nbS, J >= I
+  int* colList;
+  int rn;
+};
+
+void morton_ranks(int G, std::vector<int>& mrank, std::vector<int>& rankCell);
+// Sorted copy, block boxes, per-row tile counts and rowStart (its last entry
+// is the number of tiles); then, with list and colList sized, the lists.
+void launch_near_list_count(const double* seg, int64_t ld, int n, const GridDev& g, int rn,
+                            NearListDev& L, hipStream_t st);
+void launch_near_list_fill(const NearListDev& L, hipStream_t st);
+void launch_near_reduce(const NearListDev& L, int nc, const double* rowPart, const double* colPart,
+                        double* out, hipStream_t st);
+void launch_near_tiles(const double* segS, int64_t ldS, const int* key, int rn, const int2* list,
+                       int nTiles, const SegK<double>& K, const MixedRule& mr, double* rowPart,
+                       double* colPart, unsigned* flags, unsigned long long* counters,
+                       hipStream_t st);
 
 }  // namespace ddd

@@ -1,15 +1,16 @@
 // The far-field network (no reference counterpart): a 4-layer MLP mapping a
 // cell offset o to the 60 x 90 operator W(o) of the hybrid far field,
-//   phi(o) (32) -> 256 -> 256 -> 256 -> 5408 (5400 used), SiLU activations,
+//   phi(o) (128) -> 512 -> 512 -> 512 -> 5408 (5400 used), SiLU activations,
 // W(o)[q][k] = env(o, q, k) * y(o)[q * 90 + k] with the physical envelope
 // env = (mu / 4 pi h) |o|^-(2 + order(k) + order(q)).
 // Matrix products run as BF16 MFMA (v_mfma_f32_32x32x8bf16_1k) with FP32
 // accumulation; master weights and Adam moments are FP32.
 //
-// Training data are exact non-singular stresses: a sample is (offset, source
-// cell moments M (90), target basis P(rho) (10), exact stress (6)); the loss
-// is sum_s |sigma_hat_s - sigma_s|^2 / den_s with
-// sigma_hat_s[c] = sum_{l,k} W(o_s)[c * 10 + l][k] M_s[k] P_s[l].
+// Training data are exact local expansions: a sample is (offset o, source
+// cell s); its target is the L2 projection onto the 10-term quadratic basis
+// of the exact non-singular stress of the cell's segments at the 27 tensor
+// Gauss points of the cell at s - o, L_exact (6 x 10).  The loss is
+// sum_s |W(o_s) M_s - L_exact_s|^2 / (den_s ns), with the envelope inside W.
 #include "mlp.hpp"
 
 namespace ddd {
@@ -83,6 +84,122 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const unsigned short* __restr
   }
 }
 
+// Faster product for aligned operands: 128 x 128 tile per workgroup of 4
+// wavefronts (64 x 64 each, 2 x 2 blocks of v_mfma_f32_32x32x16_bf16), K
+// staged 32 at a time.  Both operands sit in LDS with k contiguous (A as
+// [row][k], B as [col][k]), so each lane fetches its 8 k-values with one
+// 16-byte read; global loads are 8-element vectors along whichever index is
+// contiguous.  Requires 16-byte aligned bases, and the non-unit stride of the
+// contiguous operand a multiple of 8.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+constexpr int BT = 128, BK = 32, LDK = BK + 8;
+
+__global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __restrict__ A,
+                                                        int64_t a_i, int64_t a_k,
+                                                        const unsigned short* __restrict__ B,
+                                                        int64_t b_k, int64_t b_j,
+                                                        float* __restrict__ C, int64_t ldc, int M,
+                                                        int N, int K, int accumulate) {
+  __shared__ __attribute__((aligned(16))) unsigned short sA[BT][LDK];
+  __shared__ __attribute__((aligned(16))) unsigned short sB[BT][LDK];
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int i0 = blockIdx.y * BT, j0 = blockIdx.x * BT;
+  const int wi = (wave / 2) * 64, wj = (wave % 2) * 64;
+  const bool aK = (a_k == 1), bK = (b_k == 1);
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int n = 0; n < 2; ++n)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[m][n][v] = 0.f;
+
+  for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int v = tid + 256 * h;
+      if (aK) {  // 8 consecutive k of one row
+        const int r = v / 4, kq = (v % 4) * 8;
+        const int gi = i0 + r, gk = k0 + kq;
+        u16x8 x;
+        if (gi < M && gk + 7 < K) {
+          x = *(const u16x8*)(A + (int64_t)gi * a_i + gk);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) x[q] = (gi < M && gk + q < K) ? A[(int64_t)gi * a_i + gk + q] : 0;
+        }
+        *(u16x8*)&sA[r][kq] = x;
+      } else {  // 8 consecutive rows at one k
+        const int kk = v / 16, rq = (v % 16) * 8;
+        const int gi = i0 + rq, gk = k0 + kk;
+        u16x8 x;
+        if (gk < K && gi + 7 < M) {
+          x = *(const u16x8*)(A + gi + (int64_t)gk * a_k);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) x[q] = (gk < K && gi + q < M) ? A[gi + q + (int64_t)gk * a_k] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sA[rq + q][kk] = x[q];
+      }
+      if (bK) {  // 8 consecutive k of one column
+        const int c = v / 4, kq = (v % 4) * 8;
+        const int gj = j0 + c, gk = k0 + kq;
+        u16x8 x;
+        if (gj < N && gk + 7 < K) {
+          x = *(const u16x8*)(B + gk + (int64_t)gj * b_j);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) x[q] = (gj < N && gk + q < K) ? B[gk + q + (int64_t)gj * b_j] : 0;
+        }
+        *(u16x8*)&sB[c][kq] = x;
+      } else {  // 8 consecutive columns at one k
+        const int kk = v / 16, cq = (v % 16) * 8;
+        const int gj = j0 + cq, gk = k0 + kk;
+        u16x8 x;
+        if (gk < K && gj + 7 < N) {
+          x = *(const u16x8*)(B + (int64_t)gk * b_k + gj);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) x[q] = (gk < K && gj + q < N) ? B[(int64_t)gk * b_k + gj + q] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sB[cq + q][kk] = x[q];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 16) {
+      const int kb = ks + 8 * (lane / 32);
+      bf16x8 a[2], b[2];
+#pragma unroll
+      for (int m = 0; m < 2; ++m) a[m] = *(const bf16x8*)&sA[wi + 32 * m + lane % 32][kb];
+#pragma unroll
+      for (int n = 0; n < 2; ++n) b[n] = *(const bf16x8*)&sB[wj + 32 * n + lane % 32][kb];
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int n = 0; n < 2; ++n)
+          acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[m], b[n], acc[m][n], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int n = 0; n < 2; ++n)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int gi = i0 + wi + 32 * m + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+        const int gj = j0 + wj + 32 * n + lane % 32;
+        if (gi < M && gj < N) {
+          float* p = C + (int64_t)gi * ldc + gj;
+          *p = accumulate ? *p + acc[m][n][v] : acc[m][n][v];
+        }
+      }
+}
+
 __device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
 __device__ __forceinline__ float silu_grad(float z) {
   const float s = 1.f / (1.f + __expf(-z));
@@ -108,13 +225,27 @@ __global__ void k_act_grad(const float* __restrict__ dH, const float* __restrict
   dZ[e] = f2bf(dH[e] * silu_grad(Z[e]));
 }
 
-// db[c] = sum over rows of dZ[r][c] (bf16 values, fixed row order).
-__global__ void k_colsum(const unsigned short* __restrict__ dZ, int rows, int cols,
-                         float* __restrict__ db) {
+// db[c] = sum over rows of dZ[r][c] (bf16 values): partial sums over
+// kColChunks row chunks (thread per column, coalesced), then the chunks in
+// order.
+constexpr int kColChunks = 64;
+__global__ void k_colsum_part(const unsigned short* __restrict__ dZ, int rows, int cols,
+                              float* __restrict__ part) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ch = blockIdx.y;
+  if (c >= cols) return;
+  const int per = (rows + kColChunks - 1) / kColChunks;
+  const int r0 = ch * per, r1 = min(rows, r0 + per);
+  float a = 0.f;
+  for (int r = r0; r < r1; ++r) a += bf2f(dZ[(int64_t)r * cols + c]);
+  part[(int64_t)ch * cols + c] = a;
+}
+
+__global__ void k_colsum_final(const float* __restrict__ part, int cols, float* __restrict__ db) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cols) return;
   float a = 0.f;
-  for (int r = 0; r < rows; ++r) a += bf2f(dZ[(int64_t)r * cols + c]);
+  for (int ch = 0; ch < kColChunks; ++ch) a += part[(int64_t)ch * cols + c];
   db[c] = a;
 }
 
@@ -123,59 +254,78 @@ __global__ void k_f32_to_bf16(const float* __restrict__ x, int64_t n, unsigned s
   if (e < n) y[e] = f2bf(x[e]);
 }
 
-// Offset features: u = o / |o| (3), 1 / |o|, 1 / |o|^2, u_i u_j (6); zero padded.
-__device__ void offset_features(int ox, int oy, int oz, float* f) {
+// Offset features: 1/|o|, 1/|o|^2, 1/|o|^3, then the monomials
+// ux^a uy^b uz^c of the direction u = o/|o| for degrees d = 1..7, in the
+// order a = d..0, b = d-a..0, c = d-a-b, then 1/R_a, 1/R_a^2, 1/R_a^3 with
+// R_a^2 = |o|^2 + aH^2 (the core radius in cell units); zero padded.
+__device__ void offset_features(int ox, int oy, int oz, float aH, float* f) {
   for (int q = 0; q < kMlpIn; ++q) f[q] = 0.f;
   if (ox == 0 && oy == 0 && oz == 0) return;  // the zero offset is never used
-  const float R = sqrtf((float)(ox * ox + oy * oy + oz * oz));
+  const float R2 = (float)(ox * ox + oy * oy + oz * oz);
+  const float R = sqrtf(R2);
   const float ir = 1.f / R;
   const float u[3] = {ox * ir, oy * ir, oz * ir};
-  f[0] = u[0]; f[1] = u[1]; f[2] = u[2];
-  f[3] = ir; f[4] = ir * ir;
-  f[5] = u[0] * u[0]; f[6] = u[1] * u[1]; f[7] = u[2] * u[2];
-  f[8] = u[0] * u[1]; f[9] = u[0] * u[2]; f[10] = u[1] * u[2];
+  f[0] = ir;
+  f[1] = ir * ir;
+  f[2] = ir * ir * ir;
+  float px[kMonoDeg + 1], py[kMonoDeg + 1], pz[kMonoDeg + 1];
+  px[0] = py[0] = pz[0] = 1.f;
+  for (int e = 1; e <= kMonoDeg; ++e) {
+    px[e] = px[e - 1] * u[0];
+    py[e] = py[e - 1] * u[1];
+    pz[e] = pz[e - 1] * u[2];
+  }
+  int q = 3;
+  for (int d = 1; d <= kMonoDeg; ++d)
+    for (int a = d; a >= 0; --a)
+      for (int b = d - a; b >= 0; --b) f[q++] = px[a] * py[b] * pz[d - a - b];
+  const float ira = 1.f / sqrtf(R2 + aH * aH);
+  f[q] = ira;
+  f[q + 1] = ira * ira;
+  f[q + 2] = ira * ira * ira;
 }
 
-__global__ void k_features(const int* __restrict__ off3, int ns, unsigned short* __restrict__ X) {
+__global__ void k_features(const int* __restrict__ off3, int ns, float aH, unsigned short* __restrict__ X) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   float f[kMlpIn];
-  offset_features(off3[3 * s], off3[3 * s + 1], off3[3 * s + 2], f);
+  offset_features(off3[3 * s], off3[3 * s + 1], off3[3 * s + 2], aH, f);
   for (int q = 0; q < kMlpIn; ++q) X[(int64_t)s * kMlpIn + q] = f2bf(f[q]);
 }
 
 __device__ __forceinline__ int moment_order(int k) { return k < 9 ? 0 : (k < 36 ? 1 : 2); }
 __device__ __forceinline__ int poly_order(int l) { return l == 0 ? 0 : (l < 4 ? 1 : 2); }
 
-// Per sample: prediction, loss term and dY (fp32) of the contraction.
-// One workgroup per sample; threads over the 5400 operator entries.
-__global__ void k_contract(const float* __restrict__ Y, const int* __restrict__ off3,
-                           const float* __restrict__ Ms, const float* __restrict__ Ps,
-                           const float* __restrict__ sig, const float* __restrict__ den, int ns,
-                           float envScale, float* __restrict__ dY, float* __restrict__ lossPer) {
+// Per sample: predicted local expansion L_hat[q] = sum_k W(o)[q][k] M[k]
+// (q = 10 c + l), loss term |L_hat - L|^2 / (den ns) and dY (fp32).  One
+// workgroup of 256 per sample.
+__global__ __launch_bounds__(256) void k_contract(const float* __restrict__ Y,
+                                                  const int* __restrict__ off3,
+                                                  const float* __restrict__ Ms,
+                                                  const float* __restrict__ Lt,
+                                                  const float* __restrict__ den, int ns,
+                                                  float envScale, float* __restrict__ dY,
+                                                  float* __restrict__ lossPer) {
   const int s = blockIdx.x;
   const int tid = threadIdx.x;
-  __shared__ float part[256][6];
-  __shared__ float r[6];
+  __shared__ float sM[kMoments];
+  __shared__ float envK[kMoments];
+  __shared__ float envL[kPoly];
+  __shared__ float r[kLocal];
   const int ox = off3[3 * s], oy = off3[3 * s + 1], oz = off3[3 * s + 2];
-  const float R = sqrtf((float)(ox * ox + oy * oy + oz * oz));
-  const float iR = 1.f / R;
+  const float iR = 1.f / sqrtf((float)(ox * ox + oy * oy + oz * oz));
   const float* y = Y + (int64_t)s * kMlpOutPad;
-  const float* M = Ms + (int64_t)s * kMoments;
-  const float* P = Ps + (int64_t)s * kPoly;
-  float acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int e = tid; e < kChan; e += blockDim.x) {
-    const int q = e / kMoments, k = e % kMoments;
-    const int comp = q / kPoly, l = q % kPoly;
-    const float env = envScale * powf(iR, (float)(2 + moment_order(k) + poly_order(l)));
-    acc[comp] += env * y[e] * M[k] * P[l];
+  if (tid < kMoments) {
+    sM[tid] = Ms[(int64_t)s * kMoments + tid];
+    envK[tid] = powf(iR, (float)moment_order(tid));
   }
-  for (int c = 0; c < 6; ++c) part[tid][c] = acc[c];
+  if (tid < kPoly) envL[tid] = envScale * powf(iR, (float)(2 + poly_order(tid)));
   __syncthreads();
-  if (tid < 6) {
+  if (tid < kLocal) {
+    const int l = tid % kPoly;
     float a = 0.f;
-    for (int t = 0; t < (int)blockDim.x; ++t) a += part[t][tid];
-    r[tid] = a - sig[6 * s + tid];
+    for (int k = 0; k < kMoments; ++k) a += envK[k] * y[tid * kMoments + k] * sM[k];
+    r[tid] = envL[l] * a - Lt[(int64_t)s * kLocal + tid];
   }
   __syncthreads();
   const float w = 2.f / (den[s] * ns);
@@ -183,15 +333,13 @@ __global__ void k_contract(const float* __restrict__ Y, const int* __restrict__ 
     float g = 0.f;
     if (e < kChan) {
       const int q = e / kMoments, k = e % kMoments;
-      const int comp = q / kPoly, l = q % kPoly;
-      const float env = envScale * powf(iR, (float)(2 + moment_order(k) + poly_order(l)));
-      g = w * r[comp] * env * M[k] * P[l];
+      g = w * r[q] * envL[q % kPoly] * envK[k] * sM[k];
     }
     dY[(int64_t)s * kMlpOutPad + e] = g;
   }
   if (tid == 0) {
     float a = 0.f;
-    for (int c = 0; c < 6; ++c) a += r[c] * r[c];
+    for (int q = 0; q < kLocal; ++q) a += r[q] * r[q];
     lossPer[s] = a / (den[s] * ns);
   }
 }
@@ -227,43 +375,62 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   p[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
 }
 
-// Batch b of the training set: sample idx[b] of the resident arrays.
+// Batch b of the training set: sample idx[b]; moments from the cell array
+// M (feature-major, nc cells).
 __global__ void k_gather_batch(const int* __restrict__ idx, int nb, const int* __restrict__ off3,
-                               const float* __restrict__ M, const float* __restrict__ P,
-                               const float* __restrict__ sig, const float* __restrict__ den,
-                               int* __restrict__ bOff, float* __restrict__ bM, float* __restrict__ bP,
-                               float* __restrict__ bSig, float* __restrict__ bDen) {
+                               const int* __restrict__ src, const float* __restrict__ M, int nc,
+                               const float* __restrict__ Lt, const float* __restrict__ den,
+                               int* __restrict__ bOff, float* __restrict__ bM,
+                               float* __restrict__ bL, float* __restrict__ bDen) {
   const int b = blockIdx.x;
   const int t = threadIdx.x;
   if (b >= nb) return;
   const int64_t s = idx[b];
-  if (t < kMoments) bM[(int64_t)b * kMoments + t] = M[s * kMoments + t];
-  if (t < kPoly) bP[(int64_t)b * kPoly + t] = P[s * kPoly + t];
-  if (t < 6) bSig[(int64_t)b * 6 + t] = sig[s * 6 + t];
+  const int c = src[s];
+  if (t < kMoments) bM[(int64_t)b * kMoments + t] = M[(int64_t)t * nc + c];
+  if (t < kLocal) bL[(int64_t)b * kLocal + t] = Lt[s * kLocal + t];
   if (t < 3) bOff[(int64_t)b * 3 + t] = off3[s * 3 + t];
   if (t == 0) bDen[b] = den[s];
 }
 
-// Exact stress of source cell c (its segments in cell order) at point x,
-// fp64 (the non-singular field of DislocationFEM.jl:142-299).
-__global__ void k_sample_stress(const double* __restrict__ seg, int64_t ld,
+// Exact local expansion of source cell src[s] over the cell at offset
+// -off3[s]: FP64 stress (the non-singular field of DislocationFEM.jl:142-299)
+// at the 27 Gauss points, projected with Pj (10 x 27).  Thread per sample.
+__global__ __launch_bounds__(64) void k_local_targets(const double* __restrict__ seg, int64_t ld, GridDev g,
                                 const int* __restrict__ start, const int* __restrict__ order,
-                                const int* __restrict__ srcCell, const double* __restrict__ pts,
-                                int ns, SigK<double> K, float* __restrict__ sig) {
+                                const int* __restrict__ src, const int* __restrict__ off3, int ns,
+                                SigK<double> K, Projection Pj, float* __restrict__ Lt) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
-  const int c = srcCell[s];
-  const v3<double> x = V(pts[3 * s], pts[3 * s + 1], pts[3 * s + 2]);
-  double sg[6] = {0, 0, 0, 0, 0, 0};
-  for (int e = start[c]; e < start[c + 1]; ++e) {
-    const int i = order[e];
-    const v3<double> n1 = V(seg[F_N1X * ld + i], seg[F_N1Y * ld + i], seg[F_N1Z * ld + i]);
-    const v3<double> n2 = V(seg[F_N2X * ld + i], seg[F_N2Y * ld + i], seg[F_N2Z * ld + i]);
-    const v3<double> b = V(seg[F_BX * ld + i], seg[F_BY * ld + i], seg[F_BZ * ld + i]);
-    const v3<double> t = V(seg[F_TX * ld + i], seg[F_TY * ld + i], seg[F_TZ * ld + i]);
-    seg_stress<double>(K, x - n1, n2 - n1, t, b, cross(t, b), sg);
+  const int c = src[s];
+  const int cx = c % g.G, cy = (c / g.G) % g.G, cz = c / (g.G * g.G);
+  const double tc[3] = {g.lo[0] + (cx - off3[3 * s] + 0.5) * g.h,
+                        g.lo[1] + (cy - off3[3 * s + 1] + 0.5) * g.h,
+                        g.lo[2] + (cz - off3[3 * s + 2] + 0.5) * g.h};
+  double L[kLocal];
+#pragma unroll
+  for (int q = 0; q < kLocal; ++q) L[q] = 0.0;
+  for (int p = 0; p < 27; ++p) {
+    const v3<double> x = V(tc[0] + g.h * Pj.xi[p % 3], tc[1] + g.h * Pj.xi[(p / 3) % 3],
+                           tc[2] + g.h * Pj.xi[p / 9]);
+    double sg[6] = {0, 0, 0, 0, 0, 0};
+    for (int e = start[c]; e < start[c + 1]; ++e) {
+      const int i = order[e];
+      const v3<double> n1 = V(seg[F_N1X * ld + i], seg[F_N1Y * ld + i], seg[F_N1Z * ld + i]);
+      const v3<double> n2 = V(seg[F_N2X * ld + i], seg[F_N2Y * ld + i], seg[F_N2Z * ld + i]);
+      const v3<double> b = V(seg[F_BX * ld + i], seg[F_BY * ld + i], seg[F_BZ * ld + i]);
+      const v3<double> t = V(seg[F_TX * ld + i], seg[F_TY * ld + i], seg[F_TZ * ld + i]);
+      seg_stress<double>(K, x - n1, n2 - n1, t, b, cross(t, b), sg);
+    }
+#pragma unroll
+    for (int l = 0; l < kPoly; ++l) {
+      const double w = Pj.P[l][p];
+#pragma unroll
+      for (int cc = 0; cc < 6; ++cc) L[cc * kPoly + l] += w * sg[cc];
+    }
   }
-  for (int q = 0; q < 6; ++q) sig[6 * s + q] = (float)sg[q];
+#pragma unroll
+  for (int q = 0; q < kLocal; ++q) Lt[(int64_t)s * kLocal + q] = (float)L[q];
 }
 
 }  // namespace
@@ -273,6 +440,16 @@ static unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs)
 void mlp_gemm(const unsigned short* A, int64_t a_i, int64_t a_k, const unsigned short* B,
               int64_t b_k, int64_t b_j, float* C, int64_t ldc, int M, int N, int K, int accumulate,
               hipStream_t st) {
+  const bool alignedA = ((uintptr_t)A % 16 == 0) &&
+                        (a_k == 1 ? a_i % 8 == 0 : (a_i == 1 && a_k % 8 == 0));
+  const bool alignedB = ((uintptr_t)B % 16 == 0) &&
+                        (b_k == 1 ? b_j % 8 == 0 : (b_j == 1 && b_k % 8 == 0));
+  if (alignedA && alignedB) {
+    dim3 grid(blocks(N, BT), blocks(M, BT));
+    hipLaunchKernelGGL(k_gemm_bf16_t128, grid, dim3(256), 0, st, A, a_i, a_k, B, b_k, b_j, C, ldc, M,
+                       N, K, accumulate);
+    return;
+  }
   dim3 grid(blocks(N, GT), blocks(M, GT));
   hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, A, a_i, a_k, B, b_k, b_j, C, ldc, M, N, K,
                      accumulate);
@@ -288,23 +465,26 @@ void mlp_act_grad(const float* dH, const float* Z, int64_t n, unsigned short* dZ
   hipLaunchKernelGGL(k_act_grad, dim3(blocks(n, 256)), dim3(256), 0, st, dH, Z, n, dZ);
 }
 
-void mlp_colsum(const unsigned short* dZ, int rows, int cols, float* db, hipStream_t st) {
-  hipLaunchKernelGGL(k_colsum, dim3(blocks(cols, 256)), dim3(256), 0, st, dZ, rows, cols, db);
+void mlp_colsum(const unsigned short* dZ, int rows, int cols, float* part, float* db,
+                hipStream_t st) {
+  hipLaunchKernelGGL(k_colsum_part, dim3(blocks(cols, 256), kColChunks), dim3(256), 0, st, dZ, rows,
+                     cols, part);
+  hipLaunchKernelGGL(k_colsum_final, dim3(blocks(cols, 256)), dim3(256), 0, st, part, cols, db);
 }
 
 void mlp_to_bf16(const float* x, int64_t n, unsigned short* y, hipStream_t st) {
   hipLaunchKernelGGL(k_f32_to_bf16, dim3(blocks(n, 256)), dim3(256), 0, st, x, n, y);
 }
 
-void mlp_features(const int* off3, int ns, unsigned short* X, hipStream_t st) {
-  hipLaunchKernelGGL(k_features, dim3(blocks(ns, 256)), dim3(256), 0, st, off3, ns, X);
+void mlp_features(const int* off3, int ns, float aH, unsigned short* X, hipStream_t st) {
+  hipLaunchKernelGGL(k_features, dim3(blocks(ns, 256)), dim3(256), 0, st, off3, ns, aH, X);
 }
 
-void mlp_contract(const float* Y, const int* off3, const float* M, const float* P, const float* sig,
+void mlp_contract(const float* Y, const int* off3, const float* M, const float* Lt,
                   const float* den, int ns, float envScale, float* dY, float* lossPer,
                   hipStream_t st) {
-  hipLaunchKernelGGL(k_contract, dim3(ns), dim3(256), 0, st, Y, off3, M, P, sig, den, ns, envScale,
-                     dY, lossPer);
+  hipLaunchKernelGGL(k_contract, dim3(ns), dim3(256), 0, st, Y, off3, M, Lt, den, ns, envScale, dY,
+                     lossPer);
 }
 
 void mlp_operator_from_y(const float* Y, const int* off3, int no, float envScale, float* W,
@@ -319,18 +499,18 @@ void mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr,
                      c2, eps);
 }
 
-void gather_batch(const int* idx, int nb, const int* off3, const float* M, const float* P,
-                  const float* sig, const float* den, int* bOff, float* bM, float* bP, float* bSig,
-                  float* bDen, hipStream_t st) {
-  hipLaunchKernelGGL(k_gather_batch, dim3(nb), dim3(128), 0, st, idx, nb, off3, M, P, sig, den, bOff,
-                     bM, bP, bSig, bDen);
+void gather_batch(const int* idx, int nb, const int* off3, const int* src, const float* M, int nc,
+                  const float* Lt, const float* den, int* bOff, float* bM, float* bL, float* bDen,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_batch, dim3(nb), dim3(128), 0, st, idx, nb, off3, src, M, nc, Lt, den,
+                     bOff, bM, bL, bDen);
 }
 
-void sample_stress(const double* seg, int64_t ld, const int* start, const int* order,
-                   const int* srcCell, const double* pts, int ns, const SigK<double>& K, float* sig,
-                   hipStream_t st) {
-  hipLaunchKernelGGL(k_sample_stress, dim3(blocks(ns, 128)), dim3(128), 0, st, seg, ld, start, order,
-                     srcCell, pts, ns, K, sig);
+void local_targets(const double* seg, int64_t ld, const GridDev& g, const int* start,
+                   const int* order, const int* src, const int* off3, int ns, const SigK<double>& K,
+                   const Projection& Pj, float* Lt, hipStream_t st) {
+  hipLaunchKernelGGL(k_local_targets, dim3(blocks(ns, 64)), dim3(64), 0, st, seg, ld, g, start, order,
+                     src, off3, ns, K, Pj, Lt);
 }
 
 }  // namespace ddd

@@ -4,8 +4,9 @@
 
 namespace ddd {
 
-constexpr int kMlpIn = 32;         // offset features, zero padded
-constexpr int kHidden = 256;
+constexpr int kMlpIn = 128;        // offset features, zero padded
+constexpr int kMonoDeg = 7;        // direction monomials of degree 1..7 (119)
+constexpr int kHidden = 512;
 constexpr int kMlpOutPad = 5408;   // kChan = 5400 rounded up to a multiple of 32
 
 // Parameter layout (fp32): W1, b1, W2, b2, W3, b3, W4, b4, matrices row-major
@@ -26,21 +27,32 @@ void mlp_gemm(const unsigned short* A, int64_t a_i, int64_t a_k, const unsigned 
 void mlp_bias_act(float* Z, const float* b, int rows, int cols, int act, unsigned short* H,
                   hipStream_t st);
 void mlp_act_grad(const float* dH, const float* Z, int64_t n, unsigned short* dZ, hipStream_t st);
-void mlp_colsum(const unsigned short* dZ, int rows, int cols, float* db, hipStream_t st);
+// part: scratch of 64 x cols floats.
+void mlp_colsum(const unsigned short* dZ, int rows, int cols, float* part, float* db,
+                hipStream_t st);
 void mlp_to_bf16(const float* x, int64_t n, unsigned short* y, hipStream_t st);
-void mlp_features(const int* off3, int ns, unsigned short* X, hipStream_t st);
-void mlp_contract(const float* Y, const int* off3, const float* M, const float* P, const float* sig,
+// aH: core radius in cell units (features in 1/R_a, R_a^2 = R^2 + aH^2).
+void mlp_features(const int* off3, int ns, float aH, unsigned short* X, hipStream_t st);
+void mlp_contract(const float* Y, const int* off3, const float* M, const float* Lt,
                   const float* den, int ns, float envScale, float* dY, float* lossPer,
                   hipStream_t st);
 void mlp_operator_from_y(const float* Y, const int* off3, int no, float envScale, float* W,
                          hipStream_t st);
 void mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
               float c1, float c2, float eps, hipStream_t st);
-void gather_batch(const int* idx, int nb, const int* off3, const float* M, const float* P,
-                  const float* sig, const float* den, int* bOff, float* bM, float* bP, float* bSig,
-                  float* bDen, hipStream_t st);
-void sample_stress(const double* seg, int64_t ld, const int* start, const int* order,
-                   const int* srcCell, const double* pts, int ns, const SigK<double>& K, float* sig,
-                   hipStream_t st);
+void gather_batch(const int* idx, int nb, const int* off3, const int* src, const float* M, int nc,
+                  const float* Lt, const float* den, int* bOff, float* bM, float* bL, float* bDen,
+                  hipStream_t st);
+
+// L2 projection onto the quadratic basis from the 27 tensor Gauss points of a
+// cell: node p at (xi[p % 3], xi[(p / 3) % 3], xi[p / 9]) (cell units,
+// centred), L_l = sum_p P[l][p] sigma(p).
+struct Projection {
+  double xi[3];
+  double P[kPoly][27];
+};
+void local_targets(const double* seg, int64_t ld, const GridDev& g, const int* start,
+                   const int* order, const int* src, const int* off3, int ns, const SigK<double>& K,
+                   const Projection& Pj, float* Lt, hipStream_t st);
 
 }  // namespace ddd

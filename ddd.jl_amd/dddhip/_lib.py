@@ -58,13 +58,14 @@ class ddd_stats(C.Structure):
 
 class ddd_train_opts(C.Structure):
     _fields_ = [("samples", i64), ("valSamples", i64), ("steps", C.c_int), ("batch", C.c_int),
-                ("lr", C.c_float), ("lrFinal", C.c_float), ("seed", C.c_uint64)]
+                ("lr", C.c_float), ("lrFinal", C.c_float), ("seed", C.c_uint64),
+                ("denFloor", C.c_float), ("nearFrac", C.c_float), ("nearShells", C.c_int)]
 
 
 class ddd_train_report(C.Structure):
     _fields_ = [("lossFirst", C.c_double), ("lossLast", C.c_double),
                 ("relErrVal", C.c_double), ("relErrValNear", C.c_double),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("relErrShell", C.c_double * 8)]
 
 
 class DDDError(RuntimeError):
@@ -126,13 +127,17 @@ def lib():
                 "ddd_mlp_params": (C.c_int, [P, P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
                 "ddd_mlp_loss_grad": (C.c_int, [P, P, C.c_int, C.POINTER(C.c_int),
                                                 C.POINTER(C.c_float), C.POINTER(C.c_float),
-                                                C.POINTER(C.c_float), C.POINTER(C.c_float),
-                                                C.c_float, dp, C.POINTER(C.c_float)]),
+                                                C.POINTER(C.c_float), C.c_float, dp,
+                                                C.POINTER(C.c_float)]),
                 "ddd_farfield_train": (C.c_int, [P, P, P, C.POINTER(ddd_consts),
                                                  C.POINTER(ddd_network),
                                                  C.POINTER(ddd_train_opts),
                                                  C.POINTER(ddd_train_report)]),
                 "ddd_farfield_set_operator_mlp": (C.c_int, [P, P, P, C.POINTER(ddd_consts)]),
+                "ddd_farfield_local_targets": (C.c_int, [P, P, C.POINTER(ddd_consts),
+                                                         C.POINTER(ddd_network), C.c_int,
+                                                         C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                         C.POINTER(C.c_float)]),
                 "ddd_net_segforce_device": (P, [P]),
             }
             for name, (res, args) in sigs.items():

@@ -277,21 +277,22 @@ class FarFieldMLP:
         _call(self.ctx, "ddd_mlp_params", self.h_ptr, None,
               p.ctypes.data_as(_c.POINTER(_c.c_float)))
 
-    def loss_grad(self, off3, M, P, sigma, den, envScale):
-        """Loss and parameter gradient on one batch (see ddd_mlp_loss_grad)."""
+    def loss_grad(self, off3, M, Lt, den, envScale):
+        """Loss and parameter gradient on one batch of (offset, source
+        moments M (90), target local expansion Lt (6 x 10)) samples
+        (see ddd_mlp_loss_grad)."""
         f = _c.POINTER(_c.c_float)
         off3 = np.ascontiguousarray(off3, dtype=np.int32)
         ns = off3.shape[0]
         M = np.ascontiguousarray(M, dtype=np.float32).reshape(ns, 90)
-        P = np.ascontiguousarray(P, dtype=np.float32).reshape(ns, 10)
-        sigma = np.ascontiguousarray(sigma, dtype=np.float32).reshape(ns, 6)
+        Lt = np.ascontiguousarray(Lt, dtype=np.float32).reshape(ns, 60)
         den = np.ascontiguousarray(den, dtype=np.float32).reshape(ns)
         loss = _c.c_double()
         grad = np.zeros(self.num_params(), dtype=np.float32)
         _call(self.ctx, "ddd_mlp_loss_grad", self.h_ptr, int(ns),
               off3.ctypes.data_as(_c.POINTER(_c.c_int)), M.ctypes.data_as(f),
-              P.ctypes.data_as(f), sigma.ctypes.data_as(f), den.ctypes.data_as(f),
-              float(envScale), _byref(loss), grad.ctypes.data_as(f))
+              Lt.ctypes.data_as(f), den.ctypes.data_as(f), float(envScale), _byref(loss),
+              grad.ctypes.data_as(f))
         return loss.value, grad
 
     def close(self):
@@ -307,17 +308,36 @@ class FarFieldMLP:
 
 
 def trainFarField(dlnParams, matParams, farfield, mlp, trainNetwork, samples=1 << 20,
-                  valSamples=1 << 15, steps=3000, batch=2048, lr=1e-3, lrFinal=3e-5, seed=1):
+                  valSamples=1 << 15, steps=3000, batch=2048, lr=1e-3, lrFinal=3e-5, seed=1,
+                  denFloor=0.0, nearFrac=0.0, nearShells=0):
     """Train mlp on exact stresses of trainNetwork binned on farfield's grid
     and install the operator it predicts.  Returns the report as a dict."""
     ctx = farfield.ctx
     nv = L.NetView(trainNetwork)
     k = L.consts(dlnParams, matParams)
     o = L.ddd_train_opts(samples=int(samples), valSamples=int(valSamples), steps=int(steps),
-                         batch=int(batch), lr=float(lr), lrFinal=float(lrFinal), seed=int(seed))
+                         batch=int(batch), lr=float(lr), lrFinal=float(lrFinal), seed=int(seed),
+                         denFloor=float(denFloor), nearFrac=float(nearFrac),
+                         nearShells=int(nearShells))
     r = L.ddd_train_report()
     _call(ctx, "ddd_farfield_train", farfield.h_ptr, mlp.h_ptr, k, nv.s, o, r)
-    return {f: getattr(r, f) for f, _ in L.ddd_train_report._fields_}
+    out = {f: getattr(r, f) for f, _ in L.ddd_train_report._fields_}
+    out["relErrShell"] = list(r.relErrShell)
+    return out
+
+
+def farFieldLocalTargets(dlnParams, matParams, farfield, network, srcCell, off3):
+    """Exact local expansions (ns x 60) of source cells over the cells at
+    srcCell - off3, from network binned on farfield's grid (test hook)."""
+    nv = L.NetView(network)
+    k = L.consts(dlnParams, matParams)
+    src = np.ascontiguousarray(srcCell, dtype=np.int32)
+    off = np.ascontiguousarray(off3, dtype=np.int32).reshape(-1, 3)
+    out = np.zeros((len(src), 60), dtype=np.float32)
+    _call(farfield.ctx, "ddd_farfield_local_targets", farfield.h_ptr, k, nv.s, len(src),
+          src.ctypes.data_as(_c.POINTER(_c.c_int)), off.ctypes.data_as(_c.POINTER(_c.c_int)),
+          out.ctypes.data_as(_c.POINTER(_c.c_float)))
+    return out
 
 
 def setFarFieldFromMLP(dlnParams, matParams, farfield, mlp):

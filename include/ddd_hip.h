@@ -133,7 +133,9 @@ int ddd_farfield_create(ddd_ctx* ctx, int G, const double lo[3], double h, int n
 int ddd_farfield_destroy(ddd_ctx* ctx, ddd_farfield* ff);
 int ddd_farfield_set_operator(ddd_ctx* ctx, ddd_farfield* ff, const float* W);
 /* The far-field network (no reference counterpart): an MLP from a cell
- * offset to the operator W(o), trained on exact stresses.  Parameters are
+ * offset to the operator W(o), trained on exact stresses.  Its input
+ * features include the core radius in cell units of the grid it was last
+ * trained for or installed into (0 after ddd_mlp_create).  Parameters are
  * ddd_mlp_num_params() floats (W1, b1, ..., W4, b4; see csrc/mlp.hpp). */
 typedef struct ddd_mlp ddd_mlp;
 int ddd_mlp_create(ddd_ctx* ctx, uint64_t seed, ddd_mlp** out);
@@ -142,11 +144,12 @@ int64_t ddd_mlp_num_params(void);
 /* Copy the master parameters out (get != NULL) and/or in (set != NULL). */
 int ddd_mlp_params(ddd_ctx* ctx, ddd_mlp* mlp, float* get, const float* set);
 /* Loss and gradient on one host batch of ns samples: off3 (3 ns ints, the
- * cell offset source - target), M (90 ns), P (10 ns), sigma (6 ns), den (ns);
- * loss = sum_s |sigma_hat_s - sigma_s|^2 / (den_s ns).  For gradient checks. */
+ * cell offset source - target), M (90 ns source moments), Lt (60 ns target
+ * local expansions, 6 stress components x 10 basis terms), den (ns);
+ * loss = sum_s |W(o_s) M_s - Lt_s|^2 / (den_s ns).  For gradient checks. */
 int ddd_mlp_loss_grad(ddd_ctx* ctx, ddd_mlp* mlp, int ns, const int* off3, const float* M,
-                      const float* P, const float* sigma, const float* den, float envScale,
-                      double* loss, float* grad);
+                      const float* Lt, const float* den, float envScale, double* loss,
+                      float* grad);
 
 typedef struct ddd_train_opts {
   int64_t samples;     /* training samples (source cell, target point)   */
@@ -155,18 +158,33 @@ typedef struct ddd_train_opts {
   int batch;           /* samples per step                                */
   float lr, lrFinal;   /* learning rate, decayed geometrically            */
   uint64_t seed;
+  float denFloor;      /* loss floor, relative to the offset's typical stress (0: 1e-2) */
+  float nearFrac;      /* share of sources drawn from the first shells (0: 0.5)          */
+  int nearShells;      /* number of those shells outside the near stencil (0: 3)        */
 } ddd_train_opts;
 typedef struct ddd_train_report {
   double lossFirst, lossLast;  /* mean training loss of the first / last 50 steps */
-  double relErrVal;            /* sqrt(sum |sigma_hat - sigma|^2 / sum |sigma|^2) */
+  double relErrVal;            /* held out: sqrt(sum |L_hat - L|^2 / sum |L|^2)   */
   double relErrValNear;        /* the same over samples with Chebyshev offset <= nearRadius + 2 */
   double seconds;
+  double relErrShell[8];       /* relErrVal per Chebyshev shell nearRadius + 1 + k (k = 7: and beyond) */
 } ddd_train_report;
-/* Train mlp on exact stresses of trainNet binned on ff's grid, then install
- * the operator it predicts into ff (as ddd_farfield_set_operator_mlp). */
+/* Train mlp on exact local expansions from trainNet binned on ff's grid, then
+ * install the operator it predicts into ff (as ddd_farfield_set_operator_mlp).
+ * A sample is an occupied source cell and an offset outside the near stencil
+ * (a share nearFrac within nearShells shells of it); its target is the L2
+ * projection onto the quadratic basis of the source cell's exact FP64 stress
+ * at the 27 Gauss points of the target cell.  A network uploaded with
+ * ddd_net_upload stays resident. */
 int ddd_farfield_train(ddd_ctx* ctx, ddd_farfield* ff, ddd_mlp* mlp, const ddd_consts* k,
                        const ddd_network* trainNet, const ddd_train_opts* opts,
                        ddd_train_report* report);
+/* The training targets, for tests: for sample s, the exact local expansion
+ * (6 x 10, component-major) of source cell srcCell[s] of net binned on ff's
+ * grid over the cell at srcCell[s] - off3[s]. */
+int ddd_farfield_local_targets(ddd_ctx* ctx, ddd_farfield* ff, const ddd_consts* k,
+                               const ddd_network* net, int ns, const int* srcCell,
+                               const int* off3, float* out);
 /* Evaluate mlp at every offset of ff's grid and install W(o) (BF16 MFMA). */
 int ddd_farfield_set_operator_mlp(ddd_ctx* ctx, ddd_farfield* ff, ddd_mlp* mlp,
                                   const ddd_consts* k);

@@ -346,6 +346,32 @@ void orc_segseg_near(const double* K, int64_t numSeg, const int64_t* segIdx, int
   }
 }
 
+// The same near pairs, each evaluated once with the segment of lower rank as
+// the first segment and its forces added to both segments (the pair
+// orientation of a kernel that walks segments in rank order).  Serial.
+void orc_segseg_near_once(const double* K, int64_t numSeg, const int64_t* segIdx, int64_t ld,
+                          const double* coord, const double* bVec, const uint8_t* extSeg,
+                          const int64_t* cellOf, int G, int rn, const int64_t* rank, double* out) {
+  SegView s{numSeg, segIdx, ld, coord, bVec, extSeg};
+  SegConsts<double> KK = consts_from<double>(K);
+  for (int64_t i = 0; i < numSeg; ++i) {
+    if (s.ext(i) || cellOf[i] < 0) continue;
+    const int64_t ci = cellOf[i];
+    const int ix = int(ci % G), iy = int((ci / G) % G), iz = int(ci / (int64_t(G) * G));
+    for (int64_t j = 0; j < numSeg; ++j) {
+      if (rank[j] <= rank[i] || s.ext(j) || cellOf[j] < 0) continue;
+      const int64_t cj = cellOf[j];
+      const int jx = int(cj % G), jy = int((cj / G) % G), jz = int(cj / (int64_t(G) * G));
+      if (std::abs(jx - ix) > rn || std::abs(jy - iy) > rn || std::abs(jz - iz) > rn) continue;
+      Forces4<double> f = seg_seg_force(KK, s.b(i), s.n1(i), s.n2(i), s.b(j), s.n1(j), s.n2(j));
+      add3(out + 6 * i, f.f1);
+      add3(out + 6 * i + 3, f.f2);
+      add3(out + 6 * j, f.f3);
+      add3(out + 6 * j + 3, f.f4);
+    }
+  }
+}
+
 // Rows-only seg-seg over a subset (calcSegSegForceIdx!, :831-877).  For each
 // k, row i = idx[k]-1 is computed against every j != i with segment i as the
 // first segment; F1/F2 are added to out column (byPosition ? k : i).

@@ -53,6 +53,7 @@ def lib():
             "orc_node_gather": (None, [ip, i64, i64, ip, dp, dp, d, dp]),
             "orc_set_libm": (None, [i32]),
             "orc_segseg_near": (None, [dp] + net + [ip, i32, i32, dp, i32]),
+            "orc_segseg_near_once": (None, [dp] + net + [ip, i32, i32, ip, dp]),
             "orc_segseg_mixed": (i64, [i32, i32, d, d, dp, dp] + net + [dp, i32]),
             "orc_segseg_rows_grouped": (None, [dp] + net + [ip, i64, ip, i64, dp, i32]),
             "orc_segseg_rows_binned": (None, [dp] + net + [ip, i64, dp, i32, dp, i32]),
@@ -240,6 +241,19 @@ def segseg_near(dln, mat, net, cellOf, G, rn, nthreads=None):
     out = np.zeros((3, 2, n.numSeg), order="F")
     lib().orc_segseg_near(_d(K), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord), _d(n.bVec), _u(n.ext),
                           _i(co), int(G), int(rn), _d(out), int(nthreads or os.cpu_count() or 1))
+    return out
+
+
+def segseg_near_once(dln, mat, net, cellOf, G, rn, rank):
+    """The near pairs of segseg_near, each evaluated once with the segment of
+    lower rank first, forces added to both: 3 x 2 x numSeg."""
+    n = _Net(net)
+    K = seg_consts(dln, mat)
+    co = np.ascontiguousarray(cellOf, dtype=np.int64)
+    rk = np.ascontiguousarray(rank, dtype=np.int64)
+    out = np.zeros((3, 2, n.numSeg), order="F")
+    lib().orc_segseg_near_once(_d(K), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord), _d(n.bVec),
+                               _u(n.ext), _i(co), int(G), int(rn), _i(rk), _d(out))
     return out
 
 

@@ -25,6 +25,27 @@ def bin_cells(net, G, lo, h):
     return cell
 
 
+def _spread(v):
+    r = np.zeros_like(v)
+    for b in range(10):
+        r |= ((v >> b) & 1) << (3 * b)
+    return r
+
+
+def sorted_rank(net, G, lo, h):
+    """Position of each binned segment in the near-field order: cells by the
+    Morton code of (cx, cy, cz), then segment index; -1 for unbinned."""
+    cell = bin_cells(net, G, lo, h)
+    ok = cell >= 0
+    c = np.where(ok, cell, 0).astype(np.int64)
+    code = _spread(c % G) | (_spread((c // G) % G) << 1) | (_spread(c // (G * G)) << 2)
+    idx = np.arange(len(cell))
+    o = np.lexsort((idx[ok], code[ok]))
+    rank = np.full(len(cell), -1, dtype=np.int64)
+    rank[idx[ok][o]] = np.arange(ok.sum())
+    return rank
+
+
 def moments(net, G, lo, h):
     n1, n2, b, ext = segments(net)
     cell = bin_cells(net, G, lo, h)

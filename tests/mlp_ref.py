@@ -9,7 +9,8 @@ the finite-difference check pins.
 """
 import numpy as np
 
-N_IN, N_H, N_OUT, N_CHAN = 32, 256, 5408, 5400
+N_IN, N_H, N_OUT, N_CHAN = 128, 512, 5408, 5400
+MONO_DEG = 7
 N_MOM, N_POLY = 90, 10
 
 
@@ -45,18 +46,25 @@ def bf16(x):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def features(off):
+def features(off, aH=0.0):
+    """1/R, 1/R^2, 1/R^3, then direction monomials ux^a uy^b uz^c of degree
+    d = 1..7 in the order a = d..0, b = d-a..0, then 1/R_a, 1/R_a^2, 1/R_a^3
+    with R_a^2 = R^2 + aH^2; zero padded to N_IN."""
     off = np.asarray(off, dtype=np.float64)
     R = np.sqrt((off ** 2).sum(1))
     u = off / R[:, None]
     f = np.zeros((len(off), N_IN))
-    f[:, 0:3] = u
-    f[:, 3] = 1 / R
-    f[:, 4] = 1 / R ** 2
-    f[:, 5:8] = u ** 2
-    f[:, 8] = u[:, 0] * u[:, 1]
-    f[:, 9] = u[:, 0] * u[:, 2]
-    f[:, 10] = u[:, 1] * u[:, 2]
+    f[:, 0] = 1 / R
+    f[:, 1] = 1 / R ** 2
+    f[:, 2] = 1 / R ** 3
+    q = 3
+    for d in range(1, MONO_DEG + 1):
+        for a in range(d, -1, -1):
+            for b in range(d - a, -1, -1):
+                f[:, q] = u[:, 0] ** a * u[:, 1] ** b * u[:, 2] ** (d - a - b)
+                q += 1
+    ra = 1 / np.sqrt(R ** 2 + aH ** 2)
+    f[:, q], f[:, q + 1], f[:, q + 2] = ra, ra ** 2, ra ** 3
     return f
 
 
@@ -104,16 +112,16 @@ def forward(p, off, bf=True):
     return Y, cache
 
 
-def loss_dY(Y, off, M, P, sig, den, envScale):
+def loss_dY(Y, off, M, Lt, den, envScale):
+    """Predicted local expansions L_hat[s, q] = sum_k env W y M (q = 10 c + l),
+    loss sum_s |L_hat_s - Lt_s|^2 / (den_s ns) and its gradient in Y."""
     ns = len(off)
-    C = envelope(off, envScale).reshape(ns, 6, N_POLY, N_MOM)
-    C = C * P[:, None, :, None] * M[:, None, None, :]
-    pred = (C * Y[:, :N_CHAN].reshape(ns, 6, N_POLY, N_MOM)).sum((2, 3))
-    res = pred - sig
+    C = envelope(off, envScale).reshape(ns, 60, N_MOM) * M[:, None, :]
+    pred = (C * Y[:, :N_CHAN].reshape(ns, 60, N_MOM)).sum(2)
+    res = pred - Lt
     loss = ((res ** 2).sum(1) / (den * ns)).sum()
     dY = np.zeros_like(Y)
-    dY[:, :N_CHAN] = (2 * res[:, :, None, None] * C / (den * ns)[:, None, None, None]).reshape(
-        ns, N_CHAN)
+    dY[:, :N_CHAN] = (2 * res[:, :, None] * C / (den * ns)[:, None, None]).reshape(ns, N_CHAN)
     return loss, dY, pred
 
 
@@ -135,9 +143,9 @@ def backward(p, cache, dY, bf=True):
     return pack(g)
 
 
-def loss_grad(p, off, M, P, sig, den, envScale, bf=True):
+def loss_grad(p, off, M, Lt, den, envScale, bf=True):
     Y, cache = forward(p, off, bf)
-    loss, dY, _ = loss_dY(Y, off, M, P, sig, den, envScale)
+    loss, dY, _ = loss_dY(Y, off, M, Lt, den, envScale)
     return loss, backward(p, cache, dY, bf)
 
 
@@ -158,9 +166,21 @@ def random_batch(ns, seed, span=7):
     z = np.all(off == 0, axis=1)
     off[z, 0] = 2
     M = rng.standard_normal((ns, N_MOM))
-    rho = rng.uniform(-0.5, 0.5, size=(ns, 3))
-    x, y, zz = rho.T
-    P = np.stack([np.ones(ns), x, y, zz, x * x, y * y, zz * zz, x * y, x * zz, y * zz], 1)
-    sig = rng.standard_normal((ns, 6)) * 0.05
+    Lt = rng.standard_normal((ns, 60)) * 0.05
     den = rng.uniform(0.5, 2.0, size=ns)
-    return off.astype(np.int32), M, P, sig, den
+    return off.astype(np.int32), M, Lt, den
+
+
+def projection():
+    """Nodes (3) and the 10 x 27 L2 projection onto the quadratic basis of
+    the 27-point tensor Gauss rule on the centred unit cell."""
+    r = np.sqrt(0.6) / 2
+    xi = np.array([-r, 0.0, r])
+    w = np.array([5, 8, 5]) / 18
+    B, W = [], []
+    for p in range(27):
+        x, y, z = xi[p % 3], xi[(p // 3) % 3], xi[p // 9]
+        B.append([1, x, y, z, x * x, y * y, z * z, x * y, x * z, y * z])
+        W.append(w[p % 3] * w[(p // 3) % 3] * w[p // 9])
+    B, W = np.array(B), np.array(W)
+    return xi, np.linalg.solve(B.T @ (W[:, None] * B), B.T * W)

@@ -43,10 +43,18 @@ def test_near_field_exact(prm, Gc, rn):
     n = int(net.numSeg[0])
     cell = H.bin_cells(net, Gc, (0, 0, 0), h)
     assert np.array_equal(ff.debug(2, n), cell.astype(np.int32))
-    o = O.segseg_near(dln, mat, net, cell, Gc, rn)
-    mx, rms = per_node_rel(g, o)
+    # like with like: each near pair once, oriented by the kernel's sorted order
+    once = O.segseg_near_once(dln, mat, net, cell, Gc, rn, H.sorted_rank(net, Gc, (0, 0, 0), h))
+    mx, rms = per_node_rel(g, once)
     print(f"near G={Gc} rn={rn}: pairs {pairs} exact {exact}; max {mx:.2e} rms {rms:.2e}", flush=True)
     assert pairs == exact and mx <= 1e-10
+    # against the row-oriented near field: the pair formula's conditioning
+    o = O.segseg_near(dln, mat, net, cell, Gc, rn)
+    assert per_node_rel(g, o)[1] <= 1e-7
+    # unordered pairs of live segments whose cells are within rn
+    c3 = np.stack([cell % Gc, (cell // Gc) % Gc, cell // (Gc * Gc)], 1)[cell >= 0]
+    cheb = np.abs(c3[:, None, :] - c3[None, :, :]).max(2)
+    assert pairs == int((np.triu(cheb <= rn, 1)).sum())
     # with the mixed rule the far part of the stencil runs in FP32
     gm, pm, em = G.calcSegSegForceHybrid(dln, mat, net, ff)
     assert pm == pairs and em <= pm

@@ -2,7 +2,7 @@
 
 * loss and gradient of ddd_mlp_loss_grad (BF16 MFMA products, FP32
   accumulation) against the numpy restatement that rounds to bf16 at the same
-  points: loss within 5e-3, each parameter block's gradient within 3e-2
+  points: loss within 1e-4, each parameter block's gradient within 3e-3
   normwise (FP32 vs FP64 accumulation and bf16 ties that fall differently);
 * training on the exact stresses of one network lowers the loss, and the
   operator it predicts improves the hybrid forces of another network over the
@@ -41,20 +41,49 @@ def test_loss_grad_vs_reference(ns):
     m = G.FarFieldMLP(seed=1)
     p = R.init_params(3).astype(np.float32)
     m.set_params(p)
-    off, M, P, sig, den = R.random_batch(ns, seed=ns)
+    off, M, Lt, den = R.random_batch(ns, seed=ns)
     f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
-    M, P, sig, den = f32(M), f32(P), f32(sig), f32(den)
+    M, Lt, den = f32(M), f32(Lt), f32(den)
     es = 0.5
-    loss, g = m.loss_grad(off, M, P, sig, den, es)
-    lr, gr = R.loss_grad(p.astype(np.float64), off, M, P, sig, den, es, bf=True)
+    loss, g = m.loss_grad(off, M, Lt, den, es)
+    lr, gr = R.loss_grad(p.astype(np.float64), off, M, Lt, den, es, bf=True)
     print(f"ns={ns}: loss {loss:.6e} ref {lr:.6e}", flush=True)
-    assert abs(loss - lr) <= 5e-3 * abs(lr)
+    assert abs(loss - lr) <= 1e-4 * abs(lr)
     for name, o, shp in R.layout()[0]:
         n = int(np.prod(shp))
         e = normwise(g[o:o + n], gr[o:o + n])
         print(f"  {name}: normwise {e:.2e}", flush=True)
-        assert e <= 3e-2, name
+        assert e <= 3e-3, name
     m.close()
+
+
+def test_local_targets_vs_oracle(prm):
+    """Training targets: the oracle's field stress (calc_σTilde restricted to
+    the source cell's segments) at the 27 Gauss points of the target cell,
+    projected onto the quadratic basis; FP32 output, 1e-6 relative."""
+    import hybrid_ref as H
+    dln, mat = prm
+    Gc, h = 8, 250.0
+    net = random_loop_network(64, seed=0xE3)
+    ff = G.FarField(Gc, (0.0, 0.0, 0.0), h, 1)
+    cell = H.bin_cells(net, Gc, (0, 0, 0), h)
+    rng = np.random.default_rng(1)
+    live = np.unique(cell[cell >= 0])
+    src = rng.choice(live, 6)
+    off = np.array([[2, 0, 0], [-3, 1, 2], [0, 0, -2], [5, -4, 1], [2, 2, 2], [-7, 7, 0]])
+    got = G.farFieldLocalTargets(dln, mat, ff, net, src, off)
+    xi, Pj = R.projection()
+    for s in range(len(src)):
+        c = src[s]
+        tc = (np.array([c % Gc, (c // Gc) % Gc, c // (Gc * Gc)]) - off[s] + 0.5) * h
+        pts = np.array([tc + h * np.array([xi[p % 3], xi[(p // 3) % 3], xi[p // 9]])
+                        for p in range(27)])
+        sg = O.calc_σTilde(pts.T, dln, mat, net, idx=np.nonzero(cell == c)[0] + 1)
+        want = (sg @ Pj.T).ravel()  # 6 x 10, component-major
+        err = np.abs(got[s] - want).max() / np.abs(want).max()
+        print(f"sample {s}: cell {c} offset {off[s]}: rel {err:.2e}", flush=True)
+        assert err <= 1e-6
+    ff.close()
 
 
 def test_training_and_ablation(prm):
@@ -82,7 +111,8 @@ def test_training_and_ablation(prm):
     print(f"seg-seg normwise error: no far field {e_none:.3e}, untrained {e_untrained:.3e}, "
           f"trained {e_trained:.3e}", flush=True)
     assert rep["lossLast"] < 0.3 * rep["lossFirst"]
-    assert rep["relErrVal"] < 0.5
+    # held-out error of all 60 local-expansion coefficients, falling with distance
+    assert rep["relErrVal"] < 0.8 and rep["relErrShell"][5] < rep["relErrShell"][0]
     assert e_trained < 0.7 * e_none and e_trained < 0.7 * e_untrained
     m.close()
     ff.close()
