@@ -67,10 +67,10 @@ def hybrid_paths(args, dln, mat, net, res, stream, ref, fp64_ms, log):
         ffs = {r: G.FarField(Gc, (lo, lo, lo), h, r, ctx=res.ctx) for r in rns}
         mlp = G.FarFieldMLP(seed=11, ctx=res.ctx)
         rep = G.trainFarField(dln, mat, ffs[rns[0]], mlp, train, samples=args.train_samples,
-                              valSamples=1 << 15, steps=args.train_steps, batch=args.train_batch,
-                              lr=1e-3, lrFinal=2e-5, seed=7)
+                              valSamples=1 << 16, steps=args.train_steps, batch=args.train_batch,
+                              lr=2e-3, lrFinal=5e-6, seed=7, denFloor=0.1)
         log(f"G={Gc}: trained in {rep['seconds']:.1f} s, loss {rep['lossFirst']:.3e} -> "
-            f"{rep['lossLast']:.3e}, held-out stress error {rep['relErrVal']:.3e}")
+            f"{rep['lossLast']:.3e}, held-out local-expansion error {rep['relErrVal']:.3e}")
         for r in rns[1:]:
             G.setFarFieldFromMLP(dln, mat, ffs[r], mlp)
         for r in rns:
@@ -109,6 +109,32 @@ def hybrid_paths(args, dln, mat, net, res, stream, ref, fp64_ms, log):
     return out
 
 
+def sigma_leg(args, dln, mat, ctx, log):
+    """calc_σTilde of a 512-loop network (16,384 segments) at the 6 x 128^2
+    face points of the 2000^3 cube: wall time of the synchronous ABI call
+    (host arrays in and out, so PCIe included); kernel time is in the
+    kernel-trace profile.  Work: segment-point pairs x 283 FP64 flops."""
+    import numpy as np
+    from dddhip import forces as G
+    from dddhip.generate import random_loop_network, face_grid_points
+    net = random_loop_network(args.sigma_loops, seed=0xB0)
+    x = face_grid_points(args.sigma_grid)
+    n, P = int(net.numSeg[0]), x.shape[1]
+    σ = np.zeros((6, P), order="F")
+    G.calc_σTilde_(σ, x, dln, mat, net, ctx=ctx)
+    t = []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        G.calc_σTilde_(σ, x, dln, mat, net, ctx=ctx)
+        t.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.mean(t))
+    pairs = n * P
+    log(f"calc_σTilde: {n} segments x {P} points: {ms:.2f} ms (PCIe included)")
+    return {"segments": n, "points": P, "seg_point_pairs": pairs, "ms_pcie_inclusive": round(ms, 3),
+            "tflops_pcie_inclusive": round(283 * pairs / (ms * 1e-3) / 1e12, 3),
+            "finite": bool(np.isfinite(σ).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,15 +146,19 @@ def main():
     ap.add_argument("--cpu-row-stride", type=int, default=64)
     ap.add_argument("--paths", default="fp64,mixed",
                     help="comma list of paths to time after plain FP64: mixed")
-    ap.add_argument("--hybrid", default="16:1,16:2,16:3",
+    ap.add_argument("--hybrid", default="32:4,32:5,32:6",
                     help="comma list of G:rn hybrid configurations (G^3 cells over the "
                          "network's bounding cube, near stencil of Chebyshev radius rn); "
                          "empty to skip")
     ap.add_argument("--train-loops", type=int, default=4096)
     ap.add_argument("--train-seed", type=lambda s: int(s, 0), default=0xD1)
-    ap.add_argument("--train-steps", type=int, default=3000)
-    ap.add_argument("--train-batch", type=int, default=2048)
-    ap.add_argument("--train-samples", type=int, default=1 << 20)
+    ap.add_argument("--train-steps", type=int, default=40000)
+    ap.add_argument("--train-batch", type=int, default=4096)
+    ap.add_argument("--train-samples", type=int, default=1 << 24)
+    ap.add_argument("--sigma-loops", type=int, default=512,
+                    help="calc_σTilde leg: network of this many loops (seed 0xB0) at 6 x g^2 "
+                         "face points; 0 to skip")
+    ap.add_argument("--sigma-grid", type=int, default=128)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -228,6 +258,7 @@ def main():
         res.set_mode("fp64")
 
     hybrid = hybrid_paths(args, dln, mat, net, res, stream, out, ms, log) if args.hybrid else None
+    sigma = sigma_leg(args, dln, mat, ctx, log) if args.sigma_loops > 0 else None
 
     achieved = FLOPS_PER_PAIR * pairs / (tms * 1e-3) / 1e12
     result = {
@@ -256,6 +287,7 @@ def main():
         },
         "paths": extra,
         "hybrid": hybrid,
+        "sigma_tilde": sigma,
         "roofline": {
             "bound": "valu",
             "achieved": round(achieved, 3),
