@@ -1,6 +1,7 @@
 """GPU: the hybrid path's pieces against restatements.
 
-* near field (all FP64) == oracle near-field rows to <= 1e-10 per node;
+* near field (all FP64, FMA-contracted) against the oracle's near pairs in
+  the same orientation to <= 1e-6 per node (the formula's conditioning);
   with the mixed rule, <= 2e-5;
 * cell binning identical, moments to FP32 precision, the FFT correlation
   against a direct sum, and the far-field quadrature, with a random operator.
@@ -43,11 +44,14 @@ def test_near_field_exact(prm, Gc, rn):
     n = int(net.numSeg[0])
     cell = H.bin_cells(net, Gc, (0, 0, 0), h)
     assert np.array_equal(ff.debug(2, n), cell.astype(np.int32))
-    # like with like: each near pair once, oriented by the kernel's sorted order
+    # like with like: each near pair once, oriented by the kernel's sorted
+    # order.  The near-field kernel is built with FMA contraction (the hybrid
+    # is a 1e-3 approximation), so it differs from the uncontracted oracle
+    # within the pair formula's conditioning: ~6e-8 per node here.
     once = O.segseg_near_once(dln, mat, net, cell, Gc, rn, H.sorted_rank(net, Gc, (0, 0, 0), h))
     mx, rms = per_node_rel(g, once)
     print(f"near G={Gc} rn={rn}: pairs {pairs} exact {exact}; max {mx:.2e} rms {rms:.2e}", flush=True)
-    assert pairs == exact and mx <= 1e-10
+    assert pairs == exact and mx <= 1e-6 and rms <= 1e-8
     # against the row-oriented near field: the pair formula's conditioning
     o = O.segseg_near(dln, mat, net, cell, Gc, rn)
     assert per_node_rel(g, o)[1] <= 1e-7
