@@ -135,6 +135,70 @@ def sigma_leg(args, dln, mat, ctx, log):
             "finite": bool(np.isfinite(σ).all())}
 
 
+def clustered_leg(args, dln, mat, ctx, log):
+    """The clustered stress test: 8,192 loops x 32 = 262,144 segments, 70 %
+    in 8 pile-ups of same-slip-system loops (centres N(c, 150^2)), 30 %
+    uniform (seed 0xE0).  Plain FP64 and hybrid (G = 32 over [-300, 2300]^3,
+    rn = 4, network trained on another clustered network, seed 0xE1) whole
+    evaluations with the network resident; hybrid error on 4,096 sampled
+    nodes against the FP64 result; calc_σTilde of the network at 2^20 random
+    surface points (wall time of the synchronous call, PCIe included)."""
+    import numpy as np
+    import torch
+    from dddhip import forces as G
+    from dddhip.generate import clustered_loop_network, surface_points
+    from metrics import per_node_rel
+    net = clustered_loop_network(seed=0xE0)
+    n = int(net.numSeg[0])
+    res = G.ResidentNetwork(dln, mat, net, ctx=ctx)
+    stream = torch.cuda.ExternalStream(res.stream(), device=torch.device("cuda:0"))
+
+    def timed(k):
+        res.seg_force()
+        stream.synchronize()
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(k)]
+        for e in ev:
+            e[0].record(stream)
+            res.seg_force()
+            e[1].record(stream)
+            e[1].synchronize()
+        return float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+
+    fp64_ms = timed(max(1, min(args.steps, 2)))
+    ref = res.download()
+    log(f"clustered: {n} segments, plain FP64 {fp64_ms:.1f} ms")
+    lo, side = -300.0, 2600.0
+    ff = G.FarField(32, (lo, lo, lo), side / 32, 4, ctx=ctx)
+    mlp = G.FarFieldMLP(seed=11, ctx=ctx)
+    rep = G.trainFarField(dln, mat, ff, mlp, clustered_loop_network(seed=0xE1),
+                          samples=args.train_samples, valSamples=1 << 16, steps=args.train_steps,
+                          batch=args.train_batch, lr=2e-3, lrFinal=5e-6, seed=7, denFloor=0.1)
+    res.set_mode("hybrid", farfield=ff)
+    hms = timed(max(1, args.steps))
+    got = res.download()
+    near, near64 = res.stats()
+    Fr, Fg = node_forces(net, ref), node_forces(net, got)
+    pick = np.sort(np.random.default_rng(0xE2).choice(Fr.shape[1], 4096, replace=False))
+    nmx, nrms = per_node_rel(Fg[:, pick], Fr[:, pick])
+    log(f"clustered hybrid: {hms:.1f} ms ({fp64_ms / hms:.1f}x), node err rms {nrms:.2e} "
+        f"max {nmx:.2e} on 4096 nodes")
+    res.set_mode("fp64")
+    ff.close()
+    mlp.close()
+    x = surface_points(1 << 20)
+    σ = np.zeros((6, x.shape[1]), order="F")
+    t0 = time.perf_counter()
+    G.calc_σTilde_(σ, x, dln, mat, net, ctx=ctx)
+    sms = (time.perf_counter() - t0) * 1e3
+    log(f"clustered calc_σTilde at {x.shape[1]} points: {sms:.1f} ms (PCIe included)")
+    return {"segments": n, "fp64_ms": round(fp64_ms, 3), "hybrid_ms": round(hms, 3),
+            "speedup_vs_fp64": round(fp64_ms / hms, 2), "node_rel_err_rms_4096": nrms,
+            "node_rel_err_max_4096": nmx, "near_pairs": near, "near_pairs_fp64": near64,
+            "train_relErrVal": round(rep["relErrVal"], 6), "train_seconds": round(rep["seconds"], 1),
+            "sigma_tilde_points": int(x.shape[1]), "sigma_tilde_ms_pcie_inclusive": round(sms, 1),
+            "sigma_tilde_finite": bool(np.isfinite(σ).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +223,10 @@ def main():
                     help="calc_σTilde leg: network of this many loops (seed 0xB0) at 6 x g^2 "
                          "face points; 0 to skip")
     ap.add_argument("--sigma-grid", type=int, default=128)
+    ap.add_argument("--clustered", action="store_true",
+                    help="also run the clustered stress test: 262,144 segments, 70%% in 8 "
+                         "pile-ups; plain FP64 vs hybrid (G=32, rn=4), error on 4,096 sampled "
+                         "nodes; calc_σTilde at 2^20 surface points")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -259,6 +327,7 @@ def main():
 
     hybrid = hybrid_paths(args, dln, mat, net, res, stream, out, ms, log) if args.hybrid else None
     sigma = sigma_leg(args, dln, mat, ctx, log) if args.sigma_loops > 0 else None
+    clustered = clustered_leg(args, dln, mat, ctx, log) if args.clustered else None
 
     achieved = FLOPS_PER_PAIR * pairs / (tms * 1e-3) / 1e12
     result = {
@@ -288,6 +357,7 @@ def main():
         "paths": extra,
         "hybrid": hybrid,
         "sigma_tilde": sigma,
+        "clustered": clustered,
         "roofline": {
             "bound": "valu",
             "achieved": round(achieved, 3),

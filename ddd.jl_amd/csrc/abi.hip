@@ -88,12 +88,16 @@ struct ddd_farfield {
   }
 };
 
+// Split-K workspace of the weight-gradient products (floats): 4 partials of
+// the largest one (kHidden x kMlpOutPad).
+constexpr int64_t kGemmWork = 4 * (int64_t)kHidden * kMlpOutPad;
+
 struct ddd_mlp {
   DevBuf P, G, Mm, Vv, Wb;
   float aH = 0.f;  // core radius in cell units of the grid it was trained for
   int64_t step = 0;
   int cap = 0;
-  DevBuf X0, Z1, Z2, Z3, H1, H2, H3, Y, dY, dYb, dH, dZb, lossPer, off3, colPart;
+  DevBuf X0, Z1, Z2, Z3, H1, H2, H3, Y, dY, dYb, dH, dZb, lossPer, off3, colPart, gemmWork;
   void ensure_batch(int ns) {
     if (ns <= cap) return;
     X0.ensure(sizeof(unsigned short) * ns * kMlpIn);
@@ -105,11 +109,12 @@ struct ddd_mlp {
     lossPer.ensure(sizeof(float) * ns);
     off3.ensure(sizeof(int) * 3 * ns);
     colPart.ensure(sizeof(float) * 64 * kMlpOutPad);
+    gemmWork.ensure(sizeof(float) * kGemmWork);
     cap = ns;
   }
   void release() {
     DevBuf* all[] = {&P, &G, &Mm, &Vv, &Wb, &X0, &Z1, &Z2, &Z3, &H1, &H2, &H3, &Y, &dY, &dYb,
-                     &dH, &dZb, &lossPer, &off3, &colPart};
+                     &dH, &dZb, &lossPer, &off3, &colPart, &gemmWork};
     for (DevBuf* b : all) b->release();
     cap = 0;
   }
@@ -959,28 +964,29 @@ void mlp_forward(ddd_mlp* m, int ns, hipStream_t st) {
 // Gradient of the loss into m->G from dY (fp32, ns x kMlpOutPad).
 void mlp_backward(ddd_mlp* m, int ns, hipStream_t st) {
   float* G = m->G.as<float>();
+  float* wk = m->gemmWork.as<float>();
   unsigned short* dYb = m->dYb.as<unsigned short>();
   unsigned short* dZ = m->dZb.as<unsigned short>();
   float* dH = m->dH.as<float>();
   mlp_to_bf16(m->dY.as<float>(), (int64_t)ns * kMlpOutPad, dYb, st);
   mlp_gemm(m->H3.as<unsigned short>(), 1, kHidden, dYb, kMlpOutPad, 1, G + kOffW4, kMlpOutPad,
-           kHidden, kMlpOutPad, ns, 0, st);
+           kHidden, kMlpOutPad, ns, 0, st, wk, kGemmWork);
   mlp_colsum(dYb, ns, kMlpOutPad, m->colPart.as<float>(), G + kOffB4, st);
   mlp_gemm(dYb, kMlpOutPad, 1, wb(m, kOffW4), 1, kMlpOutPad, dH, kHidden, ns, kHidden, kMlpOutPad,
            0, st);
   mlp_act_grad(dH, m->Z3.as<float>(), (int64_t)ns * kHidden, dZ, st);
   mlp_gemm(m->H2.as<unsigned short>(), 1, kHidden, dZ, kHidden, 1, G + kOffW3, kHidden, kHidden,
-           kHidden, ns, 0, st);
+           kHidden, ns, 0, st, wk, kGemmWork);
   mlp_colsum(dZ, ns, kHidden, m->colPart.as<float>(), G + kOffB3, st);
   mlp_gemm(dZ, kHidden, 1, wb(m, kOffW3), 1, kHidden, dH, kHidden, ns, kHidden, kHidden, 0, st);
   mlp_act_grad(dH, m->Z2.as<float>(), (int64_t)ns * kHidden, dZ, st);
   mlp_gemm(m->H1.as<unsigned short>(), 1, kHidden, dZ, kHidden, 1, G + kOffW2, kHidden, kHidden,
-           kHidden, ns, 0, st);
+           kHidden, ns, 0, st, wk, kGemmWork);
   mlp_colsum(dZ, ns, kHidden, m->colPart.as<float>(), G + kOffB2, st);
   mlp_gemm(dZ, kHidden, 1, wb(m, kOffW2), 1, kHidden, dH, kHidden, ns, kHidden, kHidden, 0, st);
   mlp_act_grad(dH, m->Z1.as<float>(), (int64_t)ns * kHidden, dZ, st);
   mlp_gemm(m->X0.as<unsigned short>(), 1, kMlpIn, dZ, kHidden, 1, G + kOffW1, kHidden, kMlpIn,
-           kHidden, ns, 0, st);
+           kHidden, ns, 0, st, wk, kGemmWork);
   mlp_colsum(dZ, ns, kHidden, m->colPart.as<float>(), G + kOffB1, st);
 }
 

@@ -13,6 +13,8 @@
 // sum_s |W(o_s) M_s - L_exact_s|^2 / (den_s ns), with the envelope inside W.
 #include "mlp.hpp"
 
+#include <algorithm>
+
 namespace ddd {
 
 namespace {
@@ -95,12 +97,17 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 constexpr int BT = 128, BK = 32, LDK = BK + 8;
 
+//
+// Split K: workgroup z of gridDim.z takes k in [z kChunk, (z + 1) kChunk) and
+// writes its partial product to C + z cStride (accumulate must be 0); a
+// second kernel adds the partials in z order.
 __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __restrict__ A,
                                                         int64_t a_i, int64_t a_k,
                                                         const unsigned short* __restrict__ B,
                                                         int64_t b_k, int64_t b_j,
                                                         float* __restrict__ C, int64_t ldc, int M,
-                                                        int N, int K, int accumulate) {
+                                                        int N, int K, int accumulate, int kChunk,
+                                                        int64_t cStride) {
   __shared__ __attribute__((aligned(16))) unsigned short sA[BT][LDK];
   __shared__ __attribute__((aligned(16))) unsigned short sB[BT][LDK];
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
@@ -115,7 +122,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[m][n][v] = 0.f;
 
-  for (int k0 = 0; k0 < K; k0 += BK) {
+  const int kBeg = blockIdx.z * kChunk;
+  const int kEnd = min(K, kBeg + kChunk);
+  C += blockIdx.z * cStride;
+  for (int k0 = kBeg; k0 < kEnd; k0 += BK) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int v = tid + 256 * h;
@@ -123,22 +133,22 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __
         const int r = v / 4, kq = (v % 4) * 8;
         const int gi = i0 + r, gk = k0 + kq;
         u16x8 x;
-        if (gi < M && gk + 7 < K) {
+        if (gi < M && gk + 7 < kEnd) {
           x = *(const u16x8*)(A + (int64_t)gi * a_i + gk);
         } else {
 #pragma unroll
-          for (int q = 0; q < 8; ++q) x[q] = (gi < M && gk + q < K) ? A[(int64_t)gi * a_i + gk + q] : 0;
+          for (int q = 0; q < 8; ++q) x[q] = (gi < M && gk + q < kEnd) ? A[(int64_t)gi * a_i + gk + q] : 0;
         }
         *(u16x8*)&sA[r][kq] = x;
       } else {  // 8 consecutive rows at one k
         const int kk = v / 16, rq = (v % 16) * 8;
         const int gi = i0 + rq, gk = k0 + kk;
         u16x8 x;
-        if (gk < K && gi + 7 < M) {
+        if (gk < kEnd && gi + 7 < M) {
           x = *(const u16x8*)(A + gi + (int64_t)gk * a_k);
         } else {
 #pragma unroll
-          for (int q = 0; q < 8; ++q) x[q] = (gk < K && gi + q < M) ? A[gi + q + (int64_t)gk * a_k] : 0;
+          for (int q = 0; q < 8; ++q) x[q] = (gk < kEnd && gi + q < M) ? A[gi + q + (int64_t)gk * a_k] : 0;
         }
 #pragma unroll
         for (int q = 0; q < 8; ++q) sA[rq + q][kk] = x[q];
@@ -147,22 +157,22 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __
         const int c = v / 4, kq = (v % 4) * 8;
         const int gj = j0 + c, gk = k0 + kq;
         u16x8 x;
-        if (gj < N && gk + 7 < K) {
+        if (gj < N && gk + 7 < kEnd) {
           x = *(const u16x8*)(B + gk + (int64_t)gj * b_j);
         } else {
 #pragma unroll
-          for (int q = 0; q < 8; ++q) x[q] = (gj < N && gk + q < K) ? B[gk + q + (int64_t)gj * b_j] : 0;
+          for (int q = 0; q < 8; ++q) x[q] = (gj < N && gk + q < kEnd) ? B[gk + q + (int64_t)gj * b_j] : 0;
         }
         *(u16x8*)&sB[c][kq] = x;
       } else {  // 8 consecutive columns at one k
         const int kk = v / 16, cq = (v % 16) * 8;
         const int gj = j0 + cq, gk = k0 + kk;
         u16x8 x;
-        if (gk < K && gj + 7 < N) {
+        if (gk < kEnd && gj + 7 < N) {
           x = *(const u16x8*)(B + (int64_t)gk * b_k + gj);
         } else {
 #pragma unroll
-          for (int q = 0; q < 8; ++q) x[q] = (gk < K && gj + q < N) ? B[(int64_t)gk * b_k + gj + q] : 0;
+          for (int q = 0; q < 8; ++q) x[q] = (gk < kEnd && gj + q < N) ? B[(int64_t)gk * b_k + gj + q] : 0;
         }
 #pragma unroll
         for (int q = 0; q < 8; ++q) sB[cq + q][kk] = x[q];
@@ -198,6 +208,16 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __
           *p = accumulate ? *p + acc[m][n][v] : acc[m][n][v];
         }
       }
+}
+
+// C (+)= sum over z < nSplit of the partials P + z stride (M x N, ldc N).
+__global__ void k_splitk_reduce(const float* __restrict__ P, int64_t stride, int nSplit, int64_t n,
+                                float* __restrict__ C, int accumulate) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float a = 0.f;
+  for (int z = 0; z < nSplit; ++z) a += P[z * stride + e];
+  C[e] = accumulate ? C[e] + a : a;
 }
 
 __device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
@@ -439,15 +459,34 @@ static unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs)
 
 void mlp_gemm(const unsigned short* A, int64_t a_i, int64_t a_k, const unsigned short* B,
               int64_t b_k, int64_t b_j, float* C, int64_t ldc, int M, int N, int K, int accumulate,
-              hipStream_t st) {
+              hipStream_t st, float* work, int64_t workFloats) {
   const bool alignedA = ((uintptr_t)A % 16 == 0) &&
                         (a_k == 1 ? a_i % 8 == 0 : (a_i == 1 && a_k % 8 == 0));
   const bool alignedB = ((uintptr_t)B % 16 == 0) &&
                         (b_k == 1 ? b_j % 8 == 0 : (b_j == 1 && b_k % 8 == 0));
   if (alignedA && alignedB) {
-    dim3 grid(blocks(N, BT), blocks(M, BT));
+    const int64_t tiles = (int64_t)blocks(N, BT) * blocks(M, BT);
+    // Few output tiles and a long K (the weight gradients): split K so that
+    // the grid fills the chip, if the workspace holds the partials.
+    int split = 1;
+    if (tiles < 512 && K >= 4 * BK && ldc == N) {
+      split = (int)std::min<int64_t>((512 + tiles - 1) / tiles, K / (2 * BK));
+      while (split > 1 && (int64_t)split * M * N > workFloats) --split;
+    }
+    if (split > 1) {
+      const int kChunk = ((K + split - 1) / split + BK - 1) / BK * BK;
+      split = (K + kChunk - 1) / kChunk;
+      dim3 grid(blocks(N, BT), blocks(M, BT), split);
+      hipLaunchKernelGGL(k_gemm_bf16_t128, grid, dim3(256), 0, st, A, a_i, a_k, B, b_k, b_j, work,
+                         (int64_t)N, M, N, K, 0, kChunk, (int64_t)M * N);
+      const int64_t n = (int64_t)M * N;
+      hipLaunchKernelGGL(k_splitk_reduce, dim3(blocks(n, 256)), dim3(256), 0, st, work, n, split, n, C,
+                         accumulate);
+      return;
+    }
+    dim3 grid(blocks(N, BT), blocks(M, BT), 1);
     hipLaunchKernelGGL(k_gemm_bf16_t128, grid, dim3(256), 0, st, A, a_i, a_k, B, b_k, b_j, C, ldc, M,
-                       N, K, accumulate);
+                       N, K, accumulate, K, (int64_t)0);
     return;
   }
   dim3 grid(blocks(N, GT), blocks(M, GT));

@@ -21,9 +21,12 @@ constexpr int64_t kOffW4 = kOffB3 + kHidden;
 constexpr int64_t kOffB4 = kOffW4 + (int64_t)kHidden * kMlpOutPad;
 constexpr int64_t kMlpParams = kOffB4 + kMlpOutPad;
 
+// C[M x N] (+)= A B in BF16 MFMA with FP32 accumulation; A(i, k) = A[i a_i + k a_k],
+// B(k, j) = B[k b_k + j b_j].  work (workFloats floats) holds split-K
+// partials when the output has few tiles; pass 0 floats to never split.
 void mlp_gemm(const unsigned short* A, int64_t a_i, int64_t a_k, const unsigned short* B,
               int64_t b_k, int64_t b_j, float* C, int64_t ldc, int M, int N, int K, int accumulate,
-              hipStream_t st);
+              hipStream_t st, float* work = nullptr, int64_t workFloats = 0);
 void mlp_bias_act(float* Z, const float* b, int rows, int cols, int act, unsigned short* H,
                   hipStream_t st);
 void mlp_act_grad(const float* dH, const float* Z, int64_t n, unsigned short* dZ, hipStream_t st);

@@ -96,3 +96,63 @@ def random_uhat(mesh: RegularCuboidMesh, seed: int = 0xF0, density: float = 0.05
     u = splitmix64_uniform(seed, 2 * n)
     uHat = np.where(u[:n] < density, u[n:] * scale, 0.0)
     return ForceDisplacement(uHat=uHat)
+
+
+def clustered_loop_network(numLoops: int = 8192, numSides: int = 32, seed: int = 0xE0,
+                           clusters: int = 8, cluster_frac: float = 0.7, sigma: float = 150.0,
+                           side: float = 2000.0, rmin: float = 100.0,
+                           rmax: float = 300.0) -> DislocationNetwork:
+    """A clustered network: ``cluster_frac`` of the loops form ``clusters``
+    pile-ups, the rest are placed as by random_loop_network.  The loops of a
+    pile-up share one BCC slip system (one Burgers vector and glide plane,
+    so their sides are parallel) and have centres drawn from N(c, sigma^2 I)
+    around a pile-up centre c uniform in [side/4, 3 side/4]^3.  Radius
+    U(rmin, rmax) for every loop.
+
+    Draws: 4 uniforms per pile-up (slip system, centre), then 6 per loop
+    (radius, three for the uniform centre or two Box-Muller pairs' worth for
+    the Gaussian one, slip system for uniform loops)."""
+    ss = SlipSystem.bcc()
+    nc = int(round(cluster_frac * numLoops))
+    uc = splitmix64_uniform(seed, 4 * clusters)
+    cl_sys = [1 + min(11, int(math.floor(12 * uc[4 * c]))) for c in range(clusters)]
+    cl_ctr = [side * (0.25 + 0.5 * uc[4 * c + 1:4 * c + 4]) for c in range(clusters)]
+    u = splitmix64_uniform(seed, 6 * numLoops, offset=4 * clusters)
+    loops = []
+    for k in range(numLoops):
+        v = u[6 * k:6 * k + 6]
+        R = rmin + (rmax - rmin) * v[0]
+        if k < nc:
+            c = k % clusters
+            s = cl_sys[c]
+            # Box-Muller: three normals from two uniform pairs
+            r1 = math.sqrt(-2.0 * math.log(max(v[1], 1e-300)))
+            r2 = math.sqrt(-2.0 * math.log(max(v[3], 1e-300)))
+            g = np.array([r1 * math.cos(2 * math.pi * v[2]), r1 * math.sin(2 * math.pi * v[2]),
+                          r2 * math.cos(2 * math.pi * v[4])])
+            centre = cl_ctr[c] + sigma * g
+        else:
+            s = 1 + min(11, int(math.floor(12 * v[5])))
+            centre = side * v[1:4]
+        L = 2 * R * math.sin(math.pi / numSides)
+        loops.append(DislocationLoop(
+            loopType="shear", numSides=numSides, nodeSide=1, numLoops=1,
+            segLen=L * np.ones(numSides), slipSystemIdx=s, slipSystem=ss,
+            label=[intMobDln] * numSides, buffer=0.0,
+            range=np.array([[0.0, 1.0], [0.0, 1.0], [0.0, 1.0]]),
+            dist=centre.reshape(3, 1)))
+    return DislocationNetwork.from_loops(loops)
+
+
+def surface_points(n: int, seed: int = 0xE5, side: float = 2000.0) -> np.ndarray:
+    """n points uniform on the surface of [0, side]^3 (face chosen uniformly),
+    3 x n column-major."""
+    u = splitmix64_uniform(seed, 3 * n).reshape(n, 3)
+    face = np.minimum(5, (6 * u[:, 0]).astype(np.int64))
+    axis, hi = face // 2, face % 2
+    p = np.zeros((n, 3))
+    other = np.array([[1, 2], [0, 2], [0, 1]])[axis]
+    p[np.arange(n), axis] = hi * side
+    p[np.arange(n), other[:, 0]] = side * u[:, 1]
+    p[np.arange(n), other[:, 1]] = side * u[:, 2]
+    return np.asfortranarray(p.T)

@@ -89,3 +89,60 @@ def test_far_field_pipeline(prm):
     far = H.far_forces(net, Lr, Gc, (0, 0, 0), h)
     diff = g - near
     assert np.abs(diff - far).max() <= 1e-4 * np.abs(far).max()
+
+
+def _near_once(dln, mat, net, Gc, lo, h, rn):
+    cell = H.bin_cells(net, Gc, lo, h)
+    return O.segseg_near_once(dln, mat, net, cell, Gc, rn, H.sorted_rank(net, Gc, lo, h))
+
+
+def test_hybrid_external_segments(prm):
+    """External segments take no part (reference: seg-seg skips them)."""
+    dln, mat = prm
+    net = random_loop_network(48, seed=0xD7, ext_fraction=0.15)
+    assert net.extSeg[:int(net.numSeg[0])].any()
+    Gc, h, rn = 8, 250.0, 1
+    ff = G.FarField(Gc, (0.0, 0.0, 0.0), h, rn)
+    ff.set_operator(np.zeros(((2 * Gc - 1) ** 3, 60, 90), np.float32))
+    g, pairs, exact = G.calcSegSegForceHybrid(dln, mat, net, ff, farDist=1e30)
+    ref = _near_once(dln, mat, net, Gc, (0, 0, 0), h, rn)
+    ext = net.extSeg[:int(net.numSeg[0])].astype(bool)
+    assert not g[:, :, ext].any()
+    assert per_node_rel(g, ref)[0] <= 1e-6
+
+
+def test_hybrid_grid_smaller_than_network(prm):
+    """Segments outside the grid are clamped into its border cells; the near
+    field is still exactly the pairs of the clamped cells."""
+    dln, mat = prm
+    net = random_loop_network(64, seed=0xD8)
+    Gc, lo, h, rn = 6, (500.0, 500.0, 500.0), 170.0, 1
+    ff = G.FarField(Gc, lo, h, rn)
+    ff.set_operator(np.zeros(((2 * Gc - 1) ** 3, 60, 90), np.float32))
+    g, pairs, _ = G.calcSegSegForceHybrid(dln, mat, net, ff, farDist=1e30)
+    assert np.array_equal(ff.debug(2, int(net.numSeg[0])), H.bin_cells(net, Gc, lo, h).astype(np.int32))
+    assert per_node_rel(g, _near_once(dln, mat, net, Gc, lo, h, rn))[0] <= 1e-6
+
+
+def test_hybrid_resident_matches_one_shot_and_repeats(prm):
+    """The resident path (ddd_net_set_opts) gives the one-shot result bit for
+    bit, and repeated evaluations are identical (deterministic sums)."""
+    dln, mat = prm
+    net = random_loop_network(96, seed=0xD9)
+    Gc, h, rn = 8, 250.0, 1
+    ff = G.FarField(Gc, (0.0, 0.0, 0.0), h, rn)
+    rng = np.random.default_rng(9)
+    ff.set_operator((rng.standard_normal(((2 * Gc - 1) ** 3, 60, 90)) * 1e-4).astype(np.float32))
+    one, _, _ = G.calcSegSegForceHybrid(dln, mat, net, ff)
+    res = G.ResidentNetwork(dln, mat, net)
+    res.set_mode("hybrid", farfield=ff)
+    res.seg_force()
+    a = res.download()
+    res.seg_force()
+    b = res.download()
+    s1, s2 = O.calcSelfForce(dln, mat, net)
+    self_ = np.zeros_like(one)
+    self_[:, 0], self_[:, 1] = s1, s2
+    assert np.array_equal(a, b)
+    assert np.allclose(a - self_, one, rtol=0, atol=1e-12 * np.abs(one).max())
+    res.set_mode("fp64")
