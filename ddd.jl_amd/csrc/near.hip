@@ -29,11 +29,12 @@ constexpr int kQueue = 2 * kTile;
 // Near-field tiles of the hybrid path: tile t = (I, J) of the cell-sorted
 // copy, from a list.  Only pairs whose cells lie within Chebyshev distance rn
 // take part (cell coordinates packed in key).  Each slot is classified and
-// compacted into one of two queues -- well-separated, non-parallel pairs
-// (FP32 quadrature of both sides, far_side) and the rest (FP64 analytic,
-// generic or near-parallel) -- and each queue is evaluated 64 pairs at a
-// time, one per lane.  When a pair is queued, its row lane and its column
-// lane (row + k mod 64, told by a shuffle) mark the queue position in 128-bit
+// compacted into one of three queues -- well-separated, non-parallel pairs
+// (FP32 quadrature of both sides, far_side), the other generic pairs (FP64
+// analytic) and near-parallel pairs (FP64, the reference's special branch)
+// -- and each queue is evaluated 64 pairs at a time, one per lane.  When a
+// pair enters one of the first two queues, its row lane and its column lane
+// (row + k mod 64, told by a shuffle) mark the queue position in 128-bit
 // masks; after an evaluation each lane adds the results at its own marked
 // positions in increasing order, so sums are deterministic and row and
 // column totals stay in registers.
@@ -66,6 +67,7 @@ __global__ __launch_bounds__(kTile) void k_near_tiles(const double* __restrict__
   __shared__ double sres[12][kTile];
   __shared__ unsigned short q64[kQueue];
   __shared__ unsigned short q32[kQueue];
+  __shared__ unsigned short qsp[kQueue];
 
   const int64_t i = (int64_t)I * kTile + lane;
   const int64_t jl = (int64_t)J * kTile + lane;
@@ -84,7 +86,7 @@ __global__ __launch_bounds__(kTile) void k_near_tiles(const double* __restrict__
   const int ax = ki & 1023, ay = (ki >> 10) & 1023, az = ki >> 20;
   v3<double> a1 = V(0.0, 0.0, 0.0), a2 = V(0.0, 0.0, 0.0);  // row forces (segment i)
   v3<double> b1 = V(0.0, 0.0, 0.0), b2 = V(0.0, 0.0, 0.0);  // column forces (segment J*64+lane)
-  int n64 = 0, n32 = 0;
+  int n64 = 0, n32 = 0, nsp = 0;
   QMask m64, m32;
   unsigned nNear = 0, nExact = 0;
   const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -102,8 +104,7 @@ __global__ __launch_bounds__(kTile) void k_near_tiles(const double* __restrict__
       if (exact) {
         double c;
         const double omc = one_minus_c2(P.t, Q.t, c);
-        if (omc > DDD_SQRT_EPS) pair_generic<double, true, true>(K, P, Q, c, omc, F1, F2, F3, F4);
-        else special_pair_dev(K, P, Q, F1, F2, F3, F4, flags);
+        pair_generic<double, true, true>(K, P, Q, c, omc, F1, F2, F3, F4);
       } else {
         far_side<float>(KF, P, Q, F1, F2);
         far_side<float>(KF, Q, P, F3, F4);
@@ -135,6 +136,38 @@ __global__ __launch_bounds__(kTile) void k_near_tiles(const double* __restrict__
     nq = rest;
   };
 
+  // Near-parallel pairs (the reference's special branch, several times the
+  // cost of a generic pair) have a queue of their own so that they never
+  // share a wavefront pass with generic pairs; it rarely fills, so its
+  // results are added by a plain ordered scan.
+  auto flushSpecial = [&](int cnt) {
+    if (lane < cnt) {
+      const unsigned e = qsp[lane];
+      const int rl = e & 63, sl = (rl + (e >> 6)) & 63;
+      const Seg<double> P = seg_from_lds(si, rl), Q = seg_from_lds(sj, sl);
+      v3<double> F1, F2, F3, F4;
+      special_pair_dev(K, P, Q, F1, F2, F3, F4, flags);
+      sres[0][lane] = F1.x; sres[1][lane] = F1.y; sres[2][lane] = F1.z;
+      sres[3][lane] = F2.x; sres[4][lane] = F2.y; sres[5][lane] = F2.z;
+      sres[6][lane] = F3.x; sres[7][lane] = F3.y; sres[8][lane] = F3.z;
+      sres[9][lane] = F4.x; sres[10][lane] = F4.y; sres[11][lane] = F4.z;
+    }
+    __syncthreads();
+    for (int e = 0; e < cnt; ++e) {
+      const unsigned qe = qsp[e];
+      const int rl = qe & 63, sl = (rl + (qe >> 6)) & 63;
+      if (rl == lane) {
+        a1 = a1 + V(sres[0][e], sres[1][e], sres[2][e]);
+        a2 = a2 + V(sres[3][e], sres[4][e], sres[5][e]);
+      }
+      if (sl == lane) {
+        b1 = b1 + V(sres[6][e], sres[7][e], sres[8][e]);
+        b2 = b2 + V(sres[9][e], sres[10][e], sres[11][e]);
+      }
+    }
+    __syncthreads();
+  };
+
   for (int k = 0; k < kTile; ++k) {
     const int s = (lane + k) & 63;
     const int kb = kj[s];
@@ -148,8 +181,10 @@ __global__ __launch_bounds__(kTile) void k_near_tiles(const double* __restrict__
                  dy = 0.5 * (sj[F_N1Y][s] + sj[F_N2Y][s]) - midA.y,
                  dz = 0.5 * (sj[F_N1Z][s] + sj[F_N2Z][s]) - midA.z;
     const bool far = (dx * dx + dy * dy + dz * dz > farDist2) && (omcSq > sinMin2);
-    const bool is32 = live && far, is64 = live && !far;
-    const uint64_t w32 = __ballot(is32), w64 = __ballot(is64);
+    const bool is32 = live && far, is64 = live && !far && omcSq > DDD_SQRT_EPS;
+    const bool isSp = live && !far && !(omcSq > DDD_SQRT_EPS);
+    const uint64_t w32 = __ballot(is32), w64 = __ballot(is64), wsp = __ballot(isSp);
+    if (isSp) qsp[nsp + __popcll(wsp & ltMask)] = (unsigned short)((k << 6) | lane);
     const int p32 = is32 ? n32 + __popcll(w32 & ltMask) : -1;
     const int p64 = is64 ? n64 + __popcll(w64 & ltMask) : -1;
     if (is32) {
@@ -167,14 +202,25 @@ __global__ __launch_bounds__(kTile) void k_near_tiles(const double* __restrict__
     if (c64 >= 0) mark(m64.colLo, m64.colHi, c64);
     n32 += __popcll(w32);
     n64 += __popcll(w64);
-    nNear += __popcll(w32) + __popcll(w64);
-    nExact += __popcll(w64);
+    nsp += __popcll(wsp);
+    nNear += __popcll(w32) + __popcll(w64) + __popcll(wsp);
+    nExact += __popcll(w64) + __popcll(wsp);
     __syncthreads();
     if (n32 >= kTile) flush(q32, n32, m32, false);
     if (n64 >= kTile) flush(q64, n64, m64, true);
+    if (nsp >= kTile) {
+      flushSpecial(kTile);
+      const int rest = nsp - kTile;
+      const unsigned short keep = (lane < rest) ? qsp[kTile + lane] : 0;
+      __syncthreads();
+      if (lane < rest) qsp[lane] = keep;
+      __syncthreads();
+      nsp = rest;
+    }
   }
   if (n32 > 0) flush(q32, n32, m32, false);
   if (n64 > 0) flush(q64, n64, m64, true);
+  if (nsp > 0) flushSpecial(nsp);
   if (lane == 0 && nNear) {
     atomicAdd(&counters[0], (unsigned long long)nNear);
     atomicAdd(&counters[1], (unsigned long long)nExact);
