@@ -25,6 +25,9 @@ for p in (ROOT / "ddd.jl_amd", ROOT / "oracle", ROOT / "tests"):
 METRIC = "Force-eval time @131k segs: plain FP64 vs hybrid at ≤1e-3 RMS rel force err"
 FLOPS_PER_PAIR = 1466  # counted by the operation-counting oracle (generic pair)
 FP64_PEAK_TFLOPS = 78.6
+# 26.4 GB read + 42.7 GB written per launch (tile partials, plus call-frame
+# spill traffic of the near-parallel branch), 4,096 loops x 32 segments.
+TILE_TRAFFIC_BYTES = 26.4e9 + 42.7e9
 
 
 def log(msg):
@@ -364,7 +367,11 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "traffic": None,
+            # HBM bytes per launch of the pair-tile kernel on this workload,
+            # FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB copy in
+            # the same process) from two rocprofv3 --pmc runs of the default
+            # bench; only quoted for the default 131,072-segment network.
+            "traffic": TILE_TRAFFIC_BYTES if N == 131072 else None,
             "kernel": "k_segseg_tiles",
             "flops_per_pair": FLOPS_PER_PAIR,
         },

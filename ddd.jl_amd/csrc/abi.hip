@@ -946,18 +946,19 @@ const float* pp(ddd_mlp* m, int64_t off) { return m->P.as<float>() + off; }
 
 // Y = MLP(features of off3) for ns samples (off3 already on the device).
 void mlp_forward(ddd_mlp* m, int ns, hipStream_t st) {
+  float* wk = m->gemmWork.as<float>();
   mlp_features(m->off3.as<int>(), ns, m->aH, m->X0.as<unsigned short>(), st);
   mlp_gemm(m->X0.as<unsigned short>(), kMlpIn, 1, wb(m, kOffW1), kHidden, 1, m->Z1.as<float>(),
-           kHidden, ns, kHidden, kMlpIn, 0, st);
+           kHidden, ns, kHidden, kMlpIn, 0, st, wk, kGemmWork);
   mlp_bias_act(m->Z1.as<float>(), pp(m, kOffB1), ns, kHidden, 1, m->H1.as<unsigned short>(), st);
   mlp_gemm(m->H1.as<unsigned short>(), kHidden, 1, wb(m, kOffW2), kHidden, 1, m->Z2.as<float>(),
-           kHidden, ns, kHidden, kHidden, 0, st);
+           kHidden, ns, kHidden, kHidden, 0, st, wk, kGemmWork);
   mlp_bias_act(m->Z2.as<float>(), pp(m, kOffB2), ns, kHidden, 1, m->H2.as<unsigned short>(), st);
   mlp_gemm(m->H2.as<unsigned short>(), kHidden, 1, wb(m, kOffW3), kHidden, 1, m->Z3.as<float>(),
-           kHidden, ns, kHidden, kHidden, 0, st);
+           kHidden, ns, kHidden, kHidden, 0, st, wk, kGemmWork);
   mlp_bias_act(m->Z3.as<float>(), pp(m, kOffB3), ns, kHidden, 1, m->H3.as<unsigned short>(), st);
   mlp_gemm(m->H3.as<unsigned short>(), kHidden, 1, wb(m, kOffW4), kMlpOutPad, 1, m->Y.as<float>(),
-           kMlpOutPad, ns, kMlpOutPad, kHidden, 0, st);
+           kMlpOutPad, ns, kMlpOutPad, kHidden, 0, st, wk, kGemmWork);
   mlp_bias_act(m->Y.as<float>(), pp(m, kOffB4), ns, kMlpOutPad, 0, nullptr, st);
 }
 
@@ -973,17 +974,17 @@ void mlp_backward(ddd_mlp* m, int ns, hipStream_t st) {
            kHidden, kMlpOutPad, ns, 0, st, wk, kGemmWork);
   mlp_colsum(dYb, ns, kMlpOutPad, m->colPart.as<float>(), G + kOffB4, st);
   mlp_gemm(dYb, kMlpOutPad, 1, wb(m, kOffW4), 1, kMlpOutPad, dH, kHidden, ns, kHidden, kMlpOutPad,
-           0, st);
+           0, st, wk, kGemmWork);
   mlp_act_grad(dH, m->Z3.as<float>(), (int64_t)ns * kHidden, dZ, st);
   mlp_gemm(m->H2.as<unsigned short>(), 1, kHidden, dZ, kHidden, 1, G + kOffW3, kHidden, kHidden,
            kHidden, ns, 0, st, wk, kGemmWork);
   mlp_colsum(dZ, ns, kHidden, m->colPart.as<float>(), G + kOffB3, st);
-  mlp_gemm(dZ, kHidden, 1, wb(m, kOffW3), 1, kHidden, dH, kHidden, ns, kHidden, kHidden, 0, st);
+  mlp_gemm(dZ, kHidden, 1, wb(m, kOffW3), 1, kHidden, dH, kHidden, ns, kHidden, kHidden, 0, st, wk, kGemmWork);
   mlp_act_grad(dH, m->Z2.as<float>(), (int64_t)ns * kHidden, dZ, st);
   mlp_gemm(m->H1.as<unsigned short>(), 1, kHidden, dZ, kHidden, 1, G + kOffW2, kHidden, kHidden,
            kHidden, ns, 0, st, wk, kGemmWork);
   mlp_colsum(dZ, ns, kHidden, m->colPart.as<float>(), G + kOffB2, st);
-  mlp_gemm(dZ, kHidden, 1, wb(m, kOffW2), 1, kHidden, dH, kHidden, ns, kHidden, kHidden, 0, st);
+  mlp_gemm(dZ, kHidden, 1, wb(m, kOffW2), 1, kHidden, dH, kHidden, ns, kHidden, kHidden, 0, st, wk, kGemmWork);
   mlp_act_grad(dH, m->Z1.as<float>(), (int64_t)ns * kHidden, dZ, st);
   mlp_gemm(m->X0.as<unsigned short>(), 1, kMlpIn, dZ, kHidden, 1, G + kOffW1, kHidden, kMlpIn,
            kHidden, ns, 0, st, wk, kGemmWork);
