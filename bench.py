@@ -219,7 +219,7 @@ def main():
                          "empty to skip")
     ap.add_argument("--train-loops", type=int, default=4096)
     ap.add_argument("--train-seed", type=lambda s: int(s, 0), default=0xD1)
-    ap.add_argument("--train-steps", type=int, default=40000)
+    ap.add_argument("--train-steps", type=int, default=60000)
     ap.add_argument("--train-batch", type=int, default=4096)
     ap.add_argument("--train-samples", type=int, default=1 << 24)
     ap.add_argument("--sigma-loops", type=int, default=512,

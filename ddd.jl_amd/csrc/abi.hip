@@ -297,7 +297,7 @@ void hybrid_pairs(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsig
   ff->L.ensure(sizeof(float) * kLocal * nc);
   ff->Mc.ensure(sizeof(float2) * kMoments * ff->vol);
   ff->Lc.ensure(sizeof(float2) * kLocal * ff->vol);
-  ff->counters.ensure(2 * sizeof(unsigned long long));
+  ff->counters.ensure(3 * sizeof(unsigned long long));
   const int64_t ldS = pad64(n1);
   const int64_t nbS = ldS / kTile;
   ff->mcount.ensure(sizeof(unsigned) * nc);
@@ -311,7 +311,8 @@ void hybrid_pairs(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsig
   ff->colCount.ensure(sizeof(unsigned) * nbS);
   ff->colStart.ensure(sizeof(int) * (nbS + 1));
   ff->tileIdx.ensure(sizeof(int) * nbS * nbS);
-  HIPCHK(hipMemsetAsync(ff->counters.p, 0, 2 * sizeof(unsigned long long), c->st));
+  // near pairs, of which FP64, and the near-field kernel's tile counter
+  HIPCHK(hipMemsetAsync(ff->counters.p, 0, 3 * sizeof(unsigned long long), c->st));
   const double* seg = c->seg.as<double>();
   launch_cells(seg, c->resLd, (int)n, ff->g, ff->cellOf.as<int>(), ff->counts.as<unsigned>(),
                ff->cursor.as<unsigned>(), ff->start.as<int>(), ff->tmp.as<int>(),

@@ -88,19 +88,24 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const unsigned short* __restr
 
 // Faster product for aligned operands: 128 x 128 tile per workgroup of 4
 // wavefronts (64 x 64 each, 2 x 2 blocks of v_mfma_f32_32x32x16_bf16), K
-// staged 32 at a time.  Both operands sit in LDS with k contiguous (A as
+// staged 64 at a time.  Both operands sit in LDS with k contiguous (A as
 // [row][k], B as [col][k]), so each lane fetches its 8 k-values with one
 // 16-byte read; global loads are 8-element vectors along whichever index is
-// contiguous.  Requires 16-byte aligned bases, and the non-unit stride of the
-// contiguous operand a multiple of 8.
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
-constexpr int BT = 128, BK = 32, LDK = BK + 8;
-
+// contiguous, and the next K tile is loaded into registers while the current
+// one is multiplied.  Requires 16-byte aligned bases, and the non-unit stride
+// of the contiguous operand a multiple of 8.
 //
 // Split K: workgroup z of gridDim.z takes k in [z kChunk, (z + 1) kChunk) and
 // writes its partial product to C + z cStride (accumulate must be 0); a
 // second kernel adds the partials in z order.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+constexpr int BT = 128, BK = 64, LDK = BK + 8;
+
+struct TileRegs {
+  u16x8 a[4], b[4];
+};
+
 __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __restrict__ A,
                                                         int64_t a_i, int64_t a_k,
                                                         const unsigned short* __restrict__ B,
@@ -114,6 +119,9 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __
   const int i0 = blockIdx.y * BT, j0 = blockIdx.x * BT;
   const int wi = (wave / 2) * 64, wj = (wave % 2) * 64;
   const bool aK = (a_k == 1), bK = (b_k == 1);
+  const int kBeg = blockIdx.z * kChunk;
+  const int kEnd = min(K, kBeg + kChunk);
+  C += blockIdx.z * cStride;
   f32x16 acc[2][2];
 #pragma unroll
   for (int m = 0; m < 2; ++m)
@@ -122,63 +130,85 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[m][n][v] = 0.f;
 
-  const int kBeg = blockIdx.z * kChunk;
-  const int kEnd = min(K, kBeg + kChunk);
-  C += blockIdx.z * cStride;
-  for (int k0 = kBeg; k0 < kEnd; k0 += BK) {
+  // Vector v (0..1023) of the 128 x 64 A tile and of the 64 x 128 B tile.
+  auto fetch = [&](int k0, TileRegs& R) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < 4; ++h) {
       const int v = tid + 256 * h;
       if (aK) {  // 8 consecutive k of one row
-        const int r = v / 4, kq = (v % 4) * 8;
+        const int r = v / 8, kq = (v % 8) * 8;
         const int gi = i0 + r, gk = k0 + kq;
-        u16x8 x;
         if (gi < M && gk + 7 < kEnd) {
-          x = *(const u16x8*)(A + (int64_t)gi * a_i + gk);
+          R.a[h] = *(const u16x8*)(A + (int64_t)gi * a_i + gk);
         } else {
 #pragma unroll
-          for (int q = 0; q < 8; ++q) x[q] = (gi < M && gk + q < kEnd) ? A[(int64_t)gi * a_i + gk + q] : 0;
+          for (int q = 0; q < 8; ++q)
+            R.a[h][q] = (gi < M && gk + q < kEnd) ? A[(int64_t)gi * a_i + gk + q] : 0;
         }
-        *(u16x8*)&sA[r][kq] = x;
       } else {  // 8 consecutive rows at one k
         const int kk = v / 16, rq = (v % 16) * 8;
         const int gi = i0 + rq, gk = k0 + kk;
-        u16x8 x;
         if (gk < kEnd && gi + 7 < M) {
-          x = *(const u16x8*)(A + gi + (int64_t)gk * a_k);
+          R.a[h] = *(const u16x8*)(A + gi + (int64_t)gk * a_k);
         } else {
 #pragma unroll
-          for (int q = 0; q < 8; ++q) x[q] = (gk < kEnd && gi + q < M) ? A[gi + q + (int64_t)gk * a_k] : 0;
+          for (int q = 0; q < 8; ++q)
+            R.a[h][q] = (gk < kEnd && gi + q < M) ? A[gi + q + (int64_t)gk * a_k] : 0;
         }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) sA[rq + q][kk] = x[q];
       }
       if (bK) {  // 8 consecutive k of one column
-        const int c = v / 4, kq = (v % 4) * 8;
+        const int c = v / 8, kq = (v % 8) * 8;
         const int gj = j0 + c, gk = k0 + kq;
-        u16x8 x;
         if (gj < N && gk + 7 < kEnd) {
-          x = *(const u16x8*)(B + gk + (int64_t)gj * b_j);
+          R.b[h] = *(const u16x8*)(B + gk + (int64_t)gj * b_j);
         } else {
 #pragma unroll
-          for (int q = 0; q < 8; ++q) x[q] = (gj < N && gk + q < kEnd) ? B[gk + q + (int64_t)gj * b_j] : 0;
+          for (int q = 0; q < 8; ++q)
+            R.b[h][q] = (gj < N && gk + q < kEnd) ? B[gk + q + (int64_t)gj * b_j] : 0;
         }
-        *(u16x8*)&sB[c][kq] = x;
       } else {  // 8 consecutive columns at one k
         const int kk = v / 16, cq = (v % 16) * 8;
         const int gj = j0 + cq, gk = k0 + kk;
-        u16x8 x;
         if (gk < kEnd && gj + 7 < N) {
-          x = *(const u16x8*)(B + (int64_t)gk * b_k + gj);
+          R.b[h] = *(const u16x8*)(B + (int64_t)gk * b_k + gj);
         } else {
 #pragma unroll
-          for (int q = 0; q < 8; ++q) x[q] = (gk < kEnd && gj + q < N) ? B[(int64_t)gk * b_k + gj + q] : 0;
+          for (int q = 0; q < 8; ++q)
+            R.b[h][q] = (gk < kEnd && gj + q < N) ? B[(int64_t)gk * b_k + gj + q] : 0;
         }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) sB[cq + q][kk] = x[q];
       }
     }
-    __syncthreads();
+  };
+  auto stash = [&](const TileRegs& R) {
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int v = tid + 256 * h;
+      if (aK) {
+        *(u16x8*)&sA[v / 8][(v % 8) * 8] = R.a[h];
+      } else {
+        const int kk = v / 16, rq = (v % 16) * 8;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sA[rq + q][kk] = R.a[h][q];
+      }
+      if (bK) {
+        *(u16x8*)&sB[v / 8][(v % 8) * 8] = R.b[h];
+      } else {
+        const int kk = v / 16, cq = (v % 16) * 8;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sB[cq + q][kk] = R.b[h][q];
+      }
+    }
+  };
+
+  TileRegs R;
+  if (kBeg < kEnd) {
+    fetch(kBeg, R);
+    stash(R);
+  }
+  __syncthreads();
+  for (int k0 = kBeg; k0 < kEnd; k0 += BK) {
+    const bool more = k0 + BK < kEnd;
+    if (more) fetch(k0 + BK, R);  // in flight during the products below
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 16) {
       const int kb = ks + 8 * (lane / 32);
@@ -193,6 +223,8 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __
         for (int n = 0; n < 2; ++n)
           acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[m], b[n], acc[m][n], 0, 0, 0);
     }
+    __syncthreads();
+    if (more) stash(R);
     __syncthreads();
   }
 #pragma unroll

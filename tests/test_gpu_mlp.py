@@ -36,7 +36,7 @@ def test_params_roundtrip():
     m.close()
 
 
-@pytest.mark.parametrize("ns", [200, 64])
+@pytest.mark.parametrize("ns", [1000, 200, 64])
 def test_loss_grad_vs_reference(ns):
     m = G.FarFieldMLP(seed=1)
     p = R.init_params(3).astype(np.float32)
