@@ -72,7 +72,7 @@ struct ddd_farfield {
   GridDev g{};
   int rn = 1;
   int NP = 0;
-  int64_t vol = 0;
+  int64_t vol = 0, volH = 0;  // FFT grid points, half-spectrum frequencies
   bool haveOperator = false;
   DevBuf What, tw, cellOf, counts, cursor, start, tmp, order, M, Mc, Lc, L, counters;
   // near-field tile list
@@ -295,8 +295,8 @@ void hybrid_pairs(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsig
   ff->start.ensure(sizeof(int) * (nc + 1));
   ff->M.ensure(sizeof(float) * kMoments * nc);
   ff->L.ensure(sizeof(float) * kLocal * nc);
-  ff->Mc.ensure(sizeof(float2) * kMoments * ff->vol);
-  ff->Lc.ensure(sizeof(float2) * kLocal * ff->vol);
+  ff->Mc.ensure(sizeof(float2) * kMomPairs * ff->vol);
+  ff->Lc.ensure(sizeof(float2) * kLocPairs * ff->vol);
   ff->counters.ensure(3 * sizeof(unsigned long long));
   const int64_t ldS = pad64(n1);
   const int64_t nbS = ldS / kTile;
@@ -861,6 +861,7 @@ int ddd_farfield_create(ddd_ctx* c, int G, const double lo[3], double h, int nea
     ff->rn = nearRadius;
     ff->NP = fft_size_for(G);
     ff->vol = (int64_t)ff->NP * ff->NP * ff->NP;
+    ff->volH = (int64_t)ff->NP * ff->NP * (ff->NP / 2 + 1);
     std::vector<float2> tw(ff->NP);
     for (int k = 0; k < ff->NP; ++k) {
       const double a = -2.0 * 3.14159265358979323846 * k / ff->NP;
@@ -905,8 +906,8 @@ int ddd_farfield_set_operator(ddd_ctx* c, ddd_farfield* ff, const float* W) {
     DevBuf dW, scratch;
     try {
       dW.ensure(sizeof(float) * nW);
-      scratch.ensure(sizeof(float2) * kChan * ff->vol);
-      ff->What.ensure(sizeof(float2) * kChan * ff->vol);
+      scratch.ensure(sizeof(float2) * kChanPairs * ff->vol);
+      ff->What.ensure(sizeof(float2) * kChan * ff->volH);
       HIPCHK(hipMemcpyAsync(dW.p, W, sizeof(float) * nW, hipMemcpyHostToDevice, c->st));
       launch_operator_spectrum(dW.as<float>(), ff->g, ff->rn, ff->tw.as<float2>(),
                                scratch.as<float2>(), ff->What.as<float2>(), c->st);
@@ -1025,8 +1026,8 @@ void operator_from_mlp(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_const
     mlp_operator_from_y(m->Y.as<float>(), m->off3.as<int>(), cnt, env_scale(k, ff),
                         dW.as<float>() + (int64_t)o0 * kChan, c->st);
   }
-  scratch.ensure(sizeof(float2) * kChan * ff->vol);
-  ff->What.ensure(sizeof(float2) * kChan * ff->vol);
+  scratch.ensure(sizeof(float2) * kChanPairs * ff->vol);
+  ff->What.ensure(sizeof(float2) * kChan * ff->volH);
   launch_operator_spectrum(dW.as<float>(), ff->g, ff->rn, ff->tw.as<float2>(), scratch.as<float2>(),
                            ff->What.as<float2>(), c->st);
   HIPCHK(hipGetLastError());

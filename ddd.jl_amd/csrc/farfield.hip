@@ -193,39 +193,79 @@ __global__ void k_fft_lines(float2* __restrict__ data, int64_t nChan, int axis, 
   for (int k = 0; k < NP; ++k) p[k * stride] = v[k];
 }
 
-// Moments into the zero-padded complex grid.
+// Moments into the zero-padded complex grid, two real channels per complex
+// channel: channel p holds M_{2p} + i M_{2p+1}.
 __global__ void k_pad_moments(const float* __restrict__ M, int G, int NP, float2* __restrict__ out) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t vol = (int64_t)NP * NP * NP;
-  if (e >= (int64_t)kMoments * vol) return;
-  const int f = (int)(e / vol);
+  if (e >= (int64_t)kMomPairs * vol) return;
+  const int p = (int)(e / vol);
   const int r = (int)(e % vol);
   const int x = r % NP, y = (r / NP) % NP, z = r / (NP * NP);
-  float v = 0.f;
-  if (x < G && y < G && z < G) v = M[(int64_t)f * G * G * G + x + G * (y + G * z)];
-  out[e] = make_float2(v, 0.f);
+  float2 v = make_float2(0.f, 0.f);
+  if (x < G && y < G && z < G) {
+    const int64_t c = x + G * (y + G * z), nc = (int64_t)G * G * G;
+    v = make_float2(M[(2 * p) * nc + c], M[(2 * p + 1) * nc + c]);
+  }
+  out[e] = v;
 }
 
-// Lhat(freq)[q] = sum_k What[freq][q][k] Mhat[k](freq).
-__global__ void k_spectral_apply(const float2* __restrict__ What, const float2* __restrict__ Mhat,
-                                 int64_t vol, float2* __restrict__ Lhat) {
-  const int64_t fq = blockIdx.x;
+// Frequency index of -f on the periodic grid.
+__device__ __forceinline__ int64_t neg_freq(int fx, int fy, int fz, int NP) {
+  const int gx = (NP - fx) & (NP - 1), gy = (NP - fy) & (NP - 1), gz = (NP - fz) & (NP - 1);
+  return gx + (int64_t)NP * (gy + (int64_t)NP * gz);
+}
+
+// The spectra of the two real channels packed in Z: with a = Z(f) and
+// b = conj(Z(-f)), X_even(f) = (a + b) / 2 and X_odd(f) = (a - b) / (2i).
+__device__ __forceinline__ float2 unpack_even(float2 a, float2 b) {
+  return make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+}
+__device__ __forceinline__ float2 unpack_odd(float2 a, float2 b) {
+  return make_float2(0.5f * (a.y - b.y), -0.5f * (a.x - b.x));
+}
+
+// L(f) = W(f) M(f) at the half spectrum f_z in [0, NP/2] (real fields: the
+// rest is conjugate), one workgroup per frequency.  The local expansions are
+// written packed two per complex channel at f and, off the self-conjugate
+// planes f_z = 0 and NP/2, at -f, so every frequency is written once.
+__global__ void k_spectral_apply(const float2* __restrict__ What, const float2* __restrict__ Z,
+                                 int NP, float2* __restrict__ Y) {
+  const int64_t fh = blockIdx.x;  // half-spectrum index: fx + NP (fy + NP fz)
+  const int fx = (int)(fh % NP), fy = (int)((fh / NP) % NP), fz = (int)(fh / ((int64_t)NP * NP));
+  const int64_t vol = (int64_t)NP * NP * NP;
+  const int64_t f = fx + (int64_t)NP * (fy + (int64_t)NP * fz);
+  const int64_t g = neg_freq(fx, fy, fz, NP);
   const int q = threadIdx.x;
   __shared__ float2 m[kMoments];
-  for (int k = threadIdx.x; k < kMoments; k += blockDim.x) m[k] = Mhat[(int64_t)k * vol + fq];
-  __syncthreads();
-  if (q >= kLocal) return;
-  const float2* w = What + (fq * kLocal + q) * kMoments;
-  float ax = 0.f, ay = 0.f;
-  for (int k = 0; k < kMoments; ++k) {
-    const float2 a = w[k], b = m[k];
-    ax += a.x * b.x - a.y * b.y;
-    ay += a.x * b.y + a.y * b.x;
+  __shared__ float2 l[kLocal];
+  for (int p = threadIdx.x; p < kMomPairs; p += blockDim.x) {
+    const float2 a = Z[(int64_t)p * vol + f], zb = Z[(int64_t)p * vol + g];
+    const float2 b = make_float2(zb.x, -zb.y);
+    m[2 * p] = unpack_even(a, b);
+    m[2 * p + 1] = unpack_odd(a, b);
   }
-  Lhat[(int64_t)q * vol + fq] = make_float2(ax, ay);
+  __syncthreads();
+  if (q < kLocal) {
+    const float2* w = What + (fh * kLocal + q) * kMoments;
+    float ax = 0.f, ay = 0.f;
+    for (int k = 0; k < kMoments; ++k) {
+      const float2 a = w[k], b = m[k];
+      ax += a.x * b.x - a.y * b.y;
+      ay += a.x * b.y + a.y * b.x;
+    }
+    l[q] = make_float2(ax, ay);
+  }
+  __syncthreads();
+  if (q < kLocPairs) {
+    const float2 a = l[2 * q], b = l[2 * q + 1];
+    Y[(int64_t)q * vol + f] = make_float2(a.x - b.y, a.y + b.x);
+    if (fz != 0 && 2 * fz != NP) Y[(int64_t)q * vol + g] = make_float2(a.x + b.y, b.x - a.y);
+  }
 }
 
-// Real part of the inverse transform at the G^3 cells, scaled by 1/NP^3.
+// The inverse transform at the G^3 cells, scaled by 1/NP^3: local channel
+// q is the real (q even) or imaginary (q odd) part of packed channel q / 2.
 __global__ void k_unpad_local(const float2* __restrict__ Lc, int G, int NP, float* __restrict__ L) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t nc = (int64_t)G * G * G;
@@ -234,41 +274,51 @@ __global__ void k_unpad_local(const float2* __restrict__ Lc, int G, int NP, floa
   const int c = (int)(e % nc);
   const int x = c % G, y = (c / G) % G, z = c / (G * G);
   const float scale = 1.0f / ((float)NP * NP * NP);
-  L[e] = Lc[(int64_t)q * NP * NP * NP + x + NP * (y + NP * z)].x * scale;
+  const float2 v = Lc[(int64_t)(q / 2) * NP * NP * NP + x + NP * (y + NP * z)];
+  L[e] = (q % 2 == 0 ? v.x : v.y) * scale;
 }
 
 // W(o) (offsets o in [-(G-1), G-1]^3, channel-major [o][60][90]) into the
-// periodic grid at index -o mod NP, channel-major, zero inside the near
-// stencil.
+// periodic grid at index -o mod NP, zero inside the near stencil, two real
+// channels per complex channel (p holds channels 2p + i (2p + 1)).
 __global__ void k_place_operator(const float* __restrict__ W, int G, int NP, int rn,
                                  float2* __restrict__ out) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t vol = (int64_t)NP * NP * NP;
-  if (e >= (int64_t)kChan * vol) return;
-  const int ch = (int)(e / vol);
+  if (e >= (int64_t)kChanPairs * vol) return;
+  const int p = (int)(e / vol);
   const int r = (int)(e % vol);
   const int x = r % NP, y = (r / NP) % NP, z = r / (NP * NP);
   // index x holds offset o = -x (mod NP), folded into (-NP/2, NP/2]
   auto off = [&](int i) { int o = -i; if (o <= -NP / 2) o += NP; return o; };
   const int ox = off(x), oy = off(y), oz = off(z);
-  float v = 0.f;
+  float2 v = make_float2(0.f, 0.f);
   const int S = 2 * G - 1;
   if (abs(ox) <= G - 1 && abs(oy) <= G - 1 && abs(oz) <= G - 1 &&
       max(abs(ox), max(abs(oy), abs(oz))) > rn) {
     const int64_t oi = (ox + G - 1) + (int64_t)S * ((oy + G - 1) + (int64_t)S * (oz + G - 1));
-    v = W[oi * kChan + ch];
+    v = make_float2(W[oi * kChan + 2 * p], W[oi * kChan + 2 * p + 1]);
   }
-  out[e] = make_float2(v, 0.f);
+  out[e] = v;
 }
 
-// [ch][freq] -> [freq][ch]
-__global__ void k_transpose_spectrum(const float2* __restrict__ in, int64_t vol,
+// Packed channel-major spectrum -> [half frequency][ch] of the real
+// channels (f_z in [0, NP/2]).
+__global__ void k_transpose_spectrum(const float2* __restrict__ in, int NP,
                                      float2* __restrict__ out) {
+  const int64_t volH = (int64_t)NP * NP * (NP / 2 + 1);
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= (int64_t)kChan * vol) return;
-  const int64_t fq = e / kChan;
+  if (e >= (int64_t)kChan * volH) return;
+  const int64_t fh = e / kChan;
   const int ch = (int)(e % kChan);
-  out[e] = in[(int64_t)ch * vol + fq];
+  const int fx = (int)(fh % NP), fy = (int)((fh / NP) % NP), fz = (int)(fh / ((int64_t)NP * NP));
+  const int64_t vol = (int64_t)NP * NP * NP;
+  const int64_t f = fx + (int64_t)NP * (fy + (int64_t)NP * fz);
+  const int64_t g = neg_freq(fx, fy, fz, NP);
+  const float2* z = in + (int64_t)(ch / 2) * vol;
+  const float2 a = z[f], zb = z[g];
+  const float2 b = make_float2(zb.x, -zb.y);
+  out[e] = (ch % 2 == 0) ? unpack_even(a, b) : unpack_odd(a, b);
 }
 
 __device__ __forceinline__ void local_basis(double x, double y, double z, double P[kPoly]) {
@@ -537,12 +587,13 @@ void launch_far_field(const float* M, const GridDev& g, const float2* What, cons
                       float2* Mc, float2* Lc, float* L, hipStream_t st) {
   const int NP = fft_size_for(g.G);
   const int64_t vol = (int64_t)NP * NP * NP;
+  const int64_t volH = (int64_t)NP * NP * (NP / 2 + 1);
   const int bs = 256;
-  hipLaunchKernelGGL(k_pad_moments, dim3((unsigned)((kMoments * vol + bs - 1) / bs)), dim3(bs), 0, st,
-                     M, g.G, NP, Mc);
-  fft3(Mc, kMoments, NP, 0, tw, st);
-  hipLaunchKernelGGL(k_spectral_apply, dim3((unsigned)vol), dim3(64), 0, st, What, Mc, vol, Lc);
-  fft3(Lc, kLocal, NP, 1, tw, st);
+  hipLaunchKernelGGL(k_pad_moments, dim3((unsigned)((kMomPairs * vol + bs - 1) / bs)), dim3(bs), 0,
+                     st, M, g.G, NP, Mc);
+  fft3(Mc, kMomPairs, NP, 0, tw, st);
+  hipLaunchKernelGGL(k_spectral_apply, dim3((unsigned)volH), dim3(64), 0, st, What, Mc, NP, Lc);
+  fft3(Lc, kLocPairs, NP, 1, tw, st);
   const int64_t nc = (int64_t)g.G * g.G * g.G;
   hipLaunchKernelGGL(k_unpad_local, dim3((unsigned)((kLocal * nc + bs - 1) / bs)), dim3(bs), 0, st,
                      Lc, g.G, NP, L);
@@ -552,12 +603,13 @@ void launch_operator_spectrum(const float* W, const GridDev& g, int rn, const fl
                               float2* scratch, float2* What, hipStream_t st) {
   const int NP = fft_size_for(g.G);
   const int64_t vol = (int64_t)NP * NP * NP;
+  const int64_t volH = (int64_t)NP * NP * (NP / 2 + 1);
   const int bs = 256;
-  hipLaunchKernelGGL(k_place_operator, dim3((unsigned)((kChan * vol + bs - 1) / bs)), dim3(bs), 0, st,
-                     W, g.G, NP, rn, scratch);
-  fft3(scratch, kChan, NP, 0, tw, st);
-  hipLaunchKernelGGL(k_transpose_spectrum, dim3((unsigned)((kChan * vol + bs - 1) / bs)), dim3(bs), 0,
-                     st, scratch, vol, What);
+  hipLaunchKernelGGL(k_place_operator, dim3((unsigned)((kChanPairs * vol + bs - 1) / bs)), dim3(bs), 0,
+                     st, W, g.G, NP, rn, scratch);
+  fft3(scratch, kChanPairs, NP, 0, tw, st);
+  hipLaunchKernelGGL(k_transpose_spectrum, dim3((unsigned)((kChan * volH + bs - 1) / bs)), dim3(bs), 0,
+                     st, scratch, NP, What);
 }
 
 void launch_far_apply(const double* seg, int64_t ld, int n, const GridDev& g, const int* cellOf,

@@ -11,6 +11,10 @@ constexpr int kMoments = 90;  // cell features: orders 0, 1, 2 of b (x) t L
 constexpr int kPoly = 10;     // local basis: 1, x, y, z, x^2, y^2, z^2, xy, xz, yz
 constexpr int kLocal = 6 * kPoly;          // stress components x basis
 constexpr int kChan = kLocal * kMoments;   // entries of one W(o)
+// Real channels travel through the FFTs two per complex channel.
+constexpr int kMomPairs = kMoments / 2;
+constexpr int kLocPairs = kLocal / 2;
+constexpr int kChanPairs = kChan / 2;
 
 struct GridDev {
   double lo[3];
@@ -26,6 +30,9 @@ void launch_moments(const double* seg, int64_t ld, const GridDev& g, const int* 
                     const int* order, float* M, hipStream_t st);
 void launch_far_field(const float* M, const GridDev& g, const float2* What, const float2* tw,
                       float2* Mc, float2* Lc, float* L, hipStream_t st);
+// What: [half frequency][60][90] complex, f_z in [0, NP/2] (NP^2 (NP/2 + 1)
+// frequencies); scratch: kChanPairs NP^3 complex; Mc: kMomPairs NP^3;
+// Lc: kLocPairs NP^3.
 void launch_operator_spectrum(const float* W, const GridDev& g, int rn, const float2* tw,
                               float2* scratch, float2* What, hipStream_t st);
 void launch_far_apply(const double* seg, int64_t ld, int n, const GridDev& g, const int* cellOf,
