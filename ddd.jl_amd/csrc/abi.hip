@@ -136,6 +136,8 @@ struct ddd_ctx {
   DevBuf conn, links, coord, nodeF;
   std::vector<double> h;  // host staging
   DevBuf counter;  // FP64-pair counter of the mixed path
+  // near-parallel pairs of the all-pairs tiles (count, offsets, list, forces)
+  DevBuf specCnt, specOff, specSlot, specTile, specRes;
   // device-resident network
   ddd_opts opts{DDD_MODE_FP64, 0.0, 0.0, nullptr};
   int64_t resN = -1;
@@ -369,25 +371,43 @@ void hybrid_apply(ddd_ctx* c, int64_t n, double* dev, const ddd_opts& o) {
   HIPCHK(hipGetLastError());
 }
 
-// Pair tiles of the whole network (mode from opts) without the reduction.
+// Pair tiles of the whole network (mode from opts) without the reduction:
+// count pass and scan for the near-parallel pairs (one host synchronisation
+// reads their number, which sizes the list), the tile kernel, and the
+// near-parallel pairs' evaluation added to the tile partials.
 void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
                   const ddd_opts* opts) {
   if (n < 2) return;
   const int64_t nb = (n + kTile - 1) / kTile;
   const int64_t T = tiles_for(nb);
+  if (T + 1 > INT32_MAX) bad("too many segments for the tile list");
   const size_t part = sizeof(double) * 6 * kTile * (size_t)T;
   c->rowPart.ensure(part);
   c->colPart.ensure(part);
   c->counter.ensure(sizeof(unsigned long long));
-  if (opts && opts->mode == DDD_MODE_MIXED) {
+  c->specCnt.ensure(sizeof(unsigned) * T);
+  c->specOff.ensure(sizeof(int) * (T + 1));
+  const bool mixed = opts && opts->mode == DDD_MODE_MIXED;
+  const MixedRule r = mixed ? mixed_rule(k, *opts) : MixedRule{};
+  launch_special_count(c->seg.as<double>(), c->resLd, (int)n, mixed ? &r : nullptr,
+                       c->specCnt.as<unsigned>(), c->st);
+  launch_exclusive_scan(c->specCnt.as<unsigned>(), (int)T, c->specOff.as<int>(), c->st);
+  int total = 0;
+  HIPCHK(hipMemcpyAsync(&total, c->specOff.as<int>() + T, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  const int64_t tot1 = std::max(total, 1);
+  c->specSlot.ensure(sizeof(unsigned short) * tot1);
+  c->specTile.ensure(sizeof(int) * tot1);
+  c->specRes.ensure(sizeof(double) * 12 * tot1);
+  SpecialList sp{c->specOff.as<int>(), c->specSlot.as<unsigned short>(), c->specTile.as<int>(),
+                 c->specRes.as<double>(), (int64_t)total, flags};
+  if (mixed) {
     HIPCHK(hipMemsetAsync(c->counter.p, 0, sizeof(unsigned long long), c->st));
-    const MixedRule r = mixed_rule(k, *opts);
-    launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k),
-                        c->rowPart.as<double>(), c->colPart.as<double>(), flags, c->st, &r,
-                        c->counter.as<unsigned long long>());
+    launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k), c->rowPart.as<double>(),
+                        c->colPart.as<double>(), sp, c->st, &r, c->counter.as<unsigned long long>());
   } else {
-    launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k),
-                        c->rowPart.as<double>(), c->colPart.as<double>(), flags, c->st);
+    launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k), c->rowPart.as<double>(),
+                        c->colPart.as<double>(), sp, c->st);
   }
   HIPCHK(hipGetLastError());
 }
@@ -485,7 +505,8 @@ int ddd_ctx_destroy(ddd_ctx* c) {
     (void)hipStreamSynchronize(c->st);
     DevBuf* all[] = {&c->counter, &c->seg, &c->rowPart, &c->colPart, &c->force, &c->rows, &c->out,
                      &c->flags, &c->meshC, &c->meshCoord, &c->meshConn, &c->uHat, &c->sseg,
-                     &c->pts, &c->sigPart, &c->sigma, &c->conn, &c->links, &c->coord, &c->nodeF};
+                     &c->pts, &c->sigPart, &c->sigma, &c->conn, &c->links, &c->coord, &c->nodeF,
+                     &c->specCnt, &c->specOff, &c->specSlot, &c->specTile, &c->specRes};
     for (DevBuf* b : all) b->release();
     if (c->own) (void)hipStreamDestroy(c->own);
   }

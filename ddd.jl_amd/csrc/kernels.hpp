@@ -33,8 +33,22 @@ struct MixedRule {
 };
 // mixed == nullptr: plain FP64.  nExact (mixed only) accumulates the number
 // of pairs evaluated on the FP64 path.
+// Near-parallel pairs of the all-pairs tiles, listed per tile for a separate
+// evaluation: off[t] .. off[t + 1] is tile t's range (exclusive scan of the
+// count pass), slot = (k << 6) | lane, tile = t, res = [12][total] forces.
+struct SpecialList {
+  const int* off;
+  unsigned short* slot;
+  int* tile;
+  double* res;
+  int64_t total;
+  unsigned* flags;
+};
+void launch_special_count(const double* seg, int64_t ld, int n, const MixedRule* mixed,
+                          unsigned* cnt, hipStream_t st);
+void launch_exclusive_scan(const unsigned* cnt, int n, int* off, hipStream_t st);
 void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
-                         double* rowPart, double* colPart, unsigned* flags, hipStream_t st,
+                         double* rowPart, double* colPart, const SpecialList& sp, hipStream_t st,
                          const MixedRule* mixed = nullptr, unsigned long long* nExact = nullptr);
 void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, double* out,
                           hipStream_t st);

@@ -55,29 +55,93 @@ __global__ void k_prep_segments(double* __restrict__ seg, int64_t ld, int64_t n)
 
 constexpr int kQueue = 2 * kTile;
 
-// Pair-tile kernel.  MODE 0 (plain FP64): every pair uses the FP64 analytic
-// force; generic pairs run inline and near-parallel pairs are queued.
-// MODE 1 (mixed precision): well-separated pairs (midpoint distance > farDist
-// and |t1 x t2| > sinMin) run inline in FP32 by quadrature (far_side); all
-// other pairs -- close, near-parallel or both -- are queued and evaluated in
-// FP64 (generic or near-parallel branch) 64 at a time, so the wavefront never
-// waits on a minority path.  The number of FP64 pairs is added to *nExact.
-template <int MODE>
-__global__ __launch_bounds__(kTile) void k_segseg_tiles(const double* __restrict__ seg, int64_t ld,
-                                                        int n, int64_t tile0, SegK<double> K,
-                                                        SigK<float> KF, double farDist2,
-                                                        double sinMin2,
-                                                        double* __restrict__ rowPart,
-                                                        double* __restrict__ colPart,
-                                                        unsigned* __restrict__ flags,
-                                                        unsigned long long* __restrict__ nExact) {
-  const int64_t t = tile0 + blockIdx.x;
-  // Tile t of the upper triangle, enumerated column-block major:
-  // t = J (J + 1) / 2 + I with 0 <= I <= J.
-  int J = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+// Tile t of the upper triangle, enumerated column-block major:
+// t = J (J + 1) / 2 + I with 0 <= I <= J.
+__device__ __forceinline__ void tile_blocks(int64_t t, int& I, int& J) {
+  J = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
   while ((int64_t)J * (J + 1) / 2 > t) --J;
   while ((int64_t)(J + 1) * (J + 2) / 2 <= t) ++J;
-  const int I = (int)(t - (int64_t)J * (J + 1) / 2);
+  I = (int)(t - (int64_t)J * (J + 1) / 2);
+}
+
+// The classification both the count pass and the tile kernel use, so that
+// they agree bit for bit: a live pair is near-parallel (the reference's
+// special branch, 1 - c^2 <= sqrt(eps)) unless, in the mixed mode, it is a
+// well-separated pair for the FP32 quadrature.
+template <int MODE>
+__device__ __forceinline__ bool is_special(bool live, double omcSq, v3<double> midA,
+                                           v3<double> n1B, v3<double> n2B, double farDist2,
+                                           double sinMin2) {
+  if (!live || omcSq > DDD_SQRT_EPS) return false;
+  if (MODE == 0) return true;
+  const double dx = 0.5 * (n1B.x + n2B.x) - midA.x, dy = 0.5 * (n1B.y + n2B.y) - midA.y,
+               dz = 0.5 * (n1B.z + n2B.z) - midA.z;
+  const bool far = (dx * dx + dy * dy + dz * dz > farDist2) && (omcSq > sinMin2);
+  return !far;
+}
+
+// Near-parallel pairs per tile (one wavefront per tile, the tile kernel's
+// slot order), for the offsets of the tile kernel's pair list.
+template <int MODE>
+__global__ __launch_bounds__(kTile) void k_special_count(const double* __restrict__ seg, int64_t ld,
+                                                         int n, int64_t tile0, double farDist2,
+                                                         double sinMin2, unsigned* __restrict__ cnt) {
+  const int64_t t = tile0 + blockIdx.x;
+  int I, J;
+  tile_blocks(t, I, J);
+  const int lane = threadIdx.x;
+  __shared__ double sj[10][kTile];  // n1, n2, t, ext
+  const int64_t i = (int64_t)I * kTile + lane;
+  const int64_t jl = (int64_t)J * kTile + lane;
+  const int fl[10] = {F_N1X, F_N1Y, F_N1Z, F_N2X, F_N2Y, F_N2Z, F_TX, F_TY, F_TZ, F_EXT};
+#pragma unroll
+  for (int f = 0; f < 10; ++f) sj[f][lane] = seg[fl[f] * ld + jl];
+  __syncthreads();
+  const v3<double> tA = V(seg[F_TX * ld + i], seg[F_TY * ld + i], seg[F_TZ * ld + i]);
+  const v3<double> midA = V(0.5 * (seg[F_N1X * ld + i] + seg[F_N2X * ld + i]),
+                            0.5 * (seg[F_N1Y * ld + i] + seg[F_N2Y * ld + i]),
+                            0.5 * (seg[F_N1Z * ld + i] + seg[F_N2Z * ld + i]));
+  const bool rowLive = (i < n) && seg[F_EXT * ld + i] == 0.0;
+  const bool diag = (I == J);
+  unsigned c = 0;
+  for (int k = 0; k < kTile; ++k) {
+    const int s = (lane + k) & 63;
+    const int64_t j = (int64_t)J * kTile + s;
+    const bool live = rowLive && (j < n) && sj[9][s] == 0.0 && (!diag || s > lane);
+    double cc;
+    const double omcSq = one_minus_c2(tA, V(sj[6][s], sj[7][s], sj[8][s]), cc);
+    c += __popcll(__ballot(is_special<MODE>(live, omcSq, midA, V(sj[0][s], sj[1][s], sj[2][s]),
+                                            V(sj[3][s], sj[4][s], sj[5][s]), farDist2, sinMin2)));
+  }
+  if (lane == 0) cnt[t] = c;
+}
+
+// Pair-tile kernel.  MODE 0 (plain FP64): every generic pair runs inline in
+// FP64.  MODE 1 (mixed precision): well-separated pairs (midpoint distance >
+// farDist and |t1 x t2| > sinMin) run inline in FP32 by quadrature
+// (far_side); the other generic pairs are queued and evaluated in FP64 64 at
+// a time, so the wavefront never waits on a minority path.  In both modes
+// near-parallel pairs (the reference's special branch, :1147-1162, several
+// times the cost of a generic pair) are not evaluated here: each is written
+// to the tile's range of the special-pair list (slot order (k, lane), range
+// from the count pass) and evaluated by k_special_eval, whose results
+// k_special_accum adds to the tile's partials.  Keeping that branch out of
+// this kernel lets it run two wavefronts per SIMD.  The number of FP64 pairs
+// is added to *nExact (mixed mode).
+template <int MODE>
+__global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restrict__ seg,
+                                                           int64_t ld, int n, int64_t tile0,
+                                                           SegK<double> K, SigK<float> KF,
+                                                           double farDist2, double sinMin2,
+                                                           double* __restrict__ rowPart,
+                                                           double* __restrict__ colPart,
+                                                           const int* __restrict__ specOff,
+                                                           unsigned short* __restrict__ specSlot,
+                                                           int* __restrict__ specTile,
+                                                           unsigned long long* __restrict__ nExact) {
+  const int64_t t = tile0 + blockIdx.x;
+  int I, J;
+  tile_blocks(t, I, J);
   const int lane = threadIdx.x;
 
   __shared__ double si[F_COUNT][kTile];
@@ -101,11 +165,15 @@ __global__ __launch_bounds__(kTile) void k_segseg_tiles(const double* __restrict
   const bool rowLive = (i < n) && si[F_EXT][lane] == 0.0;
   const bool diag = (I == J);
   v3<double> a1 = V(0.0, 0.0, 0.0), a2 = V(0.0, 0.0, 0.0);
-  int nq = 0;  // wave-uniform queue length
+  int nq = 0;        // wave-uniform queue length (mixed mode)
+  int nsp = specOff[t];  // next slot of this tile's special-pair range
+  const int spEnd = specOff[t + 1];
   unsigned exact = 0;
   const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const v3<double> midA = V(0.5 * (A.n1.x + A.n2.x), 0.5 * (A.n1.y + A.n2.y), 0.5 * (A.n1.z + A.n2.z));
 
+  // Mixed mode: evaluate the first cnt queued generic pairs in FP64 and add
+  // them to rows and columns in queue order.
   auto flush = [&](int cnt) {
     if (lane < cnt) {
       const unsigned e = queue[lane];
@@ -114,11 +182,7 @@ __global__ __launch_bounds__(kTile) void k_segseg_tiles(const double* __restrict
       v3<double> F1, F2, F3, F4;
       double c;
       const double omc = one_minus_c2(P.t, Q.t, c);
-      if (MODE == 1 && omc > DDD_SQRT_EPS) {
-        pair_generic<double, true, true>(K, P, Q, c, omc, F1, F2, F3, F4);
-      } else {
-        special_pair_dev(K, P, Q, F1, F2, F3, F4, flags);
-      }
+      pair_generic<double, true, true>(K, P, Q, c, omc, F1, F2, F3, F4);
       sres[0][lane] = F1.x; sres[1][lane] = F1.y; sres[2][lane] = F1.z;
       sres[3][lane] = F2.x; sres[4][lane] = F2.y; sres[5][lane] = F2.z;
       sres[6][lane] = F3.x; sres[7][lane] = F3.y; sres[8][lane] = F3.z;
@@ -147,15 +211,25 @@ __global__ __launch_bounds__(kTile) void k_segseg_tiles(const double* __restrict
     const Seg<double> B = seg_from_lds(sj, s);
     double c;
     const double omcSq = one_minus_c2(A.t, B.t, c);
-    bool queued, inlineFP64 = false;
+    const bool special = is_special<MODE>(live, omcSq, midA, B.n1, B.n2, farDist2, sinMin2);
+    const uint64_t msp = __ballot(special);
+    if (special) {
+      const int pos = nsp + __popcll(msp & ltMask);
+      if (pos < spEnd) {  // always true: the count pass classifies identically
+        specSlot[pos] = (unsigned short)((k << 6) | lane);
+        specTile[pos] = (int)t;
+      }
+    }
+    nsp += __popcll(msp);
+    bool queued = false, inlineFP64 = false;
     if (MODE == 0) {
-      queued = live && !(omcSq > DDD_SQRT_EPS);
-      inlineFP64 = live && !queued;
+      inlineFP64 = live && !special;
     } else {
       const double dx = 0.5 * (B.n1.x + B.n2.x) - midA.x, dy = 0.5 * (B.n1.y + B.n2.y) - midA.y,
                    dz = 0.5 * (B.n1.z + B.n2.z) - midA.z;
       const bool far = (dx * dx + dy * dy + dz * dz > farDist2) && (omcSq > sinMin2);
-      queued = live && !far;
+      queued = live && !far && !special;
+      exact += __popcll(msp);
       if (live && far) {
         v3<double> F1, F2, F3, F4;
         far_side<float>(KF, A, B, F1, F2);
@@ -166,11 +240,13 @@ __global__ __launch_bounds__(kTile) void k_segseg_tiles(const double* __restrict
         cacc[3][s] += F4.x; cacc[4][s] += F4.y; cacc[5][s] += F4.z;
       }
     }
-    const uint64_t m = __ballot(queued);
-    if (m) {
-      if (queued) queue[nq + __popcll(m & ltMask)] = (unsigned short)((k << 6) | lane);
-      nq += __popcll(m);
-      exact += __popcll(m);
+    if (MODE == 1) {
+      const uint64_t m = __ballot(queued);
+      if (m) {
+        if (queued) queue[nq + __popcll(m & ltMask)] = (unsigned short)((k << 6) | lane);
+        nq += __popcll(m);
+        exact += __popcll(m);
+      }
     }
     if (MODE == 0 && inlineFP64) {
       v3<double> F1, F2, F3, F4;
@@ -181,7 +257,7 @@ __global__ __launch_bounds__(kTile) void k_segseg_tiles(const double* __restrict
       cacc[3][s] += F4.x; cacc[4][s] += F4.y; cacc[5][s] += F4.z;
     }
     __syncthreads();
-    if (nq >= kTile) {
+    if (MODE == 1 && nq >= kTile) {
       flush(kTile);
       const int rest = nq - kTile;
       unsigned short keep = (lane < rest) ? queue[kTile + lane] : 0;
@@ -191,7 +267,7 @@ __global__ __launch_bounds__(kTile) void k_segseg_tiles(const double* __restrict
       nq = rest;
     }
   }
-  if (nq > 0) flush(nq);
+  if (MODE == 1 && nq > 0) flush(nq);
   if (MODE == 1 && lane == 0 && exact) atomicAdd(nExact, (unsigned long long)exact);
 
   double* rp = rowPart + t * 6 * kTile;
@@ -200,6 +276,63 @@ __global__ __launch_bounds__(kTile) void k_segseg_tiles(const double* __restrict
   rp[3 * kTile + lane] = a2.x; rp[4 * kTile + lane] = a2.y; rp[5 * kTile + lane] = a2.z;
 #pragma unroll
   for (int c = 0; c < 6; ++c) cp[c * kTile + lane] = cacc[c][lane];
+}
+
+// Near-parallel pairs of the special-pair list, one per lane, forces
+// [12][total] (F1, F2 on the row segment, F3, F4 on the column segment).
+__global__ __launch_bounds__(kTile) void k_special_eval(const double* __restrict__ seg, int64_t ld,
+                                                        const unsigned short* __restrict__ specSlot,
+                                                        const int* __restrict__ specTile,
+                                                        int64_t total, SegK<double> K,
+                                                        double* __restrict__ res,
+                                                        unsigned* __restrict__ flags) {
+  const int64_t e = blockIdx.x * (int64_t)kTile + threadIdx.x;
+  if (e >= total) return;
+  int I, J;
+  tile_blocks(specTile[e], I, J);
+  const unsigned q = specSlot[e];
+  const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
+  const Seg<double> P = seg_from(seg, ld, (int64_t)I * kTile + rl);
+  const Seg<double> Q = seg_from(seg, ld, (int64_t)J * kTile + sl);
+  v3<double> F1, F2, F3, F4;
+  special_pair_dev(K, P, Q, F1, F2, F3, F4, flags);
+  const double v[12] = {F1.x, F1.y, F1.z, F2.x, F2.y, F2.z, F3.x, F3.y, F3.z, F4.x, F4.y, F4.z};
+#pragma unroll
+  for (int c = 0; c < 12; ++c) res[c * total + e] = v[c];
+}
+
+// Per tile, its near-parallel pairs' forces added to the row and column
+// partial sums in list order (one wavefront per tile).
+__global__ __launch_bounds__(kTile) void k_special_accum(const int* __restrict__ specOff,
+                                                         const unsigned short* __restrict__ specSlot,
+                                                         const double* __restrict__ res,
+                                                         int64_t total, int64_t tile0,
+                                                         double* __restrict__ rowPart,
+                                                         double* __restrict__ colPart) {
+  const int64_t t = tile0 + blockIdx.x;
+  const int e0 = specOff[t], e1 = specOff[t + 1];
+  if (e0 == e1) return;
+  const int lane = threadIdx.x;
+  double r[6] = {0, 0, 0, 0, 0, 0}, c[6] = {0, 0, 0, 0, 0, 0};
+  for (int e = e0; e < e1; ++e) {
+    const unsigned q = specSlot[e];
+    const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
+    if (rl == lane) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) r[k] += res[k * total + e];
+    }
+    if (sl == lane) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) c[k] += res[(6 + k) * total + e];
+    }
+  }
+  double* rp = rowPart + t * 6 * kTile + lane;
+  double* cp = colPart + t * 6 * kTile + lane;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    rp[k * kTile] += r[k];
+    cp[k * kTile] += c[k];
+  }
 }
 
 // Sum of a segment's partials: row role in tiles (B, J), J >= B, then column
@@ -274,8 +407,26 @@ void launch_prep_segments(double* seg, int64_t ld, int64_t n, hipStream_t st) {
                      ld, n);
 }
 
+void launch_special_count(const double* seg, int64_t ld, int n, const MixedRule* mixed,
+                          unsigned* cnt, hipStream_t st) {
+  const int64_t nb = (n + kTile - 1) / kTile;
+  const int64_t T = tiles_for(nb);
+  const int64_t maxGrid = 1ll << 30;
+  const double fd2 = mixed ? mixed->farDist * mixed->farDist : 0.0;
+  const double sm2 = mixed ? mixed->sinMin * mixed->sinMin : 0.0;
+  for (int64_t t0 = 0; t0 < T; t0 += maxGrid) {
+    const int64_t count = (T - t0 < maxGrid) ? (T - t0) : maxGrid;
+    if (mixed)
+      hipLaunchKernelGGL(k_special_count<1>, dim3((unsigned)count), dim3(kTile), 0, st, seg, ld, n, t0,
+                         fd2, sm2, cnt);
+    else
+      hipLaunchKernelGGL(k_special_count<0>, dim3((unsigned)count), dim3(kTile), 0, st, seg, ld, n, t0,
+                         fd2, sm2, cnt);
+  }
+}
+
 void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
-                         double* rowPart, double* colPart, unsigned* flags, hipStream_t st,
+                         double* rowPart, double* colPart, const SpecialList& sp, hipStream_t st,
                          const MixedRule* mixed, unsigned long long* nExact) {
   const int64_t nb = (n + kTile - 1) / kTile;
   const int64_t T = tiles_for(nb);
@@ -287,10 +438,19 @@ void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double
     const int64_t cnt = (T - t0 < maxGrid) ? (T - t0) : maxGrid;
     if (mixed)
       hipLaunchKernelGGL(k_segseg_tiles<1>, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, t0,
-                         K, KF, fd2, sm2, rowPart, colPart, flags, nExact);
+                         K, KF, fd2, sm2, rowPart, colPart, sp.off, sp.slot, sp.tile, nExact);
     else
       hipLaunchKernelGGL(k_segseg_tiles<0>, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, t0,
-                         K, KF, fd2, sm2, rowPart, colPart, flags, nExact);
+                         K, KF, fd2, sm2, rowPart, colPart, sp.off, sp.slot, sp.tile, nExact);
+  }
+  if (sp.total > 0) {
+    hipLaunchKernelGGL(k_special_eval, dim3((unsigned)((sp.total + kTile - 1) / kTile)), dim3(kTile), 0,
+                       st, seg, ld, sp.slot, sp.tile, sp.total, K, sp.res, sp.flags);
+    for (int64_t t0 = 0; t0 < T; t0 += maxGrid) {
+      const int64_t cnt = (T - t0 < maxGrid) ? (T - t0) : maxGrid;
+      hipLaunchKernelGGL(k_special_accum, dim3((unsigned)cnt), dim3(kTile), 0, st, sp.off, sp.slot,
+                         sp.res, sp.total, t0, rowPart, colPart);
+    }
   }
 }
 
