@@ -137,7 +137,7 @@ struct ddd_ctx {
   std::vector<double> h;  // host staging
   DevBuf counter;  // FP64-pair counter of the mixed path
   // near-parallel pairs of the all-pairs tiles (count, offsets, list, forces)
-  DevBuf specCnt, specOff, specSlot, specTile, specRes;
+  DevBuf specCnt, specOff, specSlot, specTile, specRes, specWork;
   // device-resident network
   ddd_opts opts{DDD_MODE_FP64, 0.0, 0.0, nullptr};
   int64_t resN = -1;
@@ -373,8 +373,8 @@ void hybrid_apply(ddd_ctx* c, int64_t n, double* dev, const ddd_opts& o) {
 
 // Pair tiles of the whole network (mode from opts) without the reduction:
 // count pass and scan for the near-parallel pairs (one host synchronisation
-// reads their number, which sizes the list), the tile kernel, and the
-// near-parallel pairs' evaluation added to the tile partials.
+// reads their number, which sizes the list), their listing and evaluation,
+// then the tile kernel, which adds them to its partial sums.
 void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
                   const ddd_opts* opts) {
   if (n < 2) return;
@@ -391,7 +391,9 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
   const MixedRule r = mixed ? mixed_rule(k, *opts) : MixedRule{};
   launch_special_count(c->seg.as<double>(), c->resLd, (int)n, mixed ? &r : nullptr,
                        c->specCnt.as<unsigned>(), c->st);
-  launch_exclusive_scan(c->specCnt.as<unsigned>(), (int)T, c->specOff.as<int>(), c->st);
+  c->specWork.ensure(sizeof(int) * scan_work_ints((int)T));
+  launch_exclusive_scan(c->specCnt.as<unsigned>(), (int)T, c->specOff.as<int>(), c->specWork.as<int>(),
+                        c->st);
   int total = 0;
   HIPCHK(hipMemcpyAsync(&total, c->specOff.as<int>() + T, sizeof(int), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -401,6 +403,8 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
   c->specRes.ensure(sizeof(double) * 12 * tot1);
   SpecialList sp{c->specOff.as<int>(), c->specSlot.as<unsigned short>(), c->specTile.as<int>(),
                  c->specRes.as<double>(), (int64_t)total, flags};
+  launch_special_list(c->seg.as<double>(), c->resLd, (int)n, mixed ? &r : nullptr, sp, seg_consts(k),
+                      c->st);
   if (mixed) {
     HIPCHK(hipMemsetAsync(c->counter.p, 0, sizeof(unsigned long long), c->st));
     launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k), c->rowPart.as<double>(),
@@ -506,7 +510,7 @@ int ddd_ctx_destroy(ddd_ctx* c) {
     DevBuf* all[] = {&c->counter, &c->seg, &c->rowPart, &c->colPart, &c->force, &c->rows, &c->out,
                      &c->flags, &c->meshC, &c->meshCoord, &c->meshConn, &c->uHat, &c->sseg,
                      &c->pts, &c->sigPart, &c->sigma, &c->conn, &c->links, &c->coord, &c->nodeF,
-                     &c->specCnt, &c->specOff, &c->specSlot, &c->specTile, &c->specRes};
+                     &c->specCnt, &c->specOff, &c->specSlot, &c->specTile, &c->specRes, &c->specWork};
     for (DevBuf* b : all) b->release();
     if (c->own) (void)hipStreamDestroy(c->own);
   }

@@ -638,10 +638,6 @@ void morton_ranks(int G, std::vector<int>& mrank, std::vector<int>& rankCell) {
   }
 }
 
-void launch_exclusive_scan(const unsigned* cnt, int n, int* off, hipStream_t st) {
-  hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, st, cnt, n, off);
-}
-
 void launch_near_list_count(const double* seg, int64_t ld, int n, const GridDev& g, int rn,
                             NearListDev& L, hipStream_t st) {
   const int nc = g.G * g.G * g.G;

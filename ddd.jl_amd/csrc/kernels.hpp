@@ -33,9 +33,10 @@ struct MixedRule {
 };
 // mixed == nullptr: plain FP64.  nExact (mixed only) accumulates the number
 // of pairs evaluated on the FP64 path.
-// Near-parallel pairs of the all-pairs tiles, listed per tile for a separate
-// evaluation: off[t] .. off[t + 1] is tile t's range (exclusive scan of the
-// count pass), slot = (k << 6) | lane, tile = t, res = [12][total] forces.
+// Near-parallel pairs of the all-pairs tiles, listed per tile and evaluated
+// before the tile kernel: off[t] .. off[t + 1] is tile t's range (exclusive
+// scan of the count pass), slot = (k << 6) | lane, tile = t, res = [12][total]
+// forces (F1, F2 on the row segment, F3, F4 on the column segment).
 struct SpecialList {
   const int* off;
   unsigned short* slot;
@@ -44,9 +45,16 @@ struct SpecialList {
   int64_t total;
   unsigned* flags;
 };
+// Count pass (cnt[t] per tile), then, with off = exclusive scan of cnt and
+// the list sized, the listing pass and the pairs' evaluation into res.
 void launch_special_count(const double* seg, int64_t ld, int n, const MixedRule* mixed,
                           unsigned* cnt, hipStream_t st);
-void launch_exclusive_scan(const unsigned* cnt, int n, int* off, hipStream_t st);
+void launch_special_list(const double* seg, int64_t ld, int n, const MixedRule* mixed,
+                         const SpecialList& sp, const SegK<double>& K, hipStream_t st);
+// off[0..n] = exclusive scan of cnt[0..n) (off[n] = total); work holds
+// scan_work_ints(n) ints.
+int64_t scan_work_ints(int n);
+void launch_exclusive_scan(const unsigned* cnt, int n, int* off, int* work, hipStream_t st);
 void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
                          double* rowPart, double* colPart, const SpecialList& sp, hipStream_t st,
                          const MixedRule* mixed = nullptr, unsigned long long* nExact = nullptr);

@@ -80,12 +80,16 @@ __device__ __forceinline__ bool is_special(bool live, double omcSq, v3<double> m
   return !far;
 }
 
-// Near-parallel pairs per tile (one wavefront per tile, the tile kernel's
-// slot order), for the offsets of the tile kernel's pair list.
-template <int MODE>
-__global__ __launch_bounds__(kTile) void k_special_count(const double* __restrict__ seg, int64_t ld,
-                                                         int n, int64_t tile0, double farDist2,
-                                                         double sinMin2, unsigned* __restrict__ cnt) {
+// Near-parallel pairs per tile, one wavefront per tile in the tile kernel's
+// slot order (k, lane): LIST = 0 counts them (cnt[t]); LIST = 1 writes them
+// to the tile's range off[t] .. off[t + 1] of the pair list.
+template <int MODE, int LIST>
+__global__ __launch_bounds__(kTile) void k_special_pass(const double* __restrict__ seg, int64_t ld,
+                                                        int n, int64_t tile0, double farDist2,
+                                                        double sinMin2, unsigned* __restrict__ cnt,
+                                                        const int* __restrict__ off,
+                                                        unsigned short* __restrict__ slot,
+                                                        int* __restrict__ tileOf) {
   const int64_t t = tile0 + blockIdx.x;
   int I, J;
   tile_blocks(t, I, J);
@@ -103,17 +107,28 @@ __global__ __launch_bounds__(kTile) void k_special_count(const double* __restric
                             0.5 * (seg[F_N1Z * ld + i] + seg[F_N2Z * ld + i]));
   const bool rowLive = (i < n) && seg[F_EXT * ld + i] == 0.0;
   const bool diag = (I == J);
-  unsigned c = 0;
+  const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int c = LIST ? off[t] : 0;
+  const int cEnd = LIST ? off[t + 1] : 0;
   for (int k = 0; k < kTile; ++k) {
     const int s = (lane + k) & 63;
     const int64_t j = (int64_t)J * kTile + s;
     const bool live = rowLive && (j < n) && sj[9][s] == 0.0 && (!diag || s > lane);
     double cc;
     const double omcSq = one_minus_c2(tA, V(sj[6][s], sj[7][s], sj[8][s]), cc);
-    c += __popcll(__ballot(is_special<MODE>(live, omcSq, midA, V(sj[0][s], sj[1][s], sj[2][s]),
-                                            V(sj[3][s], sj[4][s], sj[5][s]), farDist2, sinMin2)));
+    const bool sp = is_special<MODE>(live, omcSq, midA, V(sj[0][s], sj[1][s], sj[2][s]),
+                                     V(sj[3][s], sj[4][s], sj[5][s]), farDist2, sinMin2);
+    const uint64_t m = __ballot(sp);
+    if (LIST && sp) {
+      const int pos = c + __popcll(m & ltMask);
+      if (pos < cEnd) {  // always true: both passes classify identically
+        slot[pos] = (unsigned short)((k << 6) | lane);
+        tileOf[pos] = (int)t;
+      }
+    }
+    c += __popcll(m);
   }
-  if (lane == 0) cnt[t] = c;
+  if (!LIST && lane == 0) cnt[t] = (unsigned)c;
 }
 
 // Pair-tile kernel.  MODE 0 (plain FP64): every generic pair runs inline in
@@ -122,12 +137,11 @@ __global__ __launch_bounds__(kTile) void k_special_count(const double* __restric
 // (far_side); the other generic pairs are queued and evaluated in FP64 64 at
 // a time, so the wavefront never waits on a minority path.  In both modes
 // near-parallel pairs (the reference's special branch, :1147-1162, several
-// times the cost of a generic pair) are not evaluated here: each is written
-// to the tile's range of the special-pair list (slot order (k, lane), range
-// from the count pass) and evaluated by k_special_eval, whose results
-// k_special_accum adds to the tile's partials.  Keeping that branch out of
-// this kernel lets it run two wavefronts per SIMD.  The number of FP64 pairs
-// is added to *nExact (mixed mode).
+// times the cost of a generic pair) are not evaluated here: they were listed
+// per tile and evaluated beforehand (k_special_pass, k_special_eval), and
+// the tile adds its list's forces, in list order, after its own pairs.
+// Keeping that branch out of this kernel lets it run two wavefronts per
+// SIMD.  The number of FP64 pairs is added to *nExact (mixed mode).
 template <int MODE>
 __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restrict__ seg,
                                                            int64_t ld, int n, int64_t tile0,
@@ -136,8 +150,9 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
                                                            double* __restrict__ rowPart,
                                                            double* __restrict__ colPart,
                                                            const int* __restrict__ specOff,
-                                                           unsigned short* __restrict__ specSlot,
-                                                           int* __restrict__ specTile,
+                                                           const unsigned short* __restrict__ specSlot,
+                                                           const double* __restrict__ specRes,
+                                                           int64_t specTotal,
                                                            unsigned long long* __restrict__ nExact) {
   const int64_t t = tile0 + blockIdx.x;
   int I, J;
@@ -166,8 +181,7 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
   const bool diag = (I == J);
   v3<double> a1 = V(0.0, 0.0, 0.0), a2 = V(0.0, 0.0, 0.0);
   int nq = 0;        // wave-uniform queue length (mixed mode)
-  int nsp = specOff[t];  // next slot of this tile's special-pair range
-  const int spEnd = specOff[t + 1];
+
   unsigned exact = 0;
   const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const v3<double> midA = V(0.5 * (A.n1.x + A.n2.x), 0.5 * (A.n1.y + A.n2.y), 0.5 * (A.n1.z + A.n2.z));
@@ -212,15 +226,6 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
     double c;
     const double omcSq = one_minus_c2(A.t, B.t, c);
     const bool special = is_special<MODE>(live, omcSq, midA, B.n1, B.n2, farDist2, sinMin2);
-    const uint64_t msp = __ballot(special);
-    if (special) {
-      const int pos = nsp + __popcll(msp & ltMask);
-      if (pos < spEnd) {  // always true: the count pass classifies identically
-        specSlot[pos] = (unsigned short)((k << 6) | lane);
-        specTile[pos] = (int)t;
-      }
-    }
-    nsp += __popcll(msp);
     bool queued = false, inlineFP64 = false;
     if (MODE == 0) {
       inlineFP64 = live && !special;
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
                    dz = 0.5 * (B.n1.z + B.n2.z) - midA.z;
       const bool far = (dx * dx + dy * dy + dz * dz > farDist2) && (omcSq > sinMin2);
       queued = live && !far && !special;
-      exact += __popcll(msp);
+      exact += __popcll(__ballot(special));
       if (live && far) {
         v3<double> F1, F2, F3, F4;
         far_side<float>(KF, A, B, F1, F2);
@@ -269,6 +274,19 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
   }
   if (MODE == 1 && nq > 0) flush(nq);
   if (MODE == 1 && lane == 0 && exact) atomicAdd(nExact, (unsigned long long)exact);
+  // this tile's near-parallel pairs, evaluated beforehand, in list order
+  for (int e = specOff[t], e1 = specOff[t + 1]; e < e1; ++e) {
+    const unsigned q = specSlot[e];
+    const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
+    if (rl == lane) {
+      a1 = a1 + V(specRes[0 * specTotal + e], specRes[1 * specTotal + e], specRes[2 * specTotal + e]);
+      a2 = a2 + V(specRes[3 * specTotal + e], specRes[4 * specTotal + e], specRes[5 * specTotal + e]);
+    }
+    if (sl == lane) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) cacc[c][lane] += specRes[(6 + c) * specTotal + e];
+    }
+  }
 
   double* rp = rowPart + t * 6 * kTile;
   double* cp = colPart + t * 6 * kTile;
@@ -301,38 +319,83 @@ __global__ __launch_bounds__(kTile) void k_special_eval(const double* __restrict
   for (int c = 0; c < 12; ++c) res[c * total + e] = v[c];
 }
 
-// Per tile, its near-parallel pairs' forces added to the row and column
-// partial sums in list order (one wavefront per tile).
-__global__ __launch_bounds__(kTile) void k_special_accum(const int* __restrict__ specOff,
-                                                         const unsigned short* __restrict__ specSlot,
-                                                         const double* __restrict__ res,
-                                                         int64_t total, int64_t tile0,
-                                                         double* __restrict__ rowPart,
-                                                         double* __restrict__ colPart) {
-  const int64_t t = tile0 + blockIdx.x;
-  const int e0 = specOff[t], e1 = specOff[t + 1];
-  if (e0 == e1) return;
-  const int lane = threadIdx.x;
-  double r[6] = {0, 0, 0, 0, 0, 0}, c[6] = {0, 0, 0, 0, 0, 0};
-  for (int e = e0; e < e1; ++e) {
-    const unsigned q = specSlot[e];
-    const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
-    if (rl == lane) {
-#pragma unroll
-      for (int k = 0; k < 6; ++k) r[k] += res[k * total + e];
+// Exclusive scan of n counts in three passes: block sums (2,048 per
+// workgroup), a one-workgroup scan of the block sums, and the blocks' own
+// scans offset by them.  off[n] receives the total.
+constexpr int kScanBlock = 2048;
+
+__global__ __launch_bounds__(256) void k_scan_sums(const unsigned* __restrict__ cnt, int n,
+                                                   int* __restrict__ sums) {
+  __shared__ int part[256];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int64_t i0 = (int64_t)b * kScanBlock + 8 * tid;
+  int a = 0;
+  for (int q = 0; q < 8; ++q)
+    if (i0 + q < n) a += (int)cnt[i0 + q];
+  part[tid] = a;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (tid < w) part[tid] += part[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) sums[b] = part[0];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(const int* __restrict__ sums, int nb,
+                                                   int* __restrict__ boff) {
+  __shared__ int part[1024];
+  const int tid = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int b0 = tid * per, b1 = min(nb, b0 + per);
+  int a = 0;
+  for (int b = b0; b < b1; ++b) a += sums[b];
+  part[tid] = a;
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int k = 0; k < 1024; ++k) {
+      const int v = part[k];
+      part[k] = acc;
+      acc += v;
     }
-    if (sl == lane) {
-#pragma unroll
-      for (int k = 0; k < 6; ++k) c[k] += res[(6 + k) * total + e];
+    boff[nb] = acc;
+  }
+  __syncthreads();
+  int acc = part[tid];
+  for (int b = b0; b < b1; ++b) {
+    boff[b] = acc;
+    acc += sums[b];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_final(const unsigned* __restrict__ cnt, int n,
+                                                    const int* __restrict__ boff, int nb,
+                                                    int* __restrict__ off) {
+  __shared__ int part[256];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int64_t i0 = (int64_t)b * kScanBlock + 8 * tid;
+  int v[8], a = 0;
+  for (int q = 0; q < 8; ++q) {
+    v[q] = (i0 + q < n) ? (int)cnt[i0 + q] : 0;
+    a += v[q];
+  }
+  part[tid] = a;
+  __syncthreads();
+  if (tid == 0) {
+    int acc = boff[b];
+    for (int k = 0; k < 256; ++k) {
+      const int x = part[k];
+      part[k] = acc;
+      acc += x;
     }
   }
-  double* rp = rowPart + t * 6 * kTile + lane;
-  double* cp = colPart + t * 6 * kTile + lane;
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    rp[k * kTile] += r[k];
-    cp[k * kTile] += c[k];
+  __syncthreads();
+  int acc = part[tid];
+  for (int q = 0; q < 8; ++q) {
+    if (i0 + q < n) off[i0 + q] = acc;
+    acc += v[q];
   }
+  if (b == 0 && tid == 0) off[n] = boff[nb];
 }
 
 // Sum of a segment's partials: row role in tiles (B, J), J >= B, then column
@@ -407,22 +470,50 @@ void launch_prep_segments(double* seg, int64_t ld, int64_t n, hipStream_t st) {
                      ld, n);
 }
 
-void launch_special_count(const double* seg, int64_t ld, int n, const MixedRule* mixed,
-                          unsigned* cnt, hipStream_t st) {
+template <int LIST>
+static void special_pass(const double* seg, int64_t ld, int n, const MixedRule* mixed, unsigned* cnt,
+                         const SpecialList* sp, hipStream_t st) {
   const int64_t nb = (n + kTile - 1) / kTile;
   const int64_t T = tiles_for(nb);
   const int64_t maxGrid = 1ll << 30;
   const double fd2 = mixed ? mixed->farDist * mixed->farDist : 0.0;
   const double sm2 = mixed ? mixed->sinMin * mixed->sinMin : 0.0;
+  const int* off = sp ? sp->off : nullptr;
+  unsigned short* slot = sp ? sp->slot : nullptr;
+  int* tileOf = sp ? sp->tile : nullptr;
   for (int64_t t0 = 0; t0 < T; t0 += maxGrid) {
     const int64_t count = (T - t0 < maxGrid) ? (T - t0) : maxGrid;
     if (mixed)
-      hipLaunchKernelGGL(k_special_count<1>, dim3((unsigned)count), dim3(kTile), 0, st, seg, ld, n, t0,
-                         fd2, sm2, cnt);
+      hipLaunchKernelGGL((k_special_pass<1, LIST>), dim3((unsigned)count), dim3(kTile), 0, st, seg, ld,
+                         n, t0, fd2, sm2, cnt, off, slot, tileOf);
     else
-      hipLaunchKernelGGL(k_special_count<0>, dim3((unsigned)count), dim3(kTile), 0, st, seg, ld, n, t0,
-                         fd2, sm2, cnt);
+      hipLaunchKernelGGL((k_special_pass<0, LIST>), dim3((unsigned)count), dim3(kTile), 0, st, seg, ld,
+                         n, t0, fd2, sm2, cnt, off, slot, tileOf);
   }
+}
+
+void launch_special_count(const double* seg, int64_t ld, int n, const MixedRule* mixed,
+                          unsigned* cnt, hipStream_t st) {
+  special_pass<0>(seg, ld, n, mixed, cnt, nullptr, st);
+}
+
+int64_t scan_work_ints(int n) { return 2 * ((int64_t)(n + kScanBlock - 1) / kScanBlock + 1); }
+
+void launch_exclusive_scan(const unsigned* cnt, int n, int* off, int* work, hipStream_t st) {
+  const int nb = (n + kScanBlock - 1) / kScanBlock;
+  int* sums = work;
+  int* boff = work + nb + 1;
+  if (nb > 0) hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(256), 0, st, cnt, n, sums);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, sums, nb, boff);
+  hipLaunchKernelGGL(k_scan_final, dim3(nb > 0 ? nb : 1), dim3(256), 0, st, cnt, n, boff, nb, off);
+}
+
+void launch_special_list(const double* seg, int64_t ld, int n, const MixedRule* mixed,
+                         const SpecialList& sp, const SegK<double>& K, hipStream_t st) {
+  if (sp.total <= 0) return;
+  special_pass<1>(seg, ld, n, mixed, nullptr, &sp, st);
+  hipLaunchKernelGGL(k_special_eval, dim3((unsigned)((sp.total + kTile - 1) / kTile)), dim3(kTile), 0,
+                     st, seg, ld, sp.slot, sp.tile, sp.total, K, sp.res, sp.flags);
 }
 
 void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
@@ -438,19 +529,10 @@ void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double
     const int64_t cnt = (T - t0 < maxGrid) ? (T - t0) : maxGrid;
     if (mixed)
       hipLaunchKernelGGL(k_segseg_tiles<1>, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, t0,
-                         K, KF, fd2, sm2, rowPart, colPart, sp.off, sp.slot, sp.tile, nExact);
+                         K, KF, fd2, sm2, rowPart, colPart, sp.off, sp.slot, sp.res, sp.total, nExact);
     else
       hipLaunchKernelGGL(k_segseg_tiles<0>, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, t0,
-                         K, KF, fd2, sm2, rowPart, colPart, sp.off, sp.slot, sp.tile, nExact);
-  }
-  if (sp.total > 0) {
-    hipLaunchKernelGGL(k_special_eval, dim3((unsigned)((sp.total + kTile - 1) / kTile)), dim3(kTile), 0,
-                       st, seg, ld, sp.slot, sp.tile, sp.total, K, sp.res, sp.flags);
-    for (int64_t t0 = 0; t0 < T; t0 += maxGrid) {
-      const int64_t cnt = (T - t0 < maxGrid) ? (T - t0) : maxGrid;
-      hipLaunchKernelGGL(k_special_accum, dim3((unsigned)cnt), dim3(kTile), 0, st, sp.off, sp.slot,
-                         sp.res, sp.total, t0, rowPart, colPart);
-    }
+                         K, KF, fd2, sm2, rowPart, colPart, sp.off, sp.slot, sp.res, sp.total, nExact);
   }
 }
 
