@@ -25,9 +25,10 @@ for p in (ROOT / "ddd.jl_amd", ROOT / "oracle", ROOT / "tests"):
 METRIC = "Force-eval time @131k segs: plain FP64 vs hybrid at ≤1e-3 RMS rel force err"
 FLOPS_PER_PAIR = 1466  # counted by the operation-counting oracle (generic pair)
 FP64_PEAK_TFLOPS = 78.6
-# 26.4 GB read + 42.7 GB written per launch (tile partials, plus call-frame
-# spill traffic of the near-parallel branch), 4,096 loops x 32 segments.
-TILE_TRAFFIC_BYTES = 26.4e9 + 42.7e9
+# 26.6 GB read + 14.9 GB written per launch of the pair-tile kernel (tile
+# partials, near-parallel forces, spill traffic of four registers),
+# 4,096 loops x 32 segments.
+TILE_TRAFFIC_BYTES = 26.6e9 + 14.9e9
 
 
 def log(msg):
