@@ -52,3 +52,24 @@ def test_mixed_all_fp64_when_far_threshold_huge(prm):
     g, ex = G.calcSegSegForceMixed(dln, mat, net, farDist=1e9)
     assert ex == n * (n - 1) // 2
     assert per_node_rel(g, O.calcSegSegForce(dln, mat, net))[0] <= 1e-10
+
+
+def test_mixed_pileups(prm):
+    """Mixed precision on pile-ups (many near-parallel pairs, listed per tile)
+    against the FP64 oracle and the oracle's mixed restatement."""
+    from dddhip.generate import clustered_loop_network
+    dln, mat = prm
+    net = clustered_loop_network(numLoops=96, seed=0xE8, clusters=2, cluster_frac=0.9,
+                                 sigma=120.0)
+    g, ex = G.calcSegSegForceMixed(dln, mat, net)
+    om, oex = O.calcSegSegForceMixed(dln, mat, net, prec=2, nq=2)
+    ref = O.calcSegSegForcePar(dln, mat, net, 16)
+    s1, s2 = O.calcSelfForce(dln, mat, net)
+    for a in (g, om, ref):
+        a[:, 0, :] += s1
+        a[:, 1, :] += s2
+    mx, rms = per_node_rel(g, ref)
+    print(f"pile-ups vs FP64 oracle: max {mx:.2e} rms {rms:.2e}", flush=True)
+    assert mx <= 2e-5 and rms <= 2e-6
+    assert per_node_rel(g, om)[0] <= 1e-5
+    assert abs(ex - oex) <= max(2, 1e-4 * oex)

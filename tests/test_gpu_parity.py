@@ -249,3 +249,30 @@ def test_config_b(prm):
     # Net force of pair interactions of closed loops vanishes.
     tot = g.reshape(3, -1, order="F").sum(axis=1)
     assert np.all(np.abs(tot) <= 1e-12 * np.abs(g).sum())
+
+
+def test_pileup_near_parallel_pairs(prm):
+    """A network of pile-ups (loops of one slip system, so their sides are
+    parallel): near-parallel pairs are a large share (listed per tile and
+    evaluated by their own kernel), against the oracle's triangle at 1e-10
+    per node, and bitwise repeatable."""
+    from dddhip.generate import clustered_loop_network
+    dln, mat = prm
+    net = clustered_loop_network(numLoops=96, seed=0xE7, clusters=2, cluster_frac=0.9,
+                                 sigma=120.0)
+    n = int(net.numSeg[0])
+    _log(f"pile-up network: {n} segments")
+    g = G.calcSegSegForce(dln, mat, net)
+    o = O.calcSegSegForcePar(dln, mat, net, 16)
+    mx, rms = per_node_rel(g, o)
+    _log(f"pile-up per-node rel err max {mx:.3e} rms {rms:.3e}")
+    assert mx <= TOL
+    assert np.array_equal(g, G.calcSegSegForce(dln, mat, net))
+    # share of near-parallel pairs (1 - c^2 <= sqrt(eps)) in this network
+    si = net.segIdx[:n]
+    d = net.coord[:, si[:, 2] - 1] - net.coord[:, si[:, 1] - 1]
+    t = d / np.linalg.norm(d, axis=0)
+    c = t.T @ t
+    par = np.triu(1 - c * c <= np.sqrt(np.finfo(float).eps), 1).sum()
+    _log(f"near-parallel pairs: {par} of {n * (n - 1) // 2}")
+    assert par > 0.02 * n * (n - 1) // 2
