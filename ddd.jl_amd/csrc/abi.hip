@@ -1239,7 +1239,6 @@ int ddd_farfield_train(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_const
     if (rn + 1 > G - 1) bad("no far offsets on this grid");
     m->aH = (float)(k->coreRad / ff->g.h);
     const int nc = G * G * G;
-    const int64_t n = net->numSeg;
     // Training network: its own segment buffer (an uploaded network stays);
     // cells and moments in the farfield's buffers, which the next hybrid
     // evaluation recomputes.

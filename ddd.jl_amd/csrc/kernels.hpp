@@ -58,6 +58,12 @@ void launch_exclusive_scan(const unsigned* cnt, int n, int* off, int* work, hipS
 void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
                          double* rowPart, double* colPart, const SpecialList& sp, hipStream_t st,
                          const MixedRule* mixed = nullptr, unsigned long long* nExact = nullptr);
+// The mixed-precision kernels' launches (mixed.hip; used by the two above).
+void launch_special_pass_mixed(const double* seg, int64_t ld, int n, const MixedRule& mr,
+                               unsigned* cnt, const SpecialList* sp, hipStream_t st);
+void launch_segseg_tiles_mixed(const double* seg, int64_t ld, int n, const SegK<double>& K,
+                               double* rowPart, double* colPart, const SpecialList& sp,
+                               hipStream_t st, const MixedRule& mr, unsigned long long* nExact);
 void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, double* out,
                           hipStream_t st);
 void launch_segseg_rows(const double* seg, int64_t ld, int n, const int64_t* rows, int nRows,

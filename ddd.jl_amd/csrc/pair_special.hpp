@@ -22,7 +22,7 @@ static __device__ __forceinline__ Seg<double> seg_from(const double* __restrict_
 // outputs are kept (:1285-1317, :1401-1433).  The tilt construction makes the
 // piece sit at the critical angle, sin^2 = 1.01 sqrt(eps), so it always takes
 // the generic branch; the flag reports it if rounding ever says otherwise.
-static __device__ __noinline__ void corrective_pair(const SegK<double>& K, v3<double> bp, v3<double> p,
+[[maybe_unused]] static __device__ __noinline__ void corrective_pair(const SegK<double>& K, v3<double> bp, v3<double> p,
                                              v3<double> q, const Seg<double>& other,
                                              v3<double>& Fo1, v3<double>& Fo2,
                                              unsigned* flags) {
@@ -41,7 +41,7 @@ static __device__ __noinline__ void corrective_pair(const SegK<double>& K, v3<do
 }
 
 // calcParSegSegForce (:1167-1442) on device.
-static __device__ __noinline__ void special_pair_dev(const SegK<double>& K, const Seg<double>& s1in,
+[[maybe_unused]] static __device__ __noinline__ void special_pair_dev(const SegK<double>& K, const Seg<double>& s1in,
                                           const Seg<double>& s2, v3<double>& F1, v3<double>& F2,
                                           v3<double>& F3, v3<double>& F4, unsigned* flags) {
   const double c = dot(s2.t, s1in.t);

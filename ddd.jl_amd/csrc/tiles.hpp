@@ -1,0 +1,280 @@
+// The all-pairs tile kernels and their near-parallel passes, as templates on
+// the mode (0: plain FP64, 1: mixed precision).  segseg.hip instantiates the
+// plain kernels with the FP64 path's strict rounding; mixed.hip the mixed
+// ones, compiled with contraction and fast FP32 division (an approximation
+// held to the FP32 quadrature's own accuracy).  Everything sits in an
+// unnamed namespace, so each translation unit has its own copy.
+#pragma once
+#include "kernels.hpp"
+#include "pair_special.hpp"
+
+namespace ddd {
+
+namespace {
+
+__device__ __forceinline__ Seg<double> seg_from_lds(const double (*s)[kTile], int l) {
+  Seg<double> r;
+  r.n1 = V(s[F_N1X][l], s[F_N1Y][l], s[F_N1Z][l]);
+  r.n2 = V(s[F_N2X][l], s[F_N2Y][l], s[F_N2Z][l]);
+  r.b = V(s[F_BX][l], s[F_BY][l], s[F_BZ][l]);
+  r.t = V(s[F_TX][l], s[F_TY][l], s[F_TZ][l]);
+  r.tN = s[F_TN][l];
+  return r;
+}
+
+constexpr int kQueue = 2 * kTile;
+
+// Tile t of the upper triangle, enumerated column-block major:
+// t = J (J + 1) / 2 + I with 0 <= I <= J.
+__device__ __forceinline__ void tile_blocks(int64_t t, int& I, int& J) {
+  J = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((int64_t)J * (J + 1) / 2 > t) --J;
+  while ((int64_t)(J + 1) * (J + 2) / 2 <= t) ++J;
+  I = (int)(t - (int64_t)J * (J + 1) / 2);
+}
+
+// Midpoint of a segment and the mixed rule's far test in explicitly rounded
+// arithmetic, so that every kernel classifies a pair identically whether or
+// not its translation unit contracts floating-point expressions.
+__device__ __forceinline__ v3<double> mid_rn(v3<double> a, v3<double> b) {
+  return V(__dmul_rn(0.5, __dadd_rn(a.x, b.x)), __dmul_rn(0.5, __dadd_rn(a.y, b.y)),
+           __dmul_rn(0.5, __dadd_rn(a.z, b.z)));
+}
+__device__ __forceinline__ bool is_far(double omcSq, v3<double> midA, v3<double> n1B,
+                                       v3<double> n2B, double farDist2, double sinMin2) {
+  const v3<double> mB = mid_rn(n1B, n2B);
+  const double dx = __dsub_rn(mB.x, midA.x), dy = __dsub_rn(mB.y, midA.y),
+               dz = __dsub_rn(mB.z, midA.z);
+  const double d2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+  return d2 > farDist2 && omcSq > sinMin2;
+}
+
+// The classification both the count pass and the tile kernel use, so that
+// they agree bit for bit: a live pair is near-parallel (the reference's
+// special branch, 1 - c^2 <= sqrt(eps)) unless, in the mixed mode, it is a
+// well-separated pair for the FP32 quadrature.
+template <int MODE>
+__device__ __forceinline__ bool is_special(bool live, double omcSq, v3<double> midA,
+                                           v3<double> n1B, v3<double> n2B, double farDist2,
+                                           double sinMin2) {
+  if (!live || omcSq > DDD_SQRT_EPS) return false;
+  if (MODE == 0) return true;
+  return !is_far(omcSq, midA, n1B, n2B, farDist2, sinMin2);
+}
+
+// Near-parallel pairs per tile, one wavefront per tile in the tile kernel's
+// slot order (k, lane): LIST = 0 counts them (cnt[t]); LIST = 1 writes them
+// to the tile's range off[t] .. off[t + 1] of the pair list.
+template <int MODE, int LIST>
+__global__ __launch_bounds__(kTile) void k_special_pass(const double* __restrict__ seg, int64_t ld,
+                                                        int n, int64_t tile0, double farDist2,
+                                                        double sinMin2, unsigned* __restrict__ cnt,
+                                                        const int* __restrict__ off,
+                                                        unsigned short* __restrict__ slot,
+                                                        int* __restrict__ tileOf) {
+  const int64_t t = tile0 + blockIdx.x;
+  int I, J;
+  tile_blocks(t, I, J);
+  const int lane = threadIdx.x;
+  __shared__ double sj[10][kTile];  // n1, n2, t, ext
+  const int64_t i = (int64_t)I * kTile + lane;
+  const int64_t jl = (int64_t)J * kTile + lane;
+  const int fl[10] = {F_N1X, F_N1Y, F_N1Z, F_N2X, F_N2Y, F_N2Z, F_TX, F_TY, F_TZ, F_EXT};
+#pragma unroll
+  for (int f = 0; f < 10; ++f) sj[f][lane] = seg[fl[f] * ld + jl];
+  __syncthreads();
+  const v3<double> tA = V(seg[F_TX * ld + i], seg[F_TY * ld + i], seg[F_TZ * ld + i]);
+  const v3<double> midA = mid_rn(V(seg[F_N1X * ld + i], seg[F_N1Y * ld + i], seg[F_N1Z * ld + i]),
+                                 V(seg[F_N2X * ld + i], seg[F_N2Y * ld + i], seg[F_N2Z * ld + i]));
+  const bool rowLive = (i < n) && seg[F_EXT * ld + i] == 0.0;
+  const bool diag = (I == J);
+  const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int c = LIST ? off[t] : 0;
+  const int cEnd = LIST ? off[t + 1] : 0;
+  for (int k = 0; k < kTile; ++k) {
+    const int s = (lane + k) & 63;
+    const int64_t j = (int64_t)J * kTile + s;
+    const bool live = rowLive && (j < n) && sj[9][s] == 0.0 && (!diag || s > lane);
+    double cc;
+    const double omcSq = one_minus_c2(tA, V(sj[6][s], sj[7][s], sj[8][s]), cc);
+    const bool sp = is_special<MODE>(live, omcSq, midA, V(sj[0][s], sj[1][s], sj[2][s]),
+                                     V(sj[3][s], sj[4][s], sj[5][s]), farDist2, sinMin2);
+    const uint64_t m = __ballot(sp);
+    if (LIST && sp) {
+      const int pos = c + __popcll(m & ltMask);
+      if (pos < cEnd) {  // always true: both passes classify identically
+        slot[pos] = (unsigned short)((k << 6) | lane);
+        tileOf[pos] = (int)t;
+      }
+    }
+    c += __popcll(m);
+  }
+  if (!LIST && lane == 0) cnt[t] = (unsigned)c;
+}
+
+// Pair-tile kernel.  MODE 0 (plain FP64): every generic pair runs inline in
+// FP64.  MODE 1 (mixed precision): well-separated pairs (midpoint distance >
+// farDist and |t1 x t2| > sinMin) run inline in FP32 by quadrature
+// (far_side); the other generic pairs are queued and evaluated in FP64 64 at
+// a time, so the wavefront never waits on a minority path.  In both modes
+// near-parallel pairs (the reference's special branch, :1147-1162, several
+// times the cost of a generic pair) are not evaluated here: they were listed
+// per tile and evaluated beforehand (k_special_pass, k_special_eval), and
+// the tile adds its list's forces, in list order, after its own pairs.
+// Keeping that branch out of this kernel lets it run two wavefronts per
+// SIMD.  The number of FP64 pairs is added to *nExact (mixed mode).
+template <int MODE>
+__global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restrict__ seg,
+                                                           int64_t ld, int n, int64_t tile0,
+                                                           SegK<double> K, SigK<float> KF,
+                                                           double farDist2, double sinMin2,
+                                                           double* __restrict__ rowPart,
+                                                           double* __restrict__ colPart,
+                                                           const int* __restrict__ specOff,
+                                                           const unsigned short* __restrict__ specSlot,
+                                                           const double* __restrict__ specRes,
+                                                           int64_t specTotal,
+                                                           unsigned long long* __restrict__ nExact) {
+  const int64_t t = tile0 + blockIdx.x;
+  int I, J;
+  tile_blocks(t, I, J);
+  const int lane = threadIdx.x;
+
+  __shared__ double si[F_COUNT][kTile];
+  __shared__ double sj[F_COUNT][kTile];
+  __shared__ double cacc[6][kTile];
+  __shared__ double sres[12][kTile];
+  __shared__ unsigned short queue[kQueue];
+
+  const int64_t i = (int64_t)I * kTile + lane;
+  const int64_t jl = (int64_t)J * kTile + lane;
+#pragma unroll
+  for (int f = 0; f < F_COUNT; ++f) {
+    si[f][lane] = seg[f * ld + i];
+    sj[f][lane] = seg[f * ld + jl];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) cacc[c][lane] = 0.0;
+  __syncthreads();
+
+  const Seg<double> A = seg_from_lds(si, lane);
+  const bool rowLive = (i < n) && si[F_EXT][lane] == 0.0;
+  const bool diag = (I == J);
+  v3<double> a1 = V(0.0, 0.0, 0.0), a2 = V(0.0, 0.0, 0.0);
+  int nq = 0;        // wave-uniform queue length (mixed mode)
+
+  unsigned exact = 0;
+  const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const v3<double> midA = mid_rn(A.n1, A.n2);
+
+  // Mixed mode: evaluate the first cnt queued generic pairs in FP64 and add
+  // them to rows and columns in queue order.
+  auto flush = [&](int cnt) {
+    if (lane < cnt) {
+      const unsigned e = queue[lane];
+      const int rl = e & 63, sl = (rl + (e >> 6)) & 63;
+      const Seg<double> P = seg_from_lds(si, rl), Q = seg_from_lds(sj, sl);
+      v3<double> F1, F2, F3, F4;
+      double c;
+      const double omc = one_minus_c2(P.t, Q.t, c);
+      pair_generic<double, true, true>(K, P, Q, c, omc, F1, F2, F3, F4);
+      sres[0][lane] = F1.x; sres[1][lane] = F1.y; sres[2][lane] = F1.z;
+      sres[3][lane] = F2.x; sres[4][lane] = F2.y; sres[5][lane] = F2.z;
+      sres[6][lane] = F3.x; sres[7][lane] = F3.y; sres[8][lane] = F3.z;
+      sres[9][lane] = F4.x; sres[10][lane] = F4.y; sres[11][lane] = F4.z;
+    }
+    __syncthreads();
+    for (int e = 0; e < cnt; ++e) {  // queue order: deterministic
+      const unsigned q = queue[e];
+      const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
+      if (rl == lane) {
+        a1 = a1 + V(sres[0][e], sres[1][e], sres[2][e]);
+        a2 = a2 + V(sres[3][e], sres[4][e], sres[5][e]);
+      }
+      if (sl == lane) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) cacc[c][lane] += sres[6 + c][e];
+      }
+    }
+    __syncthreads();
+  };
+
+  for (int k = 0; k < kTile; ++k) {
+    const int s = (lane + k) & 63;
+    const int64_t j = (int64_t)J * kTile + s;
+    const bool live = rowLive && (j < n) && sj[F_EXT][s] == 0.0 && (!diag || s > lane);
+    const Seg<double> B = seg_from_lds(sj, s);
+    double c;
+    const double omcSq = one_minus_c2(A.t, B.t, c);
+    const bool special = is_special<MODE>(live, omcSq, midA, B.n1, B.n2, farDist2, sinMin2);
+    bool queued = false, inlineFP64 = false;
+    if (MODE == 0) {
+      inlineFP64 = live && !special;
+    } else {
+      const bool far = is_far(omcSq, midA, B.n1, B.n2, farDist2, sinMin2);
+      queued = live && !far && !special;
+      exact += __popcll(__ballot(special));
+      if (live && far) {
+        v3<double> F1, F2, F3, F4;
+        far_side<float>(KF, A, B, F1, F2);
+        far_side<float>(KF, B, A, F3, F4);
+        a1 = a1 + F1;
+        a2 = a2 + F2;
+        cacc[0][s] += F3.x; cacc[1][s] += F3.y; cacc[2][s] += F3.z;
+        cacc[3][s] += F4.x; cacc[4][s] += F4.y; cacc[5][s] += F4.z;
+      }
+    }
+    if (MODE == 1) {
+      const uint64_t m = __ballot(queued);
+      if (m) {
+        if (queued) queue[nq + __popcll(m & ltMask)] = (unsigned short)((k << 6) | lane);
+        nq += __popcll(m);
+        exact += __popcll(m);
+      }
+    }
+    if (MODE == 0 && inlineFP64) {
+      v3<double> F1, F2, F3, F4;
+      pair_generic<double, true, true>(K, A, B, c, omcSq, F1, F2, F3, F4);
+      a1 = a1 + F1;
+      a2 = a2 + F2;
+      cacc[0][s] += F3.x; cacc[1][s] += F3.y; cacc[2][s] += F3.z;
+      cacc[3][s] += F4.x; cacc[4][s] += F4.y; cacc[5][s] += F4.z;
+    }
+    __syncthreads();
+    if (MODE == 1 && nq >= kTile) {
+      flush(kTile);
+      const int rest = nq - kTile;
+      unsigned short keep = (lane < rest) ? queue[kTile + lane] : 0;
+      __syncthreads();
+      if (lane < rest) queue[lane] = keep;
+      __syncthreads();
+      nq = rest;
+    }
+  }
+  if (MODE == 1 && nq > 0) flush(nq);
+  if (MODE == 1 && lane == 0 && exact) atomicAdd(nExact, (unsigned long long)exact);
+  // this tile's near-parallel pairs, evaluated beforehand, in list order
+  for (int e = specOff[t], e1 = specOff[t + 1]; e < e1; ++e) {
+    const unsigned q = specSlot[e];
+    const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
+    if (rl == lane) {
+      a1 = a1 + V(specRes[0 * specTotal + e], specRes[1 * specTotal + e], specRes[2 * specTotal + e]);
+      a2 = a2 + V(specRes[3 * specTotal + e], specRes[4 * specTotal + e], specRes[5 * specTotal + e]);
+    }
+    if (sl == lane) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) cacc[c][lane] += specRes[(6 + c) * specTotal + e];
+    }
+  }
+
+  double* rp = rowPart + t * 6 * kTile;
+  double* cp = colPart + t * 6 * kTile;
+  rp[0 * kTile + lane] = a1.x; rp[1 * kTile + lane] = a1.y; rp[2 * kTile + lane] = a1.z;
+  rp[3 * kTile + lane] = a2.x; rp[4 * kTile + lane] = a2.y; rp[5 * kTile + lane] = a2.z;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) cp[c * kTile + lane] = cacc[c][lane];
+}
+
+}  // namespace
+
+}  // namespace ddd
