@@ -175,4 +175,119 @@ calc_σTilde(x0, dlnParams, matParams, network, idx = nothing) =
     calc_σTilde!(zeros(6, x0 isa Base.Iterators.ProductIterator ? prod(size(x0)) :
                          div(length(x0), 3)), x0, dlnParams, matParams, network, idx)
 
+# ---- precision modes, device-resident evaluation, learned far field ------
+# (no counterpart in the reference; include/ddd_hip.h documents each call)
+
+const DDD_MODE_FP64 = Cint(0)
+const DDD_MODE_MIXED = Cint(1)
+const DDD_MODE_HYBRID = Cint(2)
+
+struct Opts
+    mode::Cint
+    farDist::Float64     # 0: 5 core radii
+    sinMin::Float64      # 0: 1e-3
+    farfield::Ptr{Cvoid}
+end
+
+struct Stats
+    pairs::UInt64
+    pairsFP64::UInt64
+end
+
+struct TrainOpts
+    samples::Int64
+    valSamples::Int64
+    steps::Cint
+    batch::Cint
+    lr::Float32
+    lrFinal::Float32
+    seed::UInt64
+    denFloor::Float32
+    nearFrac::Float32
+    nearShells::Cint
+end
+
+struct TrainReport
+    lossFirst::Float64
+    lossLast::Float64
+    relErrVal::Float64
+    relErrValNear::Float64
+    seconds::Float64
+    relErrShell::NTuple{8,Float64}
+end
+
+"""Whole-network seg-seg forces in a precision mode (3 x 2 x numSeg) and the
+pair statistics (pairs evaluated exactly, of which in FP64)."""
+function calcSegSegForceMode(dlnParams, matParams, network::DislocationNetwork, opts::Opts)
+    k = Ref(consts(dlnParams, matParams))
+    m = network.numSeg[1]
+    out = zeros(3, 2, m)
+    st = Ref(Stats(0, 0))
+    GC.@preserve network out begin
+        nw = Ref(netstruct(network))
+        check(ccall((:ddd_segseg_force_opts, libddd), Cint,
+            (Ptr{Cvoid}, Ref{Consts}, Ref{Network}, Ref{Opts}, Ptr{Float64}, Int64, Cint, Ref{Stats}),
+            ctx(), k, nw, Ref(opts), pointer(out), m, Cint(0), st))
+    end
+    return out, st[]
+end
+
+"""A far field on a G^3 grid of cells of edge h from corner lo, pairs of
+cells within Chebyshev distance nearRadius exact."""
+function farfield(G::Integer, lo::AbstractVector{Float64}, h::Float64, nearRadius::Integer)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    l = Vector{Float64}(lo)
+    GC.@preserve l check(ccall((:ddd_farfield_create, libddd), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Float64}, Float64, Cint, Ref{Ptr{Cvoid}}),
+        ctx(), Cint(G), pointer(l), h, Cint(nearRadius), out))
+    return out[]
+end
+
+function farfield_mlp(seed::Integer = 11)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:ddd_mlp_create, libddd), Cint, (Ptr{Cvoid}, UInt64, Ref{Ptr{Cvoid}}),
+                ctx(), UInt64(seed), out))
+    return out[]
+end
+
+"""Train the far-field network on exact local expansions of `trainNetwork`
+and install its operator into `ff`."""
+function train_farfield!(ff, mlp, dlnParams, matParams, trainNetwork::DislocationNetwork;
+                         samples = 1 << 24, valSamples = 1 << 16, steps = 60000, batch = 4096,
+                         lr = 2f-3, lrFinal = 5f-6, seed = 7, denFloor = 0.1f0)
+    k = Ref(consts(dlnParams, matParams))
+    o = Ref(TrainOpts(samples, valSamples, steps, batch, lr, lrFinal, seed, denFloor, 0f0, 0))
+    rep = Ref{TrainReport}()
+    GC.@preserve trainNetwork begin
+        nw = Ref(netstruct(trainNetwork))
+        check(ccall((:ddd_farfield_train, libddd), Cint,
+            (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{Consts}, Ref{Network}, Ref{TrainOpts},
+             Ref{TrainReport}), ctx(), ff, mlp, k, nw, o, rep))
+    end
+    return rep[]
+end
+
+"""Keep `network` on the GPU for repeated evaluations (time stepping):
+upload once, set a mode, then `resident_seg_force!` fills network.segForce."""
+function resident_upload(dlnParams, matParams, network::DislocationNetwork)
+    k = Ref(consts(dlnParams, matParams))
+    GC.@preserve network begin
+        nw = Ref(netstruct(network))
+        check(ccall((:ddd_net_upload, libddd), Cint, (Ptr{Cvoid}, Ref{Consts}, Ref{Network}, Ptr{Cvoid}),
+                    ctx(), k, nw, C_NULL))
+    end
+    return nothing
+end
+
+resident_mode!(opts::Opts) =
+    check(ccall((:ddd_net_set_opts, libddd), Cint, (Ptr{Cvoid}, Ref{Opts}), ctx(), Ref(opts)))
+
+function resident_seg_force!(network::DislocationNetwork)
+    check(ccall((:ddd_net_seg_force, libddd), Cint, (Ptr{Cvoid}, Cint), ctx(), Cint(0)))
+    GC.@preserve network check(ccall((:ddd_net_download, libddd), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Int64), ctx(), pointer(network.segForce),
+        size(network.segForce, 3)))
+    return nothing
+end
+
 end # module
