@@ -113,6 +113,49 @@ def hybrid_paths(args, dln, mat, net, res, stream, ref, fp64_ms, log):
     return out
 
 
+def mixed_leg(args, dln, mat, ctx, log):
+    """Mixed precision on its own network (2,048 loops x 32 = 65,536 segments,
+    seed 0xC0): plain FP64 and mixed whole evaluations, resident; per-node
+    error of the mixed result against the FP64 one (segment-end and node
+    forces) and the share of pairs on the FP64 path."""
+    import numpy as np
+    import torch
+    from dddhip import forces as G
+    from dddhip.generate import random_loop_network
+    from metrics import per_node_rel, normwise
+    net = random_loop_network(args.mixed_loops, seed=0xC0)
+    n = int(net.numSeg[0])
+    res = G.ResidentNetwork(dln, mat, net, ctx=ctx)
+    stream = torch.cuda.ExternalStream(res.stream(), device=torch.device("cuda:0"))
+
+    def timed():
+        res.seg_force()
+        stream.synchronize()
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+        for e in ev:
+            e[0].record(stream)
+            res.seg_force()
+            e[1].record(stream)
+            e[1].synchronize()
+        return float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+
+    fms = timed()
+    ref = res.download()
+    res.set_mode("mixed")
+    mms = timed()
+    got = res.download()
+    tot, fp64 = res.stats()
+    res.set_mode("fp64")
+    smx, srms = per_node_rel(got, ref)
+    nmx, nrms = per_node_rel(node_forces(net, got), node_forces(net, ref))
+    log(f"mixed at {n} segments: {mms:.1f} ms vs FP64 {fms:.1f} ms, per-node err max {smx:.2e} "
+        f"rms {srms:.2e}, FP64 share {fp64 / max(tot, 1):.2%}")
+    return {"segments": n, "fp64_ms": round(fms, 3), "mixed_ms": round(mms, 3),
+            "speedup_vs_fp64": round(fms / mms, 3), "segend_rel_err_max": smx,
+            "segend_rel_err_rms": srms, "node_rel_err_max": nmx, "node_rel_err_rms": nrms,
+            "normwise_rel_err": normwise(got, ref), "fp64_pair_share": round(fp64 / max(tot, 1), 5)}
+
+
 def sigma_leg(args, dln, mat, ctx, log):
     """calc_σTilde of a 512-loop network (16,384 segments) at the 6 x 128^2
     face points of the 2000^3 cube: wall time of the synchronous ABI call
@@ -214,6 +257,9 @@ def main():
     ap.add_argument("--cpu-row-stride", type=int, default=64)
     ap.add_argument("--paths", default="fp64,mixed",
                     help="comma list of paths to time after plain FP64: mixed")
+    ap.add_argument("--mixed-loops", type=int, default=2048,
+                    help="mixed-precision leg on its own network of this many loops "
+                         "(seed 0xC0, 65,536 segments by default); 0 to skip")
     ap.add_argument("--hybrid", default="32:4,32:5,32:6",
                     help="comma list of G:rn hybrid configurations (G^3 cells over the "
                          "network's bounding cube, near stencil of Chebyshev radius rn); "
@@ -330,6 +376,7 @@ def main():
         res.set_mode("fp64")
 
     hybrid = hybrid_paths(args, dln, mat, net, res, stream, out, ms, log) if args.hybrid else None
+    mixed_c = mixed_leg(args, dln, mat, ctx, log) if args.mixed_loops > 0 else None
     sigma = sigma_leg(args, dln, mat, ctx, log) if args.sigma_loops > 0 else None
     clustered = clustered_leg(args, dln, mat, ctx, log) if args.clustered else None
 
@@ -360,6 +407,7 @@ def main():
         },
         "paths": extra,
         "hybrid": hybrid,
+        "mixed_65536": mixed_c,
         "sigma_tilde": sigma,
         "clustered": clustered,
         "roofline": {

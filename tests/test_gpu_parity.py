@@ -276,3 +276,25 @@ def test_pileup_near_parallel_pairs(prm):
     par = np.triu(1 - c * c <= np.sqrt(np.finfo(float).eps), 1).sum()
     _log(f"near-parallel pairs: {par} of {n * (n - 1) // 2}")
     assert par > 0.02 * n * (n - 1) // 2
+
+
+def test_sigma_tilde_config_b(prm):
+    """calc_σTilde of the 16,384-segment network at the 6 x 128^2 face points:
+    every 48th point against the oracle (1e-10 per stress component of the
+    largest), all points finite, and the full call consistent with a call on
+    a point subset (points are independent)."""
+    dln, mat = prm
+    net = random_loop_network(512, seed=0xB0)
+    pts = face_grid_points(128)
+    _log("config B sigma tilde")
+    g = G.calc_σTilde(pts, dln, mat, net)
+    assert np.isfinite(g).all()
+    sel = np.arange(0, pts.shape[1], 48)
+    o = O.calc_σTilde(np.asfortranarray(pts[:, sel]), dln, mat, net)
+    err = np.abs(g[:, sel] - o).max() / np.abs(o).max()
+    _log(f"config B sigma tilde: {len(sel)} points, max rel err {err:.2e}")
+    assert err <= TOL
+    # a point subset is split into a different number of segment chunks, so
+    # its sums associate differently: equal to rounding, not bit for bit
+    part = G.calc_σTilde(np.asfortranarray(pts[:, 1000:3000]), dln, mat, net)
+    assert np.abs(part - g[:, 1000:3000]).max() <= 1e-12 * np.abs(g).max()
