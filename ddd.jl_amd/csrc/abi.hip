@@ -132,8 +132,8 @@ struct ddd_ctx {
   DevBuf meshC, meshCoord, meshConn, uHat;
   // field-point stress
   DevBuf sseg, pts, sigPart, sigma;
-  // gather
-  DevBuf conn, links, coord, nodeF;
+  // gather and mobility
+  DevBuf conn, links, coord, nodeF, bvec, nodeV, nodeIdx;
   std::vector<double> h;  // host staging
   DevBuf counter;  // FP64-pair counter of the mixed path
   // near-parallel pairs of the all-pairs tiles (count, offsets, list, forces)
@@ -752,6 +752,90 @@ int ddd_node_force(ddd_ctx* c, const ddd_consts* k, const ddd_network* net, cons
     HIPCHK(hipMemcpyAsync(nodeForce, c->nodeF.p, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost,
                           c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+int ddd_mobility_bcc(ddd_ctx* c, const ddd_consts* k, const ddd_drag* drag, const ddd_network* net,
+                     const double* segForce, int64_t segForceCols, const int64_t* idx,
+                     int64_t nIdx, double* nodeForce, double* nodeVel, int64_t outCols,
+                     int flags) {
+  return guard(c, "ddd_mobility_bcc", [&] {
+    if (!k || !drag || !net || !segForce || !nodeForce || !nodeVel) bad("NULL argument");
+    if (!net->connectivity || !net->links || !net->coord || !net->bVec)
+      bad("connectivity, links, coord and bVec required");
+    const int64_t nn = net->numNode;
+    if (nn < 0 || nn > net->connCols || nn > net->coordCols) bad("numNode outside arrays");
+    if (net->connLd < 1) bad("connectivity leading dimension");
+    if (idx && nIdx < 0) bad("nIdx < 0");
+    const int64_t n = idx ? nIdx : nn;
+    const bool byPos = (flags & DDD_OUT_BY_POSITION) != 0;
+    const int64_t maxC = (net->connLd - 1) / 2;
+    for (int64_t q = 0; q < n; ++q) {
+      const int64_t node = idx ? idx[q] : q + 1;
+      if (node < 1 || node > nn) bad("idx outside 1..numNode");
+      if (!byPos && node > outCols) bad("idx beyond the output columns");
+      const int64_t* cn = net->connectivity + net->connLd * (node - 1);
+      if (cn[0] < 0 || cn[0] > maxC) bad("connectivity count");
+      for (int64_t e = 1; e <= cn[0]; ++e) {
+        const int64_t lk = cn[2 * e - 1], col = cn[2 * e];
+        if (lk < 1 || lk > net->linksCols || lk > segForceCols || lk > net->bVecCols)
+          bad("connectivity link");
+        if (col != 1 && col != 2) bad("connectivity column");
+        const int64_t other = net->links[2 * (lk - 1) + (2 - col)];
+        if (other < 1 || other > net->coordCols) bad("links node outside coord");
+      }
+    }
+    if (byPos && outCols < n) bad("outCols < number of nodes");
+    if (n == 0) return;
+    c->conn.ensure(sizeof(int64_t) * net->connLd * nn);
+    c->links.ensure(sizeof(int64_t) * 2 * net->linksCols);
+    c->coord.ensure(sizeof(double) * 3 * net->coordCols);
+    c->bvec.ensure(sizeof(double) * 3 * net->bVecCols);
+    c->out.ensure(sizeof(double) * 6 * segForceCols);
+    c->nodeF.ensure(sizeof(double) * 3 * n);
+    c->nodeV.ensure(sizeof(double) * 3 * n);
+    HIPCHK(hipMemcpyAsync(c->conn.p, net->connectivity, sizeof(int64_t) * net->connLd * nn,
+                          hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->links.p, net->links, sizeof(int64_t) * 2 * net->linksCols,
+                          hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->coord.p, net->coord, sizeof(double) * 3 * net->coordCols,
+                          hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->bvec.p, net->bVec, sizeof(double) * 3 * net->bVecCols,
+                          hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->out.p, segForce, sizeof(double) * 6 * segForceCols,
+                          hipMemcpyHostToDevice, c->st));
+    if (idx) {
+      c->nodeIdx.ensure(sizeof(int64_t) * n);
+      HIPCHK(hipMemcpyAsync(c->nodeIdx.p, idx, sizeof(int64_t) * n, hipMemcpyHostToDevice, c->st));
+    }
+    const double dr[4] = {drag->edge, drag->screw, drag->climb, drag->line};
+    launch_mobility_bcc(c->conn.as<int64_t>(), net->connLd, c->links.as<int64_t>(),
+                        c->coord.as<double>(), c->bvec.as<double>(), c->out.as<double>(),
+                        idx ? c->nodeIdx.as<int64_t>() : nullptr, n, dr, k->sigmaPN,
+                        c->nodeF.as<double>(), c->nodeV.as<double>(), c->st);
+    HIPCHK(hipGetLastError());
+    if (byPos || !idx) {
+      HIPCHK(hipMemcpyAsync(nodeForce, c->nodeF.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+                            c->st));
+      HIPCHK(hipMemcpyAsync(nodeVel, c->nodeV.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+                            c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+      return;
+    }
+    // in place with a node list: columns idx[k] only
+    c->h.resize(6 * n);
+    HIPCHK(hipMemcpyAsync(c->h.data(), c->nodeF.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+                          c->st));
+    HIPCHK(hipMemcpyAsync(c->h.data() + 3 * n, c->nodeV.p, sizeof(double) * 3 * n,
+                          hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (int64_t q = 0; q < n; ++q) {
+      const int64_t col = idx[q] - 1;
+      for (int e = 0; e < 3; ++e) {
+        nodeForce[3 * col + e] = c->h[3 * q + e];
+        nodeVel[3 * col + e] = c->h[3 * n + 3 * q + e];
+      }
+    }
   });
 }
 

@@ -266,6 +266,124 @@ __global__ void k_node_gather(const int64_t* __restrict__ conn, int64_t ldConn, 
   nodeForce[3 * nd + 2] = f2;
 }
 
+// BCC mobility (dlnMobility / dlnMobility! for mobBCC,
+// DislocationMobility.jl:16-171,186-341), one thread per node: the node force
+// (the gather above, :56-80), the drag tensor of the node's links (:82-146)
+// and the velocity solving drag v = f (:148) by Cramer's rule (the method
+// of the reference's 3x3 StaticArrays solve).  Node k of the list is
+// idx[k] (1-based) or k + 1 without a list; results go to column k.
+struct DragDev {
+  double edge, screw, climb, line;
+};
+
+__global__ void __launch_bounds__(256)
+k_mobility_bcc(const int64_t* __restrict__ conn, int64_t ldConn, const int64_t* __restrict__ links,
+               const double* __restrict__ coord, const double* __restrict__ bVec,
+               const double* __restrict__ segForce, const int64_t* __restrict__ idx, int64_t n,
+               DragDev drag, double sigmaPN, double* __restrict__ nodeForce,
+               double* __restrict__ nodeVel) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double eps = 2.220446049250313e-16;
+  const double sqrtEps = 1.4901161193847656e-08;  // sqrt(eps(Float64)), exact
+  const int64_t nd = idx ? idx[k] - 1 : k;
+  const int64_t* cn = conn + ldConn * nd;
+  const double x0 = coord[3 * nd], y0 = coord[3 * nd + 1], z0 = coord[3 * nd + 2];
+  // The full 3x3 tensor, entry [i][j] accumulated as the reference forms
+  // it ((s t_i) t_j): its two halves differ by rounding, and with drag
+  // ratios up to ~1e10 that rounding reaches the velocity.
+  double D[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  double f0 = 0, f1 = 0, f2 = 0;
+  const int64_t cnt = cn[0];
+  for (int64_t c = 1; c <= cnt; ++c) {
+    const int64_t link = cn[2 * c - 1], col = cn[2 * c];
+    const int64_t node2 = links[2 * (link - 1) + (2 - col)];
+    double tx = coord[3 * (node2 - 1)] - x0;
+    double ty = coord[3 * (node2 - 1) + 1] - y0;
+    double tz = coord[3 * (node2 - 1) + 2] - z0;
+    const double nT = sqrt(tx * tx + ty * ty + tz * tz);
+    if (nT < eps) continue;
+    const double tN = 1.0 / nT;
+    tx = tx * tN;
+    ty = ty * tN;
+    tz = tz * tN;
+    const double* sf = segForce + 6 * (link - 1) + 3 * (col - 1);
+    double gx = sf[0], gy = sf[1], gz = sf[2];
+    if (sqrt(gx * gx + gy * gy + gz * gz) * tN < sigmaPN) gx = gy = gz = 0.0;
+    f0 += gx;
+    f1 += gy;
+    f2 += gz;
+
+    double bx = bVec[3 * (link - 1)], by = bVec[3 * (link - 1) + 1], bz = bVec[3 * (link - 1) + 2];
+    const double nB = sqrt(bx * bx + by * by + bz * bz);
+    // <111>-family test on |b_i| / min |b_j|; a zero component gives
+    // Inf/NaN ratios that match no family.
+    const double ax = fabs(bx), ay = fabs(by), az = fabs(bz);
+    const double myz = az < ay ? az : ay;
+    const double mn = myz < ax ? myz : ax;
+    int c1 = 0, c2 = 0, c3 = 0;
+    {
+      const double r[3] = {ax / mn, ay / mn, az / mn};
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        c1 += fabs(1.0 - r[q]) < sqrtEps;
+        c2 += fabs(2.0 - r[q]) < sqrtEps;
+        c3 += fabs(3.0 - r[q]) < sqrtEps;
+      }
+    }
+    const bool screwFamily =
+        (1.0 - nB < sqrtEps) && (c1 == 3 || (c1 == 2 && c2 == 1) || (c1 == 1 && c2 == 1 && c3 == 1));
+    bx = bx / nB;
+    by = by / nB;
+    bz = bz / nB;
+    const double w = nB * nT / 2.0;
+    const double base = screwFamily ? drag.screw : drag.climb;
+    const double tv[3] = {tx, ty, tz};
+    // w (base I + (line - base) t t)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        D[i][j] += w * ((i == j ? base : 0.0) + (drag.line - base) * tv[i] * tv[j]);
+    if (!screwFamily) continue;
+    const double tDb = tx * bx + ty * by + tz * bz;
+    const double cosSq = tDb * tDb;
+    const double sinSq = 1.0 - cosSq;
+    if (sinSq < eps) continue;
+    const double sq = sqrt(sinSq);
+    const double px = (by * tz - bz * ty) / sq;
+    const double py = (bz * tx - bx * tz) / sq;
+    const double pz = (bx * ty - by * tx) / sq;
+    const double qx = py * tz - pz * ty;
+    const double qy = pz * tx - px * tz;
+    const double qz = px * ty - py * tx;
+    const double glide =
+        1.0 / sqrt(sinSq / (drag.edge * drag.edge) + cosSq / (drag.screw * drag.screw));
+    const double climbS = sqrt(sinSq * (drag.climb * drag.climb) + cosSq * (drag.screw * drag.screw));
+    const double gq = glide - drag.screw, cp = climbS - drag.screw;
+    const double pv[3] = {px, py, pz}, qv[3] = {qx, qy, qz};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) D[i][j] += w * (gq * qv[i] * qv[j] + cp * pv[i] * pv[j]);
+  }
+  // Columns c0, c1, c2 of the tensor; the inverse's rows are c1 x c2,
+  // c2 x c0, c0 x c1 over det = c0 . (c1 x c2).
+  const double c0x = D[0][0], c0y = D[1][0], c0z = D[2][0];
+  const double c1x = D[0][1], c1y = D[1][1], c1z = D[2][1];
+  const double c2x = D[0][2], c2y = D[1][2], c2z = D[2][2];
+  const double r0x = c1y * c2z - c1z * c2y, r0y = c1z * c2x - c1x * c2z, r0z = c1x * c2y - c1y * c2x;
+  const double r1x = c2y * c0z - c2z * c0y, r1y = c2z * c0x - c2x * c0z, r1z = c2x * c0y - c2y * c0x;
+  const double r2x = c0y * c1z - c0z * c1y, r2y = c0z * c1x - c0x * c1z, r2z = c0x * c1y - c0y * c1x;
+  const double det = c0x * r0x + c0y * r0y + c0z * r0z;
+  nodeForce[3 * k] = f0;
+  nodeForce[3 * k + 1] = f1;
+  nodeForce[3 * k + 2] = f2;
+  nodeVel[3 * k] = (r0x * f0 + r0y * f1 + r0z * f2) / det;
+  nodeVel[3 * k + 1] = (r1x * f0 + r1y * f1 + r1z * f2) / det;
+  nodeVel[3 * k + 2] = (r2x * f0 + r2y * f1 + r2z * f2) / det;
+}
+
 }  // namespace
 
 void launch_self_force(const double* seg, int64_t ld, const int64_t* rows, int n, const SelfK& K,
@@ -319,6 +437,16 @@ void launch_node_gather(const int64_t* conn, int64_t ldConn, int64_t numNode,
   if (numNode <= 0) return;
   hipLaunchKernelGGL(k_node_gather, dim3((unsigned)((numNode + 255) / 256)), dim3(256), 0, st, conn,
                      ldConn, numNode, links, coord, segForce, sigmaPN, nodeForce);
+}
+
+void launch_mobility_bcc(const int64_t* conn, int64_t ldConn, const int64_t* links,
+                         const double* coord, const double* bVec, const double* segForce,
+                         const int64_t* idx, int64_t n, const double drag[4], double sigmaPN,
+                         double* nodeForce, double* nodeVel, hipStream_t st) {
+  if (n <= 0) return;
+  const DragDev d{drag[0], drag[1], drag[2], drag[3]};
+  hipLaunchKernelGGL(k_mobility_bcc, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, conn,
+                     ldConn, links, coord, bVec, segForce, idx, n, d, sigmaPN, nodeForce, nodeVel);
 }
 
 }  // namespace ddd

@@ -102,5 +102,11 @@ void launch_prep_sigma_segments(double* sseg, int64_t ld, int n, hipStream_t st)
 void launch_node_gather(const int64_t* conn, int64_t ldConn, int64_t numNode,
                         const int64_t* links, const double* coord, const double* segForce,
                         double sigmaPN, double* nodeForce, hipStream_t st);
+// BCC mobility of n nodes (idx: 1-based node list on the device, or NULL
+// for nodes 1..n); drag = (edge, screw, climb, line); outputs 3 x n.
+void launch_mobility_bcc(const int64_t* conn, int64_t ldConn, const int64_t* links,
+                         const double* coord, const double* bVec, const double* segForce,
+                         const int64_t* idx, int64_t n, const double drag[4], double sigmaPN,
+                         double* nodeForce, double* nodeVel, hipStream_t st);
 
 }  // namespace ddd

@@ -31,6 +31,10 @@ class ddd_consts(C.Structure):
         "coreRadSq", "coreEnergy", "sigmaPN")]
 
 
+class ddd_drag(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("edge", "screw", "climb", "line")]
+
+
 class ddd_network(C.Structure):
     _fields_ = [("numNode", i64), ("numSeg", i64), ("segIdx", ip), ("segIdxLd", i64),
                 ("coord", dp), ("coordCols", i64), ("bVec", dp), ("bVecCols", i64),
@@ -106,6 +110,9 @@ def lib():
                                               ip, i64, dp, i64, dp, C.c_int]),
                 "ddd_node_force": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_network),
                                              dp, i64, dp]),
+                "ddd_mobility_bcc": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_drag),
+                                               C.POINTER(ddd_network), dp, i64, ip, i64, dp, dp,
+                                               i64, C.c_int]),
                 "ddd_net_upload": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_network),
                                              C.POINTER(ddd_mesh)]),
                 "ddd_net_seg_force": (C.c_int, [P, C.c_int]),
@@ -231,3 +238,10 @@ def consts(dlnParams, matParams) -> ddd_consts:
                       nuOmNuInv=matParams.νomνInv, coreRad=dlnParams.coreRad,
                       coreRadSq=dlnParams.coreRadSq, coreEnergy=dlnParams.coreEnergy,
                       sigmaPN=matParams.σPN)
+
+
+def drag(dlnParams):
+    """ddd_drag from dlnParams.dragCoeffs; a missing entry fails as the
+    reference's NamedTuple lookup does (DislocationMobility.jl:25-28)."""
+    d = dlnParams.dragCoeffs
+    return ddd_drag(float(d["edge"]), float(d["screw"]), float(d["climb"]), float(d["line"]))

@@ -206,6 +206,45 @@ def calcNodeForce_(dlnParams, matParams, network, ctx=None):
     return nf
 
 
+def _mobility(dlnParams, matParams, network, ix, nf, nv_out, cols, flags, ctx):
+    if dlnParams.mobility != "mobBCC":
+        raise NotImplementedError(f"mobility law {dlnParams.mobility!r}: only mobBCC exists")
+    nv = L.NetView(network)
+    k = L.consts(dlnParams, matParams)
+    dr = L.drag(dlnParams)
+    sf = np.asfortranarray(network.segForce, dtype=np.float64)
+    _call(ctx, "ddd_mobility_bcc", k, _byref(dr), nv.s, L.ptr(sf, "d"), sf.shape[2], *_nidx(ix),
+          L.ptr(nf, "d"), L.ptr(nv_out, "d"), cols, flags)
+
+
+def dlnMobility(dlnParams, matParams, network, idx=None, ctx=None):
+    """dlnMobility for mobBCC (DislocationMobility.jl:16-171): (nodeForce,
+    nodeVel), each 3 x numNode, or 3 x length(idx) with column k for the
+    k-th node of idx."""
+    ctx = _ctx(ctx)
+    ix = _idx(idx)
+    n = int(network.numNode[0]) if ix is None else len(ix)
+    nf = np.zeros((3, n), order="F")
+    vel = np.zeros((3, n), order="F")
+    _mobility(dlnParams, matParams, network, ix, nf, vel, n, L.DDD_OUT_BY_POSITION, ctx)
+    return nf, vel
+
+
+def dlnMobility_(dlnParams, matParams, network, idx=None, ctx=None):
+    """dlnMobility! for mobBCC (DislocationMobility.jl:186-341): writes
+    network.nodeForce[:, node] and network.nodeVel[:, node] for the nodes of
+    idx (all numNode nodes by default) and returns the network."""
+    ctx = _ctx(ctx)
+    ix = _idx(idx)
+    for name in ("nodeForce", "nodeVel"):
+        a = getattr(network, name)
+        if not (a.flags.f_contiguous and a.dtype == np.float64 and a.shape[0] == 3):
+            raise TypeError(f"network.{name} must be a 3 x nodeBuf Float64 column-major array")
+    cols = min(network.nodeForce.shape[1], network.nodeVel.shape[1])
+    _mobility(dlnParams, matParams, network, ix, network.nodeForce, network.nodeVel, cols, 0, ctx)
+    return network
+
+
 class FarField:
     """Hybrid far field (ddd_farfield_*): a G^3 cell grid of edge h from
     corner lo, near radius rn (in cells), and the operator W

@@ -54,6 +54,12 @@ class DislocationParameters:
     coreEnergy: Optional[float] = None
     parCPU: bool = False
     parGPU: bool = True
+    # mobility law and drag coefficients (DislocationType.jl:151-152); the
+    # keyword default of DislocationConstructor.jl:57 has no "line" entry, so
+    # the mobility function fails on it unless the caller supplies one, as
+    # in the reference (DislocationMobility.jl:28).
+    mobility: str = "mobBCC"
+    dragCoeffs: dict = field(default_factory=lambda: {"edge": 1.0, "screw": 2.0, "climb": 1e9})
 
     def __post_init__(self):
         self.coreRad = float(self.coreRad)

@@ -252,6 +252,26 @@ int ddd_sigma_tilde(ddd_ctx* ctx, const ddd_consts* k, const ddd_network* net,
 int ddd_node_force(ddd_ctx* ctx, const ddd_consts* k, const ddd_network* net,
                    const double* segForce, int64_t segForceCols, double* nodeForce);
 
+/* DislocationParameters.dragCoeffs (src/Type/DislocationType.jl:151-152,
+ * read from the parameter file's "dragCoeffs", src/IO/Input.jl:247-251). */
+typedef struct ddd_drag {
+  double edge, screw, climb, line;
+} ddd_drag;
+
+/* dlnMobility / dlnMobility! for mobBCC (DislocationMobility.jl:16-171 and
+ * :186-341): per node, the node force (the gather of ddd_node_force) and the
+ * velocity that solves drag * v = f, with the BCC drag tensor summed over the
+ * node's links.  idx: nIdx 1-based node indices, NULL for nodes 1..numNode.
+ * nodeForce and nodeVel are 3 x outCols.  With DDD_OUT_BY_POSITION the
+ * result of idx[k] goes to column k (dlnMobility); otherwise to column
+ * idx[k] (dlnMobility!: the other columns are left as they are).  A node
+ * whose drag tensor is singular (no links) gets NaN velocities, as the
+ * reference's 3x3 solve gives. */
+int ddd_mobility_bcc(ddd_ctx* ctx, const ddd_consts* k, const ddd_drag* drag,
+                     const ddd_network* net, const double* segForce, int64_t segForceCols,
+                     const int64_t* idx, int64_t nIdx, double* nodeForce, double* nodeVel,
+                     int64_t outCols, int flags);
+
 /* ---- Device-resident path (no counterpart in the reference) ----------
  * For callers that evaluate forces many times on one network (time steps,
  * benchmarks): upload once, compute on the device, fetch when needed.

@@ -562,3 +562,110 @@ void orc_node_gather(const int64_t* connectivity, int64_t ldConn, int64_t numNod
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// BCC mobility of the nodes idx[0..nIdx) (all numNode nodes when idx is
+// NULL): dlnMobility / dlnMobility! for mobBCC, DislocationMobility.jl:16-171
+// and :186-341.  drag = (edge, screw, climb, line), the reference's
+// dragCoeffs.  Output column k belongs to the k-th node of the list;
+// dragOut (9 doubles per node, column-major, may be NULL) receives the drag
+// tensor the velocity solves with.
+//
+// Per connection (:56-146): skip links shorter than eps (:67-71); the force
+// column of the connection, zeroed below sigmaPN (:74-80); then the drag of
+// the link: an edge-type law climb I + (line - climb) t t when b is not a
+// <111>-family screw vector (:84-114), else the screw law screw I +
+// (line - screw) t t plus, unless the link is pure screw, the glide and
+// climb corrections along q and p (:116-145, Arsenlis et al. 2007 eq. 112).
+// Velocity (:148): the 3x3 system drag v = f solved by Cramer's rule, which
+// is what the reference's StaticArrays `\` does for a 3x3 SMatrix
+// (StaticArrays.jl, no version pinned): here as the adjugate built from
+// cross products of the columns over the determinant as a triple product.
+// The rounding of that solve therefore differs from Julia's by a few ulp
+// times the condition number of the drag (up to ~1e10 with climb = 1e10);
+// the reference's own test allows rtol 1e-5 (MobilityTest.jl:92).
+void orc_mobility_bcc(const double* drag, double sigmaPN, const int64_t* connectivity,
+                      int64_t ldConn, int64_t numNode, const int64_t* links, const double* coord,
+                      const double* bVec, const double* segForce, const int64_t* idx, int64_t nIdx,
+                      double* nodeForce, double* nodeVel, double* dragOut) {
+  const double eps = std::numeric_limits<double>::epsilon();
+  const double sqrtEps = std::sqrt(eps);
+  const double edgeD = drag[0], screwD = drag[1], climbD = drag[2], lineD = drag[3];
+  const int64_t n = idx ? nIdx : numNode;
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t node = idx ? idx[k] - 1 : k;
+    const int64_t* cn = connectivity + ldConn * node;
+    double D[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double f[3] = {0, 0, 0};
+    const V3<double> x0 = mk(coord[3 * node], coord[3 * node + 1], coord[3 * node + 2]);
+    for (int64_t c = 1; c <= cn[0]; ++c) {
+      const int64_t link = cn[2 * c - 1], col = cn[2 * c];
+      const int64_t node2 = links[2 * (link - 1) + (3 - col) - 1];
+      V3<double> t = mk(coord[3 * (node2 - 1)], coord[3 * (node2 - 1) + 1],
+                        coord[3 * (node2 - 1) + 2]) - x0;
+      const double nT = norm(t);
+      if (nT < eps) continue;
+      const double tN = 1 / nT;
+      t = t * tN;
+      const double* sf = segForce + 6 * (link - 1) + 3 * (col - 1);
+      V3<double> fc = mk(sf[0], sf[1], sf[2]);
+      if (norm(fc) * tN < sigmaPN) fc = mk(0.0, 0.0, 0.0);
+      f[0] += fc.x; f[1] += fc.y; f[2] += fc.z;
+
+      V3<double> b = mk(bVec[3 * (link - 1)], bVec[3 * (link - 1) + 1], bVec[3 * (link - 1) + 2]);
+      const double nB = norm(b);
+      // <111>-family test on |b| / min|b_i| (zero components give Inf/NaN
+      // ratios that match no family, as in the reference).
+      const double ab[3] = {std::fabs(b.x), std::fabs(b.y), std::fabs(b.z)};
+      const double mn = std::min(ab[0], std::min(ab[1], ab[2]));
+      int n1 = 0, n2 = 0, n3 = 0;
+      for (int q = 0; q < 3; ++q) {
+        const double r = ab[q] / mn;
+        n1 += std::fabs(1 - r) < sqrtEps;
+        n2 += std::fabs(2 - r) < sqrtEps;
+        n3 += std::fabs(3 - r) < sqrtEps;
+      }
+      const bool unitLike = 1 - nB < sqrtEps;
+      b = b / nB;
+      const double w = nB * nT / 2;
+      const double tv[3] = {t.x, t.y, t.z};
+      const bool screwFamily = unitLike && ((n1 == 3) || (n1 == 2 && n2 == 1) ||
+                                            (n1 == 1 && n2 == 1 && n3 == 1));
+      const double base = screwFamily ? screwD : climbD;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          D[i][j] += w * ((i == j ? base : 0.0) + (lineD - base) * tv[i] * tv[j]);
+      if (!screwFamily) continue;
+      const double cosSq = dot(t, b) * dot(t, b);
+      const double sinSq = 1 - cosSq;
+      if (sinSq < eps) continue;
+      const V3<double> p = cross(b, t) / std::sqrt(sinSq);
+      const V3<double> q = cross(p, t);
+      const double glide = 1 / std::sqrt(sinSq / (edgeD * edgeD) + cosSq / (screwD * screwD));
+      const double climbS = std::sqrt(sinSq * (climbD * climbD) + cosSq * (screwD * screwD));
+      const double pv[3] = {p.x, p.y, p.z}, qv[3] = {q.x, q.y, q.z};
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          D[i][j] += w * ((glide - screwD) * qv[i] * qv[j] + (climbS - screwD) * pv[i] * pv[j]);
+    }
+    // Cramer's rule: inverse rows are the cross products of the columns.
+    const V3<double> c0 = mk(D[0][0], D[1][0], D[2][0]);
+    const V3<double> c1 = mk(D[0][1], D[1][1], D[2][1]);
+    const V3<double> c2 = mk(D[0][2], D[1][2], D[2][2]);
+    const V3<double> r0 = cross(c1, c2), r1 = cross(c2, c0), r2 = cross(c0, c1);
+    const double det = dot(c0, r0);
+    const V3<double> fv = mk(f[0], f[1], f[2]);
+    nodeForce[3 * k] = f[0];
+    nodeForce[3 * k + 1] = f[1];
+    nodeForce[3 * k + 2] = f[2];
+    nodeVel[3 * k] = dot(r0, fv) / det;
+    nodeVel[3 * k + 1] = dot(r1, fv) / det;
+    nodeVel[3 * k + 2] = dot(r2, fv) / det;
+    if (dragOut)
+      for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) dragOut[9 * k + 3 * j + i] = D[i][j];
+  }
+}
+
+}  // extern "C"

@@ -51,6 +51,8 @@ def lib():
             "orc_pk_force": (None, mesh + net + [ip, i64, dp, i32, i32]),
             "orc_sigma_tilde": (None, [dp, i64, ip, i64, dp, dp, ip, i64, dp, i64, dp, i32]),
             "orc_node_gather": (None, [ip, i64, i64, ip, dp, dp, d, dp]),
+            "orc_mobility_bcc": (None, [dp, d, ip, i64, i64, ip, dp, dp, dp, ip, i64, dp, dp,
+                                        dp]),
             "orc_set_libm": (None, [i32]),
             "orc_segseg_near": (None, [dp] + net + [ip, i32, i32, dp, i32]),
             "orc_segseg_near_once": (None, [dp] + net + [ip, i32, i32, ip, dp]),
@@ -346,3 +348,32 @@ def nodeForce(mat, net, segForce=None):
     lib().orc_node_gather(_i(conn), conn.shape[0], numNode, _i(links), _d(coord), _d(sf),
                           C.c_double(mat.σPN), _d(out))
     return out
+
+
+def drag_coeffs(dln):
+    """(edge, screw, climb, line) of dlnParams.dragCoeffs
+    (DislocationMobility.jl:25-28)."""
+    d = dln.dragCoeffs
+    return np.array([d["edge"], d["screw"], d["climb"], d["line"]], dtype=np.float64)
+
+
+def dlnMobility(dln, mat, net, idx=None, segForce=None, with_drag=False):
+    """dlnMobility for mobBCC (DislocationMobility.jl:16-171): (nodeForce,
+    nodeVel), 3 x n with column k for the k-th node of idx (all nodes when
+    idx is None); with_drag also returns the 3 x 3 x n drag tensors."""
+    sf = _f(net.segForce if segForce is None else segForce)
+    conn = _f(net.connectivity, np.int64)
+    links = _f(net.links, np.int64)
+    coord = _f(net.coord)
+    bVec = _f(net.bVec)
+    numNode = int(net.numNode[0])
+    ix = None if idx is None else np.ascontiguousarray(np.atleast_1d(idx), dtype=np.int64)
+    n = numNode if ix is None else len(ix)
+    F = np.zeros((3, n), order="F")
+    V = np.zeros((3, n), order="F")
+    D = np.zeros((3, 3, n), order="F") if with_drag else None
+    lib().orc_mobility_bcc(_d(drag_coeffs(dln)), C.c_double(mat.σPN), _i(conn), conn.shape[0],
+                           numNode, _i(links), _d(coord), _d(bVec), _d(sf),
+                           _i(ix) if ix is not None else None, n, _d(F), _d(V),
+                           _d(D) if with_drag else None)
+    return (F, V, D) if with_drag else (F, V)

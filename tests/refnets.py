@@ -27,10 +27,12 @@ def g(name):
 
 def params():
     """loadParameters of inputs/simParams/sample{Dislocation,Material}Parameters.json
-    (Input.jl:52-64,244-269): coreRad 90.78, Ec 1, mu 1, nu 0.28, sigmaPN 0."""
+    (Input.jl:52-64,244-269): coreRad 90.78, Ec 1, mu 1, nu 0.28, sigmaPN 0,
+    drag edge 1, screw 2, climb 1e10, line 0."""
     d = GOLD["file:inputs/simParams/sampleDislocationParameters.json"]["data"]
     m = GOLD["file:inputs/simParams/sampleMaterialParameters.json"]["data"]
-    dln = DislocationParameters(coreRad=float(d["coreRad"]), coreEnergy=float(d["coreEnergy"]))
+    dln = DislocationParameters(coreRad=float(d["coreRad"]), coreEnergy=float(d["coreEnergy"]),
+                                dragCoeffs={k: float(v) for k, v in d["dragCoeffs"].items()})
     mat = MaterialParameters(μ=float(m["μ"]), ν=float(m["ν"]), μMag=float(m["μMag"]),
                              σPN=float(m["σPN"]))
     return dln, mat
