@@ -138,6 +138,12 @@ struct ddd_ctx {
   DevBuf counter;  // FP64-pair counter of the mixed path
   // near-parallel pairs of the all-pairs tiles (count, offsets, list, forces)
   DevBuf specCnt, specOff, specSlot, specTile, specRes, specWork;
+  // device-resident dynamics (deriv!, integrate!)
+  DevBuf dCoord, dCoord0, dVel, dVel0, dNodeF, dLinks, dConn, dBvec, dLabel, dSegNodes, dRedPart,
+      dRed;
+  int64_t dynNodes = -1;
+  int64_t dynConnLd = 0;
+  double dynDrag[4] = {0, 0, 0, 0};
   // device-resident network
   ddd_opts opts{DDD_MODE_FP64, 0.0, 0.0, nullptr};
   int64_t resN = -1;
@@ -215,7 +221,14 @@ int64_t pack_segments(ddd_ctx* c, const ddd_network* net, DevBuf& dst) {
   return ld;
 }
 
-void upload_segments(ddd_ctx* c, const ddd_network* net) { c->resLd = pack_segments(c, net, c->seg); }
+// The one-shot entry points and ddd_net_upload share the segment buffer
+// with the resident network, so every upload discards the resident state
+// (ddd_net_upload sets it up again afterwards).
+void upload_segments(ddd_ctx* c, const ddd_network* net) {
+  c->resN = -1;
+  c->dynNodes = -1;
+  c->resLd = pack_segments(c, net, c->seg);
+}
 
 // 0-based row list from 1-based idx, validated.
 std::vector<int64_t> rows_from(const int64_t* idx, int64_t nIdx, int64_t numSeg) {
@@ -811,7 +824,7 @@ int ddd_mobility_bcc(ddd_ctx* c, const ddd_consts* k, const ddd_drag* drag, cons
     const double dr[4] = {drag->edge, drag->screw, drag->climb, drag->line};
     launch_mobility_bcc(c->conn.as<int64_t>(), net->connLd, c->links.as<int64_t>(),
                         c->coord.as<double>(), c->bvec.as<double>(), c->out.as<double>(),
-                        idx ? c->nodeIdx.as<int64_t>() : nullptr, n, dr, k->sigmaPN,
+                        idx ? c->nodeIdx.as<int64_t>() : nullptr, n, dr, k->sigmaPN, nullptr,
                         c->nodeF.as<double>(), c->nodeV.as<double>(), c->st);
     HIPCHK(hipGetLastError());
     if (byPos || !idx) {
@@ -1512,6 +1525,189 @@ int ddd_farfield_local_targets(ddd_ctx* c, ddd_farfield* ff, const ddd_consts* k
 
 const double* ddd_net_segforce_device(ddd_ctx* c) {
   return (c && c->resN >= 0) ? c->force.as<double>() : nullptr;
+}
+
+namespace {
+
+// Julia's min/max for Float64: NaN wins, and -0.0 < 0.0.
+double jmin(double a, double b) {
+  if (std::isnan(a) || std::isnan(b)) return a + b;
+  return (b < a || (std::signbit(b) && !std::signbit(a))) ? b : a;
+}
+double jmax(double a, double b) {
+  if (std::isnan(a) || std::isnan(b)) return a + b;
+  return (b > a || (!std::signbit(b) && std::signbit(a))) ? b : a;
+}
+
+void dyn_deriv(ddd_ctx* c, int withPK) {
+  if (c->dynNodes < 0) bad("no dynamics state (ddd_net_upload_dynamics)");
+  const int64_t ns = c->resN;
+  launch_gather_coords(c->seg.as<double>(), c->resLd, ns, c->dSegNodes.as<int64_t>(),
+                       c->dCoord.as<double>(), c->st);
+  launch_prep_segments(c->seg.as<double>(), c->resLd, ns, c->st);
+  for (int s = 0; s < 3; ++s) net_stage(c, s, withPK);
+  launch_mobility_bcc(c->dConn.as<int64_t>(), c->dynConnLd, c->dLinks.as<int64_t>(),
+                      c->dCoord.as<double>(), c->dBvec.as<double>(), c->force.as<double>(),
+                      nullptr, c->dynNodes, c->dynDrag, c->resK.sigmaPN, c->dLabel.as<int32_t>(),
+                      c->dNodeF.as<double>(), c->dVel.as<double>(), c->st);
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace
+
+int ddd_net_upload_dynamics(ddd_ctx* c, const ddd_network* net, const int32_t* label,
+                            const ddd_drag* drag) {
+  return guard(c, "ddd_net_upload_dynamics", [&] {
+    if (!net || !label || !drag) bad("NULL argument");
+    if (c->resN < 0 || c->resN != net->numSeg)
+      bad("upload the same network with ddd_net_upload first");
+    if (!net->connectivity || !net->links || !net->coord || !net->bVec || !net->segIdx)
+      bad("segIdx, coord, bVec, links and connectivity required");
+    const int64_t nn = net->numNode, ns = net->numSeg, L = net->segIdxLd;
+    if (nn < 0 || nn > net->connCols || nn > net->coordCols) bad("numNode outside arrays");
+    if (ns > net->linksCols || ns > net->bVecCols) bad("numSeg outside links or bVec");
+    if (net->connLd < 1) bad("connectivity leading dimension");
+    std::vector<int64_t> nodes((size_t)std::max<int64_t>(2 * ns, 1));
+    for (int64_t i = 0; i < ns; ++i) {
+      if (net->segIdx[i] != i + 1) bad("segIdx[:, 1] must be 1:numSeg, as getSegmentIdx! makes it");
+      const int64_t a = net->segIdx[i + L], b = net->segIdx[i + 2 * L];
+      if (a < 1 || a > nn || b < 1 || b > nn) bad("segment node outside 1:numNode");
+      nodes[2 * i] = a - 1;
+      nodes[2 * i + 1] = b - 1;
+    }
+    const int64_t maxC = (net->connLd - 1) / 2;
+    for (int64_t q = 0; q < nn; ++q) {
+      if (label[q] == 3)
+        bad("srfMobDln nodes: the reference's surface velocity correction cannot run");
+      const int64_t* cn = net->connectivity + net->connLd * q;
+      if (cn[0] < 0 || cn[0] > maxC) bad("connectivity count");
+      for (int64_t e = 1; e <= cn[0]; ++e) {
+        const int64_t lk = cn[2 * e - 1], col = cn[2 * e];
+        if (lk < 1 || lk > ns) bad("connectivity link outside 1:numSeg");
+        if (col != 1 && col != 2) bad("connectivity column");
+        const int64_t other = net->links[2 * (lk - 1) + (2 - col)];
+        if (other < 1 || other > nn) bad("links node outside 1:numNode");
+      }
+    }
+    const size_t m = (size_t)std::max<int64_t>(3 * nn, 1);
+    c->dCoord.ensure(sizeof(double) * m);
+    c->dCoord0.ensure(sizeof(double) * m);
+    c->dVel.ensure(sizeof(double) * m);
+    c->dVel0.ensure(sizeof(double) * m);
+    c->dNodeF.ensure(sizeof(double) * m);
+    c->dLinks.ensure(sizeof(int64_t) * std::max<int64_t>(2 * ns, 1));
+    c->dConn.ensure(sizeof(int64_t) * std::max<int64_t>(net->connLd * nn, 1));
+    c->dBvec.ensure(sizeof(double) * std::max<int64_t>(3 * ns, 1));
+    c->dLabel.ensure(sizeof(int32_t) * std::max<int64_t>(nn, 1));
+    c->dSegNodes.ensure(sizeof(int64_t) * nodes.size());
+    c->dRedPart.ensure(sizeof(unsigned long long) * 2 * trapezoid_parts());
+    c->dRed.ensure(sizeof(double) * 2);
+    if (nn > 0) {
+      HIPCHK(hipMemcpyAsync(c->dCoord.p, net->coord, sizeof(double) * 3 * nn,
+                            hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipMemcpyAsync(c->dConn.p, net->connectivity, sizeof(int64_t) * net->connLd * nn,
+                            hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipMemcpyAsync(c->dLabel.p, label, sizeof(int32_t) * nn, hipMemcpyHostToDevice,
+                            c->st));
+      HIPCHK(hipMemsetAsync(c->dVel.p, 0, sizeof(double) * 3 * nn, c->st));
+      HIPCHK(hipMemsetAsync(c->dNodeF.p, 0, sizeof(double) * 3 * nn, c->st));
+    }
+    if (ns > 0) {
+      HIPCHK(hipMemcpyAsync(c->dLinks.p, net->links, sizeof(int64_t) * 2 * ns,
+                            hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipMemcpyAsync(c->dBvec.p, net->bVec, sizeof(double) * 3 * ns, hipMemcpyHostToDevice,
+                            c->st));
+      HIPCHK(hipMemcpyAsync(c->dSegNodes.p, nodes.data(), sizeof(int64_t) * 2 * ns,
+                            hipMemcpyHostToDevice, c->st));
+    }
+    c->dynNodes = nn;
+    c->dynConnLd = net->connLd;
+    c->dynDrag[0] = drag->edge;
+    c->dynDrag[1] = drag->screw;
+    c->dynDrag[2] = drag->climb;
+    c->dynDrag[3] = drag->line;
+    HIPCHK(hipStreamSynchronize(c->st));  // nodes (host vector) is released on return
+  });
+}
+
+int ddd_net_deriv(ddd_ctx* c, int withPK) {
+  return guard(c, "ddd_net_deriv", [&] { dyn_deriv(c, withPK); });
+}
+
+int ddd_net_integrate(ddd_ctx* c, const ddd_integ* ip, ddd_itime* t, int withPK) {
+  return guard(c, "ddd_net_integrate", [&] {
+    if (!ip || !t) bad("NULL argument");
+    if (c->dynNodes < 0) bad("no dynamics state (ddd_net_upload_dynamics)");
+    const int64_t nn = c->dynNodes;
+    if (nn == 0) return;  // IntegrationProcessing.jl:94
+    dyn_deriv(c, withPK);
+    const int64_t m = 3 * nn;
+    HIPCHK(hipMemcpyAsync(c->dCoord0.p, c->dCoord.p, sizeof(double) * m, hipMemcpyDeviceToDevice,
+                          c->st));
+    HIPCHK(hipMemcpyAsync(c->dVel0.p, c->dVel.p, sizeof(double) * m, hipMemcpyDeviceToDevice,
+                          c->st));
+    double dt = jmax(t->dt, ip->dtmin);
+    const double time = t->time;
+    const int64_t step = t->step;
+    double dtOld = 0.0;
+    bool dtOldGood = false, convergent = false;
+    int64_t counter = 1;
+    // The reference's loop has no bound; one that keeps failing at a step
+    // it once accepted would spin, which a deterministic evaluation cannot.
+    for (int64_t iter = 0; !convergent; ++iter) {
+      if (iter > 100000) bad("integrate! did not converge in 100000 trials");
+      launch_euler(c->dCoord.as<double>(), c->dCoord0.as<double>(), c->dVel0.as<double>(), dt, m,
+                   c->st);
+      dyn_deriv(c, withPK);
+      launch_trapezoid_max(c->dCoord.as<double>(), c->dCoord0.as<double>(), c->dVel.as<double>(),
+                           c->dVel0.as<double>(), dt, m,
+                           c->dRedPart.as<unsigned long long>(), c->dRed.as<double>(), c->st);
+      HIPCHK(hipGetLastError());
+      double red[2];
+      HIPCHK(hipMemcpyAsync(red, c->dRed.p, sizeof(red), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+      const double maxDist = red[0], maxErr = red[1];
+      if (dt <= ip->dtmin) {
+        counter = ip->maxiter + 1;
+      } else if (maxDist < ip->abstol && maxErr < ip->reltol) {
+        dtOld = dt;
+        const double factor =
+            ip->maxchange *
+            std::pow(1.0 / (1.0 + (std::pow(ip->maxchange, ip->exponent) - 1.0) *
+                                      (maxErr / ip->reltol)),
+                     1.0 / ip->exponent);
+        dtOldGood = true;
+        dt = jmin(dt * factor, ip->dtmax);
+        counter += 1;
+      } else if (dtOldGood) {
+        dt = dtOld;
+        counter = ip->maxiter;
+      } else {
+        dt = dt / 2.0;
+      }
+      if (counter > ip->maxiter || dt == ip->dtmax) convergent = true;
+    }
+    t->dt = dt;
+    t->time = time + dt;
+    t->step = step + 1;
+    check_flags(c);
+  });
+}
+
+int ddd_net_download_nodes(ddd_ctx* c, double* coord, double* nodeVel, double* nodeForce,
+                           int64_t cols) {
+  return guard(c, "ddd_net_download_nodes", [&] {
+    if (c->dynNodes < 0) bad("no dynamics state (ddd_net_upload_dynamics)");
+    const int64_t nn = c->dynNodes;
+    if (cols < nn) bad("cols < numNode");
+    if (nn == 0) return;
+    const size_t bytes = sizeof(double) * 3 * nn;
+    if (coord) HIPCHK(hipMemcpyAsync(coord, c->dCoord.p, bytes, hipMemcpyDeviceToHost, c->st));
+    if (nodeVel) HIPCHK(hipMemcpyAsync(nodeVel, c->dVel.p, bytes, hipMemcpyDeviceToHost, c->st));
+    if (nodeForce)
+      HIPCHK(hipMemcpyAsync(nodeForce, c->dNodeF.p, bytes, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
 }
 
 }  // extern "C"

@@ -280,8 +280,8 @@ __global__ void __launch_bounds__(256)
 k_mobility_bcc(const int64_t* __restrict__ conn, int64_t ldConn, const int64_t* __restrict__ links,
                const double* __restrict__ coord, const double* __restrict__ bVec,
                const double* __restrict__ segForce, const int64_t* __restrict__ idx, int64_t n,
-               DragDev drag, double sigmaPN, double* __restrict__ nodeForce,
-               double* __restrict__ nodeVel) {
+               DragDev drag, double sigmaPN, const int32_t* __restrict__ label,
+               double* __restrict__ nodeForce, double* __restrict__ nodeVel) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k >= n) return;
   const double eps = 2.220446049250313e-16;
@@ -379,9 +379,18 @@ k_mobility_bcc(const int64_t* __restrict__ conn, int64_t ldConn, const int64_t* 
   nodeForce[3 * k] = f0;
   nodeForce[3 * k + 1] = f1;
   nodeForce[3 * k + 2] = f2;
-  nodeVel[3 * k] = (r0x * f0 + r0y * f1 + r0z * f2) / det;
-  nodeVel[3 * k + 1] = (r1x * f0 + r1y * f1 + r1z * f2) / det;
-  nodeVel[3 * k + 2] = (r2x * f0 + r2y * f1 + r2z * f2) / det;
+  double v0 = (r0x * f0 + r0y * f1 + r0z * f2) / det;
+  double v1 = (r1x * f0 + r1y * f1 + r1z * f2) / det;
+  double v2 = (r2x * f0 + r2y * f1 + r2z * f2) / det;
+  if (label) {
+    // deriv! (IntegrationProcessing.jl:29-32): internal fixed, surface fixed
+    // and external nodes (nodeTypeDln 2, 4, 5) do not move.
+    const int32_t lb = label[nd];
+    if (lb == 2 || lb == 4 || lb == 5) v0 = v1 = v2 = 0.0;
+  }
+  nodeVel[3 * k] = v0;
+  nodeVel[3 * k + 1] = v1;
+  nodeVel[3 * k + 2] = v2;
 }
 
 }  // namespace
@@ -442,11 +451,13 @@ void launch_node_gather(const int64_t* conn, int64_t ldConn, int64_t numNode,
 void launch_mobility_bcc(const int64_t* conn, int64_t ldConn, const int64_t* links,
                          const double* coord, const double* bVec, const double* segForce,
                          const int64_t* idx, int64_t n, const double drag[4], double sigmaPN,
-                         double* nodeForce, double* nodeVel, hipStream_t st) {
+                         const int32_t* label, double* nodeForce, double* nodeVel,
+                         hipStream_t st) {
   if (n <= 0) return;
   const DragDev d{drag[0], drag[1], drag[2], drag[3]};
   hipLaunchKernelGGL(k_mobility_bcc, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, conn,
-                     ldConn, links, coord, bVec, segForce, idx, n, d, sigmaPN, nodeForce, nodeVel);
+                     ldConn, links, coord, bVec, segForce, idx, n, d, sigmaPN, label, nodeForce,
+                     nodeVel);
 }
 
 }  // namespace ddd

@@ -103,10 +103,25 @@ void launch_node_gather(const int64_t* conn, int64_t ldConn, int64_t numNode,
                         const int64_t* links, const double* coord, const double* segForce,
                         double sigmaPN, double* nodeForce, hipStream_t st);
 // BCC mobility of n nodes (idx: 1-based node list on the device, or NULL
-// for nodes 1..n); drag = (edge, screw, climb, line); outputs 3 x n.
+// for nodes 1..n); drag = (edge, screw, climb, line); outputs 3 x n.  With
+// a label array (nodeTypeDln per node) fixed and external nodes get zero
+// velocity, as deriv! does.
 void launch_mobility_bcc(const int64_t* conn, int64_t ldConn, const int64_t* links,
                          const double* coord, const double* bVec, const double* segForce,
                          const int64_t* idx, int64_t n, const double drag[4], double sigmaPN,
-                         double* nodeForce, double* nodeVel, hipStream_t st);
+                         const int32_t* label, double* nodeForce, double* nodeVel,
+                         hipStream_t st);
+// Time integration (dynamics.hip): segment end points from device
+// coordinates (nodes: 2 per segment, 0-based), the forward-Euler trial, and
+// the two maxima of the Euler-trapezoid error test (out[0] = max |distance|,
+// out[1] = max |error|; part: 2 trapezoid_parts() entries of scratch).
+void launch_gather_coords(double* seg, int64_t ld, int64_t n, const int64_t* nodes,
+                          const double* coord, hipStream_t st);
+void launch_euler(double* coord, const double* init, const double* vel, double dt, int64_t m,
+                  hipStream_t st);
+int trapezoid_parts();
+void launch_trapezoid_max(const double* coord, const double* init, const double* vel,
+                          const double* vel0, double dt, int64_t m, unsigned long long* part,
+                          double* out, hipStream_t st);
 
 }  // namespace ddd

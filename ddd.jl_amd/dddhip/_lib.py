@@ -35,6 +35,15 @@ class ddd_drag(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("edge", "screw", "climb", "line")]
 
 
+class ddd_integ(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("tmin", "tmax", "dtmin", "dtmax", "abstol", "reltol",
+                                          "maxchange", "exponent")] + [("maxiter", C.c_int64)]
+
+
+class ddd_itime(C.Structure):
+    _fields_ = [("dt", C.c_double), ("time", C.c_double), ("step", C.c_int64)]
+
+
 class ddd_network(C.Structure):
     _fields_ = [("numNode", i64), ("numSeg", i64), ("segIdx", ip), ("segIdxLd", i64),
                 ("coord", dp), ("coordCols", i64), ("bVec", dp), ("bVecCols", i64),
@@ -110,6 +119,12 @@ def lib():
                                               ip, i64, dp, i64, dp, C.c_int]),
                 "ddd_node_force": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_network),
                                              dp, i64, dp]),
+                "ddd_net_upload_dynamics": (C.c_int, [P, C.POINTER(ddd_network),
+                                                      C.POINTER(C.c_int32), C.POINTER(ddd_drag)]),
+                "ddd_net_deriv": (C.c_int, [P, C.c_int]),
+                "ddd_net_integrate": (C.c_int, [P, C.POINTER(ddd_integ), C.POINTER(ddd_itime),
+                                                C.c_int]),
+                "ddd_net_download_nodes": (C.c_int, [P, dp, dp, dp, i64]),
                 "ddd_mobility_bcc": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_drag),
                                                C.POINTER(ddd_network), dp, i64, ip, i64, dp, dp,
                                                i64, C.c_int]),
@@ -245,3 +260,12 @@ def drag(dlnParams):
     reference's NamedTuple lookup does (DislocationMobility.jl:25-28)."""
     d = dlnParams.dragCoeffs
     return ddd_drag(float(d["edge"]), float(d["screw"]), float(d["climb"]), float(d["line"]))
+
+
+def integ(intParams):
+    if intParams.method != "AdaptiveEulerTrapezoid":
+        raise NotImplementedError(f"integrator {intParams.method!r}")
+    return ddd_integ(float(intParams.tmin), float(intParams.tmax), float(intParams.dtmin),
+                     float(intParams.dtmax), float(intParams.abstol), float(intParams.reltol),
+                     float(intParams.maxchange), float(intParams.exponent),
+                     int(intParams.maxiter))

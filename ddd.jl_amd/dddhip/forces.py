@@ -439,3 +439,68 @@ class ResidentNetwork:
 
     def stream(self):
         return L.lib().ddd_ctx_stream(self.ctx.h)
+
+    # ---- dynamics: deriv! and integrate! with the nodes on the device
+    def upload_dynamics(self, dlnParams, network):
+        """Keep coord, links, connectivity, bVec and labels on the device
+        (ddd_net_upload_dynamics) for deriv/integrate."""
+        self.numNode = int(network.numNode[0])
+        self._label = np.ascontiguousarray(network.label, dtype=np.int32)
+        self._drag = L.drag(dlnParams)
+        _call(self.ctx, "ddd_net_upload_dynamics", self.nv.s,
+              self._label.ctypes.data_as(_c.POINTER(_c.c_int32)), _byref(self._drag))
+
+    def deriv(self, with_pk=False):
+        _call(self.ctx, "ddd_net_deriv", 1 if with_pk else 0)
+
+    def integrate(self, intParams, intVars, with_pk=False):
+        """One integrate! step; returns the new IntegrationTime."""
+        from .types import IntegrationTime
+        t = L.ddd_itime(float(intVars.dt), float(intVars.time), int(intVars.step))
+        ip = L.integ(intParams)
+        _call(self.ctx, "ddd_net_integrate", _byref(ip), _byref(t), 1 if with_pk else 0)
+        return IntegrationTime(t.dt, t.time, t.step)
+
+    def download_nodes(self):
+        """(coord, nodeVel, nodeForce), each 3 x numNode."""
+        out = [np.zeros((3, self.numNode), order="F") for _ in range(3)]
+        _call(self.ctx, "ddd_net_download_nodes", *(L.ptr(a, "d") for a in out), self.numNode)
+        return tuple(out)
+
+
+def _dynamics(dlnParams, matParams, mesh, forceDisplacement, network, ctx):
+    if dlnParams.mobility != "mobBCC":
+        raise NotImplementedError(f"mobility law {dlnParams.mobility!r}: only mobBCC exists")
+    r = ResidentNetwork(dlnParams, matParams, network, mesh, forceDisplacement, ctx)
+    r.upload_dynamics(dlnParams, network)
+    return r
+
+
+def _store(r, network):
+    n, ns = r.numNode, r.numSeg
+    c, v, f = r.download_nodes()
+    network.coord[:, :n] = c
+    network.nodeVel[:, :n] = v
+    network.nodeForce[:, :n] = f
+    network.segForce[:, :, :ns] = r.download()
+
+
+def deriv_(dlnParams, matParams, mesh, forceDisplacement, network, ctx=None):
+    """deriv! (IntegrationProcessing.jl:13-62): calcSegForce!, dlnMobility!
+    and zero velocity of fixed nodes, on the device; fills network.segForce,
+    nodeForce and nodeVel.  mesh None leaves out the PK force."""
+    r = _dynamics(dlnParams, matParams, mesh, forceDisplacement, network, _ctx(ctx))
+    r.deriv(with_pk=mesh is not None)
+    _store(r, network)
+
+
+def integrate_(intParams, intVars, dlnParams, matParams, mesh, forceDisplacement, network,
+               ctx=None):
+    """integrate! for AdaptiveEulerTrapezoid (IntegrationProcessing.jl:81-168)
+    on the device: moves network.coord, fills segForce, nodeForce, nodeVel
+    and returns the new IntegrationTime.  For many steps keep the network
+    resident (ResidentNetwork.upload_dynamics / integrate) instead."""
+    r = _dynamics(dlnParams, matParams, mesh, forceDisplacement, network, _ctx(ctx))
+    t = r.integrate(intParams, intVars, with_pk=mesh is not None)
+    _store(r, network)
+    return t

@@ -69,6 +69,33 @@ class DislocationParameters:
             self.coreEnergy = 1 / (4 * math.pi) * math.log(self.coreRad / 0.1)
 
 
+@dataclass
+class IntegrationParameters:
+    """IntegrationParameters (IntegrationType.jl:28-39) with the keyword
+    defaults of IntegrationConstructor.jl:17-42; the only method is
+    AdaptiveEulerTrapezoid."""
+
+    method: str = "AdaptiveEulerTrapezoid"
+    tmin: float = 0.0
+    tmax: float = 1e13
+    dtmin: float = 1e-3
+    dtmax: float = math.inf
+    abstol: float = 1e-6
+    reltol: float = 1e-6
+    maxchange: float = 1.2
+    exponent: float = 20.0
+    maxiter: int = 10
+
+
+@dataclass
+class IntegrationTime:
+    """IntegrationTime (IntegrationType.jl:51-55)."""
+
+    dt: float = 0.0
+    time: float = 0.0
+    step: int = 0
+
+
 # data/slipSystems/BCC.json: 12 {110}<111> slip systems (plane normal, Burgers
 # vector), in the file's order.  Kept as a table because the generator needs it
 # where the reference tree is absent.

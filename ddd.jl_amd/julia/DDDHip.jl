@@ -171,6 +171,51 @@ function calc_σTilde!(σ, x0, dlnParams, matParams, network::DislocationNetwork
     return σ
 end
 
+# dlnMobility / dlnMobility! for mobBCC (src/Processing/DislocationMobility.jl:
+# 16-171, 186-341): node forces gathered from network.segForce and velocities
+# from the BCC drag tensor, on the device.
+struct Drag
+    edge::Float64
+    screw::Float64
+    climb::Float64
+    line::Float64
+end
+drag(d::DislocationParameters) = Drag(d.dragCoeffs[:edge], d.dragCoeffs[:screw],
+                                      d.dragCoeffs[:climb], d.dragCoeffs[:line])
+
+function mobility_call(dlnParams, matParams, network, idx, nodeForce, nodeVel, flags)
+    k = Ref(consts(dlnParams, matParams))
+    dr = Ref(drag(dlnParams))
+    ix = idxvec(idx)
+    cols = min(size(nodeForce, 2), size(nodeVel, 2))
+    GC.@preserve network ix nodeForce nodeVel begin
+        nw = Ref(netstruct(network))
+        check(ccall((:ddd_mobility_bcc, libddd), Cint,
+            (Ptr{Cvoid}, Ref{Consts}, Ref{Drag}, Ref{Network}, Ptr{Float64}, Int64, Ptr{Int64},
+             Int64, Ptr{Float64}, Ptr{Float64}, Int64, Cint),
+            ctx(), k, dr, nw, pointer(network.segForce), size(network.segForce, 3),
+            isnothing(idx) ? C_NULL : pointer(ix), length(ix), pointer(nodeForce),
+            pointer(nodeVel), cols, flags))
+    end
+end
+
+function dlnMobility(dlnParams::DislocationParameters{T1} where {T1 <: mobBCC}, matParams,
+                     network::DislocationNetwork, idx = nothing)
+    dlnParams.parGPU || return DDD.dlnMobility(dlnParams, matParams, network, idx)
+    n = isnothing(idx) ? network.numNode[1] : length(idx)
+    nodeForce = zeros(3, n)
+    nodeVel = zeros(3, n)
+    mobility_call(dlnParams, matParams, network, idx, nodeForce, nodeVel, DDD_OUT_BY_POSITION)
+    return nodeForce, nodeVel
+end
+
+function dlnMobility!(dlnParams::DislocationParameters{T1} where {T1 <: mobBCC}, matParams,
+                      network::DislocationNetwork, idx = nothing)
+    dlnParams.parGPU || return DDD.dlnMobility!(dlnParams, matParams, network, idx)
+    mobility_call(dlnParams, matParams, network, idx, network.nodeForce, network.nodeVel, Cint(0))
+    return network
+end
+
 calc_σTilde(x0, dlnParams, matParams, network, idx = nothing) =
     calc_σTilde!(zeros(6, x0 isa Base.Iterators.ProductIterator ? prod(size(x0)) :
                          div(length(x0), 3)), x0, dlnParams, matParams, network, idx)

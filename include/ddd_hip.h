@@ -279,7 +279,9 @@ int ddd_mobility_bcc(ddd_ctx* ctx, const ddd_consts* k, const ddd_drag* drag,
  * ddd_net_seg_force computes the calcSegForce! total (mesh may be NULL)
  * into the context's device segForce; ddd_net_download copies it into a
  * 3 x 2 x outCols host array (columns 1..numSeg); ddd_net_segforce_device
- * returns the device address of that buffer (6 doubles per segment). */
+ * returns the device address of that buffer (6 doubles per segment).
+ * A one-shot force call on the same context discards the resident network
+ * and its dynamics state (they share the segment buffers). */
 int ddd_net_upload(ddd_ctx* ctx, const ddd_consts* k, const ddd_network* net,
                    const ddd_mesh* mesh);
 int ddd_net_seg_force(ddd_ctx* ctx, int withPK);
@@ -293,6 +295,43 @@ int ddd_net_download(ddd_ctx* ctx, double* out, int64_t outCols);
 int ddd_net_set_opts(ddd_ctx* ctx, const ddd_opts* opts);
 int ddd_net_stats(ddd_ctx* ctx, ddd_stats* stats);
 const double* ddd_net_segforce_device(ddd_ctx* ctx);
+
+/* ---- Device-resident dynamics: deriv! and integrate! -------------------
+ * IntegrationParameters{AdaptiveEulerTrapezoid} (src/Type/IntegrationType.jl:
+ * 28-39) and IntegrationTime (:51-55). */
+typedef struct ddd_integ {
+  double tmin, tmax, dtmin, dtmax, abstol, reltol, maxchange, exponent;
+  int64_t maxiter;
+} ddd_integ;
+typedef struct ddd_itime {
+  double dt, time;
+  int64_t step;
+} ddd_itime;
+
+/* After ddd_net_upload of the same network, keep its nodes on the device:
+ * coord (columns 1..numNode), links, connectivity, bVec, the node labels
+ * (nodeTypeDln, Int32 as Julia stores the enum) and the drag coefficients.
+ * Needs the reference's segment numbering segIdx[i, 1] == i
+ * (_getSegmentIdx!, src/Type/DislocationUtil.jl:283-298).  Surface-mobile
+ * nodes (srfMobDln) are refused: the reference's surface correction
+ * (IntegrationProcessing.jl:34-60) indexes label with a tuple, which Julia
+ * rejects, so no reference run with such nodes gets past it. */
+int ddd_net_upload_dynamics(ddd_ctx* ctx, const ddd_network* net, const int32_t* label,
+                            const ddd_drag* drag);
+/* deriv! (src/Processing/IntegrationProcessing.jl:13-62) on the device:
+ * calcSegForce! at the device coordinates (in the mode of ddd_net_set_opts;
+ * withPK adds the uploaded mesh's PK force), dlnMobility! for mobBCC, and
+ * zero velocity for fixed and external nodes. */
+int ddd_net_deriv(ddd_ctx* ctx, int withPK);
+/* integrate! for AdaptiveEulerTrapezoid (IntegrationProcessing.jl:81-168):
+ * moves the device coordinates and returns in *t the (dt, time, step) the
+ * reference returns.  Each trial runs on the device; only the two error
+ * maxima come back to the host. */
+int ddd_net_integrate(ddd_ctx* ctx, const ddd_integ* ip, ddd_itime* t, int withPK);
+/* Copy the device coord, nodeVel and nodeForce into 3 x cols host arrays
+ * (columns 1..numNode; any pointer may be NULL). */
+int ddd_net_download_nodes(ddd_ctx* ctx, double* coord, double* nodeVel, double* nodeForce,
+                           int64_t cols);
 
 #ifdef __cplusplus
 }

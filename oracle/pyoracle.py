@@ -377,3 +377,73 @@ def dlnMobility(dln, mat, net, idx=None, segForce=None, with_drag=False):
                            _i(ix) if ix is not None else None, n, _d(F), _d(V),
                            _d(D) if with_drag else None)
     return (F, V, D) if with_drag else (F, V)
+
+
+FIXED_LABELS = (2, 4, 5)   # intFixDln, srfFixDln, extDln (DislocationType.jl:24-32)
+
+
+def deriv_(dln, mat, mesh, fd, net):
+    """deriv! (IntegrationProcessing.jl:13-62): calcSegForce! (no PK when
+    mesh is None), dlnMobility! and zero velocity of fixed and external
+    nodes; fills net.segForce, nodeForce, nodeVel.  Surface-mobile nodes are
+    refused: the reference's correction for them (:34-60) indexes label with
+    a tuple, which Julia rejects."""
+    nn, ns = int(net.numNode[0]), int(net.numSeg[0])
+    lab = np.asarray(net.label[:nn])
+    if np.any(lab == 3):
+        raise ValueError("srfMobDln nodes: the reference's surface correction cannot run")
+    # the serial triangle: a fixed summation order, so that repeated runs
+    # take the same adaptive steps (the threaded one adds atomically)
+    sf = calcSegSegForce(dln, mat, net)
+    s1, s2 = calcSelfForce(dln, mat, net)
+    if mesh is not None:
+        pk = calcPKForce(mesh, fd, net)
+        s1 = s1 + 0.5 * pk
+        s2 = s2 + 0.5 * pk
+    sf[:, 0, :] += s1
+    sf[:, 1, :] += s2
+    net.segForce[:, :, :ns] = sf
+    F, V = dlnMobility(dln, mat, net)
+    V[:, np.isin(lab, FIXED_LABELS)] = 0.0
+    net.nodeForce[:, :nn] = F
+    net.nodeVel[:, :nn] = V
+
+
+def integrate_(ip, itime, dln, mat, mesh, fd, net):
+    """integrate! for AdaptiveEulerTrapezoid (IntegrationProcessing.jl:81-168):
+    moves net.coord and returns (dt, time, step) as the reference does (the
+    returned dt is the next step's proposal, and time advances by it)."""
+    import math
+    nn = int(net.numNode[0])
+    if nn == 0:
+        return itime.dt, itime.time, itime.step
+    deriv_(dln, mat, mesh, fd, net)
+    c0 = net.coord[:, :nn].copy()
+    v0 = net.nodeVel[:, :nn].copy()
+    dt = max(itime.dt, ip.dtmin)
+    dtOld, dtOldGood, convergent, counter = 0.0, False, False, 1
+    while not convergent:
+        net.coord[:, :nn] = c0 + v0 * dt
+        deriv_(dln, mat, mesh, fd, net)
+        distance = net.coord[:, :nn] - c0
+        maxDist = np.max(np.abs(distance))
+        err = distance - (net.nodeVel[:, :nn] + v0) / 2 * dt
+        maxErr = np.max(np.abs(err))
+        if dt <= ip.dtmin:
+            counter = ip.maxiter + 1
+        elif maxDist < ip.abstol and maxErr < ip.reltol:
+            dtOld = dt
+            factor = ip.maxchange * math.pow(
+                1 / (1 + (math.pow(ip.maxchange, ip.exponent) - 1) * (maxErr / ip.reltol)),
+                1 / ip.exponent)
+            dtOldGood = True
+            dt = min(dt * factor, ip.dtmax)
+            counter += 1
+        elif dtOldGood:
+            dt = dtOld
+            counter = ip.maxiter
+        else:
+            dt = dt / 2
+        if counter > ip.maxiter or dt == ip.dtmax:
+            convergent = True
+    return dt, itime.time + dt, itime.step + 1

@@ -81,7 +81,9 @@ def main(ref):
 
     for name in ("inputs/simParams/sampleDislocationParameters.json",
                  "inputs/simParams/sampleMaterialParameters.json",
-                 "test/testData/sampleFEMParameters.json"):
+                 "test/testData/sampleFEMParameters.json",
+                 "inputs/simParams/sampleIntegrationParameters.json",
+                 "test/testData/sampleIntegrationTime.json"):
         out["file:" + name] = {"provenance": name, "data": json.loads((ref / name).read_text())}
 
     dst = Path(__file__).with_name("reference_vectors.json")

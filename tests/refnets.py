@@ -38,6 +38,21 @@ def params():
     return dln, mat
 
 
+def integration():
+    """loadIntegrationParameters / loadIntegrationTime of the sample files
+    (Input.jl:196-213,352-360).  maxiter is read from the "exponent" key
+    (Input.jl:209), so the sample's maxiter is 20, not 10."""
+    from dddhip.types import IntegrationParameters, IntegrationTime
+    d = GOLD["file:inputs/simParams/sampleIntegrationParameters.json"]["data"]
+    t = GOLD["file:test/testData/sampleIntegrationTime.json"]["data"]
+    ip = IntegrationParameters(tmin=float(d["tmin"]), tmax=float(d["tmax"]),
+                               dtmin=float(d["dtmin"]), dtmax=float(d["dtmax"]),
+                               abstol=float(d["abstol"]), reltol=float(d["reltol"]),
+                               maxchange=float(d["maxchange"]), exponent=float(d["exponent"]),
+                               maxiter=int(d["exponent"]))
+    return ip, IntegrationTime(dt=float(t["dt"]), time=float(t["time"]), step=int(t["step"]))
+
+
 def mesh():
     """buildMesh(matParams, femParams) for sampleFEMParameters.json: 2000^3, 5x5x5."""
     f = GOLD["file:test/testData/sampleFEMParameters.json"]["data"]
