@@ -105,11 +105,79 @@ def hybrid_paths(args, dln, mat, net, res, stream, ref, fp64_ms, log):
                 f"{nrms:.2e} max {nmx:.2e}; {near} near pairs, {near64} FP64")
             out["configs"].append(row)
             res.set_mode("fp64")
+        if args.dynamics and "dynamics" not in out:
+            out["dynamics"] = dict(dynamics_leg(args, dln, mat, net, res, stream, ffs[rns[0]], log),
+                                   hybrid_grid={"G": Gc, "rn": rns[0]})
         for f in ffs.values():
             f.close()
         mlp.close()
     ok = [c for c in out["configs"] if c["node_rel_err_rms"] <= 1e-3]
     out["best_within_1e-3"] = min(ok, key=lambda c: c["ms"]) if ok else None
+    return out
+
+
+# Compulsory HBM bytes of the mobility kernel per node of a closed-loop
+# network (numSeg = numNode): connectivity count + 2 (link, column) pairs
+# 40 B, own coordinates 24 B, label 4 B, node force and velocity written
+# 48 B; per link: its two node numbers 16 B, Burgers vector 24 B, segForce
+# 48 B.
+MOBILITY_BYTES_PER_NODE = 40 + 24 + 4 + 48 + 16 + 24 + 48
+
+
+def dynamics_leg(args, dln, mat, net, res, stream, ff, log):
+    """deriv! and integrate! with the network resident in HBM, in plain FP64
+    and in the hybrid mode: the mobility kernel on its own (node forces and
+    velocities from segForce), one deriv! (force evaluation + mobility) and
+    one integrate! step (adaptive Euler-trapezoid with the sample parameters:
+    maxiter 20, so up to 21 trial evaluations after the first)."""
+    import numpy as np
+    import torch
+    import refnets as R
+
+    ip, t0 = R.integration()
+    n = int(net.numNode[0])
+    out = {"drag": dict(dln.dragCoeffs), "integration": {"dt0": t0.dt, "dtmin": ip.dtmin,
+           "abstol": ip.abstol, "reltol": ip.reltol, "maxiter": ip.maxiter},
+           "nodes": n}
+
+    def timed(fn, k):
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(k)]
+        for e in ev:
+            e[0].record(stream)
+            fn()
+            e[1].record(stream)
+        stream.synchronize()
+        return float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+
+    for mode in ("fp64", "hybrid"):
+        res.reupload()
+        res.set_mode(mode, farfield=ff if mode == "hybrid" else None)
+        res.upload_dynamics(dln, net)
+        res.deriv()
+        stream.synchronize()
+        mob = timed(lambda: res.stage(3), 20)
+        der = timed(res.deriv, args.steps)
+        t = time.perf_counter()
+        ti = res.integrate(ip, t0)
+        stream.synchronize()
+        step_ms = (time.perf_counter() - t) * 1e3
+        _, v, _ = res.download_nodes()
+        assert np.all(np.isfinite(v)), "non-finite velocities"
+        out[mode] = {"mobility_kernel_ms": round(mob, 4), "deriv_ms": round(der, 3),
+                     "integrate_step_ms": round(step_ms, 1),
+                     "evaluations_per_step_est": round(step_ms / der, 1),
+                     "dt_next": ti.dt, "time": ti.time}
+        log(f"dynamics {mode}: mobility {mob * 1e3:.1f} us, deriv {der:.2f} ms, integrate step "
+            f"{step_ms:.0f} ms (~{step_ms / der:.0f} evaluations), next dt {ti.dt:.4g}")
+    res.reupload()
+    res.set_mode("fp64")
+    mob = out["fp64"]["mobility_kernel_ms"]
+    gbs = MOBILITY_BYTES_PER_NODE * n / (mob * 1e-3) / 1e9
+    out["mobility_roofline"] = {"bound": "hbm", "bytes_per_node": MOBILITY_BYTES_PER_NODE,
+                                "achieved_GB_s": round(gbs, 1), "peak_GB_s": 8000.0,
+                                "frac": round(gbs / 8000.0, 4)}
+    out["hybrid_speedup_per_step"] = round(out["fp64"]["integrate_step_ms"] /
+                                           out["hybrid"]["integrate_step_ms"], 2)
     return out
 
 
@@ -273,6 +341,8 @@ def main():
                     help="calc_σTilde leg: network of this many loops (seed 0xB0) at 6 x g^2 "
                          "face points; 0 to skip")
     ap.add_argument("--sigma-grid", type=int, default=128)
+    ap.add_argument("--no-dynamics", dest="dynamics", action="store_false",
+                    help="skip the deriv!/integrate! leg (run with the first hybrid grid)")
     ap.add_argument("--clustered", action="store_true",
                     help="also run the clustered stress test: 262,144 segments, 70%% in 8 "
                          "pile-ups; plain FP64 vs hybrid (G=32, rn=4), error on 4,096 sampled "
@@ -291,7 +361,10 @@ def main():
     from dddhip.generate import random_loop_network
     from dddhip.types import DislocationParameters, MaterialParameters
 
-    dln = DislocationParameters(coreRad=90.78, coreEnergy=1.0)
+    # drag coefficients of the sample parameter file (edge 1, screw 2,
+    # climb 1e10, line 0), used by the mobility of the dynamics leg
+    dln = DislocationParameters(coreRad=90.78, coreEnergy=1.0,
+                                dragCoeffs={"edge": 1.0, "screw": 2.0, "climb": 1e10, "line": 0.0})
     mat = MaterialParameters(μ=1.0, ν=0.28)
     t0 = time.time()
     net = random_loop_network(args.loops, seed=args.seed)

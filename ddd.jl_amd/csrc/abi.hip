@@ -888,8 +888,14 @@ void net_stage(ddd_ctx* c, int stage, int withPK) {
     if (is_hybrid(&c->opts)) hybrid_apply(c, n, dev, c->opts);
     else if (n >= 2)
       launch_segseg_reduce(c->rowPart.as<double>(), c->colPart.as<double>(), (int)n, dev, c->st);
+  } else if (stage == 3) {
+    if (c->dynNodes < 0) bad("no dynamics state (ddd_net_upload_dynamics)");
+    launch_mobility_bcc(c->dConn.as<int64_t>(), c->dynConnLd, c->dLinks.as<int64_t>(),
+                        c->dCoord.as<double>(), c->dBvec.as<double>(), dev, nullptr, c->dynNodes,
+                        c->dynDrag, c->resK.sigmaPN, c->dLabel.as<int32_t>(),
+                        c->dNodeF.as<double>(), c->dVel.as<double>(), c->st);
   } else {
-    bad("stage must be 0, 1 or 2");
+    bad("stage must be 0, 1, 2 or 3");
   }
   HIPCHK(hipGetLastError());
 }
@@ -1545,12 +1551,7 @@ void dyn_deriv(ddd_ctx* c, int withPK) {
   launch_gather_coords(c->seg.as<double>(), c->resLd, ns, c->dSegNodes.as<int64_t>(),
                        c->dCoord.as<double>(), c->st);
   launch_prep_segments(c->seg.as<double>(), c->resLd, ns, c->st);
-  for (int s = 0; s < 3; ++s) net_stage(c, s, withPK);
-  launch_mobility_bcc(c->dConn.as<int64_t>(), c->dynConnLd, c->dLinks.as<int64_t>(),
-                      c->dCoord.as<double>(), c->dBvec.as<double>(), c->force.as<double>(),
-                      nullptr, c->dynNodes, c->dynDrag, c->resK.sigmaPN, c->dLabel.as<int32_t>(),
-                      c->dNodeF.as<double>(), c->dVel.as<double>(), c->st);
-  HIPCHK(hipGetLastError());
+  for (int s = 0; s < 4; ++s) net_stage(c, s, withPK);
 }
 
 }  // namespace

@@ -418,6 +418,12 @@ class ResidentNetwork:
     def seg_force(self, with_pk=False):
         _call(self.ctx, "ddd_net_seg_force", 1 if with_pk else 0)
 
+    def reupload(self):
+        """Upload the network again from the host arrays it was built from
+        (discards device coordinates moved by integrate and the dynamics
+        state)."""
+        _call(self.ctx, "ddd_net_upload", self.k, self.nv.s, self.mv.s if self.mv else None)
+
     def set_mode(self, mode="fp64", farDist=0.0, sinMin=0.0, farfield=None):
         m = {"fp64": L.DDD_MODE_FP64, "mixed": L.DDD_MODE_MIXED, "hybrid": L.DDD_MODE_HYBRID}[mode]
         _call(self.ctx, "ddd_net_set_opts",

@@ -287,7 +287,9 @@ int ddd_net_upload(ddd_ctx* ctx, const ddd_consts* k, const ddd_network* net,
 int ddd_net_seg_force(ddd_ctx* ctx, int withPK);
 /* One stage of ddd_net_seg_force, for per-kernel timing on the context's
  * stream: 0 = zero segForce + PK/2 + self force, 1 = seg-seg pair tiles,
- * 2 = reduction of the tile partials into segForce. */
+ * 2 = reduction of the tile partials into segForce; 3 = node forces and
+ * velocities from segForce (dlnMobility!, fixed nodes at rest; needs
+ * ddd_net_upload_dynamics). */
 int ddd_net_stage(ddd_ctx* ctx, int stage, int withPK);
 int ddd_net_download(ddd_ctx* ctx, double* out, int64_t outCols);
 /* Select the evaluation mode of the resident path (NULL: plain FP64) and
