@@ -335,4 +335,79 @@ function resident_seg_force!(network::DislocationNetwork)
     return nothing
 end
 
+# deriv! / integrate! (src/Processing/IntegrationProcessing.jl:13-168) with the
+# network on the device: coordinates, velocities and forces stay in HBM across
+# the adaptive step's trial evaluations; results are copied back at the end.
+struct Integ
+    tmin::Float64
+    tmax::Float64
+    dtmin::Float64
+    dtmax::Float64
+    abstol::Float64
+    reltol::Float64
+    maxchange::Float64
+    exponent::Float64
+    maxiter::Int64
+end
+struct ITime
+    dt::Float64
+    time::Float64
+    step::Int64
+end
+
+function dynamics_upload(dlnParams, matParams, mesh, forceDisplacement, network)
+    k = Ref(consts(dlnParams, matParams))
+    dr = Ref(drag(dlnParams))
+    label = Int32.(network.label)  # nodeTypeDln is an Int32 enum
+    uHat = Vector{Float64}(forceDisplacement.uHat)
+    C = Matrix(mesh.C)
+    GC.@preserve network label uHat C mesh begin
+        nw = Ref(netstruct(network))
+        ms = Ref(Mesh(pointer(C), pointer(mesh.coord), size(mesh.coord, 2),
+                      pointer(mesh.connectivity), size(mesh.connectivity, 2),
+                      mesh.mx, mesh.my, mesh.mz, mesh.w, mesh.h, mesh.d, pointer(uHat)))
+        check(ccall((:ddd_net_upload, libddd), Cint,
+            (Ptr{Cvoid}, Ref{Consts}, Ref{Network}, Ref{Mesh}), ctx(), k, nw, ms))
+        check(ccall((:ddd_net_upload_dynamics, libddd), Cint,
+            (Ptr{Cvoid}, Ref{Network}, Ptr{Int32}, Ref{Drag}), ctx(), nw, pointer(label), dr))
+    end
+end
+
+function dynamics_download!(network)
+    n = network.numNode[1]
+    GC.@preserve network begin
+        check(ccall((:ddd_net_download_nodes, libddd), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64), ctx(),
+            pointer(network.coord), pointer(network.nodeVel), pointer(network.nodeForce), n))
+        check(ccall((:ddd_net_download, libddd), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64), ctx(),
+            pointer(network.segForce), size(network.segForce, 3)))
+    end
+end
+
+function deriv!(dlnParams::DislocationParameters{T1} where {T1 <: mobBCC}, matParams, mesh,
+                forceDisplacement, network::DislocationNetwork)
+    dlnParams.parGPU || return DDD.deriv!(dlnParams, matParams, mesh, forceDisplacement, network)
+    dynamics_upload(dlnParams, matParams, mesh, forceDisplacement, network)
+    check(ccall((:ddd_net_deriv, libddd), Cint, (Ptr{Cvoid}, Cint), ctx(), Cint(1)))
+    dynamics_download!(network)
+    return nothing
+end
+
+function integrate!(intParams::IntegrationParameters{T1} where {T1 <: AdaptiveEulerTrapezoid},
+                    intVars::IntegrationTime,
+                    dlnParams::DislocationParameters{T2} where {T2 <: mobBCC}, matParams, mesh,
+                    forceDisplacement, network::DislocationNetwork)
+    dlnParams.parGPU || return DDD.integrate!(intParams, intVars, dlnParams, matParams, mesh,
+                                              forceDisplacement, network)
+    dynamics_upload(dlnParams, matParams, mesh, forceDisplacement, network)
+    ip = Ref(Integ(intParams.tmin, intParams.tmax, intParams.dtmin, intParams.dtmax,
+                   intParams.abstol, intParams.reltol, intParams.maxchange, intParams.exponent,
+                   intParams.maxiter))
+    t = Ref(ITime(intVars.dt, intVars.time, intVars.step))
+    check(ccall((:ddd_net_integrate, libddd), Cint, (Ptr{Cvoid}, Ref{Integ}, Ref{ITime}, Cint),
+                ctx(), ip, t, Cint(1)))
+    dynamics_download!(network)
+    return IntegrationTime(t[].dt, t[].time, t[].step)
+end
+
 end # module
