@@ -134,6 +134,8 @@ struct ddd_ctx {
   DevBuf sseg, pts, sigPart, sigma;
   // gather and mobility
   DevBuf conn, links, coord, nodeF, bvec, nodeV, nodeIdx;
+  // displacement triangles, points, chunk partials, result
+  DevBuf uTri, uPts, uPart, uOut;
   std::vector<double> h;  // host staging
   DevBuf counter;  // FP64-pair counter of the mixed path
   // near-parallel pairs of the all-pairs tiles (count, offsets, list, forces)
@@ -849,6 +851,223 @@ int ddd_mobility_bcc(ddd_ctx* c, const ddd_consts* k, const ddd_drag* drag, cons
         nodeVel[3 * col + e] = c->h[3 * n + 3 * q + e];
       }
     }
+  });
+}
+
+namespace {
+
+// Host geometry of calc_uTilde's external segments (DislocationFEM.jl:437-528):
+// the point where each external end was projected out of the cuboid.
+struct Box {
+  double nrm[6][3], mid[6][3], lo[3], hi[3], centre[3];
+
+  // convex-hull membership of a point computed on a face plane, with a
+  // tolerance of 1e-9 of the box extent per axis
+  bool contains(const double x[3]) const {
+    for (int d = 0; d < 3; ++d) {
+      const double tol = 1e-9 * (hi[d] - lo[d]);
+      if (!(x[d] >= lo[d] - tol && x[d] <= hi[d] + tol)) return false;
+    }
+    return true;
+  }
+  static double dot(const double a[3], const double b[3]) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+  }
+  // findIntersectVolume (DislocationTopologySpecial.jl:7-38) through
+  // linePlaneIntersect (Misc.jl:223-235): nearest in-volume intersection of
+  // the line p + s l with a face plane; squared distance, Inf if none.
+  double nearest(const double l[3], const double p[3], double out[3]) const {
+    double best = std::numeric_limits<double>::infinity();
+    out[0] = out[1] = out[2] = 0.0;
+    for (int f = 0; f < 6; ++f) {
+      const double den = dot(l, nrm[f]);
+      if (den == 0.0) continue;
+      const double diff[3] = {mid[f][0] - p[0], mid[f][1] - p[1], mid[f][2] - p[2]};
+      const double s = dot(diff, nrm[f]) / den;
+      const double x[3] = {p[0] + s * l[0], p[1] + s * l[1], p[2] + s * l[2]};
+      if (std::isinf(x[0]) || std::isinf(x[1]) || std::isinf(x[2]) || !contains(x)) continue;
+      const double r[3] = {x[0] - p[0], x[1] - p[1], x[2] - p[2]};
+      const double d2 = std::min((r[0] * r[0] + r[1] * r[1]) + r[2] * r[2], best);
+      if (d2 < best) {
+        best = d2;
+        out[0] = x[0];
+        out[1] = x[1];
+        out[2] = x[2];
+      }
+    }
+    return best;
+  }
+  // Projection of an external end: the nearest intersection along the
+  // normals of faces 1, 3 and 5; failing that, steps over the face planes
+  // towards the centroid (the reference adds the face midpoint in the plane
+  // distance and shares the running minimum dc between both ends).
+  void project(const double p[3], double& dc, double out[3]) const {
+    double best = std::numeric_limits<double>::infinity();
+    for (int j = 0; j < 6; j += 2) {
+      double x[3];
+      const double d = nearest(nrm[j], p, x);
+      if (d < best) {
+        best = d;
+        out[0] = x[0];
+        out[1] = x[1];
+        out[2] = x[2];
+      }
+    }
+    if (!std::isinf(best)) return;
+    double t[3] = {p[0], p[1], p[2]};
+    for (int f = 0; f < 6; ++f) {
+      const double tm[3] = {t[0] + mid[f][0], t[1] + mid[f][1], t[2] + mid[f][2]};
+      const double pd = dot(tm, nrm[f]);
+      const double step = std::sqrt(pd * pd);
+      const double x[3] = {t[0] - nrm[f][0] * step, t[1] - nrm[f][1] * step, t[2] - nrm[f][2] * step};
+      const double r[3] = {x[0] - centre[0], x[1] - centre[1], x[2] - centre[2]};
+      const double d = std::sqrt(dot(r, r));
+      if (d <= dc) {
+        dc = d;
+        t[0] = x[0];
+        t[1] = x[1];
+        t[2] = x[2];
+      }
+    }
+    out[0] = t[0];
+    out[1] = t[1];
+    out[2] = t[2];
+  }
+};
+
+// Triangles of calc_uTilde in segment order: 13 x ld SoA (A, B, C, b, kind).
+int64_t u_tilde_triangles(const ddd_network* net, const int32_t* label, const double* vertices,
+                          const double* faceNorm, const double* faceMidPt,
+                          std::vector<double>& tri, int64_t& ld) {
+  Box bx;
+  double cs[3] = {0, 0, 0};
+  for (int d = 0; d < 3; ++d) {
+    bx.lo[d] = std::numeric_limits<double>::infinity();
+    bx.hi[d] = -std::numeric_limits<double>::infinity();
+  }
+  for (int v = 0; v < 8; ++v)
+    for (int d = 0; d < 3; ++d) {
+      cs[d] += vertices[3 * v + d];
+      bx.lo[d] = std::min(bx.lo[d], vertices[3 * v + d]);
+      bx.hi[d] = std::max(bx.hi[d], vertices[3 * v + d]);
+    }
+  for (int d = 0; d < 3; ++d) bx.centre[d] = cs[d] / 8.0;
+  for (int f = 0; f < 6; ++f)
+    for (int d = 0; d < 3; ++d) {
+      bx.nrm[f][d] = faceNorm[3 * f + d];
+      bx.mid[f][d] = faceMidPt[3 * f + d];
+    }
+  struct T {
+    double a[3], b[3], c[3], bv[3], kind;
+  };
+  std::vector<T> out;
+  out.reserve((size_t)net->numSeg);
+  std::vector<int64_t> surfCon((size_t)net->coordCols + 1, 0);
+  const int32_t srfMob = 3, srfFix = 4, ext = 5;
+  auto xyz = [&](int64_t node, double r[3]) {
+    for (int d = 0; d < 3; ++d) r[d] = net->coord[3 * (node - 1) + d];
+  };
+  for (int64_t i = 0; i < net->numSeg; ++i) {
+    const int64_t n1 = net->links[2 * i], n2 = net->links[2 * i + 1];
+    const int32_t l1 = label[n1 - 1], l2 = label[n2 - 1];
+    if ((l1 == srfMob || l1 == srfFix) && l2 == ext) { surfCon[n2] = n1; continue; }
+    if ((l2 == srfMob || l2 == srfFix) && l1 == ext) { surfCon[n1] = n2; continue; }
+    T t{};
+    xyz(n1, t.a);
+    xyz(n2, t.b);
+    for (int d = 0; d < 3; ++d) t.bv[d] = net->bVec[3 * i + d];
+    if (l1 != ext && l2 != ext) {
+      for (int d = 0; d < 3; ++d) t.c[d] = bx.centre[d];
+      t.kind = 1.0;
+      out.push_back(t);
+      continue;
+    }
+    double dc = std::numeric_limits<double>::infinity();
+    double Ap[3], Bp[3];
+    bx.project(t.a, dc, Ap);
+    bx.project(t.b, dc, Bp);
+    if (surfCon[n1] != 0) xyz(surfCon[n1], Ap);
+    if (surfCon[n2] != 0) xyz(surfCon[n2], Bp);
+    double Cp[3];
+    for (int d = 0; d < 3; ++d) Cp[d] = (t.a[d] + Bp[d]) / 2.0;
+    const double* corners[5][3] = {{Ap, Bp, bx.centre}, {Ap, t.a, Cp}, {t.a, t.b, Cp},
+                                   {t.b, Bp, Cp}, {Bp, Ap, Cp}};
+    for (auto& q : corners) {
+      T s{};
+      for (int d = 0; d < 3; ++d) {
+        s.a[d] = q[0][d];
+        s.b[d] = q[1][d];
+        s.c[d] = q[2][d];
+        s.bv[d] = t.bv[d];
+      }
+      s.kind = 0.0;
+      out.push_back(s);
+    }
+  }
+  const int64_t n = (int64_t)out.size();
+  ld = std::max<int64_t>(n, 1);
+  tri.assign((size_t)(13 * ld), 0.0);
+  for (int64_t q = 0; q < n; ++q) {
+    const T& t = out[(size_t)q];
+    for (int d = 0; d < 3; ++d) {
+      tri[(0 + d) * ld + q] = t.a[d];
+      tri[(3 + d) * ld + q] = t.b[d];
+      tri[(6 + d) * ld + q] = t.c[d];
+      tri[(9 + d) * ld + q] = t.bv[d];
+    }
+    tri[12 * ld + q] = t.kind;
+  }
+  return n;
+}
+
+}  // namespace
+
+int ddd_u_tilde(ddd_ctx* c, double omNuInv8pi, double om2NuOmNuInv8pi, const ddd_network* net,
+                const int32_t* label, const double* vertices, const double* faceNorm,
+                const double* faceMidPt, const double* points, int64_t P, double* u, int flags) {
+  return guard(c, "ddd_u_tilde", [&] {
+    if (!net || !vertices || !faceNorm || !faceMidPt) bad("NULL argument");
+    if (!label && net->numSeg > 0) bad("label is NULL");
+    if (P < 0) bad("P < 0");
+    if (P > 0 && (!points || !u)) bad("points and u required");
+    if (!net->links || !net->coord || !net->bVec) bad("links, coord and bVec required");
+    if (net->numSeg < 0 || net->numSeg > net->linksCols || net->numSeg > net->bVecCols)
+      bad("numSeg outside links or bVec");
+    if (P > ((int64_t)1 << 31)) bad("P too large");
+    for (int64_t i = 0; i < net->numSeg; ++i)
+      for (int e = 0; e < 2; ++e) {
+        const int64_t nd = net->links[2 * i + e];
+        if (nd < 1 || nd > net->coordCols) bad("links node outside coord");
+      }
+    std::vector<double> tri;
+    int64_t ld = 0;
+    const int64_t ntri = u_tilde_triangles(net, label, vertices, faceNorm, faceMidPt, tri, ld);
+    if (ntri > ((int64_t)1 << 30)) bad("too many triangles");
+    if (P == 0) return;
+    const bool acc = (flags & DDD_ACCUMULATE) != 0;
+    c->uOut.ensure(sizeof(double) * 3 * P);
+    if (acc) {
+      HIPCHK(hipMemcpyAsync(c->uOut.p, u, sizeof(double) * 3 * P, hipMemcpyHostToDevice, c->st));
+    }
+    if (ntri == 0) {
+      if (!acc) HIPCHK(hipMemsetAsync(c->uOut.p, 0, sizeof(double) * 3 * P, c->st));
+    } else {
+      const int nChunk = u_tilde_chunks_for(P, (int)ntri);
+      c->uTri.ensure(sizeof(double) * tri.size());
+      c->uPts.ensure(sizeof(double) * 3 * P);
+      c->uPart.ensure(sizeof(double) * 3 * P * nChunk);
+      HIPCHK(hipMemcpyAsync(c->uTri.p, tri.data(), sizeof(double) * tri.size(),
+                            hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipMemcpyAsync(c->uPts.p, points, sizeof(double) * 3 * P, hipMemcpyHostToDevice,
+                            c->st));
+      launch_u_tilde(c->uTri.as<double>(), ld, (int)ntri, c->uPts.as<double>(), P, omNuInv8pi,
+                     om2NuOmNuInv8pi, c->uPart.as<double>(), nChunk, c->st);
+      launch_chunk_sum(c->uPart.as<double>(), nChunk, 3 * P, c->uOut.as<double>(), acc ? 1 : 0,
+                       c->st);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemcpyAsync(u, c->uOut.p, sizeof(double) * 3 * P, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));  // tri (host) is released on return
   });
 }
 

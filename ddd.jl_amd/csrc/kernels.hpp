@@ -120,6 +120,15 @@ void launch_gather_coords(double* seg, int64_t ld, int64_t n, const int64_t* nod
 void launch_euler(double* coord, const double* init, const double* vel, double dt, int64_t m,
                   hipStream_t st);
 int trapezoid_parts();
+// Displacement of triangles at points (displacement.hip).  tri: 13 x tld
+// SoA (A, B, C, b, kind: 1 = full triangle, 0 = solid-angle term only);
+// partial: nChunk x 3P; launch_chunk_sum adds the chunks into out (3P).
+int u_tilde_chunks_for(int64_t P, int ntri);
+void launch_u_tilde(const double* tri, int64_t tld, int ntri, const double* pts, int64_t P,
+                    double omNuInv8pi, double om2NuOmNuInv8pi, double* partial, int nChunk,
+                    hipStream_t st);
+void launch_chunk_sum(const double* partial, int nChunk, int64_t count, double* out,
+                      int accumulate, hipStream_t st);
 void launch_trapezoid_max(const double* coord, const double* init, const double* vel,
                           const double* vel0, double dt, int64_t m, unsigned long long* part,
                           double* out, hipStream_t st);

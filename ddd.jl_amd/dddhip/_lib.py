@@ -125,6 +125,9 @@ def lib():
                 "ddd_net_integrate": (C.c_int, [P, C.POINTER(ddd_integ), C.POINTER(ddd_itime),
                                                 C.c_int]),
                 "ddd_net_download_nodes": (C.c_int, [P, dp, dp, dp, i64]),
+                "ddd_u_tilde": (C.c_int, [P, C.c_double, C.c_double, C.POINTER(ddd_network),
+                                          C.POINTER(C.c_int32), dp, dp, dp, dp, i64, dp,
+                                          C.c_int]),
                 "ddd_mobility_bcc": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_drag),
                                                C.POINTER(ddd_network), dp, i64, ip, i64, dp, dp,
                                                i64, C.c_int]),

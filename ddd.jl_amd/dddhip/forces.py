@@ -206,6 +206,44 @@ def calcNodeForce_(dlnParams, matParams, network, ctx=None):
     return nf
 
 
+def u_tilde_points(matParams, network, mesh, points, ctx=None):
+    """Displacement of the network's dislocations at the points (3 x P):
+    the Barnett-triangle sum of _calc_uTilde! (DislocationFEM.jl:381-600)
+    before it is scattered into dofs; column i belongs to point i."""
+    ctx = _ctx(ctx)
+    x = np.asfortranarray(np.asarray(points, dtype=np.float64).reshape(3, -1, order="F"))
+    P = x.shape[1]
+    out = np.zeros((3, P), order="F")
+    nv = L.NetView(network)
+    lab = np.ascontiguousarray(network.label, dtype=np.int32)
+    if len(lab) < nv.coord.shape[1]:
+        raise ValueError("network.label needs one entry per coord column")
+    keep = [np.asfortranarray(a, dtype=np.float64)
+            for a in (mesh.vertices, mesh.faceNorm, mesh.faceMidPt)]
+    _call(ctx, "ddd_u_tilde", float(matParams.omνInv8π), float(matParams.om2νomνInv8π), nv.s,
+          lab.ctypes.data_as(_c.POINTER(_c.c_int32)), *(L.ptr(a, "d") for a in keep),
+          L.ptr(x, "d"), P, L.ptr(out, "d"), 0)
+    return out
+
+
+def calc_uTilde(mesh, boundaries, matParams, network, ctx=None):
+    """calc_uTilde (DislocationFEM.jl:369-379): displacements at the nodes
+    boundaries.uGammaDln, interleaved (x, y, z of each node in turn)."""
+    pts = mesh.coord[:, np.asarray(boundaries.uGammaDln) - 1]
+    return u_tilde_points(matParams, network, mesh, pts, ctx).ravel(order="F")
+
+
+def calc_uTilde_(forceDisplacement, mesh, boundaries, matParams, network, ctx=None):
+    """calc_uTilde! (DislocationFEM.jl:355-367): zero forceDisplacement.uTilde
+    and add value k of calc_uTilde into dof boundaries.uDofsDln[k] (the
+    reference's mapping; its dof list runs x, then y, then z)."""
+    v = calc_uTilde(mesh, boundaries, matParams, network, ctx)
+    u = forceDisplacement.uTilde
+    u[:] = 0.0
+    np.add.at(u, np.asarray(boundaries.uDofsDln) - 1, v)
+    return u
+
+
 def _mobility(dlnParams, matParams, network, ix, nf, nv_out, cols, flags, ctx):
     if dlnParams.mobility != "mobBCC":
         raise NotImplementedError(f"mobility law {dlnParams.mobility!r}: only mobBCC exists")

@@ -41,6 +41,8 @@ class MaterialParameters:
         self.μ8π = self.μ4π / 2
         self.μ4πν = self.μ4π * self.omνInv
         self.νμ4πν = ν * self.μ4πν
+        self.omνInv8π = self.omνInv / (8 * math.pi)
+        self.om2νomνInv8π = (1 - 2 * ν) * self.omνInv8π
 
 
 @dataclass
@@ -396,12 +398,85 @@ class RegularCuboidMesh:
                     conn[:, e] = (c1, c1 + 1, c4 + 1, c4, c5, c5 + 1, c8 + 1, c8)
         self.coord = coord
         self.connectivity = conn
+        # Bounding polytope (FEMConstructor.jl:164-236): the 8 corners, the 6
+        # faces by corner number, their midpoints and outward normals.
+        dx, dy, dz = float(self.dx), float(self.dy), float(self.dz)
+        self.vertices = np.asfortranarray(np.array(
+            [[0, 0, 0], [dx, 0, 0], [dx, dy, 0], [0, dy, 0],
+             [0, 0, dz], [dx, 0, dz], [dx, dy, dz], [0, dy, dz]], dtype=np.float64).T)
+        self.faces = np.asfortranarray(np.array(
+            [[1, 2, 6, 5], [2, 3, 7, 6], [3, 4, 8, 7], [4, 1, 5, 8], [4, 3, 2, 1], [5, 6, 7, 8]],
+            dtype=np.int64).T)
+        self.faceMidPt = np.asfortranarray(
+            self.vertices[:, self.faces - 1].sum(axis=1) / 4)
+        self.faceNorm = np.asfortranarray(np.array(
+            [[0, -1, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, 0, -1], [0, 0, 1]],
+            dtype=np.float64).T)
+        self.surfNode = _surface_nodes(mx, my, mz)
+
+
+def _surface_nodes(mx, my, mz):
+    """surfNode of buildMesh (FEMConstructor.jl:274-305,499-545): node
+    numbers of the 8 corners, the edge nodes along each of the 12 edges and
+    the interior nodes of each of the 6 faces."""
+    mx1, my1, mz1 = mx + 1, my + 1, mz + 1
+    s = {"x0y0z0": [1], "x1y0z0": [mx1], "x1y1z0": [mx1 * my1], "x0y1z0": [mx1 * my + 1],
+         "x0y0z1": [1 + mx1 * my1 * mz], "x1y0z1": [mx1 + mx1 * my1 * mz],
+         "x1y1z1": [mx1 * my1 * mz1], "x0y1z1": [mx1 * my + 1 + mx1 * my1 * mz]}
+    top = mx1 * my1 * mz
+    s["x_y0z0"] = [i + 1 for i in range(1, mx)]
+    s["x_y1z0"] = [i + 1 + mx1 * my for i in range(1, mx)]
+    s["x_y0z1"] = [i + 1 + top for i in range(1, mx)]
+    s["x_y1z1"] = [i + 1 + top + mx1 * my for i in range(1, mx)]
+    s["y_x0z0"] = [i * mx1 + 1 for i in range(1, my)]
+    s["y_x1z0"] = [i * mx1 + mx1 for i in range(1, my)]
+    s["y_x0z1"] = [i * mx1 + 1 + top for i in range(1, my)]
+    s["y_x1z1"] = [i * mx1 + mx1 + top for i in range(1, my)]
+    s["z_x0y0"] = [i * mx1 * my1 + 1 for i in range(1, mz)]
+    s["z_x1y0"] = [i * mx1 * my1 + mx1 for i in range(1, mz)]
+    s["z_x0y1"] = [i * mx1 * my1 + 1 + mx1 * my for i in range(1, mz)]
+    s["z_x1y1"] = [i * mx1 * my1 + mx1 + mx1 * my for i in range(1, mz)]
+    s["xy_z0"] = [j * mx1 + 1 + i for j in range(1, my) for i in range(1, mx)]
+    s["xy_z1"] = [j * mx1 + 1 + i + top for j in range(1, my) for i in range(1, mx)]
+    s["xz_y0"] = [j * mx1 * my1 + 1 + i for j in range(1, mz) for i in range(1, mx)]
+    s["xz_y1"] = [j * mx1 * my1 + 1 + i + mx1 * my for j in range(1, mz) for i in range(1, mx)]
+    s["yz_x0"] = [j * mx1 * my1 + 1 + i * mx1 for j in range(1, mz) for i in range(1, my)]
+    s["yz_x1"] = [j * mx1 * my1 + mx1 + i * mx1 for j in range(1, mz) for i in range(1, my)]
+    return {k: np.asarray(v, dtype=np.int64) for k, v in s.items()}
+
+
+@dataclass
+class Boundaries:
+    """The node and dof lists of Boundaries (FEMType.jl:330-370) that
+    calc_uTilde reads: uGammaDln = [uGamma.node; mGamma.node] and
+    uDofsDln = [3 uGammaDln - 2; 3 uGammaDln - 1; 3 uGammaDln]
+    (FEMConstructor.jl:968-971: all x dofs, then all y, then all z)."""
+
+    uGammaDln: np.ndarray
+    uDofsDln: np.ndarray = field(init=False)
+
+    def __post_init__(self):
+        u = np.asarray(self.uGammaDln, dtype=np.int64)
+        self.uGammaDln = u
+        self.uDofsDln = np.concatenate([3 * u - 2, 3 * u - 1, 3 * u])
+
+    @staticmethod
+    def cantilever(mesh: "RegularCuboidMesh") -> "Boundaries":
+        """Boundaries for CantileverLoad (FEMConstructor.jl:754-800): the x = 0
+        face with its corners and edges (uGamma), and the loaded corners and
+        edge at x = dx, z = dz (mGamma)."""
+        sn = mesh.surfNode
+        uIndex = ["x0y0z0", "x0y1z0", "x0y0z1", "x0y1z1", "y_x0z0", "y_x0z1", "z_x0y0",
+                  "z_x0y1", "yz_x0"]
+        mIndex = ["x1y0z1", "x1y1z1", "y_x1z1"]
+        return Boundaries(np.concatenate([sn[k] for k in uIndex + mIndex]))
 
 
 @dataclass
 class ForceDisplacement:
-    """ForceDisplacement (FEMType.jl:294-301): only ``uHat`` is read by the force
-    path.  The reference stores it as a SparseVector; here it is dense
-    (3 numNode), which is what the C-ABI takes."""
+    """ForceDisplacement (FEMType.jl:294-301): ``uHat`` (read by the force
+    path) and ``uTilde`` (written by calc_uTilde!).  The reference stores
+    them as SparseVectors; here they are dense (3 numNode)."""
 
     uHat: np.ndarray
+    uTilde: Optional[np.ndarray] = None

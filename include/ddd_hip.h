@@ -272,6 +272,23 @@ int ddd_mobility_bcc(ddd_ctx* ctx, const ddd_consts* k, const ddd_drag* drag,
                      const int64_t* idx, int64_t nIdx, double* nodeForce, double* nodeVel,
                      int64_t outCols, int flags);
 
+/* calc_uTilde / calc_uTilde! (src/Processing/DislocationFEM.jl:355-707):
+ * the dislocation displacement at P points (the FE nodes uGammaDln), u:
+ * 3 x P, point by point (the reference's out-of-place vector; the in-place
+ * form adds value k into dof uDofsDln[k], which the caller does).  Every
+ * segment i of links (2 x linksCols, 1-based) with Burgers vector bVec[:, i]
+ * is a Barnett triangle closed at the centroid of the mesh vertices; a
+ * segment with an external node (label extDln) is closed over its
+ * projections onto the mesh surface (solid-angle terms only), and a
+ * surface-external link only records the connection, as in the reference.
+ * label: nodeTypeDln (Int32) per coord column; vertices 3 x 8, faceNorm and
+ * faceMidPt 3 x 6 of the RegularCuboidMesh.  omNuInv8pi = 1/(8 pi (1-nu)),
+ * om2NuOmNuInv8pi = (1-2nu)/(8 pi (1-nu)) (MaterialConstructor.jl:28-29).
+ * flags: DDD_ACCUMULATE adds into u. */
+int ddd_u_tilde(ddd_ctx* ctx, double omNuInv8pi, double om2NuOmNuInv8pi, const ddd_network* net,
+                const int32_t* label, const double* vertices, const double* faceNorm,
+                const double* faceMidPt, const double* points, int64_t P, double* u, int flags);
+
 /* ---- Device-resident path (no counterpart in the reference) ----------
  * For callers that evaluate forces many times on one network (time steps,
  * benchmarks): upload once, compute on the device, fetch when needed.

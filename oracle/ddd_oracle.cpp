@@ -669,3 +669,221 @@ void orc_mobility_bcc(const double* drag, double sigmaPN, const int64_t* connect
 }
 
 }  // extern "C"
+
+// ---- calc_uTilde: dislocation displacements at FE nodes -------------------
+// DislocationFEM.jl:355-600 (_calc_uTilde!), the triangle displacement of
+// :602-707 (Barnett triangles, Bromage & Tarleton 2018), safeNorm
+// (Misc.jl:289-299), linePlaneIntersect (Misc.jl:223-235) and
+// findIntersectVolume (DislocationTopologySpecial.jl:7-38).
+namespace {
+
+struct UT {
+  double omNuInv8pi, om2NuOmNuInv8pi;
+};
+
+inline double sq(double x) { return x * x; }
+
+// safeNorm: (|x|, x/|x|), or (eps, 0) below eps.
+inline V3<double> safe_norm(const V3<double>& x, double& n) {
+  const double eps = std::numeric_limits<double>::epsilon();
+  n = norm(x);
+  if (n > eps) return mk(x.x / n, x.y / n, x.z / n);
+  n = eps;
+  return mk(0.0, 0.0, 0.0);
+}
+
+inline double clamp1(double x) { return x < -1 ? -1 : (x > 1 ? 1 : x); }
+inline double sgn(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : 0.0); }
+
+// Solid angle of triangle ABC seen from the point, by l'Huilier's theorem
+// on the unit vectors towards the corners, signed by RA . (CA x AB).
+inline double solid_angle(const V3<double>& RA, const V3<double>& RB, const V3<double>& RC,
+                          const V3<double>& AB, const V3<double>& CA) {
+  const double ab = clamp1(dot(RA, RB)), bc = clamp1(dot(RB, RC)), ca = clamp1(dot(RC, RA));
+  const double ta = std::acos(ab), tb = std::acos(bc), tc = std::acos(ca);
+  const double t = (ta + tb + tc) / 2;
+  double f = std::tan(t / 2) * std::tan((t - ta) / 2) * std::tan((t - tb) / 2) *
+             std::tan((t - tc) / 2);
+  f = std::max(f, 0.0);
+  const double w = 4 * std::atan(std::sqrt(f));
+  return -sgn(dot(RA, cross(CA, AB))) * w;
+}
+
+// calcDisplacementDislocationTriangle! with the material terms (:602-670).
+void tri_full(const UT& k, const V3<double>& A, const V3<double>& B, const V3<double>& C,
+              const V3<double>& b, const double* pts, int64_t P, double* u) {
+  double dummy;
+  const V3<double> AB = safe_norm(B - A, dummy), BC = safe_norm(C - B, dummy),
+                   CA = safe_norm(A - C, dummy);
+  const V3<double> bAB = cross(b, AB), bBC = cross(b, BC), bCA = cross(b, CA);
+  const double fourPi = 4 * M_PI;
+  for (int64_t i = 0; i < P; ++i) {
+    const V3<double> p = mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+    double ra, rb, rc;
+    const V3<double> RA = safe_norm(A - p, ra), RB = safe_norm(B - p, rb), RC = safe_norm(C - p, rc);
+    const V3<double> fAB = bAB * std::log((rb / ra) * (1 + dot(RB, AB)) / (1 + dot(RA, AB)));
+    const V3<double> fBC = bBC * std::log((rc / rb) * (1 + dot(RC, BC)) / (1 + dot(RB, BC)));
+    const V3<double> fCA = bCA * std::log((ra / rc) * (1 + dot(RA, CA)) / (1 + dot(RC, CA)));
+    const double ab = clamp1(dot(RA, RB)), bc = clamp1(dot(RB, RC)), ca = clamp1(dot(RC, RA));
+    const V3<double> gAB = (dot(b, cross(RA, RB)) * (RA + RB)) / (1 + ab);
+    const V3<double> gBC = (dot(b, cross(RB, RC)) * (RB + RC)) / (1 + bc);
+    const V3<double> gCA = (dot(b, cross(RC, RA)) * (RC + RA)) / (1 + ca);
+    const double w = solid_angle(RA, RB, RC, AB, CA);
+    const V3<double> ui = (((-b) * w) / fourPi - k.om2NuOmNuInv8pi * ((fAB + fBC) + fCA)) +
+                          k.omNuInv8pi * ((gAB + gBC) + gCA);
+    u[3 * i] += ui.x;
+    u[3 * i + 1] += ui.y;
+    u[3 * i + 2] += ui.z;
+  }
+}
+
+// The solid-angle term alone (:671-707), for the closures of external
+// segments.
+void tri_solid(const V3<double>& A, const V3<double>& B, const V3<double>& C, const V3<double>& b,
+               const double* pts, int64_t P, double* u) {
+  double dummy;
+  const V3<double> AB = safe_norm(B - A, dummy), CA = safe_norm(A - C, dummy);
+  const double fourPi = 4 * M_PI;
+  for (int64_t i = 0; i < P; ++i) {
+    const V3<double> p = mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+    double r;
+    const V3<double> RA = safe_norm(A - p, r), RB = safe_norm(B - p, r), RC = safe_norm(C - p, r);
+    const double w = solid_angle(RA, RB, RC, AB, CA);
+    const V3<double> ui = ((-b) * w) / fourPi;
+    u[3 * i] += ui.x;
+    u[3 * i + 1] += ui.y;
+    u[3 * i + 2] += ui.z;
+  }
+}
+
+struct Cuboid {
+  const double* faceNorm;   // 3 x 6
+  const double* faceMidPt;  // 3 x 6
+  double lo[3], hi[3];      // bounding box of the vertices
+  V3<double> n(int i) const { return mk(faceNorm[3 * i], faceNorm[3 * i + 1], faceNorm[3 * i + 2]); }
+  V3<double> m(int i) const {
+    return mk(faceMidPt[3 * i], faceMidPt[3 * i + 1], faceMidPt[3 * i + 2]);
+  }
+  // membership of the convex hull of the vertices (a box here), with an
+  // absolute tolerance of 1e-9 of the box size for points computed on a face
+  bool inside(const V3<double>& p) const {
+    const double v[3] = {p.x, p.y, p.z};
+    for (int d = 0; d < 3; ++d) {
+      const double tol = 1e-9 * (hi[d] - lo[d]);
+      if (!(v[d] >= lo[d] - tol && v[d] <= hi[d] + tol)) return false;
+    }
+    return true;
+  }
+};
+
+// findIntersectVolume(mesh, l, l0): squared distance from l0 to the nearest
+// intersection of the line l0 + s l with a face plane that lies in the
+// volume (Inf if none), and that intersection.
+double find_intersect(const Cuboid& cb, const V3<double>& l, const V3<double>& l0,
+                      V3<double>& node) {
+  double dmin = std::numeric_limits<double>::infinity();
+  node = mk(0.0, 0.0, 0.0);
+  for (int i = 0; i < 6; ++i) {
+    const double old = dmin;
+    const double den = dot(l, cb.n(i));
+    const double num = dot(cb.m(i) - l0, cb.n(i));
+    if (den == 0) continue;  // parallel: no point, or the whole line (Inf)
+    const double dd = num / den;
+    const V3<double> x = l0 + dd * l;
+    if (std::isinf(x.x) || std::isinf(x.y) || std::isinf(x.z)) continue;
+    if (!cb.inside(x)) continue;
+    const V3<double> r = x - l0;
+    dmin = std::min((r.x * r.x + r.y * r.y) + r.z * r.z, old);
+    if (dmin < old) node = x;
+  }
+  return dmin;
+}
+
+}  // namespace
+
+extern "C" {
+
+// calc_uTilde: u (3 x P, point by point) += displacement of every segment
+// of the network at the points.  k2 = (omNuInv8pi, om2NuOmNuInv8pi);
+// label: nodeTypeDln per node (Int32); vertices 3 x 8, faceNorm and
+// faceMidPt 3 x 6 of the cuboid mesh.
+void orc_u_tilde(const double* k2, int64_t numSeg, const int64_t* links, const double* coord,
+                 const double* bVec, const int32_t* label, int64_t numNodeCols,
+                 const double* vertices, const double* faceNorm, const double* faceMidPt,
+                 const double* pts, int64_t P, double* u) {
+  const UT k{k2[0], k2[1]};
+  Cuboid cb{faceNorm, faceMidPt, {INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}};
+  V3<double> C = mk(0.0, 0.0, 0.0);
+  for (int v = 0; v < 8; ++v) {
+    C = C + mk(vertices[3 * v], vertices[3 * v + 1], vertices[3 * v + 2]);
+    for (int d = 0; d < 3; ++d) {
+      cb.lo[d] = std::min(cb.lo[d], vertices[3 * v + d]);
+      cb.hi[d] = std::max(cb.hi[d], vertices[3 * v + d]);
+    }
+  }
+  C = C / 8.0;
+  const int32_t srfMob = 3, srfFix = 4, ext = 5;
+  std::vector<int64_t> surfCon((size_t)numNodeCols + 1, 0);
+  auto X = [&](int64_t node) {
+    return mk(coord[3 * (node - 1)], coord[3 * (node - 1) + 1], coord[3 * (node - 1) + 2]);
+  };
+  for (int64_t i = 0; i < numSeg; ++i) {
+    const int64_t n1 = links[2 * i], n2 = links[2 * i + 1];
+    const int32_t l1 = label[n1 - 1], l2 = label[n2 - 1];
+    const bool s1 = l1 == srfMob || l1 == srfFix, s2 = l2 == srfMob || l2 == srfFix;
+    if (s1 && l2 == ext) { surfCon[n2] = n1; continue; }
+    if (s2 && l1 == ext) { surfCon[n1] = n2; continue; }
+    const V3<double> A = X(n1), B = X(n2);
+    const V3<double> b = mk(bVec[3 * i], bVec[3 * i + 1], bVec[3 * i + 2]);
+    if (l1 != ext && l2 != ext) {
+      tri_full(k, A, B, C, b, pts, P, u);
+      continue;
+    }
+    // External segment: project both ends back to the surface along the
+    // normals of faces 1, 3, 5 (a cuboid needs only those).
+    const double inf = std::numeric_limits<double>::infinity();
+    double dA = inf, dB = inf, dC = inf;
+    V3<double> iA = mk(inf, inf, inf), iB = iA;
+    for (int j = 0; j < 6; j += 2) {
+      V3<double> xa, xb;
+      const double ta = find_intersect(cb, cb.n(j), A, xa);
+      const double tb = find_intersect(cb, cb.n(j), B, xb);
+      if (ta < dA) { dA = ta; iA = xa; }
+      if (tb < dB) { dB = tb; iB = xb; }
+    }
+    // No intersection (projected out of an edge or corner): walk towards
+    // the centroid over the face planes, as the reference does (its plane
+    // distance adds the face midpoint, and one minimum serves both ends).
+    if (std::isinf(dA)) {
+      V3<double> t = A;
+      for (int f = 0; f < 6; ++f) {
+        const double dist = sq(dot(t + cb.m(f), cb.n(f)));
+        const V3<double> x = t - cb.n(f) * std::sqrt(dist);
+        const double dc = norm(x - C);
+        if (dc <= dC) { dC = dc; t = x; }
+      }
+      iA = t;
+    }
+    if (std::isinf(dB)) {
+      V3<double> t = B;
+      for (int f = 0; f < 6; ++f) {
+        const double dist = sq(dot(t + cb.m(f), cb.n(f)));
+        const V3<double> x = t - cb.n(f) * std::sqrt(dist);
+        const double dc = norm(x - C);
+        if (dc <= dC) { dC = dc; t = x; }
+      }
+      iB = t;
+    }
+    V3<double> Ap = iA, Bp = iB;
+    if (surfCon[n1] != 0) Ap = X(surfCon[n1]);
+    if (surfCon[n2] != 0) Bp = X(surfCon[n2]);
+    const V3<double> Cp = (A + Bp) / 2.0;
+    tri_solid(Ap, Bp, C, b, pts, P, u);
+    tri_solid(Ap, A, Cp, b, pts, P, u);
+    tri_solid(A, B, Cp, b, pts, P, u);
+    tri_solid(B, Bp, Cp, b, pts, P, u);
+    tri_solid(Bp, Ap, Cp, b, pts, P, u);
+  }
+}
+
+}  // extern "C"

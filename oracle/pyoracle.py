@@ -53,6 +53,8 @@ def lib():
             "orc_node_gather": (None, [ip, i64, i64, ip, dp, dp, d, dp]),
             "orc_mobility_bcc": (None, [dp, d, ip, i64, i64, ip, dp, dp, dp, ip, i64, dp, dp,
                                         dp]),
+            "orc_u_tilde": (None, [dp, i64, ip, dp, dp, C.POINTER(C.c_int32), i64, dp, dp, dp,
+                                   dp, i64, dp]),
             "orc_set_libm": (None, [i32]),
             "orc_segseg_near": (None, [dp] + net + [ip, i32, i32, dp, i32]),
             "orc_segseg_near_once": (None, [dp] + net + [ip, i32, i32, ip, dp]),
@@ -447,3 +449,39 @@ def integrate_(ip, itime, dln, mat, mesh, fd, net):
         if counter > ip.maxiter or dt == ip.dtmax:
             convergent = True
     return dt, itime.time + dt, itime.step + 1
+
+
+def u_tilde_points(mat, net, mesh, points):
+    """Displacement of the network at the points (3 x P): the sum of
+    _calc_uTilde! (DislocationFEM.jl:381-600) before it is scattered into
+    dofs; column i belongs to point i."""
+    x = _f(np.asarray(points, dtype=np.float64).reshape(3, -1, order="F"))
+    P = x.shape[1]
+    out = np.zeros((3, P), order="F")
+    links = _f(net.links, np.int64)
+    coord = _f(net.coord)
+    bVec = _f(net.bVec)
+    lab = np.ascontiguousarray(net.label, dtype=np.int32)
+    k2 = np.array([mat.omνInv8π, mat.om2νomνInv8π])
+    lib().orc_u_tilde(_d(k2), int(net.numSeg[0]), _i(links), _d(coord), _d(bVec),
+                      lab.ctypes.data_as(C.POINTER(C.c_int32)), coord.shape[1],
+                      _d(_f(mesh.vertices)), _d(_f(mesh.faceNorm)), _d(_f(mesh.faceMidPt)), _d(x),
+                      P, _d(out))
+    return out
+
+
+def calc_uTilde(mesh, boundaries, mat, net):
+    """calc_uTilde (DislocationFEM.jl:369-379): the displacements at the
+    nodes uGammaDln, interleaved (x, y, z of each node in turn)."""
+    pts = mesh.coord[:, np.asarray(boundaries.uGammaDln) - 1]
+    return u_tilde_points(mat, net, mesh, pts).ravel(order="F")
+
+
+def calc_uTilde_(fd, mesh, boundaries, mat, net):
+    """calc_uTilde! (DislocationFEM.jl:355-367): zero fd.uTilde, then add
+    value k of the interleaved displacements into dof uDofsDln[k] (the
+    reference's mapping, whose dof list is ordered x, then y, then z)."""
+    v = calc_uTilde(mesh, boundaries, mat, net)
+    fd.uTilde[:] = 0.0
+    np.add.at(fd.uTilde, np.asarray(boundaries.uDofsDln) - 1, v)
+    return fd.uTilde

@@ -74,6 +74,13 @@ def main(ref):
     F = "test/DislocationFEMTest.jl"
     put("sigmatilde_edge", F, fem, 61, 69)
     put("sigmatilde_screw", F, fem, 138, 146, scale=1.0e-03)
+    # dislocation displacements at the cantilever's u-boundary nodes
+    # (calc_uTilde, in place into the dofs uDofsDln)
+    put("utilde_uGammaTest", F, fem, 323, 376, flatten=True)
+    put("utilde_uxTest", F, fem, 387, 440, flatten=True)
+    put("utilde_uyTest", F, fem, 443, 496, flatten=True)
+    put("utilde_uzTest", F, fem, 499, 552, flatten=True)
+    put("utilde_uTildeTest", F, fem, 564, 1139, flatten=True)
     # whole path + gather (node force), MobilityTest
     M = "test/MobilityTest.jl"
     put("mobility_testForce", M, mob, 79, 89)

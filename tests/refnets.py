@@ -53,6 +53,28 @@ def integration():
     return ip, IntegrationTime(dt=float(t["dt"]), time=float(t["time"]), step=int(t["step"]))
 
 
+def utilde_setup():
+    """The calc_uTilde test of DislocationFEMTest.jl:270-320: a 1013 x 1987
+    x 2999 cantilever meshed 3 x 5 x 7, a prismatic and a shear octagon on
+    the slip system (-1 1 0)[1 1 1], both centred in the volume, nu = 0.28."""
+    from dddhip.types import Boundaries, ForceDisplacement
+    dx, dy, dz = 1013.0, 1987.0, 2999.0
+    mat = MaterialParameters(μ=1.0, ν=0.28)
+    m = RegularCuboidMesh(dx=dx, dy=dy, dz=dz, mx=3, my=5, mz=7, μ=mat.μ, ν=mat.ν)
+    ss = SlipSystem(np.array([[-1.0], [1.0], [0.0]]), np.array([[1.0], [1.0], [1.0]]))
+    segLen = (dx + dy + dz) / 30
+    rng = np.array([[dx / 2, dx / 2], [dy / 2, dy / 2], [dz / 2, dz / 2]])
+
+    def loop(kind):
+        return DislocationLoop(loopType=kind, numSides=8, nodeSide=1, numLoops=1,
+                               segLen=segLen * np.ones(8), slipSystemIdx=1, slipSystem=ss,
+                               label=[intMobDln] * 8, buffer=0.0, range=rng, dist="zeros")
+    net = DislocationNetwork.from_loops([loop("prism"), loop("shear")])
+    bnd = Boundaries.cantilever(m)
+    fd = ForceDisplacement(uHat=np.zeros(3 * m.numNode), uTilde=np.zeros(3 * m.numNode))
+    return mat, m, bnd, fd, net
+
+
 def mesh():
     """buildMesh(matParams, femParams) for sampleFEMParameters.json: 2000^3, 5x5x5."""
     f = GOLD["file:test/testData/sampleFEMParameters.json"]["data"]
