@@ -250,6 +250,57 @@ def sigma_leg(args, dln, mat, ctx, log):
             "finite": bool(np.isfinite(σ).all())}
 
 
+# Hand count of one full Barnett triangle at one point (DislocationFEM.jl:
+# 617-664): 199 FP64 add/sub/mul (FMA = 2) plus 4 sqrt, 27 div, 3 log,
+# 3 acos, 4 tan and 1 atan.
+U_TILDE_FLOPS_PER_PAIR = 199
+
+
+def utilde_leg(args, mat, net, ctx, log):
+    """calc_uTilde of the benchmark network at the u-boundary nodes of a
+    cantilever meshed args.utilde_grid^3 over [-300, 2300]^3: wall time of
+    the synchronous ABI call (triangles built on the host, PCIe included);
+    CPU oracle timed on a few of the points and scaled by point count."""
+    import numpy as np
+    from dddhip import forces as G
+    from dddhip.types import Boundaries, RegularCuboidMesh
+    import pyoracle as O
+    g = args.utilde_grid
+    m = RegularCuboidMesh(dx=2600.0, dy=2600.0, dz=2600.0, mx=g, my=g, mz=g, μ=mat.μ, ν=mat.ν)
+    for a in ("coord", "vertices", "faceMidPt"):
+        setattr(m, a, np.asfortranarray(getattr(m, a) - 300.0))
+    bnd = Boundaries.cantilever(m)
+    pts = np.asfortranarray(m.coord[:, bnd.uGammaDln - 1])
+    n, P = int(net.numSeg[0]), pts.shape[1]
+    u = G.u_tilde_points(mat, net, m, pts, ctx=ctx)
+    t = []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        u = G.u_tilde_points(mat, net, m, pts, ctx=ctx)
+        t.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.mean(t))
+    pairs = n * P
+    ns = max(1, args.utilde_cpu_points)
+    t0 = time.perf_counter()
+    o = O.u_tilde_points(mat, net, m, pts[:, :ns])
+    cpu = (time.perf_counter() - t0) * 1e3 * P / ns
+    err = normwise_local(u[:, :ns], o)
+    log(f"calc_uTilde: {n} segments x {P} points: {ms:.2f} ms (PCIe included), "
+        f"CPU oracle {cpu / 1e3:.1f} s (scaled from {ns} points), err vs oracle {err:.1e}")
+    return {"segments": n, "points": P, "mesh": f"{g}^3 cantilever u-boundary",
+            "triangle_point_pairs": pairs, "ms_pcie_inclusive": round(ms, 3),
+            "tflops_pcie_inclusive": round(U_TILDE_FLOPS_PER_PAIR * pairs / (ms * 1e-3) / 1e12, 3),
+            "flops_per_pair": U_TILDE_FLOPS_PER_PAIR,
+            "cpu_oracle_ms_scaled": round(cpu, 1), "cpu_sample_points": ns,
+            "normwise_err_vs_oracle": err, "finite": bool(np.isfinite(u).all())}
+
+
+def normwise_local(a, b):
+    import numpy as np
+    d = max(np.linalg.norm(a), np.linalg.norm(b))
+    return float(np.linalg.norm(a - b) / d) if d > 0 else 0.0
+
+
 def clustered_leg(args, dln, mat, ctx, log):
     """The clustered stress test: 8,192 loops x 32 = 262,144 segments, 70 %
     in 8 pile-ups of same-slip-system loops (centres N(c, 150^2)), 30 %
@@ -341,6 +392,10 @@ def main():
                     help="calc_σTilde leg: network of this many loops (seed 0xB0) at 6 x g^2 "
                          "face points; 0 to skip")
     ap.add_argument("--sigma-grid", type=int, default=128)
+    ap.add_argument("--utilde-grid", type=int, default=64,
+                    help="calc_uTilde leg: cantilever mesh of this many elements per side "
+                         "(0 skips the leg)")
+    ap.add_argument("--utilde-cpu-points", type=int, default=4)
     ap.add_argument("--no-dynamics", dest="dynamics", action="store_false",
                     help="skip the deriv!/integrate! leg (run with the first hybrid grid)")
     ap.add_argument("--clustered", action="store_true",
@@ -452,6 +507,7 @@ def main():
     mixed_c = mixed_leg(args, dln, mat, ctx, log) if args.mixed_loops > 0 else None
     sigma = sigma_leg(args, dln, mat, ctx, log) if args.sigma_loops > 0 else None
     clustered = clustered_leg(args, dln, mat, ctx, log) if args.clustered else None
+    utilde = utilde_leg(args, mat, net, ctx, log) if args.utilde_grid > 0 else None
 
     achieved = FLOPS_PER_PAIR * pairs / (tms * 1e-3) / 1e12
     result = {
@@ -483,6 +539,7 @@ def main():
         "mixed_65536": mixed_c,
         "sigma_tilde": sigma,
         "clustered": clustered,
+        "u_tilde": utilde,
         "roofline": {
             "bound": "valu",
             "achieved": round(achieved, 3),
