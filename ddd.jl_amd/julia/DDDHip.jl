@@ -335,6 +335,42 @@ function resident_seg_force!(network::DislocationNetwork)
     return nothing
 end
 
+# calc_uTilde / calc_uTilde! (src/Processing/DislocationFEM.jl:355-707) on the
+# device.  The reference's signature carries no DislocationParameters, so there
+# is no parGPU switch: these methods always run on the GPU.
+function u_tilde_points(mesh, matParams, network, points::Matrix{Float64})
+    P = size(points, 2)
+    u = zeros(3, P)
+    label = Int32.(network.label)
+    vertices = reduce(hcat, mesh.vertices.vertices)  # 3 x 8
+    faceNorm = Matrix{Float64}(mesh.faceNorm)
+    faceMidPt = Matrix{Float64}(mesh.faceMidPt)
+    GC.@preserve network label vertices faceNorm faceMidPt points u begin
+        nw = Ref(netstruct(network))
+        check(ccall((:ddd_u_tilde, libddd), Cint,
+            (Ptr{Cvoid}, Float64, Float64, Ref{Network}, Ptr{Int32}, Ptr{Float64}, Ptr{Float64},
+             Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}, Cint),
+            ctx(), matParams.omνInv8π, matParams.om2νomνInv8π, nw, pointer(label),
+            pointer(vertices), pointer(faceNorm), pointer(faceMidPt), pointer(points), P,
+            pointer(u), Cint(0)))
+    end
+    return u
+end
+
+calc_uTilde(mesh, boundaries, matParams, network::DislocationNetwork) =
+    vec(u_tilde_points(mesh, matParams, network, Matrix{Float64}(mesh.coord[:, boundaries.uGammaDln])))
+
+function calc_uTilde!(forceDisplacement, mesh, boundaries, matParams, network::DislocationNetwork)
+    uTilde = forceDisplacement.uTilde
+    uTilde .= 0
+    v = calc_uTilde(mesh, boundaries, matParams, network)
+    uDofs = boundaries.uDofsDln
+    for k in eachindex(v)           # the reference's dof mapping, value k -> uDofs[k]
+        uTilde[uDofs[k]] += v[k]
+    end
+    return nothing
+end
+
 # deriv! / integrate! (src/Processing/IntegrationProcessing.jl:13-168) with the
 # network on the device: coordinates, velocities and forces stay in HBM across
 # the adaptive step's trial evaluations; results are copied back at the end.
