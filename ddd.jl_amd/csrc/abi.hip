@@ -136,6 +136,8 @@ struct ddd_ctx {
   DevBuf conn, links, coord, nodeF, bvec, nodeV, nodeIdx;
   // displacement triangles, points, chunk partials, result
   DevBuf uTri, uPts, uPart, uOut;
+  // collision detection
+  DevBuf cConn, cLinks, cCoord, cVel, cB, cLabel, cSkip, cSeg, cNodes, cFlags, cBest, cLast, cRes;
   std::vector<double> h;  // host staging
   DevBuf counter;  // FP64-pair counter of the mixed path
   // near-parallel pairs of the all-pairs tiles (count, offsets, list, forces)
@@ -1068,6 +1070,150 @@ int ddd_u_tilde(ddd_ctx* c, double omNuInv8pi, double om2NuOmNuInv8pi, const ddd
     }
     HIPCHK(hipMemcpyAsync(u, c->uOut.p, sizeof(double) * 3 * P, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));  // tri (host) is released on return
+  });
+}
+
+int ddd_detect_collision(ddd_ctx* c, double collisionDistSq, const ddd_network* net,
+                         const int32_t* label, const double* nodeVel, const int64_t* skipSegs,
+                         int64_t nSkip, ddd_collision* out) {
+  return guard(c, "ddd_detect_collision", [&] {
+    if (!net || !out) bad("NULL argument");
+    const int64_t nn = net->numNode, ns = net->numSeg;
+    if (nn < 0 || ns < 0) bad("negative sizes");
+    if ((nn > 0 || ns > 0) && (!label || !nodeVel || !net->coord || !net->links))
+      bad("label, nodeVel, coord and links required");
+    if (nn > 0 && !net->connectivity) bad("connectivity required");
+    if (ns > 0 && !net->bVec) bad("bVec required");
+    if (nSkip < 0 || (nSkip > 0 && !skipSegs)) bad("skipSegs/nSkip inconsistent");
+    if (nn > net->connCols || nn > net->coordCols) bad("numNode outside arrays");
+    if (ns > net->linksCols || ns > net->bVecCols) bad("numSeg outside links or bVec");
+    if (ns > ((int64_t)1 << 31)) bad("numSeg too large");
+    for (int64_t i = 0; i < ns; ++i)
+      for (int e = 0; e < 2; ++e) {
+        const int64_t nd = net->links[2 * i + e];
+        if (nd < 1 || nd > net->coordCols) bad("links node outside coord");
+      }
+    const int64_t maxC = net->connLd >= 1 ? (net->connLd - 1) / 2 : 0;
+    for (int64_t q = 0; q < nn; ++q) {
+      const int64_t* cn = net->connectivity + net->connLd * q;
+      if (cn[0] < 0 || cn[0] > maxC) bad("connectivity count");
+      for (int64_t e = 1; e <= cn[0]; ++e) {
+        const int64_t lk = cn[2 * e - 1], col = cn[2 * e];
+        if (lk < 1 || lk > net->linksCols) bad("connectivity link");
+        if (col != 1 && col != 2) bad("connectivity column");
+        const int64_t other = net->links[2 * (lk - 1) + (2 - col)];
+        if (other < 1 || other > net->coordCols) bad("links node outside coord");
+      }
+    }
+    *out = ddd_collision{0, 0, 0, 0, 0, 0, 0, 0, 0.0, 0.0};
+    const int64_t cols = net->coordCols;
+    c->cCoord.ensure(sizeof(double) * 3 * std::max<int64_t>(cols, 1));
+    c->cVel.ensure(sizeof(double) * 3 * std::max<int64_t>(cols, 1));
+    c->cLabel.ensure(sizeof(int32_t) * std::max<int64_t>(cols, 1));
+    c->cLinks.ensure(sizeof(int64_t) * 2 * std::max<int64_t>(net->linksCols, 1));
+    c->cSkip.ensure(sizeof(int64_t) * 2 * std::max<int64_t>(nSkip, 1));
+    c->cRes.ensure(sizeof(CollisionCand) + sizeof(CollisionLast));
+    if (cols > 0) {
+      HIPCHK(hipMemcpyAsync(c->cCoord.p, net->coord, sizeof(double) * 3 * cols,
+                            hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipMemcpyAsync(c->cVel.p, nodeVel, sizeof(double) * 3 * cols, hipMemcpyHostToDevice,
+                            c->st));
+      HIPCHK(hipMemcpyAsync(c->cLabel.p, label, sizeof(int32_t) * cols, hipMemcpyHostToDevice,
+                            c->st));
+    }
+    if (net->linksCols > 0)
+      HIPCHK(hipMemcpyAsync(c->cLinks.p, net->links, sizeof(int64_t) * 2 * net->linksCols,
+                            hipMemcpyHostToDevice, c->st));
+    // skip list as sorted keys min * 2^32 + max (binary-searched per pair)
+    std::vector<int64_t> skipKeys((size_t)nSkip);
+    for (int64_t q = 0; q < nSkip; ++q) {
+      const int64_t a = skipSegs[2 * q], b = skipSegs[2 * q + 1];
+      if (a < 0 || b < 0 || a >= ((int64_t)1 << 31) || b >= ((int64_t)1 << 31))
+        bad("skipSegs entry outside [0, 2^31)");
+      skipKeys[(size_t)q] = a < b ? (a << 32) + b : (b << 32) + a;
+    }
+    std::sort(skipKeys.begin(), skipKeys.end());
+    if (nSkip > 0)
+      HIPCHK(hipMemcpyAsync(c->cSkip.p, skipKeys.data(), sizeof(int64_t) * nSkip,
+                            hipMemcpyHostToDevice, c->st));
+    CollisionCand* resB = reinterpret_cast<CollisionCand*>(c->cRes.p);
+    CollisionLast* resL = reinterpret_cast<CollisionLast*>(static_cast<char*>(c->cRes.p) +
+                                                           sizeof(CollisionCand));
+    CollisionCand hb{0.0, -1, -1, {0, 0, 0, 0, 0, 0}};
+    CollisionLast hl{-1, -1, 0.0, 0.0};
+    // hinges
+    if (nn > 0) {
+      c->cConn.ensure(sizeof(int64_t) * net->connLd * nn);
+      c->cBest.ensure(sizeof(CollisionCand) * nn);
+      c->cLast.ensure(sizeof(CollisionLast) * nn);
+      HIPCHK(hipMemcpyAsync(c->cConn.p, net->connectivity, sizeof(int64_t) * net->connLd * nn,
+                            hipMemcpyHostToDevice, c->st));
+      launch_collision_hinge(nn, c->cConn.as<int64_t>(), net->connLd, c->cLinks.as<int64_t>(),
+                             c->cCoord.as<double>(), c->cVel.as<double>(),
+                             c->cLabel.as<int32_t>(), c->cSkip.as<int64_t>(), nSkip,
+                             collisionDistSq, c->cBest.as<CollisionCand>(),
+                             c->cLast.as<CollisionLast>(), c->st);
+      launch_collision_final(c->cBest.as<CollisionCand>(), c->cLast.as<CollisionLast>(), nn, resB,
+                             resL, c->st);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(&hb, resB, sizeof(hb), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipMemcpyAsync(&hl, resL, sizeof(hl), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+    }
+    auto fill = [&](const CollisionCand& b, int type, const CollisionLast& l) {
+      out->collision = 1;
+      out->type = type;
+      out->n1s1 = b.info[0];
+      out->n2s1 = b.info[1];
+      out->n1s2 = b.info[2];
+      out->n2s2 = b.info[3];
+      out->s1 = b.info[4];
+      out->s2 = b.info[5];
+      out->L1 = l.L1;
+      out->L2 = l.L2;
+    };
+    if (hb.a >= 0) {
+      fill(hb, 1, hl);
+      return;
+    }
+    // pairs of links without a shared node
+    CollisionCand pb{0.0, -1, -1, {0, 0, 0, 0, 0, 0}};
+    CollisionLast pl{-1, -1, 0.0, 0.0};
+    if (ns >= 2) {
+      const int64_t ld = ((ns + 63) / 64) * 64;
+      const int blocks = collision_pair_blocks(ns);
+      c->cB.ensure(sizeof(double) * 3 * ns);
+      c->cSeg.ensure(sizeof(double) * 15 * ld);
+      c->cNodes.ensure(sizeof(int64_t) * 2 * ld);
+      c->cFlags.ensure(sizeof(int) * ld);
+      c->cBest.ensure(sizeof(CollisionCand) * blocks);
+      c->cLast.ensure(sizeof(CollisionLast) * blocks);
+      HIPCHK(hipMemcpyAsync(c->cB.p, net->bVec, sizeof(double) * 3 * ns, hipMemcpyHostToDevice,
+                            c->st));
+      launch_collision_pack(ns, ld, c->cLinks.as<int64_t>(), c->cCoord.as<double>(),
+                            c->cVel.as<double>(), c->cB.as<double>(), c->cLabel.as<int32_t>(),
+                            c->cSkip.as<int64_t>(), nSkip, c->cSeg.as<double>(),
+                            c->cNodes.as<int64_t>(), c->cFlags.as<int>(), c->st);
+      launch_collision_pairs(c->cSeg.as<double>(), ld, ns, c->cNodes.as<int64_t>(),
+                             c->cFlags.as<int>(), c->cSkip.as<int64_t>(), nSkip, collisionDistSq,
+                             blocks, c->cBest.as<CollisionCand>(), c->cLast.as<CollisionLast>(),
+                             c->st);
+      launch_collision_final(c->cBest.as<CollisionCand>(), c->cLast.as<CollisionLast>(), blocks,
+                             resB, resL, c->st);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(&pb, resB, sizeof(pb), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipMemcpyAsync(&pl, resL, sizeof(pl), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+    }
+    const CollisionLast& l = pl.a >= 0 ? pl : hl;
+    if (pb.a >= 0) {
+      fill(pb, 2, l);
+      return;
+    }
+    if (l.a >= 0) {
+      out->L1 = l.L1;
+      out->L2 = l.L2;
+    }
   });
 }
 

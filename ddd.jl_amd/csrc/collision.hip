@@ -1,0 +1,387 @@
+// Collision detection on the device: detectCollision
+// (src/Processing/DislocationTopologyCollision.jl:1-222) with minimumDistance
+// (src/Misc/Misc.jl:314-367).  Built without FMA contraction so that every
+// distance, rate and 1/distance rounds as the reference's expressions.
+//
+// The reference scans hinges (two links at one mobile node) and then, if no
+// hinge collides, all unordered pairs of links that share no node, keeping the
+// candidate with the largest 1/distSq (the first one in loop order on ties)
+// and returning L1, L2 of the last pair it evaluated.  Here both scans are
+// reductions with that order made explicit: a candidate is (key, loop
+// position), compared by key, then by position; the last evaluated pair is
+// the largest position, carried with its L1, L2.
+#include "kernels.hpp"
+
+namespace ddd {
+
+namespace {
+
+constexpr double kEps = 2.220446049250313e-16;
+
+struct MinDist {
+  double distSq, dDistSqDt, L1, L2;
+};
+
+__device__ inline double dot3(const double* a, const double* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// minimumDistance (Misc.jl:314-367), the reference's branches and clamps.
+__device__ MinDist min_distance(const double* x0, const double* x1, const double* y0,
+                                const double* y1, const double* vx0, const double* vx1,
+                                const double* vy0, const double* vy1) {
+  double s1[3], s2[3], w1[3], w2[3], xy[3], yx[3], s1x2[3], s2x2[3];
+  for (int d = 0; d < 3; ++d) {
+    s1[d] = x1[d] - x0[d];
+    s2[d] = y1[d] - y0[d];
+    w1[d] = vx1[d] - vx0[d];
+    w2[d] = vy1[d] - vy0[d];
+    xy[d] = x0[d] - y0[d];
+    yx[d] = y0[d] - x0[d];
+    s1x2[d] = 2.0 * s1[d];
+    s2x2[d] = 2.0 * s2[d];
+  }
+  const double A = dot3(s1, s1);
+  const double B = dot3(s1x2, xy);
+  const double C = dot3(s1x2, s2);
+  const double D = dot3(s2x2, yx);
+  const double E = dot3(s2, s2);
+  const double G = C * C - 4.0 * A * E;
+  double L1, L2;
+  if (A < kEps) {
+    L1 = 0.0;
+    L2 = E < kEps ? 0.0 : -D / E / 2.0;
+  } else if (E < kEps) {
+    L2 = 0.0;
+    L1 = A < kEps ? 0.0 : -B / A / 2.0;
+  } else if (fabs(G) < kEps) {
+    double d[4];
+    const double* ends[4][2] = {{y0, x0}, {y1, x0}, {y0, x1}, {y1, x1}};
+    for (int q = 0; q < 4; ++q) {
+      double r[3];
+      for (int c = 0; c < 3; ++c) r[c] = ends[q][0][c] - ends[q][1][c];
+      d[q] = dot3(r, r);
+    }
+    int idx = 1;  // first minimum, 1-based
+    for (int q = 1; q < 4; ++q)
+      if (d[q] < d[idx - 1]) idx = q + 1;
+    L1 = floor(idx / 2.0);
+    L2 = (double)((idx - 1) % 2);
+  } else {
+    L2 = (2.0 * A * D + B * C) / G;
+    L1 = (C * L2 - B) / A / 2.0;
+  }
+  L1 = L1 < 0.0 ? 0.0 : (L1 > 1.0 ? 1.0 : L1);
+  L2 = L2 < 0.0 ? 0.0 : (L2 > 1.0 ? 1.0 : L2);
+  double r[3], vr[3];
+  for (int d = 0; d < 3; ++d) {
+    r[d] = ((x0[d] + s1[d] * L1) - y0[d]) - s2[d] * L2;
+    vr[d] = ((vx0[d] + w1[d] * L1) - vy0[d]) - w2[d] * L2;
+  }
+  return MinDist{dot3(r, r), 2.0 * dot3(vr, r), L1, L2};
+}
+
+__device__ inline bool mobile(int32_t l) { return l == 1 || l == 3; }
+
+// The skip list as sorted keys min(a, b) * 2^32 + max(a, b) of its link
+// pairs (built on the host); membership by binary search.
+__device__ bool skipped(const int64_t* keys, int64_t nSkip, int64_t a, int64_t b) {
+  const int64_t key = a < b ? (a << 32) + b : (b << 32) + a;
+  int64_t lo = 0, hi = nSkip;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) / 2;
+    if (keys[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < nSkip && keys[lo] == key;
+}
+
+// True if candidate (key, pa, pb) beats the current best (bestKey, qa, qb;
+// qa < 0: none yet): a larger key, or the same key earlier in loop order.
+// Keys are 1/distSq of candidates, so positive (or +Inf), never NaN.
+__device__ inline bool better(double key, int64_t pa, int64_t pb, double bestKey, int64_t qa,
+                              int64_t qb) {
+  if (qa < 0) return true;
+  if (key != bestKey) return key > bestKey;
+  return pa < qa || (pa == qa && pb < qb);
+}
+
+// ---- hinges ---------------------------------------------------------------
+// Per node (1-based i = n + 1): its best hinge and its last evaluated hinge.
+// Position of hinge (i, j, k): (i, j * maxC + k).
+__global__ void k_collision_hinge(int64_t numNode, const int64_t* __restrict__ conn, int64_t ldConn,
+                                  const int64_t* __restrict__ links,
+                                  const double* __restrict__ coord,
+                                  const double* __restrict__ vel,
+                                  const int32_t* __restrict__ label, const int64_t* __restrict__ skip,
+                                  int64_t nSkip, double collisionDistSq,
+                                  CollisionCand* __restrict__ best, CollisionLast* __restrict__ last) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (n >= numNode) return;
+  CollisionCand b{0.0, -1, -1, {0, 0, 0, 0, 0, 0}};
+  CollisionLast lv{-1, -1, 0.0, 0.0};
+  const int64_t i = n + 1;
+  if (mobile(label[n])) {
+    const int64_t* cn = conn + ldConn * n;
+    const int64_t maxC = (ldConn - 1) / 2;
+    const double* xi = coord + 3 * n;
+    const double* vi = vel + 3 * n;
+    for (int64_t j = 1; j <= cn[0]; ++j) {
+      for (int64_t k = j + 1; k <= cn[0]; ++k) {
+        int64_t lk1 = cn[2 * j - 1], lk2 = cn[2 * k - 1];
+        int64_t c1 = links[2 * (lk1 - 1) + (2 - cn[2 * j])];
+        int64_t c2 = links[2 * (lk2 - 1) + (2 - cn[2 * k])];
+        double t1[3], t2[3];
+        for (int d = 0; d < 3; ++d) {
+          t1[d] = xi[d] - coord[3 * (c1 - 1) + d];
+          t2[d] = xi[d] - coord[3 * (c2 - 1) + d];
+        }
+        if (dot3(t1, t1) < dot3(t2, t2)) {
+          const int64_t tl = lk1, tc = c1;
+          lk1 = lk2;
+          c1 = c2;
+          lk2 = tl;
+          c2 = tc;
+        }
+        if (skipped(skip, nSkip, lk1, lk2)) continue;
+        const double* y0 = coord + 3 * (c2 - 1);
+        const double* vy0 = vel + 3 * (c2 - 1);
+        const MinDist m = min_distance(xi, coord + 3 * (c1 - 1), y0, y0, vi, vel + 3 * (c1 - 1),
+                                       vy0, vy0);
+        const int64_t pos = j * maxC + k;
+        lv = CollisionLast{i, pos, m.L1, m.L2};
+        if (m.distSq < collisionDistSq && m.dDistSqDt < -kEps) {
+          const double key = 1.0 / m.distSq;
+          if (better(key, i, pos, b.key, b.a, b.b)) {
+            b.key = key;
+            b.a = i;
+            b.b = pos;
+            b.info[0] = i;
+            b.info[1] = c1;
+            b.info[2] = c2;
+            b.info[3] = c2;
+            b.info[4] = lk1;
+            b.info[5] = lk2;
+          }
+        }
+      }
+    }
+  }
+  best[n] = b;
+  last[n] = lv;
+}
+
+// ---- unconnected pairs ----------------------------------------------------
+enum SegF { G_X0 = 0, G_X1 = 3, G_V0 = 6, G_V1 = 9, G_B = 12, G_COUNT = 15 };
+
+// Pack each link's end points, velocities and b (SoA, ld) and its node
+// numbers; flags: bit 0 both ends mobile, bit 1 a skip list is present.
+__global__ void k_collision_pack(int64_t numSeg, int64_t ld, const int64_t* __restrict__ links,
+                                 const double* __restrict__ coord, const double* __restrict__ vel,
+                                 const double* __restrict__ bVec, const int32_t* __restrict__ label,
+                                 const int64_t* __restrict__ skip, int64_t nSkip,
+                                 double* __restrict__ seg, int64_t* __restrict__ nodes,
+                                 int* __restrict__ flags) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= ld) return;
+  if (i >= numSeg) {
+    for (int f = 0; f < G_COUNT; ++f) seg[f * ld + i] = 0.0;
+    nodes[2 * i] = nodes[2 * i + 1] = -1;
+    flags[i] = 0;
+    return;
+  }
+  const int64_t a = links[2 * i], b = links[2 * i + 1];
+  for (int d = 0; d < 3; ++d) {
+    seg[(G_X0 + d) * ld + i] = coord[3 * (a - 1) + d];
+    seg[(G_X1 + d) * ld + i] = coord[3 * (b - 1) + d];
+    seg[(G_V0 + d) * ld + i] = vel[3 * (a - 1) + d];
+    seg[(G_V1 + d) * ld + i] = vel[3 * (b - 1) + d];
+    seg[(G_B + d) * ld + i] = bVec[3 * i + d];
+  }
+  nodes[2 * i] = a;
+  nodes[2 * i + 1] = b;
+  int f = (mobile(label[a - 1]) && mobile(label[b - 1])) ? 1 : 0;
+  if (nSkip > 0) f |= 2;  // checked per pair by binary search
+  flags[i] = f;
+}
+
+constexpr int kCT = 64;  // tile side, one wavefront
+
+// Persistent blocks of one wavefront walk the tiles (I, J >= I) of the
+// i < j triangle; lane l keeps row 64 I + l and pairs it with the 64 columns
+// of block J staged in LDS.  Each lane keeps its best candidate and its last
+// evaluated pair; the wave reduces them and writes one partial per block.
+__global__ __launch_bounds__(kCT) void k_collision_pairs(
+    const double* __restrict__ seg, int64_t ld, int64_t numSeg, const int64_t* __restrict__ nodes,
+    const int* __restrict__ flags, const int64_t* __restrict__ skip, int64_t nSkip,
+    double collisionDistSq, int64_t nTiles, int64_t nb, CollisionCand* __restrict__ bestOut,
+    CollisionLast* __restrict__ lastOut) {
+  __shared__ double sc[G_COUNT][kCT];
+  __shared__ int64_t sn[2][kCT];
+  __shared__ int sf[kCT];
+  __shared__ CollisionCand rb[kCT];
+  __shared__ CollisionLast rl[kCT];
+  const int l = threadIdx.x;
+  CollisionCand best{0.0, -1, -1, {0, 0, 0, 0, 0, 0}};
+  CollisionLast last{-1, -1, 0.0, 0.0};
+  for (int64_t t = blockIdx.x; t < nTiles; t += gridDim.x) {
+    // tile t -> (I, J), J >= I, row-major over the upper triangle
+    int64_t I = (int64_t)((2 * nb + 1 - sqrt((double)(2 * nb + 1) * (2 * nb + 1) - 8.0 * t)) / 2);
+    while (I > 0 && I * nb - I * (I - 1) / 2 > t) --I;
+    while ((I + 1) * nb - (I + 1) * I / 2 <= t) ++I;
+    const int64_t J = I + (t - (I * nb - I * (I - 1) / 2));
+    __syncthreads();
+    {
+      const int64_t q = J * kCT + l;
+      for (int f = 0; f < G_COUNT; ++f) sc[f][l] = seg[f * ld + q];
+      sn[0][l] = nodes[2 * q];
+      sn[1][l] = nodes[2 * q + 1];
+      sf[l] = flags[q];
+    }
+    __syncthreads();
+    const int64_t i = I * kCT + l;
+    const int fi = flags[i];
+    if (i >= numSeg || !(fi & 1)) continue;
+    double x0[3], x1[3], v0[3], v1[3], bi[3];
+    for (int d = 0; d < 3; ++d) {
+      x0[d] = seg[(G_X0 + d) * ld + i];
+      x1[d] = seg[(G_X1 + d) * ld + i];
+      v0[d] = seg[(G_V0 + d) * ld + i];
+      v1[d] = seg[(G_V1 + d) * ld + i];
+      bi[d] = seg[(G_B + d) * ld + i];
+    }
+    const int64_t a1 = nodes[2 * i], a2 = nodes[2 * i + 1];
+    for (int c = 0; c < kCT; ++c) {
+      const int64_t j = J * kCT + c;
+      if (j <= i || j >= numSeg) continue;
+      const int64_t b1 = sn[0][c], b2 = sn[1][c];
+      if (a1 == b1 || a1 == b2 || a2 == b1 || a2 == b2) continue;
+      if (!(sf[c] & 1)) continue;
+      double y0[3], y1[3], w0[3], w1[3], bj[3], sx[3], sy[3], dm[3], dp[3];
+      for (int d = 0; d < 3; ++d) {
+        y0[d] = sc[G_X0 + d][c];
+        y1[d] = sc[G_X1 + d][c];
+        w0[d] = sc[G_V0 + d][c];
+        w1[d] = sc[G_V1 + d][c];
+        bj[d] = sc[G_B + d][c];
+        sx[d] = x1[d] - x0[d];
+        sy[d] = y1[d] - y0[d];
+        dm[d] = bi[d] - bj[d];
+        dp[d] = bi[d] + bj[d];
+      }
+      // no superdislocations (DislocationTopologyCollision.jl:176-184)
+      const double par = dot3(sx, sy);
+      if ((dot3(dm, dm) < kEps && par > kEps) || (dot3(dp, dp) < kEps && par < kEps)) continue;
+      if ((fi & 2) && (sf[c] & 2) && skipped(skip, nSkip, i + 1, j + 1)) continue;
+      const MinDist m = min_distance(x0, x1, y0, y1, v0, v1, w0, w1);
+      if (i + 1 > last.a || (i + 1 == last.a && j + 1 > last.b))
+        last = CollisionLast{i + 1, j + 1, m.L1, m.L2};
+      if (m.distSq < collisionDistSq && m.dDistSqDt < -kEps) {
+        const double key = 1.0 / m.distSq;
+        if (better(key, i + 1, j + 1, best.key, best.a, best.b)) {
+          best.key = key;
+          best.a = i + 1;
+          best.b = j + 1;
+          best.info[0] = a1;
+          best.info[1] = a2;
+          best.info[2] = b1;
+          best.info[3] = b2;
+          best.info[4] = i + 1;
+          best.info[5] = j + 1;
+        }
+      }
+    }
+  }
+  rb[l] = best;
+  rl[l] = last;
+  __syncthreads();
+  if (l == 0) {
+    CollisionCand b = rb[0];
+    CollisionLast v = rl[0];
+    for (int q = 1; q < kCT; ++q) {
+      const CollisionCand& o = rb[q];
+      if (o.a >= 0 && better(o.key, o.a, o.b, b.key, b.a, b.b)) b = o;
+      const CollisionLast& w = rl[q];
+      if (w.a > v.a || (w.a == v.a && w.b > v.b)) v = w;
+    }
+    bestOut[blockIdx.x] = b;
+    lastOut[blockIdx.x] = v;
+  }
+}
+
+// One block of 64 lanes: the best candidate and the last evaluated entry of
+// n partials, lane-strided then over the lanes in order.
+__global__ __launch_bounds__(kCT) void k_collision_final(const CollisionCand* __restrict__ best,
+                                                         const CollisionLast* __restrict__ last,
+                                                         int64_t n, CollisionCand* __restrict__ ob,
+                                                         CollisionLast* __restrict__ ol) {
+  __shared__ CollisionCand rb[kCT];
+  __shared__ CollisionLast rl[kCT];
+  const int l = threadIdx.x;
+  CollisionCand b{0.0, -1, -1, {0, 0, 0, 0, 0, 0}};
+  CollisionLast v{-1, -1, 0.0, 0.0};
+  for (int64_t q = l; q < n; q += kCT) {
+    const CollisionCand& o = best[q];
+    if (o.a >= 0 && better(o.key, o.a, o.b, b.key, b.a, b.b)) b = o;
+    const CollisionLast& w = last[q];
+    if (w.a > v.a || (w.a == v.a && w.b > v.b)) v = w;
+  }
+  rb[l] = b;
+  rl[l] = v;
+  __syncthreads();
+  if (l == 0) {
+    for (int q = 1; q < kCT; ++q) {
+      const CollisionCand& o = rb[q];
+      if (o.a >= 0 && better(o.key, o.a, o.b, b.key, b.a, b.b)) b = o;
+      const CollisionLast& w = rl[q];
+      if (w.a > v.a || (w.a == v.a && w.b > v.b)) v = w;
+    }
+    ob[0] = b;
+    ol[0] = v;
+  }
+}
+
+}  // namespace
+
+void launch_collision_hinge(int64_t numNode, const int64_t* conn, int64_t ldConn,
+                            const int64_t* links, const double* coord, const double* vel,
+                            const int32_t* label, const int64_t* skip, int64_t nSkip,
+                            double collisionDistSq, CollisionCand* best, CollisionLast* last,
+                            hipStream_t st) {
+  if (numNode <= 0) return;
+  hipLaunchKernelGGL(k_collision_hinge, dim3((unsigned)((numNode + 255) / 256)), dim3(256), 0, st,
+                     numNode, conn, ldConn, links, coord, vel, label, skip, nSkip,
+                     collisionDistSq, best, last);
+}
+
+void launch_collision_pack(int64_t numSeg, int64_t ld, const int64_t* links, const double* coord,
+                           const double* vel, const double* bVec, const int32_t* label,
+                           const int64_t* skip, int64_t nSkip, double* seg, int64_t* nodes,
+                           int* flags, hipStream_t st) {
+  if (ld <= 0) return;
+  hipLaunchKernelGGL(k_collision_pack, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, st,
+                     numSeg, ld, links, coord, vel, bVec, label, skip, nSkip, seg, nodes, flags);
+}
+
+int collision_pair_blocks(int64_t numSeg) {
+  const int64_t nb = (numSeg + kCT - 1) / kCT;
+  const int64_t tiles = nb * (nb + 1) / 2;
+  return (int)(tiles < 8192 ? (tiles > 0 ? tiles : 1) : 8192);
+}
+
+void launch_collision_pairs(const double* seg, int64_t ld, int64_t numSeg, const int64_t* nodes,
+                            const int* flags, const int64_t* skip, int64_t nSkip,
+                            double collisionDistSq, int blocks, CollisionCand* best,
+                            CollisionLast* last, hipStream_t st) {
+  const int64_t nb = (numSeg + kCT - 1) / kCT;
+  const int64_t tiles = nb * (nb + 1) / 2;
+  hipLaunchKernelGGL(k_collision_pairs, dim3(blocks), dim3(kCT), 0, st, seg, ld, numSeg, nodes,
+                     flags, skip, nSkip, collisionDistSq, tiles, nb, best, last);
+}
+
+void launch_collision_final(const CollisionCand* best, const CollisionLast* last, int64_t n,
+                            CollisionCand* ob, CollisionLast* ol, hipStream_t st) {
+  hipLaunchKernelGGL(k_collision_final, dim3(1), dim3(kCT), 0, st, best, last, n, ob, ol);
+}
+
+}  // namespace ddd

@@ -129,6 +129,34 @@ void launch_u_tilde(const double* tri, int64_t tld, int ntri, const double* pts,
                     hipStream_t st);
 void launch_chunk_sum(const double* partial, int nChunk, int64_t count, double* out,
                       int accumulate, hipStream_t st);
+// Collision detection (collision.hip).  A candidate: key = 1/distSq, loop
+// position (a, b), and the reference's (n1s1, n2s1, n1s2, n2s2, s1, s2); a
+// last-evaluated entry: loop position and that pair's L1, L2.  a < 0: none.
+struct CollisionCand {
+  double key;
+  int64_t a, b;
+  int64_t info[6];
+};
+struct CollisionLast {
+  int64_t a, b;
+  double L1, L2;
+};
+void launch_collision_hinge(int64_t numNode, const int64_t* conn, int64_t ldConn,
+                            const int64_t* links, const double* coord, const double* vel,
+                            const int32_t* label, const int64_t* skip, int64_t nSkip,
+                            double collisionDistSq, CollisionCand* best, CollisionLast* last,
+                            hipStream_t st);
+void launch_collision_pack(int64_t numSeg, int64_t ld, const int64_t* links, const double* coord,
+                           const double* vel, const double* bVec, const int32_t* label,
+                           const int64_t* skip, int64_t nSkip, double* seg, int64_t* nodes,
+                           int* flags, hipStream_t st);
+int collision_pair_blocks(int64_t numSeg);
+void launch_collision_pairs(const double* seg, int64_t ld, int64_t numSeg, const int64_t* nodes,
+                            const int* flags, const int64_t* skip, int64_t nSkip,
+                            double collisionDistSq, int blocks, CollisionCand* best,
+                            CollisionLast* last, hipStream_t st);
+void launch_collision_final(const CollisionCand* best, const CollisionLast* last, int64_t n,
+                            CollisionCand* ob, CollisionLast* ol, hipStream_t st);
 void launch_trapezoid_max(const double* coord, const double* init, const double* vel,
                           const double* vel0, double dt, int64_t m, unsigned long long* part,
                           double* out, hipStream_t st);

@@ -44,6 +44,12 @@ class ddd_itime(C.Structure):
     _fields_ = [("dt", C.c_double), ("time", C.c_double), ("step", C.c_int64)]
 
 
+class ddd_collision(C.Structure):
+    _fields_ = [("collision", C.c_int32), ("type", C.c_int32)] + [
+        (n, C.c_int64) for n in ("n1s1", "n2s1", "n1s2", "n2s2", "s1", "s2")] + [
+        ("L1", C.c_double), ("L2", C.c_double)]
+
+
 class ddd_network(C.Structure):
     _fields_ = [("numNode", i64), ("numSeg", i64), ("segIdx", ip), ("segIdxLd", i64),
                 ("coord", dp), ("coordCols", i64), ("bVec", dp), ("bVecCols", i64),
@@ -125,6 +131,9 @@ def lib():
                 "ddd_net_integrate": (C.c_int, [P, C.POINTER(ddd_integ), C.POINTER(ddd_itime),
                                                 C.c_int]),
                 "ddd_net_download_nodes": (C.c_int, [P, dp, dp, dp, i64]),
+                "ddd_detect_collision": (C.c_int, [P, C.c_double, C.POINTER(ddd_network),
+                                                   C.POINTER(C.c_int32), dp, ip, i64,
+                                                   C.POINTER(ddd_collision)]),
                 "ddd_u_tilde": (C.c_int, [P, C.c_double, C.c_double, C.POINTER(ddd_network),
                                           C.POINTER(C.c_int32), dp, dp, dp, dp, i64, dp,
                                           C.c_int]),

@@ -244,6 +244,28 @@ def calc_uTilde_(forceDisplacement, mesh, boundaries, matParams, network, ctx=No
     return u
 
 
+_COLLISION_TYPES = {0: "null", 1: "hinge", 2: "twoLine"}
+
+
+def detectCollision(dlnParams, network, skipSegs=(), ctx=None):
+    """detectCollision (DislocationTopologyCollision.jl:1-222): (collision,
+    type, n1s1, n2s1, n1s2, n2s2, s1, s2, L1, L2) with type "null", "hinge"
+    or "twoLine"; skipSegs: pairs of link numbers to ignore."""
+    ctx = _ctx(ctx)
+    nv = L.NetView(network)
+    lab = np.ascontiguousarray(network.label, dtype=np.int32)
+    vel = np.asfortranarray(network.nodeVel, dtype=np.float64)
+    if len(lab) < nv.coord.shape[1] or vel.shape[1] < nv.coord.shape[1]:
+        raise ValueError("network.label and nodeVel need one entry per coord column")
+    sk = np.ascontiguousarray(np.asarray(list(skipSegs), dtype=np.int64).reshape(-1, 2))
+    res = L.ddd_collision()
+    _call(ctx, "ddd_detect_collision", float(dlnParams.collisionDistSq), nv.s,
+          lab.ctypes.data_as(_c.POINTER(_c.c_int32)), L.ptr(vel, "d"),
+          L.ptr(sk, "i") if len(sk) else None, len(sk), _byref(res))
+    return (bool(res.collision), _COLLISION_TYPES[res.type], res.n1s1, res.n2s1, res.n1s2,
+            res.n2s2, res.s1, res.s2, res.L1, res.L2)
+
+
 def _mobility(dlnParams, matParams, network, ix, nf, nv_out, cols, flags, ctx):
     if dlnParams.mobility != "mobBCC":
         raise NotImplementedError(f"mobility law {dlnParams.mobility!r}: only mobBCC exists")

@@ -62,10 +62,16 @@ class DislocationParameters:
     # in the reference (DislocationMobility.jl:28).
     mobility: str = "mobBCC"
     dragCoeffs: dict = field(default_factory=lambda: {"edge": 1.0, "screw": 2.0, "climb": 1e9})
+    # collision distance; keyword default minSegLen / 2 = coreRad
+    # (DislocationConstructor.jl:57-67), squared as at :95
+    collisionDist: Optional[float] = None
 
     def __post_init__(self):
         self.coreRad = float(self.coreRad)
         self.coreRadSq = self.coreRad ** 2
+        if self.collisionDist is None:
+            self.collisionDist = 2 * self.coreRad / 2
+        self.collisionDistSq = float(self.collisionDist) ** 2
         if self.coreEnergy is None:
             # keyword default, DislocationConstructor.jl:60
             self.coreEnergy = 1 / (4 * math.pi) * math.log(self.coreRad / 0.1)

@@ -289,6 +289,28 @@ int ddd_u_tilde(ddd_ctx* ctx, double omNuInv8pi, double om2NuOmNuInv8pi, const d
                 const int32_t* label, const double* vertices, const double* faceNorm,
                 const double* faceMidPt, const double* points, int64_t P, double* u, int flags);
 
+/* Result of detectCollision (src/Processing/DislocationTopologyCollision.jl:
+ * 1-222): type 0 = :null, 1 = :hinge, 2 = :twoLine; node and link numbers
+ * 1-based (0 when there is no collision). */
+typedef struct ddd_collision {
+  int32_t collision;
+  int32_t type;
+  int64_t n1s1, n2s1, n1s2, n2s2, s1, s2;
+  double L1, L2;
+} ddd_collision;
+
+/* detectCollision: hinges (two links at a mobile node), then, if none
+ * collides, every pair of links without a shared node, both with mobile
+ * ends and not forming a superdislocation; the candidate closer than
+ * sqrt(collisionDistSq) and approaching with the largest 1/distSq wins (the
+ * first in the reference's loop order on ties).  L1, L2 are those of the
+ * last pair evaluated, as the reference returns them.  label: nodeTypeDln
+ * (Int32) and nodeVel (3 x coordCols) per coord column; skipSegs: nSkip
+ * pairs of link numbers (2 x nSkip, either order). */
+int ddd_detect_collision(ddd_ctx* ctx, double collisionDistSq, const ddd_network* net,
+                         const int32_t* label, const double* nodeVel, const int64_t* skipSegs,
+                         int64_t nSkip, ddd_collision* out);
+
 /* ---- Device-resident path (no counterpart in the reference) ----------
  * For callers that evaluate forces many times on one network (time steps,
  * benchmarks): upload once, compute on the device, fetch when needed.

@@ -887,3 +887,169 @@ void orc_u_tilde(const double* k2, int64_t numSeg, const int64_t* links, const d
 }
 
 }  // extern "C"
+
+// ---- detectCollision ------------------------------------------------------
+// minimumDistance (Misc.jl:314-367) and detectCollision
+// (DislocationTopologyCollision.jl:1-222).
+namespace {
+
+struct MinDist {
+  double distSq, dDistSqDt, L1, L2;
+};
+
+MinDist minimum_distance(const V3<double>& x0, const V3<double>& x1, const V3<double>& y0,
+                         const V3<double>& y1, const V3<double>& vx0, const V3<double>& vx1,
+                         const V3<double>& vy0, const V3<double>& vy1) {
+  const double eps = std::numeric_limits<double>::epsilon();
+  const V3<double> s1 = x1 - x0, s2 = y1 - y0, v1 = vx1 - vx0, v2 = vy1 - vy0;
+  const double A = dot(s1, s1);
+  const double B = dot(2.0 * s1, x0 - y0);
+  const double C = dot(2.0 * s1, s2);
+  const double D = dot(2.0 * s2, y0 - x0);
+  const double E = dot(s2, s2);
+  const double G = C * C - 4.0 * A * E;
+  double L1, L2;
+  if (A < eps) {
+    L1 = 0;
+    L2 = E < eps ? 0.0 : -D / E / 2.0;
+  } else if (E < eps) {
+    L2 = 0;
+    L1 = A < eps ? 0.0 : -B / A / 2.0;
+  } else if (std::fabs(G) < eps) {
+    const double d[4] = {dot(y0 - x0, y0 - x0), dot(y1 - x0, y1 - x0), dot(y0 - x1, y0 - x1),
+                         dot(y1 - x1, y1 - x1)};
+    int idx = 1;  // findmin: first minimum, 1-based
+    for (int q = 1; q < 4; ++q)
+      if (d[q] < d[idx - 1]) idx = q + 1;
+    L1 = std::floor(idx / 2.0);
+    L2 = (double)((idx - 1) % 2);
+  } else {
+    L2 = (2.0 * A * D + B * C) / G;
+    L1 = (C * L2 - B) / A / 2.0;
+  }
+  L1 = L1 < 0 ? 0.0 : (L1 > 1 ? 1.0 : L1);
+  L2 = L2 < 0 ? 0.0 : (L2 > 1 ? 1.0 : L2);
+  const V3<double> r = ((x0 + s1 * L1) - y0) - s2 * L2;
+  const V3<double> vr = ((vx0 + v1 * L1) - vy0) - v2 * L2;
+  return MinDist{dot(r, r), 2.0 * dot(vr, r), L1, L2};
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_minimum_distance(const double* x0, const double* x1, const double* y0, const double* y1,
+                          const double* vx0, const double* vx1, const double* vy0,
+                          const double* vy1, double* out) {
+  auto v = [](const double* p) { return mk(p[0], p[1], p[2]); };
+  const MinDist m = minimum_distance(v(x0), v(x1), v(y0), v(y1), v(vx0), v(vx1), v(vy0), v(vy1));
+  out[0] = m.distSq;
+  out[1] = m.dDistSqDt;
+  out[2] = m.L1;
+  out[3] = m.L2;
+}
+
+// detectCollision: res[0..7] = collision (0/1), type (0 none, 1 hinge,
+// 2 twoLine), n1s1, n2s1, n1s2, n2s2, s1, s2; resL = (L1, L2).  skip: nSkip
+// pairs of link numbers.  label: nodeTypeDln per node (Int32).
+void orc_detect_collision(double collisionDistSq, int64_t numNode, int64_t numSeg,
+                          const int64_t* links, const int64_t* connectivity, int64_t ldConn,
+                          const double* coord, const double* nodeVel, const double* bVec,
+                          const int32_t* label, const int64_t* skip, int64_t nSkip, int64_t* res,
+                          double* resL) {
+  const double eps = std::numeric_limits<double>::epsilon();
+  auto X = [&](int64_t n) { return mk(coord[3 * (n - 1)], coord[3 * (n - 1) + 1], coord[3 * (n - 1) + 2]); };
+  auto Vn = [&](int64_t n) {
+    return mk(nodeVel[3 * (n - 1)], nodeVel[3 * (n - 1) + 1], nodeVel[3 * (n - 1) + 2]);
+  };
+  auto mobile = [&](int64_t n) { return label[n - 1] == 1 || label[n - 1] == 3; };
+  auto skipped = [&](int64_t a, int64_t b) {
+    for (int64_t q = 0; q < nSkip; ++q) {
+      const int64_t s1 = skip[2 * q], s2 = skip[2 * q + 1];
+      if ((a == s1 && b == s2) || (a == s2 && b == s1)) return true;
+    }
+    return false;
+  };
+  double best = 0, L1 = 0, L2 = 0;
+  bool collision = false;
+  int64_t type = 0, n1s1 = 0, n2s1 = 0, n1s2 = 0, n2s2 = 0, s1 = 0, s2 = 0;
+  // hinges
+  for (int64_t i = 1; i <= numNode; ++i) {
+    if (!mobile(i)) continue;
+    const int64_t* cn = connectivity + ldConn * (i - 1);
+    for (int64_t j = 1; j <= cn[0]; ++j) {
+      for (int64_t k = j + 1; k <= cn[0]; ++k) {
+        int64_t lk1 = cn[2 * j - 1], lk2 = cn[2 * k - 1];
+        int64_t c1 = links[2 * (lk1 - 1) + (2 - cn[2 * j])];
+        int64_t c2 = links[2 * (lk2 - 1) + (2 - cn[2 * k])];
+        const V3<double> xi = X(i);
+        const V3<double> t1 = xi - X(c1), t2 = xi - X(c2);
+        const double tn1 = dot(t1, t1), tn2 = dot(t2, t2);
+        if (tn1 < tn2) {  // collapse the shorter one into a point
+          std::swap(lk1, lk2);
+          std::swap(c1, c2);
+        }
+        if (skipped(lk1, lk2)) continue;
+        const MinDist m = minimum_distance(xi, X(c1), X(c2), X(c2), Vn(i), Vn(c1), Vn(c2), Vn(c2));
+        L1 = m.L1;
+        L2 = m.L2;
+        if (m.distSq < collisionDistSq && m.dDistSqDt < -eps) {
+          const double key = 1 / m.distSq;
+          if (key > best) {
+            best = key;
+            collision = true;
+            type = 1;
+            n1s1 = i;
+            n2s1 = c1;
+            n1s2 = c2;
+            n2s2 = c2;
+            s1 = lk1;
+            s2 = lk2;
+          }
+        }
+      }
+    }
+  }
+  if (!collision) {
+    for (int64_t i = 1; i <= numSeg; ++i) {
+      const int64_t a1 = links[2 * (i - 1)], a2 = links[2 * (i - 1) + 1];
+      if (!mobile(a1) || !mobile(a2)) continue;
+      const V3<double> x0 = X(a1), x1 = X(a2);
+      const V3<double> bi = mk(bVec[3 * (i - 1)], bVec[3 * (i - 1) + 1], bVec[3 * (i - 1) + 2]);
+      for (int64_t j = i + 1; j <= numSeg; ++j) {
+        const int64_t b1 = links[2 * (j - 1)], b2 = links[2 * (j - 1) + 1];
+        if (a1 == b1 || a1 == b2 || a2 == b1 || a2 == b2) continue;
+        if (!mobile(b1) || !mobile(b2)) continue;
+        const V3<double> y0 = X(b1), y1 = X(b2);
+        const V3<double> bj = mk(bVec[3 * (j - 1)], bVec[3 * (j - 1) + 1], bVec[3 * (j - 1) + 2]);
+        const double par = dot(x1 - x0, y1 - y0);
+        if ((dot(bi - bj, bi - bj) < eps && par > eps) || (dot(bi + bj, bi + bj) < eps && par < eps))
+          continue;
+        if (skipped(i, j)) continue;
+        const MinDist m = minimum_distance(x0, x1, y0, y1, Vn(a1), Vn(a2), Vn(b1), Vn(b2));
+        L1 = m.L1;
+        L2 = m.L2;
+        if (m.distSq < collisionDistSq && m.dDistSqDt < -eps) {
+          const double key = 1 / m.distSq;
+          if (key > best) {
+            best = key;
+            collision = true;
+            type = 2;
+            n1s1 = a1;
+            n2s1 = a2;
+            n1s2 = b1;
+            n2s2 = b2;
+            s1 = i;
+            s2 = j;
+          }
+        }
+      }
+    }
+  }
+  const int64_t r[8] = {collision ? 1 : 0, type, n1s1, n2s1, n1s2, n2s2, s1, s2};
+  for (int q = 0; q < 8; ++q) res[q] = r[q];
+  resL[0] = L1;
+  resL[1] = L2;
+}
+
+}  // extern "C"

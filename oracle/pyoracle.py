@@ -55,6 +55,9 @@ def lib():
                                         dp]),
             "orc_u_tilde": (None, [dp, i64, ip, dp, dp, C.POINTER(C.c_int32), i64, dp, dp, dp,
                                    dp, i64, dp]),
+            "orc_minimum_distance": (None, [dp] * 9),
+            "orc_detect_collision": (None, [d, i64, i64, ip, ip, i64, dp, dp, dp,
+                                            C.POINTER(C.c_int32), ip, i64, ip, dp]),
             "orc_set_libm": (None, [i32]),
             "orc_segseg_near": (None, [dp] + net + [ip, i32, i32, dp, i32]),
             "orc_segseg_near_once": (None, [dp] + net + [ip, i32, i32, ip, dp]),
@@ -485,3 +488,34 @@ def calc_uTilde_(fd, mesh, boundaries, mat, net):
     fd.uTilde[:] = 0.0
     np.add.at(fd.uTilde, np.asarray(boundaries.uDofsDln) - 1, v)
     return fd.uTilde
+
+
+COLLISION_TYPES = {0: "null", 1: "hinge", 2: "twoLine"}
+
+
+def minimumDistance(x0, x1, y0, y1, vx0, vx1, vy0, vy1):
+    """(distSq, dDistSqDt, L1, L2) of two moving segments (Misc.jl:314-367)."""
+    v = [np.ascontiguousarray(a, dtype=np.float64) for a in (x0, x1, y0, y1, vx0, vx1, vy0, vy1)]
+    out = np.zeros(4)
+    lib().orc_minimum_distance(*[_d(a) for a in v], _d(out))
+    return tuple(float(x) for x in out)
+
+
+def detectCollision(dln, net, skipSegs=()):
+    """detectCollision (DislocationTopologyCollision.jl:1-222): (collision,
+    type, n1s1, n2s1, n1s2, n2s2, s1, s2, L1, L2)."""
+    links = _f(net.links, np.int64)
+    conn = _f(net.connectivity, np.int64)
+    coord = _f(net.coord)
+    vel = _f(net.nodeVel)
+    bVec = _f(net.bVec)
+    lab = np.ascontiguousarray(net.label, dtype=np.int32)
+    sk = np.ascontiguousarray(np.asarray(list(skipSegs), dtype=np.int64).reshape(-1, 2))
+    res = np.zeros(8, dtype=np.int64)
+    L = np.zeros(2)
+    lib().orc_detect_collision(C.c_double(dln.collisionDistSq), int(net.numNode[0]),
+                               int(net.numSeg[0]), _i(links), _i(conn), conn.shape[0], _d(coord),
+                               _d(vel), _d(bVec), lab.ctypes.data_as(C.POINTER(C.c_int32)),
+                               _i(sk), len(sk), _i(res), _d(L))
+    return (bool(res[0]), COLLISION_TYPES[int(res[1])], *[int(x) for x in res[2:]],
+            float(L[0]), float(L[1]))

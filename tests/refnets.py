@@ -75,6 +75,27 @@ def utilde_setup():
     return mat, m, bnd, fd, net
 
 
+def collision_params():
+    """DislocationParameters of test/testData/sampleDislocationParameters.json
+    (TopologyTest.jl:6-20): collision distance 157.235."""
+    d = GOLD["file:test/testData/sampleDislocationParameters.json"]["data"]
+    return DislocationParameters(coreRad=float(d["coreRad"]), coreEnergy=float(d["coreEnergy"]),
+                                 collisionDist=float(d["collisionDist"]))
+
+
+def collision_network(k):
+    """Network k of TopologyTest.jl's collision checks: DislocationNetwork
+    with the given links, coordinates, labels and node velocities, then
+    makeConnect! and getSegmentIdx!."""
+    d = GOLD["collision_cases"]["data"]["networks"][k]
+    links = np.asarray(d["links"], dtype=np.int64)
+    net = DislocationNetwork.from_arrays(links, np.asarray(d["bVec"]),
+                                         np.asarray(d["coord"]),
+                                         np.asarray(d["label"]).ravel().astype(np.int64))
+    net.nodeVel[:, :links.max()] = np.asarray(d["nodeVel"])
+    return net
+
+
 def mesh():
     """buildMesh(matParams, femParams) for sampleFEMParameters.json: 2000^3, 5x5x5."""
     f = GOLD["file:test/testData/sampleFEMParameters.json"]["data"]
