@@ -295,6 +295,45 @@ def utilde_leg(args, mat, net, ctx, log):
             "normwise_err_vs_oracle": err, "finite": bool(np.isfinite(u).all())}
 
 
+def collision_leg(args, net, ctx, log):
+    """detectCollision on the benchmark network with seeded N(0, 1) node
+    velocities and collision distance 3 (below every hinge distance of the
+    32-gons, so the scan over all 8.6e9 unconnected pairs decides): wall time
+    of the synchronous ABI call; CPU oracle timed on the first 128 loops and
+    scaled by pair count."""
+    import numpy as np
+    from dddhip import forces as G
+    from dddhip.generate import random_loop_network
+    from dddhip.types import DislocationParameters
+    import pyoracle as O
+    dln = DislocationParameters(coreRad=90.78, collisionDist=3.0)
+    n = int(net.numNode[0])
+    saved = net.nodeVel.copy()
+    net.nodeVel[:, :n] = np.random.default_rng(0xC0).standard_normal((3, n))
+    r = G.detectCollision(dln, net, ctx=ctx)
+    t = []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        r = G.detectCollision(dln, net, ctx=ctx)
+        t.append((time.perf_counter() - t0) * 1e3)
+    net.nodeVel[:] = saved
+    ms = float(np.mean(t))
+    N = int(net.numSeg[0])
+    pairs = N * (N - 1) // 2
+    small = random_loop_network(128, seed=args.seed)
+    ns = int(small.numSeg[0])
+    small.nodeVel[:, :int(small.numNode[0])] = np.random.default_rng(0xC0).standard_normal(
+        (3, int(small.numNode[0])))
+    t0 = time.perf_counter()
+    O.detectCollision(dln, small)
+    cpu = (time.perf_counter() - t0) * 1e3 * pairs / (ns * (ns - 1) // 2)
+    log(f"detectCollision: {N} segments ({pairs} pairs): {ms:.2f} ms, result {r[:2]}, "
+        f"CPU oracle {cpu / 1e3:.0f} s scaled")
+    return {"segments": N, "pairs": pairs, "collision_dist": 3.0, "ms_pcie_inclusive": round(ms, 3),
+            "pairs_per_s": round(pairs / (ms * 1e-3), 1), "result": list(r),
+            "cpu_oracle_ms_scaled": round(cpu, 1), "cpu_sample": f"{ns} segments, 1 thread"}
+
+
 def normwise_local(a, b):
     import numpy as np
     d = max(np.linalg.norm(a), np.linalg.norm(b))
@@ -396,6 +435,8 @@ def main():
                     help="calc_uTilde leg: cantilever mesh of this many elements per side "
                          "(0 skips the leg)")
     ap.add_argument("--utilde-cpu-points", type=int, default=4)
+    ap.add_argument("--no-collision", dest="collision", action="store_false",
+                    help="skip the detectCollision leg")
     ap.add_argument("--no-dynamics", dest="dynamics", action="store_false",
                     help="skip the deriv!/integrate! leg (run with the first hybrid grid)")
     ap.add_argument("--clustered", action="store_true",
@@ -508,6 +549,7 @@ def main():
     sigma = sigma_leg(args, dln, mat, ctx, log) if args.sigma_loops > 0 else None
     clustered = clustered_leg(args, dln, mat, ctx, log) if args.clustered else None
     utilde = utilde_leg(args, mat, net, ctx, log) if args.utilde_grid > 0 else None
+    collision = collision_leg(args, net, ctx, log) if args.collision else None
 
     achieved = FLOPS_PER_PAIR * pairs / (tms * 1e-3) / 1e12
     result = {
@@ -540,6 +582,7 @@ def main():
         "sigma_tilde": sigma,
         "clustered": clustered,
         "u_tilde": utilde,
+        "detect_collision": collision,
         "roofline": {
             "bound": "valu",
             "achieved": round(achieved, 3),

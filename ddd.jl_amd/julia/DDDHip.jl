@@ -371,6 +371,40 @@ function calc_uTilde!(forceDisplacement, mesh, boundaries, matParams, network::D
     return nothing
 end
 
+# detectCollision (src/Processing/DislocationTopologyCollision.jl:1-222) on the
+# device; same arguments and the same 10-tuple result.
+struct Collision
+    collision::Int32
+    type::Int32
+    n1s1::Int64
+    n2s1::Int64
+    n1s2::Int64
+    n2s2::Int64
+    s1::Int64
+    s2::Int64
+    L1::Float64
+    L2::Float64
+end
+
+function detectCollision(dlnParams::DislocationParameters, network::DislocationNetwork,
+                         skipSegs = Vector{Tuple{Int, Int}}())
+    dlnParams.parGPU || return DDD.detectCollision(dlnParams, network, skipSegs)
+    label = Int32.(network.label)
+    skip = Int64[x for p in skipSegs for x in p]          # 2 x nSkip
+    res = Ref(Collision(0, 0, 0, 0, 0, 0, 0, 0, 0.0, 0.0))
+    GC.@preserve network label skip begin
+        nw = Ref(netstruct(network))
+        check(ccall((:ddd_detect_collision, libddd), Cint,
+            (Ptr{Cvoid}, Float64, Ref{Network}, Ptr{Int32}, Ptr{Float64}, Ptr{Int64}, Int64,
+             Ref{Collision}), ctx(), dlnParams.collisionDistSq, nw, pointer(label),
+            pointer(network.nodeVel), isempty(skip) ? C_NULL : pointer(skip),
+            length(skipSegs), res))
+    end
+    r = res[]
+    type = (:null, :hinge, :twoLine)[r.type + 1]
+    return r.collision == 1, type, r.n1s1, r.n2s1, r.n1s2, r.n2s2, r.s1, r.s2, r.L1, r.L2
+end
+
 # deriv! / integrate! (src/Processing/IntegrationProcessing.jl:13-168) with the
 # network on the device: coordinates, velocities and forces stay in HBM across
 # the adaptive step's trial evaluations; results are copied back at the end.
