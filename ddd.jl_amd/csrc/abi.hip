@@ -1153,8 +1153,8 @@ int ddd_detect_collision(ddd_ctx* c, double collisionDistSq, const ddd_network* 
                              c->cLabel.as<int32_t>(), c->cSkip.as<int64_t>(), nSkip,
                              collisionDistSq, c->cBest.as<CollisionCand>(),
                              c->cLast.as<CollisionLast>(), c->st);
-      launch_collision_final(c->cBest.as<CollisionCand>(), c->cLast.as<CollisionLast>(), nn, resB,
-                             resL, c->st);
+      launch_collision_final(c->cBest.as<CollisionCand>(), c->cLast.as<CollisionLast>(), nn,
+                             nullptr, 0, resB, resL, c->st);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(&hb, resB, sizeof(hb), hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipMemcpyAsync(&hl, resL, sizeof(hl), hipMemcpyDeviceToHost, c->st));
@@ -1183,7 +1183,7 @@ int ddd_detect_collision(ddd_ctx* c, double collisionDistSq, const ddd_network* 
       const int64_t ld = ((ns + 63) / 64) * 64;
       const int blocks = collision_pair_blocks(ns);
       c->cB.ensure(sizeof(double) * 3 * ns);
-      c->cSeg.ensure(sizeof(double) * 15 * ld);
+      c->cSeg.ensure(sizeof(double) * 19 * ld);
       c->cNodes.ensure(sizeof(int64_t) * 2 * ld);
       c->cFlags.ensure(sizeof(int) * ld);
       c->cBest.ensure(sizeof(CollisionCand) * blocks);
@@ -1199,7 +1199,7 @@ int ddd_detect_collision(ddd_ctx* c, double collisionDistSq, const ddd_network* 
                              blocks, c->cBest.as<CollisionCand>(), c->cLast.as<CollisionLast>(),
                              c->st);
       launch_collision_final(c->cBest.as<CollisionCand>(), c->cLast.as<CollisionLast>(), blocks,
-                             resB, resL, c->st);
+                             c->cSeg.as<double>(), ld, resB, resL, c->st);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(&pb, resB, sizeof(pb), hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipMemcpyAsync(&pl, resL, sizeof(pl), hipMemcpyDeviceToHost, c->st));

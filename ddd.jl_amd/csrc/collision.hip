@@ -9,7 +9,8 @@
 // and returning L1, L2 of the last pair it evaluated.  Here both scans are
 // reductions with that order made explicit: a candidate is (key, loop
 // position), compared by key, then by position; the last evaluated pair is
-// the largest position, carried with its L1, L2.
+// the largest position; its L1, L2 come with it (hinges) or are computed by
+// the final reduction (pairs, which skip far pairs by bounding spheres).
 #include "kernels.hpp"
 
 namespace ddd {
@@ -172,7 +173,8 @@ __global__ void k_collision_hinge(int64_t numNode, const int64_t* __restrict__ c
 }
 
 // ---- unconnected pairs ----------------------------------------------------
-enum SegF { G_X0 = 0, G_X1 = 3, G_V0 = 6, G_V1 = 9, G_B = 12, G_COUNT = 15 };
+enum SegF { G_X0 = 0, G_X1 = 3, G_V0 = 6, G_V1 = 9, G_B = 12, G_MID = 15, G_HALF = 18,
+            G_COUNT = 19 };
 
 // Pack each link's end points, velocities and b (SoA, ld) and its node
 // numbers; flags: bit 0 both ends mobile, bit 1 a skip list is present.
@@ -198,6 +200,14 @@ __global__ void k_collision_pack(int64_t numSeg, int64_t ld, const int64_t* __re
     seg[(G_V1 + d) * ld + i] = vel[3 * (b - 1) + d];
     seg[(G_B + d) * ld + i] = bVec[3 * i + d];
   }
+  // bounding sphere: midpoint and half length
+  double hs = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    const double pa = coord[3 * (a - 1) + d], pb = coord[3 * (b - 1) + d];
+    seg[(G_MID + d) * ld + i] = 0.5 * (pa + pb);
+    hs += (pb - pa) * (pb - pa);
+  }
+  seg[G_HALF * ld + i] = 0.5 * sqrt(hs);
   nodes[2 * i] = a;
   nodes[2 * i + 1] = b;
   int f = (mobile(label[a - 1]) && mobile(label[b - 1])) ? 1 : 0;
@@ -251,6 +261,10 @@ __global__ __launch_bounds__(kCT) void k_collision_pairs(
       bi[d] = seg[(G_B + d) * ld + i];
     }
     const int64_t a1 = nodes[2 * i], a2 = nodes[2 * i + 1];
+    const double mi[3] = {seg[G_MID * ld + i], seg[(G_MID + 1) * ld + i],
+                          seg[(G_MID + 2) * ld + i]};
+    const double hi = seg[G_HALF * ld + i];
+    const double mia = fabs(mi[0]) + fabs(mi[1]) + fabs(mi[2]);
     for (int c = 0; c < kCT; ++c) {
       const int64_t j = J * kCT + c;
       if (j <= i || j >= numSeg) continue;
@@ -273,9 +287,22 @@ __global__ __launch_bounds__(kCT) void k_collision_pairs(
       const double par = dot3(sx, sy);
       if ((dot3(dm, dm) < kEps && par > kEps) || (dot3(dp, dp) < kEps && par < kEps)) continue;
       if ((fi & 2) && (sf[c] & 2) && skipped(skip, nSkip, i + 1, j + 1)) continue;
-      const MinDist m = min_distance(x0, x1, y0, y1, v0, v1, w0, w1);
+      // evaluated by the reference; its L1, L2 are recomputed at the end
       if (i + 1 > last.a || (i + 1 == last.a && j + 1 > last.b))
-        last = CollisionLast{i + 1, j + 1, m.L1, m.L2};
+        last = CollisionLast{i + 1, j + 1, 0.0, 0.0};
+      // Bounding spheres: every point pair of the two segments is at least
+      // |mi - mj| - hi - hj apart, and the reference's distSq is the squared
+      // distance of one such pair, so a pair beyond collisionDist (with a
+      // margin for the rounding of the bound) cannot be a candidate.
+      {
+        double dmv[3];
+        for (int d = 0; d < 3; ++d) dmv[d] = mi[d] - sc[G_MID + d][c];
+        const double gap = sqrt(dot3(dmv, dmv)) - hi - sc[G_HALF][c];
+        const double mj = fabs(sc[G_MID][c]) + fabs(sc[G_MID + 1][c]) + fabs(sc[G_MID + 2][c]);
+        const double slack = 1e-7 * (mia + mj + hi + sc[G_HALF][c] + 1.0);
+        if (gap - slack > 0.0 && (gap - slack) * (gap - slack) > collisionDistSq) continue;
+      }
+      const MinDist m = min_distance(x0, x1, y0, y1, v0, v1, w0, w1);
       if (m.distSq < collisionDistSq && m.dDistSqDt < -kEps) {
         const double key = 1.0 / m.distSq;
         if (better(key, i + 1, j + 1, best.key, best.a, best.b)) {
@@ -309,18 +336,21 @@ __global__ __launch_bounds__(kCT) void k_collision_pairs(
   }
 }
 
-// One block of 64 lanes: the best candidate and the last evaluated entry of
-// n partials, lane-strided then over the lanes in order.
-__global__ __launch_bounds__(kCT) void k_collision_final(const CollisionCand* __restrict__ best,
-                                                         const CollisionLast* __restrict__ last,
-                                                         int64_t n, CollisionCand* __restrict__ ob,
-                                                         CollisionLast* __restrict__ ol) {
-  __shared__ CollisionCand rb[kCT];
-  __shared__ CollisionLast rl[kCT];
+constexpr int kFinal = 256;
+
+// One block: the best candidate and the last evaluated entry of n partials,
+// thread-strided then over the threads in order.  With seg (the packed
+// links of the pair scan), the last pair's L1, L2 are computed here.
+__global__ __launch_bounds__(kFinal) void k_collision_final(
+    const CollisionCand* __restrict__ best, const CollisionLast* __restrict__ last, int64_t n,
+    const double* __restrict__ seg, int64_t ld, CollisionCand* __restrict__ ob,
+    CollisionLast* __restrict__ ol) {
+  __shared__ CollisionCand rb[kFinal];
+  __shared__ CollisionLast rl[kFinal];
   const int l = threadIdx.x;
   CollisionCand b{0.0, -1, -1, {0, 0, 0, 0, 0, 0}};
   CollisionLast v{-1, -1, 0.0, 0.0};
-  for (int64_t q = l; q < n; q += kCT) {
+  for (int64_t q = l; q < n; q += kFinal) {
     const CollisionCand& o = best[q];
     if (o.a >= 0 && better(o.key, o.a, o.b, b.key, b.a, b.b)) b = o;
     const CollisionLast& w = last[q];
@@ -330,11 +360,21 @@ __global__ __launch_bounds__(kCT) void k_collision_final(const CollisionCand* __
   rl[l] = v;
   __syncthreads();
   if (l == 0) {
-    for (int q = 1; q < kCT; ++q) {
+    for (int q = 1; q < kFinal; ++q) {
       const CollisionCand& o = rb[q];
       if (o.a >= 0 && better(o.key, o.a, o.b, b.key, b.a, b.b)) b = o;
       const CollisionLast& w = rl[q];
       if (w.a > v.a || (w.a == v.a && w.b > v.b)) v = w;
+    }
+    if (seg && v.a >= 0) {
+      const int64_t i = v.a - 1, j = v.b - 1;
+      double p[8][3];
+      const int f[8] = {G_X0, G_X1, G_X0, G_X1, G_V0, G_V1, G_V0, G_V1};
+      for (int e = 0; e < 8; ++e)
+        for (int d = 0; d < 3; ++d) p[e][d] = seg[(f[e] + d) * ld + ((e & 2) ? j : i)];
+      const MinDist m = min_distance(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7]);
+      v.L1 = m.L1;
+      v.L2 = m.L2;
     }
     ob[0] = b;
     ol[0] = v;
@@ -380,8 +420,10 @@ void launch_collision_pairs(const double* seg, int64_t ld, int64_t numSeg, const
 }
 
 void launch_collision_final(const CollisionCand* best, const CollisionLast* last, int64_t n,
-                            CollisionCand* ob, CollisionLast* ol, hipStream_t st) {
-  hipLaunchKernelGGL(k_collision_final, dim3(1), dim3(kCT), 0, st, best, last, n, ob, ol);
+                            const double* seg, int64_t ld, CollisionCand* ob, CollisionLast* ol,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_collision_final, dim3(1), dim3(kFinal), 0, st, best, last, n, seg, ld, ob,
+                     ol);
 }
 
 }  // namespace ddd

@@ -155,8 +155,11 @@ void launch_collision_pairs(const double* seg, int64_t ld, int64_t numSeg, const
                             const int* flags, const int64_t* skip, int64_t nSkip,
                             double collisionDistSq, int blocks, CollisionCand* best,
                             CollisionLast* last, hipStream_t st);
+// seg: the packed links of the pair scan (the last pair's L1, L2 are
+// computed from it), or NULL when the entries carry them (hinges).
 void launch_collision_final(const CollisionCand* best, const CollisionLast* last, int64_t n,
-                            CollisionCand* ob, CollisionLast* ol, hipStream_t st);
+                            const double* seg, int64_t ld, CollisionCand* ob, CollisionLast* ol,
+                            hipStream_t st);
 void launch_trapezoid_max(const double* coord, const double* init, const double* vel,
                           const double* vel0, double dt, int64_t m, unsigned long long* part,
                           double* out, hipStream_t st);
