@@ -179,6 +179,45 @@ void orc_segseg_par_faithful(const double* K, int64_t numSeg, const int64_t* seg
 // j > i, and the 12 force components of each pair are added atomically.
 // Used to time the reference's threaded path on a bounded share of a large
 // network; returns the number of pairs evaluated.
+// A tuned CPU design for the same sample of outer rows: dynamic scheduling
+// and per-thread accumulators summed at the end, no atomics.  Timed beside
+// the reference-faithful design as the second CPU baseline.
+int64_t orc_segseg_tuned_rows(const double* K, int64_t numSeg, const int64_t* segIdx, int64_t ld,
+                              const double* coord, const double* bVec, const uint8_t* extSeg,
+                              const int64_t* rows, int64_t nRows, double* segForce,
+                              int nthreads) {
+  SegView s{numSeg, segIdx, ld, coord, bVec, extSeg};
+  SegConsts<double> KK = consts_from<double>(K);
+  int64_t pairs = 0;
+  std::vector<std::vector<double>> priv((size_t)nthreads);
+#pragma omp parallel num_threads(nthreads) reduction(+ : pairs)
+  {
+    std::vector<double>& acc = priv[(size_t)omp_get_thread_num()];
+    acc.assign((size_t)(6 * numSeg), 0.0);
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t r = 0; r < nRows; ++r) {
+      int64_t i = rows[r] - 1;
+      if (s.ext(i)) continue;
+      V3<double> b1 = s.b(i), n11 = s.n1(i), n12 = s.n2(i);
+      double fi[6] = {0, 0, 0, 0, 0, 0};
+      for (int64_t j = i + 1; j < numSeg; ++j) {
+        if (s.ext(j)) continue;
+        Forces4<double> f = seg_seg_force(KK, b1, n11, n12, s.b(j), s.n1(j), s.n2(j));
+        fi[0] += f.f1.x; fi[1] += f.f1.y; fi[2] += f.f1.z;
+        fi[3] += f.f2.x; fi[4] += f.f2.y; fi[5] += f.f2.z;
+        double* pj = acc.data() + 6 * j;
+        pj[0] += f.f3.x; pj[1] += f.f3.y; pj[2] += f.f3.z;
+        pj[3] += f.f4.x; pj[4] += f.f4.y; pj[5] += f.f4.z;
+        ++pairs;
+      }
+      for (int q = 0; q < 6; ++q) acc[(size_t)(6 * i + q)] += fi[q];
+    }
+  }
+  for (const auto& acc : priv)
+    for (size_t e = 0; e < acc.size(); ++e) segForce[e] += acc[e];
+  return pairs;
+}
+
 int64_t orc_segseg_par_rows(const double* K, int64_t numSeg, const int64_t* segIdx, int64_t ld,
                             const double* coord, const double* bVec, const uint8_t* extSeg,
                             const int64_t* rows, int64_t nRows, double* segForce, int nthreads) {

@@ -65,6 +65,7 @@ def lib():
             "orc_segseg_rows_grouped": (None, [dp] + net + [ip, i64, ip, i64, dp, i32]),
             "orc_segseg_rows_binned": (None, [dp] + net + [ip, i64, dp, i32, dp, i32]),
             "orc_segseg_par_rows": (i64, [dp] + net + [ip, i64, dp, i32]),
+            "orc_segseg_tuned_rows": (i64, [dp] + net + [ip, i64, dp, i32]),
             "orc_libm_log": (d, [d]),
             "orc_libm_atan": (d, [d]),
         }
@@ -519,3 +520,16 @@ def detectCollision(dln, net, skipSegs=()):
                                _i(sk), len(sk), _i(res), _d(L))
     return (bool(res[0]), COLLISION_TYPES[int(res[1])], *[int(x) for x in res[2:]],
             float(L[0]), float(L[1]))
+
+
+def segseg_tuned_rows(dln, mat, net, rows, nthreads):
+    """A tuned CPU design on a sample of outer rows (dynamic schedule,
+    per-thread accumulators); returns (segForce 3 x 2 x numSeg, pairs)."""
+    n = _Net(net)
+    K = seg_consts(dln, mat)
+    r = np.ascontiguousarray(rows, dtype=np.int64)
+    out = np.zeros((3, 2, n.numSeg), order="F")
+    pairs = lib().orc_segseg_tuned_rows(_d(K), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord),
+                                        _d(n.bVec), _u(n.ext), _i(r), len(r), _d(out),
+                                        int(nthreads))
+    return out, int(pairs)
