@@ -78,12 +78,16 @@ struct ddd_farfield {
   // near-field tile list
   DevBuf mrank, rankCell, mcount, mstart, segS, key, perm, box, rowCount, rowStart, colCount,
       colStart, list, tileIdx, colList, rowPart, colPart;
+  // near-field kernel staging, and its list of near-parallel pairs (spCap
+  // entries), their per-tile offsets and counts and their forces
+  DevBuf stage, spList, specOff, specCnt, spRes;
+  long long spCap = 1 << 16;
   int64_t nearTiles = 0;
   void release() {
     DevBuf* all[] = {&What, &tw, &cellOf, &counts, &cursor, &start, &tmp, &order, &M, &Mc, &Lc, &L,
                      &counters, &mrank, &rankCell, &mcount, &mstart, &segS, &key, &perm, &box,
                      &rowCount, &rowStart, &colCount, &colStart, &list, &tileIdx, &colList,
-                     &rowPart, &colPart};
+                     &rowPart, &colPart, &stage, &spList, &specOff, &specCnt, &spRes};
     for (DevBuf* b : all) b->release();
   }
 };
@@ -316,7 +320,8 @@ void hybrid_pairs(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsig
   ff->L.ensure(sizeof(float) * kLocal * nc);
   ff->Mc.ensure(sizeof(float2) * kMomPairs * ff->vol);
   ff->Lc.ensure(sizeof(float2) * kLocPairs * ff->vol);
-  ff->counters.ensure(3 * sizeof(unsigned long long));
+  ff->counters.ensure(4 * sizeof(unsigned long long));
+  ff->stage.ensure(sizeof(unsigned short) * (size_t)kNearGrid * kNearStage);
   const int64_t ldS = pad64(n1);
   const int64_t nbS = ldS / kTile;
   ff->mcount.ensure(sizeof(unsigned) * nc);
@@ -330,8 +335,8 @@ void hybrid_pairs(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsig
   ff->colCount.ensure(sizeof(unsigned) * nbS);
   ff->colStart.ensure(sizeof(int) * (nbS + 1));
   ff->tileIdx.ensure(sizeof(int) * nbS * nbS);
-  // near pairs, of which FP64, and the near-field kernel's tile counter
-  HIPCHK(hipMemsetAsync(ff->counters.p, 0, 3 * sizeof(unsigned long long), c->st));
+  // near pairs, of which FP64, and the near-field kernels' tile counters
+  HIPCHK(hipMemsetAsync(ff->counters.p, 0, 4 * sizeof(unsigned long long), c->st));
   const double* seg = c->seg.as<double>();
   launch_cells(seg, c->resLd, (int)n, ff->g, ff->cellOf.as<int>(), ff->counts.as<unsigned>(),
                ff->cursor.as<unsigned>(), ff->start.as<int>(), ff->tmp.as<int>(),
@@ -366,6 +371,8 @@ void hybrid_pairs(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsig
   ff->colList.ensure(sizeof(int) * std::max(tiles, 1));
   ff->rowPart.ensure(sizeof(double) * 6 * kTile * std::max(tiles, 1));
   ff->colPart.ensure(sizeof(double) * 6 * kTile * std::max(tiles, 1));
+  ff->specCnt.ensure(sizeof(int) * std::max(tiles, 1));
+  ff->specOff.ensure(sizeof(long long) * std::max(tiles, 1));
   nl.list = ff->list.as<int2>();
   nl.colList = ff->colList.as<int>();
   launch_near_list_fill(nl, c->st);
@@ -374,9 +381,31 @@ void hybrid_pairs(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsig
   launch_far_field(ff->M.as<float>(), ff->g, ff->What.as<float2>(), ff->tw.as<float2>(),
                    ff->Mc.as<float2>(), ff->Lc.as<float2>(), ff->L.as<float>(), c->st);
   const MixedRule mr = mixed_rule(k, o);
-  launch_near_tiles(ff->segS.as<double>(), ldS, ff->key.as<int>(), ff->rn, ff->list.as<int2>(), tiles,
-                    seg_consts(k), mr, ff->rowPart.as<double>(), ff->colPart.as<double>(), flags,
-                    ff->counters.as<unsigned long long>(), c->st);
+  const SegK<double> K = seg_consts(k);
+  // Near-field tiles; their near-parallel pairs are listed, and the list's
+  // length (read back: a second host synchronisation) sizes their
+  // evaluation.  A list that outgrew its capacity is enlarged and the tiles
+  // run again (a one-off as the network changes).
+  long long total = 0;
+  for (;;) {
+    ff->spList.ensure(sizeof(unsigned long long) * ff->spCap);
+    launch_near_tiles(ff->segS.as<double>(), ldS, ff->key.as<int>(), ff->rn, ff->list.as<int2>(),
+                      tiles, K, mr, ff->rowPart.as<double>(), ff->colPart.as<double>(),
+                      ff->stage.as<unsigned short>(), ff->spList.as<unsigned long long>(), ff->spCap,
+                      ff->specOff.as<long long>(), ff->specCnt.as<int>(),
+                      ff->counters.as<unsigned long long>(), c->st);
+    HIPCHK(hipMemcpyAsync(&total, ff->counters.as<unsigned long long>() + 3, sizeof(total),
+                          hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (total <= ff->spCap) break;
+    ff->spCap = total + total / 4;
+    HIPCHK(hipMemsetAsync(ff->counters.p, 0, 4 * sizeof(unsigned long long), c->st));
+  }
+  ff->spRes.ensure(sizeof(double) * 12 * std::max<long long>(ff->spCap, 1));
+  launch_near_special(ff->segS.as<double>(), ldS, ff->list.as<int2>(), tiles, K,
+                      ff->spList.as<unsigned long long>(), total, ff->spCap,
+                      ff->specOff.as<long long>(), ff->specCnt.as<int>(), ff->spRes.as<double>(),
+                      ff->rowPart.as<double>(), ff->colPart.as<double>(), flags, c->st);
   launch_near_reduce(nl, nc, ff->rowPart.as<double>(), ff->colPart.as<double>(), dev, c->st);
   HIPCHK(hipGetLastError());
 }

@@ -72,9 +72,23 @@ void launch_near_list_count(const double* seg, int64_t ld, int n, const GridDev&
 void launch_near_list_fill(const NearListDev& L, hipStream_t st);
 void launch_near_reduce(const NearListDev& L, int nc, const double* rowPart, const double* colPart,
                         double* out, hipStream_t st);
+// Near-field tiles: workgroups of the main kernel and the staging entries
+// (uint16) each one owns (two per slot of a tile); stage holds
+// kNearGrid * kNearStage entries.  The main kernel lists the near-parallel
+// pairs (spList, capacity spCap; specOff / specCnt per tile) and counts them
+// in counters[3]; counters (four) are zeroed before the call.
+constexpr int kNearGrid = 2048;
+constexpr int kNearStage = 2 * kTile * kTile;
 void launch_near_tiles(const double* segS, int64_t ldS, const int* key, int rn, const int2* list,
                        int nTiles, const SegK<double>& K, const MixedRule& mr, double* rowPart,
-                       double* colPart, unsigned* flags, unsigned long long* counters,
-                       hipStream_t st);
+                       double* colPart, unsigned short* stage, unsigned long long* spList,
+                       long long spCap, long long* specOff, int* specCnt,
+                       unsigned long long* counters, hipStream_t st);
+// The listed near-parallel pairs (total <= spCap): evaluated into res
+// (12 x spCap doubles) and added to their tiles' partial sums.
+void launch_near_special(const double* segS, int64_t ldS, const int2* list, int nTiles,
+                         const SegK<double>& K, const unsigned long long* spList, long long total,
+                         long long spCap, const long long* specOff, const int* specCnt, double* res,
+                         double* rowPart, double* colPart, unsigned* flags, hipStream_t st);
 
 }  // namespace ddd
