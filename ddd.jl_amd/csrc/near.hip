@@ -244,7 +244,7 @@ __global__ __launch_bounds__(kTile, 2) void k_near_tiles(const double* __restric
                    const Seg<double> P = seg_from_lds(si, rl), Q = seg_from_lds(sj, sl);
                    double c;
                    const double omc = one_minus_c2(P.t, Q.t, c);
-                   pair_generic<double, true, true>(K, P, Q, c, omc, F1, F2, F3, F4);
+                   pair_generic<double, true, true, true>(K, P, Q, c, omc, F1, F2, F3, F4);
                  },
                  sres, cacc, rowM, colM, a1, a2);
     }

@@ -75,23 +75,35 @@ template <class T> struct Integ19 {
   T v[19];
 };
 
-template <class T>
-__device__ __forceinline__ void corner_acc(T aSq_dSq, T den, T c, T cSq, T omcSqI, T x, T y,
-                                           T sgn, bool first, Integ19<T>& I) {
+// FAST (approximate paths only): the three reciprocals 1/Ra, 1/(Ra t1),
+// 1/(Ra t2) from one division, and the division by a^2 + d^2 (1-c^2) as a
+// product with its reciprocal aSq_dSqI; other rounding than the reference's.
+template <class T, bool FAST = false>
+__device__ __forceinline__ void corner_acc(T aSq_dSq, T aSq_dSqI, T den, T c, T cSq, T omcSqI, T x,
+                                           T y, T sgn, bool first, Integ19<T>& I) {
   T xSq = x * x;
   T ySq = y * y;
   T Ra = sqrt(aSq_dSq + xSq + ySq + T(2) * x * y * c);
-  T RaInv = T(1) / Ra;
   T Ra_Rd_t1 = Ra + y + x * c;
   T Ra_Rd_t2 = Ra + x + y * c;
+  T RaInv, R1I, R2I;
+  if constexpr (FAST) {
+    const T t12 = Ra_Rd_t1 * Ra_Rd_t2;
+    const T inv = T(1) / (Ra * t12);
+    RaInv = t12 * inv;
+    R1I = Ra_Rd_t2 * inv;
+    R2I = Ra_Rd_t1 * inv;
+  } else {
+    RaInv = T(1) / Ra;
+    R1I = RaInv / Ra_Rd_t1;
+    R2I = RaInv / Ra_Rd_t2;
+  }
   T log1 = ulog(Ra_Rd_t1);
   T log2 = ulog(Ra_Rd_t2);
   T xlog1 = x * log1;
   T ylog2 = y * log2;
-  T R1I = RaInv / Ra_Rd_t1;
   T xR1I = x * R1I;
   T xSqR1I = x * xR1I;
-  T R2I = RaInv / Ra_Rd_t2;
   T yR2I = y * R2I;
   T ySqR2I = y * yR2I;
 
@@ -120,7 +132,8 @@ __device__ __forceinline__ void corner_acc(T aSq_dSq, T den, T c, T cSq, T omcSq
   r[3] = i4;
   r[4] = i5;
   r[5] = i6;
-  r[6] = (i1 - xR1I - yR2I) / aSq_dSq;
+  if constexpr (FAST) r[6] = (i1 - xR1I - yR2I) * aSq_dSqI;
+  else r[6] = (i1 - xR1I - yR2I) / aSq_dSq;
   r[7] = (R1I - c * R2I) * omcSqI;
   r[8] = (R2I - c * R1I) * omcSqI;
   r[9] = (RaInv - c * (xR1I + yR2I + i1)) * omcSqI;
@@ -139,8 +152,9 @@ __device__ __forceinline__ void corner_acc(T aSq_dSq, T den, T c, T cSq, T omcSq
 
 // Generic pair (:893-1146).  W12: forces on segment 1's nodes (F1, F2);
 // W34: forces on segment 2's nodes (F3, F4).  c and omcSq come from the
-// caller (the branch test already computed them).
-template <class T, bool W12, bool W34>
+// caller (the branch test already computed them).  FAST: corner_acc's
+// shared reciprocals (approximate paths only).
+template <class T, bool W12, bool W34, bool FAST = false>
 __device__ __forceinline__ void pair_generic(const SegK<T>& K, const Seg<T>& s1, const Seg<T>& s2,
                                              T c, T omcSq, v3<T>& F1, v3<T>& F2, v3<T>& F3,
                                              v3<T>& F4) {
@@ -178,11 +192,12 @@ __device__ __forceinline__ void pair_generic(const SegK<T>& K, const Seg<T>& s1,
 
   const T aSq_dSq = K.aSq + d * d * omcSq;
   const T den = T(1) / sqrt(omcSq * aSq_dSq);
+  const T aSq_dSqI = FAST ? T(1) / aSq_dSq : T(0);
   Integ19<T> I;
-  corner_acc(aSq_dSq, den, c, cSq, omcSqI, x1, y1, T(1), true, I);
-  corner_acc(aSq_dSq, den, c, cSq, omcSqI, x1, y2, T(-1), false, I);
-  corner_acc(aSq_dSq, den, c, cSq, omcSqI, x2, y1, T(-1), false, I);
-  corner_acc(aSq_dSq, den, c, cSq, omcSqI, x2, y2, T(1), false, I);
+  corner_acc<T, FAST>(aSq_dSq, aSq_dSqI, den, c, cSq, omcSqI, x1, y1, T(1), true, I);
+  corner_acc<T, FAST>(aSq_dSq, aSq_dSqI, den, c, cSq, omcSqI, x1, y2, T(-1), false, I);
+  corner_acc<T, FAST>(aSq_dSq, aSq_dSqI, den, c, cSq, omcSqI, x2, y1, T(-1), false, I);
+  corner_acc<T, FAST>(aSq_dSq, aSq_dSqI, den, c, cSq, omcSqI, x2, y2, T(1), false, I);
   const T* J = I.v - 1;  // 1-based, as the formulas are written
 
   if (W12) {  // segment 1, nodes 1-2: :953-1048
