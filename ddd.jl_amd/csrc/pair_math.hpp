@@ -67,6 +67,73 @@ __device__ __forceinline__ double one_minus_c2(v3<double> t1, v3<double> t2, dou
   return __dsub_rn(1.0, __dmul_rn(c, c));
 }
 
+// Approximate-path helpers (FAST below): a reciprocal and a quotient from
+// the hardware reciprocal refined by two Newton steps, and ddd_libm's log and
+// atan (the same fdlibm reductions and polynomials) with those divisions and
+// without the special-case branches, for positive finite (log) and finite
+// (atan) arguments.  They differ from the IEEE quotient and from ddd_libm by
+// rounding-level amounts; the branch-free code keeps each FP64 pair one
+// long basic block.
+__device__ __forceinline__ double frcp(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return fma(r, fma(-d, r, 1.0), r);
+}
+__device__ __forceinline__ double fdiv(double n, double d) {
+  const double r = frcp(d);
+  const double q = n * r;
+  return fma(fma(-d, q, n), r, q);
+}
+__device__ __forceinline__ double flog(double x) {
+  const bool sub = (__double_as_longlong(x) >> 52) == 0;  // subnormal: scale by 2^54
+  if (sub) x *= 18014398509481984.0;
+  const uint64_t u = (uint64_t)__double_as_longlong(x);
+  uint32_t hx = (uint32_t)(u >> 32) + (0x3ff00000u - 0x3fe6a09eu);
+  const int k = (int)(hx >> 20) - 0x3ff - (sub ? 54 : 0);
+  hx = (hx & 0x000fffffu) + 0x3fe6a09eu;
+  const double f = __longlong_as_double((long long)(((uint64_t)hx << 32) | (u & 0xffffffffull))) - 1.0;
+  const double hfsq = 0.5 * f * f;
+  const double s = fdiv(f, 2.0 + f);
+  const double z = s * s, w = z * z;
+  const double t1 = w * (3.999999999940941908e-01 +
+                         w * (2.222219843214978396e-01 + w * 1.531383769920937332e-01));
+  const double t2 = z * (6.666666666666735130e-01 +
+                         w * (2.857142874366239149e-01 +
+                              w * (1.818357216161805012e-01 + w * 1.479819860511658591e-01)));
+  const double dk = (double)k;
+  return s * (hfsq + (t2 + t1)) + dk * 1.90821492927058770002e-10 - hfsq + f +
+         dk * 6.93147180369123816490e-01;
+}
+__device__ __forceinline__ double fatan(double x) {
+  const double ax = fabs(x);
+  const int id = ax < 0.4375 ? -1 : ax < 0.6875 ? 0 : ax < 1.1875 ? 1 : ax < 2.4375 ? 2 : 3;
+  const double num = id == -1 ? ax : id == 0 ? 2.0 * ax - 1.0 : id == 1 ? ax - 1.0
+                   : id == 2 ? ax - 1.5 : -1.0;
+  const double den = id == -1 ? 1.0 : id == 0 ? 2.0 + ax : id == 1 ? ax + 1.0
+                   : id == 2 ? 1.0 + 1.5 * ax : ax;
+  const double r = fdiv(num, den);
+  const double z = r * r, w = z * z;
+  const double s1 = z * (3.33333333333329318027e-01 +
+                         w * (1.42857142725034663711e-01 +
+                              w * (9.09088713343650656196e-02 +
+                                   w * (6.66107313738753120669e-02 +
+                                        w * (4.97687799461593236017e-02 +
+                                             w * 1.62858201153657823623e-02)))));
+  const double s2 = w * (-1.99999999998764832476e-01 +
+                         w * (-1.11111104054623557880e-01 +
+                              w * (-7.69187620504482999495e-02 +
+                                   w * (-5.83357013379057348645e-02 +
+                                        w * -3.65315727442169155270e-02))));
+  const double hi = id == 0 ? 4.63647609000806093515e-01 : id == 1 ? 7.85398163397448278999e-01
+                  : id == 2 ? 9.82793723247329054082e-01 : 1.57079632679489655800e+00;
+  const double lo = id == 0 ? 2.26987774529616870924e-17 : id == 1 ? 3.06161699786838301793e-17
+                  : id == 2 ? 1.39033110312309984516e-17 : 6.12323399573676603587e-17;
+  double res = id < 0 ? r - r * (s1 + s2) : hi - ((r * (s1 + s2) - lo) - r);
+  res = ax < 3.7252902984e-09 ? ax : res;
+  res = ax >= 7.378697629483821e+19 ? 1.57079632679489655800e+00 + 6.12323399573676603587e-17 : res;
+  return x < 0.0 ? -res : res;
+}
+
 // Sum over the four integration corners of the 19 antiderivatives of
 // SegSegInteg (:1480-1561).  Terms shared by all corners (a^2 + d^2 (1-c^2)
 // and its 1/sqrt) are evaluated once.  Corners are combined in the
@@ -75,9 +142,10 @@ template <class T> struct Integ19 {
   T v[19];
 };
 
-// FAST (approximate paths only): the three reciprocals 1/Ra, 1/(Ra t1),
-// 1/(Ra t2) from one division, and the division by a^2 + d^2 (1-c^2) as a
-// product with its reciprocal aSq_dSqI; other rounding than the reference's.
+// FAST (approximate paths only, T = double): the three reciprocals 1/Ra,
+// 1/(Ra t1), 1/(Ra t2) from one reciprocal, the division by a^2 + d^2 (1-c^2)
+// as a product with its reciprocal aSq_dSqI, flog and fatan; other rounding
+// than the reference's.
 template <class T, bool FAST = false>
 __device__ __forceinline__ void corner_acc(T aSq_dSq, T aSq_dSqI, T den, T c, T cSq, T omcSqI, T x,
                                            T y, T sgn, bool first, Integ19<T>& I) {
@@ -86,20 +154,22 @@ __device__ __forceinline__ void corner_acc(T aSq_dSq, T aSq_dSqI, T den, T c, T 
   T Ra = sqrt(aSq_dSq + xSq + ySq + T(2) * x * y * c);
   T Ra_Rd_t1 = Ra + y + x * c;
   T Ra_Rd_t2 = Ra + x + y * c;
-  T RaInv, R1I, R2I;
+  T RaInv, R1I, R2I, log1, log2;
   if constexpr (FAST) {
     const T t12 = Ra_Rd_t1 * Ra_Rd_t2;
-    const T inv = T(1) / (Ra * t12);
+    const T inv = frcp(Ra * t12);
     RaInv = t12 * inv;
     R1I = Ra_Rd_t2 * inv;
     R2I = Ra_Rd_t1 * inv;
+    log1 = flog(Ra_Rd_t1);
+    log2 = flog(Ra_Rd_t2);
   } else {
     RaInv = T(1) / Ra;
     R1I = RaInv / Ra_Rd_t1;
     R2I = RaInv / Ra_Rd_t2;
+    log1 = ulog(Ra_Rd_t1);
+    log2 = ulog(Ra_Rd_t2);
   }
-  T log1 = ulog(Ra_Rd_t1);
-  T log2 = ulog(Ra_Rd_t2);
   T xlog1 = x * log1;
   T ylog2 = y * log2;
   T xR1I = x * R1I;
@@ -107,7 +177,10 @@ __device__ __forceinline__ void corner_acc(T aSq_dSq, T aSq_dSqI, T den, T c, T 
   T yR2I = y * R2I;
   T ySqR2I = y * yR2I;
 
-  T i1 = T(-2) * den * uatan((T(1) + c) * (Ra + x + y) * den);
+  const T atnArg = (T(1) + c) * (Ra + x + y) * den;
+  T i1;
+  if constexpr (FAST) i1 = T(-2) * den * fatan(atnArg);
+  else i1 = T(-2) * den * uatan(atnArg);
   T c_1 = aSq_dSq * i1;
   T c_5_6 = (c * Ra - c_1) * omcSqI;
   T i2 = (c * log2 - log1) * omcSqI;
@@ -160,7 +233,9 @@ __device__ __forceinline__ void pair_generic(const SegK<T>& K, const Seg<T>& s1,
                                              v3<T>& F4) {
   const v3<T> t1 = s1.t, t2 = s2.t, b1 = s1.b, b2 = s2.b;
   const T cSq = c * c;
-  const T omcSqI = T(1) / omcSq;
+  T omcSqI;
+  if constexpr (FAST) omcSqI = frcp(omcSq);
+  else omcSqI = T(1) / omcSq;
 
   const v3<T> t2ct1 = cross(t2, t1);
   const v3<T> t1ct2 = -t2ct1;
@@ -191,8 +266,11 @@ __device__ __forceinline__ void pair_generic(const SegK<T>& K, const Seg<T>& s1,
   const T y2 = (lim21 - c * lim22) * omcSqI;
 
   const T aSq_dSq = K.aSq + d * d * omcSq;
-  const T den = T(1) / sqrt(omcSq * aSq_dSq);
-  const T aSq_dSqI = FAST ? T(1) / aSq_dSq : T(0);
+  T den;
+  if constexpr (FAST) den = frcp(sqrt(omcSq * aSq_dSq));
+  else den = T(1) / sqrt(omcSq * aSq_dSq);
+  T aSq_dSqI = T(0);
+  if constexpr (FAST) aSq_dSqI = frcp(aSq_dSq);
   Integ19<T> I;
   corner_acc<T, FAST>(aSq_dSq, aSq_dSqI, den, c, cSq, omcSqI, x1, y1, T(1), true, I);
   corner_acc<T, FAST>(aSq_dSq, aSq_dSqI, den, c, cSq, omcSqI, x1, y2, T(-1), false, I);

@@ -1,7 +1,8 @@
 """GPU: the hybrid path's pieces against restatements.
 
-* near field (all FP64, FMA-contracted) against the oracle's near pairs in
-  the same orientation to <= 1e-6 per node (the formula's conditioning);
+* near field (all FP64, FMA-contracted, refined hardware reciprocals)
+  against the oracle's near pairs in the same orientation to <= 1e-6 per
+  node (the formula's conditioning);
   with the mixed rule, <= 2e-5;
 * cell binning identical, moments to FP32 precision, the FFT correlation
   against a direct sum, and the far-field quadrature, with a random operator.
@@ -45,9 +46,10 @@ def test_near_field_exact(prm, Gc, rn):
     cell = H.bin_cells(net, Gc, (0, 0, 0), h)
     assert np.array_equal(ff.debug(2, n), cell.astype(np.int32))
     # like with like: each near pair once, oriented by the kernel's sorted
-    # order.  The near-field kernel is built with FMA contraction (the hybrid
-    # is a 1e-3 approximation), so it differs from the uncontracted oracle
-    # within the pair formula's conditioning: ~6e-8 per node here.
+    # order.  The near-field kernel is built with FMA contraction and shared,
+    # refined hardware reciprocals (the hybrid is a 1e-3 approximation), so
+    # it differs from the IEEE oracle within the pair formula's conditioning:
+    # up to ~1.3e-7 per node here, ~4e-9 RMS.
     once = O.segseg_near_once(dln, mat, net, cell, Gc, rn, H.sorted_rank(net, Gc, (0, 0, 0), h))
     mx, rms = per_node_rel(g, once)
     print(f"near G={Gc} rn={rn}: pairs {pairs} exact {exact}; max {mx:.2e} rms {rms:.2e}", flush=True)
