@@ -85,8 +85,10 @@ __device__ __forceinline__ double fdiv(double n, double d) {
   return fma(fma(-d, q, n), r, q);
 }
 __device__ __forceinline__ double flog(double x) {
+  // Every candidate is computed and then selected, so the code has no branch.
   const bool sub = (__double_as_longlong(x) >> 52) == 0;  // subnormal: scale by 2^54
-  if (sub) x *= 18014398509481984.0;
+  const double xs = x * 18014398509481984.0;
+  x = sub ? xs : x;
   const uint64_t u = (uint64_t)__double_as_longlong(x);
   uint32_t hx = (uint32_t)(u >> 32) + (0x3ff00000u - 0x3fe6a09eu);
   const int k = (int)(hx >> 20) - 0x3ff - (sub ? 54 : 0);
@@ -106,11 +108,20 @@ __device__ __forceinline__ double flog(double x) {
 }
 __device__ __forceinline__ double fatan(double x) {
   const double ax = fabs(x);
-  const int id = ax < 0.4375 ? -1 : ax < 0.6875 ? 0 : ax < 1.1875 ? 1 : ax < 2.4375 ? 2 : 3;
-  const double num = id == -1 ? ax : id == 0 ? 2.0 * ax - 1.0 : id == 1 ? ax - 1.0
-                   : id == 2 ? ax - 1.5 : -1.0;
-  const double den = id == -1 ? 1.0 : id == 0 ? 2.0 + ax : id == 1 ? ax + 1.0
-                   : id == 2 ? 1.0 + 1.5 * ax : ax;
+  // reduction intervals about 1/2, 1, 3/2 and infinity (none below 7/16)
+  const bool b0 = ax >= 0.4375, b1 = ax >= 0.6875, b2 = ax >= 1.1875, b3 = ax >= 2.4375;
+  const double n0 = 2.0 * ax - 1.0, d0 = 2.0 + ax;
+  const double n1 = ax - 1.0, d1 = ax + 1.0;
+  const double n2 = ax - 1.5, d2 = 1.0 + 1.5 * ax;
+  double num = ax, den = 1.0, hi = 0.0, lo = 0.0;
+  num = b0 ? n0 : num;  den = b0 ? d0 : den;
+  hi = b0 ? 4.63647609000806093515e-01 : hi;  lo = b0 ? 2.26987774529616870924e-17 : lo;
+  num = b1 ? n1 : num;  den = b1 ? d1 : den;
+  hi = b1 ? 7.85398163397448278999e-01 : hi;  lo = b1 ? 3.06161699786838301793e-17 : lo;
+  num = b2 ? n2 : num;  den = b2 ? d2 : den;
+  hi = b2 ? 9.82793723247329054082e-01 : hi;  lo = b2 ? 1.39033110312309984516e-17 : lo;
+  num = b3 ? -1.0 : num;  den = b3 ? ax : den;
+  hi = b3 ? 1.57079632679489655800e+00 : hi;  lo = b3 ? 6.12323399573676603587e-17 : lo;
   const double r = fdiv(num, den);
   const double z = r * r, w = z * z;
   const double s1 = z * (3.33333333333329318027e-01 +
@@ -124,11 +135,9 @@ __device__ __forceinline__ double fatan(double x) {
                               w * (-7.69187620504482999495e-02 +
                                    w * (-5.83357013379057348645e-02 +
                                         w * -3.65315727442169155270e-02))));
-  const double hi = id == 0 ? 4.63647609000806093515e-01 : id == 1 ? 7.85398163397448278999e-01
-                  : id == 2 ? 9.82793723247329054082e-01 : 1.57079632679489655800e+00;
-  const double lo = id == 0 ? 2.26987774529616870924e-17 : id == 1 ? 3.06161699786838301793e-17
-                  : id == 2 ? 1.39033110312309984516e-17 : 6.12323399573676603587e-17;
-  double res = id < 0 ? r - r * (s1 + s2) : hi - ((r * (s1 + s2) - lo) - r);
+  const double rs = r * (s1 + s2);
+  const double reduced = hi - ((rs - lo) - r), direct = r - rs;
+  double res = b0 ? reduced : direct;
   res = ax < 3.7252902984e-09 ? ax : res;
   res = ax >= 7.378697629483821e+19 ? 1.57079632679489655800e+00 + 6.12323399573676603587e-17 : res;
   return x < 0.0 ? -res : res;

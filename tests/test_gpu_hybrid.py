@@ -49,7 +49,7 @@ def test_near_field_exact(prm, Gc, rn):
     # order.  The near-field kernel is built with FMA contraction and shared,
     # refined hardware reciprocals (the hybrid is a 1e-3 approximation), so
     # it differs from the IEEE oracle within the pair formula's conditioning:
-    # up to ~1.3e-7 per node here, ~4e-9 RMS.
+    # up to ~2.6e-7 per node here, ~5e-9 RMS.
     once = O.segseg_near_once(dln, mat, net, cell, Gc, rn, H.sorted_rank(net, Gc, (0, 0, 0), h))
     mx, rms = per_node_rel(g, once)
     print(f"near G={Gc} rn={rn}: pairs {pairs} exact {exact}; max {mx:.2e} rms {rms:.2e}", flush=True)
