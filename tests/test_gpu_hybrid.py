@@ -14,7 +14,7 @@ import hybrid_ref as H
 import pyoracle as O
 import refnets as R
 from dddhip import forces as G
-from dddhip.generate import random_loop_network
+from dddhip.generate import clustered_loop_network, random_loop_network
 from metrics import per_node_rel
 
 pytestmark = pytest.mark.gpu
@@ -148,3 +148,29 @@ def test_hybrid_resident_matches_one_shot_and_repeats(prm):
     assert np.array_equal(a, b)
     assert np.allclose(a - self_, one, rtol=0, atol=1e-12 * np.abs(one).max())
     res.set_mode("fp64")
+
+
+def test_near_parallel_list_growth(prm):
+    """A pile-up of same-slip-system loops: more near-parallel pairs than the
+    near-parallel list first holds (65,536), so the first evaluation enlarges
+    the list and runs the tiles again.  Both evaluations agree bit for bit,
+    and with the oracle's near pairs as the other tests do."""
+    dln, mat = prm
+    net = clustered_loop_network(numLoops=64, clusters=1, cluster_frac=1.0, sigma=60.0, seed=0xDA)
+    n = int(net.numSeg[0])
+    L = net.links[:, :n] - 1
+    t = net.coord[:, L[1]] - net.coord[:, L[0]]
+    t /= np.linalg.norm(t, axis=0)
+    c = t.T @ t
+    par = int(np.triu(1.0 - c * c <= 1.4901161193847656e-08, 1).sum())
+    assert par > 1 << 16
+    Gc, h, rn = 4, 500.0, 3  # every pair is near
+    ff = G.FarField(Gc, (0.0, 0.0, 0.0), h, rn)
+    ff.set_operator(np.zeros(((2 * Gc - 1) ** 3, 60, 90), np.float32))
+    g1, pairs, exact = G.calcSegSegForceHybrid(dln, mat, net, ff, farDist=1e30)
+    g2, _, _ = G.calcSegSegForceHybrid(dln, mat, net, ff, farDist=1e30)
+    assert pairs == n * (n - 1) // 2 and exact == pairs
+    assert np.array_equal(g1, g2)
+    mx, rms = per_node_rel(g1, _near_once(dln, mat, net, Gc, (0.0, 0.0, 0.0), h, rn))
+    print(f"pile-up: {n} segments, {par} near-parallel pairs; max {mx:.2e} rms {rms:.2e}", flush=True)
+    assert mx <= 1e-6
