@@ -727,9 +727,10 @@ int ddd_sigma_tilde(ddd_ctx* c, const ddd_consts* k, const ddd_network* net, con
         h[(6 + e) * ld + q] = net->bVec[3 * (lk - 1) + e];
       }
     }
-    c->sseg.ensure(sizeof(double) * 12 * ld);
+    c->sseg.ensure(sizeof(double) * kSigmaFields * ld);
     HIPCHK(hipMemcpyAsync(c->sseg.p, h, sizeof(double) * 12 * ld, hipMemcpyHostToDevice, c->st));
-    launch_prep_sigma_segments(c->sseg.as<double>(), ld, (int)m, c->st);
+    const SigmaK K{k->coreRadSq, k->mu4pi, k->mu4pinu, k->nuMu4pinu, k->coreRadSq * k->mu8pi};
+    launch_prep_sigma_segments(c->sseg.as<double>(), ld, (int)m, K, c->st);
     if (P == 0) {
       HIPCHK(hipStreamSynchronize(c->st));
       return;
@@ -743,7 +744,6 @@ int ddd_sigma_tilde(ddd_ctx* c, const ddd_consts* k, const ddd_network* net, con
     const int nChunk = m > 0 ? sigma_chunks_for(P, (int)m) : 1;
     c->sigPart.ensure(sizeof(double) * 6 * P * nChunk);
     if (m > 0) {
-      const SigmaK K{k->coreRadSq, k->mu4pi, k->mu4pinu, k->nuMu4pinu, k->coreRadSq * k->mu8pi};
       launch_sigma_tilde(c->sseg.as<double>(), ld, (int)m, c->pts.as<double>(), P, K,
                          c->sigPart.as<double>(), nChunk, c->st);
     } else {

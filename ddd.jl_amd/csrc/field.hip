@@ -118,11 +118,20 @@ __global__ void k_pk_force(const double* __restrict__ seg, int64_t ld,
   }
 }
 
-// Field-point stress: 12-field SoA per segment (n1, n2, b, t) with t the
-// normalised tangent of :133.
-enum { S_N1 = 0, S_N2 = 3, S_B = 6, S_T = 9, S_COUNT = 12 };
+// Field-point stress: SoA per segment of n1, n2, b (uploaded), t, the
+// normalised tangent of :133, t × b, and three per-segment factors of the six
+// stress components q = (xx, yy, zz, xy, xz, yz) = (a, b') (the component
+// loop of :227-262): t_a t_b', the I13 coefficient -ν μ/(4π(1-ν)) (t⊗(t×b))_q
+// and the first term a² μ/(8π) (t⊗(t×b))_q of I15.  They depend on the
+// segment only, so the pair kernel reads them instead of recomputing them at
+// every point; every value is the same IEEE expression either way.
+enum { S_N1 = 0, S_N2 = 3, S_B = 6, S_T = 9, S_TXB = 12, S_TMT = 15, S_I13 = 21, S_I15 = 27,
+       S_COUNT = 33 };
+static_assert(S_COUNT == kSigmaFields, "sigma segment SoA");
 
-__global__ void k_prep_sigma_segments(double* __restrict__ s, int64_t ld, int n) {
+constexpr int kSigQa[6] = {0, 1, 2, 0, 0, 1}, kSigQb[6] = {0, 1, 2, 1, 2, 2};
+
+__global__ void k_prep_sigma_segments(double* __restrict__ s, int64_t ld, int n, SigmaK K) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double dx = s[(S_N2 + 0) * ld + i] - s[(S_N1 + 0) * ld + i];
@@ -130,13 +139,28 @@ __global__ void k_prep_sigma_segments(double* __restrict__ s, int64_t ld, int n)
   const double dz = s[(S_N2 + 2) * ld + i] - s[(S_N1 + 2) * ld + i];
   const double nn = __dsqrt_rn(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)),
                                          __dmul_rn(dz, dz)));
-  s[(S_T + 0) * ld + i] = __ddiv_rn(dx, nn);
-  s[(S_T + 1) * ld + i] = __ddiv_rn(dy, nn);
-  s[(S_T + 2) * ld + i] = __ddiv_rn(dz, nn);
+  const v3<double> t = V(__ddiv_rn(dx, nn), __ddiv_rn(dy, nn), __ddiv_rn(dz, nn));
+  s[(S_T + 0) * ld + i] = t.x;
+  s[(S_T + 1) * ld + i] = t.y;
+  s[(S_T + 2) * ld + i] = t.z;
+  const v3<double> b = V(s[(S_B + 0) * ld + i], s[(S_B + 1) * ld + i], s[(S_B + 2) * ld + i]);
+  const v3<double> txb = cross(t, b);
+  s[(S_TXB + 0) * ld + i] = txb.x;
+  s[(S_TXB + 1) * ld + i] = txb.y;
+  s[(S_TXB + 2) * ld + i] = txb.z;
+  const double tt[3] = {t.x, t.y, t.z}, xb[3] = {txb.x, txb.y, txb.z};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int a = kSigQa[q], bq = kSigQb[q];
+    const double tmtxb = q < 3 ? 2.0 * tt[a] * xb[a] : tt[a] * xb[bq] + tt[bq] * xb[a];
+    s[(S_TMT + q) * ld + i] = tt[a] * tt[bq];
+    s[(S_I13 + q) * ld + i] = -K.nuMu4pinu * tmtxb;
+    s[(S_I15 + q) * ld + i] = K.a2mu8pi * tmtxb;
+  }
 }
 
 constexpr int kSigBlock = 256;
-constexpr int kSigTile = 128;
+constexpr int kSigTile = 64;
 
 // Stress of segments [c * chunk, (c + 1) * chunk) at one point per thread,
 // summed in segment order (the reference's order for each point).
@@ -166,7 +190,7 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_tilde(const double* __restr
       const v3<double> n2 = V(sh[S_N2][l], sh[S_N2 + 1][l], sh[S_N2 + 2][l]);
       const v3<double> b = V(sh[S_B][l], sh[S_B + 1][l], sh[S_B + 2][l]);
       const v3<double> t = V(sh[S_T][l], sh[S_T + 1][l], sh[S_T + 2][l]);
-      // DislocationFEM.jl:142-299
+      // DislocationFEM.jl:142-299 (per-segment factors from k_prep_sigma_segments)
       const v3<double> R = x - n1;
       const double Rdt = dot(R, t);
       const v3<double> nd = R - Rdt * t;
@@ -195,27 +219,24 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_tilde(const double* __restr
       const double s25b = s03b - sRa3inv;
       const double s03 = s03b - s03a, s13 = s13b - s13a, s05 = s05b - s05a, s15 = s15b - s15a,
                    s25 = s25b - s25a;
-      const v3<double> txb = cross(t, b);
       const v3<double> dxb = cross(nd, b);
       const double common = K.mu4pinu * dot(dxb, t);
       const double tt[3] = {t.x, t.y, t.z}, dd[3] = {nd.x, nd.y, nd.z};
-      const double xb[3] = {txb.x, txb.y, txb.z}, db[3] = {dxb.x, dxb.y, dxb.z};
-      const int PP[6] = {0, 1, 2, 0, 0, 1}, QQ[6] = {0, 1, 2, 1, 2, 2};
+      const double xb[3] = {sh[S_TXB][l], sh[S_TXB + 1][l], sh[S_TXB + 2][l]};
+      const double db[3] = {dxb.x, dxb.y, dxb.z};
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
-        const int a = PP[q], bq = QQ[q];
+        const int a = kSigQa[q], bq = kSigQb[q];
         const bool dg = q < 3;
         const double dmd = dd[a] * dd[bq];
-        const double tmt = tt[a] * tt[bq];
         const double tmd = dg ? 2.0 * tt[a] * dd[a] : tt[a] * dd[bq] + tt[bq] * dd[a];
-        const double tmtxb = dg ? 2.0 * tt[a] * xb[a] : tt[a] * xb[bq] + tt[bq] * xb[a];
         const double dmtxb = dg ? 2.0 * dd[a] * xb[a] : dd[a] * xb[bq] + dd[bq] * xb[a];
         const double tmdxb = dg ? 2.0 * tt[a] * db[a] : tt[a] * db[bq] + tt[bq] * db[a];
         const double I03 = (dg ? common : 0.0) + K.mu4pinu * dmtxb - K.mu4pi * tmdxb;
-        const double I13 = -K.nuMu4pinu * tmtxb;
+        const double I13 = sh[S_I13 + q][l];
         const double I05 = common * ((dg ? K.a2 : 0.0) + dmd) - K.a2mu8pi * tmdxb;
-        const double I15 = K.a2mu8pi * tmtxb - common * tmd;
-        const double I25 = common * tmt;
+        const double I15 = sh[S_I15 + q][l] - common * tmd;
+        const double I25 = common * sh[S_TMT + q][l];
         sg[q] += I03 * s03 + I13 * s13 + I05 * s05 + I15 * s15 + I25 * s25;
       }
     }
@@ -409,9 +430,9 @@ void launch_pk_force(const double* seg, int64_t ld, const int64_t* rows, int n, 
                      out, mode);
 }
 
-void launch_prep_sigma_segments(double* sseg, int64_t ld, int n, hipStream_t st) {
+void launch_prep_sigma_segments(double* sseg, int64_t ld, int n, const SigmaK& K, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_prep_sigma_segments, dim3((n + 255) / 256), dim3(256), 0, st, sseg, ld, n);
+  hipLaunchKernelGGL(k_prep_sigma_segments, dim3((n + 255) / 256), dim3(256), 0, st, sseg, ld, n, K);
 }
 
 int sigma_chunks_for(int64_t P, int nseg) {

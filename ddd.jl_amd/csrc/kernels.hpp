@@ -90,14 +90,16 @@ void launch_pk_force(const double* seg, int64_t ld, const int64_t* rows, int n, 
 struct SigmaK {
   double a2, mu4pi, mu4pinu, nuMu4pinu, a2mu8pi;
 };
-// Stress at P points (3 x P) from nseg segments given as a 12-field SoA
-// (n1, n2, b, t); partial holds nChunk x P x 6 doubles.
+// Stress at P points (3 x P) from nseg segments given as a kSigmaFields-field
+// SoA (n1, n2, b uploaded; the rest filled by launch_prep_sigma_segments);
+// partial holds nChunk x P x 6 doubles.
 int sigma_chunks_for(int64_t P, int nseg);
 void launch_sigma_tilde(const double* sseg, int64_t sld, int nseg, const double* x0, int64_t P,
                         const SigmaK& K, double* partial, int nChunk, hipStream_t st);
 void launch_sigma_reduce(const double* partial, int nChunk, int64_t P, double* sigma,
                          int accumulate, hipStream_t st);
-void launch_prep_sigma_segments(double* sseg, int64_t ld, int n, hipStream_t st);
+constexpr int kSigmaFields = 33;
+void launch_prep_sigma_segments(double* sseg, int64_t ld, int n, const SigmaK& K, hipStream_t st);
 
 void launch_node_gather(const int64_t* conn, int64_t ldConn, int64_t numNode,
                         const int64_t* links, const double* coord, const double* segForce,
