@@ -177,7 +177,9 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
       v3<double> F1, F2, F3, F4;
       double c;
       const double omc = one_minus_c2(P.t, Q.t, c);
-      pair_generic<double, true, true>(K, P, Q, c, omc, F1, F2, F3, F4);
+      // the queue is used in mixed mode only: an approximate path, so the
+      // pair takes the FAST corner integrals
+      pair_generic<double, true, true, true>(K, P, Q, c, omc, F1, F2, F3, F4);
       sres[0][lane] = F1.x; sres[1][lane] = F1.y; sres[2][lane] = F1.z;
       sres[3][lane] = F2.x; sres[4][lane] = F2.y; sres[5][lane] = F2.z;
       sres[6][lane] = F3.x; sres[7][lane] = F3.y; sres[8][lane] = F3.z;
