@@ -78,7 +78,7 @@ def hybrid_paths(args, dln, mat, net, res, stream, ref, fp64_ms, log):
         for r in rns[1:]:
             G.setFarFieldFromMLP(dln, mat, ffs[r], mlp)
         for r in rns:
-            res.set_mode("hybrid", farfield=ffs[r])
+            res.set_mode("hybrid", farfield=ffs[r], farDist=args.near_fp64_radii * dln.coreRad)
             res.seg_force()
             stream.synchronize()
             ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)]
@@ -93,7 +93,8 @@ def hybrid_paths(args, dln, mat, net, res, stream, ref, fp64_ms, log):
             near, near64 = res.stats()
             smx, srms = per_node_rel(got, ref)
             nmx, nrms = per_node_rel(node_forces(net, got), Fref)
-            row = {"G": Gc, "rn": r, "h": h, "ms": round(hms, 3),
+            row = {"G": Gc, "rn": r, "h": h, "fp64_core_radii": args.near_fp64_radii,
+                   "ms": round(hms, 3),
                    "speedup_vs_fp64": round(fp64_ms / hms, 2),
                    "node_rel_err_rms": nrms, "node_rel_err_max": nmx,
                    "segend_rel_err_rms": srms, "segend_rel_err_max": smx,
@@ -422,6 +423,9 @@ def main():
                     help="comma list of G:rn hybrid configurations (G^3 cells over the "
                          "network's bounding cube, near stencil of Chebyshev radius rn); "
                          "empty to skip")
+    ap.add_argument("--near-fp64-radii", type=float, default=5.0,
+                    help="hybrid near field: pairs closer than this many core radii (or "
+                         "near-parallel) run in FP64, the rest of the stencil in FP32")
     ap.add_argument("--train-loops", type=int, default=4096)
     ap.add_argument("--train-seed", type=lambda s: int(s, 0), default=0xD1)
     ap.add_argument("--train-steps", type=int, default=60000)
