@@ -302,9 +302,11 @@ void check_opts(const ddd_opts* o) {
 bool is_hybrid(const ddd_opts* o) { return o && o->mode == DDD_MODE_HYBRID; }
 
 // Hybrid pair stage: cells, the near-field tile list, moments, far-field
-// expansions, near-field tiles and their reduction (adds into dev).  The
-// far-field forces are added by hybrid_apply.  One host synchronisation
-// reads the number of near tiles, which sizes the partial-sum buffers.
+// expansions, near-field tiles, their near-parallel pairs and their
+// reduction (adds into dev).  The far-field forces are added by
+// hybrid_apply.  Two host synchronisations: the number of near tiles sizes
+// the partial-sum buffers, the length of the near-parallel list sizes its
+// evaluation.
 void hybrid_pairs(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsigned* flags,
                   const ddd_opts& o) {
   ddd_farfield* ff = o.farfield;
