@@ -247,10 +247,10 @@ __global__ void k_spectral_apply(const float2* __restrict__ What, const float2* 
   }
   __syncthreads();
   if (q < kLocal) {
-    const float2* w = What + (fh * kLocal + q) * kMoments;
+    const float2* w = What + fh * kChan + q;
     float ax = 0.f, ay = 0.f;
     for (int k = 0; k < kMoments; ++k) {
-      const float2 a = w[k], b = m[k];
+      const float2 a = w[k * kLocal], b = m[k];
       ax += a.x * b.x - a.y * b.y;
       ay += a.x * b.y + a.y * b.x;
     }
@@ -302,15 +302,18 @@ __global__ void k_place_operator(const float* __restrict__ W, int G, int NP, int
   out[e] = v;
 }
 
-// Packed channel-major spectrum -> [half frequency][ch] of the real
-// channels (f_z in [0, NP/2]).
+// Packed channel-major spectrum -> [half frequency][moment k][local q] of
+// the real channels ch = q * kMoments + k (f_z in [0, NP/2]): per frequency
+// the operator is stored column by column, so the 60 lanes of
+// k_spectral_apply read one column, 480 contiguous bytes, per step.
 __global__ void k_transpose_spectrum(const float2* __restrict__ in, int NP,
                                      float2* __restrict__ out) {
   const int64_t volH = (int64_t)NP * NP * (NP / 2 + 1);
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= (int64_t)kChan * volH) return;
   const int64_t fh = e / kChan;
-  const int ch = (int)(e % kChan);
+  const int s = (int)(e % kChan);
+  const int ch = (s % kLocal) * kMoments + s / kLocal;
   const int fx = (int)(fh % NP), fy = (int)((fh / NP) % NP), fz = (int)(fh / ((int64_t)NP * NP));
   const int64_t vol = (int64_t)NP * NP * NP;
   const int64_t f = fx + (int64_t)NP * (fy + (int64_t)NP * fz);

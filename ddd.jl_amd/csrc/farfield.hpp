@@ -30,7 +30,7 @@ void launch_moments(const double* seg, int64_t ld, const GridDev& g, const int* 
                     const int* order, float* M, hipStream_t st);
 void launch_far_field(const float* M, const GridDev& g, const float2* What, const float2* tw,
                       float2* Mc, float2* Lc, float* L, hipStream_t st);
-// What: [half frequency][60][90] complex, f_z in [0, NP/2] (NP^2 (NP/2 + 1)
+// What: [half frequency][90 moments][60 locals] complex, f_z in [0, NP/2] (NP^2 (NP/2 + 1)
 // frequencies); scratch: kChanPairs NP^3 complex; Mc: kMomPairs NP^3;
 // Lc: kLocPairs NP^3.
 void launch_operator_spectrum(const float* W, const GridDev& g, int rn, const float2* tw,
