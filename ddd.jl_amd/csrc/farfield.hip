@@ -525,11 +525,15 @@ __global__ void k_near_reduce(const double* __restrict__ rowPart, const double* 
   if (p >= start[nc]) return;
   const int B = p / kTile, l = p % kTile;
   double acc[6] = {0, 0, 0, 0, 0, 0};
+  // unrolled so that several tiles' loads are in flight (the sums stay in
+  // list order)
+#pragma unroll 8
   for (int t = rowStart[B]; t < rowStart[B + 1]; ++t) {
     const double* r = rowPart + (int64_t)t * 6 * kTile + l;
 #pragma unroll
     for (int c = 0; c < 6; ++c) acc[c] += r[c * kTile];
   }
+#pragma unroll 8
   for (int q = colStart[B]; q < colStart[B + 1]; ++q) {
     const double* r = colPart + (int64_t)colList[q] * 6 * kTile + l;
 #pragma unroll
