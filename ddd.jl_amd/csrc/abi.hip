@@ -1278,11 +1278,16 @@ void net_stage(ddd_ctx* c, int stage, int withPK) {
     if (withPK) launch_pk_force(c->seg.as<double>(), c->resLd, nullptr, (int)n, c->resM, dev, 1, c->st);
     launch_self_force(c->seg.as<double>(), c->resLd, nullptr, (int)n, self_consts(c->resK), dev, c->st);
   } else if (stage == 1) {
-    if (is_hybrid(&c->opts)) hybrid_pairs(c, c->resK, n, dev, c->flags.as<unsigned>(), c->opts);
-    else segseg_tiles(c, c->resK, n, c->flags.as<unsigned>(), &c->opts);
+    // no pair stage below two segments, as segseg_all
+    if (is_hybrid(&c->opts)) {
+      if (n >= 2) hybrid_pairs(c, c->resK, n, dev, c->flags.as<unsigned>(), c->opts);
+    } else {
+      segseg_tiles(c, c->resK, n, c->flags.as<unsigned>(), &c->opts);
+    }
   } else if (stage == 2) {
-    if (is_hybrid(&c->opts)) hybrid_apply(c, n, dev, c->opts);
-    else if (n >= 2)
+    if (is_hybrid(&c->opts)) {
+      if (n >= 2) hybrid_apply(c, n, dev, c->opts);
+    } else if (n >= 2)
       launch_segseg_reduce(c->rowPart.as<double>(), c->colPart.as<double>(), (int)n, dev, c->st);
   } else if (stage == 3) {
     if (c->dynNodes < 0) bad("no dynamics state (ddd_net_upload_dynamics)");

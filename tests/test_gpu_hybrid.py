@@ -15,6 +15,7 @@ import pyoracle as O
 import refnets as R
 from dddhip import forces as G
 from dddhip.generate import clustered_loop_network, random_loop_network
+from dddhip.types import DislocationNetwork, intMobDln
 from metrics import per_node_rel
 
 pytestmark = pytest.mark.gpu
@@ -174,3 +175,44 @@ def test_near_parallel_list_growth(prm):
     mx, rms = per_node_rel(g1, _near_once(dln, mat, net, Gc, (0.0, 0.0, 0.0), h, rn))
     print(f"pile-up: {n} segments, {par} near-parallel pairs; max {mx:.2e} rms {rms:.2e}", flush=True)
     assert mx <= 1e-6
+
+
+@pytest.mark.parametrize("case", ["empty", "one", "parallel_pair", "one_loop"])
+def test_hybrid_tiny_networks(prm, case):
+    """Networks below one block: no segment, one segment, two parallel
+    segments (the near-parallel list with a single entry) and one 32-sided
+    loop (one tile, its 16 antiparallel pairs).  One-shot and resident
+    evaluations agree with the oracle's near pairs; nothing is added below
+    two segments."""
+    dln, mat = prm
+    b = np.array([1.0, 1.0, 1.0]) / np.sqrt(3.0)
+    if case == "one_loop":
+        net = random_loop_network(1, seed=0xDB)
+    else:
+        nseg = {"empty": 0, "one": 1, "parallel_pair": 2}[case]
+        coord = np.array([[900.0, 1100.0, 900.0, 1100.0], [1000.0, 1000.0, 1040.0, 1040.0],
+                          [1000.0, 1000.0, 1000.0, 1000.0]])[:, :2 * nseg]
+        links = np.array([[1, 3], [2, 4]], dtype=np.int64)[:, :nseg]
+        net = DislocationNetwork.from_arrays(links, np.tile(b[:, None], (1, nseg)), coord,
+                                             [intMobDln] * (2 * nseg))
+    n = int(net.numSeg[0])
+    Gc, h, rn = 8, 250.0, 1
+    ff = G.FarField(Gc, (0.0, 0.0, 0.0), h, rn)
+    ff.set_operator(np.zeros(((2 * Gc - 1) ** 3, 60, 90), np.float32))
+    g, pairs, exact = G.calcSegSegForceHybrid(dln, mat, net, ff, farDist=1e30)
+    assert g.shape == (3, 2, n) and pairs == n * (n - 1) // 2 and exact == pairs
+    res = G.ResidentNetwork(dln, mat, net)
+    res.set_mode("hybrid", farfield=ff)
+    res.seg_force()
+    s1, s2 = O.calcSelfForce(dln, mat, net)
+    self_ = np.zeros((3, 2, n), order="F")
+    self_[:, 0], self_[:, 1] = s1, s2
+    assert np.allclose(res.download() - self_, 0.0 if n < 2 else g, rtol=0,
+                       atol=1e-12 * max(1.0, np.abs(g).max() if n else 1.0))
+    res.set_mode("fp64")
+    if n < 2:
+        assert not g.any()
+        return
+    ref = _near_once(dln, mat, net, Gc, (0.0, 0.0, 0.0), h, rn)
+    assert np.abs(ref).max() > 0
+    assert per_node_rel(g, ref)[0] <= 1e-6
