@@ -449,7 +449,7 @@ def calcSegSegForceHybrid(dlnParams, matParams, network, farfield, farDist=0.0, 
                           ctx=None):
     """All-pairs seg-seg force, hybrid: near pairs exact (mixed rule), far
     pairs through the far-field operator.  Returns (3 x 2 x numSeg forces,
-    ordered near pairs, of which FP64)."""
+    near pairs, each counted once, of which FP64)."""
     ctx = _ctx(ctx)
     nv = L.NetView(network)
     k = L.consts(dlnParams, matParams)

@@ -116,7 +116,7 @@ typedef struct ddd_opts {
 } ddd_opts;
 
 typedef struct ddd_stats {
-  uint64_t pairs;      /* pairs evaluated exactly (hybrid: ordered near pairs) */
+  uint64_t pairs;      /* pairs evaluated exactly (hybrid: near pairs, each once) */
   uint64_t pairsFP64;  /* of which on the FP64 path                            */
 } ddd_stats;
 
