@@ -76,16 +76,20 @@ __global__ __launch_bounds__(kTile) void k_special_pass(const double* __restrict
   int I, J;
   tile_blocks(t, I, J);
   const int lane = threadIdx.x;
-  __shared__ double sj[10][kTile];  // n1, n2, t, ext
+  // staged column fields: t, ext, and (mixed mode, for is_far) n1, n2
+  constexpr int NF = MODE == 0 ? 4 : 10;
+  __shared__ double sj[NF][kTile];
   const int64_t i = (int64_t)I * kTile + lane;
   const int64_t jl = (int64_t)J * kTile + lane;
-  const int fl[10] = {F_N1X, F_N1Y, F_N1Z, F_N2X, F_N2Y, F_N2Z, F_TX, F_TY, F_TZ, F_EXT};
+  const int fl[10] = {F_TX, F_TY, F_TZ, F_EXT, F_N1X, F_N1Y, F_N1Z, F_N2X, F_N2Y, F_N2Z};
 #pragma unroll
-  for (int f = 0; f < 10; ++f) sj[f][lane] = seg[fl[f] * ld + jl];
+  for (int f = 0; f < NF; ++f) sj[f][lane] = seg[fl[f] * ld + jl];
   __syncthreads();
   const v3<double> tA = V(seg[F_TX * ld + i], seg[F_TY * ld + i], seg[F_TZ * ld + i]);
-  const v3<double> midA = mid_rn(V(seg[F_N1X * ld + i], seg[F_N1Y * ld + i], seg[F_N1Z * ld + i]),
-                                 V(seg[F_N2X * ld + i], seg[F_N2Y * ld + i], seg[F_N2Z * ld + i]));
+  v3<double> midA = V(0.0, 0.0, 0.0);
+  if constexpr (MODE != 0)
+    midA = mid_rn(V(seg[F_N1X * ld + i], seg[F_N1Y * ld + i], seg[F_N1Z * ld + i]),
+                  V(seg[F_N2X * ld + i], seg[F_N2Y * ld + i], seg[F_N2Z * ld + i]));
   const bool rowLive = (i < n) && seg[F_EXT * ld + i] == 0.0;
   const bool diag = (I == J);
   const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -94,11 +98,15 @@ __global__ __launch_bounds__(kTile) void k_special_pass(const double* __restrict
   for (int k = 0; k < kTile; ++k) {
     const int s = (lane + k) & 63;
     const int64_t j = (int64_t)J * kTile + s;
-    const bool live = rowLive && (j < n) && sj[9][s] == 0.0 && (!diag || s > lane);
+    const bool live = rowLive && (j < n) && sj[3][s] == 0.0 && (!diag || s > lane);
     double cc;
-    const double omcSq = one_minus_c2(tA, V(sj[6][s], sj[7][s], sj[8][s]), cc);
-    const bool sp = is_special<MODE>(live, omcSq, midA, V(sj[0][s], sj[1][s], sj[2][s]),
-                                     V(sj[3][s], sj[4][s], sj[5][s]), farDist2, sinMin2);
+    const double omcSq = one_minus_c2(tA, V(sj[0][s], sj[1][s], sj[2][s]), cc);
+    v3<double> n1B = V(0.0, 0.0, 0.0), n2B = n1B;
+    if constexpr (MODE != 0) {
+      n1B = V(sj[4][s], sj[5][s], sj[6][s]);
+      n2B = V(sj[7][s], sj[8][s], sj[9][s]);
+    }
+    const bool sp = is_special<MODE>(live, omcSq, midA, n1B, n2B, farDist2, sinMin2);
     const uint64_t m = __ballot(sp);
     if (LIST && sp) {
       const int pos = c + __popcll(m & ltMask);
