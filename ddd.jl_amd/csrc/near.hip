@@ -1,11 +1,12 @@
 // Near-field tile kernels of the hybrid path (no reference counterpart).
 //
-// This file is compiled with floating-point contraction enabled: the hybrid
-// is an approximation measured against the FP64 path (its far field is good
-// to ~1e-3), so its near field is not held to the FP64 path's operation-for-
-// operation agreement with the oracle.  Fused multiply-adds move a near
-// pair's forces within the pair formula's own conditioning (up to 6e-8 per
-// node against the uncontracted oracle at separations of ~500) and remove
+// This file is compiled with floating-point contraction enabled and its FP64
+// pair takes the FAST corner integrals of pair_math.hpp: the hybrid is an
+// approximation measured against the FP64 path (its far field is good to
+// ~1e-3), so its near field is not held to the FP64 path's operation-for-
+// operation agreement with the oracle.  Fused multiply-adds and refined
+// hardware reciprocals move a near pair's forces within the pair formula's
+// own conditioning (up to ~3e-7 per node against the IEEE oracle) and remove
 // many of the FP64 pair's instructions.
 #include "farfield.hpp"
 #include "pair_special.hpp"
@@ -28,9 +29,8 @@ __device__ __forceinline__ Seg<double> seg_from_lds(const double (*s)[kTile], in
   return r;
 }
 
-// Tile staging of the two kernels below: segments of row block I and column
-// block J into LDS, cell keys (-1 for padding) into ki (row, register) and
-// kj (columns, LDS).
+// Tile staging: segments of row block I and column block J into LDS, cell
+// keys (-1 for padding) into ki (row, register) and kj (columns, LDS).
 __device__ __forceinline__ void stage_tile(const double* __restrict__ seg, int64_t ld,
                                            const int* __restrict__ key, int I, int J, int lane,
                                            double (*si)[kTile], double (*sj)[kTile], int* kj,
@@ -54,8 +54,8 @@ enum : int { kNone = 0, kFP32 = 1, kFP64 = 2, kPar = 3 };
 // a diagonal tile), well separated and non-parallel (kFP32: midpoints
 // farther than farDist, sin^2 of the angle above sinMin^2), near-parallel
 // (kPar: 1 - c^2 <= sqrt(eps), the reference's special branch) or generic
-// (kFP64).  Every rounding is explicit, so both kernels that classify a tile
-// reach the same decisions.
+// (kFP64).  Every rounding is explicit, so the decisions do not depend on
+// how the compiler contracts this translation unit.
 __device__ __forceinline__ int near_class(const double (*si)[kTile], const double (*sj)[kTile],
                                           int ki, int kb, int lane, int s, bool diag, int rn,
                                           double farDist2, double sinMin2) {
