@@ -389,6 +389,7 @@ void hybrid_pairs(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsig
   // evaluation.  A list that outgrew its capacity is enlarged and the tiles
   // run again (a one-off as the network changes).
   long long total = 0;
+  ff->spCap = std::max(ff->spCap, 16LL * tiles);  // ~12 per tile on random networks
   for (;;) {
     ff->spList.ensure(sizeof(unsigned long long) * ff->spCap);
     launch_near_tiles(ff->segS.as<double>(), ldS, ff->key.as<int>(), ff->rn, ff->list.as<int2>(),
