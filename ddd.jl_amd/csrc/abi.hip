@@ -38,9 +38,15 @@ struct Err {
 
 [[noreturn]] void bad(const std::string& m) { throw Err{DDD_ERR_ARG, m}; }
 
+// A device buffer that grows on demand and is freed with its owner (context,
+// far field, network); not copyable.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
   template <class T> T* as() const { return static_cast<T*>(p); }
   void ensure(size_t need) {
     if (need <= bytes) return;
@@ -58,13 +64,8 @@ struct DevBuf {
   }
 };
 
-// A device buffer freed when it leaves scope (temporaries of one call).
-struct ScopedBuf : DevBuf {
-  ScopedBuf() = default;
-  ScopedBuf(const ScopedBuf&) = delete;
-  ScopedBuf& operator=(const ScopedBuf&) = delete;
-  ~ScopedBuf() { release(); }
-};
+// A device buffer of one call's temporaries, freed when it leaves scope.
+using ScopedBuf = DevBuf;
 
 }  // namespace
 
@@ -142,6 +143,7 @@ struct ddd_ctx {
   DevBuf uTri, uPts, uPart, uOut;
   // collision detection
   DevBuf cConn, cLinks, cCoord, cVel, cB, cLabel, cSkip, cSeg, cNodes, cFlags, cBest, cLast, cRes;
+  DevBuf cKeys, cKeysOut, cIdx, cPerm, cSortWork, cBox;
   std::vector<double> h;  // host staging
   DevBuf counter;  // FP64-pair counter of the mixed path
   // near-parallel pairs of the all-pairs tiles (count, offsets, list, forces)
@@ -1185,7 +1187,7 @@ int ddd_detect_collision(ddd_ctx* c, double collisionDistSq, const ddd_network* 
                              c->cLabel.as<int32_t>(), c->cSkip.as<int64_t>(), nSkip,
                              collisionDistSq, c->cBest.as<CollisionCand>(),
                              c->cLast.as<CollisionLast>(), c->st);
-      launch_collision_final(c->cBest.as<CollisionCand>(), c->cLast.as<CollisionLast>(), nn,
+      launch_collision_final(c->cBest.as<CollisionCand>(), nn, c->cLast.as<CollisionLast>(), nn,
                              nullptr, 0, resB, resL, c->st);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(&hb, resB, sizeof(hb), hipMemcpyDeviceToHost, c->st));
@@ -1214,24 +1216,53 @@ int ddd_detect_collision(ddd_ctx* c, double collisionDistSq, const ddd_network* 
     if (ns >= 2) {
       const int64_t ld = ((ns + 63) / 64) * 64;
       const int blocks = collision_pair_blocks(ns);
+      const int lastBlocks = collision_last_blocks(ns);
       c->cB.ensure(sizeof(double) * 3 * ns);
       c->cSeg.ensure(sizeof(double) * 19 * ld);
       c->cNodes.ensure(sizeof(int64_t) * 2 * ld);
       c->cFlags.ensure(sizeof(int) * ld);
       c->cBest.ensure(sizeof(CollisionCand) * blocks);
-      c->cLast.ensure(sizeof(CollisionLast) * blocks);
+      c->cLast.ensure(sizeof(CollisionLast) * lastBlocks);
       HIPCHK(hipMemcpyAsync(c->cB.p, net->bVec, sizeof(double) * 3 * ns, hipMemcpyHostToDevice,
                             c->st));
       launch_collision_pack(ns, ld, c->cLinks.as<int64_t>(), c->cCoord.as<double>(),
                             c->cVel.as<double>(), c->cB.as<double>(), c->cLabel.as<int32_t>(),
                             c->cSkip.as<int64_t>(), nSkip, c->cSeg.as<double>(),
                             c->cNodes.as<int64_t>(), c->cFlags.as<int>(), c->st);
+      // scan order: Morton order of the midpoints over the nodes' bounding box
+      double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};
+      for (int64_t q = 0; q < ns; ++q)
+        for (int e = 0; e < 2; ++e) {
+          const double* x = net->coord + 3 * (net->links[2 * q + e] - 1);
+          for (int d = 0; d < 3; ++d) {
+            if ((q == 0 && e == 0) || x[d] < lo[d]) lo[d] = x[d];
+            if ((q == 0 && e == 0) || x[d] > hi[d]) hi[d] = x[d];
+          }
+        }
+      const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
+      const double scale = std::isfinite(ext) && ext > 0.0 ? 1024.0 / ext : 1.0;
+      size_t sortBytes = 0;
+      HIPCHK(collision_sort_bytes((int)ns, &sortBytes));
+      const int64_t nbS = (ns + 63) / 64;
+      c->cKeys.ensure(sizeof(uint32_t) * ns);
+      c->cKeysOut.ensure(sizeof(uint32_t) * ns);
+      c->cIdx.ensure(sizeof(int) * ns);
+      c->cPerm.ensure(sizeof(int) * ns);
+      c->cSortWork.ensure(std::max<size_t>(sortBytes, 1));
+      c->cBox.ensure(sizeof(double) * 6 * nbS);
+      HIPCHK(launch_collision_order(c->cSeg.as<double>(), ld, (int)ns, lo, scale,
+                                    c->cKeys.as<uint32_t>(), c->cKeysOut.as<uint32_t>(),
+                                    c->cIdx.as<int>(), c->cPerm.as<int>(), c->cSortWork.p, sortBytes,
+                                    c->cBox.as<double>(), c->st));
       launch_collision_pairs(c->cSeg.as<double>(), ld, ns, c->cNodes.as<int64_t>(),
                              c->cFlags.as<int>(), c->cSkip.as<int64_t>(), nSkip, collisionDistSq,
-                             blocks, c->cBest.as<CollisionCand>(), c->cLast.as<CollisionLast>(),
-                             c->st);
-      launch_collision_final(c->cBest.as<CollisionCand>(), c->cLast.as<CollisionLast>(), blocks,
-                             c->cSeg.as<double>(), ld, resB, resL, c->st);
+                             c->cPerm.as<int>(), c->cBox.as<double>(), blocks,
+                             c->cBest.as<CollisionCand>(), c->st);
+      launch_collision_last(c->cSeg.as<double>(), ld, ns, c->cNodes.as<int64_t>(),
+                            c->cFlags.as<int>(), c->cSkip.as<int64_t>(), nSkip, lastBlocks,
+                            c->cLast.as<CollisionLast>(), c->st);
+      launch_collision_final(c->cBest.as<CollisionCand>(), blocks, c->cLast.as<CollisionLast>(),
+                             lastBlocks, c->cSeg.as<double>(), ld, resB, resL, c->st);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(&pb, resB, sizeof(pb), hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipMemcpyAsync(&pl, resL, sizeof(pl), hipMemcpyDeviceToHost, c->st));

@@ -10,7 +10,17 @@
 // reductions with that order made explicit: a candidate is (key, loop
 // position), compared by key, then by position; the last evaluated pair is
 // the largest position; its L1, L2 come with it (hinges) or are computed by
-// the final reduction (pairs, which skip far pairs by bounding spheres).
+// the final reduction (pairs).
+//
+// The pair scan visits links in Morton order of their midpoints (radix
+// sort), 64 to a block, and skips every tile of two blocks whose bounding
+// boxes are farther apart than the collision distance: no pair in it can be
+// a candidate.  Each pair keeps the reference's orientation (lower link
+// index first) for its arithmetic and its loop position.  The last
+// evaluated pair does not depend on distances; it is found per row by
+// scanning its columns from the end (k_collision_last).
+#include <hipcub/hipcub.hpp>
+
 #include "kernels.hpp"
 
 namespace ddd {
@@ -217,79 +227,160 @@ __global__ void k_collision_pack(int64_t numSeg, int64_t ld, const int64_t* __re
 
 constexpr int kCT = 64;  // tile side, one wavefront
 
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {
+  x &= 1023u;
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+// Morton key of each link's midpoint on a 1024^3 grid over [lo, lo + 1/scale)
+// (clamped), and the identity permutation to sort with it.
+__global__ void k_collision_keys(const double* __restrict__ seg, int64_t ld, int n, double lox,
+                                 double loy, double loz, double scale, uint32_t* __restrict__ keys,
+                                 int* __restrict__ idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double lo[3] = {lox, loy, loz};
+  uint32_t q[3];
+  for (int d = 0; d < 3; ++d) {
+    const double v = (seg[(G_MID + d) * ld + i] - lo[d]) * scale;
+    q[d] = v <= 0.0 ? 0u : (v >= 1023.0 ? 1023u : (uint32_t)v);
+  }
+  keys[i] = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+  idx[i] = i;
+}
+
+// Bounding box of the end points of the links of each sorted block of 64
+// (lo > hi for an empty block): box[6 B + d] low, box[6 B + 3 + d] high.
+__global__ void k_collision_boxes(const double* __restrict__ seg, int64_t ld, int n,
+                                  const int* __restrict__ perm, int nb, double* __restrict__ box) {
+  const int B = blockIdx.x * blockDim.x + threadIdx.x;
+  if (B >= nb) return;
+  double lo[3] = {1e308, 1e308, 1e308}, hi[3] = {-1e308, -1e308, -1e308};
+  const int p1 = min(n, kCT * (B + 1));
+  for (int p = kCT * B; p < p1; ++p) {
+    const int i = perm[p];
+    for (int d = 0; d < 3; ++d) {
+      const double a = seg[(G_X0 + d) * ld + i], b = seg[(G_X1 + d) * ld + i];
+      lo[d] = fmin(lo[d], fmin(a, b));
+      hi[d] = fmax(hi[d], fmax(a, b));
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    box[6 * B + d] = lo[d];
+    box[6 * B + 3 + d] = hi[d];
+  }
+}
+
+// The reference's filters of pair (i, j), i < j (0-based): links sharing no
+// node, both with mobile ends, not a superdislocation
+// (DislocationTopologyCollision.jl:176-184), not in the skip list.
+__device__ __forceinline__ bool pair_evaluated(const int64_t* ni, const int64_t* nj, int fi, int fj,
+                                               const double* sx, const double* sy,
+                                               const double* bi, const double* bj,
+                                               const int64_t* skip, int64_t nSkip, int64_t i,
+                                               int64_t j) {
+  if (ni[0] == nj[0] || ni[0] == nj[1] || ni[1] == nj[0] || ni[1] == nj[1]) return false;
+  if (!(fi & 1) || !(fj & 1)) return false;
+  double dm[3], dp[3];
+  for (int d = 0; d < 3; ++d) {
+    dm[d] = bi[d] - bj[d];
+    dp[d] = bi[d] + bj[d];
+  }
+  const double par = dot3(sx, sy);
+  if ((dot3(dm, dm) < kEps && par > kEps) || (dot3(dp, dp) < kEps && par < kEps)) return false;
+  if ((fi & 2) && (fj & 2) && skipped(skip, nSkip, i + 1, j + 1)) return false;
+  return true;
+}
+
 // Persistent blocks of one wavefront walk the tiles (I, J >= I) of the
-// i < j triangle; lane l keeps row 64 I + l and pairs it with the 64 columns
-// of block J staged in LDS.  Each lane keeps its best candidate and its last
-// evaluated pair; the wave reduces them and writes one partial per block.
+// sorted blocks; a tile whose block boxes are farther apart than the
+// collision distance (less a rounding margin) is skipped as a whole.  Lane l
+// keeps sorted row 64 I + l and pairs it with the 64 columns of block J
+// staged in LDS; each lane keeps its best candidate; the wave reduces them
+// and writes one partial per block.
 __global__ __launch_bounds__(kCT) void k_collision_pairs(
     const double* __restrict__ seg, int64_t ld, int64_t numSeg, const int64_t* __restrict__ nodes,
     const int* __restrict__ flags, const int64_t* __restrict__ skip, int64_t nSkip,
-    double collisionDistSq, int64_t nTiles, int64_t nb, CollisionCand* __restrict__ bestOut,
-    CollisionLast* __restrict__ lastOut) {
+    double collisionDistSq, const int* __restrict__ perm, const double* __restrict__ box,
+    int64_t nTiles, int64_t nb, CollisionCand* __restrict__ bestOut) {
   __shared__ double sc[G_COUNT][kCT];
   __shared__ int64_t sn[2][kCT];
   __shared__ int sf[kCT];
+  __shared__ int so[kCT];
   __shared__ CollisionCand rb[kCT];
-  __shared__ CollisionLast rl[kCT];
   const int l = threadIdx.x;
   CollisionCand best{0.0, -1, -1, {0, 0, 0, 0, 0, 0}};
-  CollisionLast last{-1, -1, 0.0, 0.0};
   for (int64_t t = blockIdx.x; t < nTiles; t += gridDim.x) {
     // tile t -> (I, J), J >= I, row-major over the upper triangle
     int64_t I = (int64_t)((2 * nb + 1 - sqrt((double)(2 * nb + 1) * (2 * nb + 1) - 8.0 * t)) / 2);
     while (I > 0 && I * nb - I * (I - 1) / 2 > t) --I;
     while ((I + 1) * nb - (I + 1) * I / 2 <= t) ++I;
     const int64_t J = I + (t - (I * nb - I * (I - 1) / 2));
+    // Box gap: a lower bound of every point-pair distance of the two blocks,
+    // so of every pair's distSq; the margin covers its rounding.
+    {
+      const double* bI = box + 6 * I;
+      const double* bJ = box + 6 * J;
+      double g2 = 0.0, mag = 1.0;
+      for (int d = 0; d < 3; ++d) {
+        const double g = fmax(0.0, fmax(bJ[d] - bI[3 + d], bI[d] - bJ[3 + d]));
+        g2 += g * g;
+        mag += fmax(fmax(fabs(bI[d]), fabs(bI[3 + d])), fmax(fabs(bJ[d]), fabs(bJ[3 + d])));
+      }
+      const double gap = sqrt(g2) - 1e-7 * mag;
+      if (gap > 0.0 && gap * gap > collisionDistSq) continue;  // uniform over the wave
+    }
     __syncthreads();
     {
       const int64_t q = J * kCT + l;
-      for (int f = 0; f < G_COUNT; ++f) sc[f][l] = seg[f * ld + q];
-      sn[0][l] = nodes[2 * q];
-      sn[1][l] = nodes[2 * q + 1];
-      sf[l] = flags[q];
+      const int o = q < numSeg ? perm[q] : -1;
+      so[l] = o;
+      if (o >= 0) {
+        for (int f = 0; f < G_COUNT; ++f) sc[f][l] = seg[f * ld + o];
+        sn[0][l] = nodes[2 * (int64_t)o];
+        sn[1][l] = nodes[2 * (int64_t)o + 1];
+        sf[l] = flags[o];
+      }
     }
     __syncthreads();
-    const int64_t i = I * kCT + l;
+    const int64_t p = I * kCT + l;
+    if (p >= numSeg) continue;
+    const int64_t i = perm[p];
     const int fi = flags[i];
-    if (i >= numSeg || !(fi & 1)) continue;
-    double x0[3], x1[3], v0[3], v1[3], bi[3];
+    if (!(fi & 1)) continue;
+    double x0[3], x1[3], v0[3], v1[3], bi[3], sx[3];
     for (int d = 0; d < 3; ++d) {
       x0[d] = seg[(G_X0 + d) * ld + i];
       x1[d] = seg[(G_X1 + d) * ld + i];
       v0[d] = seg[(G_V0 + d) * ld + i];
       v1[d] = seg[(G_V1 + d) * ld + i];
       bi[d] = seg[(G_B + d) * ld + i];
+      sx[d] = x1[d] - x0[d];
     }
-    const int64_t a1 = nodes[2 * i], a2 = nodes[2 * i + 1];
+    const int64_t ni[2] = {nodes[2 * i], nodes[2 * i + 1]};
     const double mi[3] = {seg[G_MID * ld + i], seg[(G_MID + 1) * ld + i],
                           seg[(G_MID + 2) * ld + i]};
     const double hi = seg[G_HALF * ld + i];
     const double mia = fabs(mi[0]) + fabs(mi[1]) + fabs(mi[2]);
-    for (int c = 0; c < kCT; ++c) {
-      const int64_t j = J * kCT + c;
-      if (j <= i || j >= numSeg) continue;
-      const int64_t b1 = sn[0][c], b2 = sn[1][c];
-      if (a1 == b1 || a1 == b2 || a2 == b1 || a2 == b2) continue;
-      if (!(sf[c] & 1)) continue;
-      double y0[3], y1[3], w0[3], w1[3], bj[3], sx[3], sy[3], dm[3], dp[3];
+    for (int c = (I == J) ? l + 1 : 0; c < kCT; ++c) {
+      const int64_t j = so[c];
+      if (j < 0) continue;
+      double y0[3], y1[3], w0[3], w1[3], bj[3], sy[3];
       for (int d = 0; d < 3; ++d) {
         y0[d] = sc[G_X0 + d][c];
         y1[d] = sc[G_X1 + d][c];
         w0[d] = sc[G_V0 + d][c];
         w1[d] = sc[G_V1 + d][c];
         bj[d] = sc[G_B + d][c];
-        sx[d] = x1[d] - x0[d];
         sy[d] = y1[d] - y0[d];
-        dm[d] = bi[d] - bj[d];
-        dp[d] = bi[d] + bj[d];
       }
-      // no superdislocations (DislocationTopologyCollision.jl:176-184)
-      const double par = dot3(sx, sy);
-      if ((dot3(dm, dm) < kEps && par > kEps) || (dot3(dp, dp) < kEps && par < kEps)) continue;
-      if ((fi & 2) && (sf[c] & 2) && skipped(skip, nSkip, i + 1, j + 1)) continue;
-      // evaluated by the reference; its L1, L2 are recomputed at the end
-      if (i + 1 > last.a || (i + 1 == last.a && j + 1 > last.b))
-        last = CollisionLast{i + 1, j + 1, 0.0, 0.0};
+      const int64_t nj[2] = {sn[0][c], sn[1][c]};
+      const int64_t lo = i < j ? i : j, hiIdx = i < j ? j : i;
+      if (!pair_evaluated(ni, nj, fi, sf[c], sx, sy, bi, bj, skip, nSkip, lo, hiIdx)) continue;
       // Bounding spheres: every point pair of the two segments is at least
       // |mi - mj| - hi - hj apart, and the reference's distSq is the squared
       // distance of one such pair, so a pair beyond collisionDist (with a
@@ -302,57 +393,102 @@ __global__ __launch_bounds__(kCT) void k_collision_pairs(
         const double slack = 1e-7 * (mia + mj + hi + sc[G_HALF][c] + 1.0);
         if (gap - slack > 0.0 && (gap - slack) * (gap - slack) > collisionDistSq) continue;
       }
-      const MinDist m = min_distance(x0, x1, y0, y1, v0, v1, w0, w1);
+      // the reference's orientation: the lower link index is segment 1
+      const MinDist m = i < j ? min_distance(x0, x1, y0, y1, v0, v1, w0, w1)
+                              : min_distance(y0, y1, x0, x1, w0, w1, v0, v1);
       if (m.distSq < collisionDistSq && m.dDistSqDt < -kEps) {
         const double key = 1.0 / m.distSq;
-        if (better(key, i + 1, j + 1, best.key, best.a, best.b)) {
+        if (better(key, lo + 1, hiIdx + 1, best.key, best.a, best.b)) {
           best.key = key;
-          best.a = i + 1;
-          best.b = j + 1;
-          best.info[0] = a1;
-          best.info[1] = a2;
-          best.info[2] = b1;
-          best.info[3] = b2;
-          best.info[4] = i + 1;
-          best.info[5] = j + 1;
+          best.a = lo + 1;
+          best.b = hiIdx + 1;
+          const int64_t* n1 = i < j ? ni : nj;
+          const int64_t* n2 = i < j ? nj : ni;
+          best.info[0] = n1[0];
+          best.info[1] = n1[1];
+          best.info[2] = n2[0];
+          best.info[3] = n2[1];
+          best.info[4] = lo + 1;
+          best.info[5] = hiIdx + 1;
         }
       }
     }
   }
   rb[l] = best;
-  rl[l] = last;
   __syncthreads();
   if (l == 0) {
     CollisionCand b = rb[0];
-    CollisionLast v = rl[0];
     for (int q = 1; q < kCT; ++q) {
       const CollisionCand& o = rb[q];
       if (o.a >= 0 && better(o.key, o.a, o.b, b.key, b.a, b.b)) b = o;
+    }
+    bestOut[blockIdx.x] = b;
+  }
+}
+
+constexpr int kLastBlock = 256;
+
+// The last pair the reference evaluates: per row i (one thread, original
+// order) the largest j > i that passes the filters, scanned from the end;
+// the block's largest (i, j) goes to lastOut[blockIdx.x].
+__global__ __launch_bounds__(kLastBlock) void k_collision_last(
+    const double* __restrict__ seg, int64_t ld, int64_t numSeg, const int64_t* __restrict__ nodes,
+    const int* __restrict__ flags, const int64_t* __restrict__ skip, int64_t nSkip,
+    CollisionLast* __restrict__ lastOut) {
+  __shared__ CollisionLast rl[kLastBlock];
+  const int64_t i = blockIdx.x * (int64_t)kLastBlock + threadIdx.x;
+  CollisionLast v{-1, -1, 0.0, 0.0};
+  if (i < numSeg && (flags[i] & 1)) {
+    double bi[3], sx[3];
+    for (int d = 0; d < 3; ++d) {
+      bi[d] = seg[(G_B + d) * ld + i];
+      sx[d] = seg[(G_X1 + d) * ld + i] - seg[(G_X0 + d) * ld + i];
+    }
+    const int64_t ni[2] = {nodes[2 * i], nodes[2 * i + 1]};
+    const int fi = flags[i];
+    for (int64_t j = numSeg - 1; j > i; --j) {
+      double bj[3], sy[3];
+      for (int d = 0; d < 3; ++d) {
+        bj[d] = seg[(G_B + d) * ld + j];
+        sy[d] = seg[(G_X1 + d) * ld + j] - seg[(G_X0 + d) * ld + j];
+      }
+      const int64_t nj[2] = {nodes[2 * j], nodes[2 * j + 1]};
+      if (pair_evaluated(ni, nj, fi, flags[j], sx, sy, bi, bj, skip, nSkip, i, j)) {
+        v = CollisionLast{i + 1, j + 1, 0.0, 0.0};
+        break;
+      }
+    }
+  }
+  rl[threadIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < kLastBlock; ++q) {
       const CollisionLast& w = rl[q];
       if (w.a > v.a || (w.a == v.a && w.b > v.b)) v = w;
     }
-    bestOut[blockIdx.x] = b;
     lastOut[blockIdx.x] = v;
   }
 }
 
 constexpr int kFinal = 256;
 
-// One block: the best candidate and the last evaluated entry of n partials,
-// thread-strided then over the threads in order.  With seg (the packed
+// One block: the best candidate of nBest partials and the last evaluated
+// entry of nLast partials, thread-strided then over the threads in order.  With seg (the packed
 // links of the pair scan), the last pair's L1, L2 are computed here.
 __global__ __launch_bounds__(kFinal) void k_collision_final(
-    const CollisionCand* __restrict__ best, const CollisionLast* __restrict__ last, int64_t n,
-    const double* __restrict__ seg, int64_t ld, CollisionCand* __restrict__ ob,
+    const CollisionCand* __restrict__ best, int64_t nBest, const CollisionLast* __restrict__ last,
+    int64_t nLast, const double* __restrict__ seg, int64_t ld, CollisionCand* __restrict__ ob,
     CollisionLast* __restrict__ ol) {
   __shared__ CollisionCand rb[kFinal];
   __shared__ CollisionLast rl[kFinal];
   const int l = threadIdx.x;
   CollisionCand b{0.0, -1, -1, {0, 0, 0, 0, 0, 0}};
   CollisionLast v{-1, -1, 0.0, 0.0};
-  for (int64_t q = l; q < n; q += kFinal) {
+  for (int64_t q = l; q < nBest; q += kFinal) {
     const CollisionCand& o = best[q];
     if (o.a >= 0 && better(o.key, o.a, o.b, b.key, b.a, b.b)) b = o;
+  }
+  for (int64_t q = l; q < nLast; q += kFinal) {
     const CollisionLast& w = last[q];
     if (w.a > v.a || (w.a == v.a && w.b > v.b)) v = w;
   }
@@ -409,21 +545,56 @@ int collision_pair_blocks(int64_t numSeg) {
   return (int)(tiles < 8192 ? (tiles > 0 ? tiles : 1) : 8192);
 }
 
+int collision_last_blocks(int64_t numSeg) {
+  return (int)std::max<int64_t>(1, (numSeg + kLastBlock - 1) / kLastBlock);
+}
+
+hipError_t collision_sort_bytes(int numSeg, size_t* bytes) {
+  *bytes = 0;
+  return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (const uint32_t*)nullptr,
+                                            (uint32_t*)nullptr, (const int*)nullptr, (int*)nullptr,
+                                            numSeg, 0, 30, (hipStream_t)0);
+}
+
+hipError_t launch_collision_order(const double* seg, int64_t ld, int numSeg, const double lo[3],
+                                  double scale, uint32_t* keys, uint32_t* keysOut, int* idx,
+                                  int* perm, void* work, size_t workBytes, double* box,
+                                  hipStream_t st) {
+  if (numSeg <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_collision_keys, dim3((unsigned)((numSeg + 255) / 256)), dim3(256), 0, st, seg,
+                     ld, numSeg, lo[0], lo[1], lo[2], scale, keys, idx);
+  size_t bytes = workBytes;
+  const hipError_t e =
+      hipcub::DeviceRadixSort::SortPairs(work, bytes, keys, keysOut, idx, perm, numSeg, 0, 30, st);
+  if (e != hipSuccess) return e;
+  const int nb = (numSeg + kCT - 1) / kCT;
+  hipLaunchKernelGGL(k_collision_boxes, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, seg, ld,
+                     numSeg, perm, nb, box);
+  return hipGetLastError();
+}
+
 void launch_collision_pairs(const double* seg, int64_t ld, int64_t numSeg, const int64_t* nodes,
                             const int* flags, const int64_t* skip, int64_t nSkip,
-                            double collisionDistSq, int blocks, CollisionCand* best,
-                            CollisionLast* last, hipStream_t st) {
+                            double collisionDistSq, const int* perm, const double* box, int blocks,
+                            CollisionCand* best, hipStream_t st) {
   const int64_t nb = (numSeg + kCT - 1) / kCT;
   const int64_t tiles = nb * (nb + 1) / 2;
   hipLaunchKernelGGL(k_collision_pairs, dim3(blocks), dim3(kCT), 0, st, seg, ld, numSeg, nodes,
-                     flags, skip, nSkip, collisionDistSq, tiles, nb, best, last);
+                     flags, skip, nSkip, collisionDistSq, perm, box, tiles, nb, best);
 }
 
-void launch_collision_final(const CollisionCand* best, const CollisionLast* last, int64_t n,
-                            const double* seg, int64_t ld, CollisionCand* ob, CollisionLast* ol,
-                            hipStream_t st) {
-  hipLaunchKernelGGL(k_collision_final, dim3(1), dim3(kFinal), 0, st, best, last, n, seg, ld, ob,
-                     ol);
+void launch_collision_last(const double* seg, int64_t ld, int64_t numSeg, const int64_t* nodes,
+                           const int* flags, const int64_t* skip, int64_t nSkip, int blocks,
+                           CollisionLast* last, hipStream_t st) {
+  hipLaunchKernelGGL(k_collision_last, dim3(blocks), dim3(kLastBlock), 0, st, seg, ld, numSeg, nodes,
+                     flags, skip, nSkip, last);
+}
+
+void launch_collision_final(const CollisionCand* best, int64_t nBest, const CollisionLast* last,
+                            int64_t nLast, const double* seg, int64_t ld, CollisionCand* ob,
+                            CollisionLast* ol, hipStream_t st) {
+  hipLaunchKernelGGL(k_collision_final, dim3(1), dim3(kFinal), 0, st, best, nBest, last, nLast, seg,
+                     ld, ob, ol);
 }
 
 }  // namespace ddd

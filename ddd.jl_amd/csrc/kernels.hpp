@@ -153,15 +153,28 @@ void launch_collision_pack(int64_t numSeg, int64_t ld, const int64_t* links, con
                            const int64_t* skip, int64_t nSkip, double* seg, int64_t* nodes,
                            int* flags, hipStream_t st);
 int collision_pair_blocks(int64_t numSeg);
+int collision_last_blocks(int64_t numSeg);
+// Pair scan order: Morton keys of the link midpoints on a 1024^3 grid over
+// [lo, lo + 1/scale), radix-sorted into perm (keys, keysOut, idx: numSeg
+// each; work: collision_sort_bytes), then the bounding box of each block of
+// 64 sorted links (box: 6 doubles per block).
+hipError_t collision_sort_bytes(int numSeg, size_t* bytes);
+hipError_t launch_collision_order(const double* seg, int64_t ld, int numSeg, const double lo[3],
+                                  double scale, uint32_t* keys, uint32_t* keysOut, int* idx,
+                                  int* perm, void* work, size_t workBytes, double* box,
+                                  hipStream_t st);
 void launch_collision_pairs(const double* seg, int64_t ld, int64_t numSeg, const int64_t* nodes,
                             const int* flags, const int64_t* skip, int64_t nSkip,
-                            double collisionDistSq, int blocks, CollisionCand* best,
-                            CollisionLast* last, hipStream_t st);
+                            double collisionDistSq, const int* perm, const double* box, int blocks,
+                            CollisionCand* best, hipStream_t st);
+void launch_collision_last(const double* seg, int64_t ld, int64_t numSeg, const int64_t* nodes,
+                           const int* flags, const int64_t* skip, int64_t nSkip, int blocks,
+                           CollisionLast* last, hipStream_t st);
 // seg: the packed links of the pair scan (the last pair's L1, L2 are
 // computed from it), or NULL when the entries carry them (hinges).
-void launch_collision_final(const CollisionCand* best, const CollisionLast* last, int64_t n,
-                            const double* seg, int64_t ld, CollisionCand* ob, CollisionLast* ol,
-                            hipStream_t st);
+void launch_collision_final(const CollisionCand* best, int64_t nBest, const CollisionLast* last,
+                            int64_t nLast, const double* seg, int64_t ld, CollisionCand* ob,
+                            CollisionLast* ol, hipStream_t st);
 void launch_trapezoid_max(const double* coord, const double* init, const double* vel,
                           const double* vel0, double dt, int64_t m, unsigned long long* part,
                           double* out, hipStream_t st);
