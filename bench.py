@@ -225,6 +225,82 @@ def mixed_leg(args, dln, mat, ctx, log):
             "normwise_rel_err": normwise(got, ref), "fp64_pair_share": round(fp64 / max(tot, 1), 5)}
 
 
+def host_cpu():
+    """CPU model and logical CPU count of this host (from /proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+            else None}
+
+
+def small_configs_leg(args, dln, mat, ctx, log, nthreads):
+    """The two smaller plain FP64 workloads, each timed in full on both sides:
+    640 segments (20 loops, seed 0xA0) and 16,384 segments (512 loops, seed
+    0xB0).  GPU: whole resident evaluations (self + pair stage + reduction),
+    HIP events on the library's stream.  CPU: the reference's threaded design
+    restated (static row chunks, atomic adds) and the tuned design over every
+    row on `nthreads` threads, and at 640 segments also the single-thread
+    serial triangle.  Each GPU result is compared per node with the threaded
+    CPU result."""
+    import numpy as np
+    import torch
+    import pyoracle as O
+    from dddhip import forces as G
+    from dddhip.generate import random_loop_network
+    from metrics import per_node_rel
+    out = {}
+    for name, loops, seed in (("640", 20, 0xA0), ("16384", 512, 0xB0)):
+        net = random_loop_network(loops, seed=seed)
+        n = int(net.numSeg[0])
+        pairs = n * (n - 1) // 2
+        res = G.ResidentNetwork(dln, mat, net, ctx=ctx)
+        res.set_mode("fp64")
+        stream = torch.cuda.ExternalStream(res.stream(), device=torch.device("cuda:0"))
+        res.seg_force()
+        stream.synchronize()
+        reps = max(3, args.steps)
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(reps)]
+        for e in ev:
+            e[0].record(stream)
+            res.seg_force()
+            e[1].record(stream)
+            e[1].synchronize()
+        gms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+        g = res.download()
+        rows = np.arange(1, n + 1, dtype=np.int64)
+        t = time.perf_counter()
+        f, sp = O.segseg_par_rows(dln, mat, net, rows, nthreads)
+        fms = (time.perf_counter() - t) * 1e3
+        t = time.perf_counter()
+        O.segseg_tuned_rows(dln, mat, net, rows, nthreads)
+        tms = (time.perf_counter() - t) * 1e3
+        s1, s2 = O.calcSelfForce(dln, mat, net)
+        f[:, 0, :] += s1
+        f[:, 1, :] += s2
+        mx, rms = per_node_rel(g, f)
+        rec = {"segments": n, "pairs": pairs, "gpu_fp64_ms": round(gms, 4),
+               "cpu_faithful_ms": round(fms, 2), "cpu_tuned_ms": round(tms, 2),
+               "cpu_cores": nthreads, "gpu_vs_cpu_faithful": round(fms / gms, 1),
+               "per_node_rel_err_max": mx, "per_node_rel_err_rms": rms}
+        assert sp == pairs, (sp, pairs)
+        if n <= 4096:
+            t = time.perf_counter()
+            O.calcSegSegForce(dln, mat, net)
+            rec["cpu_serial_ms"] = round((time.perf_counter() - t) * 1e3, 2)
+        log(f"{n} segments: GPU {gms:.3f} ms, CPU threaded {fms:.1f} ms, tuned {tms:.1f} ms "
+            f"({nthreads} threads), per-node err max {mx:.2e}")
+        out[name] = rec
+    return out
+
+
 def sigma_leg(args, dln, mat, ctx, log):
     """calc_σTilde of a 512-loop network (16,384 segments) at the 6 x 128^2
     face points of the 2000^3 cube: wall time of the synchronous ABI call
@@ -554,6 +630,9 @@ def main():
     clustered = clustered_leg(args, dln, mat, ctx, log) if args.clustered else None
     utilde = utilde_leg(args, mat, net, ctx, log) if args.utilde_grid > 0 else None
     collision = collision_leg(args, net, ctx, log) if args.collision else None
+    # replaces the context's resident network: after every leg that uses it
+    small = (small_configs_leg(args, dln, mat, ctx, log, max(1, min(16, os.cpu_count() or 1)))
+             if not args.no_cpu else None)
 
     achieved = FLOPS_PER_PAIR * pairs / (tms * 1e-3) / 1e12
     result = {
@@ -587,6 +666,7 @@ def main():
         "clustered": clustered,
         "u_tilde": utilde,
         "detect_collision": collision,
+        "plain_fp64_small": small,
         "roofline": {
             "bound": "valu",
             "achieved": round(achieved, 3),
@@ -600,8 +680,19 @@ def main():
             "traffic": TILE_TRAFFIC_BYTES if N == 131072 else None,
             "kernel": "k_segseg_tiles",
             "flops_per_pair": FLOPS_PER_PAIR,
+            # The plain path rounds every product and sum separately, as the
+            # oracle and the reference do (no fused multiply-add): one flop
+            # per FP64 lane-instruction, so its ceiling is half the peak.
+            "no_fma_ceiling": FP64_PEAK_TFLOPS / 2,
+            "frac_of_no_fma_ceiling": round(achieved / (FP64_PEAK_TFLOPS / 2), 4),
         },
     }
+    best = (hybrid or {}).get("best_within_1e-3")
+    if best:
+        result["headline"] = {"fp64_ms": round(ms, 3), "hybrid_ms": best["ms"],
+                              "hybrid_speedup": round(ms / best["ms"], 2),
+                              "hybrid_node_rel_err_rms": best["node_rel_err_rms"],
+                              "hybrid_grid": {"G": best["G"], "rn": best["rn"]}}
 
     if not args.no_cpu:
         import pyoracle as O
@@ -622,6 +713,7 @@ def main():
                       f"all j > i: {sp} pairs in {dt:.2f} s, extrapolated to {pairs} pairs",
         }
         result["cpu_baseline"]["speedup_gpu_vs_cpu"] = round(full_ms / ms, 1)
+        result["cpu_baseline"]["host"] = host_cpu()
         # the second CPU design: dynamic schedule, per-thread accumulators
         t = time.perf_counter()
         ft, sp2 = O.segseg_tuned_rows(dln, mat, net, rows, nthreads)
