@@ -202,6 +202,62 @@ def test_edge_cases(prm):
     assert per_node_rel(G.calcSegSegForce(dln, mat, net), O.calcSegSegForce(dln, mat, net))[0] <= TOL
 
 
+def test_degenerate_inputs(prm):
+    """Inputs at the edges of the domain: an empty network (no segments); idx
+    entries outside 1:numSeg (an error, and the in-place form writes
+    nothing); a zero-length segment and a NaN coordinate (the reference
+    divides by the length and lets NaN through, so every segment that
+    interacts with the bad one gets NaN: the NaN entries must be the
+    oracle's); 64 and 129 segments (whole tiles, and two tiles plus a
+    one-segment ragged one)."""
+    from dddhip.types import DislocationNetwork
+    dln, mat = prm
+    empty = DislocationNetwork.from_arrays(np.zeros((2, 0), dtype=np.int64), np.zeros((3, 0)),
+                                           np.zeros((3, 0)), [])
+    _log("empty network")
+    assert G.calcSegSegForce(dln, mat, empty).shape == (3, 2, 0)
+    assert G.calcSegForce(dln, mat, None, None, empty).shape == (3, 2, 0)
+    pts = face_grid_points(4)
+    s = G.calc_σTilde(pts, dln, mat, empty)
+    assert s.shape == (6, pts.shape[1]) and np.all(s == 0)
+
+    net = random_loop_network(4, seed=0xA7)
+    n = int(net.numSeg[0])
+    _log("idx outside 1:numSeg")
+    for bad in ([0], [3, n + 1]):
+        with pytest.raises(RuntimeError, match="idx entry outside"):
+            G.calcSegSegForce(dln, mat, net, bad)
+    net.segForce[...] = 3.0
+    with pytest.raises(RuntimeError, match="idx entry outside"):
+        G.calcSegSegForce_(dln, mat, net, [n + 1])
+    assert np.all(net.segForce == 3.0)
+
+    for what in ("zero-length", "nan"):
+        net = random_loop_network(4, seed=0xA8)
+        si = net.segIdx[5]
+        if what == "zero-length":
+            net.coord[:, si[2] - 1] = net.coord[:, si[1] - 1]
+        else:
+            net.coord[1, si[1] - 1] = np.nan
+        _log(f"{what} segment")
+        g = G.calcSegSegForce(dln, mat, net)
+        o = O.calcSegSegForce(dln, mat, net)
+        # a zero-length segment makes its own forces NaN (the other
+        # segments see a degenerate projection with finite forces); a NaN
+        # coordinate reaches every pair
+        assert np.array_equal(np.isnan(g), np.isnan(o)) and np.isnan(g).any()
+        fin = np.isfinite(o)
+        if fin.any():  # the sums run in another order: equal to rounding
+            assert np.max(np.abs(g[fin] - o[fin])) <= TOL * np.max(np.abs(o[fin]))
+
+    for loops, sides in ((2, 32), (3, 43)):
+        net = random_loop_network(loops, numSides=sides, seed=0xA9)
+        assert int(net.numSeg[0]) == loops * sides
+        _log(f"{loops * sides} segments")
+        g = G.calcSegSegForce(dln, mat, net)
+        assert per_node_rel(g, O.calcSegSegForce(dln, mat, net))[0] <= TOL
+
+
 def test_sigma_tilde_random(prm):
     dln, mat = prm
     net = random_loop_network(20, seed=0xA6)
