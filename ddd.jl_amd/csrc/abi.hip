@@ -179,6 +179,12 @@ int guard(ddd_ctx* ctx, const char* fn, auto&& body) {
   } catch (const std::bad_alloc&) {
     ctx->err = std::string(fn) + ": host allocation failed";
     return DDD_ERR_OOM;
+  } catch (const std::exception& e) {
+    ctx->err = std::string(fn) + ": " + e.what();
+    return DDD_ERR_INTERNAL;
+  } catch (...) {
+    ctx->err = std::string(fn) + ": unknown exception";
+    return DDD_ERR_INTERNAL;
   }
 }
 

@@ -15,8 +15,8 @@ import numpy as np
 PKG = Path(__file__).resolve().parents[1]
 LIB_PATH = PKG / "libdddhip.so"
 
-DDD_OK, DDD_ERR_ARG, DDD_ERR_OOM, DDD_ERR_HIP, DDD_ERR_NO_DEVICE, DDD_ERR_NUMERIC = (
-    0, -1, -2, -3, -4, -5)
+DDD_OK, DDD_ERR_ARG, DDD_ERR_OOM, DDD_ERR_HIP, DDD_ERR_NO_DEVICE, DDD_ERR_NUMERIC, \
+    DDD_ERR_INTERNAL = (0, -1, -2, -3, -4, -5, -6)
 DDD_ACCUMULATE, DDD_OUT_BY_POSITION, DDD_PK_HALF_TO_NODES = 1, 2, 4
 
 dp = C.POINTER(C.c_double)

@@ -41,7 +41,8 @@ enum {
   DDD_ERR_OOM = -2,       /* device allocation failed                */
   DDD_ERR_HIP = -3,       /* HIP runtime error                       */
   DDD_ERR_NO_DEVICE = -4, /* no usable gfx950 device                 */
-  DDD_ERR_NUMERIC = -5    /* a numerical guard fired (see message)   */
+  DDD_ERR_NUMERIC = -5,   /* a numerical guard fired (see message)   */
+  DDD_ERR_INTERNAL = -6   /* unexpected C++ exception (see message)  */
 };
 
 enum {
