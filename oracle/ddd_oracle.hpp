@@ -10,6 +10,12 @@
 // conditioning check (how far rounding alone moves a result); T=float is the
 // error study for the mixed-precision path; an operation-counting type derives
 // the per-pair flop figures used for the roofline.
+//
+// Every function is pinned by the reference's golden vectors, closed forms or
+// invariants (tests/test_oracle*.py) except one part, which is parity
+// unpinned: the closure of calc_uTilde's segments with an external node
+// (orc_u_tilde's surface projection, DislocationFEM.jl:437-528), for which
+// the reference holds no vectors.
 #pragma once
 #include <cmath>
 #include <cstdint>
