@@ -102,6 +102,13 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 constexpr int BT = 128, BK = 64, LDK = BK + 8;
 
+// LDS column of k in row `row` of a staged tile: the eight 16-byte k blocks
+// of a row are permuted by an XOR with (row / 8) mod 8.  An 8-element block
+// stays contiguous and aligned, so every MFMA operand read is still one
+// 16-byte read; the transposing stash, whose 16 lanes sharing a k write rows
+// 8 apart (one LDS bank without the permutation), then spreads over eight.
+__device__ __forceinline__ int swz(int row, int k) { return k ^ (((row >> 3) & 7) << 3); }
+
 struct TileRegs {
   u16x8 a[4], b[4];
 };
@@ -184,18 +191,20 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __
     for (int h = 0; h < 4; ++h) {
       const int v = tid + 256 * h;
       if (aK) {
-        *(u16x8*)&sA[v / 8][(v % 8) * 8] = R.a[h];
+        const int r = v / 8;
+        *(u16x8*)&sA[r][swz(r, (v % 8) * 8)] = R.a[h];
       } else {
         const int kk = v / 16, rq = (v % 16) * 8;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) sA[rq + q][kk] = R.a[h][q];
+        for (int q = 0; q < 8; ++q) sA[rq + q][swz(rq + q, kk)] = R.a[h][q];
       }
       if (bK) {
-        *(u16x8*)&sB[v / 8][(v % 8) * 8] = R.b[h];
+        const int c = v / 8;
+        *(u16x8*)&sB[c][swz(c, (v % 8) * 8)] = R.b[h];
       } else {
         const int kk = v / 16, cq = (v % 16) * 8;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) sB[cq + q][kk] = R.b[h][q];
+        for (int q = 0; q < 8; ++q) sB[cq + q][swz(cq + q, kk)] = R.b[h][q];
       }
     }
   };
@@ -214,9 +223,15 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __
       const int kb = ks + 8 * (lane / 32);
       bf16x8 a[2], b[2];
 #pragma unroll
-      for (int m = 0; m < 2; ++m) a[m] = *(const bf16x8*)&sA[wi + 32 * m + lane % 32][kb];
+      for (int m = 0; m < 2; ++m) {
+        const int r = wi + 32 * m + lane % 32;
+        a[m] = *(const bf16x8*)&sA[r][swz(r, kb)];
+      }
 #pragma unroll
-      for (int n = 0; n < 2; ++n) b[n] = *(const bf16x8*)&sB[wj + 32 * n + lane % 32][kb];
+      for (int n = 0; n < 2; ++n) {
+        const int c = wj + 32 * n + lane % 32;
+        b[n] = *(const bf16x8*)&sB[c][swz(c, kb)];
+      }
 #pragma unroll
       for (int m = 0; m < 2; ++m)
 #pragma unroll
