@@ -450,22 +450,31 @@ def clustered_leg(args, dln, mat, ctx, log):
     ref = res.download()
     log(f"clustered: {n} segments, plain FP64 {fp64_ms:.1f} ms")
     lo, side = -300.0, 2600.0
-    ff = G.FarField(32, (lo, lo, lo), side / 32, 4, ctx=ctx)
+    rns = sorted(int(r) for r in args.clustered_rn.split(",") if r)
+    ffs = {r: G.FarField(32, (lo, lo, lo), side / 32, r, ctx=ctx) for r in rns}
     mlp = G.FarFieldMLP(seed=11, ctx=ctx)
-    rep = G.trainFarField(dln, mat, ff, mlp, clustered_loop_network(seed=0xE1),
+    rep = G.trainFarField(dln, mat, ffs[rns[0]], mlp, clustered_loop_network(seed=0xE1),
                           samples=args.train_samples, valSamples=1 << 16, steps=args.train_steps,
                           batch=args.train_batch, lr=2e-3, lrFinal=5e-6, seed=7, denFloor=0.1)
-    res.set_mode("hybrid", farfield=ff)
-    hms = timed(max(1, args.steps))
-    got = res.download()
-    near, near64 = res.stats()
-    Fr, Fg = node_forces(net, ref), node_forces(net, got)
+    for r in rns[1:]:
+        G.setFarFieldFromMLP(dln, mat, ffs[r], mlp)
+    Fr = node_forces(net, ref)
     pick = np.sort(np.random.default_rng(0xE2).choice(Fr.shape[1], 4096, replace=False))
-    nmx, nrms = per_node_rel(Fg[:, pick], Fr[:, pick])
-    log(f"clustered hybrid: {hms:.1f} ms ({fp64_ms / hms:.1f}x), node err rms {nrms:.2e} "
-        f"max {nmx:.2e} on 4096 nodes")
+    rows = []
+    for r in rns:
+        res.set_mode("hybrid", farfield=ffs[r])
+        hms = timed(max(1, args.steps))
+        got = res.download()
+        near, near64 = res.stats()
+        nmx, nrms = per_node_rel(node_forces(net, got)[:, pick], Fr[:, pick])
+        log(f"clustered hybrid rn={r}: {hms:.1f} ms ({fp64_ms / hms:.1f}x), node err rms "
+            f"{nrms:.2e} max {nmx:.2e} on 4096 nodes")
+        rows.append({"G": 32, "rn": r, "hybrid_ms": round(hms, 3),
+                     "speedup_vs_fp64": round(fp64_ms / hms, 2), "node_rel_err_rms_4096": nrms,
+                     "node_rel_err_max_4096": nmx, "near_pairs": near, "near_pairs_fp64": near64})
     res.set_mode("fp64")
-    ff.close()
+    for f in ffs.values():
+        f.close()
     mlp.close()
     x = surface_points(1 << 20)
     σ = np.zeros((6, x.shape[1]), order="F")
@@ -473,9 +482,7 @@ def clustered_leg(args, dln, mat, ctx, log):
     G.calc_σTilde_(σ, x, dln, mat, net, ctx=ctx)
     sms = (time.perf_counter() - t0) * 1e3
     log(f"clustered calc_σTilde at {x.shape[1]} points: {sms:.1f} ms (PCIe included)")
-    return {"segments": n, "fp64_ms": round(fp64_ms, 3), "hybrid_ms": round(hms, 3),
-            "speedup_vs_fp64": round(fp64_ms / hms, 2), "node_rel_err_rms_4096": nrms,
-            "node_rel_err_max_4096": nmx, "near_pairs": near, "near_pairs_fp64": near64,
+    return {"segments": n, "fp64_ms": round(fp64_ms, 3), **rows[0], "configs": rows,
             "train_relErrVal": round(rep["relErrVal"], 6), "train_seconds": round(rep["seconds"], 1),
             "sigma_tilde_points": int(x.shape[1]), "sigma_tilde_ms_pcie_inclusive": round(sms, 1),
             "sigma_tilde_finite": bool(np.isfinite(σ).all())}
@@ -519,6 +526,8 @@ def main():
                     help="skip the detectCollision leg")
     ap.add_argument("--no-dynamics", dest="dynamics", action="store_false",
                     help="skip the deriv!/integrate! leg (run with the first hybrid grid)")
+    ap.add_argument("--clustered-rn", default="4",
+                    help="near radii (cells) of the clustered leg's hybrid, e.g. 4,5")
     ap.add_argument("--clustered", action="store_true",
                     help="also run the clustered stress test: 262,144 segments, 70%% in 8 "
                          "pile-ups; plain FP64 vs hybrid (G=32, rn=4), error on 4,096 sampled "
