@@ -455,7 +455,8 @@ def clustered_leg(args, dln, mat, ctx, log):
     mlp = G.FarFieldMLP(seed=11, ctx=ctx)
     rep = G.trainFarField(dln, mat, ffs[rns[0]], mlp, clustered_loop_network(seed=0xE1),
                           samples=args.train_samples, valSamples=1 << 16, steps=args.train_steps,
-                          batch=args.train_batch, lr=2e-3, lrFinal=5e-6, seed=7, denFloor=0.1)
+                          batch=args.train_batch, lr=2e-3, lrFinal=5e-6, seed=7, denFloor=0.1,
+                          nearFrac=args.clustered_near_frac)
     for r in rns[1:]:
         G.setFarFieldFromMLP(dln, mat, ffs[r], mlp)
     Fr = node_forces(net, ref)
@@ -528,6 +529,10 @@ def main():
                     help="skip the deriv!/integrate! leg (run with the first hybrid grid)")
     ap.add_argument("--clustered-rn", default="4",
                     help="near radii (cells) of the clustered leg's hybrid, e.g. 4,5")
+    ap.add_argument("--clustered-near-frac", type=float, default=0.2,
+                    help="share of the clustered leg's training sources drawn from the shells "
+                         "next to the near stencil (0: the library default, 0.5); its error "
+                         "comes from long-range interactions, so fewer near samples help")
     ap.add_argument("--clustered", action="store_true",
                     help="also run the clustered stress test: 262,144 segments, 70%% in 8 "
                          "pile-ups; plain FP64 vs hybrid (G=32, rn=4), error on 4,096 sampled "
