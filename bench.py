@@ -72,7 +72,7 @@ def hybrid_paths(args, dln, mat, net, res, stream, ref, fp64_ms, log):
         mlp = G.FarFieldMLP(seed=11, ctx=res.ctx)
         rep = G.trainFarField(dln, mat, ffs[rns[0]], mlp, train, samples=args.train_samples,
                               valSamples=1 << 16, steps=args.train_steps, batch=args.train_batch,
-                              lr=2e-3, lrFinal=5e-6, seed=7, denFloor=0.1)
+                              lr=2e-3, lrFinal=5e-6, seed=7, denFloor=0.1, nearFrac=args.near_frac)
         log(f"G={Gc}: trained in {rep['seconds']:.1f} s, loss {rep['lossFirst']:.3e} -> "
             f"{rep['lossLast']:.3e}, held-out local-expansion error {rep['relErrVal']:.3e}")
         for r in rns[1:]:
@@ -529,6 +529,10 @@ def main():
                     help="skip the deriv!/integrate! leg (run with the first hybrid grid)")
     ap.add_argument("--clustered-rn", default="4",
                     help="near radii (cells) of the clustered leg's hybrid, e.g. 4,5")
+    ap.add_argument("--near-frac", type=float, default=0.85,
+                    help="share of the hybrid leg's training sources drawn from the shells next "
+                         "to the near stencil (0: the library default, 0.5); the uniform "
+                         "network's error comes from the stencil's edge, so more near samples help")
     ap.add_argument("--clustered-near-frac", type=float, default=0.2,
                     help="share of the clustered leg's training sources drawn from the shells "
                          "next to the near stencil (0: the library default, 0.5); its error "
