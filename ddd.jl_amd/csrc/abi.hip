@@ -143,7 +143,7 @@ struct ddd_ctx {
   DevBuf uTri, uPts, uPart, uOut;
   // collision detection
   DevBuf cConn, cLinks, cCoord, cVel, cB, cLabel, cSkip, cSeg, cNodes, cFlags, cBest, cLast, cRes;
-  DevBuf cKeys, cKeysOut, cIdx, cPerm, cSortWork, cBox;
+  DevBuf cKeys, cKeysOut, cIdx, cPerm, cSortWork, cBox, cPrev;
   std::vector<double> h;  // host staging
   DevBuf counter;  // FP64-pair counter of the mixed path
   // near-parallel pairs of the all-pairs tiles (count, offsets, list, forces)
@@ -443,22 +443,24 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
   c->colPart.ensure(part);
   c->counter.ensure(sizeof(unsigned long long));
   c->specCnt.ensure(sizeof(unsigned) * T);
-  c->specOff.ensure(sizeof(int) * (T + 1));
+  c->specOff.ensure(sizeof(long long) * (T + 1));
   const bool mixed = opts && opts->mode == DDD_MODE_MIXED;
   const MixedRule r = mixed ? mixed_rule(k, *opts) : MixedRule{};
   launch_special_count(c->seg.as<double>(), c->resLd, (int)n, mixed ? &r : nullptr,
                        c->specCnt.as<unsigned>(), c->st);
-  c->specWork.ensure(sizeof(int) * scan_work_ints((int)T));
-  launch_exclusive_scan(c->specCnt.as<unsigned>(), (int)T, c->specOff.as<int>(), c->specWork.as<int>(),
-                        c->st);
-  int total = 0;
-  HIPCHK(hipMemcpyAsync(&total, c->specOff.as<int>() + T, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  c->specWork.ensure(sizeof(long long) * scan_work_ints((int)T));
+  launch_exclusive_scan(c->specCnt.as<unsigned>(), (int)T, c->specOff.as<long long>(),
+                        c->specWork.as<long long>(), c->st);
+  long long total = 0;
+  HIPCHK(hipMemcpyAsync(&total, c->specOff.as<long long>() + T, sizeof(total),
+                        hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
-  const int64_t tot1 = std::max(total, 1);
+  // 64-bit offsets throughout; the list itself must still fit in HBM.
+  const int64_t tot1 = std::max<long long>(total, 1);
   c->specSlot.ensure(sizeof(unsigned short) * tot1);
   c->specTile.ensure(sizeof(int) * tot1);
   c->specRes.ensure(sizeof(double) * 12 * tot1);
-  SpecialList sp{c->specOff.as<int>(), c->specSlot.as<unsigned short>(), c->specTile.as<int>(),
+  SpecialList sp{c->specOff.as<long long>(), c->specSlot.as<unsigned short>(), c->specTile.as<int>(),
                  c->specRes.as<double>(), (int64_t)total, flags};
   launch_special_list(c->seg.as<double>(), c->resLd, (int)n, mixed ? &r : nullptr, sp, seg_consts(k),
                       c->st);
@@ -672,6 +674,26 @@ int ddd_pk_force(ddd_ctx* c, const ddd_network* net, const ddd_mesh* mesh, const
       }
     }
     c->resN = -1;
+  });
+}
+
+int ddd_sigma_hat(ddd_ctx* c, const ddd_mesh* mesh, const double* x0, int64_t P, double* sigma) {
+  return guard(c, "ddd_sigma_hat", [&] {
+    if (!mesh) bad("mesh is NULL");
+    if (P < 0 || (P > 0 && (!x0 || !sigma))) bad("points/sigma inconsistent");
+    if (P > ((int64_t)1 << 32)) bad("too many points");
+    MeshDev M = upload_mesh(c, mesh);
+    if (P == 0) {
+      HIPCHK(hipStreamSynchronize(c->st));
+      return;
+    }
+    c->pts.ensure(sizeof(double) * 3 * P);
+    c->sigma.ensure(sizeof(double) * 9 * P);
+    HIPCHK(hipMemcpyAsync(c->pts.p, x0, sizeof(double) * 3 * P, hipMemcpyHostToDevice, c->st));
+    launch_sigma_hat(c->pts.as<double>(), P, M, c->sigma.as<double>(), c->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(sigma, c->sigma.p, sizeof(double) * 9 * P, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
   });
 }
 
@@ -1127,7 +1149,7 @@ int ddd_detect_collision(ddd_ctx* c, double collisionDistSq, const ddd_network* 
     if (nSkip < 0 || (nSkip > 0 && !skipSegs)) bad("skipSegs/nSkip inconsistent");
     if (nn > net->connCols || nn > net->coordCols) bad("numNode outside arrays");
     if (ns > net->linksCols || ns > net->bVecCols) bad("numSeg outside links or bVec");
-    if (ns > ((int64_t)1 << 31)) bad("numSeg too large");
+    if (ns > INT32_MAX) bad("numSeg too large");
     for (int64_t i = 0; i < ns; ++i)
       for (int e = 0; e < 2; ++e) {
         const int64_t nd = net->links[2 * i + e];
@@ -1231,6 +1253,17 @@ int ddd_detect_collision(ddd_ctx* c, double collisionDistSq, const ddd_network* 
       c->cLast.ensure(sizeof(CollisionLast) * lastBlocks);
       HIPCHK(hipMemcpyAsync(c->cB.p, net->bVec, sizeof(double) * 3 * ns, hipMemcpyHostToDevice,
                             c->st));
+      // links with both ends mobile, as a "largest such link <= j" index
+      // (the last-evaluated scan hops over the others)
+      std::vector<int> prevMob((size_t)ns);
+      for (int64_t q = 0, last = -1; q < ns; ++q) {
+        const int32_t la = label[net->links[2 * q] - 1], lb = label[net->links[2 * q + 1] - 1];
+        if ((la == 1 || la == 3) && (lb == 1 || lb == 3)) last = q;
+        prevMob[(size_t)q] = (int)last;
+      }
+      c->cPrev.ensure(sizeof(int) * ns);
+      HIPCHK(hipMemcpyAsync(c->cPrev.p, prevMob.data(), sizeof(int) * ns, hipMemcpyHostToDevice,
+                            c->st));
       launch_collision_pack(ns, ld, c->cLinks.as<int64_t>(), c->cCoord.as<double>(),
                             c->cVel.as<double>(), c->cB.as<double>(), c->cLabel.as<int32_t>(),
                             c->cSkip.as<int64_t>(), nSkip, c->cSeg.as<double>(),
@@ -1265,8 +1298,8 @@ int ddd_detect_collision(ddd_ctx* c, double collisionDistSq, const ddd_network* 
                              c->cPerm.as<int>(), c->cBox.as<double>(), blocks,
                              c->cBest.as<CollisionCand>(), c->st);
       launch_collision_last(c->cSeg.as<double>(), ld, ns, c->cNodes.as<int64_t>(),
-                            c->cFlags.as<int>(), c->cSkip.as<int64_t>(), nSkip, lastBlocks,
-                            c->cLast.as<CollisionLast>(), c->st);
+                            c->cFlags.as<int>(), c->cPrev.as<int>(), c->cSkip.as<int64_t>(), nSkip,
+                            lastBlocks, c->cLast.as<CollisionLast>(), c->st);
       launch_collision_final(c->cBest.as<CollisionCand>(), blocks, c->cLast.as<CollisionLast>(),
                              lastBlocks, c->cSeg.as<double>(), ld, resB, resL, c->st);
       HIPCHK(hipGetLastError());

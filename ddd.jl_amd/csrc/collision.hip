@@ -138,21 +138,30 @@ __global__ void k_collision_hinge(int64_t numNode, const int64_t* __restrict__ c
     const double* xi = coord + 3 * n;
     const double* vi = vel + 3 * n;
     for (int64_t j = 1; j <= cn[0]; ++j) {
+      // Link 1 starts as connection j and keeps the longer link of every
+      // swap below across the k loop (the reference assigns the j loop's
+      // link1/connectedNode1/tVecN1 in place, so each k compares against
+      // the longest of connections j..k-1).
+      int64_t lk1 = cn[2 * j - 1];
+      int64_t c1 = links[2 * (lk1 - 1) + (2 - cn[2 * j])];
+      double t1[3];
+      for (int d = 0; d < 3; ++d) t1[d] = xi[d] - coord[3 * (c1 - 1) + d];
+      double tn1 = dot3(t1, t1);
       for (int64_t k = j + 1; k <= cn[0]; ++k) {
-        int64_t lk1 = cn[2 * j - 1], lk2 = cn[2 * k - 1];
-        int64_t c1 = links[2 * (lk1 - 1) + (2 - cn[2 * j])];
+        int64_t lk2 = cn[2 * k - 1];
         int64_t c2 = links[2 * (lk2 - 1) + (2 - cn[2 * k])];
-        double t1[3], t2[3];
-        for (int d = 0; d < 3; ++d) {
-          t1[d] = xi[d] - coord[3 * (c1 - 1) + d];
-          t2[d] = xi[d] - coord[3 * (c2 - 1) + d];
-        }
-        if (dot3(t1, t1) < dot3(t2, t2)) {
+        double t2[3];
+        for (int d = 0; d < 3; ++d) t2[d] = xi[d] - coord[3 * (c2 - 1) + d];
+        double tn2 = dot3(t2, t2);
+        if (tn1 < tn2) {  // collapse the shorter one into a point
           const int64_t tl = lk1, tc = c1;
+          const double tt = tn1;
           lk1 = lk2;
           c1 = c2;
+          tn1 = tn2;
           lk2 = tl;
           c2 = tc;
+          tn2 = tt;
         }
         if (skipped(skip, nSkip, lk1, lk2)) continue;
         const double* y0 = coord + 3 * (c2 - 1);
@@ -429,12 +438,14 @@ __global__ __launch_bounds__(kCT) void k_collision_pairs(
 constexpr int kLastBlock = 256;
 
 // The last pair the reference evaluates: per row i (one thread, original
-// order) the largest j > i that passes the filters, scanned from the end;
-// the block's largest (i, j) goes to lastOut[blockIdx.x].
+// order) the largest j > i that passes the filters, scanned from the end
+// over the links with both ends mobile only (prevMobile[j]: the largest
+// such link <= j, -1 if none), so fixed links at the end of the array cost
+// nothing; the block's largest (i, j) goes to lastOut[blockIdx.x].
 __global__ __launch_bounds__(kLastBlock) void k_collision_last(
     const double* __restrict__ seg, int64_t ld, int64_t numSeg, const int64_t* __restrict__ nodes,
-    const int* __restrict__ flags, const int64_t* __restrict__ skip, int64_t nSkip,
-    CollisionLast* __restrict__ lastOut) {
+    const int* __restrict__ flags, const int* __restrict__ prevMobile,
+    const int64_t* __restrict__ skip, int64_t nSkip, CollisionLast* __restrict__ lastOut) {
   __shared__ CollisionLast rl[kLastBlock];
   const int64_t i = blockIdx.x * (int64_t)kLastBlock + threadIdx.x;
   CollisionLast v{-1, -1, 0.0, 0.0};
@@ -446,7 +457,7 @@ __global__ __launch_bounds__(kLastBlock) void k_collision_last(
     }
     const int64_t ni[2] = {nodes[2 * i], nodes[2 * i + 1]};
     const int fi = flags[i];
-    for (int64_t j = numSeg - 1; j > i; --j) {
+    for (int64_t j = prevMobile[numSeg - 1]; j > i; j = prevMobile[j - 1]) {
       double bj[3], sy[3];
       for (int d = 0; d < 3; ++d) {
         bj[d] = seg[(G_B + d) * ld + j];
@@ -584,10 +595,10 @@ void launch_collision_pairs(const double* seg, int64_t ld, int64_t numSeg, const
 }
 
 void launch_collision_last(const double* seg, int64_t ld, int64_t numSeg, const int64_t* nodes,
-                           const int* flags, const int64_t* skip, int64_t nSkip, int blocks,
-                           CollisionLast* last, hipStream_t st) {
+                           const int* flags, const int* prevMobile, const int64_t* skip,
+                           int64_t nSkip, int blocks, CollisionLast* last, hipStream_t st) {
   hipLaunchKernelGGL(k_collision_last, dim3(blocks), dim3(kLastBlock), 0, st, seg, ld, numSeg, nodes,
-                     flags, skip, nSkip, last);
+                     flags, prevMobile, skip, nSkip, last);
 }
 
 void launch_collision_final(const CollisionCand* best, int64_t nBest, const CollisionLast* last,

@@ -118,6 +118,22 @@ __global__ void k_pk_force(const double* __restrict__ seg, int64_t ld,
   }
 }
 
+// calc_sigmaHat (:92-192) at P points x0 (3 x P): out holds 9 doubles per
+// point, the reference's SMatrix{3,3} column by column (xx, xy, xz, xy, yy,
+// yz, xz, yz, zz).
+__global__ void k_sigma_hat(const double* __restrict__ x0, int64_t P, MeshDev M,
+                            double* __restrict__ out) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const v3<double> x = V(x0[3 * p], x0[3 * p + 1], x0[3 * p + 2]);
+  double s[6];
+  sigma_hat(M, x, s);
+  const double m[9] = {s[0], s[3], s[4], s[3], s[1], s[5], s[4], s[5], s[2]};
+  double* o = out + 9 * p;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) o[q] = m[q];
+}
+
 // Field-point stress: SoA per segment of n1, n2, b (uploaded), t, the
 // normalised tangent of :133, t × b, and three per-segment factors of the six
 // stress components q = (xx, yy, zz, xy, xz, yz) = (a, b') (the component
@@ -428,6 +444,11 @@ void launch_pk_force(const double* seg, int64_t ld, const int64_t* rows, int n, 
   if (n <= 0) return;
   hipLaunchKernelGGL(k_pk_force, dim3((n + 255) / 256), dim3(256), 0, st, seg, ld, rows, n, M,
                      out, mode);
+}
+
+void launch_sigma_hat(const double* x0, int64_t P, const MeshDev& M, double* out, hipStream_t st) {
+  if (P <= 0) return;
+  hipLaunchKernelGGL(k_sigma_hat, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, x0, P, M, out);
 }
 
 void launch_prep_sigma_segments(double* sseg, int64_t ld, int n, const SigmaK& K, hipStream_t st) {

@@ -38,7 +38,7 @@ struct MixedRule {
 // scan of the count pass), slot = (k << 6) | lane, tile = t, res = [12][total]
 // forces (F1, F2 on the row segment, F3, F4 on the column segment).
 struct SpecialList {
-  const int* off;
+  const long long* off;
   unsigned short* slot;
   int* tile;
   double* res;
@@ -51,10 +51,11 @@ void launch_special_count(const double* seg, int64_t ld, int n, const MixedRule*
                           unsigned* cnt, hipStream_t st);
 void launch_special_list(const double* seg, int64_t ld, int n, const MixedRule* mixed,
                          const SpecialList& sp, const SegK<double>& K, hipStream_t st);
-// off[0..n] = exclusive scan of cnt[0..n) (off[n] = total); work holds
-// scan_work_ints(n) ints.
+// off[0..n] = exclusive scan of cnt[0..n) in 64-bit sums (off[n] = total);
+// work holds scan_work_ints(n) long longs.
 int64_t scan_work_ints(int n);
-void launch_exclusive_scan(const unsigned* cnt, int n, int* off, int* work, hipStream_t st);
+void launch_exclusive_scan(const unsigned* cnt, int n, long long* off, long long* work,
+                           hipStream_t st);
 void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
                          double* rowPart, double* colPart, const SpecialList& sp, hipStream_t st,
                          const MixedRule* mixed = nullptr, unsigned long long* nExact = nullptr);
@@ -86,6 +87,8 @@ struct MeshDev {
 // mode 0: out is 3 x n (pk per segment); mode 1: out is 6 x n, pk/2 to both nodes.
 void launch_pk_force(const double* seg, int64_t ld, const int64_t* rows, int n, const MeshDev& M,
                      double* out, int mode, hipStream_t st);
+// sigmaHat at P points (3 x P): 9 doubles per point, 3 x 3 column-major.
+void launch_sigma_hat(const double* x0, int64_t P, const MeshDev& M, double* out, hipStream_t st);
 
 struct SigmaK {
   double a2, mu4pi, mu4pinu, nuMu4pinu, a2mu8pi;
@@ -167,9 +170,11 @@ void launch_collision_pairs(const double* seg, int64_t ld, int64_t numSeg, const
                             const int* flags, const int64_t* skip, int64_t nSkip,
                             double collisionDistSq, const int* perm, const double* box, int blocks,
                             CollisionCand* best, hipStream_t st);
+// prevMobile[j]: the largest link <= j (0-based) with both ends mobile, -1
+// if none.
 void launch_collision_last(const double* seg, int64_t ld, int64_t numSeg, const int64_t* nodes,
-                           const int* flags, const int64_t* skip, int64_t nSkip, int blocks,
-                           CollisionLast* last, hipStream_t st);
+                           const int* flags, const int* prevMobile, const int64_t* skip,
+                           int64_t nSkip, int blocks, CollisionLast* last, hipStream_t st);
 // seg: the packed links of the pair scan (the last pair's L1, L2 are
 // computed from it), or NULL when the entries carry them (hinges).
 void launch_collision_final(const CollisionCand* best, int64_t nBest, const CollisionLast* last,

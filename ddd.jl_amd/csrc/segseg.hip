@@ -67,19 +67,19 @@ __global__ __launch_bounds__(kTile) void k_special_eval(const double* __restrict
   for (int c = 0; c < 12; ++c) res[c * total + e] = v[c];
 }
 
-// Exclusive scan of n counts in three passes: block sums (2,048 per
+// Exclusive scan of n counts (64-bit sums) in three passes: block sums (2,048 per
 // workgroup), a one-workgroup scan of the block sums, and the blocks' own
 // scans offset by them.  off[n] receives the total.
 constexpr int kScanBlock = 2048;
 
 __global__ __launch_bounds__(256) void k_scan_sums(const unsigned* __restrict__ cnt, int n,
-                                                   int* __restrict__ sums) {
-  __shared__ int part[256];
+                                                   long long* __restrict__ sums) {
+  __shared__ long long part[256];
   const int b = blockIdx.x, tid = threadIdx.x;
   const int64_t i0 = (int64_t)b * kScanBlock + 8 * tid;
-  int a = 0;
+  long long a = 0;
   for (int q = 0; q < 8; ++q)
-    if (i0 + q < n) a += (int)cnt[i0 + q];
+    if (i0 + q < n) a += cnt[i0 + q];
   part[tid] = a;
   __syncthreads();
   for (int w = 128; w >= 1; w >>= 1) {
@@ -89,27 +89,27 @@ __global__ __launch_bounds__(256) void k_scan_sums(const unsigned* __restrict__ 
   if (tid == 0) sums[b] = part[0];
 }
 
-__global__ __launch_bounds__(1024) void k_scan_top(const int* __restrict__ sums, int nb,
-                                                   int* __restrict__ boff) {
-  __shared__ int part[1024];
+__global__ __launch_bounds__(1024) void k_scan_top(const long long* __restrict__ sums, int nb,
+                                                   long long* __restrict__ boff) {
+  __shared__ long long part[1024];
   const int tid = threadIdx.x;
   const int per = (nb + 1023) / 1024;
   const int b0 = tid * per, b1 = min(nb, b0 + per);
-  int a = 0;
+  long long a = 0;
   for (int b = b0; b < b1; ++b) a += sums[b];
   part[tid] = a;
   __syncthreads();
   if (tid == 0) {
-    int acc = 0;
+    long long acc = 0;
     for (int k = 0; k < 1024; ++k) {
-      const int v = part[k];
+      const long long v = part[k];
       part[k] = acc;
       acc += v;
     }
     boff[nb] = acc;
   }
   __syncthreads();
-  int acc = part[tid];
+  long long acc = part[tid];
   for (int b = b0; b < b1; ++b) {
     boff[b] = acc;
     acc += sums[b];
@@ -117,28 +117,28 @@ __global__ __launch_bounds__(1024) void k_scan_top(const int* __restrict__ sums,
 }
 
 __global__ __launch_bounds__(256) void k_scan_final(const unsigned* __restrict__ cnt, int n,
-                                                    const int* __restrict__ boff, int nb,
-                                                    int* __restrict__ off) {
-  __shared__ int part[256];
+                                                    const long long* __restrict__ boff, int nb,
+                                                    long long* __restrict__ off) {
+  __shared__ long long part[256];
   const int b = blockIdx.x, tid = threadIdx.x;
   const int64_t i0 = (int64_t)b * kScanBlock + 8 * tid;
-  int v[8], a = 0;
+  long long v[8], a = 0;
   for (int q = 0; q < 8; ++q) {
-    v[q] = (i0 + q < n) ? (int)cnt[i0 + q] : 0;
+    v[q] = (i0 + q < n) ? (long long)cnt[i0 + q] : 0;
     a += v[q];
   }
   part[tid] = a;
   __syncthreads();
   if (tid == 0) {
-    int acc = boff[b];
+    long long acc = boff[b];
     for (int k = 0; k < 256; ++k) {
-      const int x = part[k];
+      const long long x = part[k];
       part[k] = acc;
       acc += x;
     }
   }
   __syncthreads();
-  int acc = part[tid];
+  long long acc = part[tid];
   for (int q = 0; q < 8; ++q) {
     if (i0 + q < n) off[i0 + q] = acc;
     acc += v[q];
@@ -240,10 +240,11 @@ void launch_special_count(const double* seg, int64_t ld, int n, const MixedRule*
 
 int64_t scan_work_ints(int n) { return 2 * ((int64_t)(n + kScanBlock - 1) / kScanBlock + 1); }
 
-void launch_exclusive_scan(const unsigned* cnt, int n, int* off, int* work, hipStream_t st) {
+void launch_exclusive_scan(const unsigned* cnt, int n, long long* off, long long* work,
+                           hipStream_t st) {
   const int nb = (n + kScanBlock - 1) / kScanBlock;
-  int* sums = work;
-  int* boff = work + nb + 1;
+  long long* sums = work;
+  long long* boff = work + nb + 1;
   if (nb > 0) hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(256), 0, st, cnt, n, sums);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, sums, nb, boff);
   hipLaunchKernelGGL(k_scan_final, dim3(nb > 0 ? nb : 1), dim3(256), 0, st, cnt, n, boff, nb, off);

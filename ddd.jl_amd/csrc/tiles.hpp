@@ -69,7 +69,7 @@ template <int MODE, int LIST>
 __global__ __launch_bounds__(kTile) void k_special_pass(const double* __restrict__ seg, int64_t ld,
                                                         int n, int64_t tile0, double farDist2,
                                                         double sinMin2, unsigned* __restrict__ cnt,
-                                                        const int* __restrict__ off,
+                                                        const long long* __restrict__ off,
                                                         unsigned short* __restrict__ slot,
                                                         int* __restrict__ tileOf) {
   const int64_t t = tile0 + blockIdx.x;
@@ -93,8 +93,8 @@ __global__ __launch_bounds__(kTile) void k_special_pass(const double* __restrict
   const bool rowLive = (i < n) && seg[F_EXT * ld + i] == 0.0;
   const bool diag = (I == J);
   const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  int c = LIST ? off[t] : 0;
-  const int cEnd = LIST ? off[t + 1] : 0;
+  long long c = LIST ? off[t] : 0;
+  const long long cEnd = LIST ? off[t + 1] : 0;
   for (int k = 0; k < kTile; ++k) {
     const int s = (lane + k) & 63;
     const int64_t j = (int64_t)J * kTile + s;
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kTile) void k_special_pass(const double* __restrict
     const bool sp = is_special<MODE>(live, omcSq, midA, n1B, n2B, farDist2, sinMin2);
     const uint64_t m = __ballot(sp);
     if (LIST && sp) {
-      const int pos = c + __popcll(m & ltMask);
+      const long long pos = c + __popcll(m & ltMask);
       if (pos < cEnd) {  // always true: both passes classify identically
         slot[pos] = (unsigned short)((k << 6) | lane);
         tileOf[pos] = (int)t;
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
                                                            double farDist2, double sinMin2,
                                                            double* __restrict__ rowPart,
                                                            double* __restrict__ colPart,
-                                                           const int* __restrict__ specOff,
+                                                           const long long* __restrict__ specOff,
                                                            const unsigned short* __restrict__ specSlot,
                                                            const double* __restrict__ specRes,
                                                            int64_t specTotal,
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
   if (MODE == 1 && nq > 0) flush(nq);
   if (MODE == 1 && lane == 0 && exact) atomicAdd(nExact, (unsigned long long)exact);
   // this tile's near-parallel pairs, evaluated beforehand, in list order
-  for (int e = specOff[t], e1 = specOff[t + 1]; e < e1; ++e) {
+  for (long long e = specOff[t], e1 = specOff[t + 1]; e < e1; ++e) {
     const unsigned q = specSlot[e];
     const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
     if (rl == lane) {

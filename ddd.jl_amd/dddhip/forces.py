@@ -131,6 +131,20 @@ def calcPKForce_(mesh, forceDisplacement, network, idx=None, ctx=None):
           L.DDD_ACCUMULATE | L.DDD_PK_HALF_TO_NODES)
 
 
+def calc_σHat(mesh, forceDisplacement, x0, ctx=None):
+    """calc_σHat (DislocationForces.jl:92-192): the 3 x 3 stress of the FE
+    correction at point x0; for a 3 x P array of points, 3 x 3 x P."""
+    ctx = _ctx(ctx)
+    mv = L.MeshView(mesh, forceDisplacement)
+    x = np.asarray(x0, dtype=np.float64)
+    single = x.ndim == 1
+    pts = _points(x)
+    P = pts.shape[1]
+    out = np.zeros((3, 3, P), order="F")
+    _call(ctx, "ddd_sigma_hat", mv.s, L.ptr(pts, "d"), P, L.ptr(out, "d"))
+    return out[:, :, 0].copy(order="F") if single else out
+
+
 def calcSegForce(dlnParams, matParams, mesh, forceDisplacement, network, idx=None, ctx=None):
     """calcSegForce (DislocationForces.jl:23-46): seg-seg + self + PK/2."""
     ctx = _ctx(ctx)

@@ -10,6 +10,7 @@ module DDDHip
 
 using DDD
 using SparseArrays
+using StaticArrays
 
 const libddd = joinpath(@__DIR__, "..", "libdddhip.so")
 
@@ -76,8 +77,10 @@ netstruct(n::DislocationNetwork) = Network(
     pointer(n.links), size(n.links, 2),
     pointer(n.connectivity), size(n.connectivity, 1), size(n.connectivity, 2))
 
-meshstruct(mesh, uHat::Vector{Float64}) = Mesh(
-    pointer(Matrix(mesh.C)), pointer(mesh.coord), size(mesh.coord, 2),
+# The caller keeps C (Matrix(mesh.C)) and uHat (the densified uHat) alive with
+# GC.@preserve for as long as the struct is used.
+meshstruct(mesh, C::Matrix{Float64}, uHat::Vector{Float64}) = Mesh(
+    pointer(C), pointer(mesh.coord), size(mesh.coord, 2),
     pointer(mesh.connectivity), size(mesh.connectivity, 2),
     mesh.mx, mesh.my, mesh.mz, mesh.w, mesh.h, mesh.d, pointer(uHat))
 
@@ -141,9 +144,7 @@ function calcSegForce!(dlnParams, matParams, mesh, forceDisplacement, network::D
     C = Matrix(mesh.C)
     GC.@preserve network ix uHat C mesh begin
         nw = Ref(netstruct(network))
-        ms = Ref(Mesh(pointer(C), pointer(mesh.coord), size(mesh.coord, 2),
-                      pointer(mesh.connectivity), size(mesh.connectivity, 2),
-                      mesh.mx, mesh.my, mesh.mz, mesh.w, mesh.h, mesh.d, pointer(uHat)))
+        ms = Ref(meshstruct(mesh, C, uHat))
         check(ccall((:ddd_seg_force, libddd), Cint,
             (Ptr{Cvoid}, Ref{Consts}, Ref{Network}, Ref{Mesh}, Ptr{Int64}, Int64, Ptr{Float64},
              Int64, Cint),
@@ -151,6 +152,99 @@ function calcSegForce!(dlnParams, matParams, mesh, forceDisplacement, network::D
             pointer(network.segForce), size(network.segForce, 3), DDD_ACCUMULATE))
     end
     return nothing
+end
+
+function calcSegForce(dlnParams, matParams, mesh, forceDisplacement, network::DislocationNetwork,
+                      idx = nothing)
+    dlnParams.parGPU ||
+        return DDD.calcSegForce(dlnParams, matParams, mesh, forceDisplacement, network, idx)
+    k = Ref(consts(dlnParams, matParams))
+    ix = idxvec(idx)
+    m = isnothing(idx) ? network.numSeg[1] : length(ix)
+    out = zeros(3, 2, m)
+    uHat = Vector{Float64}(forceDisplacement.uHat)
+    C = Matrix(mesh.C)
+    GC.@preserve network ix uHat C mesh out begin
+        nw = Ref(netstruct(network))
+        ms = Ref(meshstruct(mesh, C, uHat))
+        check(ccall((:ddd_seg_force, libddd), Cint,
+            (Ptr{Cvoid}, Ref{Consts}, Ref{Network}, Ref{Mesh}, Ptr{Int64}, Int64, Ptr{Float64},
+             Int64, Cint),
+            ctx(), k, nw, ms, isnothing(idx) ? C_NULL : pointer(ix), length(ix),
+            pointer(out), m, DDD_OUT_BY_POSITION))
+    end
+    return out
+end
+
+function self_call(dlnParams, matParams, network, idx, out, flags)
+    k = Ref(consts(dlnParams, matParams))
+    ix = idxvec(idx)
+    GC.@preserve network ix out begin
+        nw = Ref(netstruct(network))
+        check(ccall((:ddd_self_force, libddd), Cint,
+            (Ptr{Cvoid}, Ref{Consts}, Ref{Network}, Ptr{Int64}, Int64, Ptr{Float64}, Int64, Cint),
+            ctx(), k, nw, isnothing(idx) ? C_NULL : pointer(ix), length(ix),
+            pointer(out), size(out, 3), flags))
+    end
+end
+
+# calcSelfForce: (selfForceNode1, selfForceNode2), each 3 x numSeg (or
+# 3 x length(idx)), node 1 = -node 2.
+function calcSelfForce(dlnParams, matParams, network::DislocationNetwork, idx = nothing)
+    dlnParams.parGPU || return DDD.calcSelfForce(dlnParams, matParams, network, idx)
+    m = isnothing(idx) ? network.numSeg[1] : length(idxvec(idx))
+    out = zeros(3, 2, m)
+    self_call(dlnParams, matParams, network, idx, out, DDD_OUT_BY_POSITION)
+    return out[:, 1, :], out[:, 2, :]
+end
+
+function calcSelfForce!(dlnParams, matParams, network::DislocationNetwork, idx = nothing)
+    dlnParams.parGPU || return DDD.calcSelfForce!(dlnParams, matParams, network, idx)
+    self_call(dlnParams, matParams, network, idx, network.segForce, DDD_ACCUMULATE)
+    return nothing
+end
+
+# calcPKForce / calcPKForce! / calc_σHat carry no DislocationParameters in the
+# reference, so there is no parGPU switch: these methods always run on the GPU.
+function pk_call(mesh, forceDisplacement, network, idx, out, cols, flags)
+    ix = idxvec(idx)
+    uHat = Vector{Float64}(forceDisplacement.uHat)
+    C = Matrix(mesh.C)
+    GC.@preserve network ix uHat C mesh out begin
+        nw = Ref(netstruct(network))
+        ms = Ref(meshstruct(mesh, C, uHat))
+        check(ccall((:ddd_pk_force, libddd), Cint,
+            (Ptr{Cvoid}, Ref{Network}, Ref{Mesh}, Ptr{Int64}, Int64, Ptr{Float64}, Int64, Cint),
+            ctx(), nw, ms, isnothing(idx) ? C_NULL : pointer(ix), length(ix), pointer(out), cols,
+            flags))
+    end
+end
+
+function calcPKForce(mesh, forceDisplacement, network::DislocationNetwork, idx = nothing)
+    m = isnothing(idx) ? network.numSeg[1] : length(idxvec(idx))
+    out = zeros(3, m)
+    pk_call(mesh, forceDisplacement, network, idx, out, m, DDD_OUT_BY_POSITION)
+    return out
+end
+
+function calcPKForce!(mesh, forceDisplacement, network::DislocationNetwork, idx = nothing)
+    pk_call(mesh, forceDisplacement, network, idx, network.segForce, size(network.segForce, 3),
+            DDD_ACCUMULATE | DDD_PK_HALF_TO_NODES)
+    return nothing
+end
+
+function calc_σHat(mesh, forceDisplacement, x0)
+    x = Vector{Float64}(collect(x0))
+    σ = zeros(3, 3)
+    uHat = Vector{Float64}(forceDisplacement.uHat)
+    C = Matrix(mesh.C)
+    GC.@preserve x σ uHat C mesh begin
+        ms = Ref(meshstruct(mesh, C, uHat))
+        check(ccall((:ddd_sigma_hat, libddd), Cint,
+            (Ptr{Cvoid}, Ref{Mesh}, Ptr{Float64}, Int64, Ptr{Float64}),
+            ctx(), ms, pointer(x), Int64(1), pointer(σ)))
+    end
+    return SMatrix{3, 3, Float64}(σ)
 end
 
 function calc_σTilde!(σ, x0, dlnParams, matParams, network::DislocationNetwork, idx = nothing)
@@ -433,9 +527,7 @@ function dynamics_upload(dlnParams, matParams, mesh, forceDisplacement, network)
     C = Matrix(mesh.C)
     GC.@preserve network label uHat C mesh begin
         nw = Ref(netstruct(network))
-        ms = Ref(Mesh(pointer(C), pointer(mesh.coord), size(mesh.coord, 2),
-                      pointer(mesh.connectivity), size(mesh.connectivity, 2),
-                      mesh.mx, mesh.my, mesh.mz, mesh.w, mesh.h, mesh.d, pointer(uHat)))
+        ms = Ref(meshstruct(mesh, C, uHat))
         check(ccall((:ddd_net_upload, libddd), Cint,
             (Ptr{Cvoid}, Ref{Consts}, Ref{Network}, Ref{Mesh}), ctx(), k, nw, ms))
         check(ccall((:ddd_net_upload_dynamics, libddd), Cint,

@@ -231,6 +231,11 @@ int ddd_self_force(ddd_ctx* ctx, const ddd_consts* k, const ddd_network* net,
 int ddd_pk_force(ddd_ctx* ctx, const ddd_network* net, const ddd_mesh* mesh,
                  const int64_t* idx, int64_t nIdx, double* out, int64_t outCols, int flags);
 
+/* calc_sigmaHat (DislocationForces.jl:92-192): the FE correction stress of
+ * the mesh's dense uHat at P points x0 (3 x P); sigma: 3 x 3 x P, each point's
+ * symmetric tensor column-major as the reference's SMatrix{3,3}. */
+int ddd_sigma_hat(ddd_ctx* ctx, const ddd_mesh* mesh, const double* x0, int64_t P, double* sigma);
+
 /* calcSegForce / calcSegForce! (DislocationForces.jl:23-71): PK/2 + self +
  * seg-seg.  mesh may be NULL (no FE correction: PK = 0).  out: 3 x 2 x
  * outCols.  The in-place form zeroes the addressed columns first: call it

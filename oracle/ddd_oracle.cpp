@@ -1023,16 +1023,23 @@ void orc_detect_collision(double collisionDistSq, int64_t numNode, int64_t numSe
     if (!mobile(i)) continue;
     const int64_t* cn = connectivity + ldConn * (i - 1);
     for (int64_t j = 1; j <= cn[0]; ++j) {
+      // DislocationTopologyCollision.jl:30-41: link1, connectedNode1 and
+      // tVecN1 belong to the j loop; the swap at :52-68 assigns them, so it
+      // carries over to later k (link 1 is the longest of j..k-1).
+      int64_t lk1 = cn[2 * j - 1];
+      int64_t c1 = links[2 * (lk1 - 1) + (2 - cn[2 * j])];
+      const V3<double> xi = X(i);
+      const V3<double> t1 = xi - X(c1);
+      double tn1 = dot(t1, t1);
       for (int64_t k = j + 1; k <= cn[0]; ++k) {
-        int64_t lk1 = cn[2 * j - 1], lk2 = cn[2 * k - 1];
-        int64_t c1 = links[2 * (lk1 - 1) + (2 - cn[2 * j])];
+        int64_t lk2 = cn[2 * k - 1];
         int64_t c2 = links[2 * (lk2 - 1) + (2 - cn[2 * k])];
-        const V3<double> xi = X(i);
-        const V3<double> t1 = xi - X(c1), t2 = xi - X(c2);
-        const double tn1 = dot(t1, t1), tn2 = dot(t2, t2);
-        if (tn1 < tn2) {  // collapse the shorter one into a point
+        const V3<double> t2 = xi - X(c2);
+        double tn2 = dot(t2, t2);
+        if (tn1 < tn2) {  // :52-68, collapse the shorter one into a point
           std::swap(lk1, lk2);
           std::swap(c1, c2);
+          std::swap(tn1, tn2);
         }
         if (skipped(lk1, lk2)) continue;
         const MinDist m = minimum_distance(xi, X(c1), X(c2), X(c2), Vn(i), Vn(c1), Vn(c2), Vn(c2));

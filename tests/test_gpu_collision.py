@@ -78,3 +78,35 @@ def test_no_collision_returns_last_evaluated_L():
     net = _moving(32, 0xC9)
     g = G.detectCollision(dln, net)
     assert g[0] is False and g[1] == "null" and g == O.detectCollision(dln, net)
+
+
+@pytest.mark.parametrize("case", range(2))
+def test_junction_hinge_carry(case):
+    """A node with three links: the reference's hinge loop compares each
+    connection k with the longest of connections j..k-1 (its swap assigns
+    the j loop's link variables); the hand-worked results are in
+    tests/test_oracle_collision.py."""
+    from test_oracle_collision import JUNCTION_CASES, junction_network, junction_params
+    a_pos, a_vel, dist, expect = JUNCTION_CASES[case]
+    _log(f"collision junction case {case}")
+    net = junction_network(a_pos, a_vel)
+    got = G.detectCollision(junction_params(dist), net)
+    assert got[:8] == expect[:8]
+    assert got[8] == expect[8] and got[9] == expect[9]
+    assert got == O.detectCollision(junction_params(dist), net)
+
+
+def test_last_pair_behind_fixed_links():
+    """Links at the end of the array whose nodes are fixed are never
+    evaluated; the last evaluated pair lies before them."""
+    dln = DislocationParameters(coreRad=90.78, collisionDist=1e-3)
+    net = _moving(64, 0xCB)
+    n = int(net.numNode[0])
+    lab = net.label.copy()
+    lab[n - 640:n] = intFixDln  # the last 20 loops
+    net = DislocationNetwork._assemble(net.links, net.slipPlane, net.bVec, net.coord, lab, n,
+                                       int(net.numSeg[0]), net.maxConnect)
+    net.nodeVel[:, :n] = np.random.default_rng(0xCB).standard_normal((3, n))
+    _log("collision with fixed links at the end")
+    g = G.detectCollision(dln, net)
+    assert g[0] is False and g == O.detectCollision(dln, net)

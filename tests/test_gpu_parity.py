@@ -77,6 +77,28 @@ def test_golden_pk(prm):
     # is zero there, which the zero-uHat mobility test covers.
 
 
+def test_golden_sigma_hat():
+    """calc_σHat through ddd_sigma_hat against SegmentForcesTest.jl:636-650
+    (uHat :617-634), and against the oracle at points across the mesh,
+    including points outside it (the clamped element) and on element faces."""
+    m = R.mesh()
+    fd = R.sparse_uhat(m, "sigmahat_uHat_index", "sigmahat_uHat_value")
+    s1 = G.calc_σHat(m, fd, [1800.0, 1400.0, 200.0])
+    s2 = G.calc_σHat(m, fd, [1893.0, 1356.0, 345.0])
+    assert s1.shape == (3, 3)
+    assert isapprox(s1, R.g("sigmahat_1"))
+    assert isapprox(s2, R.g("sigmahat_2"))
+    rng = np.random.default_rng(0x5E)
+    faces = np.array([[0.0, 800.0, 2000.0], [400.0, 0.0, 2000.0], [2000.0, 1200.0, 0.0]])
+    pts = np.asfortranarray(np.concatenate([rng.uniform(-200.0, 2200.0, (3, 61)), faces], axis=1))
+    S = G.calc_σHat(m, fd, pts)
+    assert S.shape == (3, 3, pts.shape[1])
+    for p in range(pts.shape[1]):
+        o = O.calc_σHat(m, fd, pts[:, p])
+        assert np.array_equal(S[:, :, p], S[:, :, p].T)
+        assert normwise(S[:, :, p], o) <= TOL, p
+
+
 @pytest.mark.parametrize("b,key", [([1, 0, 0], "sigmatilde_edge"), ([0, 0, 1], "sigmatilde_screw")])
 def test_golden_sigma_tilde(b, key):
     pts, dln, mat, net = R.sigma_segment(b)

@@ -60,3 +60,49 @@ def test_detect_collision_golden(k):
     assert got[0] == exp[0] and got[1] == exp[1]
     assert list(got[2:8]) == [int(v) for v in exp[2:8]]
     assert got[8] == exp[8] and got[9] == exp[9]
+
+
+def junction_network(a_pos, a_vel):
+    """Node 1 at the origin joined to A (node 2, the shortest link, link 1),
+    B = (0, 3, 0) (node 3, the longest, link 2) and C = (0, 0, 2) (node 4,
+    link 3), in that connection order; every node mobile, only A moves."""
+    from dddhip.types import DislocationNetwork, intMobDln
+    links = np.array([[1, 1, 1], [2, 3, 4]])
+    coord = np.array([[0.0, 0.0, 0.0], a_pos, [0.0, 3.0, 0.0], [0.0, 0.0, 2.0]]).T
+    bVec = np.tile(np.array([[1.0], [1.0], [1.0]]) / np.sqrt(3.0), (1, 3))
+    net = DislocationNetwork.from_arrays(links, bVec, coord, [intMobDln] * 4)
+    net.nodeVel[:, 1] = a_vel
+    return net
+
+
+def junction_params(dist):
+    from dddhip.types import DislocationParameters
+    return DislocationParameters(coreRad=90.78, collisionDist=dist)
+
+
+# The reference's hinge loop at node 1 (DislocationTopologyCollision.jl:24-126)
+# worked by hand.  link1/connectedNode1/tVecN1 are the j loop's variables
+# and the swap at :52-68 assigns them, so each k is compared with the longest
+# of connections j..k-1:
+#   j = 1 (A): k = 2 (B): |A|^2 < |B|^2, swap -> pair (B, A);
+#              k = 3 (C): link 1 is now B, |B|^2 > |C|^2 -> pair (B, C);
+#   j = 2 (B): k = 3 (C) -> pair (B, C), the last pair evaluated.
+# Starting each k from connection j afresh would give (B, A), (C, A), (B, C).
+# Every link shares node 1, so the two-line scan evaluates nothing, and L1, L2
+# are those of (B, C): point C against segment 1 -> B, clamped at node 1.
+JUNCTION_CASES = [
+    # A next to segment 1 -> C only: (C, A) would collide, (B, A) is 0.51 away
+    ((0.1, 0.0, 0.5), (-1.0, 0.0, 0.0), 0.2,
+     (False, "null", 0, 0, 0, 0, 0, 0, 0.0, 0.0)),
+    # A within 0.6 of both lines: (C, A) at 0.316 would win, (B, A) at 0.51 does
+    ((0.1, 0.3, 0.5), (-1.0, -1.0, -1.0), 0.6,
+     (True, "hinge", 1, 3, 2, 2, 2, 1, 0.0, 0.0)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(JUNCTION_CASES)))
+def test_detect_collision_junction_hinge_carry(case):
+    a_pos, a_vel, dist, expect = JUNCTION_CASES[case]
+    got = O.detectCollision(junction_params(dist), junction_network(a_pos, a_vel))
+    assert got[:8] == expect[:8]
+    assert got[8] == expect[8] and got[9] == expect[9]
