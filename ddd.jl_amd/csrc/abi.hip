@@ -556,6 +556,9 @@ int ddd_ctx_create(ddd_ctx** out, int device) {
     return DDD_ERR_HIP;
   }
   c->st = c->own;
+  // The allocation-free entry point's scratch: its address on this device,
+  // looked up once per device; a failure is reported by that entry point.
+  (void)noalloc_init(device);
   *out = c;
   return DDD_OK;
 }
@@ -787,6 +790,49 @@ int ddd_sigma_tilde(ddd_ctx* c, const ddd_consts* k, const ddd_network* net, con
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(sigma, c->sigma.p, sizeof(double) * 6 * P, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
+int ddd_sigma_tilde_noalloc(ddd_ctx* c, const ddd_consts* k, int64_t numSeg, const int64_t* segIdx,
+                            int64_t segIdxLd, const double* coord, int64_t coordCols,
+                            const double* bVec, int64_t bVecCols, const int64_t* idx, int64_t nIdx,
+                            const double* x0, int64_t P, double* sigma, int flags) {
+  return guard(c, "ddd_sigma_tilde_noalloc", [&] {
+    if (!k) bad("consts is NULL");
+    if (numSeg < 0 || numSeg > segIdxLd) bad("numSeg outside [0, segIdxLd]");
+    if (numSeg > (int64_t)1 << 30) bad("numSeg too large");
+    if (numSeg > 0 && (!segIdx || !coord || !bVec)) bad("segIdx, coord and bVec are required");
+    if (coordCols < 0 || bVecCols < 0) bad("negative column counts");
+    if (nIdx < 0 || (nIdx > 0 && !idx) || nIdx > (int64_t)1 << 30) bad("idx/nIdx inconsistent");
+    if (P < 0 || (P > 0 && (!x0 || !sigma))) bad("points/sigma inconsistent");
+    const int64_t m = idx ? nIdx : numSeg;
+    const int64_t ld = std::max<int64_t>(m, 1);
+    if (P > ((int64_t)1 << 34)) bad("too many points");
+    // scratch: the segment SoA, then (256-byte aligned) the chunk partials;
+    // nChunk as ddd_sigma_tilde picks it (same sums, same bits), halved
+    // until it fits
+    const int64_t cap = kNoAllocScratchDoubles;
+    const int64_t soa = ((int64_t)kSigmaFields * ld + 31) / 32 * 32;
+    if (soa > cap || (P > 0 && m > 0 && (cap - soa) / 6 < P))
+      bad("segments and points exceed the 64 MiB scratch (33 x segments + 6 x points doubles "
+          "must fit in 2^23)");
+    double* base = noalloc_scratch(c->device);
+    if (!base) throw Err{DDD_ERR_HIP, "the scratch array's device address is unavailable"};
+    if (P == 0) return;
+    int nChunk = m > 0 ? sigma_chunks_for(P, (int)m) : 0;
+    while (nChunk > 1 && soa + 6 * P * (int64_t)nChunk > cap) nChunk /= 2;
+    double* sseg = base;
+    double* part = base + soa;
+    const SigmaK K{k->coreRadSq, k->mu4pi, k->mu4pinu, k->nuMu4pinu, k->coreRadSq * k->mu8pi};
+    if (m > 0) {
+      launch_gather_sigma_segments(segIdx, segIdxLd, numSeg, coord, coordCols, bVec, bVecCols, idx,
+                                   (int)m, sseg, ld, c->st);
+      launch_prep_sigma_segments(sseg, ld, (int)m, K, c->st);
+      launch_sigma_tilde(sseg, ld, (int)m, x0, P, K, part, nChunk, c->st);
+    }
+    // nChunk 0 (no segments): sigma = 0, or unchanged when accumulating
+    launch_sigma_reduce(part, nChunk, P, sigma, (flags & DDD_ACCUMULATE) ? 1 : 0, c->st);
+    HIPCHK(hipGetLastError());
   });
 }
 

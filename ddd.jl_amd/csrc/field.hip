@@ -175,6 +175,26 @@ __global__ void k_prep_sigma_segments(double* __restrict__ s, int64_t ld, int n,
   }
 }
 
+__global__ void k_gather_sigma_segments(const int64_t* __restrict__ segIdx, int64_t L,
+                                        int64_t numSeg, const double* __restrict__ coord,
+                                        int64_t coordCols, const double* __restrict__ bVec,
+                                        int64_t bVecCols, const int64_t* __restrict__ idx, int m,
+                                        double* __restrict__ s, int64_t ld) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const int64_t i = idx ? idx[q] - 1 : q;
+  bool ok = i >= 0 && i < numSeg;
+  const int64_t lk = ok ? segIdx[i] : 0, a = ok ? segIdx[i + L] : 0, b = ok ? segIdx[i + 2 * L] : 0;
+  ok = ok && lk >= 1 && lk <= bVecCols && a >= 1 && a <= coordCols && b >= 1 && b <= coordCols;
+  const double nan = __longlong_as_double(0x7ff8000000000000ll);
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    s[(S_N1 + e) * ld + q] = ok ? coord[3 * (a - 1) + e] : nan;
+    s[(S_N2 + e) * ld + q] = ok ? coord[3 * (b - 1) + e] : nan;
+    s[(S_B + e) * ld + q] = ok ? bVec[3 * (lk - 1) + e] : nan;
+  }
+}
+
 constexpr int kSigBlock = 256;
 constexpr int kSigTile = 64;
 
@@ -454,6 +474,15 @@ void launch_sigma_hat(const double* x0, int64_t P, const MeshDev& M, double* out
 void launch_prep_sigma_segments(double* sseg, int64_t ld, int n, const SigmaK& K, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_prep_sigma_segments, dim3((n + 255) / 256), dim3(256), 0, st, sseg, ld, n, K);
+}
+
+void launch_gather_sigma_segments(const int64_t* segIdx, int64_t segIdxLd, int64_t numSeg,
+                                  const double* coord, int64_t coordCols, const double* bVec,
+                                  int64_t bVecCols, const int64_t* idx, int m, double* sseg,
+                                  int64_t ld, hipStream_t st) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_gather_sigma_segments, dim3((m + 255) / 256), dim3(256), 0, st, segIdx,
+                     segIdxLd, numSeg, coord, coordCols, bVec, bVecCols, idx, m, sseg, ld);
 }
 
 int sigma_chunks_for(int64_t P, int nseg) {

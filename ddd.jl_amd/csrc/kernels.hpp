@@ -103,6 +103,22 @@ void launch_sigma_reduce(const double* partial, int nChunk, int64_t P, double* s
                          int accumulate, hipStream_t st);
 constexpr int kSigmaFields = 33;
 void launch_prep_sigma_segments(double* sseg, int64_t ld, int n, const SigmaK& K, hipStream_t st);
+// Segments of the reference's arrays, all on the device (segIdx segIdxLd x 3,
+// coord 3 x coordCols, bVec 3 x bVecCols, idx: m 1-based indices or NULL for
+// 1..m), gathered into fields n1, n2, b of the kSigmaFields SoA sseg (ld); a
+// segment whose indices fall outside the arrays gets NaN coordinates.
+void launch_gather_sigma_segments(const int64_t* segIdx, int64_t segIdxLd, int64_t numSeg,
+                                  const double* coord, int64_t coordCols, const double* bVec,
+                                  int64_t bVecCols, const int64_t* idx, int m, double* sseg,
+                                  int64_t ld, hipStream_t st);
+
+// Fixed device scratch of the allocation-free entry point (noalloc.hip):
+// 64 MiB per device.  noalloc_init looks up the array's device address with
+// `device` current (once per device); noalloc_scratch returns it (NULL if
+// the lookup failed or never ran).
+constexpr int64_t kNoAllocScratchDoubles = (int64_t)1 << 23;
+hipError_t noalloc_init(int device);
+double* noalloc_scratch(int device);
 
 void launch_node_gather(const int64_t* conn, int64_t ldConn, int64_t numNode,
                         const int64_t* links, const double* coord, const double* segForce,

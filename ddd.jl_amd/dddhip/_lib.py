@@ -124,6 +124,8 @@ def lib():
                                             C.POINTER(ddd_mesh), ip, i64, dp, i64, C.c_int]),
                 "ddd_sigma_tilde": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_network),
                                               ip, i64, dp, i64, dp, C.c_int]),
+                "ddd_sigma_tilde_noalloc": (C.c_int, [P, C.POINTER(ddd_consts), i64, P, i64, P,
+                                                      i64, P, i64, P, i64, P, i64, P, C.c_int]),
                 "ddd_node_force": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_network),
                                              dp, i64, dp]),
                 "ddd_net_upload_dynamics": (C.c_int, [P, C.POINTER(ddd_network),

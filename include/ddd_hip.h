@@ -251,6 +251,26 @@ int ddd_sigma_tilde(ddd_ctx* ctx, const ddd_consts* k, const ddd_network* net,
                     const int64_t* idx, int64_t nIdx, const double* x0, int64_t P,
                     double* sigma, int flags);
 
+/* calc_sigmaTilde without device allocation.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Every
+ * array is a DEVICE pointer in the reference's layout: segIdx segIdxLd x 3,
+ * coord 3 x coordCols, bVec 3 x bVecCols, idx nIdx 1-based segment indices
+ * (NULL: 1..numSeg), x0 3 x P, sigma 6 x P (xx, yy, zz, xy, xz, yz); k is a
+ * host pointer.  The call only enqueues work on the context's stream
+ * (ddd_ctx_set_stream) and returns; the result is ready when the stream
+ * reaches that point.  Temporaries live in a fixed 64 MiB device array per
+ * GPU, held until the process ends: kSigmaFields (33) x segments + 6 x
+ * points x chunks doubles must fit in 2^23 (calls beyond that return
+ * DDD_ERR_ARG before any work).  Calls that overlap on one GPU share that
+ * array and race.  A segment whose indices lie outside the arrays gives NaN
+ * stresses.  DDD_ACCUMULATE adds into sigma. */
+int ddd_sigma_tilde_noalloc(ddd_ctx* ctx, const ddd_consts* k, int64_t numSeg,
+                            const int64_t* segIdx, int64_t segIdxLd, const double* coord,
+                            int64_t coordCols, const double* bVec, int64_t bVecCols,
+                            const int64_t* idx, int64_t nIdx, const double* x0, int64_t P,
+                            double* sigma, int flags);
+
 /* Node-force gather of dlnMobility! (DislocationMobility.jl:220-255):
  * nodeForce[:, n] = sum over n's connections of segForce[:, col, link],
  * skipping zero-length links and forces below sigmaPN.  segForce: 3 x 2 x
