@@ -100,6 +100,7 @@ constexpr int64_t kGemmWork = 4 * (int64_t)kHidden * kMlpOutPad;
 struct ddd_mlp {
   DevBuf P, G, Mm, Vv, Wb;
   float aH = 0.f;  // core radius in cell units of the grid it was trained for
+  int cubic = 1;   // learns W at canonical offsets, expanded by cubic symmetry
   int64_t step = 0;
   int cap = 0;
   DevBuf X0, Z1, Z2, Z3, H1, H2, H3, Y, dY, dYb, dH, dZb, lossPer, off3, colPart, gemmWork;
@@ -290,7 +291,7 @@ void check_flags(ddd_ctx* c) {
 MixedRule mixed_rule(const ddd_consts& k, const ddd_opts& o) {
   MixedRule r;
   r.farDist = o.farDist > 0 ? o.farDist : 5.0 * k.coreRad;
-  r.sinMin = o.sinMin > 0 ? o.sinMin : 1e-3;
+  r.sinMin = o.sinMin != 0 ? o.sinMin : 1e-3;
   r.K = SigK<float>{float(k.coreRadSq), float(k.mu4pi), float(k.mu4pinu), float(k.nuMu4pinu),
                     float(k.coreRadSq * k.mu8pi)};
   return r;
@@ -300,7 +301,7 @@ void check_opts(const ddd_opts* o) {
   if (!o) return;
   if (o->mode != DDD_MODE_FP64 && o->mode != DDD_MODE_MIXED && o->mode != DDD_MODE_HYBRID)
     bad("unknown mode");
-  if (o->farDist < 0 || o->sinMin < 0) bad("farDist and sinMin must be >= 0");
+  if (!(o->farDist >= 0) || std::isnan(o->sinMin)) bad("farDist must be >= 0 and sinMin a number");
   if (o->mode == DDD_MODE_HYBRID) {
     if (!o->farfield) bad("hybrid mode needs a farfield (ddd_farfield_create)");
     if (!o->farfield->haveOperator) bad("the farfield has no operator (ddd_farfield_set_operator)");
@@ -1648,19 +1649,34 @@ float env_scale(const ddd_consts& k, const ddd_farfield* ff) { return (float)(k.
 void operator_from_mlp(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_consts& k) {
   const int G = ff->g.G, S = 2 * G - 1;
   m->aH = (float)(k.coreRad / ff->g.h);
-  const int no = S * S * S;
+  // The network is evaluated at every offset of the grid, or (cubic) at the
+  // canonical offsets only, whose operators the symmetry expands.
+  const int no = m->cubic ? canonical_index(G - 1, G - 1, G - 1) + 1 : S * S * S;
   std::vector<int> off((size_t)3 * no);
-  for (int z = 0; z < S; ++z)
-    for (int y = 0; y < S; ++y)
-      for (int x = 0; x < S; ++x) {
-        const int o = x + S * (y + S * z);
-        off[3 * o] = x - (G - 1);
-        off[3 * o + 1] = y - (G - 1);
-        off[3 * o + 2] = z - (G - 1);
-      }
-  ScopedBuf dW, scratch, dOff;
+  if (m->cubic) {
+    for (int a = 0; a < G; ++a)
+      for (int b = 0; b <= a; ++b)
+        for (int cc = 0; cc <= b; ++cc) {
+          const int o = canonical_index(a, b, cc);
+          off[3 * o] = a;
+          off[3 * o + 1] = b;
+          off[3 * o + 2] = cc;
+        }
+  } else {
+    for (int z = 0; z < S; ++z)
+      for (int y = 0; y < S; ++y)
+        for (int x = 0; x < S; ++x) {
+          const int o = x + S * (y + S * z);
+          off[3 * o] = x - (G - 1);
+          off[3 * o + 1] = y - (G - 1);
+          off[3 * o + 2] = z - (G - 1);
+        }
+  }
+  ScopedBuf dW, dWc, scratch, dOff;
   const int chunk = 4096;
-  dW.ensure(sizeof(float) * (int64_t)no * kChan);
+  dW.ensure(sizeof(float) * (int64_t)S * S * S * kChan);
+  if (m->cubic) dWc.ensure(sizeof(float) * (int64_t)no * kChan);
+  float* Wout = m->cubic ? dWc.as<float>() : dW.as<float>();
   dOff.ensure(sizeof(int) * 3 * no);
   HIPCHK(hipMemcpyAsync(dOff.p, off.data(), sizeof(int) * 3 * no, hipMemcpyHostToDevice, c->st));
   m->ensure_batch(chunk);
@@ -1670,8 +1686,9 @@ void operator_from_mlp(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_const
                           hipMemcpyDeviceToDevice, c->st));
     mlp_forward(m, cnt, c->st);
     mlp_operator_from_y(m->Y.as<float>(), m->off3.as<int>(), cnt, env_scale(k, ff),
-                        dW.as<float>() + (int64_t)o0 * kChan, c->st);
+                        Wout + (int64_t)o0 * kChan, c->st);
   }
+  if (m->cubic) operator_expand(dWc.as<float>(), G, dW.as<float>(), c->st);
   scratch.ensure(sizeof(float2) * kChanPairs * ff->vol);
   ff->What.ensure(sizeof(float2) * kChan * ff->volH);
   launch_operator_spectrum(dW.as<float>(), ff->g, ff->rn, ff->tw.as<float2>(), scratch.as<float2>(),
@@ -1859,6 +1876,7 @@ int ddd_farfield_train(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_const
     const int G = ff->g.G, rn = ff->rn;
     if (rn + 1 > G - 1) bad("no far offsets on this grid");
     m->aH = (float)(k->coreRad / ff->g.h);
+    m->cubic = o->plainOffsets ? 0 : 1;
     const int nc = G * G * G;
     // Training network: its own segment buffer (an uploaded network stays);
     // cells and moments in the farfield's buffers, which the next hybrid
@@ -1951,7 +1969,7 @@ int ddd_farfield_train(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_const
         cursor = 0;
       }
       gather_batch(dPerm.as<int>() + cursor, B, dOff.as<int>(), dSrc.as<int>(), ff->M.as<float>(), nc,
-                   dLt.as<float>(), dDen.as<float>(), m->off3.as<int>(), bM.as<float>(),
+                   dLt.as<float>(), dDen.as<float>(), m->cubic, m->off3.as<int>(), bM.as<float>(),
                    bL.as<float>(), bD.as<float>(), c->st);
       cursor += B;
       mlp_forward(m, B, c->st);
@@ -1984,7 +2002,7 @@ int ddd_farfield_train(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_const
       for (int q = 0; q < cnt; ++q) vi[(size_t)q] = (int)(v0 + q);
       HIPCHK(hipMemcpyAsync(vIdx.p, vi.data(), sizeof(int) * cnt, hipMemcpyHostToDevice, c->st));
       gather_batch(vIdx.as<int>(), cnt, dOff.as<int>(), dSrc.as<int>(), ff->M.as<float>(), nc,
-                   dLt.as<float>(), dDen.as<float>(), m->off3.as<int>(), bM.as<float>(),
+                   dLt.as<float>(), dDen.as<float>(), m->cubic, m->off3.as<int>(), bM.as<float>(),
                    bL.as<float>(), bD.as<float>(), c->st);
       mlp_forward(m, cnt, c->st);
       mlp_contract(m->Y.as<float>(), m->off3.as<int>(), bM.as<float>(), bL.as<float>(), bD.as<float>(),

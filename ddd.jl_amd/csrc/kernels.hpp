@@ -27,9 +27,13 @@ __host__ __device__ inline int64_t tiles_for(int64_t nb) { return nb * (nb + 1) 
 void launch_prep_segments(double* seg, int64_t ld, int64_t n, hipStream_t st);
 // Mixed-precision rule: pairs farther than farDist (midpoint distance) whose
 // tangents satisfy |t1 x t2| > sinMin go to the FP32 far-pair quadrature.
+// sinMin < 0: no angle condition (near-parallel pairs beyond farDist go to
+// the quadrature too, which their parallelism does not affect).
 struct MixedRule {
   double farDist, sinMin;
   SigK<float> K;
+  double farDist2() const { return farDist * farDist; }
+  double sinMin2() const { return sinMin < 0 ? -1.0 : sinMin * sinMin; }
 };
 // mixed == nullptr: plain FP64.  nExact (mixed only) accumulates the number
 // of pairs evaluated on the FP64 path.

@@ -16,7 +16,7 @@ void launch_special_pass_mixed(const double* seg, int64_t ld, int n, const Mixed
   const int64_t nb = (n + kTile - 1) / kTile;
   const int64_t T = tiles_for(nb);
   const int64_t maxGrid = 1ll << 30;
-  const double fd2 = mr.farDist * mr.farDist, sm2 = mr.sinMin * mr.sinMin;
+  const double fd2 = mr.farDist2(), sm2 = mr.sinMin2();
   for (int64_t t0 = 0; t0 < T; t0 += maxGrid) {
     const int64_t count = (T - t0 < maxGrid) ? (T - t0) : maxGrid;
     if (sp)
@@ -37,7 +37,7 @@ void launch_segseg_tiles_mixed(const double* seg, int64_t ld, int n, const SegK<
   for (int64_t t0 = 0; t0 < T; t0 += maxGrid) {
     const int64_t cnt = (T - t0 < maxGrid) ? (T - t0) : maxGrid;
     hipLaunchKernelGGL(k_segseg_tiles<1>, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, t0, K,
-                       mr.K, mr.farDist * mr.farDist, mr.sinMin * mr.sinMin, rowPart, colPart, sp.off,
+                       mr.K, mr.farDist2(), mr.sinMin2(), rowPart, colPart, sp.off,
                        sp.slot, sp.res, sp.total, nExact);
   }
 }

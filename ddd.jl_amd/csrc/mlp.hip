@@ -443,21 +443,81 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 // Batch b of the training set: sample idx[b]; moments from the cell array
-// M (feature-major, nc cells).
+// M (feature-major, nc cells).  cubic: the sample is carried into the frame
+// of its canonical offset o' (o = R o'): offset o', moments T_M(R)^T M and
+// target det(R) T_L(R)^T L, so that the network learns W at canonical
+// offsets only.
 __global__ void k_gather_batch(const int* __restrict__ idx, int nb, const int* __restrict__ off3,
                                const int* __restrict__ src, const float* __restrict__ M, int nc,
                                const float* __restrict__ Lt, const float* __restrict__ den,
-                               int* __restrict__ bOff, float* __restrict__ bM,
+                               int cubic, int* __restrict__ bOff, float* __restrict__ bM,
                                float* __restrict__ bL, float* __restrict__ bDen) {
   const int b = blockIdx.x;
   const int t = threadIdx.x;
   if (b >= nb) return;
   const int64_t s = idx[b];
   const int c = src[s];
-  if (t < kMoments) bM[(int64_t)b * kMoments + t] = M[(int64_t)t * nc + c];
-  if (t < kLocal) bL[(int64_t)b * kLocal + t] = Lt[s * kLocal + t];
-  if (t < 3) bOff[(int64_t)b * 3 + t] = off3[s * 3 + t];
+  AxisMap R{{0, 1, 2}, {1, 1, 1}};
+  int oc[3] = {off3[s * 3], off3[s * 3 + 1], off3[s * 3 + 2]};
+  if (cubic) canonical_offset(oc[0], oc[1], oc[2], oc, R);
+  if (t < kMoments) {
+    int k2 = t, sg = 1;
+    if (cubic) moment_map(R, t, k2, sg);
+    bM[(int64_t)b * kMoments + t] = (float)sg * M[(int64_t)k2 * nc + c];
+  }
+  if (t < kLocal) {
+    int q2 = t, sg = 1;
+    if (cubic) {
+      local_map(R, t, q2, sg);
+      sg *= axis_det(R);
+    }
+    bL[(int64_t)b * kLocal + t] = (float)sg * Lt[s * kLocal + q2];
+  }
+  if (t < 3) bOff[(int64_t)b * 3 + t] = oc[t];
   if (t == 0) bDen[b] = den[s];
+}
+
+// W(o) for every offset o of the (2G - 1)^3 grid (x fastest) from Wc, the
+// operator at the canonical offsets (canonical_index order):
+// W(o) = det(R) T_L(R) W(o') T_M(R)^T for o = R o', averaged over the maps that
+// fix o' (offsets with equal or zero components), so that the installed
+// operator is exactly equivariant.  One workgroup per offset.
+__global__ __launch_bounds__(256) void k_operator_expand(const float* __restrict__ Wc, int G,
+                                                         float* __restrict__ W) {
+  const int S = 2 * G - 1;
+  const int64_t o = blockIdx.x;
+  __shared__ AxisMap maps[48];
+  __shared__ int nmap, ci;
+  if (threadIdx.x == 0) {
+    const int ox = (int)(o % S) - (G - 1), oy = (int)((o / S) % S) - (G - 1),
+              oz = (int)(o / ((int64_t)S * S)) - (G - 1);
+    int oc[3];
+    AxisMap R;
+    canonical_offset(ox, oy, oz, oc, R);
+    int n = 0;
+    for (int k = 0; k < 48; ++k) {
+      const AxisMap Sk = axis_map(k);
+      int v[3];
+      for (int e = 0; e < 3; ++e) v[Sk.perm[e]] = Sk.sgn[e] * oc[e];
+      if (v[0] == oc[0] && v[1] == oc[1] && v[2] == oc[2]) maps[n++] = inverse(compose(R, Sk));
+    }
+    nmap = n;
+    ci = canonical_index(oc[0], oc[1], oc[2]);
+  }
+  __syncthreads();
+  const float* wc = Wc + (int64_t)ci * kChan;
+  const float inv = 1.f / (float)nmap;
+  for (int e = threadIdx.x; e < kChan; e += blockDim.x) {
+    const int Q = e / kMoments, K = e % kMoments;
+    float a = 0.f;
+    for (int m = 0; m < nmap; ++m) {
+      int q, sq, k, sk;
+      local_map(maps[m], Q, q, sq);
+      moment_map(maps[m], K, k, sk);
+      a += (float)(axis_det(maps[m]) * sq * sk) * wc[q * kMoments + k];
+    }
+    W[o * kChan + e] = a * inv;
+  }
 }
 
 // Exact local expansion of source cell src[s] over the cell at offset
@@ -586,10 +646,15 @@ void mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr,
 }
 
 void gather_batch(const int* idx, int nb, const int* off3, const int* src, const float* M, int nc,
-                  const float* Lt, const float* den, int* bOff, float* bM, float* bL, float* bDen,
-                  hipStream_t st) {
+                  const float* Lt, const float* den, int cubic, int* bOff, float* bM, float* bL,
+                  float* bDen, hipStream_t st) {
   hipLaunchKernelGGL(k_gather_batch, dim3(nb), dim3(128), 0, st, idx, nb, off3, src, M, nc, Lt, den,
-                     bOff, bM, bL, bDen);
+                     cubic, bOff, bM, bL, bDen);
+}
+
+void operator_expand(const float* Wc, int G, float* W, hipStream_t st) {
+  const int64_t S = 2 * G - 1;
+  hipLaunchKernelGGL(k_operator_expand, dim3((unsigned)(S * S * S)), dim3(256), 0, st, Wc, G, W);
 }
 
 void local_targets(const double* seg, int64_t ld, const GridDev& g, const int* start,

@@ -43,9 +43,12 @@ void mlp_operator_from_y(const float* Y, const int* off3, int no, float envScale
                          hipStream_t st);
 void mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
               float c1, float c2, float eps, hipStream_t st);
+// cubic: samples carried into the frame of their canonical offset.
 void gather_batch(const int* idx, int nb, const int* off3, const int* src, const float* M, int nc,
-                  const float* Lt, const float* den, int* bOff, float* bM, float* bL, float* bDen,
-                  hipStream_t st);
+                  const float* Lt, const float* den, int cubic, int* bOff, float* bM, float* bL,
+                  float* bDen, hipStream_t st);
+// W: (2G - 1)^3 x kChan from the canonical operator Wc (C(G + 2, 3) x kChan).
+void operator_expand(const float* Wc, int G, float* W, hipStream_t st);
 
 // L2 projection onto the quadratic basis from the 27 tensor Gauss points of a
 // cell: node p at (xi[p % 3], xi[(p / 3) % 3], xi[p / 9]) (cell units,

@@ -364,7 +364,7 @@ void launch_near_tiles(const double* segS, int64_t ldS, const int* key, int rn, 
   // SIMD busy until the list ends; each owns kNearStage staging entries.
   const int grid = nTiles < kNearGrid ? nTiles : kNearGrid;
   hipLaunchKernelGGL(k_near_tiles, dim3((unsigned)grid), dim3(kTile), 0, st, segS, ldS, key, rn, list,
-                     nTiles, K, mr.K, mr.farDist * mr.farDist, mr.sinMin * mr.sinMin, rowPart,
+                     nTiles, K, mr.K, mr.farDist2(), mr.sinMin2(), rowPart,
                      colPart, stage, spList, spCap, specOff, specCnt, counters);
 }
 

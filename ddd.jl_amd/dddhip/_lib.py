@@ -78,7 +78,8 @@ class ddd_stats(C.Structure):
 class ddd_train_opts(C.Structure):
     _fields_ = [("samples", i64), ("valSamples", i64), ("steps", C.c_int), ("batch", C.c_int),
                 ("lr", C.c_float), ("lrFinal", C.c_float), ("seed", C.c_uint64),
-                ("denFloor", C.c_float), ("nearFrac", C.c_float), ("nearShells", C.c_int)]
+                ("denFloor", C.c_float), ("nearFrac", C.c_float), ("nearShells", C.c_int),
+                ("plainOffsets", C.c_int)]
 
 
 class ddd_train_report(C.Structure):

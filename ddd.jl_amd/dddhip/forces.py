@@ -422,16 +422,18 @@ class FarFieldMLP:
 
 def trainFarField(dlnParams, matParams, farfield, mlp, trainNetwork, samples=1 << 20,
                   valSamples=1 << 15, steps=3000, batch=2048, lr=1e-3, lrFinal=3e-5, seed=1,
-                  denFloor=0.0, nearFrac=0.0, nearShells=0):
+                  denFloor=0.0, nearFrac=0.0, nearShells=0, cubic=True):
     """Train mlp on exact stresses of trainNetwork binned on farfield's grid
-    and install the operator it predicts.  Returns the report as a dict."""
+    and install the operator it predicts.  cubic: learn the operator at the
+    canonical offsets only and expand it by the cubic symmetry of isotropic
+    elasticity (otherwise at every offset).  Returns the report as a dict."""
     ctx = farfield.ctx
     nv = L.NetView(trainNetwork)
     k = L.consts(dlnParams, matParams)
     o = L.ddd_train_opts(samples=int(samples), valSamples=int(valSamples), steps=int(steps),
                          batch=int(batch), lr=float(lr), lrFinal=float(lrFinal), seed=int(seed),
                          denFloor=float(denFloor), nearFrac=float(nearFrac),
-                         nearShells=int(nearShells))
+                         nearShells=int(nearShells), plainOffsets=0 if cubic else 1)
     r = L.ddd_train_report()
     _call(ctx, "ddd_farfield_train", farfield.h_ptr, mlp.h_ptr, k, nv.s, o, r)
     out = {f: getattr(r, f) for f, _ in L.ddd_train_report._fields_}

@@ -344,6 +344,7 @@ struct TrainOpts
     denFloor::Float32
     nearFrac::Float32
     nearShells::Cint
+    plainOffsets::Cint
 end
 
 struct TrainReport
@@ -395,7 +396,7 @@ function train_farfield!(ff, mlp, dlnParams, matParams, trainNetwork::Dislocatio
                          samples = 1 << 24, valSamples = 1 << 16, steps = 60000, batch = 4096,
                          lr = 2f-3, lrFinal = 5f-6, seed = 7, denFloor = 0.1f0)
     k = Ref(consts(dlnParams, matParams))
-    o = Ref(TrainOpts(samples, valSamples, steps, batch, lr, lrFinal, seed, denFloor, 0f0, 0))
+    o = Ref(TrainOpts(samples, valSamples, steps, batch, lr, lrFinal, seed, denFloor, 0f0, 0, 0))
     rep = Ref{TrainReport}()
     GC.@preserve trainNetwork begin
         nw = Ref(netstruct(trainNetwork))

@@ -112,7 +112,7 @@ typedef struct ddd_farfield ddd_farfield;
 typedef struct ddd_opts {
   int mode;
   double farDist;  /* mixed/hybrid: FP32 beyond this midpoint distance (0: 5 coreRad) */
-  double sinMin;   /* mixed/hybrid: ... and |t1 x t2| above this (0: 1e-3)           */
+  double sinMin;   /* mixed/hybrid: ... and |t1 x t2| above this (0: 1e-3; < 0: any angle) */
   ddd_farfield* farfield; /* hybrid: cell grid and far-field operator              */
 } ddd_opts;
 
@@ -162,6 +162,8 @@ typedef struct ddd_train_opts {
   float denFloor;      /* loss floor, relative to the offset's typical stress (0: 1e-2) */
   float nearFrac;      /* share of sources drawn from the first shells (0: 0.5)          */
   int nearShells;      /* number of those shells outside the near stencil (0: 3)        */
+  int plainOffsets;    /* 1: learn W at every offset; 0: at canonical offsets only, the  */
+                       /*    rest by cubic symmetry, W(R o) = det(R) T_L W(o) T_M^T      */
 } ddd_train_opts;
 typedef struct ddd_train_report {
   double lossFirst, lossLast;  /* mean training loss of the first / last 50 steps */

@@ -28,3 +28,15 @@ def normwise(test, ref):
 
 def isapprox(test, ref, rtol=np.sqrt(np.finfo(np.float64).eps)):
     return normwise(test, ref) <= rtol
+
+
+def node_forces(net, sf):
+    """Segment-end forces 3 x 2 x numSeg gathered to the nodes they act on
+    (3 x nodes used), as dlnMobility!'s node-force gather adds them."""
+    n = sf.shape[2]
+    si = net.segIdx[:n]
+    F = np.zeros((net.coord.shape[1], 3))
+    np.add.at(F, si[:, 1] - 1, np.asarray(sf)[:, 0, :].T)
+    np.add.at(F, si[:, 2] - 1, np.asarray(sf)[:, 1, :].T)
+    used = np.unique(si[:, 1:3]) - 1
+    return F[used].T

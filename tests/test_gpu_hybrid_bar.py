@@ -1,0 +1,97 @@
+"""GPU: the hybrid's accuracy bar end to end, against the oracle.
+
+The learned far field is trained on one network and judged on another of a
+size the CPU oracle evaluates in seconds (16,384 segments), on the bench's
+grid (G = 32 over [-300, 2300]^3, near radius rn = 4): per-node relative
+error of the total force (self + seg-seg) against calcSegSegForcePar
+(DislocationForces.jl:719-777), RMS over every node, at most 1e-3, and well
+below the error without a far field.  Uniform loops (the 131,072-segment
+workload's generator) and clustered pile-ups (the 262,144-segment stress
+test's generator); calc_σTilde of the clustered network at seeded random
+surface points against the oracle."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from dddhip import forces as G
+from dddhip.generate import clustered_loop_network, random_loop_network, surface_points
+from dddhip.types import DislocationParameters, MaterialParameters
+from metrics import node_forces, normwise, per_node_rel
+
+pytestmark = pytest.mark.gpu
+
+LO, SIDE = -300.0, 2600.0
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def _prm():
+    return (DislocationParameters(coreRad=90.78, coreEnergy=1.0),
+            MaterialParameters(μ=1.0, ν=0.28))
+
+
+def _bar(ev, train, near_frac, farDist):
+    dln, mat = _prm()
+    ref = O.calcSegSegForcePar(dln, mat, ev, 16)
+    s1, s2 = O.calcSelfForce(dln, mat, ev)
+    ref[:, 0] += s1
+    ref[:, 1] += s2
+    Fr = node_forces(ev, ref)
+    ff = G.FarField(32, (LO, LO, LO), SIDE / 32, 4)
+    S = 2 * 32 - 1
+    ff.set_operator(np.zeros((S ** 3, 60, 90), np.float32))
+    g0, _, _ = G.calcSegSegForceHybrid(dln, mat, ev, ff, farDist=farDist)
+    g0[:, 0] += s1
+    g0[:, 1] += s2
+    m = G.FarFieldMLP(seed=11)
+    rep = G.trainFarField(dln, mat, ff, m, train, samples=1 << 23, valSamples=1 << 15,
+                          steps=20000, batch=4096, lr=2e-3, lrFinal=5e-6, seed=7, denFloor=0.1,
+                          nearFrac=near_frac)
+    g, near, near64 = G.calcSegSegForceHybrid(dln, mat, ev, ff, farDist=farDist)
+    g[:, 0] += s1
+    g[:, 1] += s2
+    _, rms0 = per_node_rel(node_forces(ev, g0), Fr)
+    mx, rms = per_node_rel(node_forces(ev, g), Fr)
+    _log(f"trained {rep['seconds']:.1f} s, held-out {rep['relErrVal']:.2e}; node error rms "
+         f"{rms:.3e} max {mx:.3e} (no far field: rms {rms0:.3e}); {near} near pairs, "
+         f"{near64} FP64")
+    m.close()
+    ff.close()
+    return rms, rms0
+
+
+def test_uniform_network_hybrid_bar():
+    """Uniform loops: 512 evaluated, trained on 4,096 others; the mixed
+    rule of the near field (FP64 within 5 core radii)."""
+    _log("hybrid bar: uniform 16,384 segments")
+    rms, rms0 = _bar(random_loop_network(512, seed=0xB0), random_loop_network(4096, seed=0xD1),
+                     0.85, 0.0)
+    assert rms <= 1e-3 and rms <= 0.1 * rms0
+
+
+def test_clustered_network_hybrid_bar():
+    """Pile-ups: 70 % of 512 loops in 8 clusters, trained on another
+    clustered network of 4,096 loops; FP64 within 3 core radii and for the
+    near-parallel pairs, the stress test's rule."""
+    _log("hybrid bar: clustered 16,384 segments")
+    rms, rms0 = _bar(clustered_loop_network(512, seed=0xE3),
+                     clustered_loop_network(4096, seed=0xE4), 0.2, 3 * 90.78)
+    assert rms <= 1e-3 and rms <= 0.1 * rms0
+
+
+def test_clustered_sigma_tilde_surface_points():
+    """calc_σTilde (DislocationFEM.jl:24-303) of a clustered network at 2^14
+    seeded random points on the faces of the 2000^3 cube, every 8th point
+    against the oracle."""
+    dln, mat = _prm()
+    net = clustered_loop_network(512, seed=0xE3)
+    x = surface_points(1 << 14, seed=0xE6)
+    _log("clustered calc_σTilde at 16,384 surface points")
+    s = G.calc_σTilde(x, dln, mat, net)
+    o = O.calc_σTilde(x[:, ::8], dln, mat, net)
+    assert np.all(np.isfinite(s))
+    assert normwise(s[:, ::8], o) <= 1e-10
+    for p in range(o.shape[1]):
+        assert normwise(s[:, 8 * p], o[:, p]) <= 1e-9, p

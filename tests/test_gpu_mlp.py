@@ -116,3 +116,51 @@ def test_training_and_ablation(prm):
     assert e_trained < 0.7 * e_none and e_trained < 0.7 * e_untrained
     m.close()
     ff.close()
+
+
+def _cell_map(Rm, G):
+    """Cell index t -> the cell R t of the grid rotated about its centre."""
+    import symmetry_ref as S
+    A = S.matrix(Rm)
+    t = np.arange(G ** 3)
+    c3 = np.stack([t % G, (t // G) % G, t // (G * G)]).astype(float) + 0.5 - G / 2
+    r3 = np.rint(A @ c3 + G / 2 - 0.5).astype(int)
+    return r3[0] + G * (r3[1] + G * r3[2])
+
+
+@pytest.mark.parametrize("cubic", [True, False])
+def test_operator_cubic_equivariance(prm, cubic):
+    """The installed far-field operator of a network that learns W at the
+    canonical offsets is exactly equivariant: the local expansions of a
+    network rotated or reflected about the grid centre are
+    det(R) T_L(R) of the original cells' expansions, up to the FP32 FFT
+    (an untrained network, so nothing but the symmetry makes it so).  A
+    network that learns every offset on its own is not."""
+    import symmetry_ref as S
+    dln, mat = prm
+    Gc, rn, h = 8, 1, 250.0
+    net = random_loop_network(24, seed=0x5B)
+    ff = G.FarField(Gc, (0.0, 0.0, 0.0), h, rn)
+    m = G.FarFieldMLP(seed=4)
+    if not cubic:  # ten steps, only to switch the network to every offset
+        G.trainFarField(dln, mat, ff, m, net, samples=1 << 12, valSamples=256, steps=10,
+                        batch=256, lr=1e-4, lrFinal=1e-4, seed=1, cubic=False)
+    G.setFarFieldFromMLP(dln, mat, ff, m)
+    G.calcSegSegForceHybrid(dln, mat, net, ff, farDist=1e30)
+    L0 = ff.debug(1).astype(np.float64)
+    worst = 0.0
+    for k in (9, 21, 38, 47):
+        Rm = S.maps()[k]
+        G.calcSegSegForceHybrid(dln, mat, S.rotate_network(net, Rm, (1000.0,) * 3), ff,
+                                farDist=1e30)
+        L1 = ff.debug(1).astype(np.float64)
+        want = S.det(Rm) * S.T_locals(Rm) @ L0
+        got = L1[:, _cell_map(Rm, Gc)]
+        worst = max(worst, np.linalg.norm(got - want) / np.linalg.norm(want))
+    print(f"cubic={cubic}: worst relative mismatch {worst:.2e}", flush=True)
+    if cubic:
+        assert worst <= 1e-4
+    else:
+        assert worst >= 1e-2
+    m.close()
+    ff.close()
