@@ -405,6 +405,37 @@ def collision_leg(args, net, ctx, log):
             "cpu_oracle_ms_scaled": round(cpu, 1), "cpu_sample": f"{ns} segments, 1 thread"}
 
 
+def big_leg(args, dln, mat, ctx, log):
+    """Plain FP64 on a 4x larger network (16,384 loops x 32 = 524,288
+    segments, seed 0xF0): one resident evaluation after one untimed, with
+    the chunks of the all-pairs triangle and the device memory the context
+    holds (the partial sums of an unchunked triangle would need 192 GiB)."""
+    import numpy as np
+    import torch
+    from dddhip import forces as G
+    from dddhip.generate import random_loop_network
+    net = random_loop_network(args.big_loops, seed=0xF0)
+    n = int(net.numSeg[0])
+    res = G.ResidentNetwork(dln, mat, net, ctx=ctx)
+    stream = torch.cuda.ExternalStream(res.stream(), device=torch.device("cuda:0"))
+    res.seg_force()
+    stream.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    res.seg_force()
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    nbytes, chunks = res.footprint()
+    f = res.download()
+    pairs = n * (n - 1) // 2
+    log(f"plain FP64 at {n} segments: {ms:.1f} ms, {chunks} chunks, "
+        f"{nbytes / 2**30:.2f} GiB on the device")
+    return {"segments": n, "pairs": pairs, "fp64_ms": round(ms, 2),
+            "tflops_algorithmic": round(FLOPS_PER_PAIR * pairs / (ms * 1e-3) / 1e12, 3),
+            "chunks": chunks, "device_bytes": nbytes, "finite": bool(np.isfinite(f).all())}
+
+
 def normwise_local(a, b):
     import numpy as np
     d = max(np.linalg.norm(a), np.linalg.norm(b))
@@ -605,6 +636,9 @@ def main():
                     help="calc_uTilde leg: cantilever mesh of this many elements per side "
                          "(0 skips the leg)")
     ap.add_argument("--utilde-cpu-points", type=int, default=4)
+    ap.add_argument("--big-loops", type=int, default=16384,
+                    help="plain FP64 leg on a network of this many loops (seed 0xF0, 524,288 "
+                         "segments by default): time, chunks, device memory; 0 to skip")
     ap.add_argument("--no-collision", dest="collision", action="store_false",
                     help="skip the detectCollision leg")
     ap.add_argument("--no-dynamics", dest="dynamics", action="store_false",
@@ -742,6 +776,7 @@ def main():
     clustered = clustered_leg(args, dln, mat, ctx, log) if args.clustered else None
     utilde = utilde_leg(args, mat, net, ctx, log) if args.utilde_grid > 0 else None
     collision = collision_leg(args, net, ctx, log) if args.collision else None
+    big = big_leg(args, dln, mat, ctx, log) if args.big_loops > 0 else None
     # replaces the context's resident network: after every leg that uses it
     small = (small_configs_leg(args, dln, mat, ctx, log, max(1, min(16, os.cpu_count() or 1)))
              if not args.no_cpu else None)
@@ -778,6 +813,7 @@ def main():
         "clustered": clustered,
         "u_tilde": utilde,
         "detect_collision": collision,
+        "plain_fp64_524288": big,
         "plain_fp64_small": small,
         "roofline": {
             "bound": "valu",

@@ -148,7 +148,10 @@ struct ddd_ctx {
   std::vector<double> h;  // host staging
   DevBuf counter;  // FP64-pair counter of the mixed path
   // near-parallel pairs of the all-pairs tiles (count, offsets, list, forces)
-  DevBuf specCnt, specOff, specSlot, specTile, specRes, specWork;
+  DevBuf specCnt, specOff, specSlot, specTile, specRes, specWork, specAt;
+  TileRange pending{0, 0};  // the all-pairs chunk whose reduction is still to run
+  int chunks = 0;           // chunks of the last all-pairs evaluation
+  int64_t chunkBytes = (int64_t)16 << 30;
   // device-resident dynamics (deriv!, integrate!)
   DevBuf dCoord, dCoord0, dVel, dVel0, dNodeF, dLinks, dConn, dBvec, dLabel, dSegNodes, dRedPart,
       dRed;
@@ -429,22 +432,36 @@ void hybrid_apply(ddd_ctx* c, int64_t n, double* dev, const ddd_opts& o) {
   HIPCHK(hipGetLastError());
 }
 
-// Pair tiles of the whole network (mode from opts) without the reduction:
-// count pass and scan for the near-parallel pairs (one host synchronisation
-// reads their number, which sizes the list), their listing and evaluation,
-// then the tile kernel, which adds them to its partial sums.
+// Device memory of one chunk of the all-pairs triangle: its partial sums
+// (row and column, 6 doubles x 64 lanes each per tile) and its near-parallel
+// list (slot, tile, 12 forces per pair), at most the context's chunkBytes
+// (default 16 GiB).  A chunk holds whole column blocks, so a single block
+// column above the budget (at most 65,535 tiles under the tile-count limit)
+// makes a chunk of its own.
+constexpr int64_t kTileBytes = 2 * 6 * kTile * (int64_t)sizeof(double);
+constexpr int64_t kSpecialBytes = sizeof(unsigned short) + sizeof(int) + 12 * sizeof(double);
+
+// Pair tiles of the whole network (mode from opts), chunk by chunk: a count
+// pass and scan over every tile for the near-parallel pairs (one host
+// synchronisation reads the list offsets at the column-block boundaries,
+// which size the chunks), then per chunk the listing and evaluation of its
+// near-parallel pairs, the tile kernel (which adds them to its partial
+// sums) and the reduction of its partials into dev -- except the last
+// chunk's reduction, which segseg_finish runs (the bench times the tile
+// stage on its own; with one chunk, as at 131,072 segments, that is the
+// whole reduction).  Memory is bounded by kChunkBytes instead of growing
+// with the N^2 / 8192 tiles.
 void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
-                  const ddd_opts* opts) {
+                  const ddd_opts* opts, double* dev) {
+  c->pending = TileRange{0, 0};
   if (n < 2) return;
   const int64_t nb = (n + kTile - 1) / kTile;
   const int64_t T = tiles_for(nb);
   if (T + 1 > INT32_MAX) bad("too many segments for the tile list");
-  const size_t part = sizeof(double) * 6 * kTile * (size_t)T;
-  c->rowPart.ensure(part);
-  c->colPart.ensure(part);
   c->counter.ensure(sizeof(unsigned long long));
   c->specCnt.ensure(sizeof(unsigned) * T);
   c->specOff.ensure(sizeof(long long) * (T + 1));
+  c->specAt.ensure(sizeof(long long) * (nb + 1));
   const bool mixed = opts && opts->mode == DDD_MODE_MIXED;
   const MixedRule r = mixed ? mixed_rule(k, *opts) : MixedRule{};
   launch_special_count(c->seg.as<double>(), c->resLd, (int)n, mixed ? &r : nullptr,
@@ -452,27 +469,56 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
   c->specWork.ensure(sizeof(long long) * scan_work_ints((int)T));
   launch_exclusive_scan(c->specCnt.as<unsigned>(), (int)T, c->specOff.as<long long>(),
                         c->specWork.as<long long>(), c->st);
-  long long total = 0;
-  HIPCHK(hipMemcpyAsync(&total, c->specOff.as<long long>() + T, sizeof(total),
+  launch_block_offsets(c->specOff.as<long long>(), (int)nb, c->specAt.as<long long>(), c->st);
+  std::vector<long long> offAt((size_t)nb + 1);
+  HIPCHK(hipMemcpyAsync(offAt.data(), c->specAt.p, sizeof(long long) * (nb + 1),
                         hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
-  // 64-bit offsets throughout; the list itself must still fit in HBM.
-  const int64_t tot1 = std::max<long long>(total, 1);
-  c->specSlot.ensure(sizeof(unsigned short) * tot1);
-  c->specTile.ensure(sizeof(int) * tot1);
-  c->specRes.ensure(sizeof(double) * 12 * tot1);
-  SpecialList sp{c->specOff.as<long long>(), c->specSlot.as<unsigned short>(), c->specTile.as<int>(),
-                 c->specRes.as<double>(), (int64_t)total, flags};
-  launch_special_list(c->seg.as<double>(), c->resLd, (int)n, mixed ? &r : nullptr, sp, seg_consts(k),
-                      c->st);
-  if (mixed) {
-    HIPCHK(hipMemsetAsync(c->counter.p, 0, sizeof(unsigned long long), c->st));
-    launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k), c->rowPart.as<double>(),
-                        c->colPart.as<double>(), sp, c->st, &r, c->counter.as<unsigned long long>());
-  } else {
-    launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k), c->rowPart.as<double>(),
-                        c->colPart.as<double>(), sp, c->st);
+  // chunks of whole column blocks within the budget
+  std::vector<TileRange> chunks;
+  int64_t maxTiles = 1, maxSpec = 1;
+  auto bytes = [&](int J0, int J1) {
+    return kTileBytes * (tiles_for(J1) - tiles_for(J0)) + kSpecialBytes * (offAt[J1] - offAt[J0]);
+  };
+  for (int J0 = 0; J0 < nb;) {
+    int J1 = J0 + 1;
+    while (J1 < nb && bytes(J0, J1 + 1) <= c->chunkBytes) ++J1;
+    chunks.push_back(TileRange{J0, J1});
+    maxTiles = std::max<int64_t>(maxTiles, tiles_for(J1) - tiles_for(J0));
+    maxSpec = std::max<int64_t>(maxSpec, offAt[J1] - offAt[J0]);
+    J0 = J1;
   }
+  c->chunks = (int)chunks.size();
+  c->rowPart.ensure(sizeof(double) * 6 * kTile * (size_t)maxTiles);
+  c->colPart.ensure(sizeof(double) * 6 * kTile * (size_t)maxTiles);
+  c->specSlot.ensure(sizeof(unsigned short) * maxSpec);
+  c->specTile.ensure(sizeof(int) * maxSpec);
+  c->specRes.ensure(sizeof(double) * 12 * maxSpec);
+  if (mixed) HIPCHK(hipMemsetAsync(c->counter.p, 0, sizeof(unsigned long long), c->st));
+  for (size_t q = 0; q < chunks.size(); ++q) {
+    const TileRange& ch = chunks[q];
+    SpecialList sp{c->specOff.as<long long>(), c->specSlot.as<unsigned short>(), c->specTile.as<int>(),
+                   c->specRes.as<double>(), (int64_t)(offAt[ch.J1] - offAt[ch.J0]),
+                   (int64_t)offAt[ch.J0], flags};
+    launch_special_list(c->seg.as<double>(), c->resLd, (int)n, mixed ? &r : nullptr, sp,
+                        seg_consts(k), ch, c->st);
+    launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k), c->rowPart.as<double>(),
+                        c->colPart.as<double>(), sp, ch, c->st, mixed ? &r : nullptr,
+                        mixed ? c->counter.as<unsigned long long>() : nullptr);
+    if (q + 1 < chunks.size())
+      launch_segseg_reduce(c->rowPart.as<double>(), c->colPart.as<double>(), (int)n, ch, dev, c->st);
+    else
+      c->pending = ch;
+  }
+  HIPCHK(hipGetLastError());
+}
+
+// The last chunk's reduction (see segseg_tiles).
+void segseg_finish(ddd_ctx* c, int64_t n, double* dev) {
+  if (c->pending.J1 > 0)
+    launch_segseg_reduce(c->rowPart.as<double>(), c->colPart.as<double>(), (int)n, c->pending, dev,
+                         c->st);
+  c->pending = TileRange{0, 0};
   HIPCHK(hipGetLastError());
 }
 
@@ -485,9 +531,8 @@ void segseg_all(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsigne
     hybrid_apply(c, n, dev, *opts);
     return;
   }
-  segseg_tiles(c, k, n, flags, opts);
-  launch_segseg_reduce(c->rowPart.as<double>(), c->colPart.as<double>(), (int)n, dev, c->st);
-  HIPCHK(hipGetLastError());
+  segseg_tiles(c, k, n, flags, opts, dev);
+  segseg_finish(c, n, dev);
 }
 
 // Copy a 6 x m device result into the caller's 3 x 2 x outCols array.
@@ -589,6 +634,13 @@ int ddd_ctx_set_stream(ddd_ctx* c, void* stream) {
 }
 
 void* ddd_ctx_stream(ddd_ctx* c) { return c ? (void*)c->st : nullptr; }
+
+int ddd_ctx_set_chunk_bytes(ddd_ctx* c, int64_t bytes) {
+  return guard(c, "ddd_ctx_set_chunk_bytes", [&] {
+    if (bytes < 0) bad("bytes < 0");
+    c->chunkBytes = bytes > 0 ? bytes : ((int64_t)16 << 30);
+  });
+}
 
 const char* ddd_last_error(ddd_ctx* c) { return c ? c->err.c_str() : "no context"; }
 
@@ -1376,11 +1428,7 @@ int ddd_net_upload(ddd_ctx* c, const ddd_consts* k, const ddd_network* net, cons
     c->resK = *k;
     c->resN = net->numSeg;
     c->force.ensure(sizeof(double) * 6 * std::max<int64_t>(c->resN, 1));
-    const int64_t nb = (c->resN + kTile - 1) / kTile;
-    const size_t part = sizeof(double) * 6 * kTile * (size_t)tiles_for(nb);
-    c->rowPart.ensure(part);
-    c->colPart.ensure(part);
-    reset_flags(c);
+    reset_flags(c);  // the pair stage sizes its chunk buffers itself
     HIPCHK(hipStreamSynchronize(c->st));
   });
 }
@@ -1400,13 +1448,14 @@ void net_stage(ddd_ctx* c, int stage, int withPK) {
     if (is_hybrid(&c->opts)) {
       if (n >= 2) hybrid_pairs(c, c->resK, n, dev, c->flags.as<unsigned>(), c->opts);
     } else {
-      segseg_tiles(c, c->resK, n, c->flags.as<unsigned>(), &c->opts);
+      segseg_tiles(c, c->resK, n, c->flags.as<unsigned>(), &c->opts, dev);
     }
   } else if (stage == 2) {
     if (is_hybrid(&c->opts)) {
       if (n >= 2) hybrid_apply(c, n, dev, c->opts);
-    } else if (n >= 2)
-      launch_segseg_reduce(c->rowPart.as<double>(), c->colPart.as<double>(), (int)n, dev, c->st);
+    } else {
+      segseg_finish(c, n, dev);
+    }
   } else if (stage == 3) {
     if (c->dynNodes < 0) bad("no dynamics state (ddd_net_upload_dynamics)");
     launch_mobility_bcc(c->dConn.as<int64_t>(), c->dynConnLd, c->dLinks.as<int64_t>(),
@@ -1439,8 +1488,29 @@ int ddd_net_download(ddd_ctx* c, double* out, int64_t outCols) {
   });
 }
 
+// Device memory the context's own buffers hold (far fields and networks
+// excluded).
+uint64_t ctx_device_bytes(ddd_ctx* c) {
+  const DevBuf* all[] = {&c->seg, &c->rowPart, &c->colPart, &c->force, &c->rows, &c->out, &c->flags,
+                         &c->meshC, &c->meshCoord, &c->meshConn, &c->uHat, &c->sseg, &c->pts,
+                         &c->sigPart, &c->sigma, &c->conn, &c->links, &c->coord, &c->nodeF,
+                         &c->bvec, &c->nodeV, &c->nodeIdx, &c->uTri, &c->uPts, &c->uPart, &c->uOut,
+                         &c->cConn, &c->cLinks, &c->cCoord, &c->cVel, &c->cB, &c->cLabel,
+                         &c->cSkip, &c->cSeg, &c->cNodes, &c->cFlags, &c->cBest, &c->cLast,
+                         &c->cRes, &c->cKeys, &c->cKeysOut, &c->cIdx, &c->cPerm, &c->cSortWork,
+                         &c->cBox, &c->cPrev, &c->counter, &c->specCnt, &c->specOff, &c->specSlot,
+                         &c->specTile, &c->specRes, &c->specWork, &c->specAt, &c->dCoord,
+                         &c->dCoord0, &c->dVel, &c->dVel0, &c->dNodeF, &c->dLinks, &c->dConn,
+                         &c->dBvec, &c->dLabel, &c->dSegNodes, &c->dRedPart, &c->dRed};
+  uint64_t b = 0;
+  for (const DevBuf* d : all) b += d->bytes;
+  return b;
+}
+
 void fill_stats(ddd_ctx* c, int64_t n, const ddd_opts* o, ddd_stats* st) {
   if (!st) return;
+  st->deviceBytes = ctx_device_bytes(c);
+  st->chunks = is_hybrid(o) ? 0 : (uint64_t)c->chunks;
   st->pairs = n > 1 ? (uint64_t)n * (uint64_t)(n - 1) / 2 : 0;
   st->pairsFP64 = st->pairs;
   if (is_hybrid(o) && n > 1) {

@@ -37,40 +37,63 @@ struct MixedRule {
 };
 // mixed == nullptr: plain FP64.  nExact (mixed only) accumulates the number
 // of pairs evaluated on the FP64 path.
+// The all-pairs triangle is evaluated in chunks: column blocks [J0, J1),
+// i.e. the tiles t in [tiles_for(J0), tiles_for(J1)) of the column-block-
+// major enumeration, whose partial sums sit at slot t - tiles_for(J0) of the
+// partial buffers and are reduced into segForce before the next chunk.
+struct TileRange {
+  int J0, J1;
+  int64_t t0() const { return (int64_t)J0 * (J0 + 1) / 2; }
+  int64_t t1() const { return (int64_t)J1 * (J1 + 1) / 2; }
+};
 // Near-parallel pairs of the all-pairs tiles, listed per tile and evaluated
 // before the tile kernel: off[t] .. off[t + 1] is tile t's range (exclusive
-// scan of the count pass), slot = (k << 6) | lane, tile = t, res = [12][total]
-// forces (F1, F2 on the row segment, F3, F4 on the column segment).
+// scan of the count pass over all tiles), its entries at position e - base
+// of this chunk's list (base = off[t0]): slot = (k << 6) | lane, tile = t,
+// res = [12][total] forces (F1, F2 on the row segment, F3, F4 on the column
+// segment).
 struct SpecialList {
   const long long* off;
   unsigned short* slot;
   int* tile;
   double* res;
   int64_t total;
+  int64_t base;
   unsigned* flags;
 };
-// Count pass (cnt[t] per tile), then, with off = exclusive scan of cnt and
-// the list sized, the listing pass and the pairs' evaluation into res.
+// Count pass (cnt[t] for every tile), then, with off = exclusive scan of cnt
+// and the chunk's list sized, the listing pass and the pairs' evaluation
+// into res for the tiles of range r.
 void launch_special_count(const double* seg, int64_t ld, int n, const MixedRule* mixed,
                           unsigned* cnt, hipStream_t st);
 void launch_special_list(const double* seg, int64_t ld, int n, const MixedRule* mixed,
-                         const SpecialList& sp, const SegK<double>& K, hipStream_t st);
+                         const SpecialList& sp, const SegK<double>& K, const TileRange& r,
+                         hipStream_t st);
 // off[0..n] = exclusive scan of cnt[0..n) in 64-bit sums (off[n] = total);
 // work holds scan_work_ints(n) long longs.
 int64_t scan_work_ints(int n);
 void launch_exclusive_scan(const unsigned* cnt, int n, long long* off, long long* work,
                            hipStream_t st);
+// offAt[k] = off[tiles_for(k)] for k = 0..nb (the list offsets at column-block
+// boundaries, which size the chunks).
+void launch_block_offsets(const long long* off, int nb, long long* offAt, hipStream_t st);
 void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
-                         double* rowPart, double* colPart, const SpecialList& sp, hipStream_t st,
-                         const MixedRule* mixed = nullptr, unsigned long long* nExact = nullptr);
+                         double* rowPart, double* colPart, const SpecialList& sp,
+                         const TileRange& r, hipStream_t st, const MixedRule* mixed = nullptr,
+                         unsigned long long* nExact = nullptr);
 // The mixed-precision kernels' launches (mixed.hip; used by the two above).
 void launch_special_pass_mixed(const double* seg, int64_t ld, int n, const MixedRule& mr,
-                               unsigned* cnt, const SpecialList* sp, hipStream_t st);
+                               unsigned* cnt, const SpecialList* sp, int64_t t0, int64_t t1,
+                               hipStream_t st);
 void launch_segseg_tiles_mixed(const double* seg, int64_t ld, int n, const SegK<double>& K,
                                double* rowPart, double* colPart, const SpecialList& sp,
-                               hipStream_t st, const MixedRule& mr, unsigned long long* nExact);
-void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, double* out,
-                          hipStream_t st);
+                               const TileRange& r, hipStream_t st, const MixedRule& mr,
+                               unsigned long long* nExact);
+// Adds each segment's partials of range r into out (6 x n): its row role in
+// tiles (B, J), J in [max(B, J0), J1), then its column role in tiles (I, B),
+// I <= B, if B lies in [J0, J1) -- the fixed order of every chunk.
+void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, const TileRange& r,
+                          double* out, hipStream_t st);
 void launch_segseg_rows(const double* seg, int64_t ld, int n, const int64_t* rows, int nRows,
                         const SegK<double>& K, double* out, unsigned* flags, hipStream_t st);
 

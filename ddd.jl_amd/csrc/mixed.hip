@@ -12,33 +12,31 @@
 namespace ddd {
 
 void launch_special_pass_mixed(const double* seg, int64_t ld, int n, const MixedRule& mr,
-                               unsigned* cnt, const SpecialList* sp, hipStream_t st) {
-  const int64_t nb = (n + kTile - 1) / kTile;
-  const int64_t T = tiles_for(nb);
+                               unsigned* cnt, const SpecialList* sp, int64_t t0, int64_t t1,
+                               hipStream_t st) {
   const int64_t maxGrid = 1ll << 30;
   const double fd2 = mr.farDist2(), sm2 = mr.sinMin2();
-  for (int64_t t0 = 0; t0 < T; t0 += maxGrid) {
-    const int64_t count = (T - t0 < maxGrid) ? (T - t0) : maxGrid;
+  for (int64_t a = t0; a < t1; a += maxGrid) {
+    const int64_t count = (t1 - a < maxGrid) ? (t1 - a) : maxGrid;
     if (sp)
       hipLaunchKernelGGL((k_special_pass<1, 1>), dim3((unsigned)count), dim3(kTile), 0, st, seg, ld, n,
-                         t0, fd2, sm2, cnt, sp->off, sp->slot, sp->tile);
+                         a, fd2, sm2, cnt, sp->off, sp->base, sp->slot, sp->tile);
     else
       hipLaunchKernelGGL((k_special_pass<1, 0>), dim3((unsigned)count), dim3(kTile), 0, st, seg, ld, n,
-                         t0, fd2, sm2, cnt, nullptr, nullptr, nullptr);
+                         a, fd2, sm2, cnt, nullptr, (int64_t)0, nullptr, nullptr);
   }
 }
 
 void launch_segseg_tiles_mixed(const double* seg, int64_t ld, int n, const SegK<double>& K,
                                double* rowPart, double* colPart, const SpecialList& sp,
-                               hipStream_t st, const MixedRule& mr, unsigned long long* nExact) {
-  const int64_t nb = (n + kTile - 1) / kTile;
-  const int64_t T = tiles_for(nb);
+                               const TileRange& r, hipStream_t st, const MixedRule& mr,
+                               unsigned long long* nExact) {
   const int64_t maxGrid = 1ll << 30;
-  for (int64_t t0 = 0; t0 < T; t0 += maxGrid) {
-    const int64_t cnt = (T - t0 < maxGrid) ? (T - t0) : maxGrid;
-    hipLaunchKernelGGL(k_segseg_tiles<1>, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, t0, K,
-                       mr.K, mr.farDist2(), mr.sinMin2(), rowPart, colPart, sp.off,
-                       sp.slot, sp.res, sp.total, nExact);
+  for (int64_t a = r.t0(); a < r.t1(); a += maxGrid) {
+    const int64_t cnt = (r.t1() - a < maxGrid) ? (r.t1() - a) : maxGrid;
+    hipLaunchKernelGGL(k_segseg_tiles<1>, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, a, K,
+                       mr.K, mr.farDist2(), mr.sinMin2(), rowPart, colPart, sp.off, sp.slot, sp.res,
+                       sp.total, r.t0(), sp.base, nExact);
   }
 }
 

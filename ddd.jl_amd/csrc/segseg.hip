@@ -14,6 +14,8 @@
 // :1147-1162) are not evaluated inline, which would stall the whole wavefront
 // on one lane.  They are queued in LDS and evaluated 64 at a time, one per
 // lane, then added to their row and column in queue order.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "pair_special.hpp"
 #include "tiles.hpp"
@@ -146,28 +148,41 @@ __global__ __launch_bounds__(256) void k_scan_final(const unsigned* __restrict__
   if (b == 0 && tid == 0) off[n] = boff[nb];
 }
 
-// Sum of a segment's partials: row role in tiles (B, J), J >= B, then column
-// role in tiles (I, B), I <= B.
+// Sum of a segment's partials of one chunk (column blocks [J0, J1), tile t
+// at slot t - tiles_for(J0)): row role in tiles (B, J), J in [max(B, J0),
+// J1), then column role in tiles (I, B), I <= B, when B is in the chunk.
+// With one chunk (J0 = 0, J1 = nb) this is every row partial, then every
+// column partial of the segment.
 __global__ void k_segseg_reduce(const double* __restrict__ rowPart,
-                                const double* __restrict__ colPart, int n, int nb,
+                                const double* __restrict__ colPart, int n, int J0, int J1,
                                 double* __restrict__ out) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int B = s / kTile, l = s % kTile;
+  const int64_t base = tiles_for(J0);
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int J = B; J < nb; ++J) {
-    const double* p = rowPart + (tiles_for(J) + B) * 6 * kTile + l;
+  for (int J = B > J0 ? B : J0; J < J1; ++J) {
+    const double* p = rowPart + (tiles_for(J) + B - base) * 6 * kTile + l;
 #pragma unroll
     for (int c = 0; c < 6; ++c) acc[c] += p[c * kTile];
   }
-  const int64_t t0 = tiles_for(B);
-  for (int I = 0; I <= B; ++I) {
-    const double* p = colPart + (t0 + I) * 6 * kTile + l;
+  if (B >= J0 && B < J1) {
+    const int64_t t0 = tiles_for(B) - base;
+    for (int I = 0; I <= B; ++I) {
+      const double* p = colPart + (t0 + I) * 6 * kTile + l;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) acc[c] += p[c * kTile];
+      for (int c = 0; c < 6; ++c) acc[c] += p[c * kTile];
+    }
   }
 #pragma unroll
   for (int c = 0; c < 6; ++c) out[6 * (int64_t)s + c] += acc[c];
+}
+
+// The list offsets at column-block boundaries: offAt[k] = off[tiles_for(k)].
+__global__ void k_block_offsets(const long long* __restrict__ off, int nb,
+                                long long* __restrict__ offAt) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k <= nb) offAt[k] = off[tiles_for(k)];
 }
 
 // Rows-only forces (calcSegSegForceIdx!, :831-877): row r = rows[w] against
@@ -220,22 +235,21 @@ void launch_prep_segments(double* seg, int64_t ld, int64_t n, hipStream_t st) {
 
 template <int LIST>
 static void special_pass_plain(const double* seg, int64_t ld, int n, unsigned* cnt,
-                               const SpecialList* sp, hipStream_t st) {
-  const int64_t nb = (n + kTile - 1) / kTile;
-  const int64_t T = tiles_for(nb);
+                               const SpecialList* sp, int64_t t0, int64_t t1, hipStream_t st) {
   const int64_t maxGrid = 1ll << 30;
-  for (int64_t t0 = 0; t0 < T; t0 += maxGrid) {
-    const int64_t count = (T - t0 < maxGrid) ? (T - t0) : maxGrid;
+  for (int64_t a = t0; a < t1; a += maxGrid) {
+    const int64_t count = (t1 - a < maxGrid) ? (t1 - a) : maxGrid;
     hipLaunchKernelGGL((k_special_pass<0, LIST>), dim3((unsigned)count), dim3(kTile), 0, st, seg, ld, n,
-                       t0, 0.0, 0.0, cnt, sp ? sp->off : nullptr, sp ? sp->slot : nullptr,
-                       sp ? sp->tile : nullptr);
+                       a, 0.0, 0.0, cnt, sp ? sp->off : nullptr, sp ? sp->base : 0,
+                       sp ? sp->slot : nullptr, sp ? sp->tile : nullptr);
   }
 }
 
 void launch_special_count(const double* seg, int64_t ld, int n, const MixedRule* mixed,
                           unsigned* cnt, hipStream_t st) {
-  if (mixed) launch_special_pass_mixed(seg, ld, n, *mixed, cnt, nullptr, st);
-  else special_pass_plain<0>(seg, ld, n, cnt, nullptr, st);
+  const int64_t T = tiles_for((n + kTile - 1) / kTile);
+  if (mixed) launch_special_pass_mixed(seg, ld, n, *mixed, cnt, nullptr, 0, T, st);
+  else special_pass_plain<0>(seg, ld, n, cnt, nullptr, 0, T, st);
 }
 
 int64_t scan_work_ints(int n) { return 2 * ((int64_t)(n + kScanBlock - 1) / kScanBlock + 1); }
@@ -250,39 +264,45 @@ void launch_exclusive_scan(const unsigned* cnt, int n, long long* off, long long
   hipLaunchKernelGGL(k_scan_final, dim3(nb > 0 ? nb : 1), dim3(256), 0, st, cnt, n, boff, nb, off);
 }
 
+void launch_block_offsets(const long long* off, int nb, long long* offAt, hipStream_t st) {
+  hipLaunchKernelGGL(k_block_offsets, dim3((nb + 1 + 255) / 256), dim3(256), 0, st, off, nb, offAt);
+}
+
 void launch_special_list(const double* seg, int64_t ld, int n, const MixedRule* mixed,
-                         const SpecialList& sp, const SegK<double>& K, hipStream_t st) {
+                         const SpecialList& sp, const SegK<double>& K, const TileRange& r,
+                         hipStream_t st) {
   if (sp.total <= 0) return;
-  if (mixed) launch_special_pass_mixed(seg, ld, n, *mixed, nullptr, &sp, st);
-  else special_pass_plain<1>(seg, ld, n, nullptr, &sp, st);
+  if (mixed) launch_special_pass_mixed(seg, ld, n, *mixed, nullptr, &sp, r.t0(), r.t1(), st);
+  else special_pass_plain<1>(seg, ld, n, nullptr, &sp, r.t0(), r.t1(), st);
   hipLaunchKernelGGL(k_special_eval, dim3((unsigned)((sp.total + kTile - 1) / kTile)), dim3(kTile), 0,
                      st, seg, ld, sp.slot, sp.tile, sp.total, K, sp.res, sp.flags);
 }
 
 void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
-                         double* rowPart, double* colPart, const SpecialList& sp, hipStream_t st,
-                         const MixedRule* mixed, unsigned long long* nExact) {
+                         double* rowPart, double* colPart, const SpecialList& sp,
+                         const TileRange& r, hipStream_t st, const MixedRule* mixed,
+                         unsigned long long* nExact) {
   if (mixed) {
-    launch_segseg_tiles_mixed(seg, ld, n, K, rowPart, colPart, sp, st, *mixed, nExact);
+    launch_segseg_tiles_mixed(seg, ld, n, K, rowPart, colPart, sp, r, st, *mixed, nExact);
     return;
   }
-  const int64_t nb = (n + kTile - 1) / kTile;
-  const int64_t T = tiles_for(nb);
   const int64_t maxGrid = 1ll << 30;
-  for (int64_t t0 = 0; t0 < T; t0 += maxGrid) {
-    const int64_t cnt = (T - t0 < maxGrid) ? (T - t0) : maxGrid;
-    hipLaunchKernelGGL(k_segseg_tiles<0>, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, t0, K,
+  for (int64_t a = r.t0(); a < r.t1(); a += maxGrid) {
+    const int64_t cnt = (r.t1() - a < maxGrid) ? (r.t1() - a) : maxGrid;
+    hipLaunchKernelGGL(k_segseg_tiles<0>, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, a, K,
                        SigK<float>{}, 0.0, 0.0, rowPart, colPart, sp.off, sp.slot, sp.res, sp.total,
-                       nExact);
+                       r.t0(), sp.base, nExact);
   }
 }
 
-void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, double* out,
-                          hipStream_t st) {
-  const int nb = (n + kTile - 1) / kTile;
+void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, const TileRange& r,
+                          double* out, hipStream_t st) {
+  // segments of row blocks below J1 receive partials from this chunk
+  const int64_t m = std::min<int64_t>(n, (int64_t)r.J1 * kTile);
+  if (m <= 0) return;
   const int bs = 256;
-  hipLaunchKernelGGL(k_segseg_reduce, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, st,
-                     rowPart, colPart, n, nb, out);
+  hipLaunchKernelGGL(k_segseg_reduce, dim3((unsigned)((m + bs - 1) / bs)), dim3(bs), 0, st,
+                     rowPart, colPart, (int)m, r.J0, r.J1, out);
 }
 
 void launch_segseg_rows(const double* seg, int64_t ld, int n, const int64_t* rows, int nRows,

@@ -64,12 +64,14 @@ __device__ __forceinline__ bool is_special(bool live, double omcSq, v3<double> m
 
 // Near-parallel pairs per tile, one wavefront per tile in the tile kernel's
 // slot order (k, lane): LIST = 0 counts them (cnt[t]); LIST = 1 writes them
-// to the tile's range off[t] .. off[t + 1] of the pair list.
+// to the tile's range off[t] .. off[t + 1] of the pair list, at position
+// e - base of the chunk's list.
 template <int MODE, int LIST>
 __global__ __launch_bounds__(kTile) void k_special_pass(const double* __restrict__ seg, int64_t ld,
                                                         int n, int64_t tile0, double farDist2,
                                                         double sinMin2, unsigned* __restrict__ cnt,
                                                         const long long* __restrict__ off,
+                                                        int64_t base,
                                                         unsigned short* __restrict__ slot,
                                                         int* __restrict__ tileOf) {
   const int64_t t = tile0 + blockIdx.x;
@@ -111,8 +113,8 @@ __global__ __launch_bounds__(kTile) void k_special_pass(const double* __restrict
     if (LIST && sp) {
       const long long pos = c + __popcll(m & ltMask);
       if (pos < cEnd) {  // always true: both passes classify identically
-        slot[pos] = (unsigned short)((k << 6) | lane);
-        tileOf[pos] = (int)t;
+        slot[pos - base] = (unsigned short)((k << 6) | lane);
+        tileOf[pos - base] = (int)t;
       }
     }
     c += __popcll(m);
@@ -130,7 +132,9 @@ __global__ __launch_bounds__(kTile) void k_special_pass(const double* __restrict
 // per tile and evaluated beforehand (k_special_pass, k_special_eval), and
 // the tile adds its list's forces, in list order, after its own pairs.
 // Keeping that branch out of this kernel lets it run two wavefronts per
-// SIMD.  The number of FP64 pairs is added to *nExact (mixed mode).
+// SIMD.  The number of FP64 pairs is added to *nExact (mixed mode).  Tile
+// t's partial sums go to slot t - partBase, its list entries are read at
+// e - specBase (the chunk's first tile and first entry).
 template <int MODE>
 __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restrict__ seg,
                                                            int64_t ld, int n, int64_t tile0,
@@ -141,7 +145,8 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
                                                            const long long* __restrict__ specOff,
                                                            const unsigned short* __restrict__ specSlot,
                                                            const double* __restrict__ specRes,
-                                                           int64_t specTotal,
+                                                           int64_t specTotal, int64_t partBase,
+                                                           int64_t specBase,
                                                            unsigned long long* __restrict__ nExact) {
   const int64_t t = tile0 + blockIdx.x;
   int I, J;
@@ -264,7 +269,7 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
   if (MODE == 1 && nq > 0) flush(nq);
   if (MODE == 1 && lane == 0 && exact) atomicAdd(nExact, (unsigned long long)exact);
   // this tile's near-parallel pairs, evaluated beforehand, in list order
-  for (long long e = specOff[t], e1 = specOff[t + 1]; e < e1; ++e) {
+  for (long long e = specOff[t] - specBase, e1 = specOff[t + 1] - specBase; e < e1; ++e) {
     const unsigned q = specSlot[e];
     const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
     if (rl == lane) {
@@ -277,8 +282,8 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
     }
   }
 
-  double* rp = rowPart + t * 6 * kTile;
-  double* cp = colPart + t * 6 * kTile;
+  double* rp = rowPart + (t - partBase) * 6 * kTile;
+  double* cp = colPart + (t - partBase) * 6 * kTile;
   rp[0 * kTile + lane] = a1.x; rp[1 * kTile + lane] = a1.y; rp[2 * kTile + lane] = a1.z;
   rp[3 * kTile + lane] = a2.x; rp[4 * kTile + lane] = a2.y; rp[5 * kTile + lane] = a2.z;
 #pragma unroll

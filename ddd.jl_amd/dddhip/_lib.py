@@ -72,7 +72,8 @@ class ddd_opts(C.Structure):
 
 
 class ddd_stats(C.Structure):
-    _fields_ = [("pairs", C.c_uint64), ("pairsFP64", C.c_uint64)]
+    _fields_ = [("pairs", C.c_uint64), ("pairsFP64", C.c_uint64), ("deviceBytes", C.c_uint64),
+                ("chunks", C.c_uint64)]
 
 
 class ddd_train_opts(C.Structure):
@@ -113,6 +114,7 @@ def lib():
                 "ddd_ctx_destroy": (C.c_int, [P]),
                 "ddd_ctx_set_stream": (C.c_int, [P, P]),
                 "ddd_ctx_stream": (P, [P]),
+                "ddd_ctx_set_chunk_bytes": (C.c_int, [P, i64]),
                 "ddd_last_error": (C.c_char_p, [P]),
                 "ddd_segseg_force": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_network),
                                                ip, i64, dp, i64, C.c_int]),

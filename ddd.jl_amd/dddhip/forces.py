@@ -55,6 +55,12 @@ def calcSegSegForce(dlnParams, matParams, network, idx=None, ctx=None):
     return out
 
 
+def setChunkBytes(nbytes, ctx=None):
+    """Memory budget of one chunk of the all-pairs evaluation (0: 16 GiB)."""
+    ctx = _ctx(ctx)
+    _call(ctx, "ddd_ctx_set_chunk_bytes", int(nbytes))
+
+
 def calcSegSegForceMixed(dlnParams, matParams, network, farDist=0.0, sinMin=0.0, ctx=None):
     """All-pairs seg-seg force with the mixed-precision rule (no reference
     counterpart): pairs farther than farDist (default 5 coreRad) with
@@ -510,6 +516,13 @@ class ResidentNetwork:
         st = L.ddd_stats()
         _call(self.ctx, "ddd_net_stats", st)
         return int(st.pairs), int(st.pairsFP64)
+
+    def footprint(self):
+        """(device bytes the context holds, chunks of the last all-pairs
+        evaluation)."""
+        st = L.ddd_stats()
+        _call(self.ctx, "ddd_net_stats", st)
+        return int(st.deviceBytes), int(st.chunks)
 
     def stage(self, stage, with_pk=False):
         _call(self.ctx, "ddd_net_stage", int(stage), 1 if with_pk else 0)

@@ -117,8 +117,10 @@ typedef struct ddd_opts {
 } ddd_opts;
 
 typedef struct ddd_stats {
-  uint64_t pairs;      /* pairs evaluated exactly (hybrid: near pairs, each once) */
-  uint64_t pairsFP64;  /* of which on the FP64 path                            */
+  uint64_t pairs;       /* pairs evaluated exactly (hybrid: near pairs, each once) */
+  uint64_t pairsFP64;   /* of which on the FP64 path                            */
+  uint64_t deviceBytes; /* device memory the context's own buffers hold         */
+  uint64_t chunks;      /* chunks of the all-pairs triangle (0 for the hybrid)  */
 } ddd_stats;
 
 /* Hybrid far field (no reference counterpart).  A G^3 grid of cubic cells
@@ -206,6 +208,12 @@ int ddd_ctx_destroy(ddd_ctx* ctx);
  * NULL returns to the context's own stream. */
 int ddd_ctx_set_stream(ddd_ctx* ctx, void* stream);
 void* ddd_ctx_stream(ddd_ctx* ctx);
+/* Memory budget of one chunk of the all-pairs evaluation (FP64 and mixed
+ * modes; no counterpart in the reference): the triangle of pair tiles is
+ * evaluated in chunks of whole 64-segment column blocks whose partial sums
+ * and near-parallel lists fit in `bytes` (0: the default, 16 GiB), each
+ * reduced into segForce in a fixed order before the next. */
+int ddd_ctx_set_chunk_bytes(ddd_ctx* ctx, int64_t bytes);
 const char* ddd_last_error(ddd_ctx* ctx);
 
 /* calcSegSegForce / calcSegSegForce! (DislocationForces.jl:470-669; pair

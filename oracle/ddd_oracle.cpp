@@ -250,6 +250,41 @@ int64_t orc_segseg_par_rows(const double* K, int64_t numSeg, const int64_t* segI
   return pairs;
 }
 
+// The forces of the all-pairs triangle on sampled segments: for row
+// i = rows[k] - 1, the sum over j != i in index order of the pair force on
+// segment i in the triangle's orientation (calcSegSegForceSer!,
+// DislocationForces.jl:794-825: F1, F2 of pair (i, j) for j > i, F3, F4 of
+// pair (j, i) for j < i).  out: 6 x nRows.
+void orc_segseg_rows_triangle(const double* K, int64_t numSeg, const int64_t* segIdx, int64_t ld,
+                              const double* coord, const double* bVec, const uint8_t* extSeg,
+                              const int64_t* rows, int64_t nRows, double* out, int nthreads) {
+  SegView s{numSeg, segIdx, ld, coord, bVec, extSeg};
+  SegConsts<double> KK = consts_from<double>(K);
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+  for (int64_t r = 0; r < nRows; ++r) {
+    const int64_t i = rows[r] - 1;
+    double fi[6] = {0, 0, 0, 0, 0, 0};
+    if (!s.ext(i)) {
+      for (int64_t j = 0; j < numSeg; ++j) {
+        if (j == i || s.ext(j)) continue;
+        V3<double> a, b;
+        if (j > i) {
+          Forces4<double> f = seg_seg_force(KK, s.b(i), s.n1(i), s.n2(i), s.b(j), s.n1(j), s.n2(j));
+          a = f.f1;
+          b = f.f2;
+        } else {
+          Forces4<double> f = seg_seg_force(KK, s.b(j), s.n1(j), s.n2(j), s.b(i), s.n1(i), s.n2(i));
+          a = f.f3;
+          b = f.f4;
+        }
+        fi[0] += a.x; fi[1] += a.y; fi[2] += a.z;
+        fi[3] += b.x; fi[4] += b.y; fi[5] += b.z;
+      }
+    }
+    for (int q = 0; q < 6; ++q) out[6 * r + q] = fi[q];
+  }
+}
+
 // Rows-only forces split by distance: for each row i = rows[k]-1 against
 // every j != i (row segment first), the F1/F2 contributions are summed into
 // bin b where edges[b] <= |mid_i - mid_j| < edges[b+1] (edges ascending,

@@ -65,6 +65,7 @@ def lib():
             "orc_segseg_rows_grouped": (None, [dp] + net + [ip, i64, ip, i64, dp, i32]),
             "orc_segseg_rows_binned": (None, [dp] + net + [ip, i64, dp, i32, dp, i32]),
             "orc_segseg_par_rows": (i64, [dp] + net + [ip, i64, dp, i32]),
+            "orc_segseg_rows_triangle": (None, [dp] + net + [ip, i64, dp, i32]),
             "orc_segseg_tuned_rows": (i64, [dp] + net + [ip, i64, dp, i32]),
             "orc_libm_log": (d, [d]),
             "orc_libm_atan": (d, [d]),
@@ -276,6 +277,18 @@ def segseg_par_rows(dln, mat, net, rows, nthreads):
                                       _d(n.bVec), _u(n.ext), _i(r), len(r), _d(out),
                                       int(nthreads))
     return out, int(pairs)
+
+
+def segseg_rows_triangle(dln, mat, net, rows, nthreads=16):
+    """Seg-seg forces of the all-pairs triangle (calcSegSegForceSer!'s
+    orientation) on the sampled 1-based rows only: 3 x 2 x len(rows)."""
+    n = _Net(net)
+    K = seg_consts(dln, mat)
+    r = np.ascontiguousarray(rows, dtype=np.int64)
+    out = np.zeros((3, 2, len(r)), order="F")
+    lib().orc_segseg_rows_triangle(_d(K), n.numSeg, _i(n.segIdx), n.ld, _d(n.coord), _d(n.bVec),
+                                   _u(n.ext), _i(r), len(r), _d(out), int(nthreads))
+    return out
 
 
 def calcSelfForce(dln, mat, net, idx=None):
