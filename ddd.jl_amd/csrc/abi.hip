@@ -81,14 +81,15 @@ struct ddd_farfield {
       colStart, list, tileIdx, colList, rowPart, colPart;
   // near-field kernel staging, and its list of near-parallel pairs (spCap
   // entries), their per-tile offsets and counts and their forces
-  DevBuf stage, spList, specOff, specCnt, spRes;
+  DevBuf stage, spList, specOff, specCnt, spRes, spFix, spFixN;
   long long spCap = 1 << 16;
   int64_t nearTiles = 0;
   void release() {
     DevBuf* all[] = {&What, &tw, &cellOf, &counts, &cursor, &start, &tmp, &order, &M, &Mc, &Lc, &L,
                      &counters, &mrank, &rankCell, &mcount, &mstart, &segS, &key, &perm, &box,
                      &rowCount, &rowStart, &colCount, &colStart, &list, &tileIdx, &colList,
-                     &rowPart, &colPart, &stage, &spList, &specOff, &specCnt, &spRes};
+                     &rowPart, &colPart, &stage, &spList, &specOff, &specCnt, &spRes, &spFix,
+                     &spFixN};
     for (DevBuf* b : all) b->release();
   }
 };
@@ -148,7 +149,7 @@ struct ddd_ctx {
   std::vector<double> h;  // host staging
   DevBuf counter;  // FP64-pair counter of the mixed path
   // near-parallel pairs of the all-pairs tiles (count, offsets, list, forces)
-  DevBuf specCnt, specOff, specSlot, specTile, specRes, specWork, specAt;
+  DevBuf specCnt, specOff, specSlot, specTile, specRes, specWork, specAt, specFix, specFixN;
   TileRange pending{0, 0};  // the all-pairs chunk whose reduction is still to run
   int chunks = 0;           // chunks of the last all-pairs evaluation
   int64_t chunkBytes = (int64_t)16 << 30;
@@ -417,9 +418,12 @@ void hybrid_pairs(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsig
     HIPCHK(hipMemsetAsync(ff->counters.p, 0, 4 * sizeof(unsigned long long), c->st));
   }
   ff->spRes.ensure(sizeof(double) * 12 * std::max<long long>(ff->spCap, 1));
+  ff->spFix.ensure(sizeof(long long) * std::max<long long>(ff->spCap, 1));
+  ff->spFixN.ensure(sizeof(unsigned long long));
   launch_near_special(ff->segS.as<double>(), ldS, ff->list.as<int2>(), tiles, K,
                       ff->spList.as<unsigned long long>(), total, ff->spCap,
                       ff->specOff.as<long long>(), ff->specCnt.as<int>(), ff->spRes.as<double>(),
+                      ff->spFix.as<long long>(), ff->spFixN.as<unsigned long long>(),
                       ff->rowPart.as<double>(), ff->colPart.as<double>(), flags, c->st);
   launch_near_reduce(nl, nc, ff->rowPart.as<double>(), ff->colPart.as<double>(), dev, c->st);
   HIPCHK(hipGetLastError());
@@ -439,7 +443,8 @@ void hybrid_apply(ddd_ctx* c, int64_t n, double* dev, const ddd_opts& o) {
 // column above the budget (at most 65,535 tiles under the tile-count limit)
 // makes a chunk of its own.
 constexpr int64_t kTileBytes = 2 * 6 * kTile * (int64_t)sizeof(double);
-constexpr int64_t kSpecialBytes = sizeof(unsigned short) + sizeof(int) + 12 * sizeof(double);
+constexpr int64_t kSpecialBytes =
+    sizeof(unsigned short) + sizeof(int) + 12 * sizeof(double) + sizeof(int64_t);
 
 // Pair tiles of the whole network (mode from opts), chunk by chunk: a count
 // pass and scan over every tile for the near-parallel pairs (one host
@@ -494,12 +499,15 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
   c->specSlot.ensure(sizeof(unsigned short) * maxSpec);
   c->specTile.ensure(sizeof(int) * maxSpec);
   c->specRes.ensure(sizeof(double) * 12 * maxSpec);
+  c->specFix.ensure(sizeof(int64_t) * maxSpec);
+  c->specFixN.ensure(sizeof(unsigned long long));
   if (mixed) HIPCHK(hipMemsetAsync(c->counter.p, 0, sizeof(unsigned long long), c->st));
   for (size_t q = 0; q < chunks.size(); ++q) {
     const TileRange& ch = chunks[q];
     SpecialList sp{c->specOff.as<long long>(), c->specSlot.as<unsigned short>(), c->specTile.as<int>(),
                    c->specRes.as<double>(), (int64_t)(offAt[ch.J1] - offAt[ch.J0]),
-                   (int64_t)offAt[ch.J0], flags};
+                   (int64_t)offAt[ch.J0], flags, c->specFix.as<int64_t>(),
+                   c->specFixN.as<unsigned long long>()};
     launch_special_list(c->seg.as<double>(), c->resLd, (int)n, mixed ? &r : nullptr, sp,
                         seg_consts(k), ch, c->st);
     launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k), c->rowPart.as<double>(),
@@ -1499,7 +1507,7 @@ uint64_t ctx_device_bytes(ddd_ctx* c) {
                          &c->cSkip, &c->cSeg, &c->cNodes, &c->cFlags, &c->cBest, &c->cLast,
                          &c->cRes, &c->cKeys, &c->cKeysOut, &c->cIdx, &c->cPerm, &c->cSortWork,
                          &c->cBox, &c->cPrev, &c->counter, &c->specCnt, &c->specOff, &c->specSlot,
-                         &c->specTile, &c->specRes, &c->specWork, &c->specAt, &c->dCoord,
+                         &c->specTile, &c->specRes, &c->specWork, &c->specAt, &c->specFix, &c->specFixN, &c->dCoord,
                          &c->dCoord0, &c->dVel, &c->dVel0, &c->dNodeF, &c->dLinks, &c->dConn,
                          &c->dBvec, &c->dLabel, &c->dSegNodes, &c->dRedPart, &c->dRed};
   uint64_t b = 0;

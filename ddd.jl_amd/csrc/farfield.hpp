@@ -209,10 +209,12 @@ void launch_near_tiles(const double* segS, int64_t ldS, const int* key, int rn, 
                        long long spCap, long long* specOff, int* specCnt,
                        unsigned long long* counters, hipStream_t st);
 // The listed near-parallel pairs (total <= spCap): evaluated into res
-// (12 x spCap doubles) and added to their tiles' partial sums.
+// (12 x spCap doubles; fix: spCap entries and a counter, the pairs that need
+// the branch's corrective sub-pairs) and added to their tiles' partial sums.
 void launch_near_special(const double* segS, int64_t ldS, const int2* list, int nTiles,
                          const SegK<double>& K, const unsigned long long* spList, long long total,
                          long long spCap, const long long* specOff, const int* specCnt, double* res,
-                         double* rowPart, double* colPart, unsigned* flags, hipStream_t st);
+                         long long* fix, unsigned long long* fixCount, double* rowPart,
+                         double* colPart, unsigned* flags, hipStream_t st);
 
 }  // namespace ddd

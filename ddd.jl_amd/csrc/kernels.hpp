@@ -52,6 +52,8 @@ struct TileRange {
 // of this chunk's list (base = off[t0]): slot = (k << 6) | lane, tile = t,
 // res = [12][total] forces (F1, F2 on the row segment, F3, F4 on the column
 // segment).
+// fix (total entries) and fixCount: the pairs that need the branch's
+// corrective sub-pairs, listed by the evaluation's first pass.
 struct SpecialList {
   const long long* off;
   unsigned short* slot;
@@ -60,6 +62,8 @@ struct SpecialList {
   int64_t total;
   int64_t base;
   unsigned* flags;
+  int64_t* fix;
+  unsigned long long* fixCount;
 };
 // Count pass (cnt[t] for every tile), then, with off = exclusive scan of cnt
 // and the chunk's list sized, the listing pass and the pairs' evaluation

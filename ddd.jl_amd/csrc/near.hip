@@ -8,6 +8,8 @@
 // hardware reciprocals move a near pair's forces within the pair formula's
 // own conditioning (up to ~3e-7 per node against the IEEE oracle) and remove
 // many of the FP64 pair's instructions.
+#include <algorithm>
+
 #include "farfield.hpp"
 #include "pair_special.hpp"
 
@@ -274,28 +276,75 @@ __global__ __launch_bounds__(kTile, 2) void k_near_tiles(const double* __restric
 }
 
 // Near-parallel pairs of the near-field tiles (the reference's special
-// branch, calcParSegSegForce), one list entry per lane: forces F1..F4 into
-// res[12][spCap].
+// branch, calcParSegSegForce), one list entry per lane: pass A of the branch,
+// forces F1..F4 into res[12][spCap]; the pairs that need corrective
+// sub-pairs are listed in fix (claimed by wave ballots: order immaterial,
+// each pair has its own result slot).
 __global__ __launch_bounds__(kTile) void k_near_special_eval(const double* __restrict__ seg, int64_t ld,
                                                              const int2* __restrict__ list,
                                                              const unsigned long long* __restrict__ spList,
                                                              long long total, long long spCap,
                                                              SegK<double> K, double* __restrict__ res,
-                                                             unsigned* __restrict__ flags) {
+                                                             long long* __restrict__ fix,
+                                                             unsigned long long* __restrict__ fixCount) {
   const long long e = (long long)blockIdx.x * kTile + threadIdx.x;
-  if (e >= total) return;
-  const unsigned long long en = spList[e];
-  const int t = (int)(en >> 16);
-  const unsigned code = (unsigned)(en & 0xffffu);
-  const int rl = code & 63, sl = (rl + (code >> 6)) & 63;
-  const int2 IJ = list[t];
-  const Seg<double> P = seg_from(seg, ld, (int64_t)IJ.x * kTile + rl);
-  const Seg<double> Q = seg_from(seg, ld, (int64_t)IJ.y * kTile + sl);
-  v3<double> F1, F2, F3, F4;
-  special_pair_dev(K, P, Q, F1, F2, F3, F4, flags);
-  const double v[12] = {F1.x, F1.y, F1.z, F2.x, F2.y, F2.z, F3.x, F3.y, F3.z, F4.x, F4.y, F4.z};
+  bool need = false;
+  if (e < total) {
+    const unsigned long long en = spList[e];
+    const int t = (int)(en >> 16);
+    const unsigned code = (unsigned)(en & 0xffffu);
+    const int rl = code & 63, sl = (rl + (code >> 6)) & 63;
+    const int2 IJ = list[t];
+    const Seg<double> P = seg_from(seg, ld, (int64_t)IJ.x * kTile + rl);
+    const Seg<double> Q = seg_from(seg, ld, (int64_t)IJ.y * kTile + sl);
+    v3<double> F1, F2, F3, F4;
+    need = special_pair_base(K, P, Q, F1, F2, F3, F4);
+    const double v[12] = {F1.x, F1.y, F1.z, F2.x, F2.y, F2.z, F3.x, F3.y, F3.z, F4.x, F4.y, F4.z};
 #pragma unroll
-  for (int c = 0; c < 12; ++c) res[c * spCap + e] = v[c];
+    for (int c = 0; c < 12; ++c) res[c * spCap + e] = v[c];
+  }
+  const uint64_t m = __ballot(need);
+  if (m == 0) return;
+  const int lane = threadIdx.x;
+  unsigned long long base = 0;
+  if (lane == __builtin_ctzll(m)) base = atomicAdd(fixCount, (unsigned long long)__popcll(m));
+  base = __shfl(base, __builtin_ctzll(m), kTile);
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  if (need) fix[base + __popcll(m & lt)] = e;
+}
+
+__global__ void k_near_zero_count(unsigned long long* c) { *c = 0; }
+
+// Pass B for the listed pairs: their corrective sub-pairs, added to their
+// forces in res (grid-stride over the list, whose length is on the device).
+__global__ __launch_bounds__(kTile) void k_near_special_fix(const double* __restrict__ seg, int64_t ld,
+                                                            const int2* __restrict__ list,
+                                                            const unsigned long long* __restrict__ spList,
+                                                            long long spCap, SegK<double> K,
+                                                            double* __restrict__ res,
+                                                            const long long* __restrict__ fix,
+                                                            const unsigned long long* __restrict__ fixCount,
+                                                            unsigned* __restrict__ flags) {
+  const long long cnt = (long long)*fixCount;
+  for (long long q0 = (long long)blockIdx.x * kTile + threadIdx.x; q0 < cnt;
+       q0 += (long long)gridDim.x * kTile) {
+    const long long e = fix[q0];
+    const unsigned long long en = spList[e];
+    const int t = (int)(en >> 16);
+    const unsigned code = (unsigned)(en & 0xffffu);
+    const int rl = code & 63, sl = (rl + (code >> 6)) & 63;
+    const int2 IJ = list[t];
+    const Seg<double> P = seg_from(seg, ld, (int64_t)IJ.x * kTile + rl);
+    const Seg<double> Q = seg_from(seg, ld, (int64_t)IJ.y * kTile + sl);
+    v3<double> F1 = V(res[0 * spCap + e], res[1 * spCap + e], res[2 * spCap + e]);
+    v3<double> F2 = V(res[3 * spCap + e], res[4 * spCap + e], res[5 * spCap + e]);
+    v3<double> F3 = V(res[6 * spCap + e], res[7 * spCap + e], res[8 * spCap + e]);
+    v3<double> F4 = V(res[9 * spCap + e], res[10 * spCap + e], res[11 * spCap + e]);
+    special_pair_fix(K, P, Q, F1, F2, F3, F4, flags);
+    const double v[12] = {F1.x, F1.y, F1.z, F2.x, F2.y, F2.z, F3.x, F3.y, F3.z, F4.x, F4.y, F4.z};
+#pragma unroll
+    for (int c = 0; c < 12; ++c) res[c * spCap + e] = v[c];
+  }
 }
 
 // Adds the near-parallel forces of tile t (blockIdx.x) to its partial sums,
@@ -371,10 +420,15 @@ void launch_near_tiles(const double* segS, int64_t ldS, const int* key, int rn, 
 void launch_near_special(const double* segS, int64_t ldS, const int2* list, int nTiles,
                          const SegK<double>& K, const unsigned long long* spList, long long total,
                          long long spCap, const long long* specOff, const int* specCnt, double* res,
-                         double* rowPart, double* colPart, unsigned* flags, hipStream_t st) {
+                         long long* fix, unsigned long long* fixCount, double* rowPart,
+                         double* colPart, unsigned* flags, hipStream_t st) {
   if (nTiles <= 0 || total <= 0) return;
+  hipLaunchKernelGGL(k_near_zero_count, dim3(1), dim3(1), 0, st, fixCount);
   hipLaunchKernelGGL(k_near_special_eval, dim3((unsigned)((total + kTile - 1) / kTile)), dim3(kTile), 0, st,
-                     segS, ldS, list, spList, total, spCap, K, res, flags);
+                     segS, ldS, list, spList, total, spCap, K, res, fix, fixCount);
+  const long long blocks = std::min<long long>((total + kTile - 1) / kTile, 2048);
+  hipLaunchKernelGGL(k_near_special_fix, dim3((unsigned)blocks), dim3(kTile), 0, st, segS, ldS, list,
+                     spList, spCap, K, res, fix, fixCount, flags);
   hipLaunchKernelGGL(k_near_special_add, dim3((unsigned)nTiles), dim3(kTile), 0, st, spList, res, spCap,
                      specOff, specCnt, rowPart, colPart);
 }

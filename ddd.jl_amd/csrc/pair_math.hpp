@@ -400,23 +400,37 @@ __device__ __forceinline__ void par_corner_acc(T aSq_dSq, T aSq_dSqI, T x, T y, 
 // tilted by the critical angle.  Returns the forces on A's first and second
 // node in (Fa1, Fa2), and the two tilted pieces of B for the corrective
 // generic calls in (c1a, c1b), (c2a, c2b) with doCorr set when they apply.
+// The first steps of the near-parallel branch (:1203-1213): the other
+// segment p1 -> p2 projected onto A's line (unit tangent tA) and tilted by
+// the critical angle to p1m -> p2m; magDiffSq0 = |rejection|^2, magDiff its
+// root.  par_side and the corrective pass share it, so both see the same
+// bits.
+template <class T>
+__device__ __forceinline__ void par_tilt(v3<T> tA, v3<T> p1, v3<T> p2, v3<T>& p1m, v3<T>& p2m,
+                                         T& magDiffSq0, T& magDiff) {
+  const T se = T(DDD_SQRT_EPS);
+  const T hCot = sqrt((T(1) - se * T(1.01)) / (se * T(1.01))) / T(2);
+  const T tmp = dot(p2 - p1, tA);
+  const v3<T> q = p1 + tmp * tA;
+  const v3<T> diff = p2 - q;
+  magDiffSq0 = dot(diff, diff);
+  magDiff = sqrt(magDiffSq0);
+  const v3<T> tv1 = T(0.5) * diff;
+  const v3<T> tv2 = (hCot * magDiff) * tA;
+  p1m = p1 + tv1 + tv2;
+  p2m = q + tv1 - tv2;
+}
+
 template <class T>
 __device__ __forceinline__ void par_side(const SegK<T>& K, v3<T> a1, v3<T> a2, v3<T> tA, T tNA,
                                          v3<T> bA, v3<T> p1, v3<T> p2, v3<T> bB, bool sideOne,
                                          v3<T>& Fa1, v3<T>& Fa2, v3<T>& c1a, v3<T>& c1b,
                                          v3<T>& c2a, v3<T>& c2b, bool& doCorr) {
   const T se = T(DDD_SQRT_EPS);
-  const T hCot = sqrt((T(1) - se * T(1.01)) / (se * T(1.01))) / T(2);
   // Project the other segment (p1 -> p2) on A's line and tilt it.
-  const T tmp = dot(p2 - p1, tA);
-  v3<T> p2m = p1 + tmp * tA;
-  const v3<T> diff = p2 - p2m;
-  const T magDiffSq0 = dot(diff, diff);
-  const T magDiff = sqrt(magDiffSq0);
-  const v3<T> tv1 = T(0.5) * diff;
-  const v3<T> tv2 = (hCot * magDiff) * tA;
-  const v3<T> p1m = p1 + tv1 + tv2;
-  p2m = p2m + tv1 - tv2;
+  v3<T> p1m, p2m;
+  T magDiffSq0, magDiff;
+  par_tilt(tA, p1, p2, p1m, p2m, magDiffSq0, magDiff);
 
   // Side 1 measures from A's first node to the tilted other segment; side 2
   // from the tilted segment (here the roles of "this" and "other" swap in

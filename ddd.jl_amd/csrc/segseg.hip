@@ -47,26 +47,73 @@ __global__ void k_prep_segments(double* __restrict__ seg, int64_t ld, int64_t n)
 }
 
 // Near-parallel pairs of the special-pair list, one per lane, forces
-// [12][total] (F1, F2 on the row segment, F3, F4 on the column segment).
+// [12][total] (F1, F2 on the row segment, F3, F4 on the column segment):
+// pass A of the branch; the pairs that need corrective sub-pairs are listed
+// in fix (order of the atomic claims, which does not matter: each pair's
+// result has its own slot).
 __global__ __launch_bounds__(kTile) void k_special_eval(const double* __restrict__ seg, int64_t ld,
                                                         const unsigned short* __restrict__ specSlot,
                                                         const int* __restrict__ specTile,
                                                         int64_t total, SegK<double> K,
                                                         double* __restrict__ res,
-                                                        unsigned* __restrict__ flags) {
+                                                        int64_t* __restrict__ fix,
+                                                        unsigned long long* __restrict__ fixCount) {
   const int64_t e = blockIdx.x * (int64_t)kTile + threadIdx.x;
-  if (e >= total) return;
-  int I, J;
-  tile_blocks(specTile[e], I, J);
-  const unsigned q = specSlot[e];
-  const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
-  const Seg<double> P = seg_from(seg, ld, (int64_t)I * kTile + rl);
-  const Seg<double> Q = seg_from(seg, ld, (int64_t)J * kTile + sl);
-  v3<double> F1, F2, F3, F4;
-  special_pair_dev(K, P, Q, F1, F2, F3, F4, flags);
-  const double v[12] = {F1.x, F1.y, F1.z, F2.x, F2.y, F2.z, F3.x, F3.y, F3.z, F4.x, F4.y, F4.z};
+  bool need = false;
+  if (e < total) {
+    int I, J;
+    tile_blocks(specTile[e], I, J);
+    const unsigned q = specSlot[e];
+    const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
+    const Seg<double> P = seg_from(seg, ld, (int64_t)I * kTile + rl);
+    const Seg<double> Q = seg_from(seg, ld, (int64_t)J * kTile + sl);
+    v3<double> F1, F2, F3, F4;
+    need = special_pair_base(K, P, Q, F1, F2, F3, F4);
+    const double v[12] = {F1.x, F1.y, F1.z, F2.x, F2.y, F2.z, F3.x, F3.y, F3.z, F4.x, F4.y, F4.z};
 #pragma unroll
-  for (int c = 0; c < 12; ++c) res[c * total + e] = v[c];
+    for (int c = 0; c < 12; ++c) res[c * total + e] = v[c];
+  }
+  const uint64_t m = __ballot(need);
+  if (m == 0) return;
+  const int lane = threadIdx.x;
+  unsigned long long base = 0;
+  if (lane == __builtin_ctzll(m)) base = atomicAdd(fixCount, (unsigned long long)__popcll(m));
+  base = __shfl(base, __builtin_ctzll(m), kTile);
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  if (need) fix[base + __popcll(m & lt)] = e;
+}
+
+__global__ void k_zero_count(unsigned long long* c) { *c = 0; }
+
+// Pass B for the listed pairs: their corrective sub-pairs, added to their
+// forces in res.  Grid-stride over the list (its length is on the device).
+__global__ __launch_bounds__(kTile) void k_special_fix(const double* __restrict__ seg, int64_t ld,
+                                                       const unsigned short* __restrict__ specSlot,
+                                                       const int* __restrict__ specTile,
+                                                       int64_t total, SegK<double> K,
+                                                       double* __restrict__ res,
+                                                       const int64_t* __restrict__ fix,
+                                                       const unsigned long long* __restrict__ fixCount,
+                                                       unsigned* __restrict__ flags) {
+  const int64_t cnt = (int64_t)*fixCount;
+  for (int64_t q0 = blockIdx.x * (int64_t)kTile + threadIdx.x; q0 < cnt;
+       q0 += (int64_t)gridDim.x * kTile) {
+    const int64_t e = fix[q0];
+    int I, J;
+    tile_blocks(specTile[e], I, J);
+    const unsigned q = specSlot[e];
+    const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
+    const Seg<double> P = seg_from(seg, ld, (int64_t)I * kTile + rl);
+    const Seg<double> Q = seg_from(seg, ld, (int64_t)J * kTile + sl);
+    v3<double> F1 = V(res[0 * total + e], res[1 * total + e], res[2 * total + e]);
+    v3<double> F2 = V(res[3 * total + e], res[4 * total + e], res[5 * total + e]);
+    v3<double> F3 = V(res[6 * total + e], res[7 * total + e], res[8 * total + e]);
+    v3<double> F4 = V(res[9 * total + e], res[10 * total + e], res[11 * total + e]);
+    special_pair_fix(K, P, Q, F1, F2, F3, F4, flags);
+    const double v[12] = {F1.x, F1.y, F1.z, F2.x, F2.y, F2.z, F3.x, F3.y, F3.z, F4.x, F4.y, F4.z};
+#pragma unroll
+    for (int c = 0; c < 12; ++c) res[c * total + e] = v[c];
+  }
 }
 
 // Exclusive scan of n counts (64-bit sums) in three passes: block sums (2,048 per
@@ -274,8 +321,12 @@ void launch_special_list(const double* seg, int64_t ld, int n, const MixedRule* 
   if (sp.total <= 0) return;
   if (mixed) launch_special_pass_mixed(seg, ld, n, *mixed, nullptr, &sp, r.t0(), r.t1(), st);
   else special_pass_plain<1>(seg, ld, n, nullptr, &sp, r.t0(), r.t1(), st);
+  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, st, sp.fixCount);
   hipLaunchKernelGGL(k_special_eval, dim3((unsigned)((sp.total + kTile - 1) / kTile)), dim3(kTile), 0,
-                     st, seg, ld, sp.slot, sp.tile, sp.total, K, sp.res, sp.flags);
+                     st, seg, ld, sp.slot, sp.tile, sp.total, K, sp.res, sp.fix, sp.fixCount);
+  const int64_t blocks = std::min<int64_t>((sp.total + kTile - 1) / kTile, 2048);
+  hipLaunchKernelGGL(k_special_fix, dim3((unsigned)blocks), dim3(kTile), 0, st, seg, ld, sp.slot,
+                     sp.tile, sp.total, K, sp.res, sp.fix, sp.fixCount, sp.flags);
 }
 
 void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
