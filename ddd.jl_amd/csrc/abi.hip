@@ -152,7 +152,7 @@ struct ddd_ctx {
   DevBuf specCnt, specOff, specSlot, specTile, specRes, specWork, specAt, specFix, specFixN;
   TileRange pending{0, 0};  // the all-pairs chunk whose reduction is still to run
   int chunks = 0;           // chunks of the last all-pairs evaluation
-  int64_t chunkBytes = (int64_t)16 << 30;
+  int64_t chunkBytes = (int64_t)32 << 30;
   // device-resident dynamics (deriv!, integrate!)
   DevBuf dCoord, dCoord0, dVel, dVel0, dNodeF, dLinks, dConn, dBvec, dLabel, dSegNodes, dRedPart,
       dRed;
@@ -439,7 +439,7 @@ void hybrid_apply(ddd_ctx* c, int64_t n, double* dev, const ddd_opts& o) {
 // Device memory of one chunk of the all-pairs triangle: its partial sums
 // (row and column, 6 doubles x 64 lanes each per tile) and its near-parallel
 // list (slot, tile, 12 forces per pair), at most the context's chunkBytes
-// (default 16 GiB).  A chunk holds whole column blocks, so a single block
+// (default 32 GiB).  A chunk holds whole column blocks, so a single block
 // column above the budget (at most 65,535 tiles under the tile-count limit)
 // makes a chunk of its own.
 constexpr int64_t kTileBytes = 2 * 6 * kTile * (int64_t)sizeof(double);
@@ -646,7 +646,7 @@ void* ddd_ctx_stream(ddd_ctx* c) { return c ? (void*)c->st : nullptr; }
 int ddd_ctx_set_chunk_bytes(ddd_ctx* c, int64_t bytes) {
   return guard(c, "ddd_ctx_set_chunk_bytes", [&] {
     if (bytes < 0) bad("bytes < 0");
-    c->chunkBytes = bytes > 0 ? bytes : ((int64_t)16 << 30);
+    c->chunkBytes = bytes > 0 ? bytes : ((int64_t)32 << 30);
   });
 }
 

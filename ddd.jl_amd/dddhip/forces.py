@@ -56,7 +56,7 @@ def calcSegSegForce(dlnParams, matParams, network, idx=None, ctx=None):
 
 
 def setChunkBytes(nbytes, ctx=None):
-    """Memory budget of one chunk of the all-pairs evaluation (0: 16 GiB)."""
+    """Memory budget of one chunk of the all-pairs evaluation (0: 32 GiB)."""
     ctx = _ctx(ctx)
     _call(ctx, "ddd_ctx_set_chunk_bytes", int(nbytes))
 

@@ -211,7 +211,7 @@ void* ddd_ctx_stream(ddd_ctx* ctx);
 /* Memory budget of one chunk of the all-pairs evaluation (FP64 and mixed
  * modes; no counterpart in the reference): the triangle of pair tiles is
  * evaluated in chunks of whole 64-segment column blocks whose partial sums
- * and near-parallel lists fit in `bytes` (0: the default, 16 GiB), each
+ * and near-parallel lists fit in `bytes` (0: the default, 32 GiB), each
  * reduced into segForce in a fixed order before the next. */
 int ddd_ctx_set_chunk_bytes(ddd_ctx* ctx, int64_t bytes);
 const char* ddd_last_error(ddd_ctx* ctx);

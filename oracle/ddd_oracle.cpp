@@ -749,12 +749,12 @@ void orc_mobility_bcc(const double* drag, double sigmaPN, const int64_t* connect
 // :602-707 (Barnett triangles, Bromage & Tarleton 2018), safeNorm
 // (Misc.jl:289-299), linePlaneIntersect (Misc.jl:223-235) and
 // findIntersectVolume (DislocationTopologySpecial.jl:7-38).
-// Parity unpinned: the closure of segments with an external node (their ends
-// projected onto the mesh surface, DislocationFEM.jl:437-528).  The
-// reference's tests hold no vectors with external nodes; that part is
-// checked only for consistency (closure independence, the jump of b) and
-// against the device.  Internal segments are pinned by the reference's
-// golden displacements.
+// Internal segments are pinned by the reference's golden displacements.  The
+// closure of segments with an external node (their ends projected onto the
+// mesh surface, DislocationFEM.jl:437-528), for which the reference's tests
+// hold no vectors, is pinned by geometry: the surface points in closed form
+// (and one hand-worked trace of the face-plane walk), the polygon's solid
+// angle by another triangulation and formula, up to whole Burgers vectors.
 namespace {
 
 struct UT {

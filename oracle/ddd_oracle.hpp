@@ -12,10 +12,12 @@
 // the per-pair flop figures used for the roofline.
 //
 // Every function is pinned by the reference's golden vectors, closed forms or
-// invariants (tests/test_oracle*.py) except one part, which is parity
-// unpinned: the closure of calc_uTilde's segments with an external node
-// (orc_u_tilde's surface projection, DislocationFEM.jl:437-528), for which
-// the reference holds no vectors.
+// invariants (tests/test_oracle*.py).  The closure of calc_uTilde's segments
+// with an external node (orc_u_tilde's surface projection,
+// DislocationFEM.jl:437-528), for which the reference holds no vectors, is
+// pinned by geometry: closed-form surface points, a hand-worked trace of the
+// reference's face-plane walk, and an independent solid-angle evaluation of
+// the closure polygon, equal up to whole Burgers vectors.
 #pragma once
 #include <cmath>
 #include <cstdint>

@@ -2,7 +2,7 @@
 
 The triangle of 64 x 64 pair tiles is evaluated chunk by chunk (whole
 column blocks whose partial sums and near-parallel lists fit the context's
-budget, default 16 GiB), each chunk reduced into segForce in a fixed order.
+budget, default 32 GiB), each chunk reduced into segForce in a fixed order.
 Small budgets force many chunks on networks the oracle checks
 (calcSegSegForcePar, DislocationForces.jl:719-777, 1e-10 per node); a
 524,288-segment network runs within the default budget and is checked by
@@ -82,7 +82,7 @@ def test_524288_segments_within_the_budget():
     f = res.download()
     _log(f"chunks {chunks}, device bytes {nbytes / 2**30:.2f} GiB")
     # unchunked, the partial sums alone would take 8,192 x 8,193 / 2 x 6 KiB = 192 GiB
-    assert chunks >= 10 and nbytes <= 20 * 2**30
+    assert chunks >= 5 and nbytes <= 40 * 2**30
     assert np.all(np.isfinite(f))
     # sampled rows against the oracle's triangle in the same orientation
     # (pair (min, max) of every pair); the resident total holds the self

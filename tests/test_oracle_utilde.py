@@ -8,8 +8,10 @@ point only through whole Burgers vectors (the solid angle jumps by 4 pi when
 a closure cone sweeps over a point; the line terms of the closure edges
 cancel between neighbouring triangles), and it jumps by exactly b across the
 cut surface.  The closure of external segments (projection onto the mesh
-surface, :437-528) has no reference vectors: parity unpinned beyond the
-consistency checks here and the device comparison."""
+surface, :437-528) has no reference vectors; it is pinned by geometry: the
+surface points in closed form (one by a hand-worked trace of the
+reference's walk) and the closure polygon's solid angle by an independent
+triangulation and formula, equal up to whole Burgers vectors."""
 import numpy as np
 import pytest
 
@@ -95,7 +97,7 @@ def test_jump_of_b_across_the_cut(kind):
 def test_external_segments_run_and_change_u():
     """A loop with external nodes and a surface node: the closure over the
     projected points runs, stays finite and differs from the all-internal
-    result (parity unpinned: no reference vectors exist for it)."""
+    result (its geometry is pinned case by case below)."""
     mat = MaterialParameters(μ=1.0, ν=0.28)
     net = DislocationNetwork.from_loops([_loop("shear", (1000.0, 1000.0, 1500.0), sides=8)])
     m = _mesh((0, 0, 0), (2000, 2000, 3000), mat)
@@ -109,3 +111,96 @@ def test_external_segments_run_and_change_u():
                                        int(net.numSeg[0]), net.maxConnect)
     u1 = O.u_tilde_points(mat, ext, m, pts)
     assert np.all(np.isfinite(u1)) and not np.allclose(u0, u1)
+
+
+# ---- the closure of segments with external nodes (:437-528) ----------------
+# For an external segment A -> B the reference finds A', B', the points of the
+# mesh surface its ends were projected out from (the nearest intersection of
+# the lines through A along the normals of faces 1, 3, 5 -- -y, +y, -z -- with
+# the volume; a surface node linked to an external node by a virtual segment
+# takes that role; with no intersection, a walk over the face planes), then
+# adds the solid-angle terms of the triangle (A', B', C) and of the closed
+# polygon A -> B -> B' -> A' fanned from C' = (A + B') / 2.  Everything but
+# the points is geometry: the solid angle of a closed polygon does not depend
+# on its triangulation except by whole 4 pi, so the displacement must equal
+# b / 4 pi (Omega(A', B', C) + Omega(A, B, B') + Omega(A, B', A')) up to whole
+# Burgers vectors, with Omega from the Van Oosterom-Strackee formula (not the
+# oracle's l'Huilier evaluation) and A', B' fixed in closed form or by hand.
+
+def _vos(A, B, Cc, p):
+    """Solid angle of triangle (A, B, C) seen from points p (3 x P), signed by
+    the orientation, Van Oosterom & Strackee, IEEE TBME 30 (1983) 125."""
+    R1, R2, R3 = (np.asarray(X, float)[:, None] - p for X in (A, B, Cc))
+    n1, n2, n3 = (np.linalg.norm(R, axis=0) for R in (R1, R2, R3))
+    num = np.einsum("ij,ij->j", R1, np.cross(R2.T, R3.T).T)
+    den = (n1 * n2 * n3 + np.einsum("ij,ij->j", R1, R2) * n3 + np.einsum("ij,ij->j", R1, R3) * n2
+           + np.einsum("ij,ij->j", R2, R3) * n1)
+    return 2.0 * np.arctan2(num, den)
+
+
+def _closure_u(A, B, Ap, Bp, Cc, b, p):
+    om = _vos(Ap, Bp, Cc, p) + _vos(A, B, Bp, p) + _vos(A, Bp, Ap, p)
+    return np.outer(b, om) / (4 * np.pi)
+
+
+def _ext_network(nodes, labels, links, b):
+    from dddhip.types import DislocationNetwork
+    coord = np.asarray(nodes, dtype=float).T
+    L = np.asarray(links, dtype=np.int64).T
+    bv = np.tile(np.asarray(b, float)[:, None], (1, L.shape[1]))
+    return DislocationNetwork.from_arrays(L, bv, coord, labels)
+
+
+# mesh [0, 2000] x [0, 2000] x [0, 3000], centroid (1000, 1000, 1500)
+_BOX_C = np.array([1000.0, 1000.0, 1500.0])
+_B = np.array([1.0, -1.0, 1.0]) / np.sqrt(3)
+
+EXTERNAL_CASES = {
+    # both ends above the top face: projected straight down onto z = 3000
+    "above_top": ([(700, 800, 3300), (1300, 900, 3400)], [extDln, extDln], [(1, 2)], 0,
+                  (700, 800, 3000), (1300, 900, 3000)),
+    # an internal end 100 below the top: its nearest face along the y and z
+    # lines is z = 3000 (y faces 800 and 1200 away)
+    "internal_end": ([(700, 800, 2900), (1300, 900, 3400)], [intMobDln, extDln], [(1, 2)], 0,
+                     (700, 800, 3000), (1300, 900, 3000)),
+    # beyond the y = 0 face: the y line meets the volume at y = 0
+    "beyond_y0": ([(900, -150, 1200), (1100, -250, 1400)], [extDln, extDln], [(1, 2)], 0,
+                  (900, 0, 1200), (1100, 0, 1400)),
+    # the external end 3 is linked to the surface node 1 by a virtual segment
+    # (listed first, so its connection is recorded before segment 3 -> 2 is
+    # evaluated); 3 then takes node 1 as its surface point
+    "surface_node": ([(1000, 1000, 3000), (1400, 1100, 3300), (1000, 1000, 3200)],
+                     [srfMobDln, extDln, extDln], [(1, 3), (3, 2)], 1,
+                     (1000, 1000, 3000), (1400, 1100, 3000)),
+    # beyond the x = 2000 face: faces 1, 3, 5 are searched (the y and z lines
+    # stay outside the volume), so the reference walks over the face planes,
+    # keeping the candidate closest to the centroid; worked by hand from
+    # :470-489 for A = (2100, 1000, 1500): face 1 gives (2100, 2000, 1500)
+    # (1,486.6 from C, the first candidate), faces 2-6 give (-2000, 2000,
+    # 1500), (2100, -2000, 1500), (4200, 2000, 1500), (2100, 2000, 3000),
+    # (2100, 2000, -3000), all farther; B lies above the top face
+    "corner_walk": ([(2100, 1000, 1500), (1500, 1200, 3200)], [extDln, extDln], [(1, 2)], 0,
+                    (2100, 2000, 1500), (1500, 1200, 3000)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(EXTERNAL_CASES))
+def test_external_closure_against_geometry(case):
+    nodes, labels, links, k, Ap, Bp = EXTERNAL_CASES[case]
+    mat = MaterialParameters(μ=1.0, ν=0.28)
+    net = _ext_network(nodes, labels, links, _B)
+    m = _mesh((0, 0, 0), (2000, 2000, 3000), mat)
+    pts = np.random.default_rng(8).uniform((0, 0, 0), (2000, 2000, 3000), size=(400, 3)).T
+    pts = np.asfortranarray(pts)
+    u = O.u_tilde_points(mat, net, m, pts)
+    A = np.asarray(nodes[links[k][0] - 1], float)
+    B = np.asarray(nodes[links[k][1] - 1], float)
+    want = _closure_u(A, B, np.asarray(Ap, float), np.asarray(Bp, float), _BOX_C, _B, pts)
+    d = u - want
+    kk = np.round(d.T @ _B)
+    assert np.abs(d - np.outer(_B, kk)).max() <= 1e-10
+    assert np.abs(want).max() > 1e-3
+    # a wrong surface point (here: the other end's) is not a whole-b shift
+    bad = _closure_u(A, B, np.asarray(Bp, float), np.asarray(Bp, float), _BOX_C, _B, pts)
+    db = u - bad
+    assert np.abs(db - np.outer(_B, np.round(db.T @ _B))).max() > 1e-4
