@@ -25,10 +25,12 @@ for p in (ROOT / "ddd.jl_amd", ROOT / "oracle", ROOT / "tests"):
 METRIC = "Force-eval time @131k segs: plain FP64 vs hybrid at ≤1e-3 RMS rel force err"
 FLOPS_PER_PAIR = 1466  # counted by the operation-counting oracle (generic pair)
 FP64_PEAK_TFLOPS = 78.6
-# 26.6 GB read + 14.9 GB written per launch of the pair-tile kernel (tile
-# partials, near-parallel forces, spill traffic of four registers),
-# 4,096 loops x 32 segments.
-TILE_TRAFFIC_BYTES = 26.6e9 + 14.9e9
+# HBM bytes per launch of the pair-tile kernel at 4,096 loops x 32 segments
+# (one chunk): FETCH_SIZE 13,052,754 KiB x 2 (the counter reads 524,308 KiB
+# for a 1 GiB copy) + WRITE_SIZE 14,568,639 KiB x 1 (1,048,576 KiB for the
+# same copy).  Most of it is register-spill traffic of the FP64 pair
+# evaluation (20 B/lane scratch) and the 12 x 8 B tile partials per lane.
+TILE_TRAFFIC_BYTES = 13_052_754 * 2 * 1024 + 14_568_639 * 1024
 
 
 def log(msg):
@@ -530,11 +532,16 @@ def profile_run(args, dln, mat, log):
     counter pass each): a plain 1 GiB device copy (1 GiB read + 1 GiB
     written: the scale of the FETCH_SIZE / WRITE_SIZE counters), then
       fp64:      one plain FP64 evaluation of the 131,072-segment network;
-      hybrid:    one hybrid evaluation of it (G = 32, rn = 4, a network
+      hybrid:    two hybrid evaluations of it (G = 32, rn = 4, a network
                  trained for --train-steps steps: kernel times do not depend
                  on the training length);
-      clustered: one hybrid evaluation of the 262,144-segment clustered
-                 network with the first --clustered-fp64-radii rule;
+      clustered: two hybrid evaluations of the 262,144-segment clustered
+                 network with the last --clustered-fp64-radii rule and the
+                 first --clustered-sin-min;
+                 (the first hybrid evaluation of a network sizes its
+                 near-parallel list, running the near-field tiles again when
+                 the list outgrew its first capacity; the second is the
+                 steady state that timed loops see)
       sigma:     calc_σTilde of the clustered network at 2^20 random surface
                  points, and of the 16,384-segment network at 6 x 128^2 face
                  points."""
@@ -567,6 +574,7 @@ def profile_run(args, dln, mat, log):
                             nearFrac=args.near_frac)
             res.set_mode("hybrid", farfield=ff,
                          farDist=float(str(args.near_fp64_radii).split(",")[0]) * dln.coreRad)
+            res.seg_force()
         res.seg_force()
         torch.cuda.synchronize()
         out["segments"] = int(net.numSeg[0])
@@ -581,11 +589,14 @@ def profile_run(args, dln, mat, log):
                         lr=2e-3, lrFinal=5e-6, seed=7, denFloor=0.1,
                         nearFrac=args.clustered_near_frac)
         fr = float(str(args.clustered_fp64_radii).split(",")[-1])
-        res.set_mode("hybrid", farfield=ff, farDist=fr * dln.coreRad)
+        sm = float(str(args.clustered_sin_min).split(",")[0])
+        res.set_mode("hybrid", farfield=ff, farDist=fr * dln.coreRad, sinMin=sm)
+        res.seg_force()
         res.seg_force()
         torch.cuda.synchronize()
         out["segments"] = int(net.numSeg[0])
         out["fp64_core_radii"] = fr
+        out["sin_min"] = sm
         out["pairs"] = res.stats()
     elif what == "sigma":
         net = clustered_loop_network(seed=0xE0)

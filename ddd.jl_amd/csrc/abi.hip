@@ -48,9 +48,11 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   template <class T> T* as() const { return static_cast<T*>(p); }
-  void ensure(size_t need) {
+  // Grows by half again at least (fewer reallocations as a network grows),
+  // but never past cap when need is within it.
+  void ensure(size_t need, size_t cap = SIZE_MAX) {
     if (need <= bytes) return;
-    const size_t want = std::max(need, bytes + bytes / 2);
+    const size_t want = std::max(need, std::min(bytes + bytes / 2, cap));
     if (p) HIPCHK(hipFree(p));
     p = nullptr;
     bytes = 0;
@@ -134,7 +136,7 @@ struct ddd_ctx {
   std::mutex mu;
   std::string err;
   // segment SoA and pair-kernel scratch
-  DevBuf seg, rowPart, colPart, force, rows, out, flags;
+  DevBuf seg, force, rows, out, flags;
   // mesh
   DevBuf meshC, meshCoord, meshConn, uHat;
   // field-point stress
@@ -148,9 +150,15 @@ struct ddd_ctx {
   DevBuf cKeys, cKeysOut, cIdx, cPerm, cSortWork, cBox, cPrev;
   std::vector<double> h;  // host staging
   DevBuf counter;  // FP64-pair counter of the mixed path
-  // near-parallel pairs of the all-pairs tiles (count, offsets, list, forces)
-  DevBuf specCnt, specOff, specSlot, specTile, specRes, specWork, specAt, specFix, specFixN;
+  // near-parallel pairs of the all-pairs tiles (count and offsets over every
+  // tile, the scan's work, offsets at the column-block boundaries, fix count)
+  DevBuf specCnt, specOff, specWork, specAt, specFixN;
+  // one chunk of the all-pairs triangle: its tile partials and its
+  // near-parallel list, laid out per chunk by ChunkLayout
+  DevBuf chunkMem;
   TileRange pending{0, 0};  // the all-pairs chunk whose reduction is still to run
+  double* pendRow = nullptr;  // its partial sums in chunkMem
+  double* pendCol = nullptr;
   int chunks = 0;           // chunks of the last all-pairs evaluation
   int64_t chunkBytes = (int64_t)32 << 30;
   // device-resident dynamics (deriv!, integrate!)
@@ -436,15 +444,32 @@ void hybrid_apply(ddd_ctx* c, int64_t n, double* dev, const ddd_opts& o) {
   HIPCHK(hipGetLastError());
 }
 
-// Device memory of one chunk of the all-pairs triangle: its partial sums
-// (row and column, 6 doubles x 64 lanes each per tile) and its near-parallel
-// list (slot, tile, 12 forces per pair), at most the context's chunkBytes
-// (default 32 GiB).  A chunk holds whole column blocks, so a single block
-// column above the budget (at most 65,535 tiles under the tile-count limit)
-// makes a chunk of its own.
-constexpr int64_t kTileBytes = 2 * 6 * kTile * (int64_t)sizeof(double);
-constexpr int64_t kSpecialBytes =
-    sizeof(unsigned short) + sizeof(int) + 12 * sizeof(double) + sizeof(int64_t);
+// Device memory of one chunk of the all-pairs triangle, one buffer carved
+// per chunk: row and column partial sums (6 doubles x 64 lanes each per
+// tile) and the near-parallel list (slot, tile, 12 forces and a fix-list
+// entry per pair), each part 256-byte aligned.  A chunk takes whole column
+// blocks while its layout stays within the context's chunkBytes (default
+// 32 GiB), so the buffer never exceeds the budget; a single block column
+// above it (at most 65,535 tiles under the tile-count limit) makes a chunk
+// of its own.
+struct ChunkLayout {
+  int64_t row = 0, col = 0, slot = 0, tile = 0, res = 0, fix = 0, bytes = 0;
+  ChunkLayout(int64_t tiles, int64_t spec) {
+    int64_t at = 0;
+    auto part = [&](int64_t b) {
+      const int64_t o = at;
+      at += (b + 255) / 256 * 256;
+      return o;
+    };
+    row = part(sizeof(double) * 6 * kTile * tiles);
+    col = part(sizeof(double) * 6 * kTile * tiles);
+    slot = part(sizeof(unsigned short) * spec);
+    tile = part(sizeof(int) * spec);
+    res = part(sizeof(double) * 12 * spec);
+    fix = part(sizeof(int64_t) * spec);
+    bytes = at;
+  }
+};
 
 // Pair tiles of the whole network (mode from opts), chunk by chunk: a count
 // pass and scan over every tile for the near-parallel pairs (one host
@@ -454,7 +479,7 @@ constexpr int64_t kSpecialBytes =
 // sums) and the reduction of its partials into dev -- except the last
 // chunk's reduction, which segseg_finish runs (the bench times the tile
 // stage on its own; with one chunk, as at 131,072 segments, that is the
-// whole reduction).  Memory is bounded by kChunkBytes instead of growing
+// whole reduction).  Memory is bounded by chunkBytes instead of growing
 // with the N^2 / 8192 tiles.
 void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
                   const ddd_opts* opts, double* dev) {
@@ -481,42 +506,43 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
   HIPCHK(hipStreamSynchronize(c->st));
   // chunks of whole column blocks within the budget
   std::vector<TileRange> chunks;
-  int64_t maxTiles = 1, maxSpec = 1;
-  auto bytes = [&](int J0, int J1) {
-    return kTileBytes * (tiles_for(J1) - tiles_for(J0)) + kSpecialBytes * (offAt[J1] - offAt[J0]);
+  auto layout = [&](int J0, int J1) {
+    return ChunkLayout(tiles_for(J1) - tiles_for(J0), offAt[J1] - offAt[J0]);
   };
+  int64_t most = 0;
   for (int J0 = 0; J0 < nb;) {
     int J1 = J0 + 1;
-    while (J1 < nb && bytes(J0, J1 + 1) <= c->chunkBytes) ++J1;
+    while (J1 < nb && layout(J0, J1 + 1).bytes <= c->chunkBytes) ++J1;
     chunks.push_back(TileRange{J0, J1});
-    maxTiles = std::max<int64_t>(maxTiles, tiles_for(J1) - tiles_for(J0));
-    maxSpec = std::max<int64_t>(maxSpec, offAt[J1] - offAt[J0]);
+    most = std::max(most, layout(J0, J1).bytes);
     J0 = J1;
   }
   c->chunks = (int)chunks.size();
-  c->rowPart.ensure(sizeof(double) * 6 * kTile * (size_t)maxTiles);
-  c->colPart.ensure(sizeof(double) * 6 * kTile * (size_t)maxTiles);
-  c->specSlot.ensure(sizeof(unsigned short) * maxSpec);
-  c->specTile.ensure(sizeof(int) * maxSpec);
-  c->specRes.ensure(sizeof(double) * 12 * maxSpec);
-  c->specFix.ensure(sizeof(int64_t) * maxSpec);
+  c->chunkMem.ensure((size_t)most, (size_t)std::max(most, c->chunkBytes));
   c->specFixN.ensure(sizeof(unsigned long long));
   if (mixed) HIPCHK(hipMemsetAsync(c->counter.p, 0, sizeof(unsigned long long), c->st));
+  char* mem = c->chunkMem.as<char>();
   for (size_t q = 0; q < chunks.size(); ++q) {
     const TileRange& ch = chunks[q];
-    SpecialList sp{c->specOff.as<long long>(), c->specSlot.as<unsigned short>(), c->specTile.as<int>(),
-                   c->specRes.as<double>(), (int64_t)(offAt[ch.J1] - offAt[ch.J0]),
-                   (int64_t)offAt[ch.J0], flags, c->specFix.as<int64_t>(),
-                   c->specFixN.as<unsigned long long>()};
+    const ChunkLayout L = layout(ch.J0, ch.J1);
+    double* rowPart = reinterpret_cast<double*>(mem + L.row);
+    double* colPart = reinterpret_cast<double*>(mem + L.col);
+    SpecialList sp{c->specOff.as<long long>(), reinterpret_cast<unsigned short*>(mem + L.slot),
+                   reinterpret_cast<int*>(mem + L.tile), reinterpret_cast<double*>(mem + L.res),
+                   (int64_t)(offAt[ch.J1] - offAt[ch.J0]), (int64_t)offAt[ch.J0], flags,
+                   reinterpret_cast<int64_t*>(mem + L.fix), c->specFixN.as<unsigned long long>()};
     launch_special_list(c->seg.as<double>(), c->resLd, (int)n, mixed ? &r : nullptr, sp,
                         seg_consts(k), ch, c->st);
-    launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k), c->rowPart.as<double>(),
-                        c->colPart.as<double>(), sp, ch, c->st, mixed ? &r : nullptr,
+    launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k), rowPart, colPart, sp,
+                        ch, c->st, mixed ? &r : nullptr,
                         mixed ? c->counter.as<unsigned long long>() : nullptr);
-    if (q + 1 < chunks.size())
-      launch_segseg_reduce(c->rowPart.as<double>(), c->colPart.as<double>(), (int)n, ch, dev, c->st);
-    else
+    if (q + 1 < chunks.size()) {
+      launch_segseg_reduce(rowPart, colPart, (int)n, ch, dev, c->st);
+    } else {
       c->pending = ch;
+      c->pendRow = rowPart;
+      c->pendCol = colPart;
+    }
   }
   HIPCHK(hipGetLastError());
 }
@@ -524,8 +550,7 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
 // The last chunk's reduction (see segseg_tiles).
 void segseg_finish(ddd_ctx* c, int64_t n, double* dev) {
   if (c->pending.J1 > 0)
-    launch_segseg_reduce(c->rowPart.as<double>(), c->colPart.as<double>(), (int)n, c->pending, dev,
-                         c->st);
+    launch_segseg_reduce(c->pendRow, c->pendCol, (int)n, c->pending, dev, c->st);
   c->pending = TileRange{0, 0};
   HIPCHK(hipGetLastError());
 }
@@ -623,10 +648,10 @@ int ddd_ctx_destroy(ddd_ctx* c) {
     std::lock_guard<std::mutex> lk(c->mu);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->st);
-    DevBuf* all[] = {&c->counter, &c->seg, &c->rowPart, &c->colPart, &c->force, &c->rows, &c->out,
+    DevBuf* all[] = {&c->counter, &c->seg, &c->chunkMem, &c->force, &c->rows, &c->out,
                      &c->flags, &c->meshC, &c->meshCoord, &c->meshConn, &c->uHat, &c->sseg,
                      &c->pts, &c->sigPart, &c->sigma, &c->conn, &c->links, &c->coord, &c->nodeF,
-                     &c->specCnt, &c->specOff, &c->specSlot, &c->specTile, &c->specRes, &c->specWork};
+                     &c->specCnt, &c->specOff, &c->specWork};
     for (DevBuf* b : all) b->release();
     if (c->own) (void)hipStreamDestroy(c->own);
   }
@@ -1499,15 +1524,15 @@ int ddd_net_download(ddd_ctx* c, double* out, int64_t outCols) {
 // Device memory the context's own buffers hold (far fields and networks
 // excluded).
 uint64_t ctx_device_bytes(ddd_ctx* c) {
-  const DevBuf* all[] = {&c->seg, &c->rowPart, &c->colPart, &c->force, &c->rows, &c->out, &c->flags,
+  const DevBuf* all[] = {&c->seg, &c->chunkMem, &c->force, &c->rows, &c->out, &c->flags,
                          &c->meshC, &c->meshCoord, &c->meshConn, &c->uHat, &c->sseg, &c->pts,
                          &c->sigPart, &c->sigma, &c->conn, &c->links, &c->coord, &c->nodeF,
                          &c->bvec, &c->nodeV, &c->nodeIdx, &c->uTri, &c->uPts, &c->uPart, &c->uOut,
                          &c->cConn, &c->cLinks, &c->cCoord, &c->cVel, &c->cB, &c->cLabel,
                          &c->cSkip, &c->cSeg, &c->cNodes, &c->cFlags, &c->cBest, &c->cLast,
                          &c->cRes, &c->cKeys, &c->cKeysOut, &c->cIdx, &c->cPerm, &c->cSortWork,
-                         &c->cBox, &c->cPrev, &c->counter, &c->specCnt, &c->specOff, &c->specSlot,
-                         &c->specTile, &c->specRes, &c->specWork, &c->specAt, &c->specFix, &c->specFixN, &c->dCoord,
+                         &c->cBox, &c->cPrev, &c->counter, &c->specCnt, &c->specOff,
+                         &c->specWork, &c->specAt, &c->specFixN, &c->dCoord,
                          &c->dCoord0, &c->dVel, &c->dVel0, &c->dNodeF, &c->dLinks, &c->dConn,
                          &c->dBvec, &c->dLabel, &c->dSegNodes, &c->dRedPart, &c->dRed};
   uint64_t b = 0;
