@@ -73,8 +73,9 @@ def hybrid_paths(args, dln, mat, net, res, stream, ref, fp64_ms, log):
             f"{rep['lossLast']:.3e}, held-out local-expansion error {rep['relErrVal']:.3e}")
         for r in rns[1:]:
             G.setFarFieldFromMLP(dln, mat, ffs[r], mlp)
-        for r in rns:
-            res.set_mode("hybrid", farfield=ffs[r], farDist=args.near_fp64_radii * dln.coreRad)
+        radii = [float(x) for x in str(args.near_fp64_radii).split(",") if x]
+        for r, fr in ((r, fr) for r in rns for fr in radii):
+            res.set_mode("hybrid", farfield=ffs[r], farDist=fr * dln.coreRad)
             res.seg_force()
             stream.synchronize()
             ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)]
@@ -89,7 +90,7 @@ def hybrid_paths(args, dln, mat, net, res, stream, ref, fp64_ms, log):
             near, near64 = res.stats()
             smx, srms = per_node_rel(got, ref)
             nmx, nrms = per_node_rel(node_forces(net, got), Fref)
-            row = {"G": Gc, "rn": r, "h": h, "fp64_core_radii": args.near_fp64_radii,
+            row = {"G": Gc, "rn": r, "h": h, "fp64_core_radii": fr,
                    "ms": round(hms, 3),
                    "speedup_vs_fp64": round(fp64_ms / hms, 2),
                    "node_rel_err_rms": nrms, "node_rel_err_max": nmx,
@@ -98,7 +99,7 @@ def hybrid_paths(args, dln, mat, net, res, stream, ref, fp64_ms, log):
                    "near_pairs": near, "near_pairs_fp64": near64,
                    "train": {k: ([round(x, 6) for x in v] if isinstance(v, list) else round(v, 6))
                              for k, v in rep.items()}}
-            log(f"hybrid G={Gc} rn={r}: {hms:.2f} ms ({fp64_ms / hms:.1f}x), node err rms "
+            log(f"hybrid G={Gc} rn={r} fp64<{fr}a: {hms:.2f} ms ({fp64_ms / hms:.1f}x), node err rms "
                 f"{nrms:.2e} max {nmx:.2e}; {near} near pairs, {near64} FP64")
             out["configs"].append(row)
             res.set_mode("fp64")
@@ -532,9 +533,9 @@ def profile_run(args, dln, mat, log):
     counter pass each): a plain 1 GiB device copy (1 GiB read + 1 GiB
     written: the scale of the FETCH_SIZE / WRITE_SIZE counters), then
       fp64:      one plain FP64 evaluation of the 131,072-segment network;
-      hybrid:    two hybrid evaluations of it (G = 32, rn = 4, a network
-                 trained for --train-steps steps: kernel times do not depend
-                 on the training length);
+      hybrid:    two hybrid evaluations of it (G = 32, rn = 4, the last
+                 --near-fp64-radii rule, a network trained for --train-steps
+                 steps: kernel times do not depend on the training length);
       clustered: two hybrid evaluations of the 262,144-segment clustered
                  network with the last --clustered-fp64-radii rule and the
                  first --clustered-sin-min;
@@ -573,7 +574,7 @@ def profile_run(args, dln, mat, log):
                             batch=args.train_batch, lr=2e-3, lrFinal=5e-6, seed=7, denFloor=0.1,
                             nearFrac=args.near_frac)
             res.set_mode("hybrid", farfield=ff,
-                         farDist=float(str(args.near_fp64_radii).split(",")[0]) * dln.coreRad)
+                         farDist=float(str(args.near_fp64_radii).split(",")[-1]) * dln.coreRad)
             res.seg_force()
         res.seg_force()
         torch.cuda.synchronize()
@@ -631,9 +632,9 @@ def main():
                     help="comma list of G:rn hybrid configurations (G^3 cells over the "
                          "network's bounding cube, near stencil of Chebyshev radius rn); "
                          "empty to skip")
-    ap.add_argument("--near-fp64-radii", type=float, default=5.0,
-                    help="hybrid near field: pairs closer than this many core radii (or "
-                         "near-parallel) run in FP64, the rest of the stencil in FP32")
+    ap.add_argument("--near-fp64-radii", default="5,3,2",
+                    help="hybrid near field, comma list: pairs closer than this many core "
+                         "radii (or near-parallel) run in FP64, the rest of the stencil in FP32")
     ap.add_argument("--train-loops", type=int, default=4096)
     ap.add_argument("--train-seed", type=lambda s: int(s, 0), default=0xD1)
     ap.add_argument("--train-steps", type=int, default=60000)
@@ -654,7 +655,7 @@ def main():
                     help="skip the detectCollision leg")
     ap.add_argument("--no-dynamics", dest="dynamics", action="store_false",
                     help="skip the deriv!/integrate! leg (run with the first hybrid grid)")
-    ap.add_argument("--clustered-fp64-radii", default="5,3",
+    ap.add_argument("--clustered-fp64-radii", default="5,3,2",
                     help="clustered leg's near field: pairs closer than this many core radii "
                          "run in FP64, the rest of the stencil in FP32")
     ap.add_argument("--clustered-sin-min", default="0",
@@ -851,7 +852,8 @@ def main():
         result["headline"] = {"fp64_ms": round(ms, 3), "hybrid_ms": best["ms"],
                               "hybrid_speedup": round(ms / best["ms"], 2),
                               "hybrid_node_rel_err_rms": best["node_rel_err_rms"],
-                              "hybrid_grid": {"G": best["G"], "rn": best["rn"]}}
+                              "hybrid_grid": {"G": best["G"], "rn": best["rn"]},
+                              "hybrid_fp64_core_radii": best["fp64_core_radii"]}
 
     if not args.no_cpu:
         import pyoracle as O
