@@ -153,6 +153,7 @@ struct ddd_ctx {
   // near-parallel pairs of the all-pairs tiles (count and offsets over every
   // tile, the scan's work, offsets at the column-block boundaries, fix count)
   DevBuf specCnt, specOff, specWork, specAt, specFixN;
+  DevBuf redoN;  // tiles the ranged pass left to the general one (count)
   // one chunk of the all-pairs triangle: its tile partials and its
   // near-parallel list, laid out per chunk by ChunkLayout
   DevBuf chunkMem;
@@ -446,14 +447,15 @@ void hybrid_apply(ddd_ctx* c, int64_t n, double* dev, const ddd_opts& o) {
 
 // Device memory of one chunk of the all-pairs triangle, one buffer carved
 // per chunk: row and column partial sums (6 doubles x 64 lanes each per
-// tile) and the near-parallel list (slot, tile, 12 forces and a fix-list
-// entry per pair), each part 256-byte aligned.  A chunk takes whole column
+// tile), the list of tiles the ranged pass leaves to the general one (one
+// int per tile) and the near-parallel list (slot, tile, 12 forces and a
+// fix-list entry per pair), each part 256-byte aligned.  A chunk takes whole column
 // blocks while its layout stays within the context's chunkBytes (default
 // 32 GiB), so the buffer never exceeds the budget; a single block column
 // above it (at most 65,535 tiles under the tile-count limit) makes a chunk
 // of its own.
 struct ChunkLayout {
-  int64_t row = 0, col = 0, slot = 0, tile = 0, res = 0, fix = 0, bytes = 0;
+  int64_t row = 0, col = 0, redo = 0, slot = 0, tile = 0, res = 0, fix = 0, bytes = 0;
   ChunkLayout(int64_t tiles, int64_t spec) {
     int64_t at = 0;
     auto part = [&](int64_t b) {
@@ -463,6 +465,7 @@ struct ChunkLayout {
     };
     row = part(sizeof(double) * 6 * kTile * tiles);
     col = part(sizeof(double) * 6 * kTile * tiles);
+    redo = part(sizeof(int) * tiles);
     slot = part(sizeof(unsigned short) * spec);
     tile = part(sizeof(int) * spec);
     res = part(sizeof(double) * 12 * spec);
@@ -520,6 +523,7 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
   c->chunks = (int)chunks.size();
   c->chunkMem.ensure((size_t)most, (size_t)std::max(most, c->chunkBytes));
   c->specFixN.ensure(sizeof(unsigned long long));
+  c->redoN.ensure(sizeof(unsigned long long));
   if (mixed) HIPCHK(hipMemsetAsync(c->counter.p, 0, sizeof(unsigned long long), c->st));
   char* mem = c->chunkMem.as<char>();
   for (size_t q = 0; q < chunks.size(); ++q) {
@@ -534,7 +538,8 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
     launch_special_list(c->seg.as<double>(), c->resLd, (int)n, mixed ? &r : nullptr, sp,
                         seg_consts(k), ch, c->st);
     launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k), rowPart, colPart, sp,
-                        ch, c->st, mixed ? &r : nullptr,
+                        ch, c->st, reinterpret_cast<int*>(mem + L.redo),
+                        c->redoN.as<unsigned long long>(), mixed ? &r : nullptr,
                         mixed ? c->counter.as<unsigned long long>() : nullptr);
     if (q + 1 < chunks.size()) {
       launch_segseg_reduce(rowPart, colPart, (int)n, ch, dev, c->st);
@@ -1532,7 +1537,7 @@ uint64_t ctx_device_bytes(ddd_ctx* c) {
                          &c->cSkip, &c->cSeg, &c->cNodes, &c->cFlags, &c->cBest, &c->cLast,
                          &c->cRes, &c->cKeys, &c->cKeysOut, &c->cIdx, &c->cPerm, &c->cSortWork,
                          &c->cBox, &c->cPrev, &c->counter, &c->specCnt, &c->specOff,
-                         &c->specWork, &c->specAt, &c->specFixN, &c->dCoord,
+                         &c->specWork, &c->specAt, &c->specFixN, &c->redoN, &c->dCoord,
                          &c->dCoord0, &c->dVel, &c->dVel0, &c->dNodeF, &c->dLinks, &c->dConn,
                          &c->dBvec, &c->dLabel, &c->dSegNodes, &c->dRedPart, &c->dRed};
   uint64_t b = 0;

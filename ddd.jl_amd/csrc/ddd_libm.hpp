@@ -33,28 +33,22 @@ DDD_HD double from_bits(uint64_t u) {
   return x;
 }
 
-// log(x) for x > 0 finite; other inputs follow IEEE conventions
-// (log(0) = -inf, log(x<0) = nan, log(inf) = inf, log(nan) = nan).
-DDD_HD double log(double x) {
+// IEEE division; the device-only ranged forms substitute another division
+// that gives the same bits on the operands these functions pass it.
+struct IeeeDiv {
+  DDD_HD double operator()(double a, double b) const { return a / b; }
+};
+
+// log(x) + k ln 2 for positive normal x: reduction to [sqrt(2)/2, sqrt(2)),
+// s = f / (2 + f) (through div), the polynomial in s^2.
+template <class Div> DDD_HD double log_core(double x, int k, Div div) {
   const double ln2_hi = 6.93147180369123816490e-01;
   const double ln2_lo = 1.90821492927058770002e-10;
   const double Lg1 = 6.666666666666735130e-01, Lg2 = 3.999999999940941908e-01,
                Lg3 = 2.857142874366239149e-01, Lg4 = 2.222219843214978396e-01,
                Lg5 = 1.818357216161805012e-01, Lg6 = 1.531383769920937332e-01,
                Lg7 = 1.479819860511658591e-01;
-  uint64_t u = bits_of(x);
-  int k = 0;
-  const bool special = !(x > 0.0) || !(x < 1.0 / 0.0);  // <= 0, nan or inf
-  if (special) {
-    if (x == 0.0) return -1.0 / 0.0;
-    if (x != x || x > 0.0) return x + x;  // nan or +inf
-    return (x - x) / 0.0;                 // negative -> nan
-  }
-  if ((u >> 52) == 0) {  // subnormal: scale into the normal range
-    x *= 18014398509481984.0;  // 2^54
-    u = bits_of(x);
-    k -= 54;
-  }
+  const uint64_t u = bits_of(x);
   // Normalise the mantissa into [sqrt(2)/2, sqrt(2)).
   uint32_t hx = (uint32_t)(u >> 32);
   hx += 0x3ff00000u - 0x3fe6a09eu;
@@ -63,7 +57,7 @@ DDD_HD double log(double x) {
   const double m = from_bits(((uint64_t)hx << 32) | (u & 0xffffffffull));
   const double f = m - 1.0;
   const double hfsq = 0.5 * f * f;
-  const double s = f / (2.0 + f);
+  const double s = div(f, 2.0 + f);
   const double z = s * s;
   const double w = z * z;
   const double t1 = w * (Lg2 + w * (Lg4 + w * Lg6));
@@ -73,8 +67,26 @@ DDD_HD double log(double x) {
   return s * (hfsq + R) + dk * ln2_lo - hfsq + f + dk * ln2_hi;
 }
 
-// atan(x) for all x.
-DDD_HD double atan(double x) {
+// log(x) for x > 0 finite; other inputs follow IEEE conventions
+// (log(0) = -inf, log(x<0) = nan, log(inf) = inf, log(nan) = nan).
+DDD_HD double log(double x) {
+  const bool special = !(x > 0.0) || !(x < 1.0 / 0.0);  // <= 0, nan or inf
+  if (special) {
+    if (x == 0.0) return -1.0 / 0.0;
+    if (x != x || x > 0.0) return x + x;  // nan or +inf
+    return (x - x) / 0.0;                 // negative -> nan
+  }
+  int k = 0;
+  if ((bits_of(x) >> 52) == 0) {  // subnormal: scale into the normal range
+    x *= 18014398509481984.0;     // 2^54
+    k = -54;
+  }
+  return log_core(x, k, IeeeDiv{});
+}
+
+// atan(x) for all x, its one division through div (numerator in [-1, 1),
+// denominator in [1, 4.7) or |x| itself from 2.4375 on).
+template <class Div> DDD_HD double atan_core(double x, Div div) {
   const double hi0 = 4.63647609000806093515e-01, lo0 = 2.26987774529616870924e-17;
   const double hi1 = 7.85398163397448278999e-01, lo1 = 3.06161699786838301793e-17;
   const double hi2 = 9.82793723247329054082e-01, lo2 = 1.39033110312309984516e-17;
@@ -102,7 +114,7 @@ DDD_HD double atan(double x) {
   } else {
     num = -1.0; den = ax;
   }
-  const double r = num / den;
+  const double r = div(num, den);
   const double z = r * r;
   const double w = z * z;
   const double s1 = z * (aT0 + w * (aT2 + w * (aT4 + w * (aT6 + w * (aT8 + w * aT10)))));
@@ -114,5 +126,7 @@ DDD_HD double atan(double x) {
   if (ax >= 7.378697629483821e+19) res = hi3 + lo3;  // |x| >= 2^66
   return neg ? -res : res;
 }
+
+DDD_HD double atan(double x) { return atan_core(x, IeeeDiv{}); }
 
 }  // namespace ddd_libm

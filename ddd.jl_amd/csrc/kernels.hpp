@@ -81,9 +81,13 @@ void launch_exclusive_scan(const unsigned* cnt, int n, long long* off, long long
 // offAt[k] = off[tiles_for(k)] for k = 0..nb (the list offsets at column-block
 // boundaries, which size the chunks).
 void launch_block_offsets(const long long* off, int nb, long long* offAt, hipStream_t st);
+// The pair tiles of range r into rowPart / colPart (slot t - r.t0()).  Plain
+// FP64: a ranged pass and a pass over the tiles it left undone, listed in
+// redo (capacity: the range's tile count) with their number in *redoCount.
 void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
                          double* rowPart, double* colPart, const SpecialList& sp,
-                         const TileRange& r, hipStream_t st, const MixedRule* mixed = nullptr,
+                         const TileRange& r, hipStream_t st, int* redo,
+                         unsigned long long* redoCount, const MixedRule* mixed = nullptr,
                          unsigned long long* nExact = nullptr);
 // The mixed-precision kernels' launches (mixed.hip; used by the two above).
 void launch_special_pass_mixed(const double* seg, int64_t ld, int n, const MixedRule& mr,

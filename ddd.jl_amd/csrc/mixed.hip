@@ -36,7 +36,7 @@ void launch_segseg_tiles_mixed(const double* seg, int64_t ld, int n, const SegK<
     const int64_t cnt = (r.t1() - a < maxGrid) ? (r.t1() - a) : maxGrid;
     hipLaunchKernelGGL(k_segseg_tiles<1>, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, a, K,
                        mr.K, mr.farDist2(), mr.sinMin2(), rowPart, colPart, sp.off, sp.slot, sp.res,
-                       sp.total, r.t0(), sp.base, nExact);
+                       sp.total, r.t0(), sp.base, nExact, nullptr, nullptr);
   }
 }
 

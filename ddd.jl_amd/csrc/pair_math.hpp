@@ -11,6 +11,7 @@
 #include <stdint.h>
 
 #include "ddd_libm.hpp"
+#include "fp64_ranged.hpp"
 
 namespace ddd {
 
@@ -154,13 +155,19 @@ template <class T> struct Integ19 {
 // FAST (approximate paths only, T = double): the three reciprocals 1/Ra,
 // 1/(Ra t1), 1/(Ra t2) from one reciprocal, the division by a^2 + d^2 (1-c^2)
 // as a product with its reciprocal aSq_dSqI, flog and fatan; other rounding
-// than the reference's.
-template <class T, bool FAST = false>
+// than the reference's.  RANGED (T = double, not FAST): the same operations
+// as the default, their divisions, square roots, log and atan through the
+// ranged sequences of fp64_ranged.hpp (the same bits on the tiles that use
+// them).
+template <class T, bool FAST = false, bool RANGED = false>
 __device__ __forceinline__ void corner_acc(T aSq_dSq, T aSq_dSqI, T den, T c, T cSq, T omcSqI, T x,
                                            T y, T sgn, bool first, Integ19<T>& I) {
   T xSq = x * x;
   T ySq = y * y;
-  T Ra = sqrt(aSq_dSq + xSq + ySq + T(2) * x * y * c);
+  const T RaSq = aSq_dSq + xSq + ySq + T(2) * x * y * c;
+  T Ra;
+  if constexpr (RANGED) Ra = sqrt_ranged(RaSq);
+  else Ra = sqrt(RaSq);
   T Ra_Rd_t1 = Ra + y + x * c;
   T Ra_Rd_t2 = Ra + x + y * c;
   T RaInv, R1I, R2I, log1, log2;
@@ -172,6 +179,12 @@ __device__ __forceinline__ void corner_acc(T aSq_dSq, T aSq_dSqI, T den, T c, T 
     R2I = Ra_Rd_t1 * inv;
     log1 = flog(Ra_Rd_t1);
     log2 = flog(Ra_Rd_t2);
+  } else if constexpr (RANGED) {
+    RaInv = div_ranged(T(1), Ra);
+    R1I = div_ranged(RaInv, Ra_Rd_t1);
+    R2I = div_ranged(RaInv, Ra_Rd_t2);
+    log1 = log_ranged(Ra_Rd_t1);
+    log2 = log_ranged(Ra_Rd_t2);
   } else {
     RaInv = T(1) / Ra;
     R1I = RaInv / Ra_Rd_t1;
@@ -189,6 +202,7 @@ __device__ __forceinline__ void corner_acc(T aSq_dSq, T aSq_dSqI, T den, T c, T 
   const T atnArg = (T(1) + c) * (Ra + x + y) * den;
   T i1;
   if constexpr (FAST) i1 = T(-2) * den * fatan(atnArg);
+  else if constexpr (RANGED) i1 = T(-2) * den * atan_ranged(atnArg);
   else i1 = T(-2) * den * uatan(atnArg);
   T c_1 = aSq_dSq * i1;
   T c_5_6 = (c * Ra - c_1) * omcSqI;
@@ -215,6 +229,7 @@ __device__ __forceinline__ void corner_acc(T aSq_dSq, T aSq_dSqI, T den, T c, T 
   r[4] = i5;
   r[5] = i6;
   if constexpr (FAST) r[6] = (i1 - xR1I - yR2I) * aSq_dSqI;
+  else if constexpr (RANGED) r[6] = div_ranged(i1 - xR1I - yR2I, aSq_dSq);
   else r[6] = (i1 - xR1I - yR2I) / aSq_dSq;
   r[7] = (R1I - c * R2I) * omcSqI;
   r[8] = (R2I - c * R1I) * omcSqI;
@@ -235,8 +250,9 @@ __device__ __forceinline__ void corner_acc(T aSq_dSq, T aSq_dSqI, T den, T c, T 
 // Generic pair (:893-1146).  W12: forces on segment 1's nodes (F1, F2);
 // W34: forces on segment 2's nodes (F3, F4).  c and omcSq come from the
 // caller (the branch test already computed them).  FAST: corner_acc's
-// shared reciprocals (approximate paths only).
-template <class T, bool W12, bool W34, bool FAST = false>
+// shared reciprocals (approximate paths only).  RANGED: the ranged
+// sequences of fp64_ranged.hpp (see corner_acc).
+template <class T, bool W12, bool W34, bool FAST = false, bool RANGED = false>
 __device__ __forceinline__ void pair_generic(const SegK<T>& K, const Seg<T>& s1, const Seg<T>& s2,
                                              T c, T omcSq, v3<T>& F1, v3<T>& F2, v3<T>& F3,
                                              v3<T>& F4) {
@@ -244,6 +260,7 @@ __device__ __forceinline__ void pair_generic(const SegK<T>& K, const Seg<T>& s1,
   const T cSq = c * c;
   T omcSqI;
   if constexpr (FAST) omcSqI = frcp(omcSq);
+  else if constexpr (RANGED) omcSqI = div_ranged(T(1), omcSq);
   else omcSqI = T(1) / omcSq;
 
   const v3<T> t2ct1 = cross(t2, t1);
@@ -277,14 +294,15 @@ __device__ __forceinline__ void pair_generic(const SegK<T>& K, const Seg<T>& s1,
   const T aSq_dSq = K.aSq + d * d * omcSq;
   T den;
   if constexpr (FAST) den = frcp(sqrt(omcSq * aSq_dSq));
+  else if constexpr (RANGED) den = div_ranged(T(1), sqrt_ranged(omcSq * aSq_dSq));
   else den = T(1) / sqrt(omcSq * aSq_dSq);
   T aSq_dSqI = T(0);
   if constexpr (FAST) aSq_dSqI = frcp(aSq_dSq);
   Integ19<T> I;
-  corner_acc<T, FAST>(aSq_dSq, aSq_dSqI, den, c, cSq, omcSqI, x1, y1, T(1), true, I);
-  corner_acc<T, FAST>(aSq_dSq, aSq_dSqI, den, c, cSq, omcSqI, x1, y2, T(-1), false, I);
-  corner_acc<T, FAST>(aSq_dSq, aSq_dSqI, den, c, cSq, omcSqI, x2, y1, T(-1), false, I);
-  corner_acc<T, FAST>(aSq_dSq, aSq_dSqI, den, c, cSq, omcSqI, x2, y2, T(1), false, I);
+  corner_acc<T, FAST, RANGED>(aSq_dSq, aSq_dSqI, den, c, cSq, omcSqI, x1, y1, T(1), true, I);
+  corner_acc<T, FAST, RANGED>(aSq_dSq, aSq_dSqI, den, c, cSq, omcSqI, x1, y2, T(-1), false, I);
+  corner_acc<T, FAST, RANGED>(aSq_dSq, aSq_dSqI, den, c, cSq, omcSqI, x2, y1, T(-1), false, I);
+  corner_acc<T, FAST, RANGED>(aSq_dSq, aSq_dSqI, den, c, cSq, omcSqI, x2, y2, T(1), false, I);
   const T* J = I.v - 1;  // 1-based, as the formulas are written
 
   if (W12) {  // segment 1, nodes 1-2: :953-1048

@@ -85,6 +85,25 @@ __global__ __launch_bounds__(kTile) void k_special_eval(const double* __restrict
 
 __global__ void k_zero_count(unsigned long long* c) { *c = 0; }
 
+// The tiles k_segseg_tiles<0> left undone, with the general IEEE sequences:
+// grid-stride over the list (its length is read on the device).
+__global__ __launch_bounds__(kTile, 2) void k_segseg_tiles_redo(const double* __restrict__ seg,
+                                                               int64_t ld, int n, SegK<double> K,
+                                                               double* __restrict__ rowPart,
+                                                               double* __restrict__ colPart,
+                                                               const long long* __restrict__ specOff,
+                                                               const unsigned short* __restrict__ specSlot,
+                                                               const double* __restrict__ specRes,
+                                                               int64_t specTotal, int64_t partBase,
+                                                               int64_t specBase,
+                                                               const int* __restrict__ redo,
+                                                               const unsigned long long* __restrict__ redoCount) {
+  const int64_t cnt = (int64_t)*redoCount;
+  for (int64_t q = blockIdx.x; q < cnt; q += gridDim.x)
+    (void)tile_eval<0, false>(redo[q], seg, ld, n, K, SigK<float>{}, 0.0, 0.0, rowPart, colPart,
+                              specOff, specSlot, specRes, specTotal, partBase, specBase, nullptr);
+}
+
 // Pass B for the listed pairs: their corrective sub-pairs, added to their
 // forces in res.  Grid-stride over the list (its length is on the device).
 __global__ __launch_bounds__(kTile) void k_special_fix(const double* __restrict__ seg, int64_t ld,
@@ -331,19 +350,26 @@ void launch_special_list(const double* seg, int64_t ld, int n, const MixedRule* 
 
 void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
                          double* rowPart, double* colPart, const SpecialList& sp,
-                         const TileRange& r, hipStream_t st, const MixedRule* mixed,
+                         const TileRange& r, hipStream_t st, int* redo,
+                         unsigned long long* redoCount, const MixedRule* mixed,
                          unsigned long long* nExact) {
   if (mixed) {
     launch_segseg_tiles_mixed(seg, ld, n, K, rowPart, colPart, sp, r, st, *mixed, nExact);
     return;
   }
+  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, st, redoCount);
   const int64_t maxGrid = 1ll << 30;
   for (int64_t a = r.t0(); a < r.t1(); a += maxGrid) {
     const int64_t cnt = (r.t1() - a < maxGrid) ? (r.t1() - a) : maxGrid;
     hipLaunchKernelGGL(k_segseg_tiles<0>, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, a, K,
                        SigK<float>{}, 0.0, 0.0, rowPart, colPart, sp.off, sp.slot, sp.res, sp.total,
-                       r.t0(), sp.base, nExact);
+                       r.t0(), sp.base, nullptr, redo, redoCount);
   }
+  // two workgroups per SIMD; with an empty list each reads the count and ends
+  const int64_t blocks = std::min<int64_t>(r.t1() - r.t0(), 2048);
+  hipLaunchKernelGGL(k_segseg_tiles_redo, dim3((unsigned)blocks), dim3(kTile), 0, st, seg, ld, n, K,
+                     rowPart, colPart, sp.off, sp.slot, sp.res, sp.total, r.t0(), sp.base, redo,
+                     redoCount);
 }
 
 void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, const TileRange& r,

@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kTile) void k_special_pass(const double* __restrict
   if (!LIST && lane == 0) cnt[t] = (unsigned)c;
 }
 
-// Pair-tile kernel.  MODE 0 (plain FP64): every generic pair runs inline in
+// One pair tile.  MODE 0 (plain FP64): every generic pair runs inline in
 // FP64.  MODE 1 (mixed precision): well-separated pairs (midpoint distance >
 // farDist and |t1 x t2| > sinMin) run inline in FP32 by quadrature
 // (far_side); the other generic pairs are queued and evaluated in FP64 64 at
@@ -131,24 +131,27 @@ __global__ __launch_bounds__(kTile) void k_special_pass(const double* __restrict
 // times the cost of a generic pair) are not evaluated here: they were listed
 // per tile and evaluated beforehand (k_special_pass, k_special_eval), and
 // the tile adds its list's forces, in list order, after its own pairs.
-// Keeping that branch out of this kernel lets it run two wavefronts per
-// SIMD.  The number of FP64 pairs is added to *nExact (mixed mode).  Tile
-// t's partial sums go to slot t - partBase, its list entries are read at
+// Keeping that branch out of the tile lets it run two wavefronts per SIMD.
+// The number of FP64 pairs is added to *nExact (mixed mode).  Tile t's
+// partial sums go to slot t - partBase, its list entries are read at
 // e - specBase (the chunk's first tile and first entry).
-template <int MODE>
-__global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restrict__ seg,
-                                                           int64_t ld, int n, int64_t tile0,
-                                                           SegK<double> K, SigK<float> KF,
-                                                           double farDist2, double sinMin2,
-                                                           double* __restrict__ rowPart,
-                                                           double* __restrict__ colPart,
-                                                           const long long* __restrict__ specOff,
-                                                           const unsigned short* __restrict__ specSlot,
-                                                           const double* __restrict__ specRes,
-                                                           int64_t specTotal, int64_t partBase,
-                                                           int64_t specBase,
-                                                           unsigned long long* __restrict__ nExact) {
-  const int64_t t = tile0 + blockIdx.x;
+//
+// RANGED (MODE 0): the generic pairs through the ranged sequences of
+// fp64_ranged.hpp.  The tile is done only when all 128 segments are in
+// their safe range and every partial sum comes out finite; otherwise it
+// returns false having written nothing, and the caller has it evaluated
+// with the general sequences (the same bits wherever both finish).
+template <int MODE, bool RANGED>
+__device__ __forceinline__ bool tile_eval(int64_t t, const double* __restrict__ seg, int64_t ld,
+                                          int n, const SegK<double>& K, const SigK<float>& KF,
+                                          double farDist2, double sinMin2,
+                                          double* __restrict__ rowPart,
+                                          double* __restrict__ colPart,
+                                          const long long* __restrict__ specOff,
+                                          const unsigned short* __restrict__ specSlot,
+                                          const double* __restrict__ specRes, int64_t specTotal,
+                                          int64_t partBase, int64_t specBase,
+                                          unsigned long long* __restrict__ nExact) {
   int I, J;
   tile_blocks(t, I, J);
   const int lane = threadIdx.x;
@@ -159,6 +162,7 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
   __shared__ double sres[12][kTile];
   __shared__ unsigned short queue[kQueue];
 
+  __syncthreads();  // a previous tile of this workgroup is done with the arrays
   const int64_t i = (int64_t)I * kTile + lane;
   const int64_t jl = (int64_t)J * kTile + lane;
 #pragma unroll
@@ -169,6 +173,15 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
 #pragma unroll
   for (int c = 0; c < 6; ++c) cacc[c][lane] = 0.0;
   __syncthreads();
+
+  if constexpr (RANGED) {
+    const int rf[9] = {F_N1X, F_N1Y, F_N1Z, F_N2X, F_N2Y, F_N2Z, F_BX, F_BY, F_BZ};
+    bool ok = true;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) ok = ok && ranged_value(si[rf[f]][lane]) && ranged_value(sj[rf[f]][lane]);
+    const bool aOk = K.aSq >= 0x1p-120 && K.aSq <= 0x1p120;
+    if (__ballot(!ok) != 0 || !aOk) return false;
+  }
 
   const Seg<double> A = seg_from_lds(si, lane);
   const bool rowLive = (i < n) && si[F_EXT][lane] == 0.0;
@@ -249,7 +262,7 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
     }
     if (MODE == 0 && inlineFP64) {
       v3<double> F1, F2, F3, F4;
-      pair_generic<double, true, true>(K, A, B, c, omcSq, F1, F2, F3, F4);
+      pair_generic<double, true, true, false, RANGED>(K, A, B, c, omcSq, F1, F2, F3, F4);
       a1 = a1 + F1;
       a2 = a2 + F2;
       cacc[0][s] += F3.x; cacc[1][s] += F3.y; cacc[2][s] += F3.z;
@@ -268,6 +281,14 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
   }
   if (MODE == 1 && nq > 0) flush(nq);
   if (MODE == 1 && lane == 0 && exact) atomicAdd(nExact, (unsigned long long)exact);
+  if constexpr (RANGED) {
+    // inf or NaN anywhere in a row or column sum (a sum overflowing to inf
+    // only costs a second evaluation)
+    double sum = a1.x + a1.y + a1.z + a2.x + a2.y + a2.z;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sum = sum + cacc[c][lane];
+    if (__ballot(!(sum - sum == 0.0)) != 0) return false;
+  }
   // this tile's near-parallel pairs, evaluated beforehand, in list order
   for (long long e = specOff[t] - specBase, e1 = specOff[t + 1] - specBase; e < e1; ++e) {
     const unsigned q = specSlot[e];
@@ -288,6 +309,33 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restr
   rp[3 * kTile + lane] = a2.x; rp[4 * kTile + lane] = a2.y; rp[5 * kTile + lane] = a2.z;
 #pragma unroll
   for (int c = 0; c < 6; ++c) cp[c * kTile + lane] = cacc[c][lane];
+  return true;
+}
+
+// Pair-tile kernel, one tile per workgroup (tile0 + blockIdx.x).  MODE 0
+// takes the ranged sequences and lists the tiles they leave undone in
+// redo[0 .. *redoCount) (slot order immaterial: each tile owns its
+// partial-sum slot), for k_segseg_tiles_redo; MODE 1 (mixed) does every tile.
+template <int MODE>
+__global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restrict__ seg,
+                                                           int64_t ld, int n, int64_t tile0,
+                                                           SegK<double> K, SigK<float> KF,
+                                                           double farDist2, double sinMin2,
+                                                           double* __restrict__ rowPart,
+                                                           double* __restrict__ colPart,
+                                                           const long long* __restrict__ specOff,
+                                                           const unsigned short* __restrict__ specSlot,
+                                                           const double* __restrict__ specRes,
+                                                           int64_t specTotal, int64_t partBase,
+                                                           int64_t specBase,
+                                                           unsigned long long* __restrict__ nExact,
+                                                           int* __restrict__ redo,
+                                                           unsigned long long* __restrict__ redoCount) {
+  const int64_t t = tile0 + blockIdx.x;
+  const bool done = tile_eval<MODE, MODE == 0>(t, seg, ld, n, K, KF, farDist2, sinMin2, rowPart,
+                                               colPart, specOff, specSlot, specRes, specTotal,
+                                               partBase, specBase, nExact);
+  if (MODE == 0 && !done && threadIdx.x == 0) redo[atomicAdd(redoCount, 1ull)] = (int)t;
 }
 
 }  // namespace

@@ -280,6 +280,33 @@ def test_degenerate_inputs(prm):
         assert per_node_rel(g, O.calcSegSegForce(dln, mat, net))[0] <= TOL
 
 
+def test_tiles_outside_the_ranged_range(prm):
+    """The tile kernel's ranged division and square root apply only where
+    every coordinate and Burgers component is zero or of magnitude in
+    [2^-300, 2^60] and a^2 lies in [2^-120, 2^120]; other tiles go to the
+    general IEEE sequences.  Parity with the oracle when some tiles (one loop
+    moved to x + 2^61, a node coordinate of 1e-200) or all tiles (the whole
+    network and the core radius scaled by 2^62) fall outside."""
+    from dddhip.types import DislocationParameters
+    dln, mat = prm
+    for what in ("one loop at 2^61", "a coordinate of 1e-200", "all scaled by 2^62"):
+        net = random_loop_network(8, seed=0xAB)
+        d = dln
+        if what == "one loop at 2^61":
+            nodes = np.unique(net.segIdx[:32, 1:].ravel()) - 1
+            net.coord[0, nodes] += 2.0 ** 61
+        elif what == "a coordinate of 1e-200":
+            net.coord[2, net.segIdx[40, 1] - 1] = 1e-200
+        else:
+            net.coord[:, :] *= 2.0 ** 62
+            d = DislocationParameters(coreRad=dln.coreRad * 2.0 ** 62, coreEnergy=dln.coreEnergy)
+        _log(what)
+        g = G.calcSegSegForce(d, mat, net)
+        o = O.calcSegSegForce(d, mat, net)
+        assert np.all(np.isfinite(o))
+        assert per_node_rel(g, o)[0] <= TOL
+
+
 def test_sigma_tilde_random(prm):
     dln, mat = prm
     net = random_loop_network(20, seed=0xA6)
