@@ -199,7 +199,12 @@ constexpr int kSigBlock = 256;
 constexpr int kSigTile = 64;
 
 // Stress of segments [c * chunk, (c + 1) * chunk) at one point per thread,
-// summed in segment order (the reference's order for each point).
+// summed in segment order (the reference's order for each point).  Each
+// staged 64-segment tile is evaluated with the ranged division and square
+// root of fp64_ranged.hpp when the workgroup's points, the tile's end points
+// and Burgers vectors and a^2 are in their safe range (the same bits), else
+// with the general IEEE sequences; a workgroup with a non-finite sum
+// evaluates its whole chunk again with the general ones.
 __global__ __launch_bounds__(kSigBlock) void k_sigma_tilde(const double* __restrict__ sseg,
                                                            int64_t sld, int nseg, int chunk,
                                                            const double* __restrict__ x0,
@@ -212,15 +217,12 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_tilde(const double* __restr
   const int s1 = min(nseg, s0 + chunk);
   const bool live = p < P;
   const v3<double> x = live ? V(x0[3 * p], x0[3 * p + 1], x0[3 * p + 2]) : V(0.0, 0.0, 0.0);
+  const bool pointOk = ranged_value(x.x) && ranged_value(x.y) && ranged_value(x.z);
+  const bool aOk = K.a2 >= 0x1p-120 && K.a2 <= 0x1p120;
   double sg[6] = {0, 0, 0, 0, 0, 0};
-  for (int base = s0; base < s1; base += kSigTile) {
-    const int cnt = min(kSigTile, s1 - base);
-    __syncthreads();
-    for (int q = threadIdx.x; q < S_COUNT * kSigTile; q += kSigBlock) {
-      const int f = q / kSigTile, l = q % kSigTile;
-      sh[f][l] = (l < cnt) ? sseg[f * sld + base + l] : 0.0;
-    }
-    __syncthreads();
+
+  // the stress of staged segments [0, cnt) at x, added to sg
+  auto tile = [&]<bool RANGED>(int cnt) {
     for (int l = 0; l < cnt; ++l) {
       const v3<double> n1 = V(sh[S_N1][l], sh[S_N1 + 1][l], sh[S_N1 + 2][l]);
       const v3<double> n2 = V(sh[S_N2][l], sh[S_N2 + 1][l], sh[S_N2 + 2][l]);
@@ -234,9 +236,16 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_tilde(const double* __restr
       const double sa = -Rdt;
       const double sb = -dot(x - n2, t);
       const double a2_d2 = K.a2 + d2;
-      const double a2d2inv = 1.0 / a2_d2;
-      double Ra = sqrt(a2_d2 + sa * sa);
-      double Rainv = 1.0 / Ra;
+      double a2d2inv, Ra, Rainv;
+      if constexpr (RANGED) {
+        a2d2inv = div_ranged(1.0, a2_d2);
+        Ra = sqrt_ranged(a2_d2 + sa * sa);
+        Rainv = div_ranged(1.0, Ra);
+      } else {
+        a2d2inv = 1.0 / a2_d2;
+        Ra = sqrt(a2_d2 + sa * sa);
+        Rainv = 1.0 / Ra;
+      }
       double Ra3inv = Rainv * Rainv * Rainv;
       double sRa3inv = sa * Ra3inv;
       const double s03a = sa * Rainv * a2d2inv;
@@ -244,8 +253,13 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_tilde(const double* __restr
       const double s05a = (2.0 * s03a + sRa3inv) * a2d2inv;
       const double s15a = -Ra3inv;
       const double s25a = s03a - sRa3inv;
-      Ra = sqrt(a2_d2 + sb * sb);
-      Rainv = 1.0 / Ra;
+      if constexpr (RANGED) {
+        Ra = sqrt_ranged(a2_d2 + sb * sb);
+        Rainv = div_ranged(1.0, Ra);
+      } else {
+        Ra = sqrt(a2_d2 + sb * sb);
+        Rainv = 1.0 / Ra;
+      }
       Ra3inv = Rainv * Rainv * Rainv;
       sRa3inv = sb * Ra3inv;
       const double s03b = sb * Rainv * a2d2inv;
@@ -276,6 +290,34 @@ __global__ __launch_bounds__(kSigBlock) void k_sigma_tilde(const double* __restr
         sg[q] += I03 * s03 + I13 * s13 + I05 * s05 + I15 * s15 + I25 * s25;
       }
     }
+  };
+
+  // every tile of the chunk; ranged where allowed (and not forced off)
+  auto sweep = [&](bool allowRanged) {
+    for (int base = s0; base < s1; base += kSigTile) {
+      const int cnt = min(kSigTile, s1 - base);
+      __syncthreads();
+      bool bad = !pointOk;
+      for (int q = threadIdx.x; q < S_COUNT * kSigTile; q += kSigBlock) {
+        const int f = q / kSigTile, l = q % kSigTile;
+        const double v = (l < cnt) ? sseg[f * sld + base + l] : 0.0;
+        sh[f][l] = v;
+        if (f < S_T) bad = bad || !ranged_value(v);  // n1, n2, b
+      }
+      const bool tileBad = __syncthreads_or(bad);
+      if (allowRanged && aOk && !tileBad) tile.template operator()<true>(cnt);
+      else tile.template operator()<false>(cnt);
+    }
+  };
+
+  sweep(true);
+  double tot = 0.0;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) tot = tot + sg[q];
+  if (__syncthreads_or(!(tot - tot == 0.0))) {  // inf or NaN in a workgroup's sum
+#pragma unroll
+    for (int q = 0; q < 6; ++q) sg[q] = 0.0;
+    sweep(false);
   }
   if (live) {
     double* o = partial + ((int64_t)c * P + p) * 6;
