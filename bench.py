@@ -26,11 +26,12 @@ METRIC = "Force-eval time @131k segs: plain FP64 vs hybrid at ≤1e-3 RMS rel fo
 FLOPS_PER_PAIR = 1466  # counted by the operation-counting oracle (generic pair)
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the pair-tile kernel at 4,096 loops x 32 segments
-# (one chunk): FETCH_SIZE 13,052,754 KiB x 2 (the counter reads 524,308 KiB
-# for a 1 GiB copy) + WRITE_SIZE 14,568,639 KiB x 1 (1,048,576 KiB for the
-# same copy).  Most of it is register-spill traffic of the FP64 pair
-# evaluation (20 B/lane scratch) and the 12 x 8 B tile partials per lane.
-TILE_TRAFFIC_BYTES = 13_052_754 * 2 * 1024 + 14_568_639 * 1024
+# (one chunk): FETCH_SIZE 12,786,525 KiB x 2 (the counter reads 524,308 KiB
+# for a 1 GiB copy) + WRITE_SIZE 12,589,056 KiB x 1 (1,048,576 KiB for the
+# same copy).  The writes are the tile partials exactly (2,098,176 tiles x
+# 6 KiB); the reads are the two 64-segment blocks of every tile (7 KiB each,
+# most fetched again per tile) and the near-parallel forces.
+TILE_TRAFFIC_BYTES = 12_786_525 * 2 * 1024 + 12_589_056 * 1024
 
 
 def log(msg):
