@@ -574,6 +574,55 @@ __device__ __forceinline__ void seg_stress(const SigK<T>& K, v3<T> rx, v3<T> d, 
   }
 }
 
+// The stress of seg_stress at rx (relative to p) applied to a vector ba,
+// sigma(rx) ba, without forming the six components: with the segment's
+// length L = d.t, its field regrouped as
+//   sigma = delta cD + t (x) u + u (x) t + n (x) v + v (x) n
+// (n the rejection of rx from the line, u and v vectors of the point), whose
+// product with ba takes four dot products.  The same mathematics as
+// seg_stress in about half the operations, rounded otherwise: for the
+// approximate paths (far_side).
+template <class T>
+__device__ __forceinline__ v3<T> seg_stress_times(const SigK<T>& K, v3<T> rx, T L, v3<T> t, v3<T> b,
+                                                  v3<T> txb, T tba, v3<T> ba) {
+  const T Rdt = dot(rx, t);
+  const v3<T> nd = rx - Rdt * t;
+  const T d2 = dot(nd, nd);
+  const T s1 = -Rdt;
+  const T s2 = L - Rdt;
+  const T a2_d2 = K.a2 + d2;
+  const T a2d2inv = T(1) / a2_d2;
+  T Rainv = T(1) / sqrt(a2_d2 + s1 * s1);
+  T Ra3inv = Rainv * Rainv * Rainv;
+  T sRa3inv = s1 * Ra3inv;
+  const T s03a = s1 * Rainv * a2d2inv;
+  const T s13a = -Rainv;
+  const T s05a = (T(2) * s03a + sRa3inv) * a2d2inv;
+  const T s15a = -Ra3inv;
+  const T s25a = s03a - sRa3inv;
+  Rainv = T(1) / sqrt(a2_d2 + s2 * s2);
+  Ra3inv = Rainv * Rainv * Rainv;
+  sRa3inv = s2 * Ra3inv;
+  const T s03b = s2 * Rainv * a2d2inv;
+  const T s05b = (T(2) * s03b + sRa3inv) * a2d2inv;
+  const T s03 = s03b - s03a, s13 = -Rainv - s13a, s05 = s05b - s05a, s15 = -Ra3inv - s15a,
+          s25 = (s03b - sRa3inv) - s25a;
+  const v3<T> dxb = cross(nd, b);
+  const T common = K.mu4pinu * dot(dxb, t);
+  // coefficients of delta, n(x)n, n(x)txb, t(x)dxb, t(x)txb, t(x)n, t(x)t in
+  // the integrals I03 s03 + I13 s13 + I05 s05 + I15 s15 + I25 s25
+  const T cD = common * (s03 + K.a2 * s05);
+  const T cnn = common * s05;
+  const T cnx = K.mu4pinu * s03;
+  const T cty = -(K.mu4pi * s03 + K.a2mu8pi * s05);
+  const T ctx = K.a2mu8pi * s15 - K.nuMu4pinu * s13;
+  const T ctn = -common * s15;
+  const T ctt = common * s25;
+  const v3<T> u = cty * dxb + ctx * txb + ctn * nd + (T(0.5) * ctt) * t;
+  const v3<T> v = (T(0.5) * cnn) * nd + cnx * txb;
+  return cD * ba + dot(u, ba) * t + tba * u + dot(v, ba) * nd + dot(nd, ba) * v;
+}
+
 // Far-pair nodal forces by two-point Gauss-Legendre quadrature (no reference
 // counterpart; the mixed path's formulation for well-separated pairs): on
 // segment A, F_A1 = L_A sum_q w_q (1 - u_q) (sigma_B(x_q) b_A) x t_A and
@@ -592,22 +641,17 @@ __device__ __forceinline__ void far_side(const SigK<T>& K, const Seg<double>& A,
   const v3<T> tB = V(T(B.t.x), T(B.t.y), T(B.t.z));
   const v3<T> bB = V(T(B.b.x), T(B.b.y), T(B.b.z));
   const v3<T> txb = cross(tB, bB);
-  T sA[6] = {0, 0, 0, 0, 0, 0}, sC[6] = {0, 0, 0, 0, 0, 0};
-  seg_stress(K, r0 + g0 * dA, dB, tB, bB, txb, sA);
-  seg_stress(K, r0 + g1 * dA, dB, tB, bB, txb, sC);
   const v3<T> ba = V(T(A.b.x), T(A.b.y), T(A.b.z));
   const v3<T> tA = V(T(A.t.x), T(A.t.y), T(A.t.z));
+  const T LB = dot(dB, tB);
+  const T tba = dot(tB, ba);
+  // (sigma b) at the two Gauss points
+  const v3<T> sa = seg_stress_times(K, r0 + g0 * dA, LB, tB, bB, txb, tba, ba);
+  const v3<T> sc = seg_stress_times(K, r0 + g1 * dA, LB, tB, bB, txb, tba, ba);
   // (sigma b) x t L w, w = 1/2 per node of the two-point rule
   const T hw = T(0.5) / T(A.tN);
-  auto pk = [&](const T* s) {
-    const v3<T> sb = V(s[0] * ba.x + s[3] * ba.y + s[4] * ba.z,
-                       s[3] * ba.x + s[1] * ba.y + s[5] * ba.z,
-                       s[4] * ba.x + s[5] * ba.y + s[2] * ba.z);
-    return cross(sb, tA) * hw;
-  };
-  const v3<T> fa = pk(sA), fc = pk(sC);
-  const v3<T> f1 = fa * g1 + fc * g0;  // weight (1 - u)
-  const v3<T> f2 = fa * g0 + fc * g1;  // weight u
+  const v3<T> f1 = cross(sa * g1 + sc * g0, tA) * hw;  // weight (1 - u)
+  const v3<T> f2 = cross(sa * g0 + sc * g1, tA) * hw;  // weight u
   FA1 = V(double(f1.x), double(f1.y), double(f1.z));
   FA2 = V(double(f2.x), double(f2.y), double(f2.z));
 }
