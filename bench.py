@@ -534,6 +534,8 @@ def profile_run(args, dln, mat, log):
     counter pass each): a plain 1 GiB device copy (1 GiB read + 1 GiB
     written: the scale of the FETCH_SIZE / WRITE_SIZE counters), then
       fp64:      one plain FP64 evaluation of the 131,072-segment network;
+      mixed:     one mixed-precision evaluation of it (the rule of the
+                 mixed path: FP32 beyond 5 core radii and |t1 x t2| > 1e-3);
       hybrid:    two hybrid evaluations of it (G = 32, rn = 4, the last
                  --near-fp64-radii rule, a network trained for --train-steps
                  steps: kernel times do not depend on the training length);
@@ -563,9 +565,11 @@ def profile_run(args, dln, mat, log):
     what = args.profile
     out = {"profile": what}
     lo, side = -300.0, 2600.0
-    if what in ("fp64", "hybrid"):
+    if what in ("fp64", "hybrid", "mixed"):
         net = random_loop_network(args.loops, seed=args.seed)
         res = G.ResidentNetwork(dln, mat, net, ctx=ctx)
+        if what == "mixed":
+            res.set_mode("mixed")
         if what == "hybrid":
             ff = G.FarField(32, (lo, lo, lo), side / 32, 4, ctx=ctx)
             mlp = G.FarFieldMLP(seed=11, ctx=ctx)
@@ -673,7 +677,7 @@ def main():
                     help="share of the clustered leg's training sources drawn from the shells "
                          "next to the near stencil (0: the library default, 0.5); its error "
                          "comes from long-range interactions, so fewer near samples help")
-    ap.add_argument("--profile", choices=("fp64", "hybrid", "clustered", "sigma"),
+    ap.add_argument("--profile", choices=("fp64", "mixed", "hybrid", "clustered", "sigma"),
                     help="run one launch of a workload for a rocprofv3 run and exit")
     ap.add_argument("--no-clustered", dest="clustered", action="store_false",
                     help="skip the clustered stress test (262,144 segments, 70%% in 8 "

@@ -156,20 +156,27 @@ __device__ __forceinline__ bool tile_eval(int64_t t, const double* __restrict__ 
   tile_blocks(t, I, J);
   const int lane = threadIdx.x;
 
-  __shared__ double si[F_COUNT][kTile];
-  __shared__ double sj[F_COUNT][kTile];
+  // Staged fields: n1, n2, b, t, 1/L (the external flags are kept as
+  // ballot masks in registers); six doubles per lane of pair results in
+  // the mixed mode's flush.  With at most 20 KiB of LDS a workgroup, eight
+  // fit on a CU (two wavefronts per SIMD).
+  constexpr int NF = F_EXT;
+  __shared__ double si[NF][kTile];
+  __shared__ double sj[NF][kTile];
   __shared__ double cacc[6][kTile];
-  __shared__ double sres[12][kTile];
+  __shared__ double sres[6][kTile];
   __shared__ unsigned short queue[kQueue];
 
   __syncthreads();  // a previous tile of this workgroup is done with the arrays
   const int64_t i = (int64_t)I * kTile + lane;
   const int64_t jl = (int64_t)J * kTile + lane;
 #pragma unroll
-  for (int f = 0; f < F_COUNT; ++f) {
+  for (int f = 0; f < NF; ++f) {
     si[f][lane] = seg[f * ld + i];
     sj[f][lane] = seg[f * ld + jl];
   }
+  const bool rowExt = seg[F_EXT * ld + i] != 0.0;
+  const uint64_t colExt = __ballot(seg[F_EXT * ld + jl] != 0.0);
 #pragma unroll
   for (int c = 0; c < 6; ++c) cacc[c][lane] = 0.0;
   __syncthreads();
@@ -184,7 +191,7 @@ __device__ __forceinline__ bool tile_eval(int64_t t, const double* __restrict__ 
   }
 
   const Seg<double> A = seg_from_lds(si, lane);
-  const bool rowLive = (i < n) && si[F_EXT][lane] == 0.0;
+  const bool rowLive = (i < n) && !rowExt;
   const bool diag = (I == J);
   v3<double> a1 = V(0.0, 0.0, 0.0), a2 = V(0.0, 0.0, 0.0);
   int nq = 0;        // wave-uniform queue length (mixed mode)
@@ -194,13 +201,14 @@ __device__ __forceinline__ bool tile_eval(int64_t t, const double* __restrict__ 
   const v3<double> midA = mid_rn(A.n1, A.n2);
 
   // Mixed mode: evaluate the first cnt queued generic pairs in FP64 and add
-  // them to rows and columns in queue order.
+  // them to rows and columns in queue order (row forces through LDS first,
+  // then column forces).
   auto flush = [&](int cnt) {
+    v3<double> F1 = V(0.0, 0.0, 0.0), F2 = F1, F3 = F1, F4 = F1;
     if (lane < cnt) {
       const unsigned e = queue[lane];
       const int rl = e & 63, sl = (rl + (e >> 6)) & 63;
       const Seg<double> P = seg_from_lds(si, rl), Q = seg_from_lds(sj, sl);
-      v3<double> F1, F2, F3, F4;
       double c;
       const double omc = one_minus_c2(P.t, Q.t, c);
       // the queue is used in mixed mode only: an approximate path, so the
@@ -208,20 +216,27 @@ __device__ __forceinline__ bool tile_eval(int64_t t, const double* __restrict__ 
       pair_generic<double, true, true, true>(K, P, Q, c, omc, F1, F2, F3, F4);
       sres[0][lane] = F1.x; sres[1][lane] = F1.y; sres[2][lane] = F1.z;
       sres[3][lane] = F2.x; sres[4][lane] = F2.y; sres[5][lane] = F2.z;
-      sres[6][lane] = F3.x; sres[7][lane] = F3.y; sres[8][lane] = F3.z;
-      sres[9][lane] = F4.x; sres[10][lane] = F4.y; sres[11][lane] = F4.z;
     }
     __syncthreads();
     for (int e = 0; e < cnt; ++e) {  // queue order: deterministic
-      const unsigned q = queue[e];
-      const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
+      const int rl = queue[e] & 63;
       if (rl == lane) {
         a1 = a1 + V(sres[0][e], sres[1][e], sres[2][e]);
         a2 = a2 + V(sres[3][e], sres[4][e], sres[5][e]);
       }
+    }
+    __syncthreads();
+    if (lane < cnt) {
+      sres[0][lane] = F3.x; sres[1][lane] = F3.y; sres[2][lane] = F3.z;
+      sres[3][lane] = F4.x; sres[4][lane] = F4.y; sres[5][lane] = F4.z;
+    }
+    __syncthreads();
+    for (int e = 0; e < cnt; ++e) {
+      const unsigned q = queue[e];
+      const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
       if (sl == lane) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) cacc[c][lane] += sres[6 + c][e];
+        for (int c = 0; c < 6; ++c) cacc[c][lane] += sres[c][e];
       }
     }
     __syncthreads();
@@ -230,7 +245,7 @@ __device__ __forceinline__ bool tile_eval(int64_t t, const double* __restrict__ 
   for (int k = 0; k < kTile; ++k) {
     const int s = (lane + k) & 63;
     const int64_t j = (int64_t)J * kTile + s;
-    const bool live = rowLive && (j < n) && sj[F_EXT][s] == 0.0 && (!diag || s > lane);
+    const bool live = rowLive && (j < n) && !((colExt >> s) & 1ull) && (!diag || s > lane);
     const Seg<double> B = seg_from_lds(sj, s);
     double c;
     const double omcSq = one_minus_c2(A.t, B.t, c);
