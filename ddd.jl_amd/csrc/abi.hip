@@ -149,7 +149,8 @@ struct ddd_ctx {
   DevBuf cConn, cLinks, cCoord, cVel, cB, cLabel, cSkip, cSeg, cNodes, cFlags, cBest, cLast, cRes;
   DevBuf cKeys, cKeysOut, cIdx, cPerm, cSortWork, cBox, cPrev;
   std::vector<double> h;  // host staging
-  DevBuf counter;  // FP64-pair counter of the mixed path
+  DevBuf counter;   // mixed path: tile counter, FP64-pair counter
+  DevBuf mixStage;  // mixed path: staging of the FP64 pairs per workgroup
   // near-parallel pairs of the all-pairs tiles (count and offsets over every
   // tile, the scan's work, offsets at the column-block boundaries, fix count)
   DevBuf specCnt, specOff, specWork, specAt, specFixN;
@@ -491,7 +492,7 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
   const int64_t nb = (n + kTile - 1) / kTile;
   const int64_t T = tiles_for(nb);
   if (T + 1 > INT32_MAX) bad("too many segments for the tile list");
-  c->counter.ensure(sizeof(unsigned long long));
+  c->counter.ensure(2 * sizeof(unsigned long long));
   c->specCnt.ensure(sizeof(unsigned) * T);
   c->specOff.ensure(sizeof(long long) * (T + 1));
   c->specAt.ensure(sizeof(long long) * (nb + 1));
@@ -524,7 +525,10 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
   c->chunkMem.ensure((size_t)most, (size_t)std::max(most, c->chunkBytes));
   c->specFixN.ensure(sizeof(unsigned long long));
   c->redoN.ensure(sizeof(unsigned long long));
-  if (mixed) HIPCHK(hipMemsetAsync(c->counter.p, 0, sizeof(unsigned long long), c->st));
+  if (mixed) {
+    HIPCHK(hipMemsetAsync(c->counter.p, 0, 2 * sizeof(unsigned long long), c->st));
+    c->mixStage.ensure(sizeof(unsigned short) * kMixedBlocks * kMixedStage);
+  }
   char* mem = c->chunkMem.as<char>();
   for (size_t q = 0; q < chunks.size(); ++q) {
     const TileRange& ch = chunks[q];
@@ -540,6 +544,7 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
     launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k), rowPart, colPart, sp,
                         ch, c->st, reinterpret_cast<int*>(mem + L.redo),
                         c->redoN.as<unsigned long long>(), mixed ? &r : nullptr,
+                        mixed ? c->mixStage.as<unsigned short>() : nullptr,
                         mixed ? c->counter.as<unsigned long long>() : nullptr);
     if (q + 1 < chunks.size()) {
       launch_segseg_reduce(rowPart, colPart, (int)n, ch, dev, c->st);
@@ -1536,7 +1541,7 @@ uint64_t ctx_device_bytes(ddd_ctx* c) {
                          &c->cConn, &c->cLinks, &c->cCoord, &c->cVel, &c->cB, &c->cLabel,
                          &c->cSkip, &c->cSeg, &c->cNodes, &c->cFlags, &c->cBest, &c->cLast,
                          &c->cRes, &c->cKeys, &c->cKeysOut, &c->cIdx, &c->cPerm, &c->cSortWork,
-                         &c->cBox, &c->cPrev, &c->counter, &c->specCnt, &c->specOff,
+                         &c->cBox, &c->cPrev, &c->counter, &c->mixStage, &c->specCnt, &c->specOff,
                          &c->specWork, &c->specAt, &c->specFixN, &c->redoN, &c->dCoord,
                          &c->dCoord0, &c->dVel, &c->dVel0, &c->dNodeF, &c->dLinks, &c->dConn,
                          &c->dBvec, &c->dLabel, &c->dSegNodes, &c->dRedPart, &c->dRed};
@@ -1561,7 +1566,8 @@ void fill_stats(ddd_ctx* c, int64_t n, const ddd_opts* o, ddd_stats* st) {
   }
   if (o && o->mode == DDD_MODE_MIXED && n > 1) {
     unsigned long long e = 0;
-    HIPCHK(hipMemcpyAsync(&e, c->counter.p, sizeof(e), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(&e, c->counter.as<unsigned long long>() + 1, sizeof(e),
+                          hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     st->pairsFP64 = e;
   }

@@ -81,14 +81,20 @@ void launch_exclusive_scan(const unsigned* cnt, int n, long long* off, long long
 // offAt[k] = off[tiles_for(k)] for k = 0..nb (the list offsets at column-block
 // boundaries, which size the chunks).
 void launch_block_offsets(const long long* off, int nb, long long* offAt, hipStream_t st);
+// Workgroups of the persistent mixed-precision tile kernel (two per SIMD)
+// and the uint16 staging entries each holds in HBM.
+constexpr int kMixedBlocks = 2048;
+constexpr int kMixedStage = kTile * kTile;
 // The pair tiles of range r into rowPart / colPart (slot t - r.t0()).  Plain
 // FP64: a ranged pass and a pass over the tiles it left undone, listed in
 // redo (capacity: the range's tile count) with their number in *redoCount.
+// Mixed: stage holds kMixedBlocks x kMixedStage entries, counters[0] is the
+// tile counter (zeroed here), counters[1] accumulates the FP64 pairs.
 void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double>& K,
                          double* rowPart, double* colPart, const SpecialList& sp,
                          const TileRange& r, hipStream_t st, int* redo,
                          unsigned long long* redoCount, const MixedRule* mixed = nullptr,
-                         unsigned long long* nExact = nullptr);
+                         unsigned short* stage = nullptr, unsigned long long* counters = nullptr);
 // The mixed-precision kernels' launches (mixed.hip; used by the two above).
 void launch_special_pass_mixed(const double* seg, int64_t ld, int n, const MixedRule& mr,
                                unsigned* cnt, const SpecialList* sp, int64_t t0, int64_t t1,
@@ -96,7 +102,7 @@ void launch_special_pass_mixed(const double* seg, int64_t ld, int n, const Mixed
 void launch_segseg_tiles_mixed(const double* seg, int64_t ld, int n, const SegK<double>& K,
                                double* rowPart, double* colPart, const SpecialList& sp,
                                const TileRange& r, hipStream_t st, const MixedRule& mr,
-                               unsigned long long* nExact);
+                               unsigned short* stage, unsigned long long* counters);
 // Adds each segment's partials of range r into out (6 x n): its row role in
 // tiles (B, J), J in [max(B, J0), J1), then its column role in tiles (I, B),
 // I <= B, if B lies in [J0, J1) -- the fixed order of every chunk.

@@ -10,6 +10,7 @@
 // many of the FP64 pair's instructions.
 #include <algorithm>
 
+#include "batch.hpp"
 #include "farfield.hpp"
 #include "pair_special.hpp"
 
@@ -78,70 +79,6 @@ __device__ __forceinline__ int near_class(const double (*si)[kTile], const doubl
   const double d2x4 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
   if (d2x4 > 4.0 * farDist2 && omcSq > sinMin2) return kFP32;
   return omcSq > DDD_SQRT_EPS ? kFP64 : kPar;
-}
-
-// Adds, in increasing position order, the LDS results sres[0..2][e] at the
-// set bits e of m.
-__device__ __forceinline__ void add_marked(const double (*sres)[kTile], uint64_t m, v3<double>& r) {
-  for (; m; m &= m - 1) {
-    const int e = __builtin_ctzll(m);
-    r = r + V(sres[0][e], sres[1][e], sres[2][e]);
-  }
-}
-__device__ __forceinline__ void add_marked_col(const double (*sres)[kTile], uint64_t m,
-                                               double (*cacc)[kTile], int c0, int lane) {
-  for (; m; m &= m - 1) {
-    const int e = __builtin_ctzll(m);
-    cacc[c0][lane] += sres[0][e];
-    cacc[c0 + 1][lane] += sres[1][e];
-    cacc[c0 + 2][lane] += sres[2][e];
-  }
-}
-
-// A batch of up to 64 pairs of a staged tile, one per lane (lane e holds
-// entry e: row slot e & 63, column slot (row + (e >> 6)) & 63); forces
-// F1..F4 from eval.  Each lane then adds, in entry order, the results whose
-// row (F1, F2 into a1, a2) or column (F3, F4 into cacc) it owns.  Ownership
-// masks are built with LDS atomic ORs; results pass through LDS one force at
-// a time.
-template <class Eval>
-__device__ __forceinline__ void near_batch(unsigned e, int cnt, int lane, Eval eval,
-                                           double (*sres)[kTile], double (*cacc)[kTile],
-                                           unsigned long long* rowM, unsigned long long* colM,
-                                           v3<double>& a1, v3<double>& a2) {
-  const int rl = e & 63, sl = (rl + (e >> 6)) & 63;
-  rowM[lane] = 0;
-  colM[lane] = 0;
-  __syncthreads();
-  if (lane < cnt) {
-    atomicOr(&rowM[rl], 1ull << lane);
-    atomicOr(&colM[sl], 1ull << lane);
-  }
-  __syncthreads();
-  const uint64_t mr = rowM[lane], mc = colM[lane];
-  v3<double> F1 = V(0.0, 0.0, 0.0), F2 = F1, F3 = F1, F4 = F1;
-  if (lane < cnt) eval(rl, sl, F1, F2, F3, F4);
-  auto put = [&](const v3<double>& f) {
-    sres[0][lane] = f.x;
-    sres[1][lane] = f.y;
-    sres[2][lane] = f.z;
-  };
-  put(F1);
-  __syncthreads();
-  add_marked(sres, mr, a1);
-  __syncthreads();
-  put(F2);
-  __syncthreads();
-  add_marked(sres, mr, a2);
-  __syncthreads();
-  put(F3);
-  __syncthreads();
-  add_marked_col(sres, mc, cacc, 0, lane);
-  __syncthreads();
-  put(F4);
-  __syncthreads();
-  add_marked_col(sres, mc, cacc, 3, lane);
-  __syncthreads();
 }
 
 // Near-field tiles of the hybrid path: tile t = (I, J) of the cell-sorted

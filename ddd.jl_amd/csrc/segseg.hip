@@ -85,7 +85,29 @@ __global__ __launch_bounds__(kTile) void k_special_eval(const double* __restrict
 
 __global__ void k_zero_count(unsigned long long* c) { *c = 0; }
 
-// The tiles k_segseg_tiles<0> left undone, with the general IEEE sequences:
+// Plain FP64 pair-tile kernel, one tile per workgroup (tile0 + blockIdx.x),
+// with the ranged sequences; the tiles they leave undone are listed in
+// redo[0 .. *redoCount) (slot order immaterial: each tile owns its
+// partial-sum slot) for k_segseg_tiles_redo.
+__global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restrict__ seg,
+                                                           int64_t ld, int n, int64_t tile0,
+                                                           SegK<double> K,
+                                                           double* __restrict__ rowPart,
+                                                           double* __restrict__ colPart,
+                                                           const long long* __restrict__ specOff,
+                                                           const unsigned short* __restrict__ specSlot,
+                                                           const double* __restrict__ specRes,
+                                                           int64_t specTotal, int64_t partBase,
+                                                           int64_t specBase,
+                                                           int* __restrict__ redo,
+                                                           unsigned long long* __restrict__ redoCount) {
+  const int64_t t = tile0 + blockIdx.x;
+  const bool done = tile_eval<true>(t, seg, ld, n, K, rowPart, colPart, specOff, specSlot, specRes,
+                                    specTotal, partBase, specBase);
+  if (!done && threadIdx.x == 0) redo[atomicAdd(redoCount, 1ull)] = (int)t;
+}
+
+// The tiles k_segseg_tiles left undone, with the general IEEE sequences:
 // grid-stride over the list (its length is read on the device).
 __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles_redo(const double* __restrict__ seg,
                                                                int64_t ld, int n, SegK<double> K,
@@ -100,8 +122,8 @@ __global__ __launch_bounds__(kTile, 2) void k_segseg_tiles_redo(const double* __
                                                                const unsigned long long* __restrict__ redoCount) {
   const int64_t cnt = (int64_t)*redoCount;
   for (int64_t q = blockIdx.x; q < cnt; q += gridDim.x)
-    (void)tile_eval<0, false>(redo[q], seg, ld, n, K, SigK<float>{}, 0.0, 0.0, rowPart, colPart,
-                              specOff, specSlot, specRes, specTotal, partBase, specBase, nullptr);
+    (void)tile_eval<false>(redo[q], seg, ld, n, K, rowPart, colPart, specOff, specSlot, specRes,
+                           specTotal, partBase, specBase);
 }
 
 // Pass B for the listed pairs: their corrective sub-pairs, added to their
@@ -352,18 +374,18 @@ void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double
                          double* rowPart, double* colPart, const SpecialList& sp,
                          const TileRange& r, hipStream_t st, int* redo,
                          unsigned long long* redoCount, const MixedRule* mixed,
-                         unsigned long long* nExact) {
+                         unsigned short* stage, unsigned long long* counters) {
   if (mixed) {
-    launch_segseg_tiles_mixed(seg, ld, n, K, rowPart, colPart, sp, r, st, *mixed, nExact);
+    launch_segseg_tiles_mixed(seg, ld, n, K, rowPart, colPart, sp, r, st, *mixed, stage, counters);
     return;
   }
   hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, st, redoCount);
   const int64_t maxGrid = 1ll << 30;
   for (int64_t a = r.t0(); a < r.t1(); a += maxGrid) {
     const int64_t cnt = (r.t1() - a < maxGrid) ? (r.t1() - a) : maxGrid;
-    hipLaunchKernelGGL(k_segseg_tiles<0>, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, a, K,
-                       SigK<float>{}, 0.0, 0.0, rowPart, colPart, sp.off, sp.slot, sp.res, sp.total,
-                       r.t0(), sp.base, nullptr, redo, redoCount);
+    hipLaunchKernelGGL(k_segseg_tiles, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, a, K,
+                       rowPart, colPart, sp.off, sp.slot, sp.res, sp.total, r.t0(), sp.base, redo,
+                       redoCount);
   }
   // two workgroups per SIMD; with an empty list each reads the count and ends
   const int64_t blocks = std::min<int64_t>(r.t1() - r.t0(), 2048);

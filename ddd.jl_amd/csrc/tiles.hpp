@@ -5,6 +5,7 @@
 // held to the FP32 quadrature's own accuracy).  Everything sits in an
 // unnamed namespace, so each translation unit has its own copy.
 #pragma once
+#include "batch.hpp"
 #include "kernels.hpp"
 #include "pair_special.hpp"
 
@@ -21,8 +22,6 @@ __device__ __forceinline__ Seg<double> seg_from_lds(const double (*s)[kTile], in
   r.tN = s[F_TN][l];
   return r;
 }
-
-constexpr int kQueue = 2 * kTile;
 
 // Tile t of the upper triangle, enumerated column-block major:
 // t = J (J + 1) / 2 + I with 0 <= I <= J.
@@ -122,61 +121,91 @@ __global__ __launch_bounds__(kTile) void k_special_pass(const double* __restrict
   if (!LIST && lane == 0) cnt[t] = (unsigned)c;
 }
 
-// One pair tile.  MODE 0 (plain FP64): every generic pair runs inline in
-// FP64.  MODE 1 (mixed precision): well-separated pairs (midpoint distance >
-// farDist and |t1 x t2| > sinMin) run inline in FP32 by quadrature
-// (far_side); the other generic pairs are queued and evaluated in FP64 64 at
-// a time, so the wavefront never waits on a minority path.  In both modes
-// near-parallel pairs (the reference's special branch, :1147-1162, several
+// Staged fields of the tile kernels: n1, n2, b, t, 1/L (external flags are
+// kept as ballot masks in registers).  With at most 20 KiB of LDS a
+// workgroup, eight fit on a CU (two wavefronts per SIMD).
+constexpr int kTileFields = F_EXT;
+
+__device__ __forceinline__ void stage_blocks(const double* __restrict__ seg, int64_t ld, int I,
+                                             int J, int lane, double (*si)[kTile],
+                                             double (*sj)[kTile], bool& rowExt,
+                                             uint64_t& colExt) {
+  const int64_t i = (int64_t)I * kTile + lane;
+  const int64_t jl = (int64_t)J * kTile + lane;
+#pragma unroll
+  for (int f = 0; f < kTileFields; ++f) {
+    si[f][lane] = seg[f * ld + i];
+    sj[f][lane] = seg[f * ld + jl];
+  }
+  rowExt = seg[F_EXT * ld + i] != 0.0;
+  colExt = __ballot(seg[F_EXT * ld + jl] != 0.0);
+}
+
+// Adds tile t's near-parallel pairs, evaluated beforehand, in list order,
+// and writes the tile's partial sums (slot t - partBase).
+__device__ __forceinline__ void tile_finish(int64_t t, int lane, v3<double> a1, v3<double> a2,
+                                            double (*cacc)[kTile],
+                                            double* __restrict__ rowPart,
+                                            double* __restrict__ colPart,
+                                            const long long* __restrict__ specOff,
+                                            const unsigned short* __restrict__ specSlot,
+                                            const double* __restrict__ specRes, int64_t specTotal,
+                                            int64_t partBase, int64_t specBase) {
+  for (long long e = specOff[t] - specBase, e1 = specOff[t + 1] - specBase; e < e1; ++e) {
+    const unsigned q = specSlot[e];
+    const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
+    if (rl == lane) {
+      a1 = a1 + V(specRes[0 * specTotal + e], specRes[1 * specTotal + e], specRes[2 * specTotal + e]);
+      a2 = a2 + V(specRes[3 * specTotal + e], specRes[4 * specTotal + e], specRes[5 * specTotal + e]);
+    }
+    if (sl == lane) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) cacc[c][lane] += specRes[(6 + c) * specTotal + e];
+    }
+  }
+  double* rp = rowPart + (t - partBase) * 6 * kTile;
+  double* cp = colPart + (t - partBase) * 6 * kTile;
+  rp[0 * kTile + lane] = a1.x; rp[1 * kTile + lane] = a1.y; rp[2 * kTile + lane] = a1.z;
+  rp[3 * kTile + lane] = a2.x; rp[4 * kTile + lane] = a2.y; rp[5 * kTile + lane] = a2.z;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) cp[c * kTile + lane] = cacc[c][lane];
+}
+
+// One plain FP64 pair tile: every generic pair runs inline in FP64.
+// Near-parallel pairs (the reference's special branch, :1147-1162, several
 // times the cost of a generic pair) are not evaluated here: they were listed
 // per tile and evaluated beforehand (k_special_pass, k_special_eval), and
 // the tile adds its list's forces, in list order, after its own pairs.
 // Keeping that branch out of the tile lets it run two wavefronts per SIMD.
-// The number of FP64 pairs is added to *nExact (mixed mode).  Tile t's
-// partial sums go to slot t - partBase, its list entries are read at
-// e - specBase (the chunk's first tile and first entry).
+// Tile t's partial sums go to slot t - partBase, its list entries are read
+// at e - specBase (the chunk's first tile and first entry).
 //
-// RANGED (MODE 0): the generic pairs through the ranged sequences of
+// RANGED: the generic pairs through the ranged sequences of
 // fp64_ranged.hpp.  The tile is done only when all 128 segments are in
 // their safe range and every partial sum comes out finite; otherwise it
 // returns false having written nothing, and the caller has it evaluated
 // with the general sequences (the same bits wherever both finish).
-template <int MODE, bool RANGED>
+template <bool RANGED>
 __device__ __forceinline__ bool tile_eval(int64_t t, const double* __restrict__ seg, int64_t ld,
-                                          int n, const SegK<double>& K, const SigK<float>& KF,
-                                          double farDist2, double sinMin2,
+                                          int n, const SegK<double>& K,
                                           double* __restrict__ rowPart,
                                           double* __restrict__ colPart,
                                           const long long* __restrict__ specOff,
                                           const unsigned short* __restrict__ specSlot,
                                           const double* __restrict__ specRes, int64_t specTotal,
-                                          int64_t partBase, int64_t specBase,
-                                          unsigned long long* __restrict__ nExact) {
+                                          int64_t partBase, int64_t specBase) {
   int I, J;
   tile_blocks(t, I, J);
   const int lane = threadIdx.x;
 
-  // Staged fields: n1, n2, b, t, 1/L (the external flags are kept as
-  // ballot masks in registers); six doubles per lane of pair results in
-  // the mixed mode's flush.  With at most 20 KiB of LDS a workgroup, eight
-  // fit on a CU (two wavefronts per SIMD).
-  constexpr int NF = F_EXT;
-  __shared__ double si[NF][kTile];
-  __shared__ double sj[NF][kTile];
+  __shared__ double si[kTileFields][kTile];
+  __shared__ double sj[kTileFields][kTile];
   __shared__ double cacc[6][kTile];
-  __shared__ double sres[6][kTile];
-  __shared__ unsigned short queue[kQueue];
 
   __syncthreads();  // a previous tile of this workgroup is done with the arrays
-  const int64_t i = (int64_t)I * kTile + lane;
-  const int64_t jl = (int64_t)J * kTile + lane;
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    si[f][lane] = seg[f * ld + i];
-    sj[f][lane] = seg[f * ld + jl];
-  }
-  const bool rowExt = seg[F_EXT * ld + i] != 0.0;
-  const uint64_t colExt = __ballot(seg[F_EXT * ld + jl] != 0.0);
+  bool rowExt;
+  uint64_t colExt;
+  stage_blocks(seg, ld, I, J, lane, si, sj, rowExt, colExt);
 #pragma unroll
   for (int c = 0; c < 6; ++c) cacc[c][lane] = 0.0;
   __syncthreads();
@@ -191,57 +220,10 @@ __device__ __forceinline__ bool tile_eval(int64_t t, const double* __restrict__ 
   }
 
   const Seg<double> A = seg_from_lds(si, lane);
+  const int64_t i = (int64_t)I * kTile + lane;
   const bool rowLive = (i < n) && !rowExt;
   const bool diag = (I == J);
   v3<double> a1 = V(0.0, 0.0, 0.0), a2 = V(0.0, 0.0, 0.0);
-  int nq = 0;        // wave-uniform queue length (mixed mode)
-
-  unsigned exact = 0;
-  const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const v3<double> midA = mid_rn(A.n1, A.n2);
-
-  // Mixed mode: evaluate the first cnt queued generic pairs in FP64 and add
-  // them to rows and columns in queue order (row forces through LDS first,
-  // then column forces).
-  auto flush = [&](int cnt) {
-    v3<double> F1 = V(0.0, 0.0, 0.0), F2 = F1, F3 = F1, F4 = F1;
-    if (lane < cnt) {
-      const unsigned e = queue[lane];
-      const int rl = e & 63, sl = (rl + (e >> 6)) & 63;
-      const Seg<double> P = seg_from_lds(si, rl), Q = seg_from_lds(sj, sl);
-      double c;
-      const double omc = one_minus_c2(P.t, Q.t, c);
-      // the queue is used in mixed mode only: an approximate path, so the
-      // pair takes the FAST corner integrals
-      pair_generic<double, true, true, true>(K, P, Q, c, omc, F1, F2, F3, F4);
-      sres[0][lane] = F1.x; sres[1][lane] = F1.y; sres[2][lane] = F1.z;
-      sres[3][lane] = F2.x; sres[4][lane] = F2.y; sres[5][lane] = F2.z;
-    }
-    __syncthreads();
-    for (int e = 0; e < cnt; ++e) {  // queue order: deterministic
-      const int rl = queue[e] & 63;
-      if (rl == lane) {
-        a1 = a1 + V(sres[0][e], sres[1][e], sres[2][e]);
-        a2 = a2 + V(sres[3][e], sres[4][e], sres[5][e]);
-      }
-    }
-    __syncthreads();
-    if (lane < cnt) {
-      sres[0][lane] = F3.x; sres[1][lane] = F3.y; sres[2][lane] = F3.z;
-      sres[3][lane] = F4.x; sres[4][lane] = F4.y; sres[5][lane] = F4.z;
-    }
-    __syncthreads();
-    for (int e = 0; e < cnt; ++e) {
-      const unsigned q = queue[e];
-      const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
-      if (sl == lane) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) cacc[c][lane] += sres[c][e];
-      }
-    }
-    __syncthreads();
-  };
-
   for (int k = 0; k < kTile; ++k) {
     const int s = (lane + k) & 63;
     const int64_t j = (int64_t)J * kTile + s;
@@ -249,33 +231,7 @@ __device__ __forceinline__ bool tile_eval(int64_t t, const double* __restrict__ 
     const Seg<double> B = seg_from_lds(sj, s);
     double c;
     const double omcSq = one_minus_c2(A.t, B.t, c);
-    const bool special = is_special<MODE>(live, omcSq, midA, B.n1, B.n2, farDist2, sinMin2);
-    bool queued = false, inlineFP64 = false;
-    if (MODE == 0) {
-      inlineFP64 = live && !special;
-    } else {
-      const bool far = is_far(omcSq, midA, B.n1, B.n2, farDist2, sinMin2);
-      queued = live && !far && !special;
-      exact += __popcll(__ballot(special));
-      if (live && far) {
-        v3<double> F1, F2, F3, F4;
-        far_side<float>(KF, A, B, F1, F2);
-        far_side<float>(KF, B, A, F3, F4);
-        a1 = a1 + F1;
-        a2 = a2 + F2;
-        cacc[0][s] += F3.x; cacc[1][s] += F3.y; cacc[2][s] += F3.z;
-        cacc[3][s] += F4.x; cacc[4][s] += F4.y; cacc[5][s] += F4.z;
-      }
-    }
-    if (MODE == 1) {
-      const uint64_t m = __ballot(queued);
-      if (m) {
-        if (queued) queue[nq + __popcll(m & ltMask)] = (unsigned short)((k << 6) | lane);
-        nq += __popcll(m);
-        exact += __popcll(m);
-      }
-    }
-    if (MODE == 0 && inlineFP64) {
+    if (live && !is_special<0>(true, omcSq, v3<double>{}, B.n1, B.n2, 0.0, 0.0)) {
       v3<double> F1, F2, F3, F4;
       pair_generic<double, true, true, false, RANGED>(K, A, B, c, omcSq, F1, F2, F3, F4);
       a1 = a1 + F1;
@@ -284,18 +240,7 @@ __device__ __forceinline__ bool tile_eval(int64_t t, const double* __restrict__ 
       cacc[3][s] += F4.x; cacc[4][s] += F4.y; cacc[5][s] += F4.z;
     }
     __syncthreads();
-    if (MODE == 1 && nq >= kTile) {
-      flush(kTile);
-      const int rest = nq - kTile;
-      unsigned short keep = (lane < rest) ? queue[kTile + lane] : 0;
-      __syncthreads();
-      if (lane < rest) queue[lane] = keep;
-      __syncthreads();
-      nq = rest;
-    }
   }
-  if (MODE == 1 && nq > 0) flush(nq);
-  if (MODE == 1 && lane == 0 && exact) atomicAdd(nExact, (unsigned long long)exact);
   if constexpr (RANGED) {
     // inf or NaN anywhere in a row or column sum (a sum overflowing to inf
     // only costs a second evaluation)
@@ -304,53 +249,9 @@ __device__ __forceinline__ bool tile_eval(int64_t t, const double* __restrict__ 
     for (int c = 0; c < 6; ++c) sum = sum + cacc[c][lane];
     if (__ballot(!(sum - sum == 0.0)) != 0) return false;
   }
-  // this tile's near-parallel pairs, evaluated beforehand, in list order
-  for (long long e = specOff[t] - specBase, e1 = specOff[t + 1] - specBase; e < e1; ++e) {
-    const unsigned q = specSlot[e];
-    const int rl = q & 63, sl = (rl + (q >> 6)) & 63;
-    if (rl == lane) {
-      a1 = a1 + V(specRes[0 * specTotal + e], specRes[1 * specTotal + e], specRes[2 * specTotal + e]);
-      a2 = a2 + V(specRes[3 * specTotal + e], specRes[4 * specTotal + e], specRes[5 * specTotal + e]);
-    }
-    if (sl == lane) {
-#pragma unroll
-      for (int c = 0; c < 6; ++c) cacc[c][lane] += specRes[(6 + c) * specTotal + e];
-    }
-  }
-
-  double* rp = rowPart + (t - partBase) * 6 * kTile;
-  double* cp = colPart + (t - partBase) * 6 * kTile;
-  rp[0 * kTile + lane] = a1.x; rp[1 * kTile + lane] = a1.y; rp[2 * kTile + lane] = a1.z;
-  rp[3 * kTile + lane] = a2.x; rp[4 * kTile + lane] = a2.y; rp[5 * kTile + lane] = a2.z;
-#pragma unroll
-  for (int c = 0; c < 6; ++c) cp[c * kTile + lane] = cacc[c][lane];
+  tile_finish(t, lane, a1, a2, cacc, rowPart, colPart, specOff, specSlot, specRes, specTotal,
+              partBase, specBase);
   return true;
-}
-
-// Pair-tile kernel, one tile per workgroup (tile0 + blockIdx.x).  MODE 0
-// takes the ranged sequences and lists the tiles they leave undone in
-// redo[0 .. *redoCount) (slot order immaterial: each tile owns its
-// partial-sum slot), for k_segseg_tiles_redo; MODE 1 (mixed) does every tile.
-template <int MODE>
-__global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restrict__ seg,
-                                                           int64_t ld, int n, int64_t tile0,
-                                                           SegK<double> K, SigK<float> KF,
-                                                           double farDist2, double sinMin2,
-                                                           double* __restrict__ rowPart,
-                                                           double* __restrict__ colPart,
-                                                           const long long* __restrict__ specOff,
-                                                           const unsigned short* __restrict__ specSlot,
-                                                           const double* __restrict__ specRes,
-                                                           int64_t specTotal, int64_t partBase,
-                                                           int64_t specBase,
-                                                           unsigned long long* __restrict__ nExact,
-                                                           int* __restrict__ redo,
-                                                           unsigned long long* __restrict__ redoCount) {
-  const int64_t t = tile0 + blockIdx.x;
-  const bool done = tile_eval<MODE, MODE == 0>(t, seg, ld, n, K, KF, farDist2, sinMin2, rowPart,
-                                               colPart, specOff, specSlot, specRes, specTotal,
-                                               partBase, specBase, nExact);
-  if (MODE == 0 && !done && threadIdx.x == 0) redo[atomicAdd(redoCount, 1ull)] = (int)t;
 }
 
 }  // namespace
