@@ -687,7 +687,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU layout is described in DESIGN.md).", file=sys.stderr)
+              "(the force path runs on one device; sharding it is a design note, not code).",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -796,6 +796,24 @@ int ddd_sigma_hat(ddd_ctx* c, const ddd_mesh* mesh, const double* x0, int64_t P,
   });
 }
 
+int ddd_selftest_ranged(ddd_ctx* c, int which, const double* x, int64_t n, double* y) {
+  return guard(c, "ddd_selftest_ranged", [&] {
+    if (which < 0 || which > 3) bad("which must be 0 (log), 1 (atan), 2 (sqrt) or 3 (division)");
+    if (n < 0 || (n > 0 && (!x || !y))) bad("x/y inconsistent with n");
+    if (n > ((int64_t)1 << 28)) bad("n too large");
+    if (n == 0) return;
+    const int64_t nin = which == 3 ? 2 * n : n;
+    ScopedBuf dx, dy;
+    dx.ensure(sizeof(double) * nin);
+    dy.ensure(sizeof(double) * n);
+    HIPCHK(hipMemcpyAsync(dx.p, x, sizeof(double) * nin, hipMemcpyHostToDevice, c->st));
+    launch_ranged_selftest(which, dx.as<double>(), n, dy.as<double>(), c->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(y, dy.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+  });
+}
+
 int ddd_seg_force(ddd_ctx* c, const ddd_consts* k, const ddd_network* net, const ddd_mesh* mesh,
                   const int64_t* idx, int64_t nIdx, double* out, int64_t outCols, int flags) {
   return guard(c, "ddd_seg_force", [&] {

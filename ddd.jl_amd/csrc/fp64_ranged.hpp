@@ -62,9 +62,66 @@ __device__ __forceinline__ double log_ranged(double x) {
   return __builtin_amdgcn_class(x, 0x100) ? r : __builtin_nan("");  // 0x100: positive normal
 }
 
-// ddd_libm::atan (its division's operands are safe for every x; NaN stays NaN).
+// hi and lo parts of atan at the reduction points 1/2, 1, 3/2 and infinity
+// (ddd_libm::atan_core's constants) at index id + 1, index 0 (no
+// reduction) holding zeros: atan_ranged's table, in LDS.  A kernel that
+// calls atan_ranged fills it with atan_table_fill before its first call.
+namespace {
+__shared__ double2 atanHiLo[5];
+}
+
+// Lanes 0-4 of the calling wavefront write the table; the caller orders the
+// writes before the first atan_ranged (a barrier).
+__device__ __forceinline__ void atan_table_fill(int lane) {
+  const double hi[5] = {0.0, 4.63647609000806093515e-01, 7.85398163397448278999e-01,
+                        9.82793723247329054082e-01, 1.57079632679489655800e+00};
+  const double lo[5] = {0.0, 2.26987774529616870924e-17, 3.06161699786838301793e-17,
+                        1.39033110312309984516e-17, 6.12323399573676603587e-17};
+  if (lane < 5) atanHiLo[lane] = make_double2(hi[lane], lo[lane]);
+}
+
+// ddd_libm::atan without branches, the same bits for every x (NaN stays
+// NaN).  The reduction interval e = id + 1 (0: none, 1-4: about 1/2, 1,
+// 3/2, infinity) comes from integer compares of the high word (the bounds
+// 7/16, 11/16, 19/16 and 39/16 have zero low words).  Every interval's
+// numerator and denominator is num = S |x| - K, den = K |x| + S with
+// (S, K) = (1, 0), (2, 1), (1, 1), (1, 1.5), (0, 1): the same single
+// rounding as atan_core's expressions (S |x| and 0 |x| are exact; 1.5 |x|
+// is rounded as there).  With hi = lo = 0, hi - ((r s - lo) - r) is
+// -(r s - r) = r - r s, atan_core's result without reduction, to the bit
+// (rounding is symmetric in sign and x - x = +0 either way).  hi and lo are
+// read from atanHiLo, so the interval needs no selects of 64-bit constants.
 __device__ __forceinline__ double atan_ranged(double x) {
-  return ddd_libm::atan_core(x, RangedDiv{});
+  const uint32_t hw = (uint32_t)__double2hiint(x);
+  const uint32_t hx = hw & 0x7fffffffu;
+  const double ax = fabs(x);
+  const bool b0 = hx >= 0x3FDC0000u, b1 = hx >= 0x3FE60000u, b2 = hx >= 0x3FF30000u,
+             b3 = hx >= 0x40038000u;
+  const int e = (int)b0 + (int)b1 + (int)b2 + (int)b3;
+  const double2 hl = atanHiLo[e];
+  const uint32_t sHi = b3 ? 0u : (b0 && !b1) ? 0x40000000u : 0x3FF00000u;
+  const uint32_t kHi = !b0 ? 0u : (b2 && !b3) ? 0x3FF80000u : 0x3FF00000u;
+  const double S = __hiloint2double((int)sHi, 0), K = __hiloint2double((int)kHi, 0);
+  const double num = S * ax - K;
+  const double den = K * ax + S;
+  const double r = div_ranged(num, den);
+  const double z = r * r;
+  const double w = z * z;
+  const double s1 = z * (3.33333333333329318027e-01 +
+                         w * (1.42857142725034663711e-01 +
+                              w * (9.09088713343650656196e-02 +
+                                   w * (6.66107313738753120669e-02 +
+                                        w * (4.97687799461593236017e-02 +
+                                             w * 1.62858201153657823623e-02)))));
+  const double s2 = w * (-1.99999999998764832476e-01 +
+                         w * (-1.11111104054623557880e-01 +
+                              w * (-7.69187620504482999495e-02 +
+                                   w * (-5.83357013379057348645e-02 +
+                                        w * -3.65315727442169155270e-02))));
+  double res = hl.x - ((r * (s1 + s2) - hl.y) - r);
+  res = ax < 3.7252902984e-09 ? ax : res;
+  res = ax >= 7.378697629483821e+19 ? 1.57079632679489655800e+00 + 6.12323399573676603587e-17 : res;
+  return __hiloint2double(__double2hiint(res) ^ (int)(hw & 0x80000000u), __double2loint(res));
 }
 
 // True when a segment's end points and Burgers vector lie in the range on

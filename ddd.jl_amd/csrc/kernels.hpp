@@ -110,6 +110,9 @@ void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, c
                           double* out, hipStream_t st);
 void launch_segseg_rows(const double* seg, int64_t ld, int n, const int64_t* rows, int nRows,
                         const SegK<double>& K, double* out, unsigned* flags, hipStream_t st);
+// The tile kernel's ranged log / atan / sqrt / division on given arguments
+// (which 0-3; division takes x[2i] / x[2i+1]), for bitwise tests.
+void launch_ranged_selftest(int which, const double* x, int64_t n, double* y, hipStream_t st);
 
 struct SelfK {
   double a, aSq, mu4pi, nuOmNuInv, Ec, omNuInv;

@@ -313,6 +313,25 @@ __global__ __launch_bounds__(kTile) void k_segseg_rows(const double* __restrict_
   }
 }
 
+// The tile kernel's ranged sequences on given arguments, for tests that
+// compare them bit for bit with the host's general ones: y[i] =
+// log_ranged(x[i]) (which 0), atan_ranged(x[i]) (1), sqrt_ranged(x[i]) (2),
+// div_ranged(x[2i], x[2i+1]) (3).  Grid-stride over i.
+__global__ __launch_bounds__(kTile) void k_ranged_selftest(int which, const double* __restrict__ x,
+                                                           int64_t n, double* __restrict__ y) {
+  atan_table_fill(threadIdx.x);
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)kTile + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kTile) {
+    double r;
+    if (which == 0) r = log_ranged(x[i]);
+    else if (which == 1) r = atan_ranged(x[i]);
+    else if (which == 2) r = sqrt_ranged(x[i]);
+    else r = div_ranged(x[2 * i], x[2 * i + 1]);
+    y[i] = r;
+  }
+}
+
 }  // namespace
 
 void launch_prep_segments(double* seg, int64_t ld, int64_t n, hipStream_t st) {
@@ -402,6 +421,12 @@ void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, c
   const int bs = 256;
   hipLaunchKernelGGL(k_segseg_reduce, dim3((unsigned)((m + bs - 1) / bs)), dim3(bs), 0, st,
                      rowPart, colPart, (int)m, r.J0, r.J1, out);
+}
+
+void launch_ranged_selftest(int which, const double* x, int64_t n, double* y, hipStream_t st) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + kTile - 1) / kTile, 4096);
+  hipLaunchKernelGGL(k_ranged_selftest, dim3((unsigned)blocks), dim3(kTile), 0, st, which, x, n, y);
 }
 
 void launch_segseg_rows(const double* seg, int64_t ld, int n, const int64_t* rows, int nRows,

@@ -208,6 +208,7 @@ __device__ __forceinline__ bool tile_eval(int64_t t, const double* __restrict__ 
   stage_blocks(seg, ld, I, J, lane, si, sj, rowExt, colExt);
 #pragma unroll
   for (int c = 0; c < 6; ++c) cacc[c][lane] = 0.0;
+  if constexpr (RANGED) atan_table_fill(lane);
   __syncthreads();
 
   if constexpr (RANGED) {

@@ -123,6 +123,7 @@ def lib():
                 "ddd_pk_force": (C.c_int, [P, C.POINTER(ddd_network), C.POINTER(ddd_mesh), ip,
                                            i64, dp, i64, C.c_int]),
                 "ddd_sigma_hat": (C.c_int, [P, C.POINTER(ddd_mesh), dp, i64, dp]),
+                "ddd_selftest_ranged": (C.c_int, [P, C.c_int, dp, i64, dp]),
                 "ddd_seg_force": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_network),
                                             C.POINTER(ddd_mesh), ip, i64, dp, i64, C.c_int]),
                 "ddd_sigma_tilde": (C.c_int, [P, C.POINTER(ddd_consts), C.POINTER(ddd_network),

@@ -151,6 +151,18 @@ def calc_σHat(mesh, forceDisplacement, x0, ctx=None):
     return out[:, :, 0].copy(order="F") if single else out
 
 
+def selftest_ranged(which, x, ctx=None):
+    """The all-pairs tile kernel's ranged log (which 0), atan (1), sqrt (2)
+    of every x, or its division x[0::2] / x[1::2] (3), evaluated on the GPU
+    (ddd_selftest_ranged; a self-test, no reference counterpart)."""
+    ctx = _ctx(ctx)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = x.size // 2 if which == 3 else x.size
+    y = np.empty(n)
+    _call(ctx, "ddd_selftest_ranged", int(which), L.ptr(x, "d"), n, L.ptr(y, "d"))
+    return y
+
+
 def calcSegForce(dlnParams, matParams, mesh, forceDisplacement, network, idx=None, ctx=None):
     """calcSegForce (DislocationForces.jl:23-46): seg-seg + self + PK/2."""
     ctx = _ctx(ctx)

@@ -201,6 +201,17 @@ int ddd_farfield_debug(ddd_ctx* ctx, ddd_farfield* ff, int what, void* out, int6
 
 int ddd_version(void);
 
+/* Self-test (no counterpart in the reference): the all-pairs tile kernel's
+ * division, square root, log and atan without range scaling, on given
+ * arguments, so that tests can compare them bit for bit with the general
+ * IEEE sequences and the host's log/atan.  y[i] = log(x[i]) (which 0),
+ * atan(x[i]) (1), sqrt(x[i]) (2), x[2i] / x[2i+1] (3); the ranged forms
+ * are exact for the arguments the tile kernel admits (positive normal log
+ * arguments, every atan argument, normal operands away from the exponent
+ * limits) and give NaN, never a wrong finite value, for zero or infinite
+ * log, square-root and division operands. */
+int ddd_selftest_ranged(ddd_ctx* ctx, int which, const double* x, int64_t n, double* y);
+
 /* Create a context on HIP device `device`; DDD_ERR_NO_DEVICE when none. */
 int ddd_ctx_create(ddd_ctx** ctx, int device);
 int ddd_ctx_destroy(ddd_ctx* ctx);

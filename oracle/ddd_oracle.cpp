@@ -106,6 +106,13 @@ int orc_version() { return 1; }
 void orc_set_libm(int glibc) { g_glibc_libm = glibc; }
 double orc_libm_log(double x) { return ddd_libm::log(x); }
 double orc_libm_atan(double x) { return ddd_libm::atan(x); }
+// Vector form: y[i] = log(x[i]) (which 0), atan(x[i]) (1), sqrt(x[i]) (2),
+// x[2i] / x[2i+1] (3), with the general IEEE operations.
+void orc_libm_vec(int which, const double* x, int64_t n, double* y) {
+  for (int64_t i = 0; i < n; ++i)
+    y[i] = which == 0 ? ddd_libm::log(x[i]) : which == 1 ? ddd_libm::atan(x[i])
+         : which == 2 ? std::sqrt(x[i]) : x[2 * i] / x[2 * i + 1];
+}
 
 // One pair, F1..F4 in out[12].  prec: 0 double, 1 long double, 2 float.
 // Returns 1 when the near-parallel branch was taken.

@@ -69,6 +69,7 @@ def lib():
             "orc_segseg_tuned_rows": (i64, [dp] + net + [ip, i64, dp, i32]),
             "orc_libm_log": (d, [d]),
             "orc_libm_atan": (d, [d]),
+            "orc_libm_vec": (None, [i32, dp, i64, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(_lib, name)
@@ -104,6 +105,15 @@ def libm_log(x):
 
 def libm_atan(x):
     return lib().orc_libm_atan(float(x))
+
+
+def libm_vec(which, x):
+    """log (which 0), atan (1), sqrt (2) of every x, or x[0::2] / x[1::2] (3)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = x.size // 2 if which == 3 else x.size
+    y = np.empty(n)
+    lib().orc_libm_vec(which, _d(x), n, _d(y))
+    return y
 
 
 def seg_consts(dln, mat):
