@@ -306,6 +306,9 @@ MixedRule mixed_rule(const ddd_consts& k, const ddd_opts& o) {
   MixedRule r;
   r.farDist = o.farDist > 0 ? o.farDist : 5.0 * k.coreRad;
   r.sinMin = o.sinMin != 0 ? o.sinMin : 1e-3;
+  // the FP32 quadrature's reciprocals take a^2 and larger arguments
+  if (!(k.coreRadSq >= 0x1p-60 && k.coreRadSq <= 0x1p60))
+    bad("the FP32 quadrature needs a squared core radius in [2^-60, 2^60]");
   r.K = SigK<float>{float(k.coreRadSq), float(k.mu4pi), float(k.mu4pinu), float(k.nuMu4pinu),
                     float(k.coreRadSq * k.mu8pi)};
   return r;

@@ -59,7 +59,10 @@ struct RangedDiv {
 // [-0.3, 0.42] and [1.7, 2.42]); NaN for every other x.
 __device__ __forceinline__ double log_ranged(double x) {
   const double r = ddd_libm::log_core(x, 0, RangedDiv{});
-  return __builtin_amdgcn_class(x, 0x100) ? r : __builtin_nan("");  // 0x100: positive normal
+  // 0x100: positive normal; otherwise the high word of a quiet NaN (any low
+  // word makes it NaN), one select instead of two
+  const int hi = __builtin_amdgcn_class(x, 0x100) ? __double2hiint(r) : 0x7ff80000;
+  return __hiloint2double(hi, __double2loint(r));
 }
 
 // hi and lo parts of atan at the reduction points 1/2, 1, 3/2 and infinity

@@ -583,6 +583,19 @@ __device__ __forceinline__ void seg_stress(const SigK<T>& K, v3<T> rx, v3<T> d, 
 // seg_stress in about half the operations, rounded otherwise: for the
 // approximate paths (far_side).
 template <class T>
+__device__ __forceinline__ T q_rcp(T x) { return T(1) / x; }
+template <class T>
+__device__ __forceinline__ T q_rsqrt(T x) { return T(1) / sqrt(x); }
+// FP32: the hardware reciprocal and reciprocal square root (about 1 ulp)
+// without the library forms' denormal range scaling; their arguments here
+// are at least a^2 (refused outside [2^-60, 2^60] on the host) or half a
+// segment length, far from the denormal range.
+template <>
+__device__ __forceinline__ float q_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+template <>
+__device__ __forceinline__ float q_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+
+template <class T>
 __device__ __forceinline__ v3<T> seg_stress_times(const SigK<T>& K, v3<T> rx, T L, v3<T> t, v3<T> b,
                                                   v3<T> txb, T tba, v3<T> ba) {
   const T Rdt = dot(rx, t);
@@ -591,8 +604,8 @@ __device__ __forceinline__ v3<T> seg_stress_times(const SigK<T>& K, v3<T> rx, T 
   const T s1 = -Rdt;
   const T s2 = L - Rdt;
   const T a2_d2 = K.a2 + d2;
-  const T a2d2inv = T(1) / a2_d2;
-  T Rainv = T(1) / sqrt(a2_d2 + s1 * s1);
+  const T a2d2inv = q_rcp(a2_d2);
+  T Rainv = q_rsqrt(a2_d2 + s1 * s1);
   T Ra3inv = Rainv * Rainv * Rainv;
   T sRa3inv = s1 * Ra3inv;
   const T s03a = s1 * Rainv * a2d2inv;
@@ -600,7 +613,7 @@ __device__ __forceinline__ v3<T> seg_stress_times(const SigK<T>& K, v3<T> rx, T 
   const T s05a = (T(2) * s03a + sRa3inv) * a2d2inv;
   const T s15a = -Ra3inv;
   const T s25a = s03a - sRa3inv;
-  Rainv = T(1) / sqrt(a2_d2 + s2 * s2);
+  Rainv = q_rsqrt(a2_d2 + s2 * s2);
   Ra3inv = Rainv * Rainv * Rainv;
   sRa3inv = s2 * Ra3inv;
   const T s03b = s2 * Rainv * a2d2inv;
@@ -649,7 +662,7 @@ __device__ __forceinline__ void far_side(const SigK<T>& K, const Seg<double>& A,
   const v3<T> sa = seg_stress_times(K, r0 + g0 * dA, LB, tB, bB, txb, tba, ba);
   const v3<T> sc = seg_stress_times(K, r0 + g1 * dA, LB, tB, bB, txb, tba, ba);
   // (sigma b) x t L w, w = 1/2 per node of the two-point rule
-  const T hw = T(0.5) / T(A.tN);
+  const T hw = T(0.5) * q_rcp(T(A.tN));
   const v3<T> f1 = cross(sa * g1 + sc * g0, tA) * hw;  // weight (1 - u)
   const v3<T> f2 = cross(sa * g0 + sc * g1, tA) * hw;  // weight u
   FA1 = V(double(f1.x), double(f1.y), double(f1.z));
