@@ -162,6 +162,10 @@ struct ddd_ctx {
   double* pendRow = nullptr;  // its partial sums in chunkMem
   double* pendCol = nullptr;
   int chunks = 0;           // chunks of the last all-pairs evaluation
+  // events around each chunk's pair-tile kernels (plain FP64 and mixed):
+  // their summed time is ddd_stats.tileMs
+  std::vector<hipEvent_t> tileEv;
+  int tileEvUsed = 0;
   int64_t chunkBytes = (int64_t)32 << 30;
   // device-resident dynamics (deriv!, integrate!)
   DevBuf dCoord, dCoord0, dVel, dVel0, dNodeF, dLinks, dConn, dBvec, dLabel, dSegNodes, dRedPart,
@@ -451,13 +455,14 @@ void hybrid_apply(ddd_ctx* c, int64_t n, double* dev, const ddd_opts& o) {
 
 // Device memory of one chunk of the all-pairs triangle, one buffer carved
 // per chunk: row and column partial sums (6 doubles x 64 lanes each per
-// tile), the list of tiles the ranged pass leaves to the general one (one
-// int per tile) and the near-parallel list (slot, tile, 12 forces and a
-// fix-list entry per pair), each part 256-byte aligned.  A chunk takes whole column
-// blocks while its layout stays within the context's chunkBytes (default
-// 32 GiB), so the buffer never exceeds the budget; a single block column
-// above it (at most 65,535 tiles under the tile-count limit) makes a chunk
-// of its own.
+// tile; the plain FP64 mode keeps them per super-tile, kSuper rows and
+// kSuper columns of them, counted here as kSuper tiles), the list of tiles or super-tiles
+// the ranged pass leaves to the general one (one int each) and the
+// near-parallel list (slot, tile, 12 forces and a fix-list entry per pair),
+// each part 256-byte aligned.  A chunk takes whole column blocks (whole
+// super-columns of kSuper blocks in the plain mode) while its layout stays
+// within the context's chunkBytes (default 32 GiB), so the buffer never
+// exceeds the budget; a single column above it makes a chunk of its own.
 struct ChunkLayout {
   int64_t row = 0, col = 0, redo = 0, slot = 0, tile = 0, res = 0, fix = 0, bytes = 0;
   ChunkLayout(int64_t tiles, int64_t spec) {
@@ -511,16 +516,23 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
   HIPCHK(hipMemcpyAsync(offAt.data(), c->specAt.p, sizeof(long long) * (nb + 1),
                         hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
-  // chunks of whole column blocks within the budget
+  // chunks of whole column blocks within the budget; the plain FP64 mode
+  // keeps its partials by super-tile (kSuper x kSuper tiles, 24 KiB), so
+  // its chunks take whole super-columns of kSuper column blocks
   std::vector<TileRange> chunks;
+  const int step = mixed ? 1 : kSuper;
+  auto range = [&](int J0, int J1) { return TileRange{J0, J1, !mixed}; };
   auto layout = [&](int J0, int J1) {
-    return ChunkLayout(tiles_for(J1) - tiles_for(J0), offAt[J1] - offAt[J0]);
+    const TileRange q = range(J0, J1);
+    const int64_t tiles = mixed ? q.t1() - q.t0() : (q.u1() - q.u0()) * kSuper;
+    return ChunkLayout(tiles, offAt[J1] - offAt[J0]);
   };
   int64_t most = 0;
   for (int J0 = 0; J0 < nb;) {
-    int J1 = J0 + 1;
-    while (J1 < nb && layout(J0, J1 + 1).bytes <= c->chunkBytes) ++J1;
-    chunks.push_back(TileRange{J0, J1});
+    int J1 = (int)std::min<int64_t>(J0 + step, nb);
+    while (J1 < nb && layout(J0, (int)std::min<int64_t>(J1 + step, nb)).bytes <= c->chunkBytes)
+      J1 = (int)std::min<int64_t>(J1 + step, nb);
+    chunks.push_back(range(J0, J1));
     most = std::max(most, layout(J0, J1).bytes);
     J0 = J1;
   }
@@ -532,6 +544,12 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
     HIPCHK(hipMemsetAsync(c->counter.p, 0, 2 * sizeof(unsigned long long), c->st));
     c->mixStage.ensure(sizeof(unsigned short) * kMixedBlocks * kMixedStage);
   }
+  while (c->tileEv.size() < 2 * chunks.size()) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    c->tileEv.push_back(e);
+  }
+  c->tileEvUsed = 0;
   char* mem = c->chunkMem.as<char>();
   for (size_t q = 0; q < chunks.size(); ++q) {
     const TileRange& ch = chunks[q];
@@ -544,11 +562,14 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
                    reinterpret_cast<int64_t*>(mem + L.fix), c->specFixN.as<unsigned long long>()};
     launch_special_list(c->seg.as<double>(), c->resLd, (int)n, mixed ? &r : nullptr, sp,
                         seg_consts(k), ch, c->st);
+    HIPCHK(hipEventRecord(c->tileEv[2 * q], c->st));
     launch_segseg_tiles(c->seg.as<double>(), c->resLd, (int)n, seg_consts(k), rowPart, colPart, sp,
                         ch, c->st, reinterpret_cast<int*>(mem + L.redo),
                         c->redoN.as<unsigned long long>(), mixed ? &r : nullptr,
                         mixed ? c->mixStage.as<unsigned short>() : nullptr,
                         mixed ? c->counter.as<unsigned long long>() : nullptr);
+    HIPCHK(hipEventRecord(c->tileEv[2 * q + 1], c->st));
+    c->tileEvUsed = (int)q + 1;
     if (q + 1 < chunks.size()) {
       launch_segseg_reduce(rowPart, colPart, (int)n, ch, dev, c->st);
     } else {
@@ -666,6 +687,7 @@ int ddd_ctx_destroy(ddd_ctx* c) {
                      &c->pts, &c->sigPart, &c->sigma, &c->conn, &c->links, &c->coord, &c->nodeF,
                      &c->specCnt, &c->specOff, &c->specWork};
     for (DevBuf* b : all) b->release();
+    for (hipEvent_t e : c->tileEv) (void)hipEventDestroy(e);
     if (c->own) (void)hipStreamDestroy(c->own);
   }
   delete c;
@@ -1591,6 +1613,15 @@ void fill_stats(ddd_ctx* c, int64_t n, const ddd_opts* o, ddd_stats* st) {
                           hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     st->pairsFP64 = e;
+  }
+  st->tileMs = 0.0;
+  if (!is_hybrid(o) && c->tileEvUsed > 0) {
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (int q = 0; q < c->tileEvUsed; ++q) {
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, c->tileEv[2 * q], c->tileEv[2 * q + 1]));
+      st->tileMs += ms;
+    }
   }
 }
 

@@ -102,6 +102,9 @@ __device__ __forceinline__ double atan_ranged(double x) {
              b3 = hx >= 0x40038000u;
   const int e = (int)b0 + (int)b1 + (int)b2 + (int)b3;
   const double2 hl = atanHiLo[e];
+  // keep the table read here, ahead of the division and polynomial that
+  // hide its latency (nothing moves across this point)
+  __builtin_amdgcn_sched_barrier(0);
   const uint32_t sHi = b3 ? 0u : (b0 && !b1) ? 0x40000000u : 0x3FF00000u;
   const uint32_t kHi = !b0 ? 0u : (b2 && !b3) ? 0x3FF80000u : 0x3FF00000u;
   const double S = __hiloint2double((int)sHi, 0), K = __hiloint2double((int)kHi, 0);

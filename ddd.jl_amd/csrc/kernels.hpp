@@ -41,10 +41,20 @@ struct MixedRule {
 // i.e. the tiles t in [tiles_for(J0), tiles_for(J1)) of the column-block-
 // major enumeration, whose partial sums sit at slot t - tiles_for(J0) of the
 // partial buffers and are reduced into segForce before the next chunk.
+// Plain FP64 chunks keep their partial sums by super-tile (kSuper x kSuper
+// tiles, rows and columns of kSuper blocks, enumerated like the tiles):
+// J0 is then a multiple of kSuper and super-tile u of super-columns
+// [J0 / kSuper, ceil(J1 / kSuper)) sits at slot u - u0().
+constexpr int kSuper = 4;
 struct TileRange {
   int J0, J1;
+  bool super = false;
   int64_t t0() const { return (int64_t)J0 * (J0 + 1) / 2; }
   int64_t t1() const { return (int64_t)J1 * (J1 + 1) / 2; }
+  int Q0() const { return J0 / kSuper; }
+  int Q1() const { return (J1 + kSuper - 1) / kSuper; }
+  int64_t u0() const { return tiles_for(Q0()); }
+  int64_t u1() const { return tiles_for(Q1()); }
 };
 // Near-parallel pairs of the all-pairs tiles, listed per tile and evaluated
 // before the tile kernel: off[t] .. off[t + 1] is tile t's range (exclusive

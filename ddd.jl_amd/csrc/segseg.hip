@@ -2,18 +2,20 @@
 //
 // Work decomposition: the upper triangle of segment pairs (i < j, the pairs
 // calcSegSegForceSer! visits, DislocationForces.jl:794-800) is cut into
-// 64 x 64 tiles, one wavefront per tile.  Lane l owns row i = 64 I + l and
-// holds its node forces in registers; at step k it pairs with column slot
+// 64 x 64 tiles, grouped 4 x 4 into super-tiles of one workgroup each
+// (k_segseg_super).  Lane l of a tile owns row i = 64 I + l and holds its
+// node forces in registers; at step k it pairs with column slot
 // s = (l + k) mod 64 of the J block, so within one step every lane touches a
 // different column and the column forces accumulate in LDS without atomics.
-// Each tile writes its row and column partial sums; a second kernel adds the
-// partials of every segment in a fixed order, so results are bitwise
-// reproducible run to run.
+// Each super-tile writes its row and column partial sums; a second kernel
+// adds the partials of every segment in a fixed order, so results are
+// bitwise reproducible run to run.
 //
 // Near-parallel pairs (1 - c^2 <= sqrt(eps), the reference's special branch
 // :1147-1162) are not evaluated inline, which would stall the whole wavefront
-// on one lane.  They are queued in LDS and evaluated 64 at a time, one per
-// lane, then added to their row and column in queue order.
+// on one lane: a count pass and a listing pass put them in a per-tile list,
+// two packed passes evaluate them one per lane, and the tile adds their
+// forces in list order after its own pairs.
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -85,45 +87,246 @@ __global__ __launch_bounds__(kTile) void k_special_eval(const double* __restrict
 
 __global__ void k_zero_count(unsigned long long* c) { *c = 0; }
 
-// Plain FP64 pair-tile kernel, one tile per workgroup (tile0 + blockIdx.x),
-// with the ranged sequences; the tiles they leave undone are listed in
-// redo[0 .. *redoCount) (slot order immaterial: each tile owns its
-// partial-sum slot) for k_segseg_tiles_redo.
-__global__ __launch_bounds__(kTile, 2) void k_segseg_tiles(const double* __restrict__ seg,
-                                                           int64_t ld, int n, int64_t tile0,
-                                                           SegK<double> K,
-                                                           double* __restrict__ rowPart,
-                                                           double* __restrict__ colPart,
-                                                           const long long* __restrict__ specOff,
-                                                           const unsigned short* __restrict__ specSlot,
-                                                           const double* __restrict__ specRes,
-                                                           int64_t specTotal, int64_t partBase,
-                                                           int64_t specBase,
-                                                           int* __restrict__ redo,
-                                                           unsigned long long* __restrict__ redoCount) {
-  const int64_t t = tile0 + blockIdx.x;
-  const bool done = tile_eval<true>(t, seg, ld, n, K, rowPart, colPart, specOff, specSlot, specRes,
-                                    specTotal, partBase, specBase);
-  if (!done && threadIdx.x == 0) redo[atomicAdd(redoCount, 1ull)] = (int)t;
+// Plain FP64 super-tiles: the kSuper x kSuper (4 x 4) pair tiles of row
+// blocks 4P .. 4P + 3 and column blocks 4Q .. 4Q + 3 (P <= Q) in one
+// workgroup of four wavefronts.  Wavefront w keeps row block 4P + w for the
+// whole super-tile (its row sums in registers, its row segment read into
+// registers once per tile) and in round r takes column block
+// 4Q + ((w + r) % 4); in every round the four wavefronts hold four
+// different column blocks, whose segments and column sums sit in LDS, so no
+// two wavefronts ever touch the same sums and a barrier between rounds is
+// the only synchronisation across them.  Within a tile, lane l pairs with
+// column slot (l + k) % 64 at step k, and the tile's near-parallel pairs,
+// evaluated beforehand, are added after its steps in list order.  The
+// super-tile reads its 8 blocks of segments once and writes 4 row and 4
+// column partial sums (24 KiB) where 16 separate tiles read 32 blocks and
+// wrote 96 KiB.  Two workgroups share a CU (64 KiB of LDS each), so the
+// two wavefronts of each SIMD come from different workgroups and hide each
+// other's LDS latency; one workgroup of eight wavefronts (8 x 8 tiles), its
+// pairs of wavefronts kept in step by the round barriers, ran 7 % slower.
+// Super-tile u's partials go to slot u - partBase:
+// rowPart / colPart [slot][block w][component][lane].
+//
+// RANGED: the ranged sequences, done only when all 8 blocks are in the safe
+// range and every sum comes out finite; otherwise the super-tile writes
+// nothing and returns false (the caller has it redone with the general
+// sequences).
+constexpr int kSuperThreads = kSuper * kTile;
+
+// Orders one wavefront's LDS accesses across the steps of a tile: lane l
+// adds into column slot (l + k) & 63 at step k, which lane l - 1 updates
+// at step k + 1.  A wavefront's LDS operations complete in order, so only
+// the compiler has to be kept from moving them across steps.
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The tiles k_segseg_tiles left undone, with the general IEEE sequences:
-// grid-stride over the list (its length is read on the device).
-__global__ __launch_bounds__(kTile, 2) void k_segseg_tiles_redo(const double* __restrict__ seg,
-                                                               int64_t ld, int n, SegK<double> K,
-                                                               double* __restrict__ rowPart,
-                                                               double* __restrict__ colPart,
-                                                               const long long* __restrict__ specOff,
-                                                               const unsigned short* __restrict__ specSlot,
-                                                               const double* __restrict__ specRes,
-                                                               int64_t specTotal, int64_t partBase,
-                                                               int64_t specBase,
-                                                               const int* __restrict__ redo,
-                                                               const unsigned long long* __restrict__ redoCount) {
+template <bool RANGED>
+__device__ __forceinline__ bool super_eval(int64_t u, const double* __restrict__ seg, int64_t ld,
+                                           int n, int nb, const SegK<double>& K,
+                                           double* __restrict__ rowPart,
+                                           double* __restrict__ colPart,
+                                           const long long* __restrict__ specOff,
+                                           const unsigned short* __restrict__ specSlot,
+                                           const double* __restrict__ specRes, int64_t specTotal,
+                                           int64_t partBase, int64_t specBase) {
+  __shared__ double si[kSuper][kTileFields][kTile];
+  __shared__ double sj[kSuper][kTileFields][kTile];
+  __shared__ double cacc[kSuper][6][kTile];
+  __shared__ unsigned long long colExtS[kSuper];
+
+  int P, Q;
+  tile_blocks(u, P, Q);
+  // the wavefront's index, uniform across it (scalar registers, scalar branches)
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile);
+  const int lane = threadIdx.x % kTile;
+  const int I = kSuper * P + w;
+  const int Jw = kSuper * Q + w;
+  const bool rowValid = I < nb;
+
+  __syncthreads();  // a previous super-tile of this workgroup is done with the arrays
+  bool ok = true;
+  if (Jw < nb) {  // wavefront w stages column block Jw
+    const int64_t j = (int64_t)Jw * kTile + lane;
+#pragma unroll
+    for (int f = 0; f < kTileFields; ++f) sj[w][f][lane] = seg[f * ld + j];
+    const uint64_t m = __ballot(seg[F_EXT * ld + j] != 0.0);
+    if (lane == 0) colExtS[w] = m;
+    if constexpr (RANGED) {
+      const int rf[9] = {F_N1X, F_N1Y, F_N1Z, F_N2X, F_N2Y, F_N2Z, F_BX, F_BY, F_BZ};
+#pragma unroll
+      for (int f = 0; f < 9; ++f) ok = ok && ranged_value(sj[w][rf[f]][lane]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) cacc[w][c][lane] = 0.0;
+  bool rowExt = true;
+  const int64_t i = (int64_t)I * kTile + lane;
+  if (rowValid) {  // and its row block I
+#pragma unroll
+    for (int f = 0; f < kTileFields; ++f) si[w][f][lane] = seg[f * ld + i];
+    rowExt = seg[F_EXT * ld + i] != 0.0;
+    if constexpr (RANGED) {
+      const int rf[9] = {F_N1X, F_N1Y, F_N1Z, F_N2X, F_N2Y, F_N2Z, F_BX, F_BY, F_BZ};
+#pragma unroll
+      for (int f = 0; f < 9; ++f) ok = ok && ranged_value(si[w][rf[f]][lane]);
+    }
+  }
+  if constexpr (RANGED) {
+    atan_table_fill(threadIdx.x);
+    ok = ok && K.aSq >= 0x1p-120 && K.aSq <= 0x1p120;
+    if (__syncthreads_or(!ok)) return false;  // also the barrier after staging
+  } else {
+    __syncthreads();
+  }
+
+  const bool rowLive = rowValid && (i < n) && !rowExt;
+  v3<double> a1 = V(0.0, 0.0, 0.0), a2 = V(0.0, 0.0, 0.0);
+  for (int r = 0; r < kSuper; ++r) {
+    const int cb = (w + r) % kSuper;
+    const int J = kSuper * Q + cb;
+    if (rowValid && J < nb && I <= J) {  // uniform across the wavefront
+      const Seg<double> A0 = seg_from_lds(si[w], lane);
+      const double (*sjc)[kTile] = sj[cb];
+      double (*cc)[kTile] = cacc[cb];
+      const uint64_t colExt = colExtS[cb];
+      const bool diag = (I == J);
+      for (int k = 0; k < kTile; ++k) {
+        const int s = (lane + k) % kTile;
+        const int64_t j = (int64_t)J * kTile + s;
+        const bool live = rowLive && (j < n) && !((colExt >> s) & 1ull) && (!diag || s > lane);
+        // the row segment stays in registers across the steps: reading it
+        // from LDS at every step exposes that read's latency at the start
+        // of the pair (the evaluation took 10 % longer)
+        const Seg<double> A = A0;
+        const Seg<double> B = seg_from_lds(sjc, s);
+        double c;
+        const double omcSq = one_minus_c2(A.t, B.t, c);
+        if (live && !is_special<0>(true, omcSq, v3<double>{}, B.n1, B.n2, 0.0, 0.0)) {
+          v3<double> F1, F2, F3, F4;
+          pair_generic<double, true, true, false, RANGED>(K, A, B, c, omcSq, F1, F2, F3, F4);
+          a1 = a1 + F1;
+          a2 = a2 + F2;
+          cc[0][s] += F3.x; cc[1][s] += F3.y; cc[2][s] += F3.z;
+          cc[3][s] += F4.x; cc[4][s] += F4.y; cc[5][s] += F4.z;
+        }
+        wave_lds_order();
+      }
+      // the tile's near-parallel pairs, in list order: 64 entries loaded at
+      // once, one per lane, then taken in turn from the loading lane's
+      // registers, so that the other wavefronts do not wait at the round's
+      // barrier for one memory latency per entry
+      const int64_t t = tiles_for(J) + I;
+      const long long eb = specOff[t] - specBase, ee = specOff[t + 1] - specBase;
+      for (long long b0 = eb; b0 < ee; b0 += kTile) {
+        const int cnt = (int)(ee - b0 < kTile ? ee - b0 : kTile);
+        unsigned q = 0;
+        double v[12];
+#pragma unroll
+        for (int c = 0; c < 12; ++c) v[c] = 0.0;
+        if (lane < cnt) {
+          q = specSlot[b0 + lane];
+#pragma unroll
+          for (int c = 0; c < 12; ++c) v[c] = specRes[c * specTotal + b0 + lane];
+        }
+        for (int e = 0; e < cnt; ++e) {
+          const unsigned qe = (unsigned)__builtin_amdgcn_readlane((int)q, e);
+          const int rl = qe & 63, sl = (rl + (qe >> 6)) & 63;
+          double f[12];
+#pragma unroll
+          for (int c = 0; c < 12; ++c)
+            f[c] = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v[c]), e),
+                                    __builtin_amdgcn_readlane(__double2loint(v[c]), e));
+          if (rl == lane) {
+            a1 = a1 + V(f[0], f[1], f[2]);
+            a2 = a2 + V(f[3], f[4], f[5]);
+          }
+          if (sl == lane) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) cc[c][lane] += f[6 + c];
+          }
+        }
+      }
+    }
+    __syncthreads();  // column block cb passes to another wavefront
+  }
+  if constexpr (RANGED) {
+    // inf or NaN anywhere in a row or column sum (a sum overflowing to inf
+    // only costs a second evaluation)
+    double sum = a1.x + a1.y + a1.z + a2.x + a2.y + a2.z;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sum = sum + cacc[w][c][lane];
+    if (__syncthreads_or(!(sum - sum == 0.0))) return false;
+  }
+  double* rp = rowPart + ((u - partBase) * kSuper + w) * 6 * kTile + lane;
+  double* cp = colPart + ((u - partBase) * kSuper + w) * 6 * kTile + lane;
+  rp[0 * kTile] = a1.x; rp[1 * kTile] = a1.y; rp[2 * kTile] = a1.z;
+  rp[3 * kTile] = a2.x; rp[4 * kTile] = a2.y; rp[5 * kTile] = a2.z;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) cp[c * kTile] = cacc[w][c][lane];
+  return true;
+}
+
+// Plain FP64 super-tiles with the ranged sequences, one per workgroup
+// (u0 + blockIdx.x); those they leave undone are listed in
+// redo[0 .. *redoCount) for k_segseg_super_redo.
+__global__ __launch_bounds__(kSuperThreads, 2) void k_segseg_super(
+    const double* __restrict__ seg, int64_t ld, int n, int nb, int64_t u0, SegK<double> K,
+    double* __restrict__ rowPart, double* __restrict__ colPart,
+    const long long* __restrict__ specOff, const unsigned short* __restrict__ specSlot,
+    const double* __restrict__ specRes, int64_t specTotal, int64_t partBase, int64_t specBase,
+    int* __restrict__ redo, unsigned long long* __restrict__ redoCount) {
+  const int64_t u = u0 + blockIdx.x;
+  const bool done = super_eval<true>(u, seg, ld, n, nb, K, rowPart, colPart, specOff, specSlot,
+                                     specRes, specTotal, partBase, specBase);
+  if (!done && threadIdx.x == 0) redo[atomicAdd(redoCount, 1ull)] = (int)u;
+}
+
+// The super-tiles k_segseg_super left undone, with the general IEEE
+// sequences: grid-stride over the list (its length is read on the device).
+__global__ __launch_bounds__(kSuperThreads, 2) void k_segseg_super_redo(
+    const double* __restrict__ seg, int64_t ld, int n, int nb, SegK<double> K,
+    double* __restrict__ rowPart, double* __restrict__ colPart,
+    const long long* __restrict__ specOff, const unsigned short* __restrict__ specSlot,
+    const double* __restrict__ specRes, int64_t specTotal, int64_t partBase, int64_t specBase,
+    const int* __restrict__ redo, const unsigned long long* __restrict__ redoCount) {
   const int64_t cnt = (int64_t)*redoCount;
   for (int64_t q = blockIdx.x; q < cnt; q += gridDim.x)
-    (void)tile_eval<false>(redo[q], seg, ld, n, K, rowPart, colPart, specOff, specSlot, specRes,
-                           specTotal, partBase, specBase);
+    (void)super_eval<false>(redo[q], seg, ld, n, nb, K, rowPart, colPart, specOff, specSlot,
+                            specRes, specTotal, partBase, specBase);
+}
+
+// Sum of a segment's super-tile partials of one chunk (super-columns
+// [Q0, Q1), super-tile u at slot u - tiles_for(Q0)), in fixed order: its
+// row role in super-tiles (Pb, Q), Q in [max(Pb, Q0), Q1), then its column
+// role in super-tiles (P, Pb), P <= Pb, when Pb is in the chunk (Pb its
+// block's super-block, w its block's place in it).
+__global__ void k_segseg_reduce_super(const double* __restrict__ rowPart,
+                                      const double* __restrict__ colPart, int n, int Q0, int Q1,
+                                      double* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int B = s / kTile, l = s % kTile;
+  const int Pb = B / kSuper, w = B % kSuper;
+  const int64_t base = tiles_for(Q0);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int Q = Pb > Q0 ? Pb : Q0; Q < Q1; ++Q) {
+    const double* p = rowPart + ((tiles_for(Q) + Pb - base) * kSuper + w) * 6 * kTile + l;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[c] += p[c * kTile];
+  }
+  if (Pb >= Q0 && Pb < Q1) {
+    const int64_t u0 = tiles_for(Pb) - base;
+    for (int P = 0; P <= Pb; ++P) {
+      const double* p = colPart + ((u0 + P) * kSuper + w) * 6 * kTile + l;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) acc[c] += p[c * kTile];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) out[6 * (int64_t)s + c] += acc[c];
 }
 
 // Pass B for the listed pairs: their corrective sub-pairs, added to their
@@ -399,28 +602,31 @@ void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double
     return;
   }
   hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, st, redoCount);
-  const int64_t maxGrid = 1ll << 30;
-  for (int64_t a = r.t0(); a < r.t1(); a += maxGrid) {
-    const int64_t cnt = (r.t1() - a < maxGrid) ? (r.t1() - a) : maxGrid;
-    hipLaunchKernelGGL(k_segseg_tiles, dim3((unsigned)cnt), dim3(kTile), 0, st, seg, ld, n, a, K,
-                       rowPart, colPart, sp.off, sp.slot, sp.res, sp.total, r.t0(), sp.base, redo,
-                       redoCount);
-  }
-  // two workgroups per SIMD; with an empty list each reads the count and ends
-  const int64_t blocks = std::min<int64_t>(r.t1() - r.t0(), 2048);
-  hipLaunchKernelGGL(k_segseg_tiles_redo, dim3((unsigned)blocks), dim3(kTile), 0, st, seg, ld, n, K,
-                     rowPart, colPart, sp.off, sp.slot, sp.res, sp.total, r.t0(), sp.base, redo,
-                     redoCount);
+  const int nb = (n + kTile - 1) / kTile;
+  const int64_t u0 = r.u0(), u1 = r.u1();
+  hipLaunchKernelGGL(k_segseg_super, dim3((unsigned)(u1 - u0)), dim3(kSuperThreads), 0, st, seg,
+                     ld, n, nb, u0, K, rowPart, colPart, sp.off, sp.slot, sp.res, sp.total, u0,
+                     sp.base, redo, redoCount);
+  // two workgroups per CU; with an empty list each reads the count and ends
+  const int64_t blocks = std::min<int64_t>(u1 - u0, 512);
+  hipLaunchKernelGGL(k_segseg_super_redo, dim3((unsigned)blocks), dim3(kSuperThreads), 0, st, seg,
+                     ld, n, nb, K, rowPart, colPart, sp.off, sp.slot, sp.res, sp.total, u0,
+                     sp.base, redo, redoCount);
 }
 
 void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, const TileRange& r,
                           double* out, hipStream_t st) {
-  // segments of row blocks below J1 receive partials from this chunk
-  const int64_t m = std::min<int64_t>(n, (int64_t)r.J1 * kTile);
+  // segments of row blocks below J1 (super-blocks below Q1) receive partials
+  // from this chunk
+  const int64_t m = std::min<int64_t>(n, (int64_t)(r.super ? r.Q1() * kSuper : r.J1) * kTile);
   if (m <= 0) return;
   const int bs = 256;
-  hipLaunchKernelGGL(k_segseg_reduce, dim3((unsigned)((m + bs - 1) / bs)), dim3(bs), 0, st,
-                     rowPart, colPart, (int)m, r.J0, r.J1, out);
+  if (r.super)
+    hipLaunchKernelGGL(k_segseg_reduce_super, dim3((unsigned)((m + bs - 1) / bs)), dim3(bs), 0, st,
+                       rowPart, colPart, (int)m, r.Q0(), r.Q1(), out);
+  else
+    hipLaunchKernelGGL(k_segseg_reduce, dim3((unsigned)((m + bs - 1) / bs)), dim3(bs), 0, st,
+                       rowPart, colPart, (int)m, r.J0, r.J1, out);
 }
 
 void launch_ranged_selftest(int which, const double* x, int64_t n, double* y, hipStream_t st) {

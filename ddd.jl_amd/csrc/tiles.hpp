@@ -171,90 +171,6 @@ __device__ __forceinline__ void tile_finish(int64_t t, int lane, v3<double> a1, 
   for (int c = 0; c < 6; ++c) cp[c * kTile + lane] = cacc[c][lane];
 }
 
-// One plain FP64 pair tile: every generic pair runs inline in FP64.
-// Near-parallel pairs (the reference's special branch, :1147-1162, several
-// times the cost of a generic pair) are not evaluated here: they were listed
-// per tile and evaluated beforehand (k_special_pass, k_special_eval), and
-// the tile adds its list's forces, in list order, after its own pairs.
-// Keeping that branch out of the tile lets it run two wavefronts per SIMD.
-// Tile t's partial sums go to slot t - partBase, its list entries are read
-// at e - specBase (the chunk's first tile and first entry).
-//
-// RANGED: the generic pairs through the ranged sequences of
-// fp64_ranged.hpp.  The tile is done only when all 128 segments are in
-// their safe range and every partial sum comes out finite; otherwise it
-// returns false having written nothing, and the caller has it evaluated
-// with the general sequences (the same bits wherever both finish).
-template <bool RANGED>
-__device__ __forceinline__ bool tile_eval(int64_t t, const double* __restrict__ seg, int64_t ld,
-                                          int n, const SegK<double>& K,
-                                          double* __restrict__ rowPart,
-                                          double* __restrict__ colPart,
-                                          const long long* __restrict__ specOff,
-                                          const unsigned short* __restrict__ specSlot,
-                                          const double* __restrict__ specRes, int64_t specTotal,
-                                          int64_t partBase, int64_t specBase) {
-  int I, J;
-  tile_blocks(t, I, J);
-  const int lane = threadIdx.x;
-
-  __shared__ double si[kTileFields][kTile];
-  __shared__ double sj[kTileFields][kTile];
-  __shared__ double cacc[6][kTile];
-
-  __syncthreads();  // a previous tile of this workgroup is done with the arrays
-  bool rowExt;
-  uint64_t colExt;
-  stage_blocks(seg, ld, I, J, lane, si, sj, rowExt, colExt);
-#pragma unroll
-  for (int c = 0; c < 6; ++c) cacc[c][lane] = 0.0;
-  if constexpr (RANGED) atan_table_fill(lane);
-  __syncthreads();
-
-  if constexpr (RANGED) {
-    const int rf[9] = {F_N1X, F_N1Y, F_N1Z, F_N2X, F_N2Y, F_N2Z, F_BX, F_BY, F_BZ};
-    bool ok = true;
-#pragma unroll
-    for (int f = 0; f < 9; ++f) ok = ok && ranged_value(si[rf[f]][lane]) && ranged_value(sj[rf[f]][lane]);
-    const bool aOk = K.aSq >= 0x1p-120 && K.aSq <= 0x1p120;
-    if (__ballot(!ok) != 0 || !aOk) return false;
-  }
-
-  const Seg<double> A = seg_from_lds(si, lane);
-  const int64_t i = (int64_t)I * kTile + lane;
-  const bool rowLive = (i < n) && !rowExt;
-  const bool diag = (I == J);
-  v3<double> a1 = V(0.0, 0.0, 0.0), a2 = V(0.0, 0.0, 0.0);
-  for (int k = 0; k < kTile; ++k) {
-    const int s = (lane + k) & 63;
-    const int64_t j = (int64_t)J * kTile + s;
-    const bool live = rowLive && (j < n) && !((colExt >> s) & 1ull) && (!diag || s > lane);
-    const Seg<double> B = seg_from_lds(sj, s);
-    double c;
-    const double omcSq = one_minus_c2(A.t, B.t, c);
-    if (live && !is_special<0>(true, omcSq, v3<double>{}, B.n1, B.n2, 0.0, 0.0)) {
-      v3<double> F1, F2, F3, F4;
-      pair_generic<double, true, true, false, RANGED>(K, A, B, c, omcSq, F1, F2, F3, F4);
-      a1 = a1 + F1;
-      a2 = a2 + F2;
-      cacc[0][s] += F3.x; cacc[1][s] += F3.y; cacc[2][s] += F3.z;
-      cacc[3][s] += F4.x; cacc[4][s] += F4.y; cacc[5][s] += F4.z;
-    }
-    __syncthreads();
-  }
-  if constexpr (RANGED) {
-    // inf or NaN anywhere in a row or column sum (a sum overflowing to inf
-    // only costs a second evaluation)
-    double sum = a1.x + a1.y + a1.z + a2.x + a2.y + a2.z;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) sum = sum + cacc[c][lane];
-    if (__ballot(!(sum - sum == 0.0)) != 0) return false;
-  }
-  tile_finish(t, lane, a1, a2, cacc, rowPart, colPart, specOff, specSlot, specRes, specTotal,
-              partBase, specBase);
-  return true;
-}
-
 }  // namespace
 
 }  // namespace ddd

@@ -73,7 +73,7 @@ class ddd_opts(C.Structure):
 
 class ddd_stats(C.Structure):
     _fields_ = [("pairs", C.c_uint64), ("pairsFP64", C.c_uint64), ("deviceBytes", C.c_uint64),
-                ("chunks", C.c_uint64)]
+                ("chunks", C.c_uint64), ("tileMs", C.c_double)]
 
 
 class ddd_train_opts(C.Structure):

@@ -536,6 +536,13 @@ class ResidentNetwork:
         _call(self.ctx, "ddd_net_stats", st)
         return int(st.deviceBytes), int(st.chunks)
 
+    def tile_kernel_ms(self):
+        """Time of the last evaluation's pair-tile kernels alone (plain FP64
+        and mixed modes), from HIP events the library records around them."""
+        st = L.ddd_stats()
+        _call(self.ctx, "ddd_net_stats", st)
+        return float(st.tileMs)
+
     def stage(self, stage, with_pk=False):
         _call(self.ctx, "ddd_net_stage", int(stage), 1 if with_pk else 0)
 

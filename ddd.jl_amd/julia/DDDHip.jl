@@ -333,6 +333,7 @@ struct Stats
     pairsFP64::UInt64
     deviceBytes::UInt64
     chunks::UInt64
+    tileMs::Float64
 end
 
 struct TrainOpts
@@ -364,7 +365,7 @@ function calcSegSegForceMode(dlnParams, matParams, network::DislocationNetwork, 
     k = Ref(consts(dlnParams, matParams))
     m = network.numSeg[1]
     out = zeros(3, 2, m)
-    st = Ref(Stats(0, 0, 0, 0))
+    st = Ref(Stats(0, 0, 0, 0, 0.0))
     GC.@preserve network out begin
         nw = Ref(netstruct(network))
         check(ccall((:ddd_segseg_force_opts, libddd), Cint,

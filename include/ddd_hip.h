@@ -121,6 +121,9 @@ typedef struct ddd_stats {
   uint64_t pairsFP64;   /* of which on the FP64 path                            */
   uint64_t deviceBytes; /* device memory the context's own buffers hold         */
   uint64_t chunks;      /* chunks of the all-pairs triangle (0 for the hybrid)  */
+  double tileMs;        /* the all-pairs evaluation's pair-tile kernels alone,
+                           summed over chunks (HIP events on the context's
+                           stream; plain FP64 and mixed modes, else 0)        */
 } ddd_stats;
 
 /* Hybrid far field (no reference counterpart).  A G^3 grid of cubic cells
