@@ -455,8 +455,9 @@ void hybrid_apply(ddd_ctx* c, int64_t n, double* dev, const ddd_opts& o) {
 
 // Device memory of one chunk of the all-pairs triangle, one buffer carved
 // per chunk: row and column partial sums (6 doubles x 64 lanes each per
-// tile; the plain FP64 mode keeps them per super-tile, kSuper rows and
-// kSuper columns of them, counted here as kSuper tiles), the list of tiles or super-tiles
+// tile; the plain FP64 mode on large networks keeps them per super-tile,
+// kSuper rows and kSuper columns of them, counted here as kSuper tiles),
+// the list of tiles or super-tiles
 // the ranged pass leaves to the general one (one int each) and the
 // near-parallel list (slot, tile, 12 forces and a fix-list entry per pair),
 // each part 256-byte aligned.  A chunk takes whole column blocks (whole
@@ -517,15 +518,16 @@ void segseg_tiles(ddd_ctx* c, const ddd_consts& k, int64_t n, unsigned* flags,
                         hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   // chunks of whole column blocks within the budget; the plain FP64 mode
-  // keeps its partials by super-tile (kSuper x kSuper tiles, 24 KiB), so
-  // its chunks take whole super-columns of kSuper column blocks
+  // on networks of kSuperMin super-tiles or more keeps its partials by
+  // super-tile (kSuper x kSuper tiles, 24 KiB), so its chunks take whole
+  // super-columns of kSuper column blocks; smaller networks and the mixed
+  // mode keep them by tile
   std::vector<TileRange> chunks;
-  const int step = mixed ? 1 : kSuper;
-  auto range = [&](int J0, int J1) { return TileRange{J0, J1, !mixed}; };
+  const int step = (!mixed && tiles_for((nb + kSuper - 1) / kSuper) >= kSuperMin) ? kSuper : 1;
+  auto range = [&](int J0, int J1) { return TileRange{J0, J1, step}; };
   auto layout = [&](int J0, int J1) {
     const TileRange q = range(J0, J1);
-    const int64_t tiles = mixed ? q.t1() - q.t0() : (q.u1() - q.u0()) * kSuper;
-    return ChunkLayout(tiles, offAt[J1] - offAt[J0]);
+    return ChunkLayout((q.u1() - q.u0()) * step, offAt[J1] - offAt[J0]);
   };
   int64_t most = 0;
   for (int J0 = 0; J0 < nb;) {

@@ -41,18 +41,20 @@ struct MixedRule {
 // i.e. the tiles t in [tiles_for(J0), tiles_for(J1)) of the column-block-
 // major enumeration, whose partial sums sit at slot t - tiles_for(J0) of the
 // partial buffers and are reduced into segForce before the next chunk.
-// Plain FP64 chunks keep their partial sums by super-tile (kSuper x kSuper
-// tiles, rows and columns of kSuper blocks, enumerated like the tiles):
-// J0 is then a multiple of kSuper and super-tile u of super-columns
-// [J0 / kSuper, ceil(J1 / kSuper)) sits at slot u - u0().
+// Partial sums are kept by super-tile: S x S tiles (rows and columns of S
+// blocks, enumerated like the tiles), S = 1 for one tile each.  J0 is a
+// multiple of S and super-tile u of super-columns [J0 / S, ceil(J1 / S))
+// sits at slot u - u0().  The plain FP64 mode takes S = kSuper on large
+// networks (kSuperMin super-tiles or more), the mixed mode S = 1.
 constexpr int kSuper = 4;
+constexpr int64_t kSuperMin = 16384;
 struct TileRange {
   int J0, J1;
-  bool super = false;
+  int S = 1;
   int64_t t0() const { return (int64_t)J0 * (J0 + 1) / 2; }
   int64_t t1() const { return (int64_t)J1 * (J1 + 1) / 2; }
-  int Q0() const { return J0 / kSuper; }
-  int Q1() const { return (J1 + kSuper - 1) / kSuper; }
+  int Q0() const { return J0 / S; }
+  int Q1() const { return (J1 + S - 1) / S; }
   int64_t u0() const { return tiles_for(Q0()); }
   int64_t u1() const { return tiles_for(Q1()); }
 };

@@ -87,9 +87,9 @@ __global__ __launch_bounds__(kTile) void k_special_eval(const double* __restrict
 
 __global__ void k_zero_count(unsigned long long* c) { *c = 0; }
 
-// Plain FP64 super-tiles: the kSuper x kSuper (4 x 4) pair tiles of row
-// blocks 4P .. 4P + 3 and column blocks 4Q .. 4Q + 3 (P <= Q) in one
-// workgroup of four wavefronts.  Wavefront w keeps row block 4P + w for the
+// Plain FP64 super-tiles: the S x S (4 x 4) pair tiles of row blocks
+// 4P .. 4P + 3 and column blocks 4Q .. 4Q + 3 (P <= Q) in one workgroup of
+// four wavefronts.  Wavefront w keeps row block 4P + w for the
 // whole super-tile (its row sums in registers, its row segment read into
 // registers once per tile) and in round r takes column block
 // 4Q + ((w + r) % 4); in every round the four wavefronts hold four
@@ -107,11 +107,15 @@ __global__ void k_zero_count(unsigned long long* c) { *c = 0; }
 // Super-tile u's partials go to slot u - partBase:
 // rowPart / colPart [slot][block w][component][lane].
 //
+// S = 1 is one tile per single-wavefront workgroup (eight per CU): on small
+// networks four rounds per workgroup would lengthen the critical path
+// fourfold (640 segments: 0.48 ms with one tile per workgroup, 1.38 ms with
+// 4 x 4 super-tiles), so those take S = 1.
+//
 // RANGED: the ranged sequences, done only when all 8 blocks are in the safe
 // range and every sum comes out finite; otherwise the super-tile writes
 // nothing and returns false (the caller has it redone with the general
 // sequences).
-constexpr int kSuperThreads = kSuper * kTile;
 
 // Orders one wavefront's LDS accesses across the steps of a tile: lane l
 // adds into column slot (l + k) & 63 at step k, which lane l - 1 updates
@@ -123,7 +127,7 @@ __device__ __forceinline__ void wave_lds_order() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <bool RANGED>
+template <bool RANGED, int S>
 __device__ __forceinline__ bool super_eval(int64_t u, const double* __restrict__ seg, int64_t ld,
                                            int n, int nb, const SegK<double>& K,
                                            double* __restrict__ rowPart,
@@ -132,18 +136,18 @@ __device__ __forceinline__ bool super_eval(int64_t u, const double* __restrict__
                                            const unsigned short* __restrict__ specSlot,
                                            const double* __restrict__ specRes, int64_t specTotal,
                                            int64_t partBase, int64_t specBase) {
-  __shared__ double si[kSuper][kTileFields][kTile];
-  __shared__ double sj[kSuper][kTileFields][kTile];
-  __shared__ double cacc[kSuper][6][kTile];
-  __shared__ unsigned long long colExtS[kSuper];
+  __shared__ double si[S][kTileFields][kTile];
+  __shared__ double sj[S][kTileFields][kTile];
+  __shared__ double cacc[S][6][kTile];
+  __shared__ unsigned long long colExtS[S];
 
   int P, Q;
   tile_blocks(u, P, Q);
   // the wavefront's index, uniform across it (scalar registers, scalar branches)
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile);
   const int lane = threadIdx.x % kTile;
-  const int I = kSuper * P + w;
-  const int Jw = kSuper * Q + w;
+  const int I = S * P + w;
+  const int Jw = S * Q + w;
   const bool rowValid = I < nb;
 
   __syncthreads();  // a previous super-tile of this workgroup is done with the arrays
@@ -184,9 +188,9 @@ __device__ __forceinline__ bool super_eval(int64_t u, const double* __restrict__
 
   const bool rowLive = rowValid && (i < n) && !rowExt;
   v3<double> a1 = V(0.0, 0.0, 0.0), a2 = V(0.0, 0.0, 0.0);
-  for (int r = 0; r < kSuper; ++r) {
-    const int cb = (w + r) % kSuper;
-    const int J = kSuper * Q + cb;
+  for (int r = 0; r < S; ++r) {
+    const int cb = (w + r) % S;
+    const int J = S * Q + cb;
     if (rowValid && J < nb && I <= J) {  // uniform across the wavefront
       const Seg<double> A0 = seg_from_lds(si[w], lane);
       const double (*sjc)[kTile] = sj[cb];
@@ -214,13 +218,32 @@ __device__ __forceinline__ bool super_eval(int64_t u, const double* __restrict__
         }
         wave_lds_order();
       }
-      // the tile's near-parallel pairs, in list order: 64 entries loaded at
-      // once, one per lane, then taken in turn from the loading lane's
-      // registers, so that the other wavefronts do not wait at the round's
-      // barrier for one memory latency per entry
+      // the tile's near-parallel pairs, in list order.  With several
+      // wavefronts per workgroup, 64 entries are loaded at once, one per
+      // lane, then taken in turn from the loading lane's registers, so that
+      // the other wavefronts do not wait at the round's barrier for one
+      // memory latency per entry; a single-wavefront workgroup (S = 1) reads
+      // them one at a time, which needs fewer registers, and other
+      // workgroups hide the latency.
       const int64_t t = tiles_for(J) + I;
       const long long eb = specOff[t] - specBase, ee = specOff[t + 1] - specBase;
-      for (long long b0 = eb; b0 < ee; b0 += kTile) {
+      if constexpr (S == 1) {
+        for (long long e = eb; e < ee; ++e) {
+          const unsigned qe = specSlot[e];
+          const int rl = qe & 63, sl = (rl + (qe >> 6)) & 63;
+          if (rl == lane) {
+            a1 = a1 + V(specRes[0 * specTotal + e], specRes[1 * specTotal + e],
+                        specRes[2 * specTotal + e]);
+            a2 = a2 + V(specRes[3 * specTotal + e], specRes[4 * specTotal + e],
+                        specRes[5 * specTotal + e]);
+          }
+          if (sl == lane) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) cc[c][lane] += specRes[(6 + c) * specTotal + e];
+          }
+        }
+      }
+      for (long long b0 = eb; S > 1 && b0 < ee; b0 += kTile) {
         const int cnt = (int)(ee - b0 < kTile ? ee - b0 : kTile);
         unsigned q = 0;
         double v[12];
@@ -260,8 +283,8 @@ __device__ __forceinline__ bool super_eval(int64_t u, const double* __restrict__
     for (int c = 0; c < 6; ++c) sum = sum + cacc[w][c][lane];
     if (__syncthreads_or(!(sum - sum == 0.0))) return false;
   }
-  double* rp = rowPart + ((u - partBase) * kSuper + w) * 6 * kTile + lane;
-  double* cp = colPart + ((u - partBase) * kSuper + w) * 6 * kTile + lane;
+  double* rp = rowPart + ((u - partBase) * S + w) * 6 * kTile + lane;
+  double* cp = colPart + ((u - partBase) * S + w) * 6 * kTile + lane;
   rp[0 * kTile] = a1.x; rp[1 * kTile] = a1.y; rp[2 * kTile] = a1.z;
   rp[3 * kTile] = a2.x; rp[4 * kTile] = a2.y; rp[5 * kTile] = a2.z;
 #pragma unroll
@@ -272,21 +295,23 @@ __device__ __forceinline__ bool super_eval(int64_t u, const double* __restrict__
 // Plain FP64 super-tiles with the ranged sequences, one per workgroup
 // (u0 + blockIdx.x); those they leave undone are listed in
 // redo[0 .. *redoCount) for k_segseg_super_redo.
-__global__ __launch_bounds__(kSuperThreads, 2) void k_segseg_super(
+template <int S>
+__global__ __launch_bounds__(S * kTile, 2) void k_segseg_super(
     const double* __restrict__ seg, int64_t ld, int n, int nb, int64_t u0, SegK<double> K,
     double* __restrict__ rowPart, double* __restrict__ colPart,
     const long long* __restrict__ specOff, const unsigned short* __restrict__ specSlot,
     const double* __restrict__ specRes, int64_t specTotal, int64_t partBase, int64_t specBase,
     int* __restrict__ redo, unsigned long long* __restrict__ redoCount) {
   const int64_t u = u0 + blockIdx.x;
-  const bool done = super_eval<true>(u, seg, ld, n, nb, K, rowPart, colPart, specOff, specSlot,
-                                     specRes, specTotal, partBase, specBase);
+  const bool done = super_eval<true, S>(u, seg, ld, n, nb, K, rowPart, colPart, specOff, specSlot,
+                                        specRes, specTotal, partBase, specBase);
   if (!done && threadIdx.x == 0) redo[atomicAdd(redoCount, 1ull)] = (int)u;
 }
 
 // The super-tiles k_segseg_super left undone, with the general IEEE
 // sequences: grid-stride over the list (its length is read on the device).
-__global__ __launch_bounds__(kSuperThreads, 2) void k_segseg_super_redo(
+template <int S>
+__global__ __launch_bounds__(S * kTile, 2) void k_segseg_super_redo(
     const double* __restrict__ seg, int64_t ld, int n, int nb, SegK<double> K,
     double* __restrict__ rowPart, double* __restrict__ colPart,
     const long long* __restrict__ specOff, const unsigned short* __restrict__ specSlot,
@@ -294,33 +319,36 @@ __global__ __launch_bounds__(kSuperThreads, 2) void k_segseg_super_redo(
     const int* __restrict__ redo, const unsigned long long* __restrict__ redoCount) {
   const int64_t cnt = (int64_t)*redoCount;
   for (int64_t q = blockIdx.x; q < cnt; q += gridDim.x)
-    (void)super_eval<false>(redo[q], seg, ld, n, nb, K, rowPart, colPart, specOff, specSlot,
-                            specRes, specTotal, partBase, specBase);
+    (void)super_eval<false, S>(redo[q], seg, ld, n, nb, K, rowPart, colPart, specOff, specSlot,
+                               specRes, specTotal, partBase, specBase);
 }
 
-// Sum of a segment's super-tile partials of one chunk (super-columns
-// [Q0, Q1), super-tile u at slot u - tiles_for(Q0)), in fixed order: its
-// row role in super-tiles (Pb, Q), Q in [max(Pb, Q0), Q1), then its column
-// role in super-tiles (P, Pb), P <= Pb, when Pb is in the chunk (Pb its
-// block's super-block, w its block's place in it).
-__global__ void k_segseg_reduce_super(const double* __restrict__ rowPart,
-                                      const double* __restrict__ colPart, int n, int Q0, int Q1,
-                                      double* __restrict__ out) {
+// Sum of a segment's partials of one chunk (super-tiles of S x S tiles,
+// super-columns [Q0, Q1), super-tile u at slot u - tiles_for(Q0); S = 1:
+// one tile each), in fixed order: its row role in super-tiles (Pb, Q), Q in
+// [max(Pb, Q0), Q1), then its column role in super-tiles (P, Pb), P <= Pb,
+// when Pb is in the chunk (Pb its block's super-block, w its block's place
+// in it).  With one chunk this is every row partial, then every column
+// partial of the segment.
+template <int S>
+__global__ void k_segseg_reduce(const double* __restrict__ rowPart,
+                                const double* __restrict__ colPart, int n, int Q0, int Q1,
+                                double* __restrict__ out) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int B = s / kTile, l = s % kTile;
-  const int Pb = B / kSuper, w = B % kSuper;
+  const int Pb = B / S, w = B % S;
   const int64_t base = tiles_for(Q0);
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int Q = Pb > Q0 ? Pb : Q0; Q < Q1; ++Q) {
-    const double* p = rowPart + ((tiles_for(Q) + Pb - base) * kSuper + w) * 6 * kTile + l;
+    const double* p = rowPart + ((tiles_for(Q) + Pb - base) * S + w) * 6 * kTile + l;
 #pragma unroll
     for (int c = 0; c < 6; ++c) acc[c] += p[c * kTile];
   }
   if (Pb >= Q0 && Pb < Q1) {
     const int64_t u0 = tiles_for(Pb) - base;
     for (int P = 0; P <= Pb; ++P) {
-      const double* p = colPart + ((u0 + P) * kSuper + w) * 6 * kTile + l;
+      const double* p = colPart + ((u0 + P) * S + w) * 6 * kTile + l;
 #pragma unroll
       for (int c = 0; c < 6; ++c) acc[c] += p[c * kTile];
     }
@@ -437,36 +465,6 @@ __global__ __launch_bounds__(256) void k_scan_final(const unsigned* __restrict__
     acc += v[q];
   }
   if (b == 0 && tid == 0) off[n] = boff[nb];
-}
-
-// Sum of a segment's partials of one chunk (column blocks [J0, J1), tile t
-// at slot t - tiles_for(J0)): row role in tiles (B, J), J in [max(B, J0),
-// J1), then column role in tiles (I, B), I <= B, when B is in the chunk.
-// With one chunk (J0 = 0, J1 = nb) this is every row partial, then every
-// column partial of the segment.
-__global__ void k_segseg_reduce(const double* __restrict__ rowPart,
-                                const double* __restrict__ colPart, int n, int J0, int J1,
-                                double* __restrict__ out) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const int B = s / kTile, l = s % kTile;
-  const int64_t base = tiles_for(J0);
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int J = B > J0 ? B : J0; J < J1; ++J) {
-    const double* p = rowPart + (tiles_for(J) + B - base) * 6 * kTile + l;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) acc[c] += p[c * kTile];
-  }
-  if (B >= J0 && B < J1) {
-    const int64_t t0 = tiles_for(B) - base;
-    for (int I = 0; I <= B; ++I) {
-      const double* p = colPart + (t0 + I) * 6 * kTile + l;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) acc[c] += p[c * kTile];
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 6; ++c) out[6 * (int64_t)s + c] += acc[c];
 }
 
 // The list offsets at column-block boundaries: offAt[k] = off[tiles_for(k)].
@@ -604,29 +602,37 @@ void launch_segseg_tiles(const double* seg, int64_t ld, int n, const SegK<double
   hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, st, redoCount);
   const int nb = (n + kTile - 1) / kTile;
   const int64_t u0 = r.u0(), u1 = r.u1();
-  hipLaunchKernelGGL(k_segseg_super, dim3((unsigned)(u1 - u0)), dim3(kSuperThreads), 0, st, seg,
-                     ld, n, nb, u0, K, rowPart, colPart, sp.off, sp.slot, sp.res, sp.total, u0,
-                     sp.base, redo, redoCount);
-  // two workgroups per CU; with an empty list each reads the count and ends
-  const int64_t blocks = std::min<int64_t>(u1 - u0, 512);
-  hipLaunchKernelGGL(k_segseg_super_redo, dim3((unsigned)blocks), dim3(kSuperThreads), 0, st, seg,
-                     ld, n, nb, K, rowPart, colPart, sp.off, sp.slot, sp.res, sp.total, u0,
-                     sp.base, redo, redoCount);
+  // with an empty list each redo workgroup reads the count and ends
+  const int64_t blocks = std::min<int64_t>(u1 - u0, 2048);
+  if (r.S == kSuper) {
+    hipLaunchKernelGGL(k_segseg_super<kSuper>, dim3((unsigned)(u1 - u0)), dim3(kSuper * kTile), 0,
+                       st, seg, ld, n, nb, u0, K, rowPart, colPart, sp.off, sp.slot, sp.res,
+                       sp.total, u0, sp.base, redo, redoCount);
+    hipLaunchKernelGGL(k_segseg_super_redo<kSuper>, dim3((unsigned)blocks), dim3(kSuper * kTile), 0,
+                       st, seg, ld, n, nb, K, rowPart, colPart, sp.off, sp.slot, sp.res, sp.total,
+                       u0, sp.base, redo, redoCount);
+  } else {
+    hipLaunchKernelGGL(k_segseg_super<1>, dim3((unsigned)(u1 - u0)), dim3(kTile), 0, st, seg, ld,
+                       n, nb, u0, K, rowPart, colPart, sp.off, sp.slot, sp.res, sp.total, u0,
+                       sp.base, redo, redoCount);
+    hipLaunchKernelGGL(k_segseg_super_redo<1>, dim3((unsigned)blocks), dim3(kTile), 0, st, seg, ld,
+                       n, nb, K, rowPart, colPart, sp.off, sp.slot, sp.res, sp.total, u0, sp.base,
+                       redo, redoCount);
+  }
 }
 
 void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, const TileRange& r,
                           double* out, hipStream_t st) {
-  // segments of row blocks below J1 (super-blocks below Q1) receive partials
-  // from this chunk
-  const int64_t m = std::min<int64_t>(n, (int64_t)(r.super ? r.Q1() * kSuper : r.J1) * kTile);
+  // segments of row blocks below Q1 S receive partials from this chunk
+  const int64_t m = std::min<int64_t>(n, (int64_t)r.Q1() * r.S * kTile);
   if (m <= 0) return;
   const int bs = 256;
-  if (r.super)
-    hipLaunchKernelGGL(k_segseg_reduce_super, dim3((unsigned)((m + bs - 1) / bs)), dim3(bs), 0, st,
+  if (r.S == kSuper)
+    hipLaunchKernelGGL(k_segseg_reduce<kSuper>, dim3((unsigned)((m + bs - 1) / bs)), dim3(bs), 0, st,
                        rowPart, colPart, (int)m, r.Q0(), r.Q1(), out);
   else
-    hipLaunchKernelGGL(k_segseg_reduce, dim3((unsigned)((m + bs - 1) / bs)), dim3(bs), 0, st,
-                       rowPart, colPart, (int)m, r.J0, r.J1, out);
+    hipLaunchKernelGGL(k_segseg_reduce<1>, dim3((unsigned)((m + bs - 1) / bs)), dim3(bs), 0, st,
+                       rowPart, colPart, (int)m, r.Q0(), r.Q1(), out);
 }
 
 void launch_ranged_selftest(int which, const double* x, int64_t n, double* y, hipStream_t st) {
