@@ -410,6 +410,35 @@ def collision_leg(args, net, ctx, log):
             "cpu_oracle_ms_scaled": round(cpu, 1), "cpu_sample": f"{ns} segments, 1 thread"}
 
 
+def rows_leg(args, dln, mat, net, ctx, log):
+    """calcSegSegForce with idx (calcSegSegForceIdx!, DislocationForces.jl:
+    831-877: the rows a topology change re-evaluates, each against every
+    other segment): wall time of the synchronous ABI call, network upload
+    included, for 1, 16, 256 and 4,096 random rows of the benchmark network;
+    the CPU oracle (one thread) on the same single row."""
+    import numpy as np
+    from dddhip import forces as G
+    import pyoracle as O
+    N = int(net.numSeg[0])
+    rng = np.random.default_rng(0x1D)
+    out = {"segments": N}
+    for m in (1, 16, 256, 4096):
+        idx = rng.integers(1, N + 1, m)
+        G.calcSegSegForce(dln, mat, net, idx, ctx=ctx)
+        t = []
+        for _ in range(max(3, args.steps)):
+            t0 = time.perf_counter()
+            G.calcSegSegForce(dln, mat, net, idx, ctx=ctx)
+            t.append((time.perf_counter() - t0) * 1e3)
+        out[f"rows_{m}_ms"] = round(float(np.median(t)), 3)
+        log(f"idx rows: {m} of {N}: {out[f'rows_{m}_ms']:.3f} ms per call")
+    idx = rng.integers(1, N + 1, 1)
+    t0 = time.perf_counter()
+    O.calcSegSegForce(dln, mat, net, idx)
+    out["cpu_oracle_1_row_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    return out
+
+
 def big_leg(args, dln, mat, ctx, log):
     """Plain FP64 on a 4x larger network (16,384 loops x 32 = 524,288
     segments, seed 0xF0): one resident evaluation after one untimed, with
@@ -657,6 +686,8 @@ def main():
     ap.add_argument("--big-loops", type=int, default=16384,
                     help="plain FP64 leg on a network of this many loops (seed 0xF0, 524,288 "
                          "segments by default): time, chunks, device memory; 0 to skip")
+    ap.add_argument("--no-rows", dest="rows", action="store_false",
+                    help="skip the idx-rows leg (calcSegSegForceIdx!)")
     ap.add_argument("--no-collision", dest="collision", action="store_false",
                     help="skip the detectCollision leg")
     ap.add_argument("--no-dynamics", dest="dynamics", action="store_false",
@@ -799,6 +830,7 @@ def main():
     clustered = clustered_leg(args, dln, mat, ctx, log) if args.clustered else None
     utilde = utilde_leg(args, mat, net, ctx, log) if args.utilde_grid > 0 else None
     collision = collision_leg(args, net, ctx, log) if args.collision else None
+    rows = rows_leg(args, dln, mat, net, ctx, log) if args.rows else None
     big = big_leg(args, dln, mat, ctx, log) if args.big_loops > 0 else None
     # replaces the context's resident network: after every leg that uses it
     small = (small_configs_leg(args, dln, mat, ctx, log, max(1, min(16, os.cpu_count() or 1)))
@@ -837,6 +869,7 @@ def main():
         "clustered": clustered,
         "u_tilde": utilde,
         "detect_collision": collision,
+        "segseg_rows": rows,
         "plain_fp64_524288": big,
         "plain_fp64_small": small,
         "roofline": {

@@ -136,7 +136,7 @@ struct ddd_ctx {
   std::mutex mu;
   std::string err;
   // segment SoA and pair-kernel scratch
-  DevBuf seg, force, rows, out, flags;
+  DevBuf seg, force, rows, out, flags, rowPart;
   // mesh
   DevBuf meshC, meshCoord, meshConn, uHat;
   // field-point stress
@@ -604,6 +604,15 @@ void segseg_all(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsigne
   segseg_finish(c, n, dev);
 }
 
+// Rows-only seg-seg forces of the rows dr (nIdx of them) added into dev.
+void launch_rows(ddd_ctx* c, int64_t n, const int64_t* dr, int64_t nIdx, const ddd_consts& k,
+                 double* dev, unsigned* fl) {
+  const int nChunk = segseg_rows_chunks((int)n, (int)nIdx);
+  if (nChunk > 1) c->rowPart.ensure(sizeof(double) * 6 * (size_t)nChunk * (size_t)nIdx);
+  launch_segseg_rows(c->seg.as<double>(), c->resLd, (int)n, dr, (int)nIdx, seg_consts(k), dev,
+                     nChunk > 1 ? c->rowPart.as<double>() : nullptr, nChunk, fl, c->st);
+}
+
 // Copy a 6 x m device result into the caller's 3 x 2 x outCols array.
 void store6(ddd_ctx* c, const double* dev, int64_t m, const std::vector<int64_t>* rows,
             double* out, int64_t outCols, int flags) {
@@ -733,8 +742,7 @@ int ddd_segseg_force(ddd_ctx* c, const ddd_consts* k, const ddd_network* net, co
       check_out(out, outCols, (flags & DDD_OUT_BY_POSITION) ? nIdx : 0);
       int64_t* dr = upload_rows(c, r);
       double* dev = zeroed(c, c->out, 6 * nIdx);
-      launch_segseg_rows(c->seg.as<double>(), c->resLd, (int)n, dr, (int)nIdx, seg_consts(*k), dev,
-                         fl, c->st);
+      launch_rows(c, n, dr, nIdx, *k, dev, fl);
       HIPCHK(hipGetLastError());
       check_flags(c);
       store6(c, dev, nIdx, &r, out, outCols, flags);
@@ -867,8 +875,7 @@ int ddd_seg_force(ddd_ctx* c, const ddd_consts* k, const ddd_network* net, const
       double* dev = zeroed(c, c->out, 6 * nIdx);
       if (mesh) launch_pk_force(c->seg.as<double>(), c->resLd, dr, (int)nIdx, M, dev, 1, c->st);
       launch_self_force(c->seg.as<double>(), c->resLd, dr, (int)nIdx, self_consts(*k), dev, c->st);
-      launch_segseg_rows(c->seg.as<double>(), c->resLd, (int)n, dr, (int)nIdx, seg_consts(*k), dev,
-                         fl, c->st);
+      launch_rows(c, n, dr, nIdx, *k, dev, fl);
       HIPCHK(hipGetLastError());
       check_flags(c);
       store6(c, dev, nIdx, &r, out, outCols, flags);

@@ -120,8 +120,13 @@ void launch_segseg_tiles_mixed(const double* seg, int64_t ld, int n, const SegK<
 // I <= B, if B lies in [J0, J1) -- the fixed order of every chunk.
 void launch_segseg_reduce(const double* rowPart, const double* colPart, int n, const TileRange& r,
                           double* out, hipStream_t st);
+// Rows-only forces added into out (6 x nRows): each row split over nChunk
+// column chunks (segseg_rows_chunks), part holds 6 x nChunk x nRows doubles
+// (unused with one chunk).
+int segseg_rows_chunks(int n, int nRows);
 void launch_segseg_rows(const double* seg, int64_t ld, int n, const int64_t* rows, int nRows,
-                        const SegK<double>& K, double* out, unsigned* flags, hipStream_t st);
+                        const SegK<double>& K, double* out, double* part, int nChunk,
+                        unsigned* flags, hipStream_t st);
 // The tile kernel's ranged log / atan / sqrt / division on given arguments
 // (which 0-3; division takes x[2i] / x[2i+1]), for bitwise tests.
 void launch_ranged_selftest(int which, const double* x, int64_t n, double* y, hipStream_t st);

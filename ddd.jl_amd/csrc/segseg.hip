@@ -474,20 +474,29 @@ __global__ void k_block_offsets(const long long* __restrict__ off, int nb,
   if (k <= nb) offAt[k] = off[tiles_for(k)];
 }
 
-// Rows-only forces (calcSegSegForceIdx!, :831-877): row r = rows[w] against
-// every other live segment, with the row segment as segment 1; one
-// wavefront per row, reduced across lanes in a fixed butterfly order.
-__global__ __launch_bounds__(kTile) void k_segseg_rows(const double* __restrict__ seg, int64_t ld,
+// Rows-only forces (calcSegSegForceIdx!, :831-877): row r = rows[blockIdx.x]
+// against the live segments of column chunk blockIdx.y ([y chunk, (y + 1)
+// chunk)), with the row segment as segment 1; one wavefront per (row,
+// chunk), its lanes striding over the chunk, reduced across lanes in a fixed
+// butterfly order into part[row][chunk][6] (or, with one chunk and part
+// NULL, added into out[6 row .. 6 row + 5]).  Splitting the row over chunks
+// keeps the latency of a short idx list (a topology update re-evaluates a
+// few rows) at a few pairs per lane instead of n / 64.
+__global__ __launch_bounds__(kTile, 2) void k_segseg_rows(const double* __restrict__ seg, int64_t ld,
                                                        int n, const int64_t* __restrict__ rows,
-                                                       SegK<double> K, double* __restrict__ out,
+                                                       int chunk, SegK<double> K,
+                                                       double* __restrict__ part,
+                                                       double* __restrict__ out,
                                                        unsigned* __restrict__ flags) {
-  const int w = blockIdx.x;
+  const int64_t w = blockIdx.x;
   const int lane = threadIdx.x;
   const int64_t r = rows[w];
+  const int64_t j0 = (int64_t)blockIdx.y * chunk;
+  const int64_t j1 = j0 + chunk < n ? j0 + chunk : n;
   double acc[6] = {0, 0, 0, 0, 0, 0};
   if (r >= 0 && r < n && seg[F_EXT * ld + r] == 0.0) {
     const Seg<double> A = seg_from(seg, ld, r);
-    for (int64_t j = lane; j < n; j += kTile) {
+    for (int64_t j = j0 + lane; j < j1; j += kTile) {
       if (j == r || seg[F_EXT * ld + j] != 0.0) continue;
       const Seg<double> B = seg_from(seg, ld, j);
       double c;
@@ -509,9 +518,26 @@ __global__ __launch_bounds__(kTile) void k_segseg_rows(const double* __restrict_
     acc[c] = v;
   }
   if (lane == 0) {
+    if (part) {
+      double* p = part + (w * gridDim.y + blockIdx.y) * 6;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) out[6 * (int64_t)w + c] += acc[c];
+      for (int c = 0; c < 6; ++c) p[c] = acc[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) out[6 * w + c] += acc[c];
+    }
   }
+}
+
+// out[6 w + c] += the row's chunk partials, in chunk order.
+__global__ void k_segseg_rows_sum(const double* __restrict__ part, int nRows, int nChunk,
+                                  double* __restrict__ out) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nRows * 6) return;
+  const int64_t w = e / 6, c = e % 6;
+  double a = 0.0;
+  for (int q = 0; q < nChunk; ++q) a += part[(w * nChunk + q) * 6 + c];
+  out[e] += a;
 }
 
 // The tile kernel's ranged sequences on given arguments, for tests that
@@ -641,11 +667,29 @@ void launch_ranged_selftest(int which, const double* x, int64_t n, double* y, hi
   hipLaunchKernelGGL(k_ranged_selftest, dim3((unsigned)blocks), dim3(kTile), 0, st, which, x, n, y);
 }
 
+int segseg_rows_chunks(int n, int nRows) {
+  if (nRows <= 0 || n <= 0) return 1;
+  // about 4,096 wavefronts in all, but at least 256 columns (4 per lane) each
+  const int64_t want = (4096 + nRows - 1) / nRows;
+  const int64_t most = ((int64_t)n + 255) / 256;
+  return (int)std::max<int64_t>(1, std::min(want, most));
+}
+
 void launch_segseg_rows(const double* seg, int64_t ld, int n, const int64_t* rows, int nRows,
-                        const SegK<double>& K, double* out, unsigned* flags, hipStream_t st) {
+                        const SegK<double>& K, double* out, double* part, int nChunk,
+                        unsigned* flags, hipStream_t st) {
   if (nRows <= 0) return;
-  hipLaunchKernelGGL(k_segseg_rows, dim3((unsigned)nRows), dim3(kTile), 0, st, seg, ld, n, rows, K,
-                     out, flags);
+  const int chunk = (int)((((int64_t)n + nChunk - 1) / nChunk + kTile - 1) / kTile * kTile);
+  if (nChunk == 1) {
+    hipLaunchKernelGGL(k_segseg_rows, dim3((unsigned)nRows, 1u), dim3(kTile), 0, st, seg, ld, n, rows,
+                       chunk, K, nullptr, out, flags);
+    return;
+  }
+  hipLaunchKernelGGL(k_segseg_rows, dim3((unsigned)nRows, (unsigned)nChunk), dim3(kTile), 0, st, seg,
+                     ld, n, rows, chunk, K, part, out, flags);
+  const int64_t m = (int64_t)nRows * 6;
+  hipLaunchKernelGGL(k_segseg_rows_sum, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, part,
+                     nRows, nChunk, out);
 }
 
 }  // namespace ddd

@@ -196,6 +196,27 @@ def test_idx_rows(prm, netA):
     assert per_node_rel(t, ot)[0] <= TOL
 
 
+def test_idx_rows_chunked(prm, netA):
+    """Rows split over column chunks (one row: 64 chunks at 16,384 segments;
+    100 rows: 41 chunks) and, with 4,096 rows or more, one chunk per row
+    added straight into the output; duplicates as rand(1:n, m) gives."""
+    dln, mat = prm
+    net = random_loop_network(512, seed=0xB0)
+    n = int(net.numSeg[0])
+    rng = np.random.default_rng(3)
+    for m in (1, 100):
+        idx = rng.integers(1, n + 1, m)
+        _log(f"idx rows: {m} of {n} segments")
+        g = G.calcSegSegForce(dln, mat, net, idx)
+        o = O.calcSegSegForce(dln, mat, net, idx)
+        assert per_node_rel(g, o)[0] <= TOL
+    idx = rng.integers(1, int(netA.numSeg[0]) + 1, 5000)
+    g = G.calcSegSegForce(dln, mat, netA, idx)
+    o = O.calcSegSegForce(dln, mat, netA, idx)
+    assert per_node_rel(g, o)[0] <= TOL
+    assert np.array_equal(g, G.calcSegSegForce(dln, mat, netA, idx))
+
+
 def test_external_segments(prm):
     dln, mat = prm
     net = random_loop_network(12, seed=0xA2, ext_fraction=0.15)
