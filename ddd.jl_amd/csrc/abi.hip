@@ -96,9 +96,9 @@ struct ddd_farfield {
   }
 };
 
-// Split-K workspace of the weight-gradient products (floats): 4 partials of
-// the largest one (kHidden x kMlpOutPad).
-constexpr int64_t kGemmWork = 4 * (int64_t)kHidden * kMlpOutPad;
+// Split-K workspace of the MLP products (floats): 8 partials of the largest
+// split output (kHidden x kMlpOutPad).
+constexpr int64_t kGemmWork = 8 * (int64_t)kHidden * kMlpOutPad;
 
 struct ddd_mlp {
   DevBuf P, G, Mm, Vv, Wb;
@@ -106,23 +106,22 @@ struct ddd_mlp {
   int cubic = 1;   // learns W at canonical offsets, expanded by cubic symmetry
   int64_t step = 0;
   int cap = 0;
-  DevBuf X0, Z1, Z2, Z3, H1, H2, H3, Y, dY, dYb, dH, dZb, lossPer, off3, colPart, gemmWork;
+  DevBuf X0, Z1, Z2, Z3, H1, H2, H3, Y, dYb, dH, dZb, lossPer, off3, colPart, gemmWork;
   void ensure_batch(int ns) {
     if (ns <= cap) return;
     X0.ensure(sizeof(unsigned short) * ns * kMlpIn);
     for (DevBuf* b : {&Z1, &Z2, &Z3, &dH}) b->ensure(sizeof(float) * ns * kHidden);
     for (DevBuf* b : {&H1, &H2, &H3, &dZb}) b->ensure(sizeof(unsigned short) * ns * kHidden);
     Y.ensure(sizeof(float) * (int64_t)ns * kMlpOutPad);
-    dY.ensure(sizeof(float) * (int64_t)ns * kMlpOutPad);
     dYb.ensure(sizeof(unsigned short) * (int64_t)ns * kMlpOutPad);
     lossPer.ensure(sizeof(float) * ns);
     off3.ensure(sizeof(int) * 3 * ns);
-    colPart.ensure(sizeof(float) * 64 * kMlpOutPad);
+    colPart.ensure(sizeof(float) * mlp_colsum_part_floats(ns, kMlpOutPad));
     gemmWork.ensure(sizeof(float) * kGemmWork);
     cap = ns;
   }
   void release() {
-    DevBuf* all[] = {&P, &G, &Mm, &Vv, &Wb, &X0, &Z1, &Z2, &Z3, &H1, &H2, &H3, &Y, &dY, &dYb,
+    DevBuf* all[] = {&P, &G, &Mm, &Vv, &Wb, &X0, &Z1, &Z2, &Z3, &H1, &H2, &H3, &Y, &dYb,
                      &dH, &dZb, &lossPer, &off3, &colPart, &gemmWork};
     for (DevBuf* b : all) b->release();
     cap = 0;
@@ -1769,50 +1768,54 @@ const unsigned short* wb(ddd_mlp* m, int64_t off) { return m->Wb.as<unsigned sho
 const float* pp(ddd_mlp* m, int64_t off) { return m->P.as<float>() + off; }
 
 // Y = MLP(features of off3) for ns samples (off3 already on the device).
+// Each product's epilogue adds the layer's bias and writes the activation.
 void mlp_forward(ddd_mlp* m, int ns, hipStream_t st) {
   float* wk = m->gemmWork.as<float>();
+  auto layer = [&](int64_t offB, unsigned short* H) {
+    MlpEpilogue ep;
+    ep.bias = pp(m, offB);
+    ep.act = 1;
+    ep.H = H;
+    return ep;
+  };
   mlp_features(m->off3.as<int>(), ns, m->aH, m->X0.as<unsigned short>(), st);
   mlp_gemm(m->X0.as<unsigned short>(), kMlpIn, 1, wb(m, kOffW1), kHidden, 1, m->Z1.as<float>(),
-           kHidden, ns, kHidden, kMlpIn, 0, st, wk, kGemmWork);
-  mlp_bias_act(m->Z1.as<float>(), pp(m, kOffB1), ns, kHidden, 1, m->H1.as<unsigned short>(), st);
+           kHidden, ns, kHidden, kMlpIn, 0, st, wk, kGemmWork,
+           layer(kOffB1, m->H1.as<unsigned short>()));
   mlp_gemm(m->H1.as<unsigned short>(), kHidden, 1, wb(m, kOffW2), kHidden, 1, m->Z2.as<float>(),
-           kHidden, ns, kHidden, kHidden, 0, st, wk, kGemmWork);
-  mlp_bias_act(m->Z2.as<float>(), pp(m, kOffB2), ns, kHidden, 1, m->H2.as<unsigned short>(), st);
+           kHidden, ns, kHidden, kHidden, 0, st, wk, kGemmWork,
+           layer(kOffB2, m->H2.as<unsigned short>()));
   mlp_gemm(m->H2.as<unsigned short>(), kHidden, 1, wb(m, kOffW3), kHidden, 1, m->Z3.as<float>(),
-           kHidden, ns, kHidden, kHidden, 0, st, wk, kGemmWork);
-  mlp_bias_act(m->Z3.as<float>(), pp(m, kOffB3), ns, kHidden, 1, m->H3.as<unsigned short>(), st);
+           kHidden, ns, kHidden, kHidden, 0, st, wk, kGemmWork,
+           layer(kOffB3, m->H3.as<unsigned short>()));
   mlp_gemm(m->H3.as<unsigned short>(), kHidden, 1, wb(m, kOffW4), kMlpOutPad, 1, m->Y.as<float>(),
-           kMlpOutPad, ns, kMlpOutPad, kHidden, 0, st, wk, kGemmWork);
-  mlp_bias_act(m->Y.as<float>(), pp(m, kOffB4), ns, kMlpOutPad, 0, nullptr, st);
+           kMlpOutPad, ns, kMlpOutPad, kHidden, 0, st, wk, kGemmWork, layer(kOffB4, nullptr));
 }
 
-// Gradient of the loss into m->G from dY (fp32, ns x kMlpOutPad).
+// Gradient of the loss into m->G from dY (bf16, ns x kMlpOutPad, in m->dYb).
 void mlp_backward(ddd_mlp* m, int ns, hipStream_t st) {
   float* G = m->G.as<float>();
   float* wk = m->gemmWork.as<float>();
+  float* part = m->colPart.as<float>();
   unsigned short* dYb = m->dYb.as<unsigned short>();
   unsigned short* dZ = m->dZb.as<unsigned short>();
   float* dH = m->dH.as<float>();
-  mlp_to_bf16(m->dY.as<float>(), (int64_t)ns * kMlpOutPad, dYb, st);
   mlp_gemm(m->H3.as<unsigned short>(), 1, kHidden, dYb, kMlpOutPad, 1, G + kOffW4, kMlpOutPad,
            kHidden, kMlpOutPad, ns, 0, st, wk, kGemmWork);
-  mlp_colsum(dYb, ns, kMlpOutPad, m->colPart.as<float>(), G + kOffB4, st);
+  mlp_colsum(dYb, ns, kMlpOutPad, part, G + kOffB4, st);
   mlp_gemm(dYb, kMlpOutPad, 1, wb(m, kOffW4), 1, kMlpOutPad, dH, kHidden, ns, kHidden, kMlpOutPad,
            0, st, wk, kGemmWork);
-  mlp_act_grad(dH, m->Z3.as<float>(), (int64_t)ns * kHidden, dZ, st);
+  mlp_act_grad_colsum(dH, m->Z3.as<float>(), ns, kHidden, dZ, part, G + kOffB3, st);
   mlp_gemm(m->H2.as<unsigned short>(), 1, kHidden, dZ, kHidden, 1, G + kOffW3, kHidden, kHidden,
            kHidden, ns, 0, st, wk, kGemmWork);
-  mlp_colsum(dZ, ns, kHidden, m->colPart.as<float>(), G + kOffB3, st);
   mlp_gemm(dZ, kHidden, 1, wb(m, kOffW3), 1, kHidden, dH, kHidden, ns, kHidden, kHidden, 0, st, wk, kGemmWork);
-  mlp_act_grad(dH, m->Z2.as<float>(), (int64_t)ns * kHidden, dZ, st);
+  mlp_act_grad_colsum(dH, m->Z2.as<float>(), ns, kHidden, dZ, part, G + kOffB2, st);
   mlp_gemm(m->H1.as<unsigned short>(), 1, kHidden, dZ, kHidden, 1, G + kOffW2, kHidden, kHidden,
            kHidden, ns, 0, st, wk, kGemmWork);
-  mlp_colsum(dZ, ns, kHidden, m->colPart.as<float>(), G + kOffB2, st);
   mlp_gemm(dZ, kHidden, 1, wb(m, kOffW2), 1, kHidden, dH, kHidden, ns, kHidden, kHidden, 0, st, wk, kGemmWork);
-  mlp_act_grad(dH, m->Z1.as<float>(), (int64_t)ns * kHidden, dZ, st);
+  mlp_act_grad_colsum(dH, m->Z1.as<float>(), ns, kHidden, dZ, part, G + kOffB1, st);
   mlp_gemm(m->X0.as<unsigned short>(), 1, kMlpIn, dZ, kHidden, 1, G + kOffW1, kHidden, kMlpIn,
            kHidden, ns, 0, st, wk, kGemmWork);
-  mlp_colsum(dZ, ns, kHidden, m->colPart.as<float>(), G + kOffB1, st);
 }
 
 void mlp_refresh_bf16(ddd_mlp* m, hipStream_t st) {
@@ -1950,7 +1953,7 @@ int ddd_mlp_loss_grad(ddd_ctx* c, ddd_mlp* m, int ns, const int* off3, const flo
     HIPCHK(hipMemcpyAsync(dD.p, den, sizeof(float) * ns, hipMemcpyHostToDevice, c->st));
     mlp_forward(m, ns, c->st);
     mlp_contract(m->Y.as<float>(), m->off3.as<int>(), dM.as<float>(), dL.as<float>(), dD.as<float>(),
-                 ns, envScale, m->dY.as<float>(), m->lossPer.as<float>(), c->st);
+                 ns, envScale, m->dYb.as<unsigned short>(), m->lossPer.as<float>(), c->st);
     mlp_backward(m, ns, c->st);
     std::vector<float> lp((size_t)ns);
     HIPCHK(hipMemcpyAsync(lp.data(), m->lossPer.p, sizeof(float) * ns, hipMemcpyDeviceToHost, c->st));
@@ -2149,15 +2152,15 @@ int ddd_farfield_train(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_const
       cursor += B;
       mlp_forward(m, B, c->st);
       mlp_contract(m->Y.as<float>(), m->off3.as<int>(), bM.as<float>(), bL.as<float>(), bD.as<float>(),
-                   B, es, m->dY.as<float>(), m->lossPer.as<float>(), c->st);
+                   B, es, m->dYb.as<unsigned short>(), m->lossPer.as<float>(), c->st);
       mlp_backward(m, B, c->st);
       ++m->step;
       const double frac = (double)stp / std::max(1, o->steps - 1);
       const double lr = o->lr * std::pow(o->lrFinal / o->lr, frac);
       mlp_adam(m->P.as<float>(), m->G.as<float>(), m->Mm.as<float>(), m->Vv.as<float>(), kMlpParams,
                (float)lr, (float)b1, (float)b2, (float)(1 - std::pow(b1, (double)m->step)),
-               (float)(1 - std::pow(b2, (double)m->step)), 1e-8f, c->st);
-      mlp_refresh_bf16(m, c->st);
+               (float)(1 - std::pow(b2, (double)m->step)), 1e-8f, m->Wb.as<unsigned short>(),
+               c->st);
       if (stp < 50 || stp >= o->steps - 50) {
         HIPCHK(hipMemcpyAsync(lp.data(), m->lossPer.p, sizeof(float) * B, hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
@@ -2181,7 +2184,7 @@ int ddd_farfield_train(ddd_ctx* c, ddd_farfield* ff, ddd_mlp* m, const ddd_const
                    bL.as<float>(), bD.as<float>(), c->st);
       mlp_forward(m, cnt, c->st);
       mlp_contract(m->Y.as<float>(), m->off3.as<int>(), bM.as<float>(), bL.as<float>(), bD.as<float>(),
-                   cnt, es, m->dY.as<float>(), m->lossPer.as<float>(), c->st);
+                   cnt, es, m->dYb.as<unsigned short>(), m->lossPer.as<float>(), c->st);
       HIPCHK(hipMemcpyAsync(lp.data(), m->lossPer.p, sizeof(float) * cnt, hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
       for (int q = 0; q < cnt; ++q) {

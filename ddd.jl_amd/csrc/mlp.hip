@@ -14,6 +14,7 @@
 #include "mlp.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace ddd {
 
@@ -30,6 +31,22 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 
+__device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z) {
+  const float s = 1.f / (1.f + __expf(-z));
+  return s * (1.f + z * (1.f - s));
+}
+
+// The layer epilogue on element (i, j) of C (row stride ldc): c = C value
+// after the product, returned as stored; H written if given.
+__device__ __forceinline__ float epilogue(float c, const float* __restrict__ bias, int act,
+                                          unsigned short* __restrict__ H, int64_t e, int j) {
+  if (!bias) return c;
+  const float z = c + bias[j];
+  if (H) H[e] = f2bf(act ? silu(z) : z);
+  return z;
+}
+
 // C[M x N] (fp32, row-major, ldc) = sum_k A(i, k) B(k, j), A and B bf16 with
 // element strides (a_i, a_k) and (b_k, b_j).  Workgroup: 4 wavefronts, a
 // 64 x 64 tile of C, 32 x 32 per wavefront; K staged through LDS 32 at a time.
@@ -38,7 +55,9 @@ constexpr int GT = 64, GK = 32;
 __global__ __launch_bounds__(256) void k_gemm_bf16(const unsigned short* __restrict__ A, int64_t a_i,
                                                    int64_t a_k, const unsigned short* __restrict__ B,
                                                    int64_t b_k, int64_t b_j, float* __restrict__ C,
-                                                   int64_t ldc, int M, int N, int K, int accumulate) {
+                                                   int64_t ldc, int M, int N, int K, int accumulate,
+                                                   const float* __restrict__ bias, int act,
+                                                   unsigned short* __restrict__ H) {
   __shared__ unsigned short sA[GT][GK + 4];
   __shared__ unsigned short sB[GK][GT + 4];
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
@@ -80,8 +99,8 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const unsigned short* __restr
     const int gi = i0 + wi + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
     const int gj = j0 + wj + lane % 32;
     if (gi < M && gj < N) {
-      float* p = C + (int64_t)gi * ldc + gj;
-      *p = accumulate ? *p + acc[v] : acc[v];
+      const int64_t e = (int64_t)gi * ldc + gj;
+      C[e] = epilogue(accumulate ? C[e] + acc[v] : acc[v], bias, act, H, e, gj);
     }
   }
 }
@@ -97,7 +116,8 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const unsigned short* __restr
 //
 // Split K: workgroup z of gridDim.z takes k in [z kChunk, (z + 1) kChunk) and
 // writes its partial product to C + z cStride (accumulate must be 0); a
-// second kernel adds the partials in z order.
+// second kernel adds the partials in z order.  EPI selects the epilogue as
+// for k_gemm_bf16_t256 below.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 constexpr int BT = 128, BK = 64, LDK = BK + 8;
@@ -113,13 +133,16 @@ struct TileRegs {
   u16x8 a[4], b[4];
 };
 
+template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __restrict__ A,
                                                         int64_t a_i, int64_t a_k,
                                                         const unsigned short* __restrict__ B,
                                                         int64_t b_k, int64_t b_j,
                                                         float* __restrict__ C, int64_t ldc, int M,
                                                         int N, int K, int accumulate, int kChunk,
-                                                        int64_t cStride) {
+                                                        int64_t cStride,
+                                                        const float* __restrict__ bias, int act,
+                                                        unsigned short* __restrict__ H) {
   __shared__ __attribute__((aligned(16))) unsigned short sA[BT][LDK];
   __shared__ __attribute__((aligned(16))) unsigned short sB[BT][LDK];
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
@@ -129,6 +152,13 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __
   const int kBeg = blockIdx.z * kChunk;
   const int kEnd = min(K, kBeg + kChunk);
   C += blockIdx.z * cStride;
+  // the bias of the lane's two columns, loaded now so that the epilogue
+  // does not wait for it
+  float bj[2] = {0.f, 0.f};
+  if constexpr (EPI >= 2) {
+#pragma unroll
+    for (int n = 0; n < 2; ++n) bj[n] = bias[min(j0 + wj + 32 * n + lane % 32, N - 1)];
+  }
   f32x16 acc[2][2];
 #pragma unroll
   for (int m = 0; m < 2; ++m)
@@ -242,78 +272,282 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __
     if (more) stash(R);
     __syncthreads();
   }
+  // Epilogue (as epilogue() above, with the bias preloaded): the lane holds
+  // column wj + 32 n + lane % 32 of rows wi + 32 m + 4 (lane / 32) +
+  // 8 (v / 4) + v % 4.
+  const int ld32 = (int)ldc;
 #pragma unroll
-  for (int m = 0; m < 2; ++m)
+  for (int m = 0; m < 2; ++m) {
+    const int gi0 = i0 + wi + 32 * m + 4 * (lane / 32);
 #pragma unroll
-    for (int n = 0; n < 2; ++n)
+    for (int n = 0; n < 2; ++n) {
+      const int gj = j0 + wj + 32 * n + lane % 32;
+      if (gj >= N) continue;
+      float* cb = C + (int64_t)gi0 * ldc + gj;
+      unsigned short* hb = EPI == 3 ? H + (int64_t)gi0 * ldc + gj : nullptr;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const int gi = i0 + wi + 32 * m + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-        const int gj = j0 + wj + 32 * n + lane % 32;
-        if (gi < M && gj < N) {
-          float* p = C + (int64_t)gi * ldc + gj;
-          *p = accumulate ? *p + acc[m][n][v] : acc[m][n][v];
-        }
+        const int dr = 8 * (v / 4) + (v % 4);
+        if (gi0 + dr >= M) continue;
+        const int off = dr * ld32;
+        float x = EPI == 1 ? cb[off] + acc[m][n][v] : acc[m][n][v];
+        if constexpr (EPI >= 2) x = x + bj[n];
+        if constexpr (EPI == 3) hb[off] = f2bf(act ? silu(x) : x);
+        cb[off] = x;
       }
+    }
+  }
 }
 
-// C (+)= sum over z < nSplit of the partials P + z stride (M x N, ldc N).
+// The large products: a 256 x 256 tile per workgroup of 4 wavefronts, each
+// wavefront 128 x 128 (4 x 4 blocks of v_mfma_f32_32x32x16_bf16; its 256
+// accumulator registers fill the AGPR half of the register file, one
+// wavefront per SIMD).  A 128 x 128 wavefront tile reads 8 KB of operands
+// from LDS per 16 MFMAs, half the LDS traffic per flop of a 64 x 64 one,
+// whose operand reads alone would match the LDS bandwidth at the MFMA rate.
+// K is staged 64 at a time through two LDS buffers: the next K tile is
+// loaded into registers while the current one is multiplied, then written
+// to the other buffer, with one barrier per K step.  AK / BKC: the operand is
+// contiguous along k (else along i / j, transposed on the way into LDS).
+// Same layout, alignment and split-K conventions as k_gemm_bf16_t128.
+// EPI: 0 stores the product, 1 adds it to C, 2 adds the bias, 3 adds the
+// bias and writes the activation to H (the unrolled activation code of 256
+// values is kept out of the other variants: it would not fit the
+// instruction cache next to the main loop).
+constexpr int HT = 256;
+
+template <bool AK, bool BKC, int EPI>
+__global__ __launch_bounds__(256, 1) void k_gemm_bf16_t256(const unsigned short* __restrict__ A,
+                                                           int64_t a_i, int64_t a_k,
+                                                           const unsigned short* __restrict__ B,
+                                                           int64_t b_k, int64_t b_j,
+                                                           float* __restrict__ C, int64_t ldc,
+                                                           int M, int N, int K, int accumulate,
+                                                           int kChunk, int64_t cStride,
+                                                           const float* __restrict__ bias, int act,
+                                                           unsigned short* __restrict__ H) {
+  __shared__ __attribute__((aligned(16))) unsigned short sA[2][HT][LDK];
+  __shared__ __attribute__((aligned(16))) unsigned short sB[2][HT][LDK];
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int i0 = blockIdx.y * HT, j0 = blockIdx.x * HT;
+  const int wi = (wave / 2) * 128, wj = (wave % 2) * 128;
+  const int kBeg = blockIdx.z * kChunk;
+  const int kEnd = min(K, kBeg + kChunk);
+  C += blockIdx.z * cStride;
+  // the bias of the lane's four columns, loaded now so that the epilogue
+  // does not wait for it
+  float bj[4] = {0.f, 0.f, 0.f, 0.f};
+  if constexpr (EPI >= 2) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n) bj[n] = bias[min(j0 + wj + 32 * n + lane % 32, N - 1)];
+  }
+  f32x16 acc[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[m][n][v] = 0.f;
+
+  // Vector v = tid + 256 h (h < 8) of the 256 x 64 operand tiles: k
+  // contiguous, row v / 8 and k (v % 8) * 8; else k v / 32 and rows
+  // (v % 32) * 8 .. + 7.  The launcher guarantees that the contiguous
+  // extent of each operand is a multiple of 8, so a vector lies wholly
+  // inside or wholly outside it: outside ones read a clamped in-range
+  // address, and are replaced by zeros when written to LDS (bit h of okA /
+  // okB), so that nothing waits for the loads before the products.
+  u16x8 ra[8], rb[8];
+  unsigned okA = 0, okB = 0;
+  auto fetch = [&](int k0) {
+    okA = okB = 0;
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      const int v = tid + 256 * h;
+      if constexpr (AK) {
+        const int gi = i0 + v / 8, gk = k0 + (v % 8) * 8;
+        okA |= (unsigned)(gi < M && gk < kEnd) << h;
+        ra[h] = *(const u16x8*)(A + (int64_t)min(gi, M - 1) * a_i + min(gk, K - 8));
+      } else {
+        const int gk = k0 + v / 32, gi = i0 + (v % 32) * 8;
+        okA |= (unsigned)(gk < kEnd && gi < M) << h;
+        ra[h] = *(const u16x8*)(A + min(gi, M - 8) + (int64_t)min(gk, K - 1) * a_k);
+      }
+      if constexpr (BKC) {
+        const int gj = j0 + v / 8, gk = k0 + (v % 8) * 8;
+        okB |= (unsigned)(gj < N && gk < kEnd) << h;
+        rb[h] = *(const u16x8*)(B + min(gk, K - 8) + (int64_t)min(gj, N - 1) * b_j);
+      } else {
+        const int gk = k0 + v / 32, gj = j0 + (v % 32) * 8;
+        okB |= (unsigned)(gk < kEnd && gj < N) << h;
+        rb[h] = *(const u16x8*)(B + (int64_t)min(gk, K - 1) * b_k + min(gj, N - 8));
+      }
+    }
+  };
+  auto stash = [&](int buf) {
+    const u16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      const int v = tid + 256 * h;
+      const u16x8 ta = (okA >> h) & 1 ? ra[h] : zero8;
+      const u16x8 tb = (okB >> h) & 1 ? rb[h] : zero8;
+      if constexpr (AK) {
+        const int r = v / 8;
+        *(u16x8*)&sA[buf][r][swz(r, (v % 8) * 8)] = ta;
+      } else {
+        const int kk = v / 32, rq = (v % 32) * 8;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sA[buf][rq + q][swz(rq + q, kk)] = ta[q];
+      }
+      if constexpr (BKC) {
+        const int c = v / 8;
+        *(u16x8*)&sB[buf][c][swz(c, (v % 8) * 8)] = tb;
+      } else {
+        const int kk = v / 32, cq = (v % 32) * 8;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sB[buf][cq + q][swz(cq + q, kk)] = tb[q];
+      }
+    }
+  };
+
+  int cur = 0;
+  if (kBeg < kEnd) {
+    fetch(kBeg);
+    stash(0);
+  }
+  __syncthreads();
+  for (int k0 = kBeg; k0 < kEnd; k0 += BK) {
+    const bool more = k0 + BK < kEnd;
+    if (more) fetch(k0 + BK);  // in flight during the products below
+    // Operands of k sub-step ks + 16 are read from LDS while the 16
+    // products of sub-step ks run (two register sets).
+    bf16x8 a[2][4], b[2][4];
+    auto operands = [&](int ks, int set) {
+      const int kb = ks + 8 * (lane / 32);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int r = wi + 32 * m + lane % 32;
+        a[set][m] = *(const bf16x8*)&sA[cur][r][swz(r, kb)];
+      }
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        const int c = wj + 32 * n + lane % 32;
+        b[set][n] = *(const bf16x8*)&sB[cur][c][swz(c, kb)];
+      }
+    };
+    operands(0, 0);
+#pragma unroll
+    for (int q = 0; q < BK / 16; ++q) {
+      if (q + 1 < BK / 16) operands(16 * (q + 1), (q + 1) & 1);
+      // keeps the reads ahead of the products (the scheduler would
+      // otherwise reuse one register set and wait on every read)
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+          acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q & 1][m], b[q & 1][n], acc[m][n],
+                                                              0, 0, 0);
+    }
+    // The other buffer was last read before the previous barrier.
+    if (more) stash(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+  // Epilogue (as epilogue() above): the lane holds column
+  // wj + 32 n + lane % 32 of rows wi + 32 m + 4 (lane / 32) + 8 (v / 4) + v % 4;
+  // per block one base address, per v a row offset common to all blocks.
+  const int ld32 = (int)ldc;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int gi0 = i0 + wi + 32 * m + 4 * (lane / 32);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int gj = j0 + wj + 32 * n + lane % 32;
+      if (gj >= N) continue;
+      float* cb = C + (int64_t)gi0 * ldc + gj;
+      unsigned short* hb = EPI == 3 ? H + (int64_t)gi0 * ldc + gj : nullptr;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int dr = 8 * (v / 4) + (v % 4);
+        if (gi0 + dr >= M) continue;
+        const int off = dr * ld32;
+        float x = EPI == 1 ? cb[off] + acc[m][n][v] : acc[m][n][v];
+        if constexpr (EPI >= 2) x = x + bj[n];
+        if constexpr (EPI == 3) hb[off] = f2bf(act ? silu(x) : x);
+        cb[off] = x;
+      }
+    }
+  }
+}
+
+// C (+)= sum over z < nSplit of the partials P + z stride (M x N, ldc N),
+// then the epilogue.
 __global__ void k_splitk_reduce(const float* __restrict__ P, int64_t stride, int nSplit, int64_t n,
-                                float* __restrict__ C, int accumulate) {
+                                int N, float* __restrict__ C, int accumulate,
+                                const float* __restrict__ bias, int act,
+                                unsigned short* __restrict__ H) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= n) return;
   float a = 0.f;
   for (int z = 0; z < nSplit; ++z) a += P[z * stride + e];
-  C[e] = accumulate ? C[e] + a : a;
+  C[e] = epilogue(accumulate ? C[e] + a : a, bias, act, H, e, (int)(e % N));
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
-__device__ __forceinline__ float silu_grad(float z) {
-  const float s = 1.f / (1.f + __expf(-z));
-  return s * (1.f + z * (1.f - s));
-}
+// Column sums of bf16 matrices in a fixed order.  A workgroup of 4
+// wavefronts takes 64 columns (one a lane) of a chunk of kColRows rows;
+// wavefront w sums rows w kColRows / 4 .. (w + 1) kColRows / 4 - 1 of the
+// chunk in order, and the four sums are added in wavefront order.  ACT: the
+// matrix is formed on the way, dZ = bf16(dH * SiLU'(Z)), and stored.
+constexpr int kColWaveRows = kColRows / 4;
 
-// Z = C + b (in place), H = act(Z) as bf16 (act = SiLU, or identity).
-__global__ void k_bias_act(float* __restrict__ Z, const float* __restrict__ bias, int rows, int cols,
-                           int act, unsigned short* __restrict__ H) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= (int64_t)rows * cols) return;
-  const int c = (int)(e % cols);
-  const float z = Z[e] + bias[c];
-  Z[e] = z;
-  if (H) H[e] = f2bf(act ? silu(z) : z);
-}
-
-// dZ = dH * act'(Z); output bf16 (the next products' operand).
-__global__ void k_act_grad(const float* __restrict__ dH, const float* __restrict__ Z, int64_t n,
-                           unsigned short* __restrict__ dZ) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  dZ[e] = f2bf(dH[e] * silu_grad(Z[e]));
-}
-
-// db[c] = sum over rows of dZ[r][c] (bf16 values): partial sums over
-// kColChunks row chunks (thread per column, coalesced), then the chunks in
-// order.
-constexpr int kColChunks = 64;
-__global__ void k_colsum_part(const unsigned short* __restrict__ dZ, int rows, int cols,
-                              float* __restrict__ part) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ch = blockIdx.y;
-  if (c >= cols) return;
-  const int per = (rows + kColChunks - 1) / kColChunks;
-  const int r0 = ch * per, r1 = min(rows, r0 + per);
+template <bool ACT>
+__global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ dH,
+                                                     const float* __restrict__ Z,
+                                                     unsigned short* __restrict__ dZ, int rows,
+                                                     int cols, float* __restrict__ part) {
+  __shared__ float sw[4][64];
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  const int c = blockIdx.x * 64 + lane;
+  const int r0 = blockIdx.y * kColRows + w * kColWaveRows;
+  const int r1 = min(rows, r0 + kColWaveRows);
   float a = 0.f;
-  for (int r = r0; r < r1; ++r) a += bf2f(dZ[(int64_t)r * cols + c]);
-  part[(int64_t)ch * cols + c] = a;
+  if (c < cols) {
+#pragma unroll 8
+    for (int r = r0; r < r1; ++r) {
+      const int64_t e = (int64_t)r * cols + c;
+      if constexpr (ACT) {
+        const unsigned short d = f2bf(dH[e] * silu_grad(Z[e]));
+        dZ[e] = d;
+        a += bf2f(d);
+      } else {
+        a += bf2f(dZ[e]);
+      }
+    }
+  }
+  sw[w][lane] = a;
+  __syncthreads();
+  if (w == 0 && c < cols)
+    part[(int64_t)blockIdx.y * cols + c] = ((sw[0][lane] + sw[1][lane]) + sw[2][lane]) + sw[3][lane];
 }
 
-__global__ void k_colsum_final(const float* __restrict__ part, int cols, float* __restrict__ db) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+// db[c] = sum of the chunk partials: wavefront w adds chunks
+// w q .. (w + 1) q - 1 (q = ceil(nChunks / 4)) in order, then the four sums
+// in wavefront order.
+__global__ __launch_bounds__(256) void k_colsum_final(const float* __restrict__ part, int cols,
+                                                      int nChunks, float* __restrict__ db) {
+  __shared__ float sw[4][64];
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  const int c = blockIdx.x * 64 + lane;
+  const int q = (nChunks + 3) / 4;
+  const int h0 = w * q, h1 = min(nChunks, h0 + q);
   float a = 0.f;
-  for (int ch = 0; ch < kColChunks; ++ch) a += part[(int64_t)ch * cols + c];
-  db[c] = a;
+  if (c < cols) {
+#pragma unroll 8
+    for (int h = h0; h < h1; ++h) a += part[(int64_t)h * cols + c];
+  }
+  sw[w][lane] = a;
+  __syncthreads();
+  if (w == 0 && c < cols) db[c] = ((sw[0][lane] + sw[1][lane]) + sw[2][lane]) + sw[3][lane];
 }
 
 __global__ void k_f32_to_bf16(const float* __restrict__ x, int64_t n, unsigned short* __restrict__ y) {
@@ -364,14 +598,15 @@ __device__ __forceinline__ int moment_order(int k) { return k < 9 ? 0 : (k < 36 
 __device__ __forceinline__ int poly_order(int l) { return l == 0 ? 0 : (l < 4 ? 1 : 2); }
 
 // Per sample: predicted local expansion L_hat[q] = sum_k W(o)[q][k] M[k]
-// (q = 10 c + l), loss term |L_hat - L|^2 / (den ns) and dY (fp32).  One
-// workgroup of 256 per sample.
+// (q = 10 c + l), loss term |L_hat - L|^2 / (den ns) and dY (bf16, the
+// backward products' operand).  One workgroup of 256 per sample.
 __global__ __launch_bounds__(256) void k_contract(const float* __restrict__ Y,
                                                   const int* __restrict__ off3,
                                                   const float* __restrict__ Ms,
                                                   const float* __restrict__ Lt,
                                                   const float* __restrict__ den, int ns,
-                                                  float envScale, float* __restrict__ dY,
+                                                  float envScale,
+                                                  unsigned short* __restrict__ dY,
                                                   float* __restrict__ lossPer) {
   const int s = blockIdx.x;
   const int tid = threadIdx.x;
@@ -402,7 +637,7 @@ __global__ __launch_bounds__(256) void k_contract(const float* __restrict__ Y,
       const int q = e / kMoments, k = e % kMoments;
       g = w * r[q] * envL[q % kPoly] * envK[k] * sM[k];
     }
-    dY[(int64_t)s * kMlpOutPad + e] = g;
+    dY[(int64_t)s * kMlpOutPad + e] = f2bf(g);
   }
   if (tid == 0) {
     float a = 0.f;
@@ -429,9 +664,10 @@ __global__ void k_operator_from_y(const float* __restrict__ Y, const int* __rest
   W[e] = v;
 }
 
+// One Adam step; pb receives the bf16 copy of the updated parameters.
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, int64_t n, float lr, float b1, float b2, float c1,
-                       float c2, float eps) {
+                       float c2, float eps, unsigned short* __restrict__ pb) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= n) return;
   const float gg = g[e];
@@ -439,7 +675,9 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   const float vv = b2 * v[e] + (1.f - b2) * gg * gg;
   m[e] = mm;
   v[e] = vv;
-  p[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+  const float q = p[e] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+  p[e] = q;
+  pb[e] = f2bf(q);
 }
 
 // Batch b of the training set: sample idx[b]; moments from the cell array
@@ -566,56 +804,111 @@ static unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs)
 
 void mlp_gemm(const unsigned short* A, int64_t a_i, int64_t a_k, const unsigned short* B,
               int64_t b_k, int64_t b_j, float* C, int64_t ldc, int M, int N, int K, int accumulate,
-              hipStream_t st, float* work, int64_t workFloats) {
+              hipStream_t st, float* work, int64_t workFloats, MlpEpilogue ep) {
   const bool alignedA = ((uintptr_t)A % 16 == 0) &&
                         (a_k == 1 ? a_i % 8 == 0 : (a_i == 1 && a_k % 8 == 0));
   const bool alignedB = ((uintptr_t)B % 16 == 0) &&
                         (b_k == 1 ? b_j % 8 == 0 : (b_j == 1 && b_k % 8 == 0));
+  // Each operand's contiguous extent a multiple of 8 (k for k-contiguous
+  // operands; M or N otherwise).
+  const bool whole8 = K % 8 == 0 && (a_k == 1 || M % 8 == 0) && (b_k == 1 || N % 8 == 0);
+  if (alignedA && alignedB && whole8 && (int64_t)M * N * K >= (int64_t(1) << 33)) {
+    // The large products on 256 x 256 tiles; fewer tiles than CUs: split K
+    // into at most as many parts as fill the chip once.
+    const int64_t tiles = (int64_t)blocks(N, HT) * blocks(M, HT);
+    int split = 1;
+    if (tiles < 256 && K >= 8 * BK && ldc == N) {
+      split = (int)std::min<int64_t>(256 / tiles, K / (4 * BK));
+      while (split > 1 && (int64_t)split * M * N > workFloats) --split;
+    }
+    int kChunk = K;
+    if (split > 1) {
+      kChunk = ((K + split - 1) / split + BK - 1) / BK * BK;
+      split = (K + kChunk - 1) / kChunk;
+    }
+    const bool ak = (a_k == 1), bk = (b_k == 1);
+    // epilogue variant: partials and plain products store, accumulate adds,
+    // a bias applies the layer epilogue (bias with accumulate: the general
+    // kernel below)
+    const int epi = split > 1 ? 0 : (ep.bias ? (ep.H ? 3 : 2) : (accumulate ? 1 : 0));
+    if (!(epi >= 2 && accumulate)) {
+      auto pick = [&](auto e) {
+        constexpr int E = decltype(e)::value;
+        return ak ? (bk ? k_gemm_bf16_t256<true, true, E> : k_gemm_bf16_t256<true, false, E>)
+                  : (bk ? k_gemm_bf16_t256<false, true, E> : k_gemm_bf16_t256<false, false, E>);
+      };
+      auto kern = epi == 0   ? pick(std::integral_constant<int, 0>{})
+                  : epi == 1 ? pick(std::integral_constant<int, 1>{})
+                  : epi == 2 ? pick(std::integral_constant<int, 2>{})
+                             : pick(std::integral_constant<int, 3>{});
+      dim3 grid(blocks(N, HT), blocks(M, HT), split);
+      if (split > 1) {
+        hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, A, a_i, a_k, B, b_k, b_j, work, (int64_t)N,
+                           M, N, K, 0, kChunk, (int64_t)M * N, nullptr, 0, nullptr);
+        const int64_t n = (int64_t)M * N;
+        hipLaunchKernelGGL(k_splitk_reduce, dim3(blocks(n, 256)), dim3(256), 0, st, work, n, split,
+                           n, N, C, accumulate, ep.bias, ep.act, ep.H);
+      } else {
+        hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, A, a_i, a_k, B, b_k, b_j, C, ldc, M, N, K,
+                           accumulate, K, (int64_t)0, ep.bias, ep.act, ep.H);
+      }
+      return;
+    }
+  }
   if (alignedA && alignedB) {
     const int64_t tiles = (int64_t)blocks(N, BT) * blocks(M, BT);
     // Few output tiles and a long K (the weight gradients): split K so that
-    // the grid fills the chip, if the workspace holds the partials.
+    // the grid has about one workgroup a CU, if the workspace holds the
+    // partials.  (Measured at 4,096 samples: 512 x 512 x 4,096 takes 113 us
+    // unsplit, 23 us in 16 parts, 26 us in 32; the products with K = 512
+    // are fastest unsplit.)
     int split = 1;
-    if (tiles < 512 && K >= 4 * BK && ldc == N) {
-      split = (int)std::min<int64_t>((512 + tiles - 1) / tiles, K / (2 * BK));
+    if (tiles < 256 && K >= 2048 && ldc == N) {
+      split = (int)std::min<int64_t>((256 + tiles - 1) / tiles, K / (4 * BK));
       while (split > 1 && (int64_t)split * M * N > workFloats) --split;
     }
     if (split > 1) {
       const int kChunk = ((K + split - 1) / split + BK - 1) / BK * BK;
       split = (K + kChunk - 1) / kChunk;
       dim3 grid(blocks(N, BT), blocks(M, BT), split);
-      hipLaunchKernelGGL(k_gemm_bf16_t128, grid, dim3(256), 0, st, A, a_i, a_k, B, b_k, b_j, work,
-                         (int64_t)N, M, N, K, 0, kChunk, (int64_t)M * N);
+      hipLaunchKernelGGL(k_gemm_bf16_t128<0>, grid, dim3(256), 0, st, A, a_i, a_k, B, b_k, b_j,
+                         work, (int64_t)N, M, N, K, 0, kChunk, (int64_t)M * N, nullptr, 0, nullptr);
       const int64_t n = (int64_t)M * N;
-      hipLaunchKernelGGL(k_splitk_reduce, dim3(blocks(n, 256)), dim3(256), 0, st, work, n, split, n, C,
-                         accumulate);
+      hipLaunchKernelGGL(k_splitk_reduce, dim3(blocks(n, 256)), dim3(256), 0, st, work, n, split, n,
+                         N, C, accumulate, ep.bias, ep.act, ep.H);
       return;
     }
-    dim3 grid(blocks(N, BT), blocks(M, BT), 1);
-    hipLaunchKernelGGL(k_gemm_bf16_t128, grid, dim3(256), 0, st, A, a_i, a_k, B, b_k, b_j, C, ldc, M,
-                       N, K, accumulate, K, (int64_t)0);
-    return;
+    const int epi = ep.bias ? (ep.H ? 3 : 2) : (accumulate ? 1 : 0);
+    if (!(epi >= 2 && accumulate)) {
+      auto kern = epi == 0   ? k_gemm_bf16_t128<0>
+                  : epi == 1 ? k_gemm_bf16_t128<1>
+                  : epi == 2 ? k_gemm_bf16_t128<2>
+                             : k_gemm_bf16_t128<3>;
+      dim3 grid(blocks(N, BT), blocks(M, BT), 1);
+      hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, A, a_i, a_k, B, b_k, b_j, C, ldc, M, N, K,
+                         accumulate, K, (int64_t)0, ep.bias, ep.act, ep.H);
+      return;
+    }
   }
   dim3 grid(blocks(N, GT), blocks(M, GT));
   hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, A, a_i, a_k, B, b_k, b_j, C, ldc, M, N, K,
-                     accumulate);
-}
-
-void mlp_bias_act(float* Z, const float* b, int rows, int cols, int act, unsigned short* H,
-                  hipStream_t st) {
-  hipLaunchKernelGGL(k_bias_act, dim3(blocks((int64_t)rows * cols, 256)), dim3(256), 0, st, Z, b,
-                     rows, cols, act, H);
-}
-
-void mlp_act_grad(const float* dH, const float* Z, int64_t n, unsigned short* dZ, hipStream_t st) {
-  hipLaunchKernelGGL(k_act_grad, dim3(blocks(n, 256)), dim3(256), 0, st, dH, Z, n, dZ);
+                     accumulate, ep.bias, ep.act, ep.H);
 }
 
 void mlp_colsum(const unsigned short* dZ, int rows, int cols, float* part, float* db,
                 hipStream_t st) {
-  hipLaunchKernelGGL(k_colsum_part, dim3(blocks(cols, 256), kColChunks), dim3(256), 0, st, dZ, rows,
-                     cols, part);
-  hipLaunchKernelGGL(k_colsum_final, dim3(blocks(cols, 256)), dim3(256), 0, st, part, cols, db);
+  const int nCh = (rows + kColRows - 1) / kColRows;
+  hipLaunchKernelGGL(k_colsum_part<false>, dim3(blocks(cols, 64), nCh), dim3(256), 0, st, nullptr,
+                     nullptr, const_cast<unsigned short*>(dZ), rows, cols, part);
+  hipLaunchKernelGGL(k_colsum_final, dim3(blocks(cols, 64)), dim3(256), 0, st, part, cols, nCh, db);
+}
+
+void mlp_act_grad_colsum(const float* dH, const float* Z, int rows, int cols, unsigned short* dZ,
+                         float* part, float* db, hipStream_t st) {
+  const int nCh = (rows + kColRows - 1) / kColRows;
+  hipLaunchKernelGGL(k_colsum_part<true>, dim3(blocks(cols, 64), nCh), dim3(256), 0, st, dH, Z, dZ,
+                     rows, cols, part);
+  hipLaunchKernelGGL(k_colsum_final, dim3(blocks(cols, 64)), dim3(256), 0, st, part, cols, nCh, db);
 }
 
 void mlp_to_bf16(const float* x, int64_t n, unsigned short* y, hipStream_t st) {
@@ -627,7 +920,7 @@ void mlp_features(const int* off3, int ns, float aH, unsigned short* X, hipStrea
 }
 
 void mlp_contract(const float* Y, const int* off3, const float* M, const float* Lt,
-                  const float* den, int ns, float envScale, float* dY, float* lossPer,
+                  const float* den, int ns, float envScale, unsigned short* dY, float* lossPer,
                   hipStream_t st) {
   hipLaunchKernelGGL(k_contract, dim3(ns), dim3(256), 0, st, Y, off3, M, Lt, den, ns, envScale, dY,
                      lossPer);
@@ -640,9 +933,9 @@ void mlp_operator_from_y(const float* Y, const int* off3, int no, float envScale
 }
 
 void mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
-              float c1, float c2, float eps, hipStream_t st) {
+              float c1, float c2, float eps, unsigned short* pb, hipStream_t st) {
   hipLaunchKernelGGL(k_adam, dim3(blocks(n, 256)), dim3(256), 0, st, p, g, m, v, n, lr, b1, b2, c1,
-                     c2, eps);
+                     c2, eps, pb);
 }
 
 void gather_batch(const int* idx, int nb, const int* off3, const int* src, const float* M, int nc,

@@ -3,7 +3,10 @@
 * loss and gradient of ddd_mlp_loss_grad (BF16 MFMA products, FP32
   accumulation) against the numpy restatement that rounds to bf16 at the same
   points: loss within 1e-4, each parameter block's gradient within 3e-3
-  normwise (FP32 vs FP64 accumulation and bf16 ties that fall differently);
+  normwise (FP32 vs FP64 accumulation and bf16 ties that fall differently).
+  At 4,096 samples (the training batch) the output layer's three products
+  run on the 256 x 256 tiles (one with split K over a ragged K), the
+  smaller batches on the 128 x 128 ones;
 * training on the exact stresses of one network lowers the loss, and the
   operator it predicts improves the hybrid forces of another network over the
   no-far-field ablation and over the untrained network.
@@ -36,7 +39,7 @@ def test_params_roundtrip():
     m.close()
 
 
-@pytest.mark.parametrize("ns", [1000, 200, 64])
+@pytest.mark.parametrize("ns", [4096, 1000, 200, 64])
 def test_loss_grad_vs_reference(ns):
     m = G.FarFieldMLP(seed=1)
     p = R.init_params(3).astype(np.float32)
