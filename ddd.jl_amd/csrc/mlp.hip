@@ -110,9 +110,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const unsigned short* __restr
 // staged 64 at a time.  Both operands sit in LDS with k contiguous (A as
 // [row][k], B as [col][k]), so each lane fetches its 8 k-values with one
 // 16-byte read; global loads are 8-element vectors along whichever index is
-// contiguous, and the next K tile is loaded into registers while the current
-// one is multiplied.  Requires 16-byte aligned bases, and the non-unit stride
-// of the contiguous operand a multiple of 8.
+// contiguous, and the next two K tiles are loaded into registers while the
+// current one is multiplied.  Requires 16-byte aligned bases, the non-unit
+// stride of the contiguous operand a multiple of 8, and the contiguous
+// extent of each operand a multiple of 8 (k, or M / N).
 //
 // Split K: workgroup z of gridDim.z takes k in [z kChunk, (z + 1) kChunk) and
 // writes its partial product to C + z cStride (accumulate must be 0); a
@@ -131,10 +132,11 @@ __device__ __forceinline__ int swz(int row, int k) { return k ^ (((row >> 3) & 7
 
 struct TileRegs {
   u16x8 a[4], b[4];
+  unsigned okA, okB;  // bit h: vector h lies inside the operand
 };
 
-template <int EPI>
-__global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __restrict__ A,
+template <bool AK, bool BKC, int EPI>
+__global__ __launch_bounds__(256, 2) void k_gemm_bf16_t128(const unsigned short* __restrict__ A,
                                                         int64_t a_i, int64_t a_k,
                                                         const unsigned short* __restrict__ B,
                                                         int64_t b_k, int64_t b_j,
@@ -148,7 +150,6 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
   const int i0 = blockIdx.y * BT, j0 = blockIdx.x * BT;
   const int wi = (wave / 2) * 64, wj = (wave % 2) * 64;
-  const bool aK = (a_k == 1), bK = (b_k == 1);
   const int kBeg = blockIdx.z * kChunk;
   const int kEnd = min(K, kBeg + kChunk);
   C += blockIdx.z * cStride;
@@ -168,86 +169,64 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __
       for (int v = 0; v < 16; ++v) acc[m][n][v] = 0.f;
 
   // Vector v (0..1023) of the 128 x 64 A tile and of the 64 x 128 B tile.
+  // The launcher guarantees that the contiguous extent of each operand is a
+  // multiple of 8, so a vector lies wholly inside or wholly outside it:
+  // outside ones read a clamped in-range address and are replaced by zeros
+  // when written to LDS, so that nothing waits for the loads before then.
   auto fetch = [&](int k0, TileRegs& R) {
+    R.okA = R.okB = 0;
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
       const int v = tid + 256 * h;
-      if (aK) {  // 8 consecutive k of one row
-        const int r = v / 8, kq = (v % 8) * 8;
-        const int gi = i0 + r, gk = k0 + kq;
-        if (gi < M && gk + 7 < kEnd) {
-          R.a[h] = *(const u16x8*)(A + (int64_t)gi * a_i + gk);
-        } else {
-#pragma unroll
-          for (int q = 0; q < 8; ++q)
-            R.a[h][q] = (gi < M && gk + q < kEnd) ? A[(int64_t)gi * a_i + gk + q] : 0;
-        }
+      if constexpr (AK) {  // 8 consecutive k of one row
+        const int gi = i0 + v / 8, gk = k0 + (v % 8) * 8;
+        R.okA |= (unsigned)(gi < M && gk < kEnd) << h;
+        R.a[h] = *(const u16x8*)(A + (int64_t)min(gi, M - 1) * a_i + min(gk, K - 8));
       } else {  // 8 consecutive rows at one k
         const int kk = v / 16, rq = (v % 16) * 8;
         const int gi = i0 + rq, gk = k0 + kk;
-        if (gk < kEnd && gi + 7 < M) {
-          R.a[h] = *(const u16x8*)(A + gi + (int64_t)gk * a_k);
-        } else {
-#pragma unroll
-          for (int q = 0; q < 8; ++q)
-            R.a[h][q] = (gk < kEnd && gi + q < M) ? A[gi + q + (int64_t)gk * a_k] : 0;
-        }
+        R.okA |= (unsigned)(gk < kEnd && gi < M) << h;
+        R.a[h] = *(const u16x8*)(A + min(gi, M - 8) + (int64_t)min(gk, K - 1) * a_k);
       }
-      if (bK) {  // 8 consecutive k of one column
-        const int c = v / 8, kq = (v % 8) * 8;
-        const int gj = j0 + c, gk = k0 + kq;
-        if (gj < N && gk + 7 < kEnd) {
-          R.b[h] = *(const u16x8*)(B + gk + (int64_t)gj * b_j);
-        } else {
-#pragma unroll
-          for (int q = 0; q < 8; ++q)
-            R.b[h][q] = (gj < N && gk + q < kEnd) ? B[gk + q + (int64_t)gj * b_j] : 0;
-        }
+      if constexpr (BKC) {  // 8 consecutive k of one column
+        const int gj = j0 + v / 8, gk = k0 + (v % 8) * 8;
+        R.okB |= (unsigned)(gj < N && gk < kEnd) << h;
+        R.b[h] = *(const u16x8*)(B + min(gk, K - 8) + (int64_t)min(gj, N - 1) * b_j);
       } else {  // 8 consecutive columns at one k
         const int kk = v / 16, cq = (v % 16) * 8;
         const int gj = j0 + cq, gk = k0 + kk;
-        if (gk < kEnd && gj + 7 < N) {
-          R.b[h] = *(const u16x8*)(B + (int64_t)gk * b_k + gj);
-        } else {
-#pragma unroll
-          for (int q = 0; q < 8; ++q)
-            R.b[h][q] = (gk < kEnd && gj + q < N) ? B[(int64_t)gk * b_k + gj + q] : 0;
-        }
+        R.okB |= (unsigned)(gk < kEnd && gj < N) << h;
+        R.b[h] = *(const u16x8*)(B + (int64_t)min(gk, K - 1) * b_k + min(gj, N - 8));
       }
     }
   };
   auto stash = [&](const TileRegs& R) {
+    const u16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
       const int v = tid + 256 * h;
-      if (aK) {
+      const u16x8 ta = (R.okA >> h) & 1 ? R.a[h] : zero8;
+      const u16x8 tb = (R.okB >> h) & 1 ? R.b[h] : zero8;
+      if constexpr (AK) {
         const int r = v / 8;
-        *(u16x8*)&sA[r][swz(r, (v % 8) * 8)] = R.a[h];
+        *(u16x8*)&sA[r][swz(r, (v % 8) * 8)] = ta;
       } else {
         const int kk = v / 16, rq = (v % 16) * 8;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) sA[rq + q][swz(rq + q, kk)] = R.a[h][q];
+        for (int q = 0; q < 8; ++q) sA[rq + q][swz(rq + q, kk)] = ta[q];
       }
-      if (bK) {
+      if constexpr (BKC) {
         const int c = v / 8;
-        *(u16x8*)&sB[c][swz(c, (v % 8) * 8)] = R.b[h];
+        *(u16x8*)&sB[c][swz(c, (v % 8) * 8)] = tb;
       } else {
         const int kk = v / 16, cq = (v % 16) * 8;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) sB[cq + q][swz(cq + q, kk)] = R.b[h][q];
+        for (int q = 0; q < 8; ++q) sB[cq + q][swz(cq + q, kk)] = tb[q];
       }
     }
   };
 
-  TileRegs R;
-  if (kBeg < kEnd) {
-    fetch(kBeg, R);
-    stash(R);
-  }
-  __syncthreads();
-  for (int k0 = kBeg; k0 < kEnd; k0 += BK) {
-    const bool more = k0 + BK < kEnd;
-    if (more) fetch(k0 + BK, R);  // in flight during the products below
+  auto products = [&] {
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 16) {
       const int kb = ks + 8 * (lane / 32);
@@ -268,8 +247,29 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_t128(const unsigned short* __
         for (int n = 0; n < 2; ++n)
           acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[m], b[n], acc[m][n], 0, 0, 0);
     }
+  };
+  // Two K tiles in flight: while tile t is multiplied from LDS, tile t + 1
+  // waits in one register set and tile t + 2 is loaded into the other (one
+  // step of lookahead leaves these small products waiting on memory
+  // latency).
+  TileRegs R0, R1;
+  if (kBeg < kEnd) {
+    fetch(kBeg, R0);
+    stash(R0);
+  }
+  if (kBeg + BK < kEnd) fetch(kBeg + BK, R1);
+  __syncthreads();
+  for (int k0 = kBeg; k0 < kEnd; k0 += 2 * BK) {
+    if (k0 + 2 * BK < kEnd) fetch(k0 + 2 * BK, R0);
+    products();
     __syncthreads();
-    if (more) stash(R);
+    if (k0 + BK >= kEnd) break;
+    stash(R1);
+    __syncthreads();
+    if (k0 + 3 * BK < kEnd) fetch(k0 + 3 * BK, R1);
+    products();
+    __syncthreads();
+    if (k0 + 2 * BK < kEnd) stash(R0);
     __syncthreads();
   }
   // Epilogue (as epilogue() above, with the bias preloaded): the lane holds
@@ -855,7 +855,13 @@ void mlp_gemm(const unsigned short* A, int64_t a_i, int64_t a_k, const unsigned 
       return;
     }
   }
-  if (alignedA && alignedB) {
+  if (alignedA && alignedB && whole8) {
+    const bool ak = (a_k == 1), bk = (b_k == 1);
+    auto pick128 = [&](auto e) {
+      constexpr int E = decltype(e)::value;
+      return ak ? (bk ? k_gemm_bf16_t128<true, true, E> : k_gemm_bf16_t128<true, false, E>)
+                : (bk ? k_gemm_bf16_t128<false, true, E> : k_gemm_bf16_t128<false, false, E>);
+    };
     const int64_t tiles = (int64_t)blocks(N, BT) * blocks(M, BT);
     // Few output tiles and a long K (the weight gradients): split K so that
     // the grid has about one workgroup a CU, if the workspace holds the
@@ -871,8 +877,9 @@ void mlp_gemm(const unsigned short* A, int64_t a_i, int64_t a_k, const unsigned 
       const int kChunk = ((K + split - 1) / split + BK - 1) / BK * BK;
       split = (K + kChunk - 1) / kChunk;
       dim3 grid(blocks(N, BT), blocks(M, BT), split);
-      hipLaunchKernelGGL(k_gemm_bf16_t128<0>, grid, dim3(256), 0, st, A, a_i, a_k, B, b_k, b_j,
-                         work, (int64_t)N, M, N, K, 0, kChunk, (int64_t)M * N, nullptr, 0, nullptr);
+      hipLaunchKernelGGL(pick128(std::integral_constant<int, 0>{}), grid, dim3(256), 0, st, A, a_i,
+                         a_k, B, b_k, b_j, work, (int64_t)N, M, N, K, 0, kChunk, (int64_t)M * N,
+                         nullptr, 0, nullptr);
       const int64_t n = (int64_t)M * N;
       hipLaunchKernelGGL(k_splitk_reduce, dim3(blocks(n, 256)), dim3(256), 0, st, work, n, split, n,
                          N, C, accumulate, ep.bias, ep.act, ep.H);
@@ -880,10 +887,10 @@ void mlp_gemm(const unsigned short* A, int64_t a_i, int64_t a_k, const unsigned 
     }
     const int epi = ep.bias ? (ep.H ? 3 : 2) : (accumulate ? 1 : 0);
     if (!(epi >= 2 && accumulate)) {
-      auto kern = epi == 0   ? k_gemm_bf16_t128<0>
-                  : epi == 1 ? k_gemm_bf16_t128<1>
-                  : epi == 2 ? k_gemm_bf16_t128<2>
-                             : k_gemm_bf16_t128<3>;
+      auto kern = epi == 0   ? pick128(std::integral_constant<int, 0>{})
+                  : epi == 1 ? pick128(std::integral_constant<int, 1>{})
+                  : epi == 2 ? pick128(std::integral_constant<int, 2>{})
+                             : pick128(std::integral_constant<int, 3>{});
       dim3 grid(blocks(N, BT), blocks(M, BT), 1);
       hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, A, a_i, a_k, B, b_k, b_j, C, ldc, M, N, K,
                          accumulate, K, (int64_t)0, ep.bias, ep.act, ep.H);
