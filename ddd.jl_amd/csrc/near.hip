@@ -60,25 +60,26 @@ enum : int { kNone = 0, kFP32 = 1, kFP64 = 2, kPar = 3 };
 // (kFP64).  Every rounding is explicit, so the decisions do not depend on
 // how the compiler contracts this translation unit.
 __device__ __forceinline__ int near_class(const double (*si)[kTile], const double (*sj)[kTile],
-                                          int ki, int kb, int lane, int s, bool diag, int rn,
-                                          double farDist2, double sinMin2) {
-  if (ki < 0 || kb < 0 || (diag && s <= lane)) return kNone;
-  if (abs((ki & 1023) - (kb & 1023)) > rn || abs(((ki >> 10) & 1023) - ((kb >> 10) & 1023)) > rn ||
-      abs((ki >> 20) - (kb >> 20)) > rn)
-    return kNone;
+                                          const double (*cmid)[kTile], int ki, int kb, int lane,
+                                          int s, bool diag, int rn, double farDist2,
+                                          double sinMin2) {
+  // Without early returns: every lane evaluates the tests and the class is
+  // selected, so the slot loop has no exec-mask branches.
+  const int dCell = max(max(abs((ki & 1023) - (kb & 1023)),
+                            abs(((ki >> 10) & 1023) - ((kb >> 10) & 1023))),
+                        abs((ki >> 20) - (kb >> 20)));
+  const bool live = (ki >= 0) & (kb >= 0) & !(diag & (s <= lane)) & (dCell <= rn);
   double c;
   const double omcSq = one_minus_c2(V(si[F_TX][lane], si[F_TY][lane], si[F_TZ][lane]),
                                     V(sj[F_TX][s], sj[F_TY][s], sj[F_TZ][s]), c);
-  // twice the midpoint separation
-  const double dx = __dsub_rn(__dadd_rn(sj[F_N1X][s], sj[F_N2X][s]),
-                              __dadd_rn(si[F_N1X][lane], si[F_N2X][lane]));
-  const double dy = __dsub_rn(__dadd_rn(sj[F_N1Y][s], sj[F_N2Y][s]),
-                              __dadd_rn(si[F_N1Y][lane], si[F_N2Y][lane]));
-  const double dz = __dsub_rn(__dadd_rn(sj[F_N1Z][s], sj[F_N2Z][s]),
-                              __dadd_rn(si[F_N1Z][lane], si[F_N2Z][lane]));
+  // twice the midpoint separation (cmid: n1 + n2 of the columns)
+  const double dx = __dsub_rn(cmid[0][s], __dadd_rn(si[F_N1X][lane], si[F_N2X][lane]));
+  const double dy = __dsub_rn(cmid[1][s], __dadd_rn(si[F_N1Y][lane], si[F_N2Y][lane]));
+  const double dz = __dsub_rn(cmid[2][s], __dadd_rn(si[F_N1Z][lane], si[F_N2Z][lane]));
   const double d2x4 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-  if (d2x4 > 4.0 * farDist2 && omcSq > sinMin2) return kFP32;
-  return omcSq > DDD_SQRT_EPS ? kFP64 : kPar;
+  const int cls = ((d2x4 > 4.0 * farDist2) & (omcSq > sinMin2)) ? kFP32
+                  : (omcSq > DDD_SQRT_EPS ? kFP64 : kPar);
+  return live ? cls : kNone;
 }
 
 // Near-field tiles of the hybrid path: tile t = (I, J) of the cell-sorted
@@ -142,6 +143,11 @@ __global__ __launch_bounds__(kTile, 2) void k_near_tiles(const double* __restric
     stage_tile(seg, ld, key, I, J, lane, si, sj, kj, ki);
 #pragma unroll
     for (int c = 0; c < 6; ++c) cacc[c][lane] = 0.0;
+    // n1 + n2 of the columns for the classification, in the batch result
+    // buffer (free until the evaluation)
+    sres[0][lane] = __dadd_rn(sj[F_N1X][lane], sj[F_N2X][lane]);
+    sres[1][lane] = __dadd_rn(sj[F_N1Y][lane], sj[F_N2Y][lane]);
+    sres[2][lane] = __dadd_rn(sj[F_N1Z][lane], sj[F_N2Z][lane]);
     __syncthreads();
 
     // 1. classification
@@ -149,13 +155,17 @@ __global__ __launch_bounds__(kTile, 2) void k_near_tiles(const double* __restric
     int n64 = 0, n32 = 0, nsp = 0;
     for (int k = 0; k < kTile; ++k) {
       const int s = (lane + k) & 63;
-      const int cls = near_class(si, sj, ki, kj[s], lane, s, diag, rn, farDist2, sinMin2);
+      const int cls = near_class(si, sj, sres, ki, kj[s], lane, s, diag, rn, farDist2, sinMin2);
       const uint64_t w32 = __ballot(cls == kFP32), w64 = __ballot(cls == kFP64);
       const uint64_t wsp = __ballot(cls == kPar);
       const unsigned short code = (unsigned short)((k << 6) | lane);
-      if (cls == kFP64) q[n64 + __popcll(w64 & ltMask)] = code;
-      if (cls == kFP32) q[kSlots - 1 - (n32 + __popcll(w32 & ltMask))] = code;
-      if (cls == kPar) q[kSlots + nsp + __popcll(wsp & ltMask)] = code;
+      // one store a slot: FP64 entries from the front, FP32 from the back of
+      // the first kSlots entries, near-parallel ones in the second kSlots
+      const bool f32 = cls == kFP32, f64 = cls == kFP64;
+      const uint64_t wsel = f32 ? w32 : (f64 ? w64 : wsp);
+      const int pos = (f32 ? n32 : (f64 ? n64 : nsp)) + __popcll(wsel & ltMask);
+      const int dest = f32 ? kSlots - 1 - pos : (f64 ? pos : kSlots + pos);
+      if (cls != kNone) q[dest] = code;
       n64 += __popcll(w64);
       n32 += __popcll(w32);
       nsp += __popcll(wsp);
