@@ -6,8 +6,9 @@
   normwise (FP32 vs FP64 accumulation and bf16 ties that fall differently).
   At 4,096 samples (the training batch) the output layer's three products
   run on the 256 x 256 tiles (one with split K over a ragged K), the
-  smaller batches on the 128 x 128 ones, and at 99 samples the weight
-  gradients (K = 99, not a multiple of 8) on the generic kernel;
+  smaller batches on the 128 x 128 ones; at 4,100 samples the 256 x 256
+  tiles have a ragged last row tile, and at 4,100 and 99 samples the weight
+  gradients (K not a multiple of 8) take the generic kernel;
 * training on the exact stresses of one network lowers the loss, and the
   operator it predicts improves the hybrid forces of another network over the
   no-far-field ablation and over the untrained network.
@@ -40,7 +41,7 @@ def test_params_roundtrip():
     m.close()
 
 
-@pytest.mark.parametrize("ns", [4096, 1000, 200, 99, 64])
+@pytest.mark.parametrize("ns", [4100, 4096, 1000, 200, 99, 64])
 def test_loss_grad_vs_reference(ns):
     m = G.FarFieldMLP(seed=1)
     p = R.init_params(3).astype(np.float32)
