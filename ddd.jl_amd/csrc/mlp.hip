@@ -3,7 +3,8 @@
 //   phi(o) (128) -> 512 -> 512 -> 512 -> 5408 (5400 used), SiLU activations,
 // W(o)[q][k] = env(o, q, k) * y(o)[q * 90 + k] with the physical envelope
 // env = (mu / 4 pi h) |o|^-(2 + order(k) + order(q)).
-// Matrix products run as BF16 MFMA (v_mfma_f32_32x32x8bf16_1k) with FP32
+// Matrix products run as BF16 MFMA (v_mfma_f32_32x32x16_bf16 in the tiled
+// kernels, v_mfma_f32_32x32x8bf16_1k in the generic one) with FP32
 // accumulation; master weights and Adam moments are FP32.
 //
 // Training data are exact local expansions: a sample is (offset o, source
