@@ -631,14 +631,27 @@ __global__ __launch_bounds__(256) void k_contract(const float* __restrict__ Y,
     r[tid] = envL[l] * a - Lt[(int64_t)s * kLocal + tid];
   }
   __syncthreads();
+  // dY[q][k] = A[q] B[k], A = w r envL, B = envK M (the padding zero)
   const float w = 2.f / (den[s] * ns);
-  for (int e = tid; e < kMlpOutPad; e += blockDim.x) {
-    float g = 0.f;
-    if (e < kChan) {
-      const int q = e / kMoments, k = e % kMoments;
-      g = w * r[q] * envL[q % kPoly] * envK[k] * sM[k];
+  __shared__ float A[kLocal], B[kMoments];
+  if (tid < kLocal) A[tid] = w * r[tid] * envL[tid % kPoly];
+  if (tid < kMoments) B[tid] = envK[tid] * sM[tid];
+  __syncthreads();
+  // two k a thread (one 32-bit store), 45 pairs x 5 rows a step
+  constexpr int kPairs = kMoments / 2, kRows = 256 / kPairs;
+  static_assert(kMoments % 2 == 0 && (kMlpOutPad - kChan) % 2 == 0 &&
+                    kPairs * kRows + (kMlpOutPad - kChan) / 2 <= 256,
+                "one workgroup of 256 writes a sample's dY in 32-bit words");
+  unsigned* d32 = reinterpret_cast<unsigned*>(dY + (int64_t)s * kMlpOutPad);
+  if (tid < kPairs * kRows) {
+    const int kp = tid % kPairs;
+    const float b0 = B[2 * kp], b1 = B[2 * kp + 1];
+    for (int q = tid / kPairs; q < kLocal; q += kRows) {
+      const float a = A[q];
+      d32[q * kPairs + kp] = (unsigned)f2bf(a * b0) | ((unsigned)f2bf(a * b1) << 16);
     }
-    dY[(int64_t)s * kMlpOutPad + e] = f2bf(g);
+  } else if (tid < kPairs * kRows + (kMlpOutPad - kChan) / 2) {
+    d32[kChan / 2 + (tid - kPairs * kRows)] = 0u;
   }
   if (tid == 0) {
     float a = 0.f;
