@@ -118,8 +118,9 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const unsigned short* __restr
 //
 // Split K: workgroup z of gridDim.z takes k in [z kChunk, (z + 1) kChunk) and
 // writes its partial product to C + z cStride (accumulate must be 0); a
-// second kernel adds the partials in z order.  EPI selects the epilogue as
-// for k_gemm_bf16_t256 below.
+// second kernel adds the partials in z order.  AK / BKC (which operands are
+// contiguous along k) and EPI (the epilogue) are as for k_gemm_bf16_t256
+// below; two wavefronts a SIMD.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 constexpr int BT = 128, BK = 64, LDK = BK + 8;
