@@ -95,12 +95,13 @@ __device__ __forceinline__ int near_class(const double (*si)[kTile], const doubl
 //     recording where; entries past spCap are counted but not written (the
 //     caller enlarges the list and runs again);
 //  2. the FP64 entries (the generic analytic pair), then the FP32 entries
-//     (FP32 quadrature of both sides, far_side), 64 at a time (near_batch).
+//     (FP32 quadrature of both sides, far_side), 64 at a time (near_batch2).
 // Sums are in a fixed order, so results are reproducible.  Keeping the
 // classification state out of the evaluation, column totals in LDS and
 // the near-parallel branch out of the kernel, the ~200-register FP64 pair
-// fits two wavefronts per SIMD; the LDS stays under 20 KB for eight
-// workgroups per CU.
+// fits two wavefronts per SIMD; the LDS is 20 KB for eight workgroups per
+// CU (the batch buffer doubles as classification storage, the mask arrays
+// as the tile and list-base broadcasts).
 // counters[0] += near pairs, counters[1] += FP64 pairs (generic and
 // near-parallel).
 __global__ __launch_bounds__(kTile, 2) void k_near_tiles(const double* __restrict__ seg, int64_t ld,
@@ -121,11 +122,17 @@ __global__ __launch_bounds__(kTile, 2) void k_near_tiles(const double* __restric
   __shared__ double si[kLdsFields][kTile];
   __shared__ double sj[kLdsFields][kTile];
   __shared__ double cacc[6][kTile];  // column totals of the current tile
-  __shared__ double sres[3][kTile];  // one force of each pair of a batch
+  // Two forces of each pair of a batch (near_batch2); during the
+  // classification, rows 0-2 hold the columns' n1 + n2 and row 3 their
+  // cell keys (kj).
+  __shared__ double sres[6][kTile];
+  // Ownership masks of a batch; outside the batches, rowM[0] passes the
+  // next tile's index and colM[0] the tile's base in the near-parallel list.
+  // (20 KB of LDS in all: eight workgroups a CU.)
   __shared__ unsigned long long rowM[kTile], colM[kTile];
-  __shared__ int kj[kTile];
-  __shared__ int tNext;
-  __shared__ long long spBase;
+  int* const kj = reinterpret_cast<int*>(&sres[3][0]);
+  int& tNext = *reinterpret_cast<int*>(&rowM[0]);
+  long long& spBase = *reinterpret_cast<long long*>(&colM[0]);
 
   constexpr int kSlots = kTile * kTile;
   unsigned short* const q = stage + (size_t)blockIdx.x * kNearStage;
@@ -188,7 +195,7 @@ __global__ __launch_bounds__(kTile, 2) void k_near_tiles(const double* __restric
     for (int b = 0; b < n64; b += kTile) {
       const int cnt = min(n64 - b, kTile);
       const unsigned e = lane < cnt ? q[b + lane] : 0u;
-      near_batch(e, cnt, lane,
+      near_batch2(e, cnt, lane,
                  [&](int rl, int sl, v3<double>& F1, v3<double>& F2, v3<double>& F3, v3<double>& F4) {
                    const Seg<double> P = seg_from_lds(si, rl), Q = seg_from_lds(sj, sl);
                    double c;
@@ -200,7 +207,7 @@ __global__ __launch_bounds__(kTile, 2) void k_near_tiles(const double* __restric
     for (int b = 0; b < n32; b += kTile) {
       const int cnt = min(n32 - b, kTile);
       const unsigned e = lane < cnt ? q[kSlots - 1 - (b + lane)] : 0u;
-      near_batch(e, cnt, lane,
+      near_batch2(e, cnt, lane,
                  [&](int rl, int sl, v3<double>& F1, v3<double>& F2, v3<double>& F3, v3<double>& F4) {
                    const Seg<double> P = seg_from_lds(si, rl), Q = seg_from_lds(sj, sl);
                    far_side<float>(KF, P, Q, F1, F2);
