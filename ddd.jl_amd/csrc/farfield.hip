@@ -148,13 +148,19 @@ __global__ void k_cell_moments(const double* __restrict__ seg, int64_t ld, GridD
 
 // ---- 3-D FFT on an NP^3 periodic grid, one line per thread ----------------
 
+// Workgroup size of the line transforms.  Stated in the kernel's launch
+// bounds: without them the compiler budgets for 1,024-thread workgroups (128
+// registers a lane) and a 64-point line, held in 128 registers, spills.
+constexpr int kFftBlock = 128;
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
 template <int NP>
-__global__ void k_fft_lines(float2* __restrict__ data, int64_t nChan, int axis, int inverse,
-                            const float2* __restrict__ tw) {
+__global__ __launch_bounds__(kFftBlock) void k_fft_lines(float2* __restrict__ data, int64_t nChan,
+                                                         int axis, int inverse,
+                                                         const float2* __restrict__ tw) {
   const int64_t line = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t per = (int64_t)NP * NP;
   if (line >= nChan * per) return;
@@ -554,7 +560,7 @@ int fft_size_for(int G) {
 
 static void fft3(float2* data, int64_t nChan, int NP, int inverse, const float2* tw, hipStream_t st) {
   const int64_t lines = nChan * (int64_t)NP * NP;
-  const int bs = 128;
+  const int bs = kFftBlock;
   const unsigned grid = (unsigned)((lines + bs - 1) / bs);
   for (int axis = 0; axis < 3; ++axis) {
     if (NP == 32)
