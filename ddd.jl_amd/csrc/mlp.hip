@@ -588,7 +588,8 @@ __device__ void offset_features(int ox, int oy, int oz, float aH, float* f) {
   f[q + 2] = ira * ira * ira;
 }
 
-__global__ void k_features(const int* __restrict__ off3, int ns, float aH, unsigned short* __restrict__ X) {
+__global__ __launch_bounds__(256) void k_features(const int* __restrict__ off3, int ns, float aH,
+                                                  unsigned short* __restrict__ X) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   float f[kMlpIn];
