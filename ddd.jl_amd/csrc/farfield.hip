@@ -157,22 +157,11 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// In-register radix-2 transform of one line whose element k sits at
+// p[k stride] (loaded in bit-reversed order, written back in order).
 template <int NP>
-__global__ __launch_bounds__(kFftBlock) void k_fft_lines(float2* __restrict__ data, int64_t nChan,
-                                                         int axis, int inverse,
-                                                         const float2* __restrict__ tw) {
-  const int64_t line = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t per = (int64_t)NP * NP;
-  if (line >= nChan * per) return;
-  const int64_t ch = line / per;
-  const int rem = (int)(line % per);
-  const int a = rem % NP, b = rem / NP;
-  int64_t base, stride;
-  // element (x, y, z) of channel ch at ch NP^3 + x + NP (y + NP z)
-  if (axis == 0) { base = (int64_t)NP * (a + (int64_t)NP * b); stride = 1; }
-  else if (axis == 1) { base = a + (int64_t)NP * NP * b; stride = NP; }
-  else { base = a + (int64_t)NP * b; stride = (int64_t)NP * NP; }
-  float2* p = data + ch * per * NP + base;
+__device__ __forceinline__ void fft_line(float2* p, int64_t stride, int inverse,
+                                         const float2* __restrict__ tw) {
   constexpr int LOG = NP == 32 ? 5 : (NP == 64 ? 6 : (NP == 16 ? 4 : 3));
   float2 v[NP];
 #pragma unroll
@@ -197,6 +186,43 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_lines(float2* __restrict__ da
   }
 #pragma unroll
   for (int k = 0; k < NP; ++k) p[k * stride] = v[k];
+}
+
+// Lines along y (axis 1) or z (axis 2), one a thread: consecutive threads
+// take consecutive x, so each of a line's loads is coalesced across lanes.
+template <int NP>
+__global__ __launch_bounds__(kFftBlock) void k_fft_lines(float2* __restrict__ data, int64_t nChan,
+                                                         int axis, int inverse,
+                                                         const float2* __restrict__ tw) {
+  const int64_t line = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)NP * NP;
+  if (line >= nChan * per) return;
+  const int64_t ch = line / per;
+  const int rem = (int)(line % per);
+  const int a = rem % NP, b = rem / NP;
+  // element (x, y, z) of channel ch at ch NP^3 + x + NP (y + NP z)
+  const int64_t base = axis == 1 ? a + per * b : a + (int64_t)NP * b;
+  const int64_t stride = axis == 1 ? NP : per;
+  fft_line<NP>(data + ch * per * NP + base, stride, inverse, tw);
+}
+
+// Lines along x (axis 0): a workgroup of NP threads takes NP consecutive
+// lines, one contiguous NP x NP block, loaded and stored coalesced through
+// LDS (rows padded by one element against bank conflicts); each thread
+// transforms one line from LDS as k_fft_lines does.
+template <int NP>
+__global__ __launch_bounds__(NP) void k_fft_rows(float2* __restrict__ data, int inverse,
+                                                 const float2* __restrict__ tw) {
+  __shared__ float2 sm[NP][NP + 1];
+  const int t = threadIdx.x;
+  float2* blk = data + (int64_t)blockIdx.x * NP * NP;
+#pragma unroll 8
+  for (int i = t; i < NP * NP; i += NP) sm[i / NP][i % NP] = blk[i];
+  __syncthreads();
+  fft_line<NP>(&sm[t][0], 1, inverse, tw);
+  __syncthreads();
+#pragma unroll 8
+  for (int i = t; i < NP * NP; i += NP) blk[i] = sm[i / NP][i % NP];
 }
 
 // Moments into the zero-padded complex grid, two real channels per complex
@@ -562,7 +588,15 @@ static void fft3(float2* data, int64_t nChan, int NP, int inverse, const float2*
   const int64_t lines = nChan * (int64_t)NP * NP;
   const int bs = kFftBlock;
   const unsigned grid = (unsigned)((lines + bs - 1) / bs);
-  for (int axis = 0; axis < 3; ++axis) {
+  const unsigned rowBlocks = (unsigned)(lines / NP);  // whole NP x NP blocks
+  auto rows = [&](auto kern, int np) {
+    hipLaunchKernelGGL(kern, dim3(rowBlocks), dim3(np), 0, st, data, inverse, tw);
+  };
+  if (NP == 32) rows(k_fft_rows<32>, 32);
+  else if (NP == 64) rows(k_fft_rows<64>, 64);
+  else if (NP == 16) rows(k_fft_rows<16>, 16);
+  else rows(k_fft_rows<8>, 8);
+  for (int axis = 1; axis < 3; ++axis) {
     if (NP == 32)
       hipLaunchKernelGGL(k_fft_lines<32>, dim3(grid), dim3(bs), 0, st, data, nChan, axis, inverse, tw);
     else if (NP == 64)
