@@ -58,31 +58,40 @@ __global__ void k_cell_assign(const double* __restrict__ seg, int64_t ld, int n,
   atomicAdd(&counts[c], 1u);
 }
 
-// Exclusive scan of nc counts into start[0..nc] by one workgroup.
-__global__ void k_cell_scan(const unsigned* __restrict__ counts, int nc, int* __restrict__ start) {
-  __shared__ int part[1024];
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int per = (nc + nt - 1) / nt;
-  const int b0 = tid * per, b1 = min(nc, b0 + per);
-  int s = 0;
-  for (int c = b0; c < b1; ++c) s += (int)counts[c];
-  part[tid] = s;
-  __syncthreads();
-  if (tid == 0) {
-    int acc = 0;
-    for (int k = 0; k < nt; ++k) {
-      const int v = part[k];
-      part[k] = acc;
-      acc += v;
+// Exclusive scan of nc counts into start[0..nc] by one workgroup of 1,024:
+// tiles of 1,024 counts, read coalesced, scanned within each wavefront by
+// shuffles and across the 16 wavefronts through LDS, with a running carry.
+__global__ __launch_bounds__(1024) void k_cell_scan(const unsigned* __restrict__ counts, int nc,
+                                                    int* __restrict__ start) {
+  __shared__ int wsum[16];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int carry = 0;
+  for (int base = 0; base < nc; base += 1024) {
+    const int c = base + tid;
+    const int v = c < nc ? (int)counts[c] : 0;
+    int x = v;  // inclusive within the wavefront
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(x, d);
+      if (lane >= d) x += y;
     }
-    start[nc] = acc;
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {  // inclusive over the wavefronts' totals
+      int t = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+      for (int d = 1; d < 16; d <<= 1) {
+        const int y = __shfl_up(t, d);
+        if (lane >= d) t += y;
+      }
+      if (lane < 16) wsum[lane] = t;
+    }
+    __syncthreads();
+    if (c < nc) start[c] = carry + (w ? wsum[w - 1] : 0) + x - v;
+    carry += wsum[15];
+    __syncthreads();  // wsum is rewritten by the next tile
   }
-  __syncthreads();
-  int acc = part[tid];
-  for (int c = b0; c < b1; ++c) {
-    start[c] = acc;
-    acc += (int)counts[c];
-  }
+  if (tid == 0) start[nc] = carry;
 }
 
 __global__ void k_cell_scatter(const int* __restrict__ cellOf, int n, const int* __restrict__ start,
