@@ -437,11 +437,11 @@ void hybrid_pairs(ddd_ctx* c, const ddd_consts& k, int64_t n, double* dev, unsig
   ff->spFix.ensure(sizeof(long long) * std::max<long long>(ff->spCap, 1));
   ff->spFixN.ensure(sizeof(unsigned long long));
   launch_near_special(ff->segS.as<double>(), ldS, ff->list.as<int2>(), tiles, K,
-                      ff->spList.as<unsigned long long>(), total, ff->spCap, ff->spRes.as<double>(),
-                      ff->spFix.as<long long>(), ff->spFixN.as<unsigned long long>(), flags, c->st);
-  const NearSpecialDev sp{ff->spList.as<unsigned long long>(), ff->spRes.as<double>(), ff->spCap,
-                          ff->specOff.as<long long>(), ff->specCnt.as<int>()};
-  launch_near_reduce(nl, nc, ff->rowPart.as<double>(), ff->colPart.as<double>(), sp, dev, c->st);
+                      ff->spList.as<unsigned long long>(), total, ff->spCap,
+                      ff->specOff.as<long long>(), ff->specCnt.as<int>(), ff->spRes.as<double>(),
+                      ff->spFix.as<long long>(), ff->spFixN.as<unsigned long long>(),
+                      ff->rowPart.as<double>(), ff->colPart.as<double>(), flags, c->st);
+  launch_near_reduce(nl, nc, ff->rowPart.as<double>(), ff->colPart.as<double>(), dev, c->st);
   HIPCHK(hipGetLastError());
 }
 

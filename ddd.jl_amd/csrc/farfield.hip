@@ -556,109 +556,33 @@ __global__ __launch_bounds__(64) void k_near_fill_cols(const int* __restrict__ t
   }
 }
 
-__device__ __forceinline__ double readlane_f64(double v, int k) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)b, k);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-// The near-parallel pairs of a tile's slot that lane l holds: their forces
-// (row role: on the row segment, res components 0-5; column role: on the
-// column segment, components 6-11; component-major, stride spCap) summed in
-// list order, then added to x, the lane's partial sums of the tile.  An
-// entry's low 16 bits hold its row slot (bits 0-5) and its column slot minus
-// the row slot, mod 64 (bits 6-11).
-__device__ __forceinline__ void add_special(const NearSpecialDev& S, int t, bool col, int l,
-                                            double x[6]) {
-  const int n = S.cnt[t];
-  if (n == 0) return;
-  const long long off = S.off[t];
-  double s[6] = {0, 0, 0, 0, 0, 0};
-  for (int e0 = 0; e0 < n; e0 += kTile) {
-    const int cnt = min(n - e0, kTile);
-    int slot = -1;
-    double v[6] = {0, 0, 0, 0, 0, 0};
-    if (l < cnt) {
-      const long long e = off + e0 + l;
-      const unsigned code = (unsigned)S.list[e];
-      const int rl = code & 63;
-      slot = col ? ((rl + (int)(code >> 6)) & 63) : rl;
-      const double* r = S.res + (col ? 6 : 0) * S.cap + e;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) v[c] = r[c * S.cap];
-    }
-    for (int k = 0; k < cnt; ++k) {
-      const bool mine = __builtin_amdgcn_readlane(slot, k) == l;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        const double vk = readlane_f64(v[c], k);
-        if (mine) s[c] += vk;
-      }
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 6; ++c) x[c] += s[c];
-}
-
-// Sum of a sorted segment's tile partials into out at its original index,
-// one workgroup of kReduceWaves wavefronts per tile of 64 sorted segments.
-// The tile's list is its row-role partials, then its column-role ones, each
-// with the tile's near-parallel forces of that role added first; wavefront w
-// sums items w, w + kReduceWaves, ... in list order, and the wavefronts' sums
-// are added in wavefront order (a fixed order: results do not depend on
-// timing).  Each wavefront loads the partials of kReduceAhead items before
-// it adds their near-parallel forces, so that enough loads are in flight to
-// stream the partials near the HBM rate.
-constexpr int kReduceWaves = 8;
-constexpr int kReduceAhead = 2;
-__global__ __launch_bounds__(64 * kReduceWaves) void k_near_reduce(
-    const double* __restrict__ rowPart, const double* __restrict__ colPart,
-    const int* __restrict__ rowStart, const int* __restrict__ colStart,
-    const int* __restrict__ colList, const NearSpecialDev S, const int* __restrict__ perm,
-    const int* __restrict__ start, int nc, double* __restrict__ out) {
-  __shared__ double part[kReduceWaves - 1][6][kTile];
-  const int B = blockIdx.x, l = threadIdx.x % kTile;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile);
-  const int r0 = rowStart[B], nr = rowStart[B + 1] - r0;
-  const int c0 = colStart[B], n = nr + colStart[B + 1] - c0;
+// Sum of a sorted segment's tile partials (row role, then column role, in
+// list order) into out at its original index.
+__global__ void k_near_reduce(const double* __restrict__ rowPart, const double* __restrict__ colPart,
+                              const int* __restrict__ rowStart, const int* __restrict__ colStart,
+                              const int* __restrict__ colList, const int* __restrict__ perm,
+                              const int* __restrict__ start, int nc, double* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= start[nc]) return;
+  const int B = p / kTile, l = p % kTile;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int i0 = w; i0 < n; i0 += kReduceAhead * kReduceWaves) {
-    double x[kReduceAhead][6];
-    int tile[kReduceAhead];
+  // unrolled so that several tiles' loads are in flight (the sums stay in
+  // list order)
+#pragma unroll 8
+  for (int t = rowStart[B]; t < rowStart[B + 1]; ++t) {
+    const double* r = rowPart + (int64_t)t * 6 * kTile + l;
 #pragma unroll
-    for (int u = 0; u < kReduceAhead; ++u) {
-      const int i = i0 + u * kReduceWaves;
-      if (i >= n) break;
-      tile[u] = i < nr ? r0 + i : colList[c0 + i - nr];
-      const double* r = (i < nr ? rowPart : colPart) + (int64_t)tile[u] * 6 * kTile + l;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) x[u][c] = r[c * kTile];
-    }
-#pragma unroll
-    for (int u = 0; u < kReduceAhead; ++u) {
-      const int i = i0 + u * kReduceWaves;
-      if (i >= n) break;
-      add_special(S, tile[u], i >= nr, l, x[u]);
-#pragma unroll
-      for (int c = 0; c < 6; ++c) acc[c] += x[u][c];
-    }
+    for (int c = 0; c < 6; ++c) acc[c] += r[c * kTile];
   }
-  if (w > 0) {
+#pragma unroll 8
+  for (int q = colStart[B]; q < colStart[B + 1]; ++q) {
+    const double* r = colPart + (int64_t)colList[q] * 6 * kTile + l;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) part[w - 1][c][l] = acc[c];
+    for (int c = 0; c < 6; ++c) acc[c] += r[c * kTile];
   }
-  __syncthreads();
-  const int p = B * kTile + l;
-  if (w != 0 || p >= start[nc]) return;
   double* o = out + 6 * (int64_t)perm[p];
 #pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    double s = acc[c];
-#pragma unroll
-    for (int v = 0; v < kReduceWaves - 1; ++v) s += part[v][c][l];
-    o[c] += s;
-  }
+  for (int c = 0; c < 6; ++c) o[c] += acc[c];
 }
 
 }  // namespace
@@ -801,10 +725,10 @@ void launch_near_list_fill(const NearListDev& L, hipStream_t st) {
 }
 
 void launch_near_reduce(const NearListDev& L, int nc, const double* rowPart, const double* colPart,
-                        const NearSpecialDev& S, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_near_reduce, dim3((unsigned)(L.ldS / kTile)), dim3(64 * kReduceWaves), 0, st,
-                     rowPart, colPart, L.rowStart, L.colStart, L.colList, S, L.perm, L.start, nc,
-                     out);
+                        double* out, hipStream_t st) {
+  const int bs = 256;
+  hipLaunchKernelGGL(k_near_reduce, dim3((unsigned)((L.ldS + bs - 1) / bs)), dim3(bs), 0, st, rowPart,
+                     colPart, L.rowStart, L.colStart, L.colList, L.perm, L.start, nc, out);
 }
 
 }  // namespace ddd

@@ -194,19 +194,8 @@ void morton_ranks(int G, std::vector<int>& mrank, std::vector<int>& rankCell);
 void launch_near_list_count(const double* seg, int64_t ld, int n, const GridDev& g, int rn,
                             NearListDev& L, hipStream_t st);
 void launch_near_list_fill(const NearListDev& L, hipStream_t st);
-// The listed near-parallel pairs of the near tiles and their forces (see
-// launch_near_tiles and launch_near_special).
-struct NearSpecialDev {
-  const unsigned long long* list;
-  const double* res;
-  long long cap;
-  const long long* off;
-  const int* cnt;
-};
-// out[perm[p]] += the sum of sorted segment p's tile partials, each with its
-// tile's near-parallel forces added.
 void launch_near_reduce(const NearListDev& L, int nc, const double* rowPart, const double* colPart,
-                        const NearSpecialDev& S, double* out, hipStream_t st);
+                        double* out, hipStream_t st);
 // Near-field tiles: workgroups of the main kernel and the staging entries
 // (uint16) each one owns (two per slot of a tile); stage holds
 // kNearGrid * kNearStage entries.  The main kernel lists the near-parallel
@@ -220,12 +209,12 @@ void launch_near_tiles(const double* segS, int64_t ldS, const int* key, int rn, 
                        long long spCap, long long* specOff, int* specCnt,
                        unsigned long long* counters, hipStream_t st);
 // The listed near-parallel pairs (total <= spCap): evaluated into res
-// (12 x spCap doubles: the forces on the row segment's ends, then on the
-// column segment's; fix: spCap entries and a counter, the pairs that need the
-// branch's corrective sub-pairs).  launch_near_reduce adds them.
+// (12 x spCap doubles; fix: spCap entries and a counter, the pairs that need
+// the branch's corrective sub-pairs) and added to their tiles' partial sums.
 void launch_near_special(const double* segS, int64_t ldS, const int2* list, int nTiles,
                          const SegK<double>& K, const unsigned long long* spList, long long total,
-                         long long spCap, double* res, long long* fix,
-                         unsigned long long* fixCount, unsigned* flags, hipStream_t st);
+                         long long spCap, const long long* specOff, const int* specCnt, double* res,
+                         long long* fix, unsigned long long* fixCount, double* rowPart,
+                         double* colPart, unsigned* flags, hipStream_t st);
 
 }  // namespace ddd

@@ -301,6 +301,59 @@ __global__ __launch_bounds__(kTile) void k_near_special_fix(const double* __rest
   }
 }
 
+// Adds the near-parallel forces of tile t (blockIdx.x) to its partial sums,
+// 64 entries at a time: each lane takes, in slot order, the entries whose
+// row (column) slot it is, marked with LDS atomic ORs.
+__global__ __launch_bounds__(kTile) void k_near_special_add(const unsigned long long* __restrict__ spList,
+                                                            const double* __restrict__ res,
+                                                            long long spCap,
+                                                            const long long* __restrict__ specOff,
+                                                            const int* __restrict__ specCnt,
+                                                            double* __restrict__ rowPart,
+                                                            double* __restrict__ colPart) {
+  __shared__ double sv[12][kTile];
+  __shared__ unsigned long long rowM[kTile], colM[kTile];
+  const int t = blockIdx.x, lane = threadIdx.x;
+  const int n = specCnt[t];
+  if (n == 0) return;
+  const long long off = specOff[t];
+  double a[6] = {0, 0, 0, 0, 0, 0}, b[6] = {0, 0, 0, 0, 0, 0};
+  for (int e0 = 0; e0 < n; e0 += kTile) {
+    const int cnt = min(n - e0, kTile);
+    rowM[lane] = 0;
+    colM[lane] = 0;
+    __syncthreads();
+    if (lane < cnt) {
+      const long long e = off + e0 + lane;
+      const unsigned code = (unsigned)(spList[e] & 0xffffu);
+      const int rl = code & 63, sl = (rl + (code >> 6)) & 63;
+      atomicOr(&rowM[rl], 1ull << lane);
+      atomicOr(&colM[sl], 1ull << lane);
+#pragma unroll
+      for (int c = 0; c < 12; ++c) sv[c][lane] = res[c * spCap + e];
+    }
+    __syncthreads();
+    for (uint64_t m = rowM[lane]; m; m &= m - 1) {
+      const int e = __builtin_ctzll(m);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) a[c] += sv[c][e];
+    }
+    for (uint64_t m = colM[lane]; m; m &= m - 1) {
+      const int e = __builtin_ctzll(m);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) b[c] += sv[6 + c][e];
+    }
+    __syncthreads();
+  }
+  double* rp = rowPart + (int64_t)t * 6 * kTile + lane;
+  double* cp = colPart + (int64_t)t * 6 * kTile + lane;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    rp[c * kTile] += a[c];
+    cp[c * kTile] += b[c];
+  }
+}
+
 }  // namespace
 
 void launch_near_tiles(const double* segS, int64_t ldS, const int* key, int rn, const int2* list,
@@ -320,8 +373,9 @@ void launch_near_tiles(const double* segS, int64_t ldS, const int* key, int rn, 
 
 void launch_near_special(const double* segS, int64_t ldS, const int2* list, int nTiles,
                          const SegK<double>& K, const unsigned long long* spList, long long total,
-                         long long spCap, double* res, long long* fix,
-                         unsigned long long* fixCount, unsigned* flags, hipStream_t st) {
+                         long long spCap, const long long* specOff, const int* specCnt, double* res,
+                         long long* fix, unsigned long long* fixCount, double* rowPart,
+                         double* colPart, unsigned* flags, hipStream_t st) {
   if (nTiles <= 0 || total <= 0) return;
   hipLaunchKernelGGL(k_near_zero_count, dim3(1), dim3(1), 0, st, fixCount);
   hipLaunchKernelGGL(k_near_special_eval, dim3((unsigned)((total + kTile - 1) / kTile)), dim3(kTile), 0, st,
@@ -329,6 +383,8 @@ void launch_near_special(const double* segS, int64_t ldS, const int2* list, int 
   const long long blocks = std::min<long long>((total + kTile - 1) / kTile, 2048);
   hipLaunchKernelGGL(k_near_special_fix, dim3((unsigned)blocks), dim3(kTile), 0, st, segS, ldS, list,
                      spList, spCap, K, res, fix, fixCount, flags);
+  hipLaunchKernelGGL(k_near_special_add, dim3((unsigned)nTiles), dim3(kTile), 0, st, spList, res, spCap,
+                     specOff, specCnt, rowPart, colPart);
 }
 
 }  // namespace ddd
